@@ -1,0 +1,523 @@
+// fmi_api.cpp -- the extern "C" boundary of libhpgfm.so (declared in
+// include/fmi.h).  Validates arguments on the host, owns the device copy of
+// the index and per-stream work counters, and times every launch with HIP
+// events on the stream it runs on.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/fmi.h"
+#include "fmi_index.h"
+#include "fmi_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+thread_local hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+thread_local bool g_timed = false;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return FMI_EDEVICE;
+}
+
+#define HIP_TRY(expr, what)                         \
+  do {                                              \
+    hipError_t e_ = (expr);                         \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+struct Tuning {
+  uint32_t wg_per_cu = 0;   // 0: occupancy limit of the kernel
+  uint32_t chunk = 32;
+};
+Tuning g_tuning;
+std::mutex g_mu;
+
+// Per-(device, stream) work counters for the persistent launch, so that
+// launches on different streams never share a queue.
+struct Workspace {
+  uint32_t *queue = nullptr;
+  unsigned long long *stats = nullptr;
+};
+std::unordered_map<std::string, Workspace> g_ws;
+
+int workspace_for(hipStream_t s, Workspace *out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  char key[64];
+  snprintf(key, sizeof key, "%d:%p", dev, (void *)s);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_ws.find(key);
+  if (it != g_ws.end()) { *out = it->second; return FMI_OK; }
+  Workspace w;
+  void *p = nullptr;
+  HIP_TRY(hipMalloc(&p, 256), "hipMalloc(workspace)");
+  w.queue = static_cast<uint32_t *>(p);
+  w.stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
+  g_ws[key] = w;
+  *out = w;
+  return FMI_OK;
+}
+
+int cu_count(int *cus) {
+  static int cached[64] = {0};
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev >= 0 && dev < 64 && cached[dev]) { *cus = cached[dev]; return FMI_OK; }
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  *cus = prop.multiProcessorCount;
+  if (dev >= 0 && dev < 64) cached[dev] = *cus;
+  return FMI_OK;
+}
+
+int begin_timed(hipStream_t s) {
+  if (!g_ev0) {
+    HIP_TRY(hipEventCreate(&g_ev0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&g_ev1), "hipEventCreate");
+  }
+  HIP_TRY(hipEventRecord(g_ev0, s), "hipEventRecord");
+  return FMI_OK;
+}
+
+int end_timed(hipStream_t s, hipError_t launch_err, const char *what) {
+  if (launch_err != hipSuccess) return hip_fail(launch_err, what);
+  HIP_TRY(hipEventRecord(g_ev1, s), "hipEventRecord");
+  g_timed = true;
+  return FMI_OK;
+}
+
+int need_device(const fmi_index *ix) {
+  if (!ix) return fail(FMI_EINVAL, "index is NULL");
+  if (!ix->on_device) return fail(FMI_ESTATE, "index is not on the device (call fmi_index_to_device)");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != ix->device) return fail(FMI_ESTATE, "index lives on another device");
+  return FMI_OK;
+}
+
+int view_params(int view, int direction, uint32_t *dsel, uint32_t *comp) {
+  if (view < 0 || view > 3) return fail(FMI_EINVAL, "view must be 0..3");
+  if (direction != 0 && direction != 1) return fail(FMI_EINVAL, "direction must be 0 or 1");
+  *dsel = (view == FMI_VIEW_FORWARD || view == FMI_VIEW_FORWARD_REV) ? 1u : 0u;
+  *comp = view >= FMI_VIEW_BACKWARD_REV ? 1u : 0u;
+  return FMI_OK;
+}
+
+int search1_common(const fmi_index *ix, int mode, int strand_mask, const uint8_t *d_reads,
+                   const uint64_t *d_offsets, uint32_t nreads, uint32_t cap, fmi_hit *d_hits,
+                   uint32_t *d_counts, fmi_stream s, bool persistent) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (mode != FMI_MODE_EDIT && mode != FMI_MODE_MISMATCH) return fail(FMI_EINVAL, "mode must be EDIT or MISMATCH");
+  if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "strand_mask must be 1, 2 or 3");
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_counts) return fail(FMI_EINVAL, "NULL buffer");
+  if (cap > 0 && !d_hits) return fail(FMI_EINVAL, "d_hits is NULL with cap > 0");
+  if (nreads > (1u << 30)) return fail(FMI_ERANGE, "more than 2^30 reads in one batch");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  Workspace w;
+  rc = workspace_for(st, &w);
+  if (rc) return rc;
+  uint32_t grid = 0, chunk = 32;
+  if (persistent) {
+    int cus = 0;
+    rc = cu_count(&cus);
+    if (rc) return rc;
+    Tuning t;
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      t = g_tuning;
+    }
+    uint32_t per_cu = t.wg_per_cu ? t.wg_per_cu : 8u;
+    grid = (uint32_t)cus * per_cu;
+    chunk = t.chunk ? t.chunk : 32u;
+    if (chunk > 4096) chunk = 4096;
+    HIP_TRY(fmi::launch_zero(w.queue, 1, st), "zero queue");
+  }
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, persistent, false,
+                                     (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap,
+                                     d_hits, d_counts, w.queue, w.stats, grid, chunk, st);
+  return end_timed(st, e, "search1 launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fmi_status_str(int status) {
+  switch (status) {
+    case FMI_OK: return "ok";
+    case FMI_EINVAL: return "invalid argument";
+    case FMI_ENOMEM: return "out of memory";
+    case FMI_EIO: return "i/o error";
+    case FMI_EFORMAT: return "bad index format";
+    case FMI_EDEVICE: return "device error";
+    case FMI_ERANGE: return "size out of range";
+    case FMI_ESTATE: return "call out of order";
+    default: return "unknown status";
+  }
+}
+
+const char *fmi_last_error(void) { return g_err.c_str(); }
+
+int fmi_index_build(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_index **out) {
+  if (!out) return fail(FMI_EINVAL, "out is NULL");
+  *out = nullptr;
+  fmi_index *ix = new (std::nothrow) fmi_index();
+  if (!ix) return fail(FMI_ENOMEM, "out of host memory");
+  std::string err;
+  int rc = fmi::build_index(text, n, sa_ratio, ix, &err);
+  if (rc) { delete ix; return fail(rc, err); }
+  *out = ix;
+  return FMI_OK;
+}
+
+int fmi_index_save(const fmi_index *ix, const char *path) {
+  if (!ix || !path) return fail(FMI_EINVAL, "NULL argument");
+  std::string err;
+  int rc = fmi::save_index(ix, path, &err);
+  return rc ? fail(rc, err) : FMI_OK;
+}
+
+int fmi_index_load(const char *path, fmi_index **out) {
+  if (!path || !out) return fail(FMI_EINVAL, "NULL argument");
+  *out = nullptr;
+  fmi_index *ix = new (std::nothrow) fmi_index();
+  if (!ix) return fail(FMI_ENOMEM, "out of host memory");
+  std::string err;
+  int rc = fmi::load_index(path, ix, &err);
+  if (rc) { delete ix; return fail(rc, err); }
+  *out = ix;
+  return FMI_OK;
+}
+
+void fmi_index_free(fmi_index *ix) {
+  if (!ix) return;
+  if (ix->on_device) {
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(ix->device);
+    (void)hipFree(ix->d_mem);
+    (void)hipFree(ix->d_koffset);
+    (void)hipFree(ix->d_kstart);
+    if (cur >= 0) (void)hipSetDevice(cur);
+  }
+  delete ix;
+}
+
+uint64_t fmi_index_size_sa(const fmi_index *ix) { return ix ? ix->n + 1 : 0; }
+uint32_t fmi_index_sa_ratio(const fmi_index *ix) { return ix ? ix->ratio : 0; }
+
+uint64_t fmi_index_device_bytes(const fmi_index *ix) {
+  if (!ix) return 0;
+  uint64_t b = 0;
+  for (int d = 0; d < 2; ++d)
+    b += ix->h[d].nblocks * sizeof(fmi::OccBlock) + 2 * ((ix->h[d].nsamp * 4 + 255) & ~255ull);
+  return b;
+}
+
+int fmi_index_to_device(fmi_index *ix) {
+  if (!ix) return fail(FMI_EINVAL, "index is NULL");
+  if (ix->on_device) return FMI_OK;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice (no usable GPU?)");
+  const uint64_t bytes = fmi_index_device_bytes(ix);
+  void *mem = nullptr;
+  hipError_t e = hipMalloc(&mem, bytes);
+  if (e != hipSuccess) return fail(FMI_ENOMEM, std::string("hipMalloc(index): ") + hipGetErrorString(e));
+  char *p = static_cast<char *>(mem);
+  fmi::DevIndex dv{};
+  for (int d = 0; d < 2; ++d) {
+    const fmi::HostDir &h = ix->h[d];
+    const uint64_t bb = h.nblocks * sizeof(fmi::OccBlock);
+    const uint64_t sb = (h.nsamp * 4 + 255) & ~255ull;
+    HIP_TRY(hipMemcpy(p, h.blk, bb, hipMemcpyHostToDevice), "copy blocks");
+    dv.d[d].blk = reinterpret_cast<const fmi::OccBlock *>(p);
+    p += bb;
+    HIP_TRY(hipMemcpy(p, h.S, h.nsamp * 4, hipMemcpyHostToDevice), "copy S");
+    dv.d[d].S = reinterpret_cast<const uint32_t *>(p);
+    p += sb;
+    HIP_TRY(hipMemcpy(p, h.R, h.nsamp * 4, hipMemcpyHostToDevice), "copy R");
+    dv.d[d].R = reinterpret_cast<const uint32_t *>(p);
+    p += sb;
+    for (int b = 0; b < 4; ++b) dv.d[d].C[b] = h.C[b];
+    dv.d[d].dollar = (uint32_t)h.dollar;
+    dv.d[d].dollar_blk = (uint32_t)(h.dollar / fmi::kBlockRows);
+    dv.d[d].dollar_off = (uint32_t)(h.dollar % fmi::kBlockRows);
+  }
+  dv.n = (uint32_t)ix->n;
+  dv.ratio = ix->ratio;
+  const uint64_t kc = ix->kstart.size();
+  void *ko = nullptr, *ks = nullptr;
+  HIP_TRY(hipMalloc(&ko, (kc + 1) * 8), "hipMalloc(karyotype)");
+  HIP_TRY(hipMalloc(&ks, (kc ? kc : 1) * 8), "hipMalloc(karyotype)");
+  HIP_TRY(hipMemcpy(ko, ix->koffset.data(), (kc + 1) * 8, hipMemcpyHostToDevice), "copy karyotype");
+  if (kc) HIP_TRY(hipMemcpy(ks, ix->kstart.data(), kc * 8, hipMemcpyHostToDevice), "copy karyotype");
+  ix->d_mem = mem;
+  ix->d_koffset = static_cast<uint64_t *>(ko);
+  ix->d_kstart = static_cast<uint64_t *>(ks);
+  ix->dev = dv;
+  ix->device = dev;
+  ix->device_bytes = bytes;
+  ix->on_device = true;
+  return FMI_OK;
+}
+
+int fmi_index_get_bwt(const fmi_index *ix, int forward, uint8_t *out, uint64_t *dollar) {
+  if (!ix || !out || !dollar) return fail(FMI_EINVAL, "NULL argument");
+  const fmi::HostDir &h = ix->h[forward ? 1 : 0];
+  uint64_t o = 0;
+  for (uint64_t r = 0; r <= ix->n; ++r) {
+    if (r == h.dollar) continue;
+    out[o++] = (uint8_t)fmi::host_bwt_sym(h, r);
+  }
+  *dollar = h.dollar;
+  return FMI_OK;
+}
+
+int fmi_index_set_karyotype(fmi_index *ix, uint64_t count, const uint64_t *start,
+                            const uint64_t *end) {
+  if (!ix || (count && (!start || !end))) return fail(FMI_EINVAL, "NULL argument");
+  if (ix->on_device) return fail(FMI_ESTATE, "set the karyotype before fmi_index_to_device");
+  std::vector<uint64_t> off(count + 1, 0);
+  for (uint64_t c = 0; c < count; ++c) {
+    if (end[c] + 1 < start[c]) return fail(FMI_EINVAL, "contig end < start - 1");
+    off[c + 1] = off[c] + (end[c] - start[c] + 1);
+  }
+  if (off[count] != ix->n) return fail(FMI_EINVAL, "contig lengths do not add up to the text length");
+  ix->kstart.assign(start, start + count);
+  ix->kend.assign(end, end + count);
+  ix->koffset = off;
+  return FMI_OK;
+}
+
+int fmi_device_count(int *count) {
+  if (!count) return fail(FMI_EINVAL, "NULL argument");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) { *count = 0; return hip_fail(e, "hipGetDeviceCount"); }
+  return FMI_OK;
+}
+
+int fmi_device_alloc(void **d_ptr, size_t bytes) {
+  if (!d_ptr) return fail(FMI_EINVAL, "NULL argument");
+  *d_ptr = nullptr;
+  hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) return fail(FMI_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return FMI_OK;
+}
+
+int fmi_device_free(void *d_ptr) {
+  HIP_TRY(hipFree(d_ptr), "hipFree");
+  return FMI_OK;
+}
+
+int fmi_memcpy_h2d(void *d_dst, const void *src, size_t bytes, fmi_stream s) {
+  if (!bytes) return FMI_OK;
+  if (!d_dst || !src) return fail(FMI_EINVAL, "NULL argument");
+  HIP_TRY(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(s)),
+          "hipMemcpyAsync H2D");
+  return FMI_OK;
+}
+
+int fmi_memcpy_d2h(void *dst, const void *d_src, size_t bytes, fmi_stream s) {
+  if (!bytes) return FMI_OK;
+  if (!dst || !d_src) return fail(FMI_EINVAL, "NULL argument");
+  HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(s)),
+          "hipMemcpyAsync D2H");
+  return FMI_OK;
+}
+
+int fmi_memset_d(void *d_dst, int value, size_t bytes, fmi_stream s) {
+  if (!bytes) return FMI_OK;
+  if (!d_dst) return fail(FMI_EINVAL, "NULL argument");
+  HIP_TRY(hipMemsetAsync(d_dst, value, bytes, static_cast<hipStream_t>(s)), "hipMemsetAsync");
+  return FMI_OK;
+}
+
+int fmi_stream_create(fmi_stream *s) {
+  if (!s) return fail(FMI_EINVAL, "NULL argument");
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+  *s = st;
+  return FMI_OK;
+}
+
+int fmi_stream_destroy(fmi_stream s) {
+  HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(s)), "hipStreamDestroy");
+  return FMI_OK;
+}
+
+int fmi_stream_sync(fmi_stream s) {
+  HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(s)), "hipStreamSynchronize");
+  return FMI_OK;
+}
+
+int fmi_last_kernel_ms(float *ms) {
+  if (!ms) return fail(FMI_EINVAL, "NULL argument");
+  if (!g_timed) return fail(FMI_ESTATE, "no kernel launched on this thread");
+  HIP_TRY(hipEventSynchronize(g_ev1), "hipEventSynchronize");
+  HIP_TRY(hipEventElapsedTime(ms, g_ev0, g_ev1), "hipEventElapsedTime");
+  return FMI_OK;
+}
+
+int fmi_exact_batch(const fmi_index *ix, int view, int direction, const uint8_t *d_reads,
+                    const uint64_t *d_offsets, uint32_t nreads, fmi_hit *d_out, fmi_hit *d_anchor,
+                    fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  uint32_t dsel, comp;
+  rc = view_params(view, direction, &dsel, &comp);
+  if (rc) return rc;
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_out) return fail(FMI_EINVAL, "NULL buffer");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_exact(ix->dev, dsel, comp, (uint32_t)direction, d_reads, d_offsets,
+                                   nreads, d_out, d_anchor, st);
+  return end_timed(st, e, "exact launch");
+}
+
+int fmi_exact_vector_batch(const fmi_index *ix, int view, int direction, const uint8_t *d_reads,
+                           const uint64_t *d_offsets, uint32_t nreads, int64_t *d_vec_k,
+                           int64_t *d_vec_l, int16_t *d_last_good, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  uint32_t dsel, comp;
+  rc = view_params(view, direction, &dsel, &comp);
+  if (rc) return rc;
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_vec_k || !d_vec_l || !d_last_good) return fail(FMI_EINVAL, "NULL buffer");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_vector(ix->dev, dsel, comp, (uint32_t)direction, d_reads, d_offsets,
+                                    nreads, d_vec_k, d_vec_l, d_last_good, st);
+  return end_timed(st, e, "vector launch");
+}
+
+int fmi_search1_batch(const fmi_index *ix, int mode, int strand_mask, const uint8_t *d_reads,
+                      const uint64_t *d_offsets, uint32_t nreads, uint32_t cap, fmi_hit *d_hits,
+                      uint32_t *d_counts, fmi_stream s) {
+  return search1_common(ix, mode, strand_mask, d_reads, d_offsets, nreads, cap, d_hits, d_counts,
+                        s, true);
+}
+
+int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
+                            const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                            uint32_t cap, fmi_hit *d_hits, uint32_t *d_counts, fmi_stream s) {
+  return search1_common(ix, mode, strand_mask, d_reads, d_offsets, nreads, cap, d_hits, d_counts,
+                        s, false);
+}
+
+int fmi_set_tuning(uint32_t wg_per_cu, uint32_t chunk) {
+  if (wg_per_cu > 64 || chunk > 4096) return fail(FMI_EINVAL, "tuning out of range");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuning.wg_per_cu = wg_per_cu;
+  g_tuning.chunk = chunk ? chunk : 32;
+  return FMI_OK;
+}
+
+int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k, const int64_t *d_l,
+                     uint32_t nintervals, uint32_t cap, uint64_t *d_pos, uint32_t *d_nrows,
+                     fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (nintervals == 0 || cap == 0) return FMI_OK;
+  if (!d_k || !d_l || !d_pos || !d_nrows) return fail(FMI_EINVAL, "NULL buffer");
+  if ((uint64_t)nintervals * cap > (1ull << 36)) return fail(FMI_ERANGE, "nintervals * cap too large");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_locate(ix->dev, forward ? 1u : 0u, d_k, d_l, nintervals, cap, d_pos,
+                                    d_nrows, st);
+  return end_timed(st, e, "locate launch");
+}
+
+int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
+                     const uint64_t *d_offsets, uint32_t nreads, uint32_t cap, uint32_t row_cap,
+                     int32_t *d_chrom, uint64_t *d_coord, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (nreads == 0 || cap == 0 || row_cap == 0) return FMI_OK;
+  if (!d_hits || !d_counts || !d_offsets || !d_chrom || !d_coord) return fail(FMI_EINVAL, "NULL buffer");
+  if ((uint64_t)2 * nreads * cap * row_cap > (1ull << 36)) return fail(FMI_ERANGE, "output too large");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_resolve(ix->dev, d_hits, d_counts, d_offsets, nreads, cap, row_cap,
+                                     ix->d_koffset, ix->d_kstart, (uint32_t)ix->kstart.size(),
+                                     d_chrom, d_coord, st);
+  return end_timed(st, e, "resolve launch");
+}
+
+int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uint64_t nlines,
+                     uint64_t *d_sink, float *ms, uint64_t *lines_done, fmi_stream s) {
+  if (!d_buf || !d_sink || !ms || !lines_done) return fail(FMI_EINVAL, "NULL argument");
+  if (line_bytes == 0 || line_bytes % 16 || line_bytes > 4096) return fail(FMI_EINVAL, "line_bytes must be a multiple of 16 up to 4096");
+  if (bytes < (64ull << 20)) return fail(FMI_EINVAL, "buffer must be >= 64 MiB");
+  int cus = 0;
+  int rc = cu_count(&cus);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_gather(d_buf, bytes, line_bytes, nlines,
+                                    reinterpret_cast<unsigned long long *>(d_sink),
+                                    (uint32_t)cus * 16u, st);
+  rc = end_timed(st, e, "gather launch");
+  if (rc) return rc;
+  *lines_done = fmi::gather_loads_done(nlines, (uint32_t)cus * 16u);
+  return fmi_last_kernel_ms(ms);
+}
+
+int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
+                             const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                             uint64_t *blocks, uint64_t *lf_steps) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (!blocks || !lf_steps) return fail(FMI_EINVAL, "NULL argument");
+  if (mode != FMI_MODE_EDIT && mode != FMI_MODE_MISMATCH) return fail(FMI_EINVAL, "bad mode");
+  if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "bad strand_mask");
+  *blocks = 0;
+  *lf_steps = 0;
+  if (nreads == 0) return FMI_OK;
+  hipStream_t st = nullptr;
+  Workspace w;
+  rc = workspace_for(st, &w);
+  if (rc) return rc;
+  uint32_t *counts = nullptr;
+  HIP_TRY(hipMalloc(&counts, (size_t)2 * nreads * 4), "hipMalloc(counts)");
+  int cus = 0;
+  rc = cu_count(&cus);
+  if (rc) { (void)hipFree(counts); return rc; }
+  HIP_TRY(fmi::launch_zero(w.queue, 1, st), "zero queue");
+  HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 4, st), "zero stats");
+  hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, true, true,
+                                     (uint32_t)strand_mask, d_reads, d_offsets, nreads, 0, nullptr,
+                                     counts, w.queue, w.stats, (uint32_t)cus * 8u, 32u, st);
+  if (e != hipSuccess) { (void)hipFree(counts); return hip_fail(e, "count launch"); }
+  unsigned long long st2[2] = {0, 0};
+  e = hipMemcpy(st2, w.stats, 16, hipMemcpyDeviceToHost);
+  (void)hipFree(counts);
+  if (e != hipSuccess) return hip_fail(e, "copy stats");
+  *blocks = st2[0];
+  *lf_steps = st2[1];
+  return FMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,40 @@
+// fmi_index.h -- the opaque fmi_index object behind the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fmi_internal.h"
+
+struct fmi_index {
+  uint64_t n = 0;                 // text length (rows are 0..n)
+  uint32_t ratio = 0;             // SA / ISA sample rate
+  fmi::HostDir h[2];              // 0: BWT of X, 1: BWT of reverse(X)
+  std::vector<uint64_t> kstart;   // karyotype: start coordinate of contig c
+  std::vector<uint64_t> kend;     //            end coordinate of contig c
+  std::vector<uint64_t> koffset;  //            text offset of contig c (count+1)
+
+  // device copy
+  bool on_device = false;
+  int device = -1;
+  void *d_mem = nullptr;          // one allocation for blocks + samples
+  uint64_t *d_koffset = nullptr;  // karyotype offsets (count + 1)
+  uint64_t *d_kstart = nullptr;
+  uint64_t device_bytes = 0;
+  fmi::DevIndex dev{};
+
+  ~fmi_index();
+};
+
+namespace fmi {
+
+// Builds both directions from codes 0..3; returns an fmi_status.
+int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix,
+                std::string *err);
+int save_index(const fmi_index *ix, const char *path, std::string *err);
+int load_index(const char *path, fmi_index *ix, std::string *err);
+// Symbol of BWT row r (0xFF for the '$' row).
+uint32_t host_bwt_sym(const HostDir &d, uint64_t r);
+
+}  // namespace fmi

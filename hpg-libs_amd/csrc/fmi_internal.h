@@ -1,0 +1,62 @@
+// fmi_internal.h -- index layout shared by the builder, the HIP kernels and
+// the C ABI of libhpgfm.so.
+//
+// Occurrence table: one 64-byte block per 192 BWT rows, so that an LF step
+// touches one HBM line (two when k and l+1 fall in different blocks):
+//
+//   bytes  0..15  uint32 cnt[4]   occurrences of A,C,G,T in rows [0, 192*j)
+//   bytes 16..63  uint64 bits[6]  2-bit codes of rows 192*j .. 192*j+191;
+//                                 row 192*j+t sits in bits[t/32] at bit 2*(t%32)
+//
+// The '$' row is stored as code 0 (A) and corrected out of band: Occ(A, m)
+// subtracts one when the '$' row lies in [192*j, m).  This replaces the
+// reference's two per-symbol arrays (uint32 desp[4][m] checkpoints and
+// uint64 count[4][m] bitmaps, search/csafm.h:51-81), whose get_O
+// (csafm.h:166-181) needs two dependent loads from separate arrays per
+// symbol per bound.
+#pragma once
+
+#include <cstdint>
+
+namespace fmi {
+
+constexpr uint32_t kBlockRows = 192;
+constexpr uint32_t kBlockBytes = 64;
+constexpr uint32_t kMaxReadLen = 1000;
+
+struct alignas(64) OccBlock {
+  uint32_t cnt[4];
+  uint64_t bits[6];
+};
+static_assert(sizeof(OccBlock) == 64, "occurrence block must be one 64-byte line");
+
+// One direction of the index on the device (passed by value to kernels).
+struct DevDir {
+  const OccBlock *blk;   // nblocks
+  const uint32_t *S;     // S[i / ratio] = SA[i] for rows i % ratio == 0
+  const uint32_t *R;     // R[j / ratio] = ISA[j] for text positions j % ratio == 0
+  uint32_t C[4];         // # symbols < b
+  uint32_t dollar;       // row holding '$'
+  uint32_t dollar_blk;   // dollar / 192
+  uint32_t dollar_off;   // dollar % 192
+  uint32_t pad;
+};
+
+struct DevIndex {
+  DevDir d[2];           // 0: BWT of X ("backward"), 1: BWT of reverse X ("forward")
+  uint32_t n;            // text length; rows are 0..n
+  uint32_t ratio;        // SA / ISA sample rate
+};
+
+// Host-side view of one direction.
+struct HostDir {
+  uint64_t dollar = 0;
+  uint32_t C[4] = {0, 0, 0, 0};
+  OccBlock *blk = nullptr;
+  uint64_t nblocks = 0;
+  uint32_t *S = nullptr;
+  uint32_t *R = nullptr;
+  uint64_t nsamp = 0;
+};
+
+}  // namespace fmi

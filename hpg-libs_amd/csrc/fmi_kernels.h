@@ -1,0 +1,36 @@
+// fmi_kernels.h -- host-side launchers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fmi.h"
+#include "fmi_internal.h"
+
+namespace fmi {
+
+hipError_t launch_exact(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
+                        const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
+                        fmi_hit *out, fmi_hit *anchor, hipStream_t s);
+hipError_t launch_vector(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
+                         const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
+                         int64_t *vk, int64_t *vl, int16_t *last_good, hipStream_t s);
+hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, bool count,
+                          uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
+                          uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
+                          uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
+                          uint32_t chunk, hipStream_t s);
+hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t s);
+hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
+                         uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
+                         hipStream_t s);
+hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_t *counts,
+                          const uint64_t *offsets, uint32_t nreads, uint32_t cap,
+                          uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
+                          uint32_t kcount, int32_t *chrom, uint64_t *coord, hipStream_t s);
+hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
+                         unsigned long long *sink, uint32_t grid, hipStream_t s);
+uint64_t gather_loads_done(uint64_t nloads, uint32_t grid);
+
+}  // namespace fmi

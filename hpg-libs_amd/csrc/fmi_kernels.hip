@@ -1,0 +1,739 @@
+// fmi_kernels.hip -- HIP kernels of the FM-index read-search engine (gfx950).
+//
+// All arithmetic is exact integer.  Intervals are held as uint32 on the
+// device (rows are 0..n with n + 3 < 2^32) and widened to int64 on output.
+//
+// The unit of work is one LF step: one 64-byte occurrence block (two when k
+// and l+1 fall in different blocks) gives Occ of all four symbols at both
+// bounds, so a 4-way branch costs the same fetch as a single step.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fmi.h"
+#include "fmi_internal.h"
+#include "fmi_kernels.h"
+
+namespace fmi {
+
+// --------------------------------------------------------------------------
+// occurrence block access
+// --------------------------------------------------------------------------
+
+struct Blk {
+  uint32_t c[4];
+  uint64_t w[6];
+};
+
+__device__ __forceinline__ Blk load_block(const OccBlock *base, uint32_t j) {
+  const uint4 *q = reinterpret_cast<const uint4 *>(base + j);
+  uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+  Blk B;
+  B.c[0] = a.x; B.c[1] = a.y; B.c[2] = a.z; B.c[3] = a.w;
+  B.w[0] = ((uint64_t)b.y << 32) | b.x;
+  B.w[1] = ((uint64_t)b.w << 32) | b.z;
+  B.w[2] = ((uint64_t)c.y << 32) | c.x;
+  B.w[3] = ((uint64_t)c.w << 32) | c.z;
+  B.w[4] = ((uint64_t)d.y << 32) | d.x;
+  B.w[5] = ((uint64_t)d.w << 32) | d.z;
+  return B;
+}
+
+// Number of rows among the first t (0..192) rows of the block whose code is s.
+__device__ __forceinline__ uint32_t rank_in(const Blk &B, uint32_t s, uint32_t t) {
+  const uint64_t M = 0x5555555555555555ull;
+  const uint64_t pat = (uint64_t)s * M;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int wi = 0; wi < 6; ++wi) {
+    int lim = (int)t - 32 * wi;
+    lim = lim < 0 ? 0 : (lim > 32 ? 32 : lim);
+    const uint64_t mask = lim == 32 ? ~0ull : ((1ull << (2 * lim)) - 1ull);
+    const uint64_t x = B.w[wi] ^ pat;
+    cnt += (uint32_t)__popcll(~(x | (x >> 1)) & M & mask);
+  }
+  return cnt;
+}
+
+// a[i] for a runtime i in 0..3 without dynamic register indexing.
+__device__ __forceinline__ uint32_t pick4(const uint32_t (&a)[4], uint32_t i) {
+  const uint32_t lo = (i & 1u) ? a[1] : a[0];
+  const uint32_t hi = (i & 1u) ? a[3] : a[2];
+  return (i & 2u) ? hi : lo;
+}
+
+// Occ(s, m) for m in block j (t = m - 192*j): checkpoint + in-block rank,
+// minus the '$' row (stored as code 0) when it lies before m in the block.
+__device__ __forceinline__ uint32_t occ_blk(const Blk &B, uint32_t dollar_blk,
+                                            uint32_t dollar_off, uint32_t j, uint32_t s,
+                                            uint32_t t) {
+  uint32_t v = pick4(B.c, s) + rank_in(B, s, t);
+  if (s == 0 && j == dollar_blk && dollar_off < t) v -= 1;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t code_at(const Blk &B, uint32_t t) {
+  return (uint32_t)((B.w[t >> 5] >> (2 * (t & 31))) & 3u);
+}
+
+// LF step of one physical symbol s on [k, l] (BWiteration, runtime.h:38-42).
+__device__ __forceinline__ void lf_one(const DevDir &D, uint32_t k, uint32_t l, uint32_t s,
+                                       uint32_t &ko, uint32_t &lo, uint32_t &nblk) {
+  const uint32_t m2 = l + 1;
+  const uint32_t j1 = k / kBlockRows, j2 = m2 / kBlockRows;
+  const Blk B1 = load_block(D.blk, j1);
+  Blk B2;
+  if (j2 != j1) B2 = load_block(D.blk, j2);
+  else B2 = B1;
+  nblk = (j2 != j1) ? 2u : 1u;
+  const uint32_t c = pick4(D.C, s);
+  ko = c + 1u + occ_blk(B1, D.dollar_blk, D.dollar_off, j1, s, k - j1 * kBlockRows);
+  lo = c + occ_blk(B2, D.dollar_blk, D.dollar_off, j2, s, m2 - j2 * kBlockRows);
+}
+
+__device__ __forceinline__ const DevDir &dir_of(const DevIndex &ix, uint32_t d) {
+  return d ? ix.d[1] : ix.d[0];
+}
+
+// --------------------------------------------------------------------------
+// exact search: BWExactSearchBackward / Forward (search/search.h:9-65)
+// --------------------------------------------------------------------------
+
+struct ExactParams {
+  DevIndex ix;
+  uint32_t dsel, comp, direction, nreads;
+  const uint8_t *reads;
+  const uint64_t *offsets;
+  fmi_hit *out;
+  fmi_hit *anchor;
+};
+
+__device__ __forceinline__ void store_hit(fmi_hit *dst, int64_t k, int64_t l, int16_t start,
+                                          int16_t pos, int16_t end, int16_t err_pos, uint8_t dir,
+                                          uint8_t nmm, uint8_t kind, uint8_t base, uint8_t strand) {
+  uint4 a, b;
+  a.x = (uint32_t)(uint64_t)k; a.y = (uint32_t)((uint64_t)k >> 32);
+  a.z = (uint32_t)(uint64_t)l; a.w = (uint32_t)((uint64_t)l >> 32);
+  b.x = (uint32_t)(uint16_t)start | ((uint32_t)(uint16_t)pos << 16);
+  b.y = (uint32_t)(uint16_t)end | ((uint32_t)(uint16_t)err_pos << 16);
+  b.z = (uint32_t)dir | ((uint32_t)nmm << 8) | ((uint32_t)kind << 16) | ((uint32_t)base << 24);
+  b.w = (uint32_t)strand;
+  uint4 *d = reinterpret_cast<uint4 *>(dst);
+  d[0] = a;
+  d[1] = b;
+}
+
+__global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P.nreads) return;
+  const DevDir &D = dir_of(P.ix, P.dsel);
+  const uint8_t *W = P.reads + P.offsets[r];
+  const int32_t len = (int32_t)(P.offsets[r + 1] - P.offsets[r]);
+  const uint32_t n = P.ix.n;
+  uint32_t k = 0, l = 0, k2 = 0, l2 = n, nb;
+  int32_t i;
+  if (P.direction == 0) {
+    for (i = len - 1; i >= 0; --i) {
+      const uint32_t sym = W[i];
+      if (sym > 3u) { k2 = n + 1; l2 = n; break; }
+      lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+      if (k2 > l2) break;
+      k = k2; l = l2;
+    }
+    if (P.anchor)
+      store_hit(P.anchor + r, k, l, (int16_t)(i + 1), (int16_t)i, (int16_t)(len - 1), 0, 0, 0, 0, 0, 0);
+  } else {
+    for (i = 0; i <= len - 1; ++i) {
+      const uint32_t sym = W[i];
+      if (sym > 3u) { k2 = n + 1; l2 = n; break; }
+      lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+      if (k2 > l2) break;
+      k = k2; l = l2;
+    }
+    if (P.anchor)
+      store_hit(P.anchor + r, k, l, 0, (int16_t)i, (int16_t)(i - 1), 0, 1, 0, 0, 0, 0);
+  }
+  store_hit(P.out + r, k2, l2, 0, (int16_t)i, (int16_t)(len - 1), 0, (uint8_t)P.direction, 0, 0, 0, 0);
+}
+
+// BWExactSearchVectorBackward / Forward (search/search.c:1407-1489).
+struct VectorParams {
+  DevIndex ix;
+  uint32_t dsel, comp, direction, nreads;
+  const uint8_t *reads;
+  const uint64_t *offsets;
+  int64_t *vk, *vl;
+  int16_t *last_good;
+};
+
+__global__ __launch_bounds__(256) void vector_kernel(VectorParams P) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P.nreads) return;
+  const DevDir &D = dir_of(P.ix, P.dsel);
+  const uint64_t off = P.offsets[r];
+  const uint8_t *W = P.reads + off;
+  const int32_t len = (int32_t)(P.offsets[r + 1] - off);
+  const uint32_t n = P.ix.n;
+  int64_t *vk = P.vk + off, *vl = P.vl + off;
+  int32_t good = 0;
+  uint32_t k2 = 0, l2 = n, nb;
+  if (P.direction == 0) {
+    int32_t i = len - 1, j = len - 1;
+    for (; i >= 0; --i, --j) {
+      const uint32_t sym = W[i];
+      if (sym > 3u) { k2 = n + 1; l2 = n; }
+      else lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+      vk[j] = k2; vl[j] = l2;
+      if (k2 > l2) { --i; --j; break; }
+      ++good;
+    }
+    for (; i >= 0; --i, --j) { vk[j] = k2; vl[j] = l2; }
+  } else {
+    int32_t i = 0, j = 0;
+    for (; i <= len - 1; ++i, ++j) {
+      const uint32_t sym = W[i];
+      if (sym > 3u) { k2 = n + 1; l2 = n; }
+      else lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+      vk[j] = k2; vl[j] = l2;
+      if (k2 > l2) { ++i; ++j; break; }
+      ++good;
+    }
+    for (; i <= len - 1; ++i, ++j) { vk[j] = k2; vl[j] = l2; }
+  }
+  P.last_good[r] = (int16_t)good;
+}
+
+// --------------------------------------------------------------------------
+// 1-error search: BWSearch1CPU (search/search.c:1876-1933) with
+// BWSimpleSearch1Backward / Forward (search/search.c:1711-1874)
+// --------------------------------------------------------------------------
+//
+// One lane runs one (read, strand) task as a state machine whose every
+// iteration is exactly one LF query, so all lanes of a wavefront issue their
+// block loads together:
+//   EXACT  exact extension over the anchor half (pass 0: [len/2, len-1]
+//          right to left on the first index; pass 1: [0, len/2 - 1] left to
+//          right on the second index)
+//   SCAN   one node of the other half: the block gives the four children;
+//          if the size of the interval does not change, every occurrence
+//          extends by W[i] and no branch can succeed (search.c:1747)
+//   CONT   exact continuation of one branch (deletion, insertion b,
+//          mismatch b, in the reference's order) to the end of the read
+// Records are written in the order the reference adds them.
+
+enum : uint32_t { PH_IDLE = 0, PH_EXACT = 1, PH_SCAN = 2, PH_CONT = 3 };
+
+struct S1Params {
+  DevIndex ix;
+  const uint8_t *reads;
+  const uint64_t *offsets;
+  fmi_hit *hits;
+  uint32_t *counts;
+  uint32_t *queue;                 // work counter (persistent launch)
+  unsigned long long *stats;       // [0] blocks, [1] LF queries (count mode)
+  uint32_t ntasks, strand_mask, cap, chunk;
+};
+
+struct S1Lane {
+  uint32_t phase, pass, strand, read, nout, comp, bi;
+  const uint8_t *W;
+  int32_t len, i, cp;
+  uint32_t k, l, kn, ln, qk, ql, wsym;
+  uint32_t ck[4], cl[4];           // children of the SCAN node, physical symbol order
+  const OccBlock *blk;             // the current pass's direction
+  uint32_t C[4], dollar_blk, dollar_off;
+  fmi_hit *out;
+};
+
+template <bool MM_ONLY>
+struct S1 {
+  const S1Params &P;
+  S1Lane &L;
+
+  __device__ __forceinline__ int32_t step() const { return L.pass ? 1 : -1; }
+  __device__ __forceinline__ bool in_read(int32_t p) const { return p >= 0 && p <= L.len - 1; }
+  __device__ __forceinline__ bool in_exact(int32_t p) const {
+    return L.pass ? (p <= L.len / 2 - 1) : (p >= L.len / 2);
+  }
+
+  __device__ __forceinline__ void emit(uint32_t k, uint32_t l, uint8_t nmm, uint8_t kind,
+                                       uint8_t base, int16_t err_pos) {
+    if (L.nout < P.cap) {
+      int16_t start, end, pos;
+      if (nmm == 0) {               // the exact record of pass 0
+        start = 0; end = (int16_t)(L.len - 1); pos = -1;
+      } else if (L.pass == 0) {     // bound_result(start, half-1), pos = start-1
+        start = 0; end = (int16_t)(L.len / 2 - 1); pos = -1;
+      } else {                      // bound_result(half+1, end), pos = end+1
+        start = (int16_t)(L.len / 2); end = (int16_t)(L.len - 1); pos = (int16_t)L.len;
+      }
+      store_hit(L.out + L.nout, k, l, start, pos, end, err_pos, (uint8_t)L.pass, nmm, kind, base,
+                (uint8_t)L.strand);
+    }
+    L.nout++;
+  }
+
+  __device__ __forceinline__ void finish_task() {
+    P.counts[2 * L.read + L.strand] = L.nout;
+    L.phase = PH_IDLE;
+  }
+
+  __device__ __forceinline__ void end_pass() {
+    if (L.pass == 0) enter_pass(1);
+    else finish_task();
+  }
+
+  __device__ __forceinline__ void enter_pass(uint32_t pass) {
+    L.pass = pass;
+    // strand 0: (backward, forward); strand 1: (forward_rev, backward_rev)
+    const bool second = (pass ^ L.strand) != 0u;
+    const DevDir &D0 = P.ix.d[0], &D1 = P.ix.d[1];
+    L.blk = second ? D1.blk : D0.blk;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) L.C[b] = second ? D1.C[b] : D0.C[b];
+    L.dollar_blk = second ? D1.dollar_blk : D0.dollar_blk;
+    L.dollar_off = second ? D1.dollar_off : D0.dollar_off;
+    L.comp = L.strand;
+    L.k = 0;
+    L.l = P.ix.n;
+    L.i = pass ? 0 : L.len - 1;
+    if (in_read(L.i) && in_exact(L.i)) L.phase = PH_EXACT;
+    else enter_scan();
+  }
+
+  __device__ __forceinline__ void enter_scan() {
+    L.kn = L.k;
+    L.ln = L.l;
+    L.i = L.pass ? L.len / 2 : L.len / 2 - 1;
+    scan_next();
+  }
+
+  __device__ __forceinline__ void scan_next() {
+    if (!in_read(L.i)) {
+      if (L.pass == 0 && L.kn <= L.ln) emit(L.kn, L.ln, 0, 0, 0, 0);
+      end_pass();
+      return;
+    }
+    if (L.kn > L.ln) { end_pass(); return; }
+    L.k = L.kn;
+    L.l = L.ln;
+    L.phase = PH_SCAN;
+  }
+
+  // Next branch of the current node, from L.bi on (0: deletion; 1 + 2b:
+  // insertion of b; 2 + 2b: mismatch to b).  Branches whose continuation is
+  // already past the end of the read succeed at once.
+  __device__ __forceinline__ void branch_next() {
+    while (L.bi < 9) {
+      const uint32_t bi = L.bi;
+      bool take = false;
+      if (bi == 0) {
+        if (!MM_ONLY) { L.qk = L.k; L.ql = L.l; L.cp = L.i + step(); take = true; }
+      } else {
+        const uint32_t b = (bi - 1) >> 1;
+        const uint32_t ps = L.comp ? 3u - b : b;
+        const uint32_t cb = pick4(L.ck, ps), lb = pick4(L.cl, ps);
+        const bool nonempty = cb <= lb;
+        if (bi & 1u) {
+          if (!MM_ONLY && nonempty) { L.qk = cb; L.ql = lb; L.cp = L.i; take = true; }
+        } else if (nonempty && b != L.wsym) {
+          L.qk = cb; L.ql = lb; L.cp = L.i + step(); take = true;
+        }
+      }
+      if (take) {
+        if (in_read(L.cp)) { L.phase = PH_CONT; return; }
+        emit_branch();
+      }
+      L.bi++;
+    }
+    L.i += step();
+    scan_next();
+  }
+
+  __device__ __forceinline__ void emit_branch() {
+    const uint32_t bi = L.bi;
+    if (bi == 0) emit(L.qk, L.ql, 1, FMI_DELETION, 0xFF, (int16_t)L.i);
+    else emit(L.qk, L.ql, 1, (bi & 1u) ? FMI_INSERTION : FMI_MISMATCH, (uint8_t)((bi - 1) >> 1),
+              (int16_t)L.i);
+  }
+
+  __device__ __forceinline__ void start(uint32_t task) {
+    if (P.strand_mask == 3u) { L.read = task >> 1; L.strand = task & 1u; }
+    else { L.read = task; L.strand = (P.strand_mask == 2u) ? 1u : 0u; }
+    const uint64_t off = P.offsets[L.read];
+    L.W = P.reads + off;
+    L.len = (int32_t)(P.offsets[L.read + 1] - off);
+    L.out = P.hits + (size_t)(2 * L.read + L.strand) * P.cap;
+    L.nout = 0;
+    if (P.strand_mask != 3u) P.counts[2 * L.read + (L.strand ^ 1u)] = 0;
+    enter_pass(0);
+  }
+
+  // One LF query and the transition it drives.
+  template <bool COUNT>
+  __device__ __forceinline__ void advance(unsigned long long &nblk_acc, unsigned long long &nq_acc) {
+    const bool cont = L.phase == PH_CONT;
+    const uint32_t ak = cont ? L.qk : L.k;
+    const uint32_t al = cont ? L.ql : L.l;
+    const int32_t p = cont ? L.cp : L.i;
+    const uint32_t sym = L.W[p];
+    const uint32_t m2 = al + 1;
+    const uint32_t j1 = ak / kBlockRows, j2 = m2 / kBlockRows;
+    const Blk B1 = load_block(L.blk, j1);
+    Blk B2;
+    if (j2 != j1) B2 = load_block(L.blk, j2);
+    else B2 = B1;
+    if (COUNT) { nblk_acc += (j2 != j1) ? 2u : 1u; nq_acc += 1u; }
+    const uint32_t t1 = ak - j1 * kBlockRows, t2 = m2 - j2 * kBlockRows;
+
+    if (L.phase == PH_SCAN) {
+#pragma unroll
+      for (uint32_t s = 0; s < 4; ++s) {
+        L.ck[s] = L.C[s] + 1u + occ_blk(B1, L.dollar_blk, L.dollar_off, j1, s, t1);
+        L.cl[s] = L.C[s] + occ_blk(B2, L.dollar_blk, L.dollar_off, j2, s, t2);
+      }
+      L.wsym = sym;
+      const int64_t size_node = (int64_t)L.l - (int64_t)L.k;
+      if (sym < 4u) {
+        const uint32_t ps = L.comp ? 3u - sym : sym;
+        L.kn = pick4(L.ck, ps);
+        L.ln = pick4(L.cl, ps);
+      } else {
+        L.kn = P.ix.n + 1;
+        L.ln = P.ix.n;
+      }
+      if (size_node == (int64_t)L.ln - (int64_t)L.kn) {
+        L.i += step();
+        scan_next();
+      } else {
+        L.bi = 0;
+        branch_next();
+      }
+      return;
+    }
+
+    uint32_t rk, rl;
+    if (sym < 4u) {
+      const uint32_t s = L.comp ? 3u - sym : sym;
+      const uint32_t c = pick4(L.C, s);
+      rk = c + 1u + occ_blk(B1, L.dollar_blk, L.dollar_off, j1, s, t1);
+      rl = c + occ_blk(B2, L.dollar_blk, L.dollar_off, j2, s, t2);
+    } else {
+      rk = P.ix.n + 1;
+      rl = P.ix.n;
+    }
+    if (!cont) {                     // PH_EXACT
+      if (rk > rl) { end_pass(); return; }
+      L.k = rk; L.l = rl;
+      L.i += step();
+      if (in_read(L.i) && in_exact(L.i)) return;
+      enter_scan();
+      return;
+    }
+    if (rk > rl) {                   // continuation died
+      L.bi++;
+      branch_next();
+      return;
+    }
+    L.qk = rk; L.ql = rl;
+    L.cp += step();
+    if (in_read(L.cp)) return;
+    emit_branch();
+    L.bi++;
+    branch_next();
+  }
+};
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+template <bool PERSISTENT, bool MM_ONLY, bool COUNT>
+__global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
+  S1Lane L;
+  L.phase = PH_IDLE;
+  S1<MM_ONLY> m{P, L};
+  unsigned long long nblk = 0, nq = 0;
+  const uint32_t lane = lane_id();
+
+  if (!PERSISTENT) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < P.ntasks) m.start(t);
+    while (L.phase != PH_IDLE) m.template advance<COUNT>(nblk, nq);
+  } else {
+    // Wavefront-level work queue: the wave owns tasks [wq_next, wq_end) and
+    // hands them to idle lanes by rank among the idle lanes (ballot +
+    // popcount); a new chunk is taken with one atomic when it runs dry.
+    uint32_t wq_next = 0, wq_end = 0;
+    bool more = true;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    while (true) {
+      uint64_t idle = __ballot(L.phase == PH_IDLE);
+      while (idle != 0ull && more) {
+        if (wq_next >= wq_end) {
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(P.queue, P.chunk);
+          base = __shfl(base, 0);
+          if (base >= P.ntasks) { more = false; break; }
+          wq_next = base;
+          wq_end = min(base + P.chunk, P.ntasks);
+        }
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
+        const uint32_t avail = wq_end - wq_next;
+        if (L.phase == PH_IDLE && rank < avail) m.start(wq_next + rank);
+        wq_next += min(avail, nidle);
+        idle = __ballot(L.phase == PH_IDLE);
+      }
+      if (__ballot(L.phase != PH_IDLE) == 0ull) break;
+      if (L.phase != PH_IDLE) m.template advance<COUNT>(nblk, nq);
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&P.stats[0], nblk);
+    atomicAdd(&P.stats[1], nq);
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// --------------------------------------------------------------------------
+// locate: getScompValue (search/csafm.h:208-235)
+// --------------------------------------------------------------------------
+// Each LF step of the walk needs only the row's own block: the symbol b at
+// row i and Occ(b, i+1) = Occ(b, i) + 1.
+
+__device__ __forceinline__ uint32_t sa_value(const DevDir &D, uint32_t ratio, uint32_t n,
+                                             uint32_t row) {
+  uint32_t i = row, steps = 0;
+  while (i % ratio) {
+    if (i == D.dollar) {
+      i = 0;
+    } else {
+      const uint32_t j = i / kBlockRows, t = i - j * kBlockRows;
+      const Blk B = load_block(D.blk, j);
+      const uint32_t b = code_at(B, t);
+      i = pick4(D.C, b) + occ_blk(B, D.dollar_blk, D.dollar_off, j, b, t) + 1u;
+    }
+    ++steps;
+  }
+  const uint64_t v = (uint64_t)D.S[i / ratio] + steps;
+  return (uint32_t)(v % ((uint64_t)n + 1));
+}
+
+struct LocateParams {
+  DevIndex ix;
+  uint32_t dsel, nintervals, cap;
+  const int64_t *k, *l;
+  uint64_t *pos;
+  uint32_t *nrows;
+};
+
+__global__ __launch_bounds__(256) void locate_kernel(LocateParams P) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t iv = g / P.cap;
+  const uint32_t j = (uint32_t)(g - iv * P.cap);
+  if (iv >= P.nintervals) return;
+  const int64_t k = P.k[iv], l = P.l[iv];
+  // Rows outside 0..n are not in the index: such an interval yields no rows.
+  const bool valid = k >= 0 && l <= (int64_t)P.ix.n;
+  const int64_t cnt = (valid && k <= l) ? (l - k + 1) : 0;
+  const uint32_t rows = cnt < (int64_t)P.cap ? (uint32_t)cnt : P.cap;
+  if (j == 0) P.nrows[iv] = rows;
+  if (j >= rows) return;
+  const DevDir &D = dir_of(P.ix, P.dsel);
+  P.pos[iv * P.cap + j] = sa_value(D, P.ix.ratio, P.ix.n, (uint32_t)(k + j));
+}
+
+// --------------------------------------------------------------------------
+// position resolution (bwt.c:2158-2166, binsearch search/io.h:43-64)
+// --------------------------------------------------------------------------
+
+struct ResolveParams {
+  DevIndex ix;
+  const fmi_hit *hits;
+  const uint32_t *counts;
+  const uint64_t *offsets;
+  const uint64_t *koff;     // kcount + 1 entries
+  const uint64_t *kstart;
+  uint32_t nslots, cap, row_cap;
+  uint32_t kcount;
+  int32_t *chrom;
+  uint64_t *coord;
+};
+
+__global__ __launch_bounds__(256) void resolve_kernel(ResolveParams P) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t per_slot = (uint64_t)P.cap * P.row_cap;
+  const uint64_t slot = g / per_slot;
+  if (slot >= P.nslots) return;
+  const uint32_t rem = (uint32_t)(g - slot * per_slot);
+  const uint32_t h = rem / P.row_cap, j = rem - h * P.row_cap;
+  const uint32_t stored = min(P.counts[slot], P.cap);
+  const uint64_t o = slot * per_slot + (uint64_t)h * P.row_cap + j;
+  if (h >= stored) return;
+  const fmi_hit hit = P.hits[slot * P.cap + h];
+  const bool valid = hit.k >= 0 && hit.l <= (int64_t)P.ix.n;
+  const int64_t nrows = (valid && hit.k <= hit.l) ? hit.l - hit.k + 1 : 0;
+  if ((int64_t)j >= nrows) { P.chrom[o] = -1; P.coord[o] = 0; return; }
+  const uint32_t read = (uint32_t)(slot >> 1), strand = (uint32_t)(slot & 1u);
+  const uint64_t len = P.offsets[read + 1] - P.offsets[read];
+  uint64_t len_calc = len;
+  if (hit.num_mismatches) {
+    if (hit.err_kind == FMI_DELETION) len_calc--;
+    else if (hit.err_kind == FMI_INSERTION) len_calc++;
+  }
+  const uint32_t direction = strand ? (hit.dir ? 0u : 1u) : hit.dir;
+  const uint32_t row = (uint32_t)(hit.k + j);
+  uint64_t key;
+  if (direction) key = (uint64_t)P.ix.n + 1 - sa_value(P.ix.d[1], P.ix.ratio, P.ix.n, row) - len_calc - 1;
+  else key = sa_value(P.ix.d[0], P.ix.ratio, P.ix.n, row);
+  // upper_bound of key in koff[0 .. kcount)
+  uint64_t lo = 0, size = P.kcount;
+  while (size > 0) {
+    const uint64_t w = size / 2;
+    if (P.koff[lo + w] <= key) { lo += w + 1; size -= w + 1; }
+    else size = w;
+  }
+  if (lo >= 1 && key + len_calc <= P.koff[lo]) {
+    P.chrom[o] = (int32_t)(lo - 1);
+    P.coord[o] = P.kstart[lo - 1] + key - P.koff[lo - 1];
+  } else {
+    P.chrom[o] = -1;
+    P.coord[o] = 0;
+  }
+}
+
+// --------------------------------------------------------------------------
+// random-gather roofline probe
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33; x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const uint4 *buf, uint64_t nlines,
+                                                     uint32_t vec_per_line, uint64_t per_thread,
+                                                     unsigned long long *sink) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t st = mix64(tid * 0x9E3779B97F4A7C15ull + 1);
+  uint32_t acc = 0;
+  for (uint64_t it = 0; it < per_thread; it += 4) {
+    uint64_t li[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { st = mix64(st + u + 1); li[u] = st % nlines; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint4 *p = buf + li[u] * vec_per_line;
+      for (uint32_t v = 0; v < vec_per_line; ++v) {
+        const uint4 x = p[v];
+        acc ^= x.x ^ x.y ^ x.z ^ x.w;
+      }
+    }
+  }
+  if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);   // keeps the loads live
+}
+
+// --------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------
+
+hipError_t launch_exact(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
+                        const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
+                        fmi_hit *out, fmi_hit *anchor, hipStream_t s) {
+  ExactParams P{ix, dsel, comp, direction, nreads, reads, offsets, out, anchor};
+  const uint32_t grid = (nreads + 255) / 256;
+  hipLaunchKernelGGL(exact_kernel, dim3(grid), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_vector(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
+                         const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
+                         int64_t *vk, int64_t *vl, int16_t *last_good, hipStream_t s) {
+  VectorParams P{ix, dsel, comp, direction, nreads, reads, offsets, vk, vl, last_good};
+  const uint32_t grid = (nreads + 255) / 256;
+  hipLaunchKernelGGL(vector_kernel, dim3(grid), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, bool count,
+                          uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
+                          uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
+                          uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
+                          uint32_t chunk, hipStream_t s) {
+  const uint32_t ntasks = (strand_mask == 3u) ? 2u * nreads : nreads;
+  S1Params P{ix, reads, offsets, hits, counts, queue, stats, ntasks, strand_mask, cap, chunk};
+  if (ntasks == 0) return hipSuccess;
+  if (persistent) {
+    if (grid_persistent == 0 || chunk == 0) return hipErrorInvalidValue;
+    const dim3 g(grid_persistent), b(256);
+    if (mm_only) {
+      if (count) hipLaunchKernelGGL((search1_kernel<true, true, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((search1_kernel<true, true, false>), g, b, 0, s, P);
+    } else {
+      if (count) hipLaunchKernelGGL((search1_kernel<true, false, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((search1_kernel<true, false, false>), g, b, 0, s, P);
+    }
+  } else {
+    const dim3 g((ntasks + 255) / 256), b(256);
+    if (mm_only) hipLaunchKernelGGL((search1_kernel<false, true, false>), g, b, 0, s, P);
+    else hipLaunchKernelGGL((search1_kernel<false, false, false>), g, b, 0, s, P);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
+                         uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
+                         hipStream_t s) {
+  LocateParams P{ix, dsel, nintervals, cap, k, l, pos, nrows};
+  const uint64_t total = (uint64_t)nintervals * cap;
+  const uint64_t grid = (total + 255) / 256;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(locate_kernel, dim3((uint32_t)grid), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_t *counts,
+                          const uint64_t *offsets, uint32_t nreads, uint32_t cap,
+                          uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
+                          uint32_t kcount, int32_t *chrom, uint64_t *coord, hipStream_t s) {
+  ResolveParams P{ix, hits, counts, offsets, koff, kstart, 2u * nreads, cap, row_cap, kcount,
+                  chrom, coord};
+  const uint64_t total = (uint64_t)2 * nreads * cap * row_cap;
+  const uint64_t grid = (total + 255) / 256;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(resolve_kernel, dim3((uint32_t)grid), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
+                         unsigned long long *sink, uint32_t grid, hipStream_t s) {
+  const uint64_t nlines = bytes / line_bytes;
+  const uint64_t threads = (uint64_t)grid * 256;
+  uint64_t per_thread = (nloads + threads - 1) / threads;
+  per_thread = (per_thread + 3) & ~3ull;
+  hipLaunchKernelGGL(gather_kernel, dim3(grid), dim3(256), 0, s, (const uint4 *)buf, nlines,
+                     line_bytes / 16, per_thread, sink);
+  return hipGetLastError();
+}
+
+uint64_t gather_loads_done(uint64_t nloads, uint32_t grid) {
+  const uint64_t threads = (uint64_t)grid * 256;
+  uint64_t per_thread = (nloads + threads - 1) / threads;
+  per_thread = (per_thread + 3) & ~3ull;
+  return per_thread * threads;
+}
+
+}  // namespace fmi

@@ -1,0 +1,405 @@
+"""hpgfm -- Python binding of libhpgfm.so, the MI355X FM-index search engine.
+
+The engine itself is the C ABI in include/fmi.h (HIP kernels for gfx950);
+this module wraps it with ctypes and numpy so that tests, the benchmark and
+the serving loop can drive it.  There is no CPU fallback: every search runs
+on the GPU, and loading fails loudly when the library is not built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhpgfm.so")
+
+# fmi_hit (include/fmi.h), 32 bytes
+HIT_DTYPE = np.dtype([
+    ("k", "<i8"), ("l", "<i8"),
+    ("start", "<i2"), ("pos", "<i2"), ("end", "<i2"), ("err_pos", "<i2"),
+    ("dir", "u1"), ("num_mismatches", "u1"), ("err_kind", "u1"), ("err_base", "u1"),
+    ("strand", "u1"), ("pad", "u1", (3,)),
+])
+assert HIT_DTYPE.itemsize == 32
+
+OK = 0
+MATCH, DELETION, MISMATCH, INSERTION = 0, 1, 2, 3
+VIEW_BACKWARD, VIEW_FORWARD, VIEW_BACKWARD_REV, VIEW_FORWARD_REV = 0, 1, 2, 3
+MODE_EDIT, MODE_MISMATCH = 0, 1
+STRAND_PLUS, STRAND_MINUS = 1, 2
+MAX_READ_LEN = 1000
+
+# Every symbol include/fmi.h declares; tests check the library exports them.
+EXPORTS = [
+    "fmi_status_str", "fmi_last_error", "fmi_index_build", "fmi_index_save",
+    "fmi_index_load", "fmi_index_free", "fmi_index_size_sa", "fmi_index_sa_ratio",
+    "fmi_index_device_bytes", "fmi_index_to_device", "fmi_index_get_bwt",
+    "fmi_index_set_karyotype", "fmi_device_count", "fmi_device_alloc", "fmi_device_free",
+    "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
+    "fmi_stream_destroy", "fmi_stream_sync", "fmi_last_kernel_ms", "fmi_exact_batch",
+    "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
+    "fmi_set_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
+    "fmi_search1_count_blocks",
+]
+
+
+class FmiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fmi error {code}: {msg}")
+        self.code = code
+
+
+_p = C.c_void_p
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_i = C.c_int
+_LIB = None
+
+
+def _bind(lib):
+    sig = {
+        "fmi_status_str": ([_i], C.c_char_p),
+        "fmi_last_error": ([], C.c_char_p),
+        "fmi_index_build": ([_p, _u64, _u32, C.POINTER(_p)], _i),
+        "fmi_index_save": ([_p, C.c_char_p], _i),
+        "fmi_index_load": ([C.c_char_p, C.POINTER(_p)], _i),
+        "fmi_index_free": ([_p], None),
+        "fmi_index_size_sa": ([_p], _u64),
+        "fmi_index_sa_ratio": ([_p], _u32),
+        "fmi_index_device_bytes": ([_p], _u64),
+        "fmi_index_to_device": ([_p], _i),
+        "fmi_index_get_bwt": ([_p, _i, _p, C.POINTER(_u64)], _i),
+        "fmi_index_set_karyotype": ([_p, _u64, _p, _p], _i),
+        "fmi_device_count": ([C.POINTER(_i)], _i),
+        "fmi_device_alloc": ([C.POINTER(_p), C.c_size_t], _i),
+        "fmi_device_free": ([_p], _i),
+        "fmi_memcpy_h2d": ([_p, _p, C.c_size_t, _p], _i),
+        "fmi_memcpy_d2h": ([_p, _p, C.c_size_t, _p], _i),
+        "fmi_memset_d": ([_p, _i, C.c_size_t, _p], _i),
+        "fmi_stream_create": ([C.POINTER(_p)], _i),
+        "fmi_stream_destroy": ([_p], _i),
+        "fmi_stream_sync": ([_p], _i),
+        "fmi_last_kernel_ms": ([C.POINTER(C.c_float)], _i),
+        "fmi_exact_batch": ([_p, _i, _i, _p, _p, _u32, _p, _p, _p], _i),
+        "fmi_exact_vector_batch": ([_p, _i, _i, _p, _p, _u32, _p, _p, _p, _p], _i),
+        "fmi_search1_batch": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
+        "fmi_search1_batch_plain": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
+        "fmi_set_tuning": ([_u32, _u32], _i),
+        "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
+        "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
+        "fmi_gather_bench": ([_p, _u64, _u32, _u64, _p, C.POINTER(C.c_float), C.POINTER(_u64),
+                              _p], _i),
+        "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
+                                      C.POINTER(_u64)], _i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+def lib():
+    """The loaded engine library; raises when it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FmiError(-5, f"{LIB_PATH} is not built (python -c 'import __graft_entry__ as g; g.build()')")
+        _LIB = _bind(C.CDLL(LIB_PATH))
+    return _LIB
+
+
+def check(rc):
+    if rc != OK:
+        L = lib()
+        raise FmiError(rc, f"{L.fmi_status_str(rc).decode()}: {L.fmi_last_error().decode()}")
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = lib().fmi_device_count(C.byref(n))
+    return n.value if rc == OK else 0
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().fmi_device_alloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().fmi_memcpy_h2d(self.ptr, _ptr(a), a.nbytes, stream))
+        check(lib().fmi_stream_sync(stream))
+
+    def download(self, dtype, count, stream=None):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().fmi_memcpy_d2h(_ptr(out), self.ptr, out.nbytes, stream))
+        check(lib().fmi_stream_sync(stream))
+        return out
+
+    def zero(self, stream=None):
+        check(lib().fmi_memset_d(self.ptr, 0, self.nbytes, stream))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().fmi_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib().fmi_stream_create(C.byref(s)))
+        self.handle = s
+
+    def sync(self):
+        check(lib().fmi_stream_sync(self.handle))
+
+    def close(self):
+        if self.handle is not None:
+            lib().fmi_stream_destroy(self.handle)
+            self.handle = None
+
+
+def sync(stream=None):
+    check(lib().fmi_stream_sync(stream))
+
+
+def last_kernel_ms():
+    ms = C.c_float(0)
+    check(lib().fmi_last_kernel_ms(C.byref(ms)))
+    return float(ms.value)
+
+
+def set_tuning(wg_per_cu=0, chunk=0):
+    check(lib().fmi_set_tuning(wg_per_cu, chunk))
+
+
+class Index:
+    """A two-direction FM index (host arrays, plus an HBM copy once on device)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def build(cls, text, sa_ratio=16):
+        text = np.ascontiguousarray(text, dtype=np.uint8)
+        h = C.c_void_p()
+        check(lib().fmi_index_build(_ptr(text), len(text), sa_ratio, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load(cls, path):
+        h = C.c_void_p()
+        check(lib().fmi_index_load(os.fsencode(path), C.byref(h)))
+        return cls(h)
+
+    def save(self, path):
+        check(lib().fmi_index_save(self.h, os.fsencode(path)))
+
+    def close(self):
+        if self.h is not None:
+            lib().fmi_index_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size_sa(self):
+        return int(lib().fmi_index_size_sa(self.h))
+
+    @property
+    def n(self):
+        return self.size_sa - 1
+
+    @property
+    def sa_ratio(self):
+        return int(lib().fmi_index_sa_ratio(self.h))
+
+    @property
+    def device_bytes(self):
+        return int(lib().fmi_index_device_bytes(self.h))
+
+    def to_device(self):
+        check(lib().fmi_index_to_device(self.h))
+        return self
+
+    def bwt(self, forward=0):
+        out = np.empty(max(self.n, 1), np.uint8)
+        d = C.c_uint64()
+        check(lib().fmi_index_get_bwt(self.h, forward, _ptr(out), C.byref(d)))
+        return out[:self.n], int(d.value)
+
+    def set_karyotype(self, start, end):
+        start = np.ascontiguousarray(start, dtype=np.uint64)
+        end = np.ascontiguousarray(end, dtype=np.uint64)
+        check(lib().fmi_index_set_karyotype(self.h, len(start), _ptr(start), _ptr(end)))
+
+
+def validate_reads(reads, offsets):
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    if len(offsets) < 1 or offsets[0] != 0:
+        raise ValueError("offsets must start at 0")
+    lens = np.diff(offsets.astype(np.int64))
+    if np.any(lens < 0) or np.any(lens > MAX_READ_LEN):
+        raise ValueError(f"read lengths must be in 0..{MAX_READ_LEN}")
+    if int(offsets[-1]) > len(reads):
+        raise ValueError("offsets run past the reads buffer")
+    return offsets
+
+
+class ReadBatch:
+    """Reads resident in HBM (codes as uint8, concatenated, with offsets)."""
+
+    def __init__(self, reads, offsets, stream=None):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = validate_reads(reads, offsets)
+        self.nreads = len(offsets) - 1
+        self.total = int(offsets[-1])
+        self.d_reads = DeviceBuffer.from_array(reads if len(reads) else np.zeros(1, np.uint8), stream)
+        self.d_offsets = DeviceBuffer.from_array(offsets, stream)
+        self.lengths = np.diff(offsets.astype(np.int64))
+
+    def free(self):
+        self.d_reads.free()
+        self.d_offsets.free()
+
+
+def exact_batch(ix, batch, view, direction, anchors=False, stream=None):
+    """fmi_exact_batch; returns (records, anchors or None) as host arrays."""
+    n = batch.nreads
+    out = DeviceBuffer(max(n, 1) * 32)
+    anc = DeviceBuffer(max(n, 1) * 32) if anchors else None
+    check(lib().fmi_exact_batch(ix.h, view, direction, batch.d_reads.ptr, batch.d_offsets.ptr, n,
+                                out.ptr, anc.ptr if anc else None, stream))
+    sync(stream)
+    r = out.download(HIT_DTYPE, n, stream)
+    a = anc.download(HIT_DTYPE, n, stream) if anc else None
+    return r, a
+
+
+def exact_vector_batch(ix, batch, view, direction, stream=None):
+    n = batch.nreads
+    tot = max(batch.total, 1)
+    vk = DeviceBuffer(tot * 8)
+    vl = DeviceBuffer(tot * 8)
+    lg = DeviceBuffer(max(n, 1) * 2)
+    check(lib().fmi_exact_vector_batch(ix.h, view, direction, batch.d_reads.ptr,
+                                       batch.d_offsets.ptr, n, vk.ptr, vl.ptr, lg.ptr, stream))
+    sync(stream)
+    return (vk.download(np.int64, batch.total, stream), vl.download(np.int64, batch.total, stream),
+            lg.download(np.int16, n, stream))
+
+
+class Search1Output:
+    """Device buffers for fmi_search1_batch results."""
+
+    def __init__(self, nreads, cap):
+        self.nreads, self.cap = nreads, cap
+        self.d_hits = DeviceBuffer(max(nreads * 2 * cap, 1) * 32)
+        self.d_counts = DeviceBuffer(max(nreads * 2, 1) * 4)
+
+    def counts(self, stream=None):
+        return self.d_counts.download(np.uint32, self.nreads * 2, stream)
+
+    def hits(self, stream=None):
+        return self.d_hits.download(HIT_DTYPE, self.nreads * 2 * self.cap, stream).reshape(
+            self.nreads, 2, self.cap)
+
+    def free(self):
+        self.d_hits.free()
+        self.d_counts.free()
+
+
+def search1_launch(ix, batch, out, mode=MODE_MISMATCH, strand_mask=3, plain=False, stream=None):
+    f = lib().fmi_search1_batch_plain if plain else lib().fmi_search1_batch
+    check(f(ix.h, mode, strand_mask, batch.d_reads.ptr, batch.d_offsets.ptr, batch.nreads, out.cap,
+            out.d_hits.ptr, out.d_counts.ptr, stream))
+
+
+def search1_batch(ix, batch, cap=64, mode=MODE_MISMATCH, strand_mask=3, plain=False, stream=None):
+    """Returns (counts[n, 2], hits[n, 2, cap]) on the host."""
+    out = Search1Output(batch.nreads, cap)
+    search1_launch(ix, batch, out, mode, strand_mask, plain, stream)
+    sync(stream)
+    c = out.counts(stream).reshape(batch.nreads, 2)
+    h = out.hits(stream)
+    out.free()
+    return c, h
+
+
+def search1_count_blocks(ix, batch, mode=MODE_MISMATCH, strand_mask=3):
+    b = C.c_uint64()
+    s = C.c_uint64()
+    check(lib().fmi_search1_count_blocks(ix.h, mode, strand_mask, batch.d_reads.ptr,
+                                         batch.d_offsets.ptr, batch.nreads, C.byref(b), C.byref(s)))
+    return int(b.value), int(s.value)
+
+
+def locate_batch(ix, k, l, cap, forward=0, stream=None):
+    k = np.ascontiguousarray(k, dtype=np.int64)
+    l = np.ascontiguousarray(l, dtype=np.int64)
+    n = len(k)
+    dk = DeviceBuffer.from_array(k if n else np.zeros(1, np.int64), stream)
+    dl = DeviceBuffer.from_array(l if n else np.zeros(1, np.int64), stream)
+    dp = DeviceBuffer(max(n * cap, 1) * 8)
+    dn = DeviceBuffer(max(n, 1) * 4)
+    check(lib().fmi_locate_batch(ix.h, forward, dk.ptr, dl.ptr, n, cap, dp.ptr, dn.ptr, stream))
+    sync(stream)
+    return dp.download(np.uint64, n * cap, stream).reshape(n, cap), dn.download(np.uint32, n, stream)
+
+
+def resolve_hits(ix, batch, out, row_cap, stream=None):
+    n, cap = batch.nreads, out.cap
+    tot = max(2 * n * cap * row_cap, 1)
+    dc = DeviceBuffer(tot * 4)
+    dd = DeviceBuffer(tot * 8)
+    check(lib().fmi_resolve_hits(ix.h, out.d_hits.ptr, out.d_counts.ptr, batch.d_offsets.ptr, n,
+                                 cap, row_cap, dc.ptr, dd.ptr, stream))
+    sync(stream)
+    shape = (n, 2, cap, row_cap)
+    return (dc.download(np.int32, 2 * n * cap * row_cap, stream).reshape(shape),
+            dd.download(np.uint64, 2 * n * cap * row_cap, stream).reshape(shape))
+
+
+def gather_bench(nbytes=8 << 30, line_bytes=64, nlines=1 << 28, stream=None):
+    """Random line gather over an nbytes HBM buffer; returns (ms, lines loaded)."""
+    buf = DeviceBuffer(nbytes)
+    buf.zero(stream)
+    sink = DeviceBuffer(8)
+    sink.zero(stream)
+    ms = C.c_float(0)
+    done = C.c_uint64(0)
+    check(lib().fmi_gather_bench(buf.ptr, nbytes, line_bytes, nlines, sink.ptr, C.byref(ms),
+                                 C.byref(done), stream))
+    buf.free()
+    sink.free()
+    return float(ms.value), int(done.value)

@@ -1,0 +1,128 @@
+"""Deterministic synthetic genomes and reads for the FM-index search engine.
+
+Bases are the codes 0..3 of the alphabet "ACGT" (the code of a base is its
+position in the index's alphabet string, bwt_commons.c:60-91 of the
+reference); 4 marks an N.
+
+genome():  i.i.d. bases at a given GC fraction, optionally overlaid with
+           repeat families (mutated copies of random consensus sequences)
+           and runs of N, split into named contigs.
+reads():   fixed-length or variable-length reads sampled from random
+           positions and strands, with uniform substitutions, a fraction of
+           i.i.d. random reads, and N bases.
+"""
+import numpy as np
+
+A, Cb, G, T, N = 0, 1, 2, 3, 4
+
+
+def iid_bases(rng, n, gc=0.41):
+    at = (1.0 - gc) / 2.0
+    cg = gc / 2.0
+    return rng.choice(4, size=n, p=[at, cg, cg, at]).astype(np.uint8)
+
+
+def genome(n, seed=1, gc=0.41, repeat_frac=0.0, fam_len=(300, 6000),
+           divergence=(0.02, 0.15), n_frac=0.0, n_run=(50, 5000), contigs=1,
+           copies_per_family=20):
+    """Returns (codes uint8[n], contig_starts uint64[c], contig_names list)."""
+    rng = np.random.default_rng(seed)
+    g = iid_bases(rng, n, gc)
+    if repeat_frac > 0 and n > fam_len[1] * 4:
+        target = int(repeat_frac * n)
+        covered = 0
+        while covered < target:
+            L = int(rng.integers(fam_len[0], fam_len[1] + 1))
+            cons = iid_bases(rng, L, gc)
+            for _ in range(copies_per_family):
+                if covered >= target:
+                    break
+                p = int(rng.integers(0, n - L))
+                d = float(rng.uniform(divergence[0], divergence[1]))
+                cp = cons.copy()
+                m = rng.random(L) < d
+                cp[m] = (cp[m] + rng.integers(1, 4, size=int(m.sum()))) % 4
+                if rng.random() < 0.5:
+                    cp = (3 - cp[::-1]).astype(np.uint8)
+                g[p:p + L] = cp
+                covered += L
+    if n_frac > 0:
+        target = int(n_frac * n)
+        covered = 0
+        while covered < target:
+            L = int(rng.integers(n_run[0], n_run[1] + 1))
+            p = int(rng.integers(0, max(1, n - L)))
+            g[p:p + L] = N
+            covered += L
+    if contigs > 1:
+        cuts = np.sort(rng.choice(np.arange(1000, n - 1000), size=contigs - 1, replace=False))
+        starts = np.concatenate([[0], cuts]).astype(np.uint64)
+    else:
+        starts = np.zeros(1, np.uint64)
+    names = [f"chr{i + 1}" for i in range(len(starts))]
+    return g, starts, names
+
+
+def revcomp(codes):
+    """Reverse complement of codes; N (4) stays N."""
+    c = np.asarray(codes, dtype=np.uint8)[::-1]
+    out = np.where(c < 4, 3 - c, c).astype(np.uint8)
+    return out
+
+
+def reads(g, nreads, length=100, seed=2, sub_rate=0.0, random_frac=0.0,
+          both_strands=True, n_rate=0.0, lengths=None):
+    """Samples reads from genome g.
+
+    length: fixed read length, or ignored when `lengths` (an int array) is
+    given.  Returns (concatenated codes uint8, offsets uint64[nreads+1],
+    truth dict with pos, strand, is_random)."""
+    rng = np.random.default_rng(seed)
+    n = len(g)
+    if lengths is None:
+        lengths = np.full(nreads, length, dtype=np.int64)
+    lengths = np.asarray(lengths, dtype=np.int64)
+    offsets = np.zeros(nreads + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum(lengths)
+    total = int(offsets[-1])
+    out = np.empty(total, dtype=np.uint8)
+    maxlen = int(lengths.max()) if nreads else 0
+    pos = rng.integers(0, n - maxlen + 1, size=nreads)
+    strand = rng.integers(0, 2, size=nreads) if both_strands else np.zeros(nreads, np.int64)
+    is_rand = rng.random(nreads) < random_frac
+    if nreads and np.all(lengths == lengths[0]):
+        L = int(lengths[0])
+        idx = pos[:, None] + np.arange(L)[None, :]
+        mat = g[idx]
+        rc = strand == 1
+        mat[rc] = np.where(mat[rc] < 4, 3 - mat[rc], mat[rc])[:, ::-1]
+        nr = int(is_rand.sum())
+        if nr:
+            mat[is_rand] = rng.integers(0, 4, size=(nr, L), dtype=np.uint8)
+        if sub_rate > 0:
+            m = rng.random(mat.shape) < sub_rate
+            mat[m] = (mat[m] % 4 + rng.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) % 4
+        if n_rate > 0:
+            m = rng.random(mat.shape) < n_rate
+            mat[m] = N
+        out[:] = mat.reshape(-1)
+    else:
+        for i in range(nreads):
+            L = int(lengths[i])
+            s = g[pos[i]:pos[i] + L].copy()
+            if strand[i]:
+                s = revcomp(s)
+            if is_rand[i]:
+                s = rng.integers(0, 4, size=L, dtype=np.uint8)
+            if sub_rate > 0:
+                m = rng.random(L) < sub_rate
+                s[m] = (s[m] % 4 + rng.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) % 4
+            if n_rate > 0:
+                s[rng.random(L) < n_rate] = N
+            out[int(offsets[i]):int(offsets[i + 1])] = s
+    return out, offsets, dict(pos=pos, strand=strand, is_random=is_rand)
+
+
+def lognormal_lengths(rng, nreads, median=120, sigma=0.35, lo=50, hi=250):
+    x = rng.lognormal(mean=np.log(median), sigma=sigma, size=nreads)
+    return np.clip(np.rint(x), lo, hi).astype(np.int64)

@@ -1,0 +1,230 @@
+/*
+ * fmi.h -- C ABI of the MI355X FM-index read-search engine (libhpgfm.so).
+ *
+ * Drop-in boundary for the search path of opencb/hpg-libs
+ * (c/src/aligners/bwt/search/ and the GPU glue gpu.h / gpu/gpu.cuh).  Each
+ * batch entry point below computes, for every read of a batch, exactly what
+ * the cited reference function computes for one read; the reference per-read
+ * functions are CPU code called inside an OpenMP loop (bwt.c:1112,1134), the
+ * engine runs the whole batch on one GPU.
+ *
+ * Conventions
+ *  - Plain C types only.  Buffers named d_* are device (HBM) pointers
+ *    allocated with fmi_device_alloc; everything else is host memory.
+ *  - Bases are the codes 0..3 of the alphabet "ACGT" (the reference encodes a
+ *    base as its position in config.txt's alphabet, bwt_commons.c:60-91).  A
+ *    code > 3 (N) is a symbol that never occurs in the index: an LF step on it
+ *    yields the empty interval (size_SA, size_SA - 1).  The reference leaves
+ *    this case undefined.
+ *  - Reads are concatenated; read i is reads[offsets[i] .. offsets[i+1]).
+ *    Read length is at most FMI_MAX_READ_LEN (MAXLINE, bwt_commons.h:11).
+ *  - Intervals are int64 [k, l], empty when k > l; positions are int16 like
+ *    the reference's `result` (search/results.h:13-21).
+ *  - Every call returns an fmi_status; nothing calls exit() (the reference's
+ *    check_* macros do, bwt_commons.h:48-70).  Calls that take an
+ *    fmi_stream are ordered on it and return before the GPU finishes;
+ *    fmi_stream_sync waits.
+ *  - The index is read-only after fmi_index_to_device and may be shared by
+ *    streams; calls on one stream are serialised, calls on different streams
+ *    may overlap.
+ */
+#ifndef FMI_H
+#define FMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  FMI_OK = 0,
+  FMI_EINVAL = -1,    /* bad argument                                   */
+  FMI_ENOMEM = -2,    /* host or device allocation failed               */
+  FMI_EIO = -3,       /* file could not be read or written              */
+  FMI_EFORMAT = -4,   /* index file corrupt or of another version       */
+  FMI_EDEVICE = -5,   /* HIP runtime error or no usable GPU             */
+  FMI_ERANGE = -6,    /* a size is beyond what the engine supports      */
+  FMI_ESTATE = -7     /* call out of order (e.g. index not on device)   */
+} fmi_status;
+
+#define FMI_MAX_READ_LEN 1000
+
+/* Error kinds: the codes of bwt_commons.h:15-18. */
+#define FMI_MATCH 0
+#define FMI_DELETION 1
+#define FMI_MISMATCH 2
+#define FMI_INSERTION 3
+
+/* The four logical indexes of bwt_index_new (bwt.c:682-711): BACKWARD is
+ * the BWT of the reference X, FORWARD the BWT of reverse(X); the *_REV views
+ * swap A<->T and C<->G (reverse_strand_C/O, search/csafm.c:3-100). */
+#define FMI_VIEW_BACKWARD 0
+#define FMI_VIEW_FORWARD 1
+#define FMI_VIEW_BACKWARD_REV 2
+#define FMI_VIEW_FORWARD_REV 3
+
+/* One search record: the fields of the reference's `result`
+ * (search/results.h:13-21) that a search with at most one error sets.
+ * num_mismatches is 0 for an exact record (err_* are then 0) and 1 for a
+ * record with one error (err_kind, err_base, err_pos of err_*[0]).  strand
+ * is 0 for the (backward, forward) pairing and 1 for (forward_rev,
+ * backward_rev) (bwt.c:1693-1701). 32 bytes. */
+typedef struct fmi_hit {
+  int64_t k, l;
+  int16_t start, pos, end, err_pos;
+  uint8_t dir, num_mismatches, err_kind, err_base;
+  uint8_t strand, pad[3];
+} fmi_hit;
+
+typedef struct fmi_index fmi_index;   /* opaque: host arrays + device copy */
+typedef void *fmi_stream;             /* a hipStream_t; NULL = default stream */
+
+/* ---------------------------------------------------------------- errors */
+const char *fmi_status_str(int status);
+/* Text of the last error raised on the calling thread. */
+const char *fmi_last_error(void);
+
+/* ------------------------------------------------------------- the index */
+/* Builds both directions of the index (BWT of text and of reverse(text),
+ * C/C1, occurrence blocks, SA samples every sa_ratio rows, ISA samples every
+ * sa_ratio text positions) from codes 0..3.  Replaces
+ * bwt_generate_index_files (bwt.c:744-819; calculate_C/O/S_and_R,
+ * search/preprocess.c:7-130) for an in-memory text.  n < 2^32 - 2. */
+int fmi_index_build(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_index **out);
+/* Writes / reads the single-file index (see DESIGN: versioned header,
+ * occurrence blocks, samples, karyotype).  Replaces the reference's index
+ * directory reader load_bwt_index (search/runtime.h:90-124). */
+int fmi_index_save(const fmi_index *ix, const char *path);
+int fmi_index_load(const char *path, fmi_index **out);
+void fmi_index_free(fmi_index *ix);
+/* size_SA (search/runtime.h:86): n + 1. */
+uint64_t fmi_index_size_sa(const fmi_index *ix);
+uint32_t fmi_index_sa_ratio(const fmi_index *ix);
+/* Bytes the device copy occupies. */
+uint64_t fmi_index_device_bytes(const fmi_index *ix);
+/* Copies the index into HBM of the current device (once). */
+int fmi_index_to_device(fmi_index *ix);
+/* The BWT of one direction in the reference's B.vec convention
+ * (dbwt.c:732-763: n symbols with the '$' row left out, plus its row). */
+int fmi_index_get_bwt(const fmi_index *ix, int forward, uint8_t *out_n_bytes, uint64_t *dollar);
+/* Karyotype: contig c occupies text[offset[c] .. offset[c+1]) and starts at
+ * coordinate start[c] of its chromosome (load_exome_file, search/io.c:26-80:
+ * offset[c+1] = offset[c] + end[c] - start[c] + 1). */
+int fmi_index_set_karyotype(fmi_index *ix, uint64_t count, const uint64_t *start,
+                            const uint64_t *end);
+
+/* ------------------------------------------------------ device plumbing */
+int fmi_device_count(int *count);
+int fmi_device_alloc(void **d_ptr, size_t bytes);
+int fmi_device_free(void *d_ptr);
+int fmi_memcpy_h2d(void *d_dst, const void *src, size_t bytes, fmi_stream s);
+int fmi_memcpy_d2h(void *dst, const void *d_src, size_t bytes, fmi_stream s);
+int fmi_memset_d(void *d_dst, int value, size_t bytes, fmi_stream s);
+int fmi_stream_create(fmi_stream *s);
+int fmi_stream_destroy(fmi_stream s);
+int fmi_stream_sync(fmi_stream s);
+/* Milliseconds the last kernel launched by the calling thread took on the
+ * GPU (HIP events around that launch; valid after fmi_stream_sync). */
+int fmi_last_kernel_ms(float *ms);
+
+/* ------------------------------------------------------- exact search */
+/* BWExactSearchBackward (direction 0, search/search.h:9-36) or
+ * BWExactSearchForward (direction 1, search/search.h:38-65) of every whole
+ * read on `view`, starting from [0, size_SA - 1]: the backward search
+ * starts at pos = len-1, the forward one at pos = 0.  Per read, d_out gets the
+ * final record (k, l, pos; start = 0, end = len - 1) and, when d_anchor is
+ * not NULL, the anchor record (last non-empty interval).  This is the kernel
+ * pair of gpu/gpu.cu:333-383 with the reference's per-read semantics.
+ * Pairing for reverse-complement search: FORWARD direction on
+ * BACKWARD_REV (bwt.c:1233-1237). */
+int fmi_exact_batch(const fmi_index *ix, int view, int direction,
+                    const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                    fmi_hit *d_out, fmi_hit *d_anchor, fmi_stream s);
+
+/* BWExactSearchVectorBackward / Forward (search/search.c:1407-1489) of every
+ * whole read from [0, size_SA - 1]: d_vec_k/l[offsets[i] + j] is the interval
+ * after the j-th step (same indexing as the reference's vec_k[j]);
+ * d_last_good[i] is the return value. */
+int fmi_exact_vector_batch(const fmi_index *ix, int view, int direction,
+                           const uint8_t *d_reads, const uint64_t *d_offsets,
+                           uint32_t nreads, int64_t *d_vec_k, int64_t *d_vec_l,
+                           int16_t *d_last_good, fmi_stream s);
+
+/* -------------------------------------------------- 1-error search */
+#define FMI_MODE_EDIT 0      /* mismatches, insertions and deletions    */
+#define FMI_MODE_MISMATCH 1  /* the {exact, MISMATCH} projection         */
+
+#define FMI_STRAND_PLUS 1
+#define FMI_STRAND_MINUS 2
+
+/* BWSearch1CPU (search/search.c:1876-1933, with BWSimpleSearch1Backward/
+ * Forward :1711-1874) on every read, for the strands in strand_mask: strand
+ * 0 runs (backward, forward), strand 1 runs (forward_rev, backward_rev)
+ * (bwt.c:1690-1701), each with res = {k=0, l=size_SA-1, start=0,
+ * end=len-1}.  d_hits[(2*i + s) * cap ...] receives the records of read i,
+ * strand s in the reference's order; d_counts[2*i + s] the total number of
+ * records, which may exceed cap: then the first cap are stored, which is
+ * the list the reference's results_list of capacity cap holds when its
+ * add_result reports full (search/results.h:93-103).  MODE_MISMATCH keeps
+ * only exact and MISMATCH records, in the same relative order.
+ * Runs as one persistent launch with a device work queue. */
+int fmi_search1_batch(const fmi_index *ix, int mode, int strand_mask,
+                      const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                      uint32_t cap, fmi_hit *d_hits, uint32_t *d_counts, fmi_stream s);
+
+/* The same results from the plain design of the reference's GPU glue
+ * (gpu.cu:118-203): one thread per (read, strand), no work queue, one launch
+ * per batch.  Kept as the comparison point for the persistent launch. */
+int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
+                            const uint8_t *d_reads, const uint64_t *d_offsets,
+                            uint32_t nreads, uint32_t cap, fmi_hit *d_hits,
+                            uint32_t *d_counts, fmi_stream s);
+
+/* Tuning of the persistent launch (the autotuner writes it): workgroups per
+ * CU, tasks a wavefront takes from the queue at a time. 0 = default. */
+int fmi_set_tuning(uint32_t wg_per_cu, uint32_t chunk);
+
+/* ------------------------------------------------------------ locate */
+/* get_SA (search/runtime.h:87, getScompValue search/csafm.h:208-235) for
+ * every row of the intervals [d_k[i], d_l[i]] (empty intervals give no
+ * rows), at most cap rows per interval starting at k: d_pos[i*cap + j] =
+ * SA(k_i + j) of the BACKWARD (forward = 0) or FORWARD (forward = 1) index;
+ * d_nrows[i] = min(l - k + 1, cap) (0 when empty). */
+int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k,
+                     const int64_t *d_l, uint32_t nintervals, uint32_t cap,
+                     uint64_t *d_pos, uint32_t *d_nrows, fmi_stream s);
+
+/* Position resolution of bwt.c:2158-2166 for 1-error records: for each hit
+ * (as stored by fmi_search1_batch: hits[(2*i+s)*cap ...], counts) and each
+ * SA row j of [k, l] (at most row_cap rows per hit), key =
+ *   direction ? size_SA - SA_fwd(j) - len_calc - 1 : SA_bwd(j),
+ * direction = strand ? !dir : dir, len_calc = len -1 for a DELETION and
+ * len + 1 for an INSERTION; then idx = binsearch(offset, key) and the
+ * mapping is kept when key + len_calc <= offset[idx], at coordinate
+ * start[idx-1] + key - offset[idx-1].  Per stored hit h (in the flat order
+ * of the stored records) d_chrom[h*row_cap + j] = idx - 1 or -1 when
+ * rejected, d_coord[...] the coordinate. */
+int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
+                     const uint64_t *d_offsets, uint32_t nreads, uint32_t cap,
+                     uint32_t row_cap, int32_t *d_chrom, uint64_t *d_coord, fmi_stream s);
+
+/* ------------------------------------------------------ measurement */
+/* Random gather of line_bytes-sized aligned lines over d_buf (bytes >= 64
+ * MiB), at least nlines loads: the random-access roofline of the path.
+ * *ms is the kernel time and *lines_done the loads performed. */
+int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes,
+                     uint64_t nlines, uint64_t *d_sink, float *ms, uint64_t *lines_done,
+                     fmi_stream s);
+/* Counts the distinct 64-byte occurrence blocks one fmi_search1_batch with
+ * the same arguments fetches (the algorithmic bytes of the launch) and the
+ * LF steps it takes. */
+int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
+                             const uint8_t *d_reads, const uint64_t *d_offsets,
+                             uint32_t nreads, uint64_t *blocks, uint64_t *lf_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

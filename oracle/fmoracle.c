@@ -1,0 +1,527 @@
+/*
+ * fmoracle.c -- CPU restatement of the reference FM-index search path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the smoke check and
+ * the CPU-baseline leg of the benchmark load this library; the HIP engine
+ * never links or calls it.
+ *
+ * Reference paths are relative to c/src/aligners/bwt/ of opencb/hpg-libs.
+ * Each function states the reference lines whose behaviour it restates.
+ * Symbols are the codes 0..3 of the alphabet "ACGT" (bwt_commons.c:60-91);
+ * a read code > 3 (an N) is this build's policy: a fifth symbol, after T,
+ * that never occurs in the text, so a LF step on it yields the empty
+ * interval (n + 1, n).  The reference leaves that case undefined (its base
+ * table only sets ACGT).
+ */
+#include "fmoracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* builder                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Prefix doubling over (rank[i], rank[i+h]) pairs; '$' has rank 0 and is the
+ * unique smallest symbol, so ties never need positions beyond the text. */
+static const int64_t *g_rank;
+static uint64_t g_h, g_n1;
+static int cmp_suffix(const void *a, const void *b) {
+  uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  if (g_rank[x] != g_rank[y]) return g_rank[x] < g_rank[y] ? -1 : 1;
+  int64_t rx = x + g_h < g_n1 ? g_rank[x + g_h] : -1;
+  int64_t ry = y + g_h < g_n1 ? g_rank[y + g_h] : -1;
+  return rx < ry ? -1 : (rx > ry ? 1 : 0);
+}
+
+int fmo_sa_naive(const uint8_t *text, uint64_t n, uint64_t *sa) {
+  uint64_t n1 = n + 1;
+  int64_t *rank = (int64_t *)malloc(n1 * sizeof(int64_t));
+  int64_t *tmp = (int64_t *)malloc(n1 * sizeof(int64_t));
+  if (!rank || !tmp) { free(rank); free(tmp); return -1; }
+  for (uint64_t i = 0; i < n; i++) { rank[i] = (int64_t)text[i] + 1; sa[i] = i; }
+  rank[n] = 0; sa[n] = n;
+  g_rank = rank; g_n1 = n1;
+  for (uint64_t h = 1;; h <<= 1) {
+    g_h = h;
+    qsort(sa, n1, sizeof(uint64_t), cmp_suffix);
+    tmp[sa[0]] = 0;
+    for (uint64_t i = 1; i < n1; i++)
+      tmp[sa[i]] = tmp[sa[i - 1]] + (cmp_suffix(&sa[i - 1], &sa[i]) < 0 ? 1 : 0);
+    memcpy(rank, tmp, n1 * sizeof(int64_t));
+    if ((uint64_t)rank[sa[n1 - 1]] == n1 - 1) break;
+    if (h > n1) break;
+  }
+  free(rank); free(tmp);
+  return 0;
+}
+
+/* B.vec convention of dbwt.c:732-763: the n symbols of the BWT with the '$'
+ * row left out; `dollar` is the row that holds '$' (the row of suffix 0). */
+void fmo_bwt_from_sa(const uint8_t *text, uint64_t n, const uint64_t *sa,
+                     uint8_t *bwt, uint64_t *dollar) {
+  uint64_t o = 0;
+  for (uint64_t i = 0; i <= n; i++) {
+    if (sa[i] == 0) { *dollar = i; continue; }
+    bwt[o++] = text[sa[i] - 1];
+  }
+}
+
+static inline uint8_t bwt_row(const uint8_t *bwt, uint64_t dollar, uint64_t i) {
+  /* row i of the full BWT, i != dollar (preprocess.c:33-34) */
+  return i > dollar ? bwt[i - 1] : bwt[i];
+}
+
+static inline uint64_t occ_dir(const fmo_dir *d, unsigned b, uint64_t m) {
+  /* get_O, csafm.h:166-181 (FM_COMP_64): checkpoint + popcount of the bits
+   * of rows < m within word m/64. */
+  uint64_t w = m >> 6;
+  unsigned sh = 63u - (unsigned)(m & 63u);
+  return (uint64_t)d->desp[b][w] + (uint64_t)__builtin_popcountll(d->count[b][w] << sh);
+}
+
+static inline unsigned bwt_sym_dir(const fmo_dir *d, uint64_t m) {
+  /* get_B_from_O, csafm.h:183-206: probe bit (m+1)%64 of each symbol's
+   * bitmap; 0xFF for the '$' row. */
+  m++;
+  uint64_t w = m >> 6;
+  unsigned bit = (unsigned)(m & 63u);
+  for (unsigned b = 0; b < 4; b++)
+    if ((d->count[b][w] >> bit) & 1u) return b;
+  return 0xFFu;
+}
+
+int fmo_dir_from_bwt(fmo_dir *d, const uint8_t *bwt, uint64_t n, uint64_t dollar,
+                     uint32_t ratio) {
+  memset(d, 0, sizeof(*d));
+  d->n = n; d->dollar = dollar; d->ratio = ratio;
+  /* calculate_C, preprocess.c:44-72 */
+  uint64_t cnt[4] = {0, 0, 0, 0};
+  for (uint64_t i = 0; i < n; i++) cnt[bwt[i] & 3]++;
+  uint64_t acc = 0;
+  for (int b = 0; b < 4; b++) { d->C[b] = (uint32_t)acc; d->C1[b] = (uint32_t)(acc + 1); acc += cnt[b]; }
+  /* calculate_O, preprocess.c:74-130 */
+  uint64_t siz = n + 2;
+  d->words = siz / 64 + (siz % 64 ? 1 : 0);
+  for (int b = 0; b < 4; b++) {
+    d->desp[b] = (uint32_t *)calloc(d->words, sizeof(uint32_t));
+    d->count[b] = (uint64_t *)calloc(d->words, sizeof(uint64_t));
+    if (!d->desp[b] || !d->count[b]) return -1;
+  }
+  uint64_t run[4] = {0, 0, 0, 0};
+  for (uint64_t row = 0; row <= n; row++) {
+    uint64_t pos = (row + 1) >> 6;
+    unsigned bit = (unsigned)((row + 1) & 63u);
+    if (bit == 0)
+      for (int b = 0; b < 4; b++) d->desp[b][pos] = (uint32_t)run[b];
+    if (row == dollar) continue;
+    unsigned s = bwt_row(bwt, dollar, row);
+    run[s]++;
+    d->count[s][pos] |= (uint64_t)1 << bit;
+  }
+  /* calculate_S_and_R, preprocess.c:7-42: walk LF from row 0 (suffix n)
+   * down to suffix 1, sampling S by row and R by text position. */
+  uint64_t n1 = n + 1;
+  d->nsamp = n1 / ratio + (n1 % ratio ? 1 : 0);
+  d->S = (uint32_t *)calloc(d->nsamp, sizeof(uint32_t));
+  d->R = (uint32_t *)calloc(d->nsamp, sizeof(uint32_t));
+  if (!d->S || !d->R) return -1;
+  if (dollar % ratio == 0) d->S[dollar / ratio] = 0;
+  d->R[0] = (uint32_t)dollar;
+  uint64_t i = 0;
+  for (uint64_t j = n1 - 1; j > 0; j--) {
+    if (i % ratio == 0) d->S[i / ratio] = (uint32_t)j;
+    if (j % ratio == 0) d->R[j / ratio] = (uint32_t)i;
+    unsigned s = bwt_row(bwt, dollar, i);
+    i = d->C[s] + occ_dir(d, s, i + 1);
+  }
+  return 0;
+}
+
+void fmo_dir_free(fmo_dir *d) {
+  for (int b = 0; b < 4; b++) { free(d->desp[b]); free(d->count[b]); }
+  free(d->S); free(d->R);
+  memset(d, 0, sizeof(*d));
+}
+
+int fmo_index_from_bwts(fmo_index *ix, const uint8_t *bwt, uint64_t dollar,
+                        const uint8_t *bwt_rev, uint64_t dollar_rev, uint64_t n,
+                        uint32_t ratio) {
+  memset(ix, 0, sizeof(*ix));
+  if (fmo_dir_from_bwt(&ix->bwd, bwt, n, dollar, ratio)) return -1;
+  if (fmo_dir_from_bwt(&ix->fwd, bwt_rev, n, dollar_rev, ratio)) return -1;
+  return 0;
+}
+
+int fmo_index_build_naive(fmo_index *ix, const uint8_t *text, uint64_t n, uint32_t ratio) {
+  uint64_t *sa = (uint64_t *)malloc((n + 1) * sizeof(uint64_t));
+  uint8_t *rev = (uint8_t *)malloc(n + 1);
+  uint8_t *b1 = (uint8_t *)malloc(n + 1), *b2 = (uint8_t *)malloc(n + 1);
+  int rc = -1;
+  if (!sa || !rev || !b1 || !b2) goto out;
+  uint64_t d1, d2;
+  if (fmo_sa_naive(text, n, sa)) goto out;
+  fmo_bwt_from_sa(text, n, sa, b1, &d1);
+  for (uint64_t i = 0; i < n; i++) rev[i] = text[n - 1 - i];   /* Xi, bwt.c:791-793 */
+  if (fmo_sa_naive(rev, n, sa)) goto out;
+  fmo_bwt_from_sa(rev, n, sa, b2, &d2);
+  rc = fmo_index_from_bwts(ix, b1, d1, b2, d2, n, ratio);
+out:
+  free(sa); free(rev); free(b1); free(b2);
+  return rc;
+}
+
+void fmo_index_free(fmo_index *ix) { fmo_dir_free(&ix->bwd); fmo_dir_free(&ix->fwd); }
+
+uint64_t fmo_dir_words(const fmo_index *ix, int fwd) { return fwd ? ix->fwd.words : ix->bwd.words; }
+
+void fmo_dir_export(const fmo_index *ix, int fwd, uint32_t *C, uint32_t *C1,
+                    uint32_t *desp, uint64_t *count, uint32_t *S, uint32_t *R,
+                    uint64_t *dollar, uint64_t *nsamp) {
+  const fmo_dir *d = fwd ? &ix->fwd : &ix->bwd;
+  for (int b = 0; b < 4; b++) {
+    if (C) C[b] = d->C[b];
+    if (C1) C1[b] = d->C1[b];
+    if (desp) memcpy(desp + (size_t)b * d->words, d->desp[b], d->words * sizeof(uint32_t));
+    if (count) memcpy(count + (size_t)b * d->words, d->count[b], d->words * sizeof(uint64_t));
+  }
+  if (S) memcpy(S, d->S, d->nsamp * sizeof(uint32_t));
+  if (R) memcpy(R, d->R, d->nsamp * sizeof(uint32_t));
+  if (dollar) *dollar = d->dollar;
+  if (nsamp) *nsamp = d->nsamp;
+}
+
+/* ------------------------------------------------------------------------ */
+/* runtime                                                                  */
+/* ------------------------------------------------------------------------ */
+
+static inline const fmo_dir *view_dir(const fmo_index *ix, int view) {
+  return (view == FMO_VIEW_BACKWARD || view == FMO_VIEW_BACKWARD_REV) ? &ix->bwd : &ix->fwd;
+}
+static inline int view_comp(int view) { return view >= FMO_VIEW_BACKWARD_REV; }
+
+uint64_t fmo_size_sa(const fmo_index *ix) { return ix->bwd.n + 1; }   /* S.siz, runtime.h:86 */
+
+uint64_t fmo_occ(const fmo_index *ix, int view, unsigned b, uint64_t m) {
+  const fmo_dir *d = view_dir(ix, view);
+  return occ_dir(d, view_comp(view) ? 3u - b : b, m);
+}
+
+/* BWiteration, runtime.h:38-42, on the view's (possibly complement-swapped,
+ * csafm.c:3-100) C/C1/O. */
+static inline void lf(const fmo_dir *d, int comp, int64_t k, int64_t l, unsigned b,
+                      int64_t *ko, int64_t *lo) {
+  if (b > 3u) {               /* N: C1 = n + 1, C = n, never occurs */
+    *ko = (int64_t)d->n + 1; *lo = (int64_t)d->n; return;
+  }
+  unsigned s = comp ? 3u - b : b;
+  *ko = (int64_t)((uint64_t)d->C1[s] + occ_dir(d, s, (uint64_t)k));
+  *lo = (int64_t)((uint64_t)d->C[s] + occ_dir(d, s, (uint64_t)l + 1));
+}
+
+void fmo_lf(const fmo_index *ix, int view, int64_t k, int64_t l, unsigned b,
+            int64_t *ko, int64_t *lo) {
+  lf(view_dir(ix, view), view_comp(view), k, l, b, ko, lo);
+}
+
+uint64_t fmo_get_sa(const fmo_index *ix, int fwd, uint64_t m) {
+  /* getScompValue, csafm.h:208-235 */
+  const fmo_dir *d = fwd ? &ix->fwd : &ix->bwd;
+  uint64_t i = m, j = 0;
+  while (i % d->ratio) {
+    unsigned b = bwt_sym_dir(d, i);
+    if (b == 0xFFu) i = 0;
+    else i = d->C[b] + occ_dir(d, b, i + 1);
+    j++;
+  }
+  return ((uint64_t)d->S[i / d->ratio] + j) % (d->n + 1);
+}
+
+uint64_t fmo_get_isa(const fmo_index *ix, int fwd, uint64_t m) {
+  /* getRcompValue, csafm.h:237-272 */
+  const fmo_dir *d = fwd ? &ix->fwd : &ix->bwd;
+  uint64_t siz = d->n + 1;
+  uint64_t i = (d->ratio - (m % d->ratio)) % d->ratio;
+  uint64_t k = m + i, j;
+  if (k < siz) j = d->R[k / d->ratio];
+  else { j = d->R[0]; i = siz - m; }
+  while (i) {
+    unsigned b = bwt_sym_dir(d, j);
+    if (b == 0xFFu) j = 0;
+    else j = d->C[b] + occ_dir(d, b, j + 1);
+    i--;
+  }
+  return j;
+}
+
+/* ------------------------------------------------------------------------ */
+/* search                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* BWExactSearchBackward / BWExactSearchForward, search/search.h:9-65.
+ * The last non-empty interval before death is the anchor; both start from
+ * (k, l) = (0, 0) when no step succeeds. */
+static void exact_dir(const fmo_dir *d, int comp, int dir, const uint8_t *W,
+                      fmo_hit *r, fmo_hit *anchor) {
+  int64_t k = 0, l = 0, k2 = r->k, l2 = r->l;
+  int16_t i;
+  if (dir == 0) {
+    for (i = r->pos; i >= r->start; i--) {
+      lf(d, comp, k2, l2, W[i], &k2, &l2);
+      if (k2 > l2) break;
+      k = k2; l = l2;
+    }
+    if (anchor) {
+      anchor->dir = 0; anchor->num_mismatches = 0;
+      anchor->start = (int16_t)(i + 1); anchor->end = r->pos;
+      anchor->k = k; anchor->l = l; anchor->pos = i;
+    }
+  } else {
+    for (i = r->pos; i <= r->end; i++) {
+      lf(d, comp, k2, l2, W[i], &k2, &l2);
+      if (k2 > l2) break;
+      k = k2; l = l2;
+    }
+    if (anchor) {
+      anchor->dir = 1; anchor->num_mismatches = 0;
+      anchor->start = r->pos; anchor->end = (int16_t)(i - 1);
+      anchor->k = k; anchor->l = l; anchor->pos = i;
+    }
+  }
+  r->k = k2; r->l = l2; r->pos = i;
+}
+
+void fmo_exact(const fmo_index *ix, int view, int dir, const uint8_t *W,
+               fmo_hit *r, fmo_hit *anchor) {
+  exact_dir(view_dir(ix, view), view_comp(view), dir, W, r, anchor);
+}
+
+int16_t fmo_exact_vector(const fmo_index *ix, int view, int dir, const uint8_t *W,
+                         int16_t start, int16_t end, int64_t k, int64_t l,
+                         int64_t *vec_k, int64_t *vec_l) {
+  /* BWExactSearchVectorBackward (search.c:1407-1447) and
+   * BWExactSearchVectorForward (search.c:1449-1489): the interval after
+   * each step; after death the dead interval is repeated to the end. */
+  const fmo_dir *d = view_dir(ix, view);
+  int comp = view_comp(view);
+  int16_t good = 0, i, j;
+  if (k > l || start > end) return 0;
+  int64_t k2 = k, l2 = l;
+  if (dir == 0) {
+    for (i = end, j = (int16_t)(end - start); i >= start; i--, j--) {
+      lf(d, comp, k2, l2, W[i], &k2, &l2);
+      vec_k[j] = k2; vec_l[j] = l2;
+      if (k2 > l2) { i--; j--; break; }
+      good++;
+    }
+    for (; i >= start; i--, j--) { vec_k[j] = k2; vec_l[j] = l2; }
+  } else {
+    for (i = start, j = 0; i <= end; i++, j++) {
+      lf(d, comp, k2, l2, W[i], &k2, &l2);
+      vec_k[j] = k2; vec_l[j] = l2;
+      if (k2 > l2) { i++; j++; break; }
+      good++;
+    }
+    for (; i <= end; i++, j++) { vec_k[j] = k2; vec_l[j] = l2; }
+  }
+  return good;
+}
+
+typedef struct {
+  const uint8_t *W;
+  fmo_hit *out;
+  uint32_t cap, cnt;
+  int mm_only;
+  uint8_t strand;
+} s1_sink;
+
+static inline void sink_add(s1_sink *s, const fmo_hit *h) {
+  /* add_result (results.h:93-103) keeps the first max_results records; the
+   * reference then stops, which leaves the kept prefix unchanged.  Here the
+   * search goes on so that the total count is known. */
+  if (s->cnt < s->cap) { s->out[s->cnt] = *h; s->out[s->cnt].strand = s->strand; }
+  s->cnt++;
+}
+
+/* BWSimpleSearch1Backward (dir 0, search.c:1711-1793) and
+ * BWSimpleSearch1Forward (dir 1, search.c:1795-1874).  res carries the
+ * surviving exact interval; on exit res->k,l hold the exact extension over
+ * the scanned half (or the dead interval) and res->pos = -1. */
+static void simple_search1(const fmo_dir *d, int comp, int dir, s1_sink *s, fmo_hit *res) {
+  const uint8_t *W = s->W;
+  int16_t start, end;
+  if (dir == 0) { start = res->start; end = res->pos; }
+  else          { start = res->pos;   end = res->end; }
+  const int step = dir ? 1 : -1;
+
+  fmo_hit r;
+  memset(&r, 0, sizeof(r));
+  r.dir = (uint8_t)dir; r.start = start; r.end = end;
+  r.num_mismatches = 1; r.err_kind = FMO_MATCH; r.err_base = 0xFF; r.err_pos = start;
+
+  int64_t kn = res->k, ln = res->l, k, l, ka, la;
+  int64_t size_next = ln - kn, size;
+  int16_t i = dir ? start : end;
+  for (; dir ? (i <= end) : (i >= start); i = (int16_t)(i + step)) {
+    k = kn; l = ln;
+    if (k > l) { res->k = k; res->l = l; res->pos = -1; return; }
+    lf(d, comp, k, l, W[i], &kn, &ln);
+    size = size_next; size_next = ln - kn;
+    if (size == size_next) continue;   /* every occurrence extends by W[i] */
+
+    if (!s->mm_only) {                 /* deletion of W[i] */
+      r.k = k; r.l = l; r.pos = (int16_t)(i + step);
+      exact_dir(d, comp, dir, W, &r, NULL);
+      if (r.k <= r.l) {
+        r.err_kind = FMO_DELETION; r.err_base = 0xFF; r.err_pos = i;
+        sink_add(s, &r);
+      }
+    }
+    for (unsigned b = 0; b < 4; b++) {
+      lf(d, comp, k, l, b, &ka, &la);
+      if (ka > la) continue;
+      if (!s->mm_only) {               /* insertion of b before W[i] */
+        r.k = ka; r.l = la; r.pos = i;
+        exact_dir(d, comp, dir, W, &r, NULL);
+        if (r.k <= r.l) {
+          r.err_kind = FMO_INSERTION; r.err_base = (uint8_t)b; r.err_pos = i;
+          sink_add(s, &r);
+        }
+      }
+      if (b != (unsigned)W[i]) {       /* W[i] -> b */
+        r.k = ka; r.l = la; r.pos = (int16_t)(i + step);
+        exact_dir(d, comp, dir, W, &r, NULL);
+        if (r.k <= r.l) {
+          r.err_kind = FMO_MISMATCH; r.err_base = (uint8_t)b; r.err_pos = i;
+          sink_add(s, &r);
+        }
+      }
+    }
+  }
+  res->k = kn; res->l = ln; res->pos = -1;
+}
+
+uint32_t fmo_search1(const fmo_index *ix, int strand, int mismatch_only,
+                     const uint8_t *W, int16_t len, fmo_hit *out, uint32_t cap,
+                     fmo_hit *anchors) {
+  /* BWSearch1CPU, search.c:1876-1933.  Strand pairing of bwt.c:1693-1701:
+   * + strand (backward, forward); - strand (forward_rev, backward_rev). */
+  const int vb = strand ? FMO_VIEW_FORWARD_REV : FMO_VIEW_BACKWARD;
+  const int vf = strand ? FMO_VIEW_BACKWARD_REV : FMO_VIEW_FORWARD;
+  const fmo_dir *db = view_dir(ix, vb), *df = view_dir(ix, vf);
+  const int cb = view_comp(vb), cf = view_comp(vf);
+  s1_sink s = {W, out, cap, 0, mismatch_only, (uint8_t)strand};
+
+  int16_t start = 0, end = (int16_t)(len - 1);
+  int16_t n = (int16_t)(end - start + 1), half = (int16_t)(n / 2);
+  int64_t k0 = 0, l0 = (int64_t)fmo_size_sa(ix) - 1;
+
+  fmo_hit r;
+  memset(&r, 0, sizeof(r));
+  r.dir = 0; r.start = half; r.end = end; r.k = k0; r.l = l0; r.pos = end;
+  exact_dir(db, cb, 0, W, &r, anchors ? &anchors[0] : NULL);
+  if (r.k <= r.l) {
+    r.start = start; r.pos = (int16_t)(half - 1);
+    simple_search1(db, cb, 0, &s, &r);
+    if (r.k <= r.l) sink_add(&s, &r);   /* the exact match */
+  }
+
+  half--;
+  memset(&r, 0, sizeof(r));
+  r.dir = 1; r.start = start; r.end = half; r.k = k0; r.l = l0; r.pos = start;
+  exact_dir(df, cf, 1, W, &r, anchors ? &anchors[1] : NULL);
+  if (r.k <= r.l) {
+    r.pos = (int16_t)(half + 1); r.end = end;
+    simple_search1(df, cf, 1, &s, &r);
+  }
+  if (anchors) { anchors[0].strand = (uint8_t)strand; anchors[1].strand = (uint8_t)strand; }
+  return s.cnt;
+}
+
+uint64_t fmo_binsearch(const uint64_t *array, uint64_t size, uint64_t key) {
+  /* binsearch, search/io.h:43-64: first index whose value is > key. */
+  if (!array) return 0;
+  uint64_t lo = 0;
+  while (size > 0) {
+    uint64_t w = size / 2;
+    if (array[lo + w] <= key) { lo += w + 1; size -= w + 1; }
+    else size = w;
+  }
+  return lo;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batch drivers (CPU baseline)                                             */
+/* ------------------------------------------------------------------------ */
+
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+double fmo_search1_batch(const fmo_index *ix, const uint8_t *reads,
+                         const uint64_t *offsets, uint64_t nreads, int mismatch_only,
+                         uint32_t cap, fmo_hit *hits, uint32_t *counts, int threads) {
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+#endif
+  double t0 = now_s();
+#ifdef _OPENMP
+#pragma omp parallel
+  {
+    fmo_hit *scratch = hits ? NULL : (fmo_hit *)malloc((size_t)(cap ? cap : 1) * sizeof(fmo_hit));
+#pragma omp for schedule(dynamic, chunk)
+#else
+  {
+    fmo_hit *scratch = hits ? NULL : (fmo_hit *)malloc((size_t)(cap ? cap : 1) * sizeof(fmo_hit));
+#endif
+    for (long i = 0; i < (long)nreads; i++) {
+      const uint8_t *W = reads + offsets[i];
+      int16_t len = (int16_t)(offsets[i + 1] - offsets[i]);
+      for (int s = 0; s < 2; s++) {
+        fmo_hit *dst = hits ? hits + ((size_t)i * 2 + s) * cap : scratch;
+        counts[2 * i + s] = fmo_search1(ix, s, mismatch_only, W, len, dst, cap, NULL);
+      }
+    }
+    free(scratch);
+  }
+  return now_s() - t0;
+}
+
+double fmo_exact_batch(const fmo_index *ix, const uint8_t *reads,
+                       const uint64_t *offsets, uint64_t nreads, int64_t *out_kl,
+                       int threads) {
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+#endif
+  double t0 = now_s();
+  int64_t lsa = (int64_t)fmo_size_sa(ix) - 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, chunk)
+#endif
+  for (long i = 0; i < (long)nreads; i++) {
+    const uint8_t *W = reads + offsets[i];
+    int16_t len = (int16_t)(offsets[i + 1] - offsets[i]);
+    fmo_hit r;
+    memset(&r, 0, sizeof(r));
+    r.k = 0; r.l = lsa; r.start = 0; r.end = (int16_t)(len - 1); r.pos = (int16_t)(len - 1);
+    fmo_exact(ix, FMO_VIEW_BACKWARD, 0, W, &r, NULL);
+    out_kl[4 * i + 0] = r.k; out_kl[4 * i + 1] = r.l;
+    memset(&r, 0, sizeof(r));
+    r.k = 0; r.l = lsa; r.start = 0; r.end = (int16_t)(len - 1); r.pos = 0;
+    fmo_exact(ix, FMO_VIEW_BACKWARD_REV, 1, W, &r, NULL);
+    out_kl[4 * i + 2] = r.k; out_kl[4 * i + 3] = r.l;
+  }
+  return now_s() - t0;
+}

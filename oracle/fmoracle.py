@@ -1,0 +1,371 @@
+"""ctypes loader for the checker libraries (test infrastructure only).
+
+`Oracle` wraps oracle/libfmoracle.so, the C restatement of the reference
+search path.  `RefCore` wraps oracle/_ref/libfmref_ref.so, the reference's
+own search core compiled from its sources (present only where it was built).
+Only tests/, the smoke check and the CPU-baseline leg of bench.py import this
+module; the HIP engine never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libfmoracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libfmref_ref.so")
+
+# Same layout as fmo_hit / fmi_hit (32 bytes).
+HIT_DTYPE = np.dtype([
+    ("k", "<i8"), ("l", "<i8"),
+    ("start", "<i2"), ("pos", "<i2"), ("end", "<i2"), ("err_pos", "<i2"),
+    ("dir", "u1"), ("num_mismatches", "u1"), ("err_kind", "u1"), ("err_base", "u1"),
+    ("strand", "u1"), ("pad", "u1", (3,)),
+])
+assert HIT_DTYPE.itemsize == 32
+
+MATCH, DELETION, MISMATCH, INSERTION = 0, 1, 2, 3
+VIEW_BACKWARD, VIEW_FORWARD, VIEW_BACKWARD_REV, VIEW_FORWARD_REV = 0, 1, 2, 3
+
+_p = C.c_void_p
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class _FmoDir(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("dollar", C.c_uint64),
+                ("C", C.c_uint32 * 4), ("C1", C.c_uint32 * 4),
+                ("words", C.c_uint64),
+                ("desp", C.c_void_p * 4), ("count", C.c_void_p * 4),
+                ("ratio", C.c_uint32), ("nsamp", C.c_uint64),
+                ("S", C.c_void_p), ("R", C.c_void_p)]
+
+
+class _FmoIndex(C.Structure):
+    _fields_ = [("bwd", _FmoDir), ("fwd", _FmoDir)]
+
+
+def _load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        raise RuntimeError(f"{ORACLE_SO} is not built (make -C oracle)")
+    lib = C.CDLL(ORACLE_SO)
+    lib.fmo_sa_naive.argtypes = [_p, C.c_uint64, _p]
+    lib.fmo_bwt_from_sa.argtypes = [_p, C.c_uint64, _p, _p, C.POINTER(C.c_uint64)]
+    lib.fmo_index_build_naive.argtypes = [_p, _p, C.c_uint64, C.c_uint32]
+    lib.fmo_index_from_bwts.argtypes = [_p, _p, C.c_uint64, _p, C.c_uint64, C.c_uint64, C.c_uint32]
+    lib.fmo_index_free.argtypes = [_p]
+    lib.fmo_dir_words.argtypes = [_p, C.c_int]
+    lib.fmo_dir_words.restype = C.c_uint64
+    lib.fmo_dir_export.argtypes = [_p, C.c_int, _p, _p, _p, _p, _p, _p,
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.fmo_size_sa.argtypes = [_p]
+    lib.fmo_size_sa.restype = C.c_uint64
+    lib.fmo_occ.argtypes = [_p, C.c_int, C.c_uint, C.c_uint64]
+    lib.fmo_occ.restype = C.c_uint64
+    lib.fmo_lf.argtypes = [_p, C.c_int, C.c_int64, C.c_int64, C.c_uint,
+                           C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.fmo_get_sa.argtypes = [_p, C.c_int, C.c_uint64]
+    lib.fmo_get_sa.restype = C.c_uint64
+    lib.fmo_get_isa.argtypes = [_p, C.c_int, C.c_uint64]
+    lib.fmo_get_isa.restype = C.c_uint64
+    lib.fmo_exact.argtypes = [_p, C.c_int, C.c_int, _p, _p, _p]
+    lib.fmo_exact_vector.argtypes = [_p, C.c_int, C.c_int, _p, C.c_int16, C.c_int16,
+                                     C.c_int64, C.c_int64, _p, _p]
+    lib.fmo_exact_vector.restype = C.c_int16
+    lib.fmo_search1.argtypes = [_p, C.c_int, C.c_int, _p, C.c_int16, _p, C.c_uint32, _p]
+    lib.fmo_search1.restype = C.c_uint32
+    lib.fmo_binsearch.argtypes = [_p, C.c_uint64, C.c_uint64]
+    lib.fmo_binsearch.restype = C.c_uint64
+    lib.fmo_search1_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_int, C.c_uint32, _p, _p, C.c_int]
+    lib.fmo_search1_batch.restype = C.c_double
+    lib.fmo_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
+    lib.fmo_exact_batch.restype = C.c_double
+    return lib
+
+
+_ORACLE_LIB = None
+
+
+def oracle_lib():
+    global _ORACLE_LIB
+    if _ORACLE_LIB is None:
+        _ORACLE_LIB = _load_oracle()
+    return _ORACLE_LIB
+
+
+def sa_naive(text):
+    text = np.ascontiguousarray(text, dtype=np.uint8)
+    sa = np.empty(len(text) + 1, dtype=np.uint64)
+    if oracle_lib().fmo_sa_naive(_ptr(text), len(text), _ptr(sa)) != 0:
+        raise MemoryError("fmo_sa_naive")
+    return sa
+
+
+def bwt_from_sa(text, sa):
+    text = np.ascontiguousarray(text, dtype=np.uint8)
+    out = np.empty(max(len(text), 1), dtype=np.uint8)
+    d = C.c_uint64(0)
+    oracle_lib().fmo_bwt_from_sa(_ptr(text), len(text), _ptr(sa), _ptr(out), C.byref(d))
+    return out[:len(text)], int(d.value)
+
+
+class Oracle:
+    """The C restatement of the reference index + search (per-symbol layout)."""
+
+    def __init__(self, text=None, ratio=16, bwts=None):
+        self.lib = oracle_lib()
+        self._ix = _FmoIndex()
+        if bwts is not None:
+            b, d, bi, di = bwts
+            b = np.ascontiguousarray(b, dtype=np.uint8)
+            bi = np.ascontiguousarray(bi, dtype=np.uint8)
+            n = len(b)
+            rc = self.lib.fmo_index_from_bwts(C.byref(self._ix), _ptr(b), d, _ptr(bi), di, n, ratio)
+        else:
+            text = np.ascontiguousarray(text, dtype=np.uint8)
+            n = len(text)
+            rc = self.lib.fmo_index_build_naive(C.byref(self._ix), _ptr(text), n, ratio)
+        if rc != 0:
+            raise MemoryError("oracle index build failed")
+        self.n = n
+        self.ratio = ratio
+        self.ref = C.byref(self._ix)
+
+    def close(self):
+        if self._ix is not None:
+            self.lib.fmo_index_free(self.ref)
+            self._ix = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size_sa(self):
+        return int(self.lib.fmo_size_sa(self.ref))
+
+    def arrays(self, fwd):
+        w = int(self.lib.fmo_dir_words(self.ref, fwd))
+        d = self._ix.fwd if fwd else self._ix.bwd
+        ns = int(d.nsamp)
+        Cv = np.empty(4, np.uint32); C1 = np.empty(4, np.uint32)
+        desp = np.empty(4 * w, np.uint32); cnt = np.empty(4 * w, np.uint64)
+        S = np.empty(ns, np.uint32); R = np.empty(ns, np.uint32)
+        dol = C.c_uint64(); nsamp = C.c_uint64()
+        self.lib.fmo_dir_export(self.ref, fwd, _ptr(Cv), _ptr(C1), _ptr(desp), _ptr(cnt),
+                                _ptr(S), _ptr(R), C.byref(dol), C.byref(nsamp))
+        return dict(C=Cv, C1=C1, desp=desp.reshape(4, w), count=cnt.reshape(4, w), S=S, R=R,
+                    dollar=int(dol.value))
+
+    def occ(self, view, b, m):
+        return int(self.lib.fmo_occ(self.ref, view, b, m))
+
+    def lf(self, view, k, l, b):
+        ko = C.c_int64(); lo = C.c_int64()
+        self.lib.fmo_lf(self.ref, view, k, l, b, C.byref(ko), C.byref(lo))
+        return ko.value, lo.value
+
+    def get_sa(self, fwd, m):
+        return int(self.lib.fmo_get_sa(self.ref, fwd, m))
+
+    def get_isa(self, fwd, m):
+        return int(self.lib.fmo_get_isa(self.ref, fwd, m))
+
+    def exact(self, view, direction, W, k, l, start, pos, end, anchor=False):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        r = np.zeros(1, HIT_DTYPE)
+        r["k"], r["l"], r["start"], r["pos"], r["end"] = k, l, start, pos, end
+        a = np.zeros(1, HIT_DTYPE) if anchor else None
+        self.lib.fmo_exact(self.ref, view, direction, _ptr(W), _ptr(r), _ptr(a))
+        return (r[0], a[0]) if anchor else r[0]
+
+    def exact_vector(self, view, direction, W, start, end, k, l):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        m = max(end - start + 1, 1)
+        vk = np.zeros(m, np.int64); vl = np.zeros(m, np.int64)
+        g = self.lib.fmo_exact_vector(self.ref, view, direction, _ptr(W), start, end, k, l,
+                                      _ptr(vk), _ptr(vl))
+        return int(g), vk, vl
+
+    def search1(self, strand, W, cap=1 << 14, mismatch_only=False, anchors=False):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        out = np.zeros(max(cap, 1), HIT_DTYPE)
+        an = np.zeros(2, HIT_DTYPE) if anchors else None
+        n = self.lib.fmo_search1(self.ref, strand, int(mismatch_only), _ptr(W), len(W),
+                                 _ptr(out), cap, _ptr(an))
+        res = out[:min(n, cap)].copy()
+        return (int(n), res, an) if anchors else (int(n), res)
+
+    def search1_batch(self, reads, offsets, cap=64, mismatch_only=False, threads=0,
+                      keep_hits=True):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nr = len(offsets) - 1
+        hits = np.zeros(nr * 2 * cap, HIT_DTYPE) if keep_hits else None
+        counts = np.zeros(nr * 2, np.uint32)
+        t = self.lib.fmo_search1_batch(self.ref, _ptr(reads), _ptr(offsets), nr,
+                                       int(mismatch_only), cap, _ptr(hits), _ptr(counts), threads)
+        return t, counts, hits
+
+    def exact_batch(self, reads, offsets, threads=0):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nr = len(offsets) - 1
+        out = np.zeros(nr * 4, np.int64)
+        t = self.lib.fmo_exact_batch(self.ref, _ptr(reads), _ptr(offsets), nr, _ptr(out), threads)
+        return t, out.reshape(nr, 4)
+
+
+def binsearch(array, key):
+    array = np.ascontiguousarray(array, dtype=np.uint64)
+    return int(oracle_lib().fmo_binsearch(_ptr(array), len(array), key))
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+class RefCore:
+    """The reference's own search core (compiled from its sources)."""
+
+    def __init__(self, text, ratio=16, workdir=None):
+        import tempfile
+        if not ref_available():
+            raise RuntimeError(f"{REF_SO} is not built (make -C oracle ref)")
+        lib = C.CDLL(REF_SO)
+        lib.ref_build.argtypes = [_p, C.c_uint64, C.c_uint32, C.c_char_p]
+        lib.ref_build.restype = C.c_void_p
+        lib.ref_free.argtypes = [_p]
+        lib.ref_size_sa.argtypes = [_p]
+        lib.ref_size_sa.restype = C.c_uint64
+        lib.ref_bwt.argtypes = [_p, C.c_int, _p, C.POINTER(C.c_uint64)]
+        lib.ref_words.argtypes = [_p]
+        lib.ref_words.restype = C.c_uint64
+        lib.ref_nsamp.argtypes = [_p]
+        lib.ref_nsamp.restype = C.c_uint64
+        lib.ref_arrays.argtypes = [_p, C.c_int, _p, _p, _p, _p, _p, _p]
+        lib.ref_exact.argtypes = [_p, C.c_int, C.c_int, _p, _p, _p]
+        lib.ref_exact_vector.argtypes = [_p, C.c_int, C.c_int, _p, C.c_int16, C.c_int16,
+                                         C.c_int64, C.c_int64, _p, _p]
+        lib.ref_exact_vector.restype = C.c_int16
+        lib.ref_search1.argtypes = [_p, C.c_int, _p, C.c_int16, C.c_uint32, _p, _p,
+                                    C.POINTER(C.c_int)]
+        lib.ref_search1.restype = C.c_uint32
+        lib.ref_get_sa.argtypes = [_p, C.c_int, C.c_uint64]
+        lib.ref_get_sa.restype = C.c_uint64
+        lib.ref_get_isa.argtypes = [_p, C.c_int, C.c_uint64]
+        lib.ref_get_isa.restype = C.c_uint64
+        lib.ref_binsearch.argtypes = [_p, C.c_uint64, C.c_uint64]
+        lib.ref_binsearch.restype = C.c_uint64
+        lib.ref_search1_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, _p, C.c_int]
+        lib.ref_search1_batch.restype = C.c_double
+        lib.ref_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
+        lib.ref_exact_batch.restype = C.c_double
+        self.lib = lib
+        text = np.ascontiguousarray(text, dtype=np.uint8)
+        self.n = len(text)
+        self._tmp = tempfile.TemporaryDirectory() if workdir is None else None
+        wd = workdir or self._tmp.name
+        self.h = lib.ref_build(_ptr(text), self.n, ratio, wd.encode())
+
+    def close(self):
+        if self.h:
+            self.lib.ref_free(self.h)
+            self.h = None
+        if self._tmp is not None:
+            self._tmp.cleanup()
+            self._tmp = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size_sa(self):
+        return int(self.lib.ref_size_sa(self.h))
+
+    def bwt(self, fwd):
+        out = np.empty(max(self.n, 1), np.uint8)
+        d = C.c_uint64()
+        self.lib.ref_bwt(self.h, fwd, _ptr(out), C.byref(d))
+        return out[:self.n], int(d.value)
+
+    def arrays(self, fwd):
+        w = int(self.lib.ref_words(self.h)); ns = int(self.lib.ref_nsamp(self.h))
+        Cv = np.empty(4, np.uint32); C1 = np.empty(4, np.uint32)
+        desp = np.empty(4 * w, np.uint32); cnt = np.empty(4 * w, np.uint64)
+        S = np.empty(ns, np.uint32); R = np.empty(ns, np.uint32)
+        self.lib.ref_arrays(self.h, fwd, _ptr(Cv), _ptr(C1), _ptr(desp), _ptr(cnt), _ptr(S), _ptr(R))
+        return dict(C=Cv, C1=C1, desp=desp.reshape(4, w), count=cnt.reshape(4, w), S=S, R=R)
+
+    def exact(self, view, direction, W, k, l, start, pos, end, anchor=False):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        r = np.zeros(1, HIT_DTYPE)
+        r["k"], r["l"], r["start"], r["pos"], r["end"] = k, l, start, pos, end
+        a = np.zeros(1, HIT_DTYPE) if anchor else None
+        self.lib.ref_exact(self.h, view, direction, _ptr(W), _ptr(r), _ptr(a))
+        return (r[0], a[0]) if anchor else r[0]
+
+    def exact_vector(self, view, direction, W, start, end, k, l):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        m = max(end - start + 1, 1)
+        vk = np.zeros(m, np.int64); vl = np.zeros(m, np.int64)
+        g = self.lib.ref_exact_vector(self.h, view, direction, _ptr(W), start, end, k, l,
+                                      _ptr(vk), _ptr(vl))
+        return int(g), vk, vl
+
+    def search1(self, strand, W, cap=1 << 14, anchors=False):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        out = np.zeros(max(cap, 1), HIT_DTYPE)
+        an = np.zeros(2, HIT_DTYPE) if anchors else None
+        full = C.c_int(0)
+        n = self.lib.ref_search1(self.h, strand, _ptr(W), len(W), cap, _ptr(out), _ptr(an),
+                                 C.byref(full))
+        res = out[:n].copy()
+        return (int(n), res, bool(full.value), an) if anchors else (int(n), res, bool(full.value))
+
+    def get_sa(self, fwd, m):
+        return int(self.lib.ref_get_sa(self.h, fwd, m))
+
+    def get_isa(self, fwd, m):
+        return int(self.lib.ref_get_isa(self.h, fwd, m))
+
+    def search1_batch(self, reads, offsets, cap=64, threads=0):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nr = len(offsets) - 1
+        counts = np.zeros(nr * 2, np.uint32)
+        t = self.lib.ref_search1_batch(self.h, _ptr(reads), _ptr(offsets), nr, cap,
+                                       _ptr(counts), threads)
+        return t, counts
+
+    def exact_batch(self, reads, offsets, threads=0):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nr = len(offsets) - 1
+        out = np.zeros(nr * 4, np.int64)
+        t = self.lib.ref_exact_batch(self.h, _ptr(reads), _ptr(offsets), nr, _ptr(out), threads)
+        return t, out.reshape(nr, 4)
+
+
+def hits_equal(a, b):
+    """Record-wise equality; error fields compare only where num_mismatches > 0
+    (the reference leaves them unset on exact records)."""
+    if len(a) != len(b):
+        return False
+    keys = ["k", "l", "start", "pos", "end", "dir", "num_mismatches", "strand"]
+    for key in keys:
+        if not np.array_equal(a[key], b[key]):
+            return False
+    m = a["num_mismatches"] > 0
+    for key in ["err_kind", "err_base", "err_pos"]:
+        if not np.array_equal(a[key][m], b[key][m]):
+            return False
+    return True

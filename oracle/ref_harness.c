@@ -1,0 +1,282 @@
+/*
+ * ref_harness.c -- thin driver around the reference's own search core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) from the
+ * reference sources where they lie (c/src/aligners/bwt/{search,dbwt}/ and
+ * bwt_commons.c of opencb/hpg-libs) into oracle/_ref/libfmref_ref.so.  It is
+ * used to pin the oracle restatement, to write the golden fixtures, and as
+ * the reference CPU baseline.  Nothing in the product links it.
+ *
+ * The index is built exactly as bwt_generate_index_files does
+ * (bwt.c:744-819): direct_bwt -> calculate_C -> calculate_O ->
+ * calculate_S_and_R, then the same on the reversed text, and the complement
+ * views of load_bwt_index (runtime.h:97-124).  The harness keeps the arrays
+ * in memory instead of writing and reading the index directory; only the BWT
+ * goes through a file because direct_bwt writes one.
+ */
+#include "search/search.h"
+#include "search/preprocess.h"
+#include "search/io.h"
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <omp.h>
+
+typedef struct {
+  int64_t k, l;
+  int16_t start, pos, end, err_pos;
+  uint8_t dir, num_mismatches, err_kind, err_base;
+  uint8_t strand, pad[3];
+} ref_hit;
+
+typedef struct {
+  bwt_index backward, forward, backward_rev, forward_rev;
+  bwt_config_t cfg;
+  ref_vector B, Bi;
+  uint64_t n;
+} ref_ctx;
+
+static bwt_index *view_of(ref_ctx *c, int view) {
+  switch (view) {
+    case 0: return &c->backward;
+    case 1: return &c->forward;
+    case 2: return &c->backward_rev;
+    default: return &c->forward_rev;
+  }
+}
+
+void *ref_build(const uint8_t *text, uint64_t n, uint32_t ratio, const char *workdir) {
+  ref_ctx *c = (ref_ctx *)calloc(1, sizeof(ref_ctx));
+  c->n = n;
+  bwt_init_replace_table(&c->cfg, "ACGT");
+  c->cfg.duplicate_strand = false;
+
+  ref_vector X;
+  X.vector = (uint8_t *)malloc(n + 1);
+  memcpy(X.vector, text, n);
+  X.vector[n] = 0;
+  X.n = n;
+  X.dollar = 0;
+  calculate_and_save_B(&X, workdir, "B");
+  read_ref_vector(&c->B, workdir, "B");
+
+  calculate_C(&c->backward.C, &c->backward.C1, &c->B, 4);
+  calculate_O(&c->backward.O, &c->B, 4);
+  calculate_S_and_R(&c->backward.S, &c->backward.R, &c->B, &c->backward.C, &c->backward.O, ratio);
+
+  ref_vector Xi;
+  Xi.vector = (uint8_t *)malloc(n + 1);
+  memcpy(Xi.vector, text, n);
+  revstring(Xi.vector, n);
+  Xi.vector[n] = 0;
+  Xi.n = n;
+  Xi.dollar = 0;
+  calculate_and_save_B(&Xi, workdir, "Bi");
+  read_ref_vector(&c->Bi, workdir, "Bi");
+
+  calculate_C(&c->forward.C, &c->forward.C1, &c->Bi, 4);
+  calculate_O(&c->forward.O, &c->Bi, 4);
+  calculate_S_and_R(&c->forward.S, &c->forward.R, &c->Bi, &c->backward.C, &c->forward.O, ratio);
+
+  reverse_strand_C(&c->backward_rev.C, &c->backward.C, &c->backward_rev.C1, &c->backward.C1, c->cfg);
+  reverse_strand_O(&c->backward_rev.O, &c->backward.O, c->cfg);
+  c->backward_rev.S = c->backward.S;
+  c->backward_rev.R = c->backward.R;
+  reverse_strand_C(&c->forward_rev.C, &c->forward.C, &c->forward_rev.C1, &c->forward.C1, c->cfg);
+  reverse_strand_O(&c->forward_rev.O, &c->forward.O, c->cfg);
+  c->forward_rev.S = c->forward.S;
+  c->forward_rev.R = c->forward.R;
+
+  /* direct_bwt releases its input text itself (dbwt.c:526). */
+  return c;
+}
+
+void ref_free(void *h) {
+  ref_ctx *c = (ref_ctx *)h;
+  if (!c) return;
+  free_bwt_index(&c->backward_rev, &c->backward, true);
+  free_bwt_index(&c->forward_rev, &c->forward, true);
+  free(c->B.vector);
+  free(c->Bi.vector);
+  bwt_free_replace_table(&c->cfg);
+  free(c);
+}
+
+uint64_t ref_size_sa(void *h) { return size_SA(&((ref_ctx *)h)->backward); }
+
+/* The BWT as written by direct_bwt (B.vec: n bytes without '$', dollar row). */
+void ref_bwt(void *h, int fwd, uint8_t *out, uint64_t *dollar) {
+  ref_ctx *c = (ref_ctx *)h;
+  ref_vector *B = fwd ? &c->Bi : &c->B;
+  memcpy(out, B->vector, B->n);
+  *dollar = B->dollar;
+}
+
+uint64_t ref_words(void *h) { return ((ref_ctx *)h)->backward.O.m_desp; }
+uint64_t ref_nsamp(void *h) { return ((ref_ctx *)h)->backward.S.n; }
+
+void ref_arrays(void *h, int fwd, uint32_t *C, uint32_t *C1, uint32_t *desp,
+                uint64_t *count, uint32_t *S, uint32_t *R) {
+  ref_ctx *c = (ref_ctx *)h;
+  bwt_index *ix = fwd ? &c->forward : &c->backward;
+  uint64_t m = ix->O.m_desp;
+  for (int b = 0; b < 4; b++) {
+    C[b] = ix->C.vector[b];
+    C1[b] = ix->C1.vector[b];
+    memcpy(desp + (size_t)b * m, ix->O.desp[b], m * sizeof(uint32_t));
+    memcpy(count + (size_t)b * m, ix->O.count[b], m * sizeof(uint64_t));
+  }
+  memcpy(S, ix->S.vector, ix->S.n * sizeof(uint32_t));
+  memcpy(R, ix->R.vector, ix->R.n * sizeof(uint32_t));
+}
+
+static void to_hit(const result *r, ref_hit *h, uint8_t strand) {
+  memset(h, 0, sizeof(*h));
+  h->k = r->k; h->l = r->l;
+  h->start = r->start; h->pos = r->pos; h->end = r->end;
+  h->dir = r->dir ? 1 : 0;
+  h->num_mismatches = r->num_mismatches;
+  if (r->num_mismatches) {
+    h->err_kind = r->err_kind[0];
+    h->err_base = r->err_base[0];
+    h->err_pos = r->err_pos[0];
+  }
+  h->strand = strand;
+}
+
+static void from_hit(const ref_hit *h, result *r) {
+  init_result(r, h->dir);
+  bound_result(r, h->start, h->end);
+  change_result(r, h->k, h->l, h->pos);
+}
+
+/* BWExactSearchBackward (dir 0) / BWExactSearchForward (dir 1). */
+void ref_exact(void *h, int view, int dir, const uint8_t *W, ref_hit *io, ref_hit *anchor) {
+  ref_ctx *c = (ref_ctx *)h;
+  result r, a;
+  from_hit(io, &r);
+  if (dir == 0) BWExactSearchBackward((uint8_t *)W, view_of(c, view), &r, anchor ? &a : NULL);
+  else          BWExactSearchForward((uint8_t *)W, view_of(c, view), &r, anchor ? &a : NULL);
+  to_hit(&r, io, 0);
+  if (anchor) to_hit(&a, anchor, 0);
+}
+
+int16_t ref_exact_vector(void *h, int view, int dir, const uint8_t *W, int16_t start,
+                         int16_t end, int64_t k, int64_t l, int64_t *vk, int64_t *vl) {
+  ref_ctx *c = (ref_ctx *)h;
+  intmax_t *a = (intmax_t *)vk, *b = (intmax_t *)vl;
+  if (dir == 0) return BWExactSearchVectorBackward((uint8_t *)W, start, end, k, l, a, b, view_of(c, view));
+  return BWExactSearchVectorForward((uint8_t *)W, start, end, k, l, a, b, view_of(c, view));
+}
+
+/* BWSearch1CPU with the strand pairing of bwt.c:1690-1701.  Returns the
+ * number of records the reference kept (<= cap); *full is set when the
+ * reference's add_result reported a full list. */
+uint32_t ref_search1(void *h, int strand, const uint8_t *W, int16_t len, uint32_t cap,
+                     ref_hit *out, ref_hit *anchors, int *full) {
+  ref_ctx *c = (ref_ctx *)h;
+  results_list rl, ra;
+  new_results_list(&rl, cap ? cap : 1);
+  rl.max_results = cap;
+  new_results_list(&ra, 2);
+  result res;
+  bound_result(&res, 0, len - 1);
+  change_result(&res, 0, size_SA(&c->backward) - 1, 0);
+  bool f;
+  if (strand == 0)
+    f = BWSearch1CPU((uint8_t *)W, &c->backward, &c->forward, &res, &rl, anchors ? &ra : NULL, 4);
+  else
+    f = BWSearch1CPU((uint8_t *)W, &c->forward_rev, &c->backward_rev, &res, &rl, anchors ? &ra : NULL, 4);
+  for (uintmax_t i = 0; i < rl.num_results; i++) to_hit(&rl.list[i], &out[i], (uint8_t)strand);
+  if (anchors)
+    for (uintmax_t i = 0; i < ra.num_results && i < 2; i++) to_hit(&ra.list[i], &anchors[i], (uint8_t)strand);
+  if (full) *full = f ? 1 : 0;
+  uint32_t nres = (uint32_t)rl.num_results;
+  free(rl.list);
+  free(ra.list);
+  return nres;
+}
+
+uint64_t ref_get_sa(void *h, int fwd, uint64_t m) {
+  ref_ctx *c = (ref_ctx *)h;
+  return get_SA((SA_TYPE)m, fwd ? &c->forward : &c->backward);
+}
+
+uint64_t ref_get_isa(void *h, int fwd, uint64_t m) {
+  ref_ctx *c = (ref_ctx *)h;
+  return get_ISA((SA_TYPE)m, fwd ? &c->forward : &c->backward);
+}
+
+uint64_t ref_binsearch(uint64_t *array, uint64_t size, uint64_t key) {
+  return binsearch((uintmax_t *)array, size, key);
+}
+
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+/* CPU baseline: BWSearch1CPU on both strands of every read, parallel over
+ * reads as bwt_map_exact_batch does (bwt.c:1112,1134). */
+double ref_search1_batch(void *h, const uint8_t *reads, const uint64_t *offsets,
+                         uint64_t nreads, uint32_t cap, uint32_t *counts, int threads) {
+  ref_ctx *c = (ref_ctx *)h;
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+  double t0 = now_s();
+#pragma omp parallel
+  {
+    results_list rl;
+    new_results_list(&rl, cap ? cap : 1);
+    rl.max_results = cap;
+#pragma omp for schedule(dynamic, chunk)
+    for (long i = 0; i < (long)nreads; i++) {
+      uint8_t *W = (uint8_t *)reads + offsets[i];
+      int16_t len = (int16_t)(offsets[i + 1] - offsets[i]);
+      for (int s = 0; s < 2; s++) {
+        result res;
+        rl.num_results = 0;
+        bound_result(&res, 0, len - 1);
+        change_result(&res, 0, size_SA(&c->backward) - 1, 0);
+        if (s == 0) BWSearch1CPU(W, &c->backward, &c->forward, &res, &rl, NULL, 4);
+        else        BWSearch1CPU(W, &c->forward_rev, &c->backward_rev, &res, &rl, NULL, 4);
+        counts[2 * i + s] = (uint32_t)rl.num_results;
+      }
+    }
+    free(rl.list);
+  }
+  return now_s() - t0;
+}
+
+/* CPU baseline for exact search: + strand backward on `backward`, - strand
+ * forward on `backward_rev` (bwt.c:1222-1238). */
+double ref_exact_batch(void *h, const uint8_t *reads, const uint64_t *offsets,
+                       uint64_t nreads, int64_t *out_kl, int threads) {
+  ref_ctx *c = (ref_ctx *)h;
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+  double t0 = now_s();
+#pragma omp parallel for schedule(dynamic, chunk)
+  for (long i = 0; i < (long)nreads; i++) {
+    uint8_t *W = (uint8_t *)reads + offsets[i];
+    int16_t len = (int16_t)(offsets[i + 1] - offsets[i]);
+    result r;
+    bound_result(&r, 0, len - 1);
+    change_result(&r, 0, size_SA(&c->backward) - 1, len - 1);
+    BWExactSearchBackward(W, &c->backward, &r, NULL);
+    out_kl[4 * i + 0] = r.k; out_kl[4 * i + 1] = r.l;
+    bound_result(&r, 0, len - 1);
+    change_result(&r, 0, size_SA(&c->backward) - 1, 0);
+    BWExactSearchForward(W, &c->backward_rev, &r, NULL);
+    out_kl[4 * i + 2] = r.k; out_kl[4 * i + 3] = r.l;
+  }
+  return now_s() - t0;
+}

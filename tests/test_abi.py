@@ -1,0 +1,69 @@
+"""The C-ABI library loads and exports every function include/fmi.h
+declares, and rejects bad calls with status codes before touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hpgfm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "fmi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fmi_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(hpgfm.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(hpgfm.LIB_PATH)
+    missing = [f for f in declared_functions() if not hasattr(lib, f)]
+    assert not missing
+
+
+def test_status_strings():
+    L = hpgfm.lib()
+    for code in range(0, -8, -1):
+        assert L.fmi_status_str(code) not in (None, b"unknown status")
+    assert L.fmi_status_str(-99) == b"unknown status"
+
+
+def test_calls_before_to_device_are_rejected():
+    g = np.random.default_rng(0).integers(0, 4, 1000).astype(np.uint8)
+    ix = hpgfm.Index.build(g, 8)
+    L = hpgfm.lib()
+    rc = L.fmi_exact_batch(ix.h, 0, 0, None, None, 1, None, None, None)
+    assert rc == -7                                        # FMI_ESTATE
+    rc = L.fmi_search1_batch(ix.h, 1, 3, None, None, 1, 4, None, None, None)
+    assert rc == -7
+    rc = L.fmi_locate_batch(ix.h, 0, None, None, 1, 1, None, None, None)
+    assert rc == -7
+    assert b"fmi_index_to_device" in L.fmi_last_error()
+    rc = L.fmi_exact_batch(None, 0, 0, None, None, 1, None, None, None)
+    assert rc == -1
+
+
+def test_tuning_range():
+    with pytest.raises(hpgfm.FmiError):
+        hpgfm.set_tuning(65, 32)
+    with pytest.raises(hpgfm.FmiError):
+        hpgfm.set_tuning(8, 5000)
+    hpgfm.set_tuning(0, 0)
+
+
+def test_read_validation():
+    reads = np.zeros(10, np.uint8)
+    with pytest.raises(ValueError):
+        hpgfm.validate_reads(reads, np.array([0, 5, 3], np.uint64))
+    with pytest.raises(ValueError):
+        hpgfm.validate_reads(reads, np.array([0, 5, 20], np.uint64))
+    with pytest.raises(ValueError):
+        hpgfm.validate_reads(np.zeros(2000, np.uint8), np.array([0, 1500], np.uint64))
+    hpgfm.validate_reads(reads, np.array([0, 5, 10], np.uint64))

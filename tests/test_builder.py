@@ -1,0 +1,129 @@
+"""The product index builder (hpg-libs_amd/csrc/fmi_build.cpp): its BWTs
+against brute force and against the reference builder (golden SHA of
+dbwt's B.vec and of the reference's C/O/S/R arrays), the file format, and
+its argument checks.  Host-only: runs without a GPU."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import fmoracle as fo
+import hpgfm
+from hpgfm import synth
+
+
+def naive_bwts(text):
+    sa = fo.sa_naive(text)
+    b, d = fo.bwt_from_sa(text, sa)
+    rev = np.ascontiguousarray(text[::-1])
+    sa2 = fo.sa_naive(rev)
+    bi, di = fo.bwt_from_sa(rev, sa2)
+    return b, d, bi, di
+
+
+TEXTS = {
+    "one": np.array([2], np.uint8),
+    "two": np.array([3, 0], np.uint8),
+    "allA": np.zeros(97, np.uint8),
+    "period": np.tile(np.array([0, 1, 2], np.uint8), 50),
+    "desc": np.array([3] * 10 + [2] * 10 + [1] * 10 + [0] * 10, np.uint8),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TEXTS))
+def test_bwt_edge_texts(name):
+    t = TEXTS[name]
+    ix = hpgfm.Index.build(t, 4)
+    b, d, bi, di = naive_bwts(t)
+    assert ix.bwt(0)[1] == d and np.array_equal(ix.bwt(0)[0], b)
+    assert ix.bwt(1)[1] == di and np.array_equal(ix.bwt(1)[0], bi)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_bwt_random_and_repeats(seed):
+    n = [1000, 5000, 20000, 60000][seed]
+    g, _, _ = synth.genome(n, seed=seed, repeat_frac=0.3, fam_len=(300, 2000),
+                           divergence=(0.0, 0.02))
+    ix = hpgfm.Index.build(g, 16)
+    b, d, bi, di = naive_bwts(g)
+    assert ix.bwt(0)[1] == d and np.array_equal(ix.bwt(0)[0], b)
+    assert ix.bwt(1)[1] == di and np.array_equal(ix.bwt(1)[0], bi)
+
+
+@pytest.mark.parametrize("name", ["golden_small", "golden_repeats"])
+def test_bwt_matches_reference_builder(name, request):
+    """dbwt's B.vec (golden SHA) and, from the same BWT, the reference's
+    C/C1/desp/count/S/R arrays (golden SHA, via the oracle's restatement of
+    calculate_C/O/S_and_R)."""
+    g = request.getfixturevalue(name)
+    ix = hpgfm.Index.build(g["genome"], int(g["ratio"]))
+    b, d = ix.bwt(0)
+    bi, di = ix.bwt(1)
+    assert hashlib.sha256(b.tobytes()).hexdigest() == str(g["bwt_sha"])
+    assert d == int(g["bwt_dollar"])
+    assert hashlib.sha256(bi.tobytes()).hexdigest() == str(g["bwti_sha"])
+    assert di == int(g["bwti_dollar"])
+    o = fo.Oracle(bwts=(b, d, bi, di), ratio=int(g["ratio"]))
+    h = hashlib.sha256()
+    for fwd in (0, 1):
+        a = o.arrays(fwd)
+        for key in ("C", "C1", "desp", "count", "S", "R"):
+            h.update(np.ascontiguousarray(a[key]).tobytes())
+    assert h.hexdigest() == str(g["arrays_sha"])
+
+
+def test_save_load_roundtrip(tmp_path):
+    g, _, _ = synth.genome(50000, seed=3)
+    ix = hpgfm.Index.build(g, 8)
+    ix.set_karyotype([0, 0], [19999, 29999])
+    p = str(tmp_path / "x.fmi")
+    ix.save(p)
+    ix2 = hpgfm.Index.load(p)
+    assert ix2.size_sa == ix.size_sa and ix2.sa_ratio == 8
+    for f in (0, 1):
+        assert np.array_equal(ix.bwt(f)[0], ix2.bwt(f)[0])
+        assert ix.bwt(f)[1] == ix2.bwt(f)[1]
+    assert ix2.device_bytes == ix.device_bytes
+
+
+def test_load_rejects_corrupt_and_truncated(tmp_path):
+    g, _, _ = synth.genome(5000, seed=4)
+    ix = hpgfm.Index.build(g, 8)
+    p = str(tmp_path / "x.fmi")
+    ix.save(p)
+    data = bytearray(open(p, "rb").read())
+    bad = bytearray(data)
+    bad[200] ^= 0x40
+    open(p + ".bad", "wb").write(bad)
+    with pytest.raises(hpgfm.FmiError) as e:
+        hpgfm.Index.load(p + ".bad")
+    assert e.value.code == -4
+    open(p + ".short", "wb").write(data[:len(data) // 2])
+    with pytest.raises(hpgfm.FmiError) as e:
+        hpgfm.Index.load(p + ".short")
+    assert e.value.code == -4
+    open(p + ".magic", "wb").write(b"NOTANIDX" + data[8:])
+    with pytest.raises(hpgfm.FmiError) as e:
+        hpgfm.Index.load(p + ".magic")
+    assert e.value.code == -4
+    with pytest.raises(hpgfm.FmiError) as e:
+        hpgfm.Index.load(os.path.join(str(tmp_path), "missing.fmi"))
+    assert e.value.code == -3
+
+
+def test_build_argument_errors():
+    with pytest.raises(hpgfm.FmiError) as e:
+        hpgfm.Index.build(np.array([0, 1, 4, 2], np.uint8), 8)      # N must be replaced
+    assert e.value.code == -1
+    with pytest.raises(hpgfm.FmiError) as e:
+        hpgfm.Index.build(np.array([0, 1, 2], np.uint8), 0)
+    assert e.value.code == -1
+
+
+def test_karyotype_checks():
+    g, _, _ = synth.genome(1000, seed=5)
+    ix = hpgfm.Index.build(g, 8)
+    with pytest.raises(hpgfm.FmiError):
+        ix.set_karyotype([0], [500])            # does not add up to 1000
+    ix.set_karyotype([10, 0], [409, 599])       # 400 + 600
