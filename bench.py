@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X FM-index 1-mismatch read search.
+
+Workload (the reference's 1-mismatch GPU configuration): a 64 Mbp synthetic
+reference (41% GC, 20% made of mutated copies of 300 bp - 6 kbp repeat
+families at 2-15% divergence), 1M reads of 150 bp from both strands with a
+1% uniform substitution rate, searched with at most one mismatch on both
+strands (BWSearch1CPU's {exact, MISMATCH} records, bit-exact).  A step is one
+fmi_search1_batch launch over the whole batch, with reads, offsets, index and
+outputs resident in HBM.
+
+stdout: one JSON line.  stderr: progress.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hpg-libs_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "reads/sec + HBM GB/s, 150bp ≤1-mismatch, bit-exact; p50/p99 served latency"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--genome-mbp", type=float, default=64.0)
+    ap.add_argument("--reads", type=int, default=1_000_000)
+    ap.add_argument("--read-len", type=int, default=150)
+    ap.add_argument("--sub-rate", type=float, default=0.01)
+    ap.add_argument("--sa-ratio", type=int, default=16)
+    ap.add_argument("--cap", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reads", type=int, default=0,
+                    help="CPU baseline sample size (0: all reads, capped by time)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the gather roofline probe and the plain / edit comparisons")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(reads are independent: a multi-GPU run replicates the index and splits the "
+              "batch, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import hpgfm
+    from hpgfm import synth
+
+    if hpgfm.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    n_bp = int(args.genome_mbp * 1_000_000)
+    t0 = time.time()
+    g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=0.2, fam_len=(300, 6000),
+                           divergence=(0.02, 0.15))
+    log(f"genome {n_bp} bp in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ix = hpgfm.Index.build(g, args.sa_ratio)
+    log(f"index built in {time.time() - t0:.1f}s ({ix.device_bytes / 1e6:.1f} MB on device)")
+    ix.to_device()
+    t0 = time.time()
+    reads, offs, _ = synth.reads(g, args.reads, length=args.read_len, seed=2,
+                                 sub_rate=args.sub_rate, both_strands=True)
+    batch = hpgfm.ReadBatch(reads, offs)
+    log(f"{args.reads} reads generated and resident in {time.time() - t0:.1f}s")
+
+    mode = hpgfm.MODE_MISMATCH
+    out = hpgfm.Search1Output(batch.nreads, args.cap)
+
+    # algorithmic bytes of one launch: distinct 64-byte blocks per LF query
+    blocks, lf_steps = hpgfm.search1_count_blocks(ix, batch, mode=mode)
+
+    for _ in range(args.warmup):
+        hpgfm.search1_launch(ix, batch, out, mode=mode)
+    hpgfm.sync()
+
+    with hpgfm.TimingRegion() as tr:
+        hpgfm.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            hpgfm.search1_launch(ix, batch, out, mode=mode)
+        hpgfm.sync()
+        wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    reads_per_s = args.reads / (wall / args.steps)
+    counts = out.counts().reshape(-1, 2)
+    stored = int(np.minimum(counts, args.cap).sum())
+    alg_bytes = (blocks * 64 + int(offs[-1]) + 8 * (args.reads + 1) + stored * 32 +
+                 4 * 2 * args.reads)
+    kernel_ms = tr.mean_ms
+    achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
+    log(f"mismatch mode: {ms_per_step:.3f} ms/step wall, kernel {kernel_ms:.3f} ms "
+        f"(min {tr.min_ms:.3f}, max {tr.max_ms:.3f}), {reads_per_s / 1e6:.2f} M reads/s, "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    extras = {}
+    if not args.no_extras:
+        ms, done = hpgfm.gather_bench(8 << 30, 64, 1 << 28)
+        extras["random_gather_64B_GBps"] = round(done * 64 / (ms / 1000.0) / 1e9, 1)
+        log(f"random 64-byte gather over 8 GiB: {extras['random_gather_64B_GBps']} GB/s")
+        hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
+        hpgfm.sync()
+        with hpgfm.TimingRegion() as tp:
+            for _ in range(3):
+                hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
+            hpgfm.sync()
+        extras["plain_one_launch_reads_per_s"] = round(args.reads / (tp.mean_ms / 1000.0))
+        extras["persistent_vs_plain"] = round(tp.mean_ms / kernel_ms, 3)
+        log(f"plain one-thread-per-task launch: {tp.mean_ms:.3f} ms")
+        eout = hpgfm.Search1Output(batch.nreads, args.cap)
+        hpgfm.search1_launch(ix, batch, eout, mode=hpgfm.MODE_EDIT)
+        hpgfm.sync()
+        with hpgfm.TimingRegion() as te:
+            for _ in range(3):
+                hpgfm.search1_launch(ix, batch, eout, mode=hpgfm.MODE_EDIT)
+            hpgfm.sync()
+        extras["edit_mode_reads_per_s"] = round(args.reads / (te.mean_ms / 1000.0))
+        edit_counts = eout.counts().reshape(-1, 2)
+        eout.free()
+        log(f"edit mode (mismatch + indel records): {te.mean_ms:.3f} ms")
+    else:
+        edit_counts = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fmoracle as fo
+        threads = cpu_threads()
+        t0 = time.time()
+        o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
+        kind = "reference" if fo.ref_available() else "port"
+        core = fo.RefCore(oracle=o) if kind == "reference" else o
+        log(f"CPU index ({kind}) ready in {time.time() - t0:.1f}s")
+        nr = args.cpu_reads or args.reads
+        # probe, then size the sample to about 15 s
+        probe = min(nr, 20000)
+        if kind == "reference":
+            tprobe, _ = core.search1_batch(reads, offs[:probe + 1], cap=args.cap, threads=threads)
+        else:
+            tprobe, _, _ = core.search1_batch(reads, offs[:probe + 1], cap=args.cap, threads=threads,
+                                              keep_hits=False)
+        nr = int(min(nr, max(probe, probe * 15.0 / max(tprobe, 1e-6))))
+        if kind == "reference":
+            tcpu, ccpu = core.search1_batch(reads, offs[:nr + 1], cap=args.cap, threads=threads)
+        else:
+            tcpu, ccpu, _ = core.search1_batch(reads, offs[:nr + 1], cap=args.cap, threads=threads,
+                                               keep_hits=False)
+        cpu = {"value": round(nr / tcpu), "unit": "reads/s", "cores": threads, "kind": kind,
+               "sample": f"first {nr} reads of the workload, both strands; BWSearch1CPU "
+                         f"(search/search.c:1876) per read and strand as bwt.c:1690-1701, "
+                         f"OpenMP over reads as bwt.c:1134, -O2 -fopenmp -DFM_COMP_64; the "
+                         f"reference has no mismatch-only mode, so this is its full 1-error "
+                         f"search (the GPU edit-mode figure is the like-for-like one)"}
+        log(f"CPU baseline: {nr} reads in {tcpu:.2f}s on {threads} threads = {nr / tcpu:.0f} reads/s")
+        if edit_counts is not None:
+            same = np.array_equal(np.minimum(edit_counts[:nr].reshape(-1), args.cap),
+                                  np.minimum(ccpu, args.cap))
+            extras["parity_counts_vs_cpu_sample"] = bool(same)
+            log(f"GPU edit-mode record counts == reference counts on the sample: {same}")
+
+    res = {
+        "metric": METRIC,
+        "value": round(reads_per_s),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"1-mismatch search (BWSearch1CPU mismatch projection), "
+                         f"{args.genome_mbp:g} Mbp synthetic reference (41% GC, 20% repeat "
+                         f"families 300 bp-6 kbp at 2-15% divergence), {args.reads} reads x "
+                         f"{args.read_len} bp, both strands, {args.sub_rate * 100:g}% "
+                         f"substitutions"),
+            "genome_bp": n_bp, "reads": args.reads, "read_len": args.read_len,
+            "mode": "mismatch", "cap": args.cap, "sa_ratio": args.sa_ratio,
+            "parallelism": "single GPU, persistent work-queue launch",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "kernel": "fmi::search1_kernel<true, true, false>",
+        "kernel_ms_mean": round(kernel_ms, 4),
+        "alg_bytes_per_launch": alg_bytes,
+        "lf_queries_per_launch": lf_steps,
+        "blocks_per_lf_query": round(blocks / max(lf_steps, 1), 4),
+        "lf_queries_per_read": round(lf_steps / args.reads, 2),
+        "index_device_bytes": ix.device_bytes,
+        "index_cache_resident": ix.device_bytes < 256 * 2 ** 20,
+    }
+    res.update(extras)
+    out.free()
+    batch.free()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

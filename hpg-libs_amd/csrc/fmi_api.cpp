@@ -9,6 +9,8 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <utility>
+#include <vector>
 
 #include "../../include/fmi.h"
 #include "fmi_index.h"
@@ -81,10 +83,32 @@ int cu_count(int *cus) {
   return FMI_OK;
 }
 
+// Timing region (fmi_timing_begin/end): one event pair per launch, taken
+// from a per-thread pool that grows as needed and is reused across regions.
+thread_local bool g_region = false;
+thread_local std::vector<std::pair<hipEvent_t, hipEvent_t>> g_pool;
+thread_local size_t g_used = 0;
+
+thread_local hipEvent_t g_single0 = nullptr, g_single1 = nullptr;
+
 int begin_timed(hipStream_t s) {
-  if (!g_ev0) {
-    HIP_TRY(hipEventCreate(&g_ev0), "hipEventCreate");
-    HIP_TRY(hipEventCreate(&g_ev1), "hipEventCreate");
+  if (g_region) {
+    if (g_used == g_pool.size()) {
+      hipEvent_t a, b;
+      HIP_TRY(hipEventCreate(&a), "hipEventCreate");
+      HIP_TRY(hipEventCreate(&b), "hipEventCreate");
+      g_pool.emplace_back(a, b);
+    }
+    g_ev0 = g_pool[g_used].first;
+    g_ev1 = g_pool[g_used].second;
+    g_used++;
+  } else {
+    if (!g_single0) {
+      HIP_TRY(hipEventCreate(&g_single0), "hipEventCreate");
+      HIP_TRY(hipEventCreate(&g_single1), "hipEventCreate");
+    }
+    g_ev0 = g_single0;
+    g_ev1 = g_single1;
   }
   HIP_TRY(hipEventRecord(g_ev0, s), "hipEventRecord");
   return FMI_OK;
@@ -371,6 +395,34 @@ int fmi_last_kernel_ms(float *ms) {
   if (!g_timed) return fail(FMI_ESTATE, "no kernel launched on this thread");
   HIP_TRY(hipEventSynchronize(g_ev1), "hipEventSynchronize");
   HIP_TRY(hipEventElapsedTime(ms, g_ev0, g_ev1), "hipEventElapsedTime");
+  return FMI_OK;
+}
+
+int fmi_timing_begin(void) {
+  if (g_region) return fail(FMI_ESTATE, "timing region already open");
+  g_region = true;
+  g_used = 0;
+  return FMI_OK;
+}
+
+int fmi_timing_end(uint32_t *launches, double *sum_ms, float *min_ms, float *max_ms) {
+  if (!g_region) return fail(FMI_ESTATE, "no timing region open");
+  if (!launches || !sum_ms || !min_ms || !max_ms) return fail(FMI_EINVAL, "NULL argument");
+  g_region = false;
+  double sum = 0;
+  float mn = 0, mx = 0;
+  for (size_t i = 0; i < g_used; ++i) {
+    float ms = 0;
+    HIP_TRY(hipEventSynchronize(g_pool[i].second), "hipEventSynchronize");
+    HIP_TRY(hipEventElapsedTime(&ms, g_pool[i].first, g_pool[i].second), "hipEventElapsedTime");
+    sum += ms;
+    if (i == 0 || ms < mn) mn = ms;
+    if (i == 0 || ms > mx) mx = ms;
+  }
+  *launches = (uint32_t)g_used;
+  *sum_ms = sum;
+  *min_ms = mn;
+  *max_ms = mx;
   return FMI_OK;
 }
 

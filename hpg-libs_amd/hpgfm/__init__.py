@@ -39,7 +39,7 @@ EXPORTS = [
     "fmi_stream_destroy", "fmi_stream_sync", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
     "fmi_set_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
-    "fmi_search1_count_blocks",
+    "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
 ]
 
 
@@ -80,6 +80,9 @@ def _bind(lib):
         "fmi_stream_destroy": ([_p], _i),
         "fmi_stream_sync": ([_p], _i),
         "fmi_last_kernel_ms": ([C.POINTER(C.c_float)], _i),
+        "fmi_timing_begin": ([], _i),
+        "fmi_timing_end": ([C.POINTER(_u32), C.POINTER(C.c_double), C.POINTER(C.c_float),
+                            C.POINTER(C.c_float)], _i),
         "fmi_exact_batch": ([_p, _i, _i, _p, _p, _u32, _p, _p, _p], _i),
         "fmi_exact_vector_batch": ([_p, _i, _i, _p, _p, _u32, _p, _p, _p, _p], _i),
         "fmi_search1_batch": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
@@ -193,6 +196,29 @@ def last_kernel_ms():
     ms = C.c_float(0)
     check(lib().fmi_last_kernel_ms(C.byref(ms)))
     return float(ms.value)
+
+
+class TimingRegion:
+    """Per-launch HIP-event timing of every kernel launched inside the
+    `with` block; after it: .launches, .sum_ms, .min_ms, .max_ms."""
+
+    def __enter__(self):
+        check(lib().fmi_timing_begin())
+        return self
+
+    def __exit__(self, *exc):
+        n = C.c_uint32()
+        s = C.c_double()
+        mn = C.c_float()
+        mx = C.c_float()
+        check(lib().fmi_timing_end(C.byref(n), C.byref(s), C.byref(mn), C.byref(mx)))
+        self.launches, self.sum_ms = int(n.value), float(s.value)
+        self.min_ms, self.max_ms = float(mn.value), float(mx.value)
+        return False
+
+    @property
+    def mean_ms(self):
+        return self.sum_ms / max(self.launches, 1)
 
 
 def set_tuning(wg_per_cu=0, chunk=0):

@@ -128,6 +128,12 @@ int fmi_stream_sync(fmi_stream s);
 /* Milliseconds the last kernel launched by the calling thread took on the
  * GPU (HIP events around that launch; valid after fmi_stream_sync). */
 int fmi_last_kernel_ms(float *ms);
+/* Timing region: between fmi_timing_begin and fmi_timing_end every search
+ * kernel the calling thread launches is bracketed by its own pair of HIP
+ * events on its stream; fmi_timing_end waits for them and returns the
+ * number of launches and the sum / minimum / maximum of their durations. */
+int fmi_timing_begin(void);
+int fmi_timing_end(uint32_t *launches, double *sum_ms, float *min_ms, float *max_ms);
 
 /* ------------------------------------------------------- exact search */
 /* BWExactSearchBackward (direction 0, search/search.h:9-36) or

@@ -124,7 +124,9 @@ int fmo_dir_from_bwt(fmo_dir *d, const uint8_t *bwt, uint64_t n, uint64_t dollar
     d->count[s][pos] |= (uint64_t)1 << bit;
   }
   /* calculate_S_and_R, preprocess.c:7-42: walk LF from row 0 (suffix n)
-   * down to suffix 1, sampling S by row and R by text position. */
+   * down to suffix 1, sampling S by row and R by text position.  ratio 0
+   * leaves the samples out (search only; get_SA/get_ISA then unusable). */
+  if (ratio == 0) return 0;
   uint64_t n1 = n + 1;
   d->nsamp = n1 / ratio + (n1 % ratio ? 1 : 0);
   d->S = (uint32_t *)calloc(d->nsamp, sizeof(uint32_t));
@@ -152,9 +154,22 @@ int fmo_index_from_bwts(fmo_index *ix, const uint8_t *bwt, uint64_t dollar,
                         const uint8_t *bwt_rev, uint64_t dollar_rev, uint64_t n,
                         uint32_t ratio) {
   memset(ix, 0, sizeof(*ix));
-  if (fmo_dir_from_bwt(&ix->bwd, bwt, n, dollar, ratio)) return -1;
-  if (fmo_dir_from_bwt(&ix->fwd, bwt_rev, n, dollar_rev, ratio)) return -1;
-  return 0;
+  int rc0 = 0, rc1 = 0;
+  /* the two directions are independent (bwt.c:772-809 builds them in turn) */
+#ifdef _OPENMP
+#pragma omp parallel sections num_threads(2)
+#endif
+  {
+#ifdef _OPENMP
+#pragma omp section
+#endif
+    rc0 = fmo_dir_from_bwt(&ix->bwd, bwt, n, dollar, ratio);
+#ifdef _OPENMP
+#pragma omp section
+#endif
+    rc1 = fmo_dir_from_bwt(&ix->fwd, bwt_rev, n, dollar_rev, ratio);
+  }
+  return (rc0 || rc1) ? -1 : 0;
 }
 
 int fmo_index_build_naive(fmo_index *ix, const uint8_t *text, uint64_t n, uint32_t ratio) {
@@ -231,6 +246,7 @@ void fmo_lf(const fmo_index *ix, int view, int64_t k, int64_t l, unsigned b,
 uint64_t fmo_get_sa(const fmo_index *ix, int fwd, uint64_t m) {
   /* getScompValue, csafm.h:208-235 */
   const fmo_dir *d = fwd ? &ix->fwd : &ix->bwd;
+  if (!d->S) return UINT64_MAX;          /* built without samples */
   uint64_t i = m, j = 0;
   while (i % d->ratio) {
     unsigned b = bwt_sym_dir(d, i);
@@ -244,6 +260,7 @@ uint64_t fmo_get_sa(const fmo_index *ix, int fwd, uint64_t m) {
 uint64_t fmo_get_isa(const fmo_index *ix, int fwd, uint64_t m) {
   /* getRcompValue, csafm.h:237-272 */
   const fmo_dir *d = fwd ? &ix->fwd : &ix->bwd;
+  if (!d->R) return UINT64_MAX;          /* built without samples */
   uint64_t siz = d->n + 1;
   uint64_t i = (d->ratio - (m % d->ratio)) % d->ratio;
   uint64_t k = m + i, j;

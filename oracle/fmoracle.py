@@ -230,16 +230,30 @@ def ref_available():
     return os.path.exists(REF_SO)
 
 
-class RefCore:
-    """The reference's own search core (compiled from its sources)."""
+class _RefDirArrays(C.Structure):
+    _fields_ = [("C", C.c_void_p), ("C1", C.c_void_p), ("desp", C.c_void_p * 4),
+                ("count", C.c_void_p * 4), ("words", C.c_uint64), ("S", C.c_void_p),
+                ("R", C.c_void_p), ("nsamp", C.c_uint64)]
 
-    def __init__(self, text, ratio=16, workdir=None):
+
+class RefCore:
+    """The reference's own search core (compiled from its sources).
+
+    RefCore(text) builds the index with the reference builder (dbwt and
+    calculate_C/O/S_and_R).  RefCore(oracle=o) runs the reference search
+    code over the oracle's arrays, which equal the reference builder's
+    (pinned by the golden SHA of C/C1/desp/count/S/R); the oracle must stay
+    alive while this object is used."""
+
+    def __init__(self, text=None, ratio=16, workdir=None, oracle=None):
         import tempfile
         if not ref_available():
             raise RuntimeError(f"{REF_SO} is not built (make -C oracle ref)")
         lib = C.CDLL(REF_SO)
         lib.ref_build.argtypes = [_p, C.c_uint64, C.c_uint32, C.c_char_p]
         lib.ref_build.restype = C.c_void_p
+        lib.ref_wrap.argtypes = [_p, _p, C.c_uint64, C.c_uint32]
+        lib.ref_wrap.restype = C.c_void_p
         lib.ref_free.argtypes = [_p]
         lib.ref_size_sa.argtypes = [_p]
         lib.ref_size_sa.restype = C.c_uint64
@@ -267,6 +281,23 @@ class RefCore:
         lib.ref_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
         lib.ref_exact_batch.restype = C.c_double
         self.lib = lib
+        self._tmp = None
+        if oracle is not None:
+            self._oracle = oracle
+            dirs = []
+            for d in (oracle._ix.bwd, oracle._ix.fwd):
+                a = _RefDirArrays()
+                a.C = C.cast(C.addressof(d.C), C.c_void_p)
+                a.C1 = C.cast(C.addressof(d.C1), C.c_void_p)
+                for b in range(4):
+                    a.desp[b] = d.desp[b]
+                    a.count[b] = d.count[b]
+                a.words, a.S, a.R, a.nsamp = d.words, d.S, d.R, d.nsamp
+                dirs.append(a)
+            self._dirs = dirs
+            self.n = oracle.n
+            self.h = lib.ref_wrap(C.byref(dirs[0]), C.byref(dirs[1]), self.n, oracle.ratio)
+            return
         text = np.ascontiguousarray(text, dtype=np.uint8)
         self.n = len(text)
         self._tmp = tempfile.TemporaryDirectory() if workdir is None else None

@@ -37,6 +37,7 @@ typedef struct {
   bwt_config_t cfg;
   ref_vector B, Bi;
   uint64_t n;
+  int wrapped;   /* arrays owned by the caller (ref_wrap) */
 } ref_ctx;
 
 static bwt_index *view_of(ref_ctx *c, int view) {
@@ -94,13 +95,70 @@ void *ref_build(const uint8_t *text, uint64_t n, uint32_t ratio, const char *wor
   return c;
 }
 
+/* Wraps index arrays the caller owns (the reference layout: C, C1, per-symbol
+ * desp/count, S, R) into the reference's bwt_index structs, without running
+ * the reference builder.  Used for large texts, where dbwt and the serial
+ * S/R walk would take minutes; the arrays must equal what the reference
+ * builder writes.  d = 0 backward, 1 forward. */
+typedef struct {
+  const uint32_t *C, *C1;
+  uint32_t *desp[4];
+  uint64_t *count[4];
+  uint64_t words;
+  uint32_t *S, *R;
+  uint64_t nsamp;
+} ref_dir_arrays;
+
+static void wrap_dir(bwt_index *ix, const ref_dir_arrays *a, uint64_t n, uint32_t ratio) {
+  ix->C.n = 4; ix->C1.n = 4;
+  ix->C.vector = (SA_TYPE *)malloc(4 * sizeof(SA_TYPE));
+  ix->C1.vector = (SA_TYPE *)malloc(4 * sizeof(SA_TYPE));
+  for (int b = 0; b < 4; b++) { ix->C.vector[b] = a->C[b]; ix->C1.vector[b] = a->C1[b]; }
+  ix->O.siz = (SA_TYPE)(n + 2);
+  ix->O.n_desp = 4; ix->O.m_desp = (SA_TYPE)a->words;
+  ix->O.n_count = 4; ix->O.m_count = (SA_TYPE)a->words;
+  ix->O.desp = (SA_TYPE **)malloc(4 * sizeof(SA_TYPE *));
+  ix->O.count = (FM_COMP_TYPE **)malloc(4 * sizeof(FM_COMP_TYPE *));
+  for (int b = 0; b < 4; b++) { ix->O.desp[b] = a->desp[b]; ix->O.count[b] = a->count[b]; }
+  ix->S.vector = a->S; ix->S.siz = (SA_TYPE)(n + 1); ix->S.n = (SA_TYPE)a->nsamp; ix->S.ratio = ratio;
+  ix->R.vector = a->R; ix->R.siz = (SA_TYPE)(n + 1); ix->R.n = (SA_TYPE)a->nsamp; ix->R.ratio = ratio;
+}
+
+void *ref_wrap(const ref_dir_arrays *bwd, const ref_dir_arrays *fwd, uint64_t n, uint32_t ratio) {
+  ref_ctx *c = (ref_ctx *)calloc(1, sizeof(ref_ctx));
+  c->n = n;
+  c->wrapped = 1;
+  bwt_init_replace_table(&c->cfg, "ACGT");
+  wrap_dir(&c->backward, bwd, n, ratio);
+  wrap_dir(&c->forward, fwd, n, ratio);
+  reverse_strand_C(&c->backward_rev.C, &c->backward.C, &c->backward_rev.C1, &c->backward.C1, c->cfg);
+  reverse_strand_O(&c->backward_rev.O, &c->backward.O, c->cfg);
+  c->backward_rev.S = c->backward.S;
+  c->backward_rev.R = c->backward.R;
+  reverse_strand_C(&c->forward_rev.C, &c->forward.C, &c->forward_rev.C1, &c->forward.C1, c->cfg);
+  reverse_strand_O(&c->forward_rev.O, &c->forward.O, c->cfg);
+  c->forward_rev.S = c->forward.S;
+  c->forward_rev.R = c->forward.R;
+  return c;
+}
+
 void ref_free(void *h) {
   ref_ctx *c = (ref_ctx *)h;
   if (!c) return;
-  free_bwt_index(&c->backward_rev, &c->backward, true);
-  free_bwt_index(&c->forward_rev, &c->forward, true);
-  free(c->B.vector);
-  free(c->Bi.vector);
+  if (c->wrapped) {
+    bwt_index *all[4] = {&c->backward, &c->forward, &c->backward_rev, &c->forward_rev};
+    for (int i = 0; i < 4; i++) {
+      free(all[i]->C.vector);
+      free(all[i]->C1.vector);
+      free(all[i]->O.desp);
+      free(all[i]->O.count);
+    }
+  } else {
+    free_bwt_index(&c->backward_rev, &c->backward, true);
+    free_bwt_index(&c->forward_rev, &c->forward, true);
+    free(c->B.vector);
+    free(c->Bi.vector);
+  }
   bwt_free_replace_table(&c->cfg);
   free(c);
 }
