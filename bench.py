@@ -54,6 +54,8 @@ def main():
                     help="CPU baseline sample size (0: all reads, capped by time)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the gather roofline probe and the plain / edit comparisons")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shape")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -89,6 +91,14 @@ def main():
 
     # algorithmic bytes of one launch: distinct 64-byte blocks per LF query
     blocks, lf_steps = hpgfm.search1_count_blocks(ix, batch, mode=mode)
+
+    tuning = None
+    if not args.no_autotune:
+        from hpgfm.autotune import autotune
+        best, best_ms, table = autotune(ix, batch, out, mode=mode, log=log)
+        tuning = {"wg_per_cu": best[0], "chunk": best[1], "refill_min": best[2],
+                  "tuned_ms": round(best_ms, 4), "points": len(table) - 1,
+                  "plain_ms": round(table["plain"], 4)}
 
     for _ in range(args.warmup):
         hpgfm.search1_launch(ix, batch, out, mode=mode)
@@ -214,6 +224,7 @@ def main():
         "lf_queries_per_read": round(lf_steps / args.reads, 2),
         "index_device_bytes": ix.device_bytes,
         "index_cache_resident": ix.device_bytes < 256 * 2 ** 20,
+        "tuning": tuning,
     }
     res.update(extras)
     out.free()

@@ -38,9 +38,15 @@ int hip_fail(hipError_t e, const char *what) {
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
+// Defaults of the persistent launch (the autotuner overrides them).
+constexpr uint32_t kDefaultWgPerCu = 8;
+constexpr uint32_t kDefaultChunk = 64;
+constexpr uint32_t kDefaultRefill = 16;
+
 struct Tuning {
-  uint32_t wg_per_cu = 0;   // 0: occupancy limit of the kernel
-  uint32_t chunk = 32;
+  uint32_t wg_per_cu = 0;   // workgroups of 256 per CU; 0 = default
+  uint32_t chunk = kDefaultChunk;            // tasks a wave takes per atomic
+  uint32_t refill_min = kDefaultRefill;      // idle lanes before a wave refills
 };
 Tuning g_tuning;
 std::mutex g_mu;
@@ -153,27 +159,25 @@ int search1_common(const fmi_index *ix, int mode, int strand_mask, const uint8_t
   Workspace w;
   rc = workspace_for(st, &w);
   if (rc) return rc;
-  uint32_t grid = 0, chunk = 32;
+  uint32_t grid = 0;
+  Tuning t;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    t = g_tuning;
+  }
   if (persistent) {
     int cus = 0;
     rc = cu_count(&cus);
     if (rc) return rc;
-    Tuning t;
-    {
-      std::lock_guard<std::mutex> lk(g_mu);
-      t = g_tuning;
-    }
-    uint32_t per_cu = t.wg_per_cu ? t.wg_per_cu : 8u;
-    grid = (uint32_t)cus * per_cu;
-    chunk = t.chunk ? t.chunk : 32u;
-    if (chunk > 4096) chunk = 4096;
+    grid = (uint32_t)cus * (t.wg_per_cu ? t.wg_per_cu : kDefaultWgPerCu);
     HIP_TRY(fmi::launch_zero(w.queue, 1, st), "zero queue");
   }
   rc = begin_timed(st);
   if (rc) return rc;
   hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, persistent, false,
                                      (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap,
-                                     d_hits, d_counts, w.queue, w.stats, grid, chunk, st);
+                                     d_hits, d_counts, w.queue, w.stats, grid, t.chunk,
+                                     t.refill_min, st);
   return end_timed(st, e, "search1 launch");
 }
 
@@ -476,11 +480,21 @@ int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
                         s, false);
 }
 
-int fmi_set_tuning(uint32_t wg_per_cu, uint32_t chunk) {
-  if (wg_per_cu > 64 || chunk > 4096) return fail(FMI_EINVAL, "tuning out of range");
+int fmi_set_tuning(uint32_t wg_per_cu, uint32_t chunk, uint32_t refill_min) {
+  if (wg_per_cu > 64 || chunk > 4096 || refill_min > 64) return fail(FMI_EINVAL, "tuning out of range");
   std::lock_guard<std::mutex> lk(g_mu);
   g_tuning.wg_per_cu = wg_per_cu;
-  g_tuning.chunk = chunk ? chunk : 32;
+  g_tuning.chunk = chunk ? chunk : kDefaultChunk;
+  g_tuning.refill_min = refill_min ? refill_min : kDefaultRefill;
+  return FMI_OK;
+}
+
+int fmi_get_tuning(uint32_t *wg_per_cu, uint32_t *chunk, uint32_t *refill_min) {
+  if (!wg_per_cu || !chunk || !refill_min) return fail(FMI_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  *wg_per_cu = g_tuning.wg_per_cu ? g_tuning.wg_per_cu : kDefaultWgPerCu;
+  *chunk = g_tuning.chunk;
+  *refill_min = g_tuning.refill_min;
   return FMI_OK;
 }
 
@@ -520,7 +534,7 @@ int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t 
 int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uint64_t nlines,
                      uint64_t *d_sink, float *ms, uint64_t *lines_done, fmi_stream s) {
   if (!d_buf || !d_sink || !ms || !lines_done) return fail(FMI_EINVAL, "NULL argument");
-  if (line_bytes == 0 || line_bytes % 16 || line_bytes > 4096) return fail(FMI_EINVAL, "line_bytes must be a multiple of 16 up to 4096");
+  if (line_bytes != 64 && line_bytes != 128 && line_bytes != 256) return fail(FMI_EINVAL, "line_bytes must be 64, 128 or 256");
   if (bytes < (64ull << 20)) return fail(FMI_EINVAL, "buffer must be >= 64 MiB");
   int cus = 0;
   int rc = cu_count(&cus);
@@ -561,7 +575,8 @@ int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
   HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 4, st), "zero stats");
   hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, true, true,
                                      (uint32_t)strand_mask, d_reads, d_offsets, nreads, 0, nullptr,
-                                     counts, w.queue, w.stats, (uint32_t)cus * 8u, 32u, st);
+                                     counts, w.queue, w.stats, (uint32_t)cus * kDefaultWgPerCu,
+                                     kDefaultChunk, kDefaultRefill, st);
   if (e != hipSuccess) { (void)hipFree(counts); return hip_fail(e, "count launch"); }
   unsigned long long st2[2] = {0, 0};
   e = hipMemcpy(st2, w.stats, 16, hipMemcpyDeviceToHost);

@@ -73,7 +73,7 @@ int fill_direction(const uint8_t *T, uint64_t n, const I *SA, uint32_t ratio, Ho
     for (uint64_t j = b0; j < b1; ++j) {
       OccBlock &B = d->blk[j];
       for (int b = 0; b < 4; ++b) B.cnt[b] = (uint32_t)run[b];
-      for (int w = 0; w < 6; ++w) B.bits[w] = 0;
+      for (int w = 0; w < 3; ++w) { B.lo[w] = 0; B.hi[w] = 0; }
       uint64_t r0 = j * kBlockRows, r1 = std::min<uint64_t>(rows, r0 + kBlockRows);
       for (uint64_t r = r0; r < r1; ++r) {
         uint64_t p = row_pos(r);
@@ -82,7 +82,8 @@ int fill_direction(const uint8_t *T, uint64_t n, const I *SA, uint32_t ratio, Ho
         if (p % ratio == 0) d->R[p / ratio] = (uint32_t)r;
         if (p == 0) { dollar_row = r; continue; }   // '$' kept as code 0, never counted
         uint32_t sym = T[p - 1];
-        B.bits[t >> 5] |= (uint64_t)sym << (2 * (t & 31));
+        B.lo[t >> 6] |= (uint64_t)(sym & 1u) << (t & 63);
+        B.hi[t >> 6] |= (uint64_t)(sym >> 1) << (t & 63);
         run[sym]++;
       }
     }
@@ -137,7 +138,7 @@ uint32_t host_bwt_sym(const HostDir &d, uint64_t r) {
   if (r == d.dollar) return 0xFFu;
   const OccBlock &B = d.blk[r / kBlockRows];
   uint32_t t = (uint32_t)(r % kBlockRows);
-  return (uint32_t)((B.bits[t >> 5] >> (2 * (t & 31))) & 3u);
+  return (uint32_t)((B.lo[t >> 6] >> (t & 63)) & 1u) | ((uint32_t)((B.hi[t >> 6] >> (t & 63)) & 1u) << 1);
 }
 
 int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix, std::string *err) {
@@ -173,7 +174,7 @@ int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix, 
 
 // ---------------------------------------------------------------- file format
 // Little-endian single file:
-//   char magic[8] = "HPGFMI01"; uint32 version = 1; uint32 block_rows = 192;
+//   char magic[8] = "HPGFMI01"; uint32 version = 2; uint32 block_rows = 192;
 //   uint64 n; uint32 ratio; uint32 reserved;
 //   per direction: uint64 dollar; uint32 C[4]; uint64 nblocks; uint64 nsamp;
 //   uint64 kcount; uint64 kstart[kcount]; uint64 kend[kcount];
@@ -182,6 +183,8 @@ int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix, 
 namespace {
 
 const char kMagic[8] = {'H', 'P', 'G', 'F', 'M', 'I', '0', '1'};
+// 2: bit-plane occurrence blocks (1 held 2-bit packed codes)
+const uint32_t kFormatVersion = 2;
 
 struct Fnv {
   uint64_t h = 1469598103934665603ull;
@@ -219,7 +222,7 @@ int save_index(const fmi_index *ix, const char *path, std::string *err) {
   FILE *f = fopen(path, "wb");
   if (!f) { *err = std::string("cannot open ") + path; return FMI_EIO; }
   Writer w{f};
-  uint32_t version = 1, brows = kBlockRows, reserved = 0;
+  uint32_t version = kFormatVersion, brows = kBlockRows, reserved = 0;
   w.put(kMagic, 8);
   w.put(&version, 4);
   w.put(&brows, 4);
@@ -257,7 +260,7 @@ int load_index(const char *path, fmi_index *ix, std::string *err) {
   r.get(magic, 8);
   r.get(&version, 4);
   r.get(&brows, 4);
-  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || version != 1 || brows != kBlockRows) {
+  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || version != kFormatVersion || brows != kBlockRows) {
     fclose(f);
     *err = "not an index file of this version";
     return FMI_EFORMAT;

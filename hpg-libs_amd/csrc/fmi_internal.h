@@ -4,9 +4,13 @@
 // Occurrence table: one 64-byte block per 192 BWT rows, so that an LF step
 // touches one HBM line (two when k and l+1 fall in different blocks):
 //
-//   bytes  0..15  uint32 cnt[4]   occurrences of A,C,G,T in rows [0, 192*j)
-//   bytes 16..63  uint64 bits[6]  2-bit codes of rows 192*j .. 192*j+191;
-//                                 row 192*j+t sits in bits[t/32] at bit 2*(t%32)
+//   bytes  0..15  uint32 cnt[4]  occurrences of A,C,G,T in rows [0, 192*j)
+//   bytes 16..39  uint64 lo[3]   low bit of the code of rows 192*j .. +191
+//   bytes 40..63  uint64 hi[3]   high bit of the code
+//                                row 192*j+t is bit t%64 of word t/64
+//
+// The two bit-planes make a rank three masked popcounts per plane word (one
+// symbol: (lo^x) & (hi^y) & mask; all four: lo, hi, lo&hi).
 //
 // The '$' row is stored as code 0 (A) and corrected out of band: Occ(A, m)
 // subtracts one when the '$' row lies in [192*j, m).  This replaces the
@@ -26,7 +30,8 @@ constexpr uint32_t kMaxReadLen = 1000;
 
 struct alignas(64) OccBlock {
   uint32_t cnt[4];
-  uint64_t bits[6];
+  uint64_t lo[3];
+  uint64_t hi[3];
 };
 static_assert(sizeof(OccBlock) == 64, "occurrence block must be one 64-byte line");
 

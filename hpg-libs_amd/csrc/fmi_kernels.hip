@@ -22,37 +22,55 @@ namespace fmi {
 
 struct Blk {
   uint32_t c[4];
-  uint64_t w[6];
+  uint64_t lo[3], hi[3];   // bit-planes: bit t%64 of word t/64 = code bit of row t
 };
 
 __device__ __forceinline__ Blk load_block(const OccBlock *base, uint32_t j) {
   const uint4 *q = reinterpret_cast<const uint4 *>(base + j);
-  uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+  const uint4 a = q[0], b = q[1], c = q[2], d = q[3];
   Blk B;
   B.c[0] = a.x; B.c[1] = a.y; B.c[2] = a.z; B.c[3] = a.w;
-  B.w[0] = ((uint64_t)b.y << 32) | b.x;
-  B.w[1] = ((uint64_t)b.w << 32) | b.z;
-  B.w[2] = ((uint64_t)c.y << 32) | c.x;
-  B.w[3] = ((uint64_t)c.w << 32) | c.z;
-  B.w[4] = ((uint64_t)d.y << 32) | d.x;
-  B.w[5] = ((uint64_t)d.w << 32) | d.z;
+  B.lo[0] = ((uint64_t)b.y << 32) | b.x;
+  B.lo[1] = ((uint64_t)b.w << 32) | b.z;
+  B.lo[2] = ((uint64_t)c.y << 32) | c.x;
+  B.hi[0] = ((uint64_t)c.w << 32) | c.z;
+  B.hi[1] = ((uint64_t)d.y << 32) | d.x;
+  B.hi[2] = ((uint64_t)d.w << 32) | d.z;
   return B;
 }
 
-// Number of rows among the first t (0..192) rows of the block whose code is s.
-__device__ __forceinline__ uint32_t rank_in(const Blk &B, uint32_t s, uint32_t t) {
-  const uint64_t M = 0x5555555555555555ull;
-  const uint64_t pat = (uint64_t)s * M;
-  uint32_t cnt = 0;
-#pragma unroll
-  for (int wi = 0; wi < 6; ++wi) {
-    int lim = (int)t - 32 * wi;
-    lim = lim < 0 ? 0 : (lim > 32 ? 32 : lim);
-    const uint64_t mask = lim == 32 ? ~0ull : ((1ull << (2 * lim)) - 1ull);
-    const uint64_t x = B.w[wi] ^ pat;
-    cnt += (uint32_t)__popcll(~(x | (x >> 1)) & M & mask);
-  }
-  return cnt;
+// Masks selecting the first t (0..192) rows of the block in each plane word.
+__device__ __forceinline__ void row_masks(uint32_t t, uint64_t &m0, uint64_t &m1, uint64_t &m2) {
+  const uint32_t r = t & 63u, q = t >> 6;
+  const uint64_t pm = r ? (~0ull >> (64u - r)) : 0ull;
+  m0 = q >= 1u ? ~0ull : pm;
+  m1 = q >= 2u ? ~0ull : (q == 1u ? pm : 0ull);
+  m2 = q >= 3u ? ~0ull : (q == 2u ? pm : 0ull);
+}
+
+// Rows among the first t of the block whose code is s (the '$' row holds 0).
+__device__ __forceinline__ uint32_t rank1(const Blk &B, uint32_t s, uint32_t t) {
+  uint64_t m0, m1, m2;
+  row_masks(t, m0, m1, m2);
+  const uint64_t xl = (s & 1u) ? 0ull : ~0ull, xh = (s & 2u) ? 0ull : ~0ull;
+  return (uint32_t)(__popcll((B.lo[0] ^ xl) & (B.hi[0] ^ xh) & m0) +
+                    __popcll((B.lo[1] ^ xl) & (B.hi[1] ^ xh) & m1) +
+                    __popcll((B.lo[2] ^ xl) & (B.hi[2] ^ xh) & m2));
+}
+
+// The same for all four codes at once: three popcounts per plane word.
+__device__ __forceinline__ void rank4(const Blk &B, uint32_t t, uint32_t (&r)[4]) {
+  uint64_t m0, m1, m2;
+  row_masks(t, m0, m1, m2);
+  const uint64_t a0 = B.lo[0] & m0, a1 = B.lo[1] & m1, a2 = B.lo[2] & m2;
+  const uint64_t b0 = B.hi[0] & m0, b1 = B.hi[1] & m1, b2 = B.hi[2] & m2;
+  const uint32_t nl = (uint32_t)(__popcll(a0) + __popcll(a1) + __popcll(a2));
+  const uint32_t nh = (uint32_t)(__popcll(b0) + __popcll(b1) + __popcll(b2));
+  const uint32_t nb = (uint32_t)(__popcll(a0 & b0) + __popcll(a1 & b1) + __popcll(a2 & b2));
+  r[3] = nb;
+  r[2] = nh - nb;
+  r[1] = nl - nb;
+  r[0] = t - nl - nh + nb;
 }
 
 // a[i] for a runtime i in 0..3 without dynamic register indexing.
@@ -62,32 +80,52 @@ __device__ __forceinline__ uint32_t pick4(const uint32_t (&a)[4], uint32_t i) {
   return (i & 2u) ? hi : lo;
 }
 
+__device__ __forceinline__ uint64_t pick3(const uint64_t (&a)[3], uint32_t i) {
+  return i == 0u ? a[0] : (i == 1u ? a[1] : a[2]);
+}
+
+// Code stored at row t of the block.
+__device__ __forceinline__ uint32_t code_at(const Blk &B, uint32_t t) {
+  const uint32_t w = t >> 6, b = t & 63u;
+  return (uint32_t)((pick3(B.lo, w) >> b) & 1ull) | ((uint32_t)((pick3(B.hi, w) >> b) & 1ull) << 1);
+}
+
 // Occ(s, m) for m in block j (t = m - 192*j): checkpoint + in-block rank,
 // minus the '$' row (stored as code 0) when it lies before m in the block.
 __device__ __forceinline__ uint32_t occ_blk(const Blk &B, uint32_t dollar_blk,
                                             uint32_t dollar_off, uint32_t j, uint32_t s,
                                             uint32_t t) {
-  uint32_t v = pick4(B.c, s) + rank_in(B, s, t);
+  uint32_t v = pick4(B.c, s) + rank1(B, s, t);
   if (s == 0 && j == dollar_blk && dollar_off < t) v -= 1;
   return v;
 }
 
-__device__ __forceinline__ uint32_t code_at(const Blk &B, uint32_t t) {
-  return (uint32_t)((B.w[t >> 5] >> (2 * (t & 31))) & 3u);
+// Symbol of row k (block j, offset t), 4 for the '$' row.
+__device__ __forceinline__ uint32_t sym_at(const Blk &B, uint32_t dollar_blk, uint32_t dollar_off,
+                                           uint32_t j, uint32_t t) {
+  return (j == dollar_blk && t == dollar_off) ? 4u : code_at(B, t);
 }
 
 // LF step of one physical symbol s on [k, l] (BWiteration, runtime.h:38-42).
+// A size-1 interval needs only row k's block: Occ(s, k+1) = Occ(s, k) +
+// [BWT[k] == s].
 __device__ __forceinline__ void lf_one(const DevDir &D, uint32_t k, uint32_t l, uint32_t s,
                                        uint32_t &ko, uint32_t &lo, uint32_t &nblk) {
   const uint32_t m2 = l + 1;
   const uint32_t j1 = k / kBlockRows, j2 = m2 / kBlockRows;
+  const uint32_t t1 = k - j1 * kBlockRows;
   const Blk B1 = load_block(D.blk, j1);
+  const uint32_t c = pick4(D.C, s);
+  ko = c + 1u + occ_blk(B1, D.dollar_blk, D.dollar_off, j1, s, t1);
+  if (k == l) {
+    nblk = 1u;
+    lo = ko - 1u + (sym_at(B1, D.dollar_blk, D.dollar_off, j1, t1) == s ? 1u : 0u);
+    return;
+  }
   Blk B2;
   if (j2 != j1) B2 = load_block(D.blk, j2);
   else B2 = B1;
   nblk = (j2 != j1) ? 2u : 1u;
-  const uint32_t c = pick4(D.C, s);
-  ko = c + 1u + occ_blk(B1, D.dollar_blk, D.dollar_off, j1, s, k - j1 * kBlockRows);
   lo = c + occ_blk(B2, D.dollar_blk, D.dollar_off, j2, s, m2 - j2 * kBlockRows);
 }
 
@@ -221,7 +259,7 @@ __global__ __launch_bounds__(256) void vector_kernel(VectorParams P) {
 //          mismatch b, in the reference's order) to the end of the read
 // Records are written in the order the reference adds them.
 
-enum : uint32_t { PH_IDLE = 0, PH_EXACT = 1, PH_SCAN = 2, PH_CONT = 3 };
+enum : uint32_t { PH_IDLE = 0, PH_EXACT = 1, PH_SCAN = 2, PH_CONT = 3, PH_NEXT_STRAND = 4 };
 
 struct S1Params {
   DevIndex ix;
@@ -232,6 +270,7 @@ struct S1Params {
   uint32_t *queue;                 // work counter (persistent launch)
   unsigned long long *stats;       // [0] blocks, [1] LF queries (count mode)
   uint32_t ntasks, strand_mask, cap, chunk;
+  uint32_t refill_min;             // idle lanes before a wave takes new tasks
 };
 
 struct S1Lane {
@@ -273,9 +312,22 @@ struct S1 {
     L.nout++;
   }
 
+  // A task is one read: its + strand, then its - strand when both are asked
+  // for, so that every lane carries one true-strand and one wrong-strand
+  // search and lanes finish at similar times.
+  // The - strand is started from the main loop (PH_NEXT_STRAND), not from
+  // here: calling start_strand would close a call cycle that the compiler
+  // can only lower as recursion on a scratch stack.
   __device__ __forceinline__ void finish_task() {
     P.counts[2 * L.read + L.strand] = L.nout;
-    L.phase = PH_IDLE;
+    L.phase = (P.strand_mask == 3u && L.strand == 0u) ? PH_NEXT_STRAND : PH_IDLE;
+  }
+
+  __device__ __forceinline__ void start_strand(uint32_t strand) {
+    L.strand = strand;
+    L.out = P.hits + (size_t)(2 * L.read + strand) * P.cap;
+    L.nout = 0;
+    enter_pass(0);
   }
 
   __device__ __forceinline__ void end_pass() {
@@ -358,15 +410,13 @@ struct S1 {
   }
 
   __device__ __forceinline__ void start(uint32_t task) {
-    if (P.strand_mask == 3u) { L.read = task >> 1; L.strand = task & 1u; }
-    else { L.read = task; L.strand = (P.strand_mask == 2u) ? 1u : 0u; }
+    L.read = task;
     const uint64_t off = P.offsets[L.read];
     L.W = P.reads + off;
     L.len = (int32_t)(P.offsets[L.read + 1] - off);
-    L.out = P.hits + (size_t)(2 * L.read + L.strand) * P.cap;
-    L.nout = 0;
-    if (P.strand_mask != 3u) P.counts[2 * L.read + (L.strand ^ 1u)] = 0;
-    enter_pass(0);
+    const uint32_t strand = (P.strand_mask == 2u) ? 1u : 0u;
+    if (P.strand_mask != 3u) P.counts[2 * L.read + (strand ^ 1u)] = 0;
+    start_strand(strand);
   }
 
   // One LF query and the transition it drives.
@@ -379,18 +429,34 @@ struct S1 {
     const uint32_t sym = L.W[p];
     const uint32_t m2 = al + 1;
     const uint32_t j1 = ak / kBlockRows, j2 = m2 / kBlockRows;
+    const uint32_t t1 = ak - j1 * kBlockRows, t2 = m2 - j2 * kBlockRows;
+    // A size-1 interval needs only row k's block (Occ(s, k+1) = Occ(s, k) +
+    // [BWT[k] == s]); otherwise the block of l+1 too when it differs.
+    const bool single = ak == al;
+    const bool two = !single && j2 != j1;
     const Blk B1 = load_block(L.blk, j1);
     Blk B2;
-    if (j2 != j1) B2 = load_block(L.blk, j2);
+    if (two) B2 = load_block(L.blk, j2);
     else B2 = B1;
-    if (COUNT) { nblk_acc += (j2 != j1) ? 2u : 1u; nq_acc += 1u; }
-    const uint32_t t1 = ak - j1 * kBlockRows, t2 = m2 - j2 * kBlockRows;
+    if (COUNT) { nblk_acc += two ? 2u : 1u; nq_acc += 1u; }
+    const uint32_t dfix1 = (j1 == L.dollar_blk && L.dollar_off < t1) ? 1u : 0u;
+    const uint32_t dfix2 = (j2 == L.dollar_blk && L.dollar_off < t2) ? 1u : 0u;
+    const uint32_t srow = single ? sym_at(B1, L.dollar_blk, L.dollar_off, j1, t1) : 4u;
 
     if (L.phase == PH_SCAN) {
+      uint32_t r1[4], r2[4];
+      rank4(B1, t1, r1);
+      r1[0] -= dfix1;
 #pragma unroll
-      for (uint32_t s = 0; s < 4; ++s) {
-        L.ck[s] = L.C[s] + 1u + occ_blk(B1, L.dollar_blk, L.dollar_off, j1, s, t1);
-        L.cl[s] = L.C[s] + occ_blk(B2, L.dollar_blk, L.dollar_off, j2, s, t2);
+      for (uint32_t s = 0; s < 4; ++s) L.ck[s] = L.C[s] + 1u + pick4(B1.c, s) + r1[s];
+      if (single) {
+#pragma unroll
+        for (uint32_t s = 0; s < 4; ++s) L.cl[s] = L.ck[s] - 1u + (srow == s ? 1u : 0u);
+      } else {
+        rank4(B2, t2, r2);
+        r2[0] -= dfix2;
+#pragma unroll
+        for (uint32_t s = 0; s < 4; ++s) L.cl[s] = L.C[s] + pick4(B2.c, s) + r2[s];
       }
       L.wsym = sym;
       const int64_t size_node = (int64_t)L.l - (int64_t)L.k;
@@ -416,8 +482,9 @@ struct S1 {
     if (sym < 4u) {
       const uint32_t s = L.comp ? 3u - sym : sym;
       const uint32_t c = pick4(L.C, s);
-      rk = c + 1u + occ_blk(B1, L.dollar_blk, L.dollar_off, j1, s, t1);
-      rl = c + occ_blk(B2, L.dollar_blk, L.dollar_off, j2, s, t2);
+      rk = c + 1u + pick4(B1.c, s) + rank1(B1, s, t1) - (s == 0u ? dfix1 : 0u);
+      if (single) rl = rk - 1u + (srow == s ? 1u : 0u);
+      else rl = c + pick4(B2.c, s) + rank1(B2, s, t2) - (s == 0u ? dfix2 : 0u);
     } else {
       rk = P.ix.n + 1;
       rl = P.ix.n;
@@ -459,16 +526,23 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
   if (!PERSISTENT) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < P.ntasks) m.start(t);
-    while (L.phase != PH_IDLE) m.template advance<COUNT>(nblk, nq);
+    while (L.phase != PH_IDLE) {
+      if (L.phase == PH_NEXT_STRAND) m.start_strand(1u);
+      else m.template advance<COUNT>(nblk, nq);
+    }
   } else {
     // Wavefront-level work queue: the wave owns tasks [wq_next, wq_end) and
     // hands them to idle lanes by rank among the idle lanes (ballot +
     // popcount); a new chunk is taken with one atomic when it runs dry.
+    // Lanes are refilled only once refill_min of them are idle (or the whole
+    // wave is), so the task set-up runs for many lanes at a time.
     uint32_t wq_next = 0, wq_end = 0;
     bool more = true;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     while (true) {
       uint64_t idle = __ballot(L.phase == PH_IDLE);
+      const uint32_t nidle0 = (uint32_t)__popcll(idle);
+      if (nidle0 < P.refill_min && nidle0 < 64u) idle = 0ull;
       while (idle != 0ull && more) {
         if (wq_next >= wq_end) {
           uint32_t base = 0;
@@ -486,7 +560,8 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
         idle = __ballot(L.phase == PH_IDLE);
       }
       if (__ballot(L.phase != PH_IDLE) == 0ull) break;
-      if (L.phase != PH_IDLE) m.template advance<COUNT>(nblk, nq);
+      if (L.phase == PH_NEXT_STRAND) m.start_strand(1u);
+      else if (L.phase != PH_IDLE) m.template advance<COUNT>(nblk, nq);
     }
   }
   if (COUNT) {
@@ -618,24 +693,29 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   return x;
 }
 
-__global__ __launch_bounds__(256) void gather_kernel(const uint4 *buf, uint64_t nlines,
-                                                     uint32_t vec_per_line, uint64_t per_thread,
+// Each lane issues the loads of four random lines (VEC 16-byte pieces each)
+// before consuming any, so 4*VEC loads are in flight per lane; line indexes
+// come from a 64-bit mix over a power-of-two line count (a mask).
+template <int VEC>
+__global__ __launch_bounds__(256) void gather_kernel(const uint4 *buf, uint64_t line_mask,
+                                                     uint64_t per_thread,
                                                      unsigned long long *sink) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t st = mix64(tid * 0x9E3779B97F4A7C15ull + 1);
   uint32_t acc = 0;
   for (uint64_t it = 0; it < per_thread; it += 4) {
-    uint64_t li[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) { st = mix64(st + u + 1); li[u] = st % nlines; }
+    uint4 x[4][VEC];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const uint4 *p = buf + li[u] * vec_per_line;
-      for (uint32_t v = 0; v < vec_per_line; ++v) {
-        const uint4 x = p[v];
-        acc ^= x.x ^ x.y ^ x.z ^ x.w;
-      }
+      st = mix64(st + u + 1);
+      const uint4 *p = buf + (st & line_mask) * VEC;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) x[u][v] = p[v];
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc ^= x[u][v].x ^ x[u][v].y ^ x[u][v].z ^ x[u][v].w;
   }
   if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);   // keeps the loads live
 }
@@ -666,12 +746,14 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
                           uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
                           uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
-                          uint32_t chunk, hipStream_t s) {
-  const uint32_t ntasks = (strand_mask == 3u) ? 2u * nreads : nreads;
-  S1Params P{ix, reads, offsets, hits, counts, queue, stats, ntasks, strand_mask, cap, chunk};
+                          uint32_t chunk, uint32_t refill_min, hipStream_t s) {
+  const uint32_t ntasks = nreads;   // one task per read (both strands in turn)
+  S1Params P{ix, reads, offsets, hits, counts, queue, stats, ntasks, strand_mask, cap, chunk,
+             refill_min};
   if (ntasks == 0) return hipSuccess;
   if (persistent) {
-    if (grid_persistent == 0 || chunk == 0) return hipErrorInvalidValue;
+    if (grid_persistent == 0 || chunk == 0 || refill_min == 0 || refill_min > 64)
+      return hipErrorInvalidValue;
     const dim3 g(grid_persistent), b(256);
     if (mm_only) {
       if (count) hipLaunchKernelGGL((search1_kernel<true, true, true>), g, b, 0, s, P);
@@ -720,12 +802,18 @@ hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_
 
 hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
                          unsigned long long *sink, uint32_t grid, hipStream_t s) {
-  const uint64_t nlines = bytes / line_bytes;
+  uint64_t nlines = 1;
+  while (nlines * 2 <= bytes / line_bytes) nlines *= 2;   // power of two <= bytes/line
   const uint64_t threads = (uint64_t)grid * 256;
   uint64_t per_thread = (nloads + threads - 1) / threads;
   per_thread = (per_thread + 3) & ~3ull;
-  hipLaunchKernelGGL(gather_kernel, dim3(grid), dim3(256), 0, s, (const uint4 *)buf, nlines,
-                     line_bytes / 16, per_thread, sink);
+  const uint4 *b = (const uint4 *)buf;
+  switch (line_bytes) {
+    case 64: hipLaunchKernelGGL(gather_kernel<4>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
+    case 128: hipLaunchKernelGGL(gather_kernel<8>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
+    case 256: hipLaunchKernelGGL(gather_kernel<16>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

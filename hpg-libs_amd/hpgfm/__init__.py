@@ -38,7 +38,7 @@ EXPORTS = [
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
     "fmi_stream_destroy", "fmi_stream_sync", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
-    "fmi_set_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
+    "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
 ]
 
@@ -87,7 +87,8 @@ def _bind(lib):
         "fmi_exact_vector_batch": ([_p, _i, _i, _p, _p, _u32, _p, _p, _p, _p], _i),
         "fmi_search1_batch": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_search1_batch_plain": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
-        "fmi_set_tuning": ([_u32, _u32], _i),
+        "fmi_set_tuning": ([_u32, _u32, _u32], _i),
+        "fmi_get_tuning": ([C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)], _i),
         "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
         "fmi_gather_bench": ([_p, _u64, _u32, _u64, _p, C.POINTER(C.c_float), C.POINTER(_u64),
@@ -221,8 +222,14 @@ class TimingRegion:
         return self.sum_ms / max(self.launches, 1)
 
 
-def set_tuning(wg_per_cu=0, chunk=0):
-    check(lib().fmi_set_tuning(wg_per_cu, chunk))
+def set_tuning(wg_per_cu=0, chunk=0, refill_min=0):
+    check(lib().fmi_set_tuning(wg_per_cu, chunk, refill_min))
+
+
+def get_tuning():
+    a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    check(lib().fmi_get_tuning(C.byref(a), C.byref(b), C.byref(c)))
+    return int(a.value), int(b.value), int(c.value)
 
 
 class Index:

@@ -188,9 +188,12 @@ int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
                             uint32_t nreads, uint32_t cap, fmi_hit *d_hits,
                             uint32_t *d_counts, fmi_stream s);
 
-/* Tuning of the persistent launch (the autotuner writes it): workgroups per
- * CU, tasks a wavefront takes from the queue at a time. 0 = default. */
-int fmi_set_tuning(uint32_t wg_per_cu, uint32_t chunk);
+/* Tuning of the persistent launch (the autotuner writes it): workgroups of
+ * 256 lanes per CU, tasks a wavefront takes from the queue per atomic, and
+ * the number of idle lanes (1..64) at which a wavefront refills them.  0
+ * selects the default.  Results never depend on the tuning. */
+int fmi_set_tuning(uint32_t wg_per_cu, uint32_t chunk, uint32_t refill_min);
+int fmi_get_tuning(uint32_t *wg_per_cu, uint32_t *chunk, uint32_t *refill_min);
 
 /* ------------------------------------------------------------ locate */
 /* get_SA (search/runtime.h:87, getScompValue search/csafm.h:208-235) for

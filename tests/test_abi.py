@@ -52,10 +52,16 @@ def test_calls_before_to_device_are_rejected():
 
 def test_tuning_range():
     with pytest.raises(hpgfm.FmiError):
-        hpgfm.set_tuning(65, 32)
+        hpgfm.set_tuning(65, 32, 16)
     with pytest.raises(hpgfm.FmiError):
-        hpgfm.set_tuning(8, 5000)
-    hpgfm.set_tuning(0, 0)
+        hpgfm.set_tuning(8, 5000, 16)
+    with pytest.raises(hpgfm.FmiError):
+        hpgfm.set_tuning(8, 32, 65)
+    hpgfm.set_tuning(4, 128, 8)
+    assert hpgfm.get_tuning() == (4, 128, 8)
+    hpgfm.set_tuning(0, 0, 0)
+    wg, chunk, refill = hpgfm.get_tuning()
+    assert wg > 0 and chunk > 0 and 1 <= refill <= 64
 
 
 def test_read_validation():

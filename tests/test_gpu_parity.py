@@ -226,6 +226,27 @@ def test_results_independent_of_batch_mix(gpu_engine):
     big.free()
 
 
+def test_results_independent_of_tuning(gpu_engine):
+    """Workgroups per CU, queue chunk and refill threshold change the
+    schedule, never the records."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(400_000, seed=81, repeat_frac=0.3)
+    ix = on_device(hp, g, 16)
+    reads, offs, _ = synth.reads(g, 4000, seed=82, sub_rate=0.01,
+                                 lengths=np.random.default_rng(83).integers(30, 200, 4000))
+    batch = hp.ReadBatch(reads, offs)
+    base_c, base_h = hp.search1_batch(ix, batch, cap=8, mode=hp.MODE_EDIT, plain=True)
+    try:
+        for wg, chunk, refill in ((1, 1, 1), (8, 64, 16), (2, 1024, 64), (16, 7, 33)):
+            hp.set_tuning(wg, chunk, refill)
+            c, h = hp.search1_batch(ix, batch, cap=8, mode=hp.MODE_EDIT)
+            assert np.array_equal(c, base_c)
+            assert np.array_equal(h, base_h)
+    finally:
+        hp.set_tuning(0, 0, 0)
+    batch.free()
+
+
 def test_count_blocks_bounds(gpu_engine):
     """Counted blocks lie between one and two per LF query."""
     hp = gpu_engine
