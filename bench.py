@@ -56,6 +56,8 @@ def main():
                     help="skip the gather roofline probe and the plain / edit comparisons")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shape")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="length of each serving run (0 skips serving)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -188,6 +190,26 @@ def main():
             extras["parity_counts_vs_cpu_sample"] = bool(same)
             log(f"GPU edit-mode record counts == reference counts on the sample: {same}")
 
+    served = None
+    if args.serve_seconds > 0:
+        from hpgfm import serve
+        t0 = time.time()
+        pool = serve.read_pool(g, 1_000_000, seed=5, sub_rate=args.sub_rate)
+        log(f"serving read pool (1M reads of 50-250 bp) in {time.time() - t0:.1f}s")
+        served = {"index": "the same resident index", "requests": "Poisson arrivals, U[1, 50000] "
+                  "reads of U[50, 250] bp, 1% substitutions, <=1 mismatch, both strands",
+                  "runs": []}
+        srv = hpgfm.Server(ix, budget_bytes=2 << 30, max_inflight=4, max_batch_reads=1 << 20,
+                           cap=args.cap, mode=mode)
+        for rate in (20, 200, 2000):
+            r = serve.run(srv, pool, rate=rate, duration_s=args.serve_seconds, seed=rate)
+            r.pop("server", None)
+            served["runs"].append(r)
+            log(f"served at {rate} req/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
+                f"{r['reads_per_s'] / 1e6:.2f} M reads/s over {r['requests']} requests")
+        served["server_stats"] = srv.stats()
+        srv.close()
+
     res = {
         "metric": METRIC,
         "value": round(reads_per_s),
@@ -226,6 +248,11 @@ def main():
         "index_cache_resident": ix.device_bytes < 256 * 2 ** 20,
         "tuning": tuning,
     }
+    if served:
+        at200 = next(r for r in served["runs"] if r["rate_req_s"] == 200)
+        res["served_p50_ms"] = at200["p50_ms"]
+        res["served_p99_ms"] = at200["p99_ms"]
+        res["served"] = served
     res.update(extras)
     out.free()
     batch.free()

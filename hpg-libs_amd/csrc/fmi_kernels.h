@@ -22,6 +22,13 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
                           uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
                           uint32_t chunk, uint32_t refill_min, hipStream_t s);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t s);
+// Dense copy of the stored records of nslots (read, strand) slots, slot after
+// slot; *total (device) receives the record count.  scratch holds
+// compact_scratch_bytes(nslots) bytes.
+uint64_t compact_scratch_bytes(uint32_t nslots);
+hipError_t launch_compact(const uint32_t *counts, const fmi_hit *hits, uint32_t nslots,
+                          uint32_t cap, fmi_hit *out, unsigned long long *total, void *scratch,
+                          hipStream_t s);
 hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
                          uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
                          hipStream_t s);

@@ -683,6 +683,75 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveParams P) {
 }
 
 // --------------------------------------------------------------------------
+// record compaction: the stored records of every (read, strand) slot,
+// slot after slot, in one dense array (exclusive scan of min(count, cap))
+// --------------------------------------------------------------------------
+// Three launches: per-1024-slot sums; one workgroup scans the block sums;
+// each block rescans its slots and copies its records.
+
+constexpr uint32_t kScanBlock = 1024;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds, uint32_t *total) {
+  const uint32_t t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
+    const uint32_t x = t >= d ? lds[t - d] : 0u;
+    __syncthreads();
+    lds[t] += x;
+    __syncthreads();
+  }
+  const uint32_t incl = lds[t];
+  *total = lds[blockDim.x - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void compact_sums_kernel(const uint32_t *counts,
+                                                                 uint32_t nslots, uint32_t cap,
+                                                                 uint32_t *block_sums) {
+  __shared__ uint32_t lds[kScanBlock];
+  const uint32_t s = blockIdx.x * kScanBlock + threadIdx.x;
+  const uint32_t v = s < nslots ? min(counts[s], cap) : 0u;
+  uint32_t total;
+  block_exclusive_scan(v, lds, &total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void compact_scan_kernel(uint32_t *block_sums,
+                                                                 uint32_t nblocks,
+                                                                 unsigned long long *total_out) {
+  __shared__ uint32_t lds[kScanBlock];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nblocks; base += kScanBlock) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nblocks ? block_sums[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, lds, &total);
+    if (i < nblocks) block_sums[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void compact_copy_kernel(const uint32_t *counts,
+                                                                 const fmi_hit *hits,
+                                                                 uint32_t nslots, uint32_t cap,
+                                                                 const uint32_t *block_offsets,
+                                                                 fmi_hit *out) {
+  __shared__ uint32_t lds[kScanBlock];
+  const uint32_t s = blockIdx.x * kScanBlock + threadIdx.x;
+  const uint32_t v = s < nslots ? min(counts[s], cap) : 0u;
+  uint32_t total;
+  const uint32_t ex = block_exclusive_scan(v, lds, &total);
+  if (s >= nslots) return;
+  const uint64_t dst = (uint64_t)block_offsets[blockIdx.x] + ex;
+  const uint4 *src = reinterpret_cast<const uint4 *>(hits + (uint64_t)s * cap);
+  uint4 *d = reinterpret_cast<uint4 *>(out + dst);
+  for (uint32_t j = 0; j < 2u * v; ++j) d[j] = src[j];
+}
+
+// --------------------------------------------------------------------------
 // random-gather roofline probe
 // --------------------------------------------------------------------------
 
@@ -767,6 +836,26 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
     if (mm_only) hipLaunchKernelGGL((search1_kernel<false, true, false>), g, b, 0, s, P);
     else hipLaunchKernelGGL((search1_kernel<false, false, false>), g, b, 0, s, P);
   }
+  return hipGetLastError();
+}
+
+uint64_t compact_scratch_bytes(uint32_t nslots) {
+  return ((uint64_t)(nslots + kScanBlock - 1) / kScanBlock) * 4 + 64;
+}
+
+hipError_t launch_compact(const uint32_t *counts, const fmi_hit *hits, uint32_t nslots,
+                          uint32_t cap, fmi_hit *out, unsigned long long *total, void *scratch,
+                          hipStream_t s) {
+  const uint32_t nblocks = (nslots + kScanBlock - 1) / kScanBlock;
+  if (nblocks == 0 || cap == 0) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(total), 2u);
+    return hipGetLastError();
+  }
+  uint32_t *sums = static_cast<uint32_t *>(scratch);
+  hipLaunchKernelGGL(compact_sums_kernel, dim3(nblocks), dim3(kScanBlock), 0, s, counts, nslots, cap, sums);
+  hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, sums, nblocks, total);
+  hipLaunchKernelGGL(compact_copy_kernel, dim3(nblocks), dim3(kScanBlock), 0, s, counts, hits, nslots,
+                     cap, sums, out);
   return hipGetLastError();
 }
 

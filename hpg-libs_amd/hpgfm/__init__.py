@@ -40,7 +40,21 @@ EXPORTS = [
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
     "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
+    "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
+    "fmi_server_get_stats", "fmi_server_destroy",
 ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32),
+                ("max_batch_reads", C.c_uint32), ("cap", C.c_uint32), ("mode", C.c_int),
+                ("strand_mask", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("reads_done", C.c_uint64),
+                ("launches", C.c_uint64), ("max_batch_reads_seen", C.c_uint64),
+                ("max_inflight_seen", C.c_uint32), ("pending", C.c_uint32)]
 
 
 class FmiError(RuntimeError):
@@ -95,6 +109,12 @@ def _bind(lib):
                               _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
                                       C.POINTER(_u64)], _i),
+        "fmi_server_create": ([_p, C.POINTER(ServerConfig), C.POINTER(_p)], _i),
+        "fmi_server_submit": ([_p, _p, _p, _u32, C.POINTER(_u64)], _i),
+        "fmi_server_wait": ([_p, _u64, C.c_int64, C.POINTER(C.c_double), C.POINTER(_u64)], _i),
+        "fmi_server_fetch": ([_p, _u64, _p, _p], _i),
+        "fmi_server_get_stats": ([_p, C.POINTER(ServerStats)], _i),
+        "fmi_server_destroy": ([_p], _i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -421,6 +441,63 @@ def resolve_hits(ix, batch, out, row_cap, stream=None):
     shape = (n, 2, cap, row_cap)
     return (dc.download(np.int32, 2 * n * cap * row_cap, stream).reshape(shape),
             dd.download(np.uint64, 2 * n * cap * row_cap, stream).reshape(shape))
+
+
+class Server:
+    """Continuous-batching request loop over a device-resident index
+    (fmi_server_*).  submit() copies the reads and returns a request id;
+    wait(id) returns (counts[n, 2], records, latency_ms) where records holds
+    the stored records of every (read, strand) slot, slot after slot."""
+
+    def __init__(self, ix, budget_bytes=1 << 30, max_inflight=4, max_batch_reads=1 << 20,
+                 cap=16, mode=MODE_MISMATCH, strand_mask=3):
+        self.cfg = ServerConfig(budget_bytes, max_inflight, max_batch_reads, cap, mode,
+                                strand_mask)
+        h = C.c_void_p()
+        check(lib().fmi_server_create(ix.h, C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self.ix = ix
+        self.cap = cap
+        self._n = {}
+
+    def submit(self, reads, offsets):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = len(offsets) - 1
+        rid = C.c_uint64()
+        check(lib().fmi_server_submit(self.h, _ptr(reads) if len(reads) else None, _ptr(offsets),
+                                      n, C.byref(rid)))
+        self._n[rid.value] = n
+        return rid.value
+
+    def wait(self, rid, timeout_us=-1):
+        lat = C.c_double()
+        nrec = C.c_uint64()
+        rc = lib().fmi_server_wait(self.h, rid, timeout_us, C.byref(lat), C.byref(nrec))
+        if rc == -8:
+            return None
+        check(rc)
+        n = self._n.pop(rid)
+        counts = np.zeros(max(2 * n, 1), np.uint32)
+        recs = np.zeros(max(int(nrec.value), 1), HIT_DTYPE)
+        check(lib().fmi_server_fetch(self.h, rid, _ptr(counts), _ptr(recs)))
+        return counts[:2 * n].reshape(n, 2), recs[:int(nrec.value)], lat.value
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().fmi_server_get_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h is not None:
+            lib().fmi_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gather_bench(nbytes=8 << 30, line_bytes=64, nlines=1 << 28, stream=None):

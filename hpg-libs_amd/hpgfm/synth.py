@@ -106,20 +106,26 @@ def reads(g, nreads, length=100, seed=2, sub_rate=0.0, random_frac=0.0,
             m = rng.random(mat.shape) < n_rate
             mat[m] = N
         out[:] = mat.reshape(-1)
-    else:
-        for i in range(nreads):
-            L = int(lengths[i])
-            s = g[pos[i]:pos[i] + L].copy()
-            if strand[i]:
-                s = revcomp(s)
-            if is_rand[i]:
-                s = rng.integers(0, 4, size=L, dtype=np.uint8)
-            if sub_rate > 0:
-                m = rng.random(L) < sub_rate
-                s[m] = (s[m] % 4 + rng.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) % 4
-            if n_rate > 0:
-                s[rng.random(L) < n_rate] = N
-            out[int(offsets[i]):int(offsets[i + 1])] = s
+    elif nreads:
+        # variable lengths, vectorised over all bases: base j of read i comes
+        # from pos[i] + j (+ strand) or pos[i] + L_i - 1 - j (- strand)
+        starts = offsets[:-1].astype(np.int64)
+        local = np.arange(total, dtype=np.int64) - np.repeat(starts, lengths)
+        rev = np.repeat(strand == 1, lengths)
+        lrep = np.repeat(lengths, lengths)
+        src = np.repeat(pos.astype(np.int64), lengths) + np.where(rev, lrep - 1 - local, local)
+        s = g[src]
+        s = np.where(rev & (s < 4), 3 - s, s).astype(np.uint8)
+        rnd = np.repeat(is_rand, lengths)
+        nr = int(rnd.sum())
+        if nr:
+            s[rnd] = rng.integers(0, 4, size=nr, dtype=np.uint8)
+        if sub_rate > 0:
+            m = rng.random(total) < sub_rate
+            s[m] = (s[m] % 4 + rng.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) % 4
+        if n_rate > 0:
+            s[rng.random(total) < n_rate] = N
+        out[:] = s
     return out, offsets, dict(pos=pos, strand=strand, is_random=is_rand)
 
 

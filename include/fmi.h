@@ -46,7 +46,8 @@ typedef enum {
   FMI_EFORMAT = -4,   /* index file corrupt or of another version       */
   FMI_EDEVICE = -5,   /* HIP runtime error or no usable GPU             */
   FMI_ERANGE = -6,    /* a size is beyond what the engine supports      */
-  FMI_ESTATE = -7     /* call out of order (e.g. index not on device)   */
+  FMI_ESTATE = -7,    /* call out of order (e.g. index not on device)   */
+  FMI_EAGAIN = -8     /* not finished yet (fmi_server_wait timed out)   */
 } fmi_status;
 
 #define FMI_MAX_READ_LEN 1000
@@ -218,6 +219,58 @@ int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k,
 int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
                      const uint64_t *d_offsets, uint32_t nreads, uint32_t cap,
                      uint32_t row_cap, int32_t *d_chrom, uint64_t *d_coord, fmi_stream s);
+
+/* ----------------------------------------------------------- serving */
+/* A request loop over a device-resident index (no counterpart in the
+ * reference, whose closest analog is the bounded work queue of
+ * commons/workflow_scheduler.c:252-271 sized from free memory,
+ * commons/system_utils.c:10-145).  A scheduler thread admits queued
+ * requests first-come first-served into launches while the device arena
+ * (device_budget_bytes) has room, coalesces up to max_batch_reads reads per
+ * launch, keeps up to max_inflight launches in flight on separate streams
+ * (requests that arrive meanwhile join the next launch as memory frees up),
+ * and scatters each request's records back to the buffers given at submit.
+ * Each request gets exactly the records fmi_search1_batch gives for its reads
+ * alone. */
+typedef struct fmi_server fmi_server;
+
+typedef struct fmi_server_config {
+  uint64_t device_budget_bytes;  /* arena for in-flight launches (and as much pinned host) */
+  uint32_t max_inflight;         /* launches in flight at once, 1..8          */
+  uint32_t max_batch_reads;      /* reads coalesced into one launch           */
+  uint32_t cap;                  /* records kept per (read, strand)           */
+  int mode;                      /* FMI_MODE_EDIT or FMI_MODE_MISMATCH        */
+  int strand_mask;               /* FMI_STRAND_PLUS | FMI_STRAND_MINUS        */
+} fmi_server_config;
+
+typedef struct fmi_server_stats {
+  uint64_t requests_done;
+  uint64_t reads_done;
+  uint64_t launches;
+  uint64_t max_batch_reads_seen;  /* largest launch                         */
+  uint32_t max_inflight_seen;     /* most launches in flight at once         */
+  uint32_t pending;               /* requests waiting for admission now      */
+} fmi_server_stats;
+
+int fmi_server_create(const fmi_index *ix, const fmi_server_config *cfg, fmi_server **out);
+/* Queues a request: reads/offsets as for fmi_search1_batch, copied before
+ * returning.  FMI_ERANGE when the request alone exceeds the budget or
+ * max_batch_reads (it is rejected, never queued). */
+int fmi_server_submit(fmi_server *srv, const uint8_t *reads, const uint64_t *offsets,
+                      uint32_t nreads, uint64_t *request_id);
+/* Waits up to timeout_us (< 0: forever) for a request; FMI_EAGAIN on time-out.
+ * latency_ms: from submit to the moment its records reached host memory;
+ * nrecords: the number of stored records (sum over its 2*nreads slots of
+ * min(count, cap)). */
+int fmi_server_wait(fmi_server *srv, uint64_t request_id, int64_t timeout_us,
+                    double *latency_ms, uint64_t *nrecords);
+/* Copies a finished request's results and releases them: counts[2*i + s]
+ * as fmi_search1_batch gives, and the stored records of every (read,
+ * strand) slot, slot after slot, densely (nrecords of them). */
+int fmi_server_fetch(fmi_server *srv, uint64_t request_id, uint32_t *counts, fmi_hit *records);
+int fmi_server_get_stats(fmi_server *srv, fmi_server_stats *stats);
+/* Finishes queued and in-flight work, then frees everything. */
+int fmi_server_destroy(fmi_server *srv);
 
 /* ------------------------------------------------------ measurement */
 /* Random gather of line_bytes-sized aligned lines over d_buf (bytes >= 64
