@@ -38,13 +38,24 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
+# Genome / read models.  "c2" is the configuration the metric is quoted on
+# (64 Mbp, 20% repeats at 2-15% divergence, 1M x 150 bp); "t3g" is the
+# 3.1 Gbp target point (45% repeats at 1-20% divergence, 0.1% N runs).
+PRESETS = {
+    "c2": dict(genome_mbp=64.0, repeat_frac=0.2, div=(0.02, 0.15), n_frac=0.0, reads=1_000_000),
+    "t3g": dict(genome_mbp=3100.0, repeat_frac=0.45, div=(0.01, 0.20), n_frac=0.001,
+                reads=2_000_000),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--genome-mbp", type=float, default=64.0)
-    ap.add_argument("--reads", type=int, default=1_000_000)
+    ap.add_argument("--preset", choices=sorted(PRESETS), default="c2")
+    ap.add_argument("--genome-mbp", type=float, default=None)
+    ap.add_argument("--reads", type=int, default=None)
     ap.add_argument("--read-len", type=int, default=150)
     ap.add_argument("--sub-rate", type=float, default=0.01)
     ap.add_argument("--sa-ratio", type=int, default=16)
@@ -73,15 +84,20 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    pre = PRESETS[args.preset]
+    if args.genome_mbp is None:
+        args.genome_mbp = pre["genome_mbp"]
+    if args.reads is None:
+        args.reads = pre["reads"]
     n_bp = int(args.genome_mbp * 1_000_000)
     t0 = time.time()
-    g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=0.2, fam_len=(300, 6000),
-                           divergence=(0.02, 0.15))
+    g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=pre["repeat_frac"],
+                           fam_len=(300, 6000), divergence=pre["div"], n_frac=pre["n_frac"])
     log(f"genome {n_bp} bp in {time.time() - t0:.1f}s")
     t0 = time.time()
-    ix = hpgfm.Index.build(g, args.sa_ratio)
-    log(f"index built in {time.time() - t0:.1f}s ({ix.device_bytes / 1e6:.1f} MB on device)")
-    ix.to_device()
+    ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
+    log(f"index built on the GPU in {time.time() - t0:.1f}s "
+        f"({ix.device_bytes / 1e6:.1f} MB on device)")
     t0 = time.time()
     reads, offs, _ = synth.reads(g, args.reads, length=args.read_len, seed=2,
                                  sub_rate=args.sub_rate, both_strands=True)
@@ -130,6 +146,14 @@ def main():
         ms, done = hpgfm.gather_bench(8 << 30, 64, 1 << 28)
         extras["random_gather_64B_GBps"] = round(done * 64 / (ms / 1000.0) / 1e9, 1)
         log(f"random 64-byte gather over 8 GiB: {extras['random_gather_64B_GBps']} GB/s")
+        # the same probe over a buffer the size of this index's blocks + samples
+        ibytes = max(64 << 20, ix.device_bytes)
+        ms2, done2 = hpgfm.gather_bench(ibytes, 64, 1 << 28)
+        extras["random_gather_64B_index_size_GBps"] = round(done2 * 64 / (ms2 / 1000.0) / 1e9, 1)
+        extras["frac_of_measured_random_access"] = round(
+            achieved / extras["random_gather_64B_index_size_GBps"], 3)
+        log(f"random 64-byte gather over {ibytes / 2**30:.2f} GiB (index size): "
+            f"{extras['random_gather_64B_index_size_GBps']} GB/s")
         hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
         hpgfm.sync()
         with hpgfm.TimingRegion() as tp:
@@ -225,10 +249,13 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"1-mismatch search (BWSearch1CPU mismatch projection), "
-                         f"{args.genome_mbp:g} Mbp synthetic reference (41% GC, 20% repeat "
-                         f"families 300 bp-6 kbp at 2-15% divergence), {args.reads} reads x "
-                         f"{args.read_len} bp, both strands, {args.sub_rate * 100:g}% "
-                         f"substitutions"),
+                         f"{args.genome_mbp:g} Mbp synthetic reference (41% GC, "
+                         f"{pre['repeat_frac'] * 100:g}% repeat families 300 bp-6 kbp at "
+                         f"{pre['div'][0] * 100:g}-{pre['div'][1] * 100:g}% divergence"
+                         + (f", {pre['n_frac'] * 100:g}% N runs" if pre['n_frac'] else "")
+                         + f"), {args.reads} reads x {args.read_len} bp, both strands, "
+                         f"{args.sub_rate * 100:g}% substitutions"),
+            "preset": args.preset,
             "genome_bp": n_bp, "reads": args.reads, "read_len": args.read_len,
             "mode": "mismatch", "cap": args.cap, "sa_ratio": args.sa_ratio,
             "parallelism": "single GPU, persistent work-queue launch",

@@ -213,6 +213,18 @@ int fmi_index_build(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_inde
   return FMI_OK;
 }
 
+int fmi_index_build_gpu(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_index **out) {
+  if (!out) return fail(FMI_EINVAL, "out is NULL");
+  *out = nullptr;
+  fmi_index *ix = new (std::nothrow) fmi_index();
+  if (!ix) return fail(FMI_ENOMEM, "out of host memory");
+  std::string err;
+  int rc = fmi::build_index_gpu(text, n, sa_ratio, ix, &err);
+  if (rc) { delete ix; return fail(rc, err); }
+  *out = ix;
+  return FMI_OK;
+}
+
 int fmi_index_save(const fmi_index *ix, const char *path) {
   if (!ix || !path) return fail(FMI_EINVAL, "NULL argument");
   std::string err;

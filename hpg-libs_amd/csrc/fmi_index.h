@@ -32,6 +32,9 @@ namespace fmi {
 // Builds both directions from codes 0..3; returns an fmi_status.
 int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix,
                 std::string *err);
+// The same on the current GPU; leaves the index on the device.
+int build_index_gpu(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix,
+                    std::string *err);
 int save_index(const fmi_index *ix, const char *path, std::string *err);
 int load_index(const char *path, fmi_index *ix, std::string *err);
 // Symbol of BWT row r (0xFF for the '$' row).

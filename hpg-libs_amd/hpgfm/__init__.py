@@ -31,7 +31,7 @@ MAX_READ_LEN = 1000
 
 # Every symbol include/fmi.h declares; tests check the library exports them.
 EXPORTS = [
-    "fmi_status_str", "fmi_last_error", "fmi_index_build", "fmi_index_save",
+    "fmi_status_str", "fmi_last_error", "fmi_index_build", "fmi_index_build_gpu", "fmi_index_save",
     "fmi_index_load", "fmi_index_free", "fmi_index_size_sa", "fmi_index_sa_ratio",
     "fmi_index_device_bytes", "fmi_index_to_device", "fmi_index_get_bwt",
     "fmi_index_set_karyotype", "fmi_device_count", "fmi_device_alloc", "fmi_device_free",
@@ -75,6 +75,7 @@ def _bind(lib):
         "fmi_status_str": ([_i], C.c_char_p),
         "fmi_last_error": ([], C.c_char_p),
         "fmi_index_build": ([_p, _u64, _u32, C.POINTER(_p)], _i),
+        "fmi_index_build_gpu": ([_p, _u64, _u32, C.POINTER(_p)], _i),
         "fmi_index_save": ([_p, C.c_char_p], _i),
         "fmi_index_load": ([C.c_char_p, C.POINTER(_p)], _i),
         "fmi_index_free": ([_p], None),
@@ -263,6 +264,14 @@ class Index:
         text = np.ascontiguousarray(text, dtype=np.uint8)
         h = C.c_void_p()
         check(lib().fmi_index_build(_ptr(text), len(text), sa_ratio, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def build_gpu(cls, text, sa_ratio=16):
+        """Builds on the current GPU; the index is left on the device."""
+        text = np.ascontiguousarray(text, dtype=np.uint8)
+        h = C.c_void_p()
+        check(lib().fmi_index_build_gpu(_ptr(text), len(text), sa_ratio, C.byref(h)))
         return cls(h)
 
     @classmethod

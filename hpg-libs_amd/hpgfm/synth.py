@@ -16,10 +16,29 @@ import numpy as np
 A, Cb, G, T, N = 0, 1, 2, 3, 4
 
 
-def iid_bases(rng, n, gc=0.41):
+def iid_bases(rng, n, gc=0.41, chunk=1 << 26):
+    """n i.i.d. bases with P(C) = P(G) = gc/2, generated in chunks so that
+    multi-gigabase genomes need no float temporaries of their own size."""
     at = (1.0 - gc) / 2.0
-    cg = gc / 2.0
-    return rng.choice(4, size=n, p=[at, cg, cg, at]).astype(np.uint8)
+    cut = np.array([at, 0.5, 0.5 + gc / 2.0], dtype=np.float64)   # A | C | G | T
+    out = np.empty(n, dtype=np.uint8)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        out[a:b] = np.searchsorted(cut, rng.random(b - a), side="right").astype(np.uint8)
+    return out
+
+
+def replace_n(codes, seed=0):
+    """Index policy for N (code 4) in a reference: each N becomes a
+    pseudo-random base drawn from a generator seeded with `seed`, so the
+    indexed text is over ACGT only and the choice is reproducible."""
+    codes = np.asarray(codes, dtype=np.uint8)
+    m = codes > 3
+    if not m.any():
+        return codes
+    out = codes.copy()
+    out[m] = np.random.default_rng(seed).integers(0, 4, int(m.sum()), dtype=np.uint8)
+    return out
 
 
 def genome(n, seed=1, gc=0.41, repeat_frac=0.0, fam_len=(300, 6000),
@@ -55,8 +74,10 @@ def genome(n, seed=1, gc=0.41, repeat_frac=0.0, fam_len=(300, 6000),
             g[p:p + L] = N
             covered += L
     if contigs > 1:
-        cuts = np.sort(rng.choice(np.arange(1000, n - 1000), size=contigs - 1, replace=False))
-        starts = np.concatenate([[0], cuts]).astype(np.uint64)
+        cuts = set()
+        while len(cuts) < contigs - 1:
+            cuts.add(int(rng.integers(1000, n - 1000)))
+        starts = np.concatenate([[0], np.sort(np.array(sorted(cuts), dtype=np.int64))]).astype(np.uint64)
     else:
         starts = np.zeros(1, np.uint64)
     names = [f"chr{i + 1}" for i in range(len(starts))]

@@ -94,6 +94,11 @@ const char *fmi_last_error(void);
  * bwt_generate_index_files (bwt.c:744-819; calculate_C/O/S_and_R,
  * search/preprocess.c:7-130) for an in-memory text.  n < 2^32 - 2. */
 int fmi_index_build(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_index **out);
+/* The same index, built on the current GPU (suffix sorting by prefix
+ * doubling, blocks and samples written by kernels) and left on that device
+ * (no fmi_index_to_device needed); host copies are kept for saving.  Needs
+ * about 30 bytes of free device memory per base (3.1 Gbp: ~95 GB). */
+int fmi_index_build_gpu(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_index **out);
 /* Writes / reads the single-file index (see DESIGN: versioned header,
  * occurrence blocks, samples, karyotype).  Replaces the reference's index
  * directory reader load_bwt_index (search/runtime.h:90-124). */
