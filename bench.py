@@ -48,6 +48,54 @@ PRESETS = {
 }
 
 
+def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000):
+    """Exact seed mapping: every read cut into seed_len-mers at `stride`, both
+    orientations, up to `filt` SA positions resolved per seed (the
+    reference's per-strand skip), located and resolved to regions."""
+    read_len = 250 if args.read_len == 150 else args.read_len
+    t0 = time.time()
+    reads, offs, _ = synth.reads(g, args.reads, length=read_len, seed=3, sub_rate=args.sub_rate)
+    batch = hpgfm.ReadBatch(reads, offs)
+    log(f"{args.reads} reads of {read_len} bp resident in {time.time() - t0:.1f}s")
+    ns, nr, _, _ = hpgfm.seed_map_batch(ix, batch, seed_len, stride, filt, region_cap=0,
+                                        with_info=False)
+    regs = hpgfm.DeviceBuffer(max(nr, 1) * 32)
+    L = hpgfm.lib()
+    import ctypes as C
+    a, b = C.c_uint64(), C.c_uint64()
+
+    def step():
+        hpgfm.check(L.fmi_seed_map_batch(ix.h, batch.d_reads.ptr, batch.d_offsets.ptr,
+                                         batch.nreads, seed_len, stride, filt, None, 0, regs.ptr,
+                                         nr, C.byref(a), C.byref(b), None))
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    wall = (time.perf_counter() - t0) / args.steps
+    out = regs.download(hpgfm.SEED_REGION_DTYPE, nr)
+    valid = int(out["valid"].sum())
+    log(f"seeds: {ns} seeds, {nr} rows ({valid} regions) per batch in {wall * 1000:.2f} ms")
+    res = {
+        "metric": "seeds/s, reads/s, located positions/s (exact seed mapping, bwt_map_exact_seed)",
+        "value": round(ns / wall), "unit": "seeds/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1000, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"seed mapping, {args.genome_mbp:g} Mbp synthetic reference "
+                               f"({pre['repeat_frac'] * 100:g}% repeats), {args.reads} reads x "
+                               f"{read_len} bp cut into {seed_len}-mers at stride {stride}, both "
+                               f"orientations, <= {filt} positions per seed",
+                   "preset": args.preset, "seed_len": seed_len, "stride": stride,
+                   "filter_seed_mappings": filt},
+        "reads_per_s": round(args.reads / wall), "positions_per_s": round(nr / wall),
+        "seeds": ns, "rows": nr, "regions": valid,
+    }
+    regs.free()
+    batch.free()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +117,9 @@ def main():
                     help="keep the library's default launch shape")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of each serving run (0 skips serving)")
+    ap.add_argument("--workload", choices=["search1", "seeds"], default="search1",
+                    help="search1: the 1-mismatch search (the metric); seeds: exact seed mapping "
+                         "(18-mers at stride 5, <=1000 positions per seed)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -98,6 +149,10 @@ def main():
     ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
     log(f"index built on the GPU in {time.time() - t0:.1f}s "
         f"({ix.device_bytes / 1e6:.1f} MB on device)")
+    if args.workload == "seeds":
+        bench_seeds(args, pre, g, ix, hpgfm, synth)
+        return
+
     t0 = time.time()
     reads, offs, _ = synth.reads(g, args.reads, length=args.read_len, seed=2,
                                  sub_rate=args.sub_rate, both_strands=True)

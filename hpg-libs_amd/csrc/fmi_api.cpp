@@ -543,6 +543,29 @@ int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t 
   return end_timed(st, e, "resolve launch");
 }
 
+int fmi_seed_map_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64_t *d_offsets,
+                       uint32_t nreads, uint32_t seed_len, uint32_t stride,
+                       uint32_t filter_seed_mappings, fmi_seed_info *d_seed_info,
+                       uint64_t seed_info_cap, fmi_seed_region *d_regions, uint64_t region_cap,
+                       uint64_t *nseeds, uint64_t *nrows, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (!nseeds || !nrows) return fail(FMI_EINVAL, "NULL argument");
+  *nseeds = 0;
+  *nrows = 0;
+  if (seed_len == 0 || seed_len > FMI_MAX_READ_LEN || stride == 0)
+    return fail(FMI_EINVAL, "seed_len must be 1..1000 and stride >= 1");
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets) return fail(FMI_EINVAL, "NULL buffer");
+  if (region_cap && !d_regions) return fail(FMI_EINVAL, "d_regions is NULL with region_cap > 0");
+  if (seed_info_cap && !d_seed_info) return fail(FMI_EINVAL, "d_seed_info is NULL with a capacity");
+  std::string err;
+  rc = fmi::seed_map(ix, d_reads, d_offsets, nreads, seed_len, stride, filter_seed_mappings,
+                     seed_info_cap ? d_seed_info : nullptr, seed_info_cap, d_regions, region_cap,
+                     nseeds, nrows, s, &err);
+  return rc ? fail(rc, err) : FMI_OK;
+}
+
 int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uint64_t nlines,
                      uint64_t *d_sink, float *ms, uint64_t *lines_done, fmi_stream s) {
   if (!d_buf || !d_sink || !ms || !lines_done) return fail(FMI_EINVAL, "NULL argument");

@@ -1,0 +1,168 @@
+// fmi_device.h -- device-side building blocks shared by the HIP kernels:
+// occurrence-block loads, ranks, the LF step, the locate walk, record stores.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fmi.h"
+#include "fmi_internal.h"
+
+namespace fmi {
+
+// --------------------------------------------------------------------------
+// occurrence block access
+// --------------------------------------------------------------------------
+
+struct Blk {
+  uint32_t c[4];
+  uint64_t lo[3], hi[3];   // bit-planes: bit t%64 of word t/64 = code bit of row t
+};
+
+__device__ __forceinline__ Blk load_block(const OccBlock *base, uint32_t j) {
+  const uint4 *q = reinterpret_cast<const uint4 *>(base + j);
+  const uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+  Blk B;
+  B.c[0] = a.x; B.c[1] = a.y; B.c[2] = a.z; B.c[3] = a.w;
+  B.lo[0] = ((uint64_t)b.y << 32) | b.x;
+  B.lo[1] = ((uint64_t)b.w << 32) | b.z;
+  B.lo[2] = ((uint64_t)c.y << 32) | c.x;
+  B.hi[0] = ((uint64_t)c.w << 32) | c.z;
+  B.hi[1] = ((uint64_t)d.y << 32) | d.x;
+  B.hi[2] = ((uint64_t)d.w << 32) | d.z;
+  return B;
+}
+
+// Masks selecting the first t (0..192) rows of the block in each plane word.
+__device__ __forceinline__ void row_masks(uint32_t t, uint64_t &m0, uint64_t &m1, uint64_t &m2) {
+  const uint32_t r = t & 63u, q = t >> 6;
+  const uint64_t pm = r ? (~0ull >> (64u - r)) : 0ull;
+  m0 = q >= 1u ? ~0ull : pm;
+  m1 = q >= 2u ? ~0ull : (q == 1u ? pm : 0ull);
+  m2 = q >= 3u ? ~0ull : (q == 2u ? pm : 0ull);
+}
+
+// Rows among the first t of the block whose code is s (the '$' row holds 0).
+__device__ __forceinline__ uint32_t rank1(const Blk &B, uint32_t s, uint32_t t) {
+  uint64_t m0, m1, m2;
+  row_masks(t, m0, m1, m2);
+  const uint64_t xl = (s & 1u) ? 0ull : ~0ull, xh = (s & 2u) ? 0ull : ~0ull;
+  return (uint32_t)(__popcll((B.lo[0] ^ xl) & (B.hi[0] ^ xh) & m0) +
+                    __popcll((B.lo[1] ^ xl) & (B.hi[1] ^ xh) & m1) +
+                    __popcll((B.lo[2] ^ xl) & (B.hi[2] ^ xh) & m2));
+}
+
+// The same for all four codes at once: three popcounts per plane word.
+__device__ __forceinline__ void rank4(const Blk &B, uint32_t t, uint32_t (&r)[4]) {
+  uint64_t m0, m1, m2;
+  row_masks(t, m0, m1, m2);
+  const uint64_t a0 = B.lo[0] & m0, a1 = B.lo[1] & m1, a2 = B.lo[2] & m2;
+  const uint64_t b0 = B.hi[0] & m0, b1 = B.hi[1] & m1, b2 = B.hi[2] & m2;
+  const uint32_t nl = (uint32_t)(__popcll(a0) + __popcll(a1) + __popcll(a2));
+  const uint32_t nh = (uint32_t)(__popcll(b0) + __popcll(b1) + __popcll(b2));
+  const uint32_t nb = (uint32_t)(__popcll(a0 & b0) + __popcll(a1 & b1) + __popcll(a2 & b2));
+  r[3] = nb;
+  r[2] = nh - nb;
+  r[1] = nl - nb;
+  r[0] = t - nl - nh + nb;
+}
+
+// a[i] for a runtime i in 0..3 without dynamic register indexing.
+__device__ __forceinline__ uint32_t pick4(const uint32_t (&a)[4], uint32_t i) {
+  const uint32_t lo = (i & 1u) ? a[1] : a[0];
+  const uint32_t hi = (i & 1u) ? a[3] : a[2];
+  return (i & 2u) ? hi : lo;
+}
+
+__device__ __forceinline__ uint64_t pick3(const uint64_t (&a)[3], uint32_t i) {
+  return i == 0u ? a[0] : (i == 1u ? a[1] : a[2]);
+}
+
+// Code stored at row t of the block.
+__device__ __forceinline__ uint32_t code_at(const Blk &B, uint32_t t) {
+  const uint32_t w = t >> 6, b = t & 63u;
+  return (uint32_t)((pick3(B.lo, w) >> b) & 1ull) | ((uint32_t)((pick3(B.hi, w) >> b) & 1ull) << 1);
+}
+
+// Occ(s, m) for m in block j (t = m - 192*j): checkpoint + in-block rank,
+// minus the '$' row (stored as code 0) when it lies before m in the block.
+__device__ __forceinline__ uint32_t occ_blk(const Blk &B, uint32_t dollar_blk,
+                                            uint32_t dollar_off, uint32_t j, uint32_t s,
+                                            uint32_t t) {
+  uint32_t v = pick4(B.c, s) + rank1(B, s, t);
+  if (s == 0 && j == dollar_blk && dollar_off < t) v -= 1;
+  return v;
+}
+
+// Symbol of row k (block j, offset t), 4 for the '$' row.
+__device__ __forceinline__ uint32_t sym_at(const Blk &B, uint32_t dollar_blk, uint32_t dollar_off,
+                                           uint32_t j, uint32_t t) {
+  return (j == dollar_blk && t == dollar_off) ? 4u : code_at(B, t);
+}
+
+// LF step of one physical symbol s on [k, l] (BWiteration, runtime.h:38-42).
+// A size-1 interval needs only row k's block: Occ(s, k+1) = Occ(s, k) +
+// [BWT[k] == s].
+__device__ __forceinline__ void lf_one(const DevDir &D, uint32_t k, uint32_t l, uint32_t s,
+                                       uint32_t &ko, uint32_t &lo, uint32_t &nblk) {
+  const uint32_t m2 = l + 1;
+  const uint32_t j1 = k / kBlockRows, j2 = m2 / kBlockRows;
+  const uint32_t t1 = k - j1 * kBlockRows;
+  const Blk B1 = load_block(D.blk, j1);
+  const uint32_t c = pick4(D.C, s);
+  ko = c + 1u + occ_blk(B1, D.dollar_blk, D.dollar_off, j1, s, t1);
+  if (k == l) {
+    nblk = 1u;
+    lo = ko - 1u + (sym_at(B1, D.dollar_blk, D.dollar_off, j1, t1) == s ? 1u : 0u);
+    return;
+  }
+  Blk B2;
+  if (j2 != j1) B2 = load_block(D.blk, j2);
+  else B2 = B1;
+  nblk = (j2 != j1) ? 2u : 1u;
+  lo = c + occ_blk(B2, D.dollar_blk, D.dollar_off, j2, s, m2 - j2 * kBlockRows);
+}
+
+__device__ __forceinline__ const DevDir &dir_of(const DevIndex &ix, uint32_t d) {
+  return d ? ix.d[1] : ix.d[0];
+}
+
+// --------------------------------------------------------------------------
+// records and locate
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ void store_hit(fmi_hit *dst, int64_t k, int64_t l, int16_t start,
+                                          int16_t pos, int16_t end, int16_t err_pos, uint8_t dir,
+                                          uint8_t nmm, uint8_t kind, uint8_t base, uint8_t strand) {
+  uint4 a, b;
+  a.x = (uint32_t)(uint64_t)k; a.y = (uint32_t)((uint64_t)k >> 32);
+  a.z = (uint32_t)(uint64_t)l; a.w = (uint32_t)((uint64_t)l >> 32);
+  b.x = (uint32_t)(uint16_t)start | ((uint32_t)(uint16_t)pos << 16);
+  b.y = (uint32_t)(uint16_t)end | ((uint32_t)(uint16_t)err_pos << 16);
+  b.z = (uint32_t)dir | ((uint32_t)nmm << 8) | ((uint32_t)kind << 16) | ((uint32_t)base << 24);
+  b.w = (uint32_t)strand;
+  uint4 *d = reinterpret_cast<uint4 *>(dst);
+  d[0] = a;
+  d[1] = b;
+}
+
+__device__ __forceinline__ uint32_t sa_value(const DevDir &D, uint32_t ratio, uint32_t n,
+                                             uint32_t row) {
+  uint32_t i = row, steps = 0;
+  while (i % ratio) {
+    if (i == D.dollar) {
+      i = 0;
+    } else {
+      const uint32_t j = i / kBlockRows, t = i - j * kBlockRows;
+      const Blk B = load_block(D.blk, j);
+      const uint32_t b = code_at(B, t);
+      i = pick4(D.C, b) + occ_blk(B, D.dollar_blk, D.dollar_off, j, b, t) + 1u;
+    }
+    ++steps;
+  }
+  const uint64_t v = (uint64_t)D.S[i / ratio] + steps;
+  return (uint32_t)(v % ((uint64_t)n + 1));
+}
+
+}  // namespace fmi

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/fmi.h"
 #include "fmi_internal.h"
 
 struct fmi_index {
@@ -35,6 +36,11 @@ int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix,
 // The same on the current GPU; leaves the index on the device.
 int build_index_gpu(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix,
                     std::string *err);
+// fmi_seed_map_batch (fmi_seed.hip); s is a hipStream_t.
+int seed_map(const fmi_index *ix, const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
+             uint32_t seed_len, uint32_t stride, uint32_t filter, fmi_seed_info *info,
+             uint64_t info_cap, fmi_seed_region *regions, uint64_t region_cap, uint64_t *nseeds,
+             uint64_t *nrows, void *s, std::string *err);
 int save_index(const fmi_index *ix, const char *path, std::string *err);
 int load_index(const char *path, fmi_index *ix, std::string *err);
 // Symbol of BWT row r (0xFF for the '$' row).

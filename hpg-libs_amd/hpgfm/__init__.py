@@ -40,6 +40,7 @@ EXPORTS = [
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
     "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
+    "fmi_seed_map_batch",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
     "fmi_server_get_stats", "fmi_server_destroy",
 ]
@@ -110,6 +111,8 @@ def _bind(lib):
                               _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
                                       C.POINTER(_u64)], _i),
+        "fmi_seed_map_batch": ([_p, _p, _p, _u32, _u32, _u32, _u32, _p, _u64, _p, _u64,
+                                C.POINTER(_u64), C.POINTER(_u64), _p], _i),
         "fmi_server_create": ([_p, C.POINTER(ServerConfig), C.POINTER(_p)], _i),
         "fmi_server_submit": ([_p, _p, _p, _u32, C.POINTER(_u64)], _i),
         "fmi_server_wait": ([_p, _u64, C.c_int64, C.POINTER(C.c_double), C.POINTER(_u64)], _i),
@@ -450,6 +453,53 @@ def resolve_hits(ix, batch, out, row_cap, stream=None):
     shape = (n, 2, cap, row_cap)
     return (dc.download(np.int32, 2 * n * cap * row_cap, stream).reshape(shape),
             dd.download(np.uint64, 2 * n * cap * row_cap, stream).reshape(shape))
+
+
+SEED_REGION_DTYPE = np.dtype([
+    ("start", "<u8"), ("end", "<u8"), ("chrom_idx", "<u4"), ("read", "<u4"),
+    ("seq_start", "<u2"), ("seq_end", "<u2"), ("seed_id", "<u2"), ("strand", "u1"),
+    ("valid", "u1"),
+])
+SEED_INFO_DTYPE = np.dtype([
+    ("k_plus", "<i8"), ("l_plus", "<i8"), ("k_minus", "<i8"), ("l_minus", "<i8"),
+    ("rows_plus", "<u4"), ("rows_minus", "<u4"),
+])
+assert SEED_REGION_DTYPE.itemsize == 32 and SEED_INFO_DTYPE.itemsize == 40
+
+
+def seed_map_batch(ix, batch, seed_len, stride, filter_seed_mappings=1000, region_cap=None,
+                   with_info=True, stream=None):
+    """fmi_seed_map_batch; returns (nseeds, nrows, regions, infos) on the host.
+    region_cap defaults to enough for every resolved row (a first call
+    counts them)."""
+    L = lib()
+    ns, nr = C.c_uint64(), C.c_uint64()
+    if region_cap is None:
+        check(L.fmi_seed_map_batch(ix.h, batch.d_reads.ptr, batch.d_offsets.ptr, batch.nreads,
+                                   seed_len, stride, filter_seed_mappings, None, 0, None, 0,
+                                   C.byref(ns), C.byref(nr), stream))
+        region_cap = int(nr.value)
+        info_cap = int(ns.value)
+    else:
+        info_cap = 0
+        if with_info:
+            check(L.fmi_seed_map_batch(ix.h, batch.d_reads.ptr, batch.d_offsets.ptr, batch.nreads,
+                                       seed_len, stride, filter_seed_mappings, None, 0, None, 0,
+                                       C.byref(ns), C.byref(nr), stream))
+            info_cap = int(ns.value)
+    info = DeviceBuffer(max(info_cap, 1) * 40) if with_info else None
+    regs = DeviceBuffer(max(region_cap, 1) * 32)
+    check(L.fmi_seed_map_batch(ix.h, batch.d_reads.ptr, batch.d_offsets.ptr, batch.nreads,
+                               seed_len, stride, filter_seed_mappings,
+                               info.ptr if info else None, info_cap if info else 0, regs.ptr,
+                               region_cap, C.byref(ns), C.byref(nr), stream))
+    nrows = int(nr.value)
+    out = regs.download(SEED_REGION_DTYPE, min(nrows, region_cap), stream)
+    inf = info.download(SEED_INFO_DTYPE, int(ns.value), stream) if info else None
+    regs.free()
+    if info:
+        info.free()
+    return int(ns.value), nrows, out, inf
 
 
 class Server:

@@ -225,6 +225,46 @@ int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t 
                      const uint64_t *d_offsets, uint32_t nreads, uint32_t cap,
                      uint32_t row_cap, int32_t *d_chrom, uint64_t *d_coord, fmi_stream s);
 
+/* ------------------------------------------------------ seed mapping */
+/* One region of bwt_map_exact_seed (bwt.c:1200-1294): an occurrence of a
+ * seed on one strand that lies inside one contig. 32 bytes. */
+typedef struct fmi_seed_region {
+  uint64_t start;      /* start_mapping: start[idx-1] + key - offset[idx-1] + 1 (bwt.c:1271-1277) */
+  uint64_t end;        /* start + len, len = seq_end - seq_start (bwt.c:1214, one short) */
+  uint32_t chrom_idx;  /* idx as region_bwt_new receives it: binsearch result (contig + 1) */
+  uint32_t read;       /* read index in the batch */
+  uint16_t seq_start;  /* aux_seq_start: on the read (+) or on its reverse complement (-) */
+  uint16_t seq_end;    /* aux_seq_end */
+  uint16_t seed_id;    /* index of the seed in its read's tiling */
+  uint8_t strand;      /* !type: 0 for +, 1 for - */
+  uint8_t valid;       /* 0: the row's occurrence crosses a contig end (no region) */
+} fmi_seed_region;
+
+/* Per-seed summary: the exact intervals of both strands (+: backward search
+ * on BACKWARD; -: forward search on BACKWARD_REV, bwt.c:1222-1238) and the
+ * rows resolved for each after the cumulative filter (a strand is skipped
+ * once the running count l - k + 1, summed over + then -, exceeds
+ * filter_seed_mappings, bwt.c:1250-1269). 40 bytes. */
+typedef struct fmi_seed_info {
+  int64_t k_plus, l_plus, k_minus, l_minus;
+  uint32_t rows_plus, rows_minus;
+} fmi_seed_info;
+
+/* bwt_map_exact_seed for every seed of every read: read i is tiled with
+ * seeds [s, s + seed_len - 1] for s = 0, stride, 2*stride, ... while s +
+ * seed_len <= len.  Seeds are numbered read after read, seed after seed;
+ * d_seed_info (nseeds entries) is optional.  Every resolved SA row becomes
+ * one entry of d_regions, in seed order, + strand rows then - strand rows,
+ * each strand in row order (the reference's insertion order); entries past
+ * region_cap are counted but not written.  *nseeds and *nrows are written
+ * on the host: the call synchronises the stream (twice, for the seed and
+ * row totals) and allocates its scratch (about 24 bytes per seed). */
+int fmi_seed_map_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64_t *d_offsets,
+                       uint32_t nreads, uint32_t seed_len, uint32_t stride,
+                       uint32_t filter_seed_mappings, fmi_seed_info *d_seed_info,
+                       uint64_t seed_info_cap, fmi_seed_region *d_regions, uint64_t region_cap,
+                       uint64_t *nseeds, uint64_t *nrows, fmi_stream s);
+
 /* ----------------------------------------------------------- serving */
 /* A request loop over a device-resident index (no counterpart in the
  * reference, whose closest analog is the bounded work queue of

@@ -459,6 +459,56 @@ uint32_t fmo_search1(const fmo_index *ix, int strand, int mismatch_only,
   return s.cnt;
 }
 
+uint64_t fmo_seed_map(const fmo_index *ix, const uint8_t *W, uint32_t len, uint32_t seed_len,
+                      uint32_t stride, int32_t filter, const uint64_t *koff,
+                      const uint64_t *kstart, uint64_t kcount, uint32_t read,
+                      fmo_seed_region *out, uint64_t cap, int64_t *info) {
+  /* bwt_map_exact_seed, bwt.c:1200-1294, for each seed of the tiling */
+  uint64_t nrow = 0;
+  if (len < seed_len) return 0;
+  const uint32_t nseeds = (len - seed_len) / stride + 1;
+  const int64_t lsa = (int64_t)fmo_size_sa(ix) - 1;
+  for (uint32_t q = 0; q < nseeds; q++) {
+    const uint64_t seq_start = (uint64_t)q * stride, seq_end = seq_start + seed_len - 1;
+    const uint64_t slen = seq_end - seq_start;                   /* one short, bwt.c:1214 */
+    const uint8_t *code = W + seq_start;
+    int32_t actual = 0;                                         /* int actual_mappings */
+    for (int type = 1; type >= 0; type--) {
+      fmo_hit r;
+      memset(&r, 0, sizeof(r));
+      r.k = 0; r.l = lsa; r.start = 0; r.end = (int16_t)(seed_len - 1);
+      if (type == 1) { r.pos = r.end; fmo_exact(ix, FMO_VIEW_BACKWARD, 0, code, &r, NULL); }
+      else { r.pos = 0; fmo_exact(ix, FMO_VIEW_BACKWARD_REV, 1, code, &r, NULL); }
+      actual = (int32_t)((uint32_t)actual + (uint32_t)(r.l - r.k + 1));
+      const int skip = actual > filter;
+      int64_t rows = (!skip && r.k <= r.l) ? r.l - r.k + 1 : 0;
+      if (info) {
+        info[6 * q + (type ? 0 : 2)] = r.k;
+        info[6 * q + (type ? 1 : 3)] = r.l;
+        info[6 * q + (type ? 4 : 5)] = rows;
+      }
+      for (int64_t j = 0; j < rows; j++) {
+        const uint64_t key = fmo_get_sa(ix, 0, (uint64_t)(r.k + j));
+        const uint64_t idx = fmo_binsearch(koff, kcount, key);
+        fmo_seed_region g;
+        memset(&g, 0, sizeof(g));
+        g.read = read; g.seed_id = (uint16_t)q; g.strand = (uint8_t)(!type);
+        g.seq_start = (uint16_t)(type ? seq_start : len - seq_end - 1);
+        g.seq_end = (uint16_t)(type ? seq_end : len - seq_start - 1);
+        g.chrom_idx = (uint32_t)idx;
+        if (idx >= 1 && key + slen <= koff[idx]) {
+          g.start = kstart[idx - 1] + (key - koff[idx - 1]) + 1;
+          g.end = g.start + slen;
+          g.valid = 1;
+        }
+        if (nrow < cap) out[nrow] = g;
+        nrow++;
+      }
+    }
+  }
+  return nrow;
+}
+
 uint64_t fmo_binsearch(const uint64_t *array, uint64_t size, uint64_t key) {
   /* binsearch, search/io.h:43-64: first index whose value is > key. */
   if (!array) return 0;

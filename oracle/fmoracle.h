@@ -128,6 +128,27 @@ uint32_t fmo_search1(const fmo_index *ix, int strand, int mismatch_only,
                      const uint8_t *W, int16_t len, fmo_hit *out, uint32_t cap,
                      fmo_hit *anchors);
 
+/* ---- seed mapping ------------------------------------------------------ */
+/* One region of bwt_map_exact_seed (same layout as fmi_seed_region). */
+typedef struct {
+  uint64_t start, end;
+  uint32_t chrom_idx, read;
+  uint16_t seq_start, seq_end, seed_id;
+  uint8_t strand, valid;
+} fmo_seed_region;
+
+/* bwt_map_exact_seed (bwt.c:1200-1294) for seeds [s, s + seed_len - 1],
+ * s = 0, stride, ... of one read.  koff has kcount + 1 entries (contig text
+ * offsets, offset[c+1] = offset[c] + length), kstart kcount entries.  Every
+ * resolved row yields one region (valid = 0 when the contig-end check
+ * fails, where the reference makes none).  info (may be NULL) gets 6 int64
+ * per seed: k+, l+, k-, l-, rows+, rows-.  Returns the number of rows; at
+ * most cap regions are stored. */
+uint64_t fmo_seed_map(const fmo_index *ix, const uint8_t *W, uint32_t len, uint32_t seed_len,
+                      uint32_t stride, int32_t filter, const uint64_t *koff,
+                      const uint64_t *kstart, uint64_t kcount, uint32_t read,
+                      fmo_seed_region *out, uint64_t cap, int64_t *info);
+
 /* ---- position resolution ---------------------------------------------- */
 /* binsearch, search/io.h:43-64: upper_bound of key in array[0..size). */
 uint64_t fmo_binsearch(const uint64_t *array, uint64_t size, uint64_t key);

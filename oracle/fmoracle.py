@@ -24,6 +24,12 @@ HIT_DTYPE = np.dtype([
 ])
 assert HIT_DTYPE.itemsize == 32
 
+SEED_REGION_DTYPE = np.dtype([
+    ("start", "<u8"), ("end", "<u8"), ("chrom_idx", "<u4"), ("read", "<u4"),
+    ("seq_start", "<u2"), ("seq_end", "<u2"), ("seed_id", "<u2"), ("strand", "u1"),
+    ("valid", "u1"),
+])
+
 MATCH, DELETION, MISMATCH, INSERTION = 0, 1, 2, 3
 VIEW_BACKWARD, VIEW_FORWARD, VIEW_BACKWARD_REV, VIEW_FORWARD_REV = 0, 1, 2, 3
 
@@ -77,6 +83,9 @@ def _load_oracle():
     lib.fmo_exact_vector.restype = C.c_int16
     lib.fmo_search1.argtypes = [_p, C.c_int, C.c_int, _p, C.c_int16, _p, C.c_uint32, _p]
     lib.fmo_search1.restype = C.c_uint32
+    lib.fmo_seed_map.argtypes = [_p, _p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _p, _p,
+                                 C.c_uint64, C.c_uint32, _p, C.c_uint64, _p]
+    lib.fmo_seed_map.restype = C.c_uint64
     lib.fmo_binsearch.argtypes = [_p, C.c_uint64, C.c_uint64]
     lib.fmo_binsearch.restype = C.c_uint64
     lib.fmo_search1_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_int, C.c_uint32, _p, _p, C.c_int]
@@ -200,6 +209,21 @@ class Oracle:
                                  _ptr(out), cap, _ptr(an))
         res = out[:min(n, cap)].copy()
         return (int(n), res, an) if anchors else (int(n), res)
+
+    def seed_map(self, W, seed_len, stride, filter_seed_mappings, koff, kstart, read=0):
+        """bwt_map_exact_seed over the tiling of one read: (regions, info[nseeds, 6])."""
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        koff = np.ascontiguousarray(koff, dtype=np.uint64)
+        kstart = np.ascontiguousarray(kstart, dtype=np.uint64)
+        L = len(W)
+        ns = (L - seed_len) // stride + 1 if L >= seed_len else 0
+        info = np.zeros((max(ns, 1), 6), np.int64)
+        n = self.lib.fmo_seed_map(self.ref, _ptr(W), L, seed_len, stride, filter_seed_mappings,
+                                  _ptr(koff), _ptr(kstart), len(kstart), read, None, 0, _ptr(info))
+        out = np.zeros(max(int(n), 1), SEED_REGION_DTYPE)
+        self.lib.fmo_seed_map(self.ref, _ptr(W), L, seed_len, stride, filter_seed_mappings,
+                              _ptr(koff), _ptr(kstart), len(kstart), read, _ptr(out), n, None)
+        return out[:int(n)], info[:ns]
 
     def search1_batch(self, reads, offsets, cap=64, mismatch_only=False, threads=0,
                       keep_hits=True):
