@@ -96,6 +96,23 @@ def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000
     print(json.dumps(res), flush=True)
 
 
+# Measured HBM traffic of one timed launch, from the committed rocprofv3
+# FETCH_SIZE / WRITE_SIZE summaries (scaled by the copy-kernel calibration).
+TRAFFIC_FILES = {
+    ("c2", 1_000_000, 150): "search1_mismatch_64Mbp_1Mx150bp_pmc_traffic.json",
+    ("t3g", 2_000_000, 150): "search1_mismatch_3100Mbp_2Mx150bp_pmc_traffic.json",
+}
+
+
+def measured_traffic(preset, reads, read_len):
+    name = TRAFFIC_FILES.get((preset, reads, read_len))
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +134,14 @@ def main():
                     help="keep the library's default launch shape")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of each serving run (0 skips serving)")
+    ap.add_argument("--tuning", default=None,
+                    help="wg_per_cu,chunk,refill_min of the persistent launch (skips the "
+                         "autotuner)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run kernels of known traffic for the profiler's FETCH_SIZE / "
+                         "WRITE_SIZE counters: a 1 GiB streaming copy (fmi_copy_bench) and "
+                         "2^24-line random gathers of 64 and 128 bytes over a buffer the size "
+                         "of the index and over 8 GiB")
     ap.add_argument("--workload", choices=["search1", "seeds"], default="search1",
                     help="search1: the 1-mismatch search (the metric); seeds: exact seed mapping "
                          "(18-mers at stride 5, <=1000 positions per seed)")
@@ -166,7 +191,11 @@ def main():
     blocks, lf_steps = hpgfm.search1_count_blocks(ix, batch, mode=mode)
 
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        w, c, r = (int(x) for x in args.tuning.split(","))
+        hpgfm.set_tuning(w, c, r)
+        tuning = {"wg_per_cu": w, "chunk": c, "refill_min": r, "source": "--tuning"}
+    elif not args.no_autotune:
         from hpgfm.autotune import autotune
         best, best_ms, table = autotune(ix, batch, out, mode=mode, log=log)
         tuning = {"wg_per_cu": best[0], "chunk": best[1], "refill_min": best[2],
@@ -197,7 +226,18 @@ def main():
         f"{achieved:.0f} GB/s algorithmic")
 
     extras = {}
+    if args.calibrate:
+        cms = hpgfm.copy_bench(1 << 30, reps=1)
+        log(f"calibration: copy 1 GiB read + 1 GiB written in {cms:.3f} ms")
+        for nbytes in (max(64 << 20, ix.device_bytes), 8 << 30):
+            for lb in (64, 128):
+                gms, done = hpgfm.gather_bench(nbytes, lb, 1 << 24)
+                log(f"calibration: gather {done} lines of {lb} B over {nbytes} B "
+                    f"({done * lb} B read) in {gms:.3f} ms")
     if not args.no_extras:
+        cms = hpgfm.copy_bench(2 << 30, reps=3)
+        extras["streaming_copy_GBps"] = round(2 * (2 << 30) / (cms / 1000.0) / 1e9, 1)
+        log(f"streaming copy 2 GiB -> 2 GiB: {extras['streaming_copy_GBps']} GB/s (read + write)")
         ms, done = hpgfm.gather_bench(8 << 30, 64, 1 << 28)
         extras["random_gather_64B_GBps"] = round(done * 64 / (ms / 1000.0) / 1e9, 1)
         log(f"random 64-byte gather over 8 GiB: {extras['random_gather_64B_GBps']} GB/s")
@@ -317,7 +357,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": measured_traffic(args.preset, args.reads, args.read_len),
         },
         "cpu_baseline": cpu,
         "kernel": "fmi::search1_kernel<true, true, false>",

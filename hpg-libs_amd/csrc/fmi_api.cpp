@@ -526,6 +526,37 @@ int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k, const
   return end_timed(st, e, "locate launch");
 }
 
+int fmi_inverse_locate_batch(const fmi_index *ix, int forward, const uint64_t *d_text_pos,
+                             uint64_t count, uint64_t *d_rows, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (count == 0) return FMI_OK;
+  if (!d_text_pos || !d_rows) return fail(FMI_EINVAL, "NULL buffer");
+  if (count > (1ull << 36)) return fail(FMI_ERANGE, "count too large");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_isa(ix->dev, forward ? 1u : 0u, d_text_pos, count, d_rows, st);
+  return end_timed(st, e, "inverse locate launch");
+}
+
+int fmi_encode_bases(const char *src, uint64_t n, uint8_t *dst) {
+  if (n == 0) return FMI_OK;
+  if (!src || !dst) return fail(FMI_EINVAL, "NULL buffer");
+  static const struct Table {
+    uint8_t t[256];
+    Table() {
+      for (int i = 0; i < 256; ++i) t[i] = 4;
+      t['A'] = t['a'] = 0;
+      t['C'] = t['c'] = 1;
+      t['G'] = t['g'] = 2;
+      t['T'] = t['t'] = 3;
+    }
+  } table;
+  for (uint64_t i = 0; i < n; ++i) dst[i] = table.t[(uint8_t)src[i]];
+  return FMI_OK;
+}
+
 int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
                      const uint64_t *d_offsets, uint32_t nreads, uint32_t cap, uint32_t row_cap,
                      int32_t *d_chrom, uint64_t *d_coord, fmi_stream s) {
@@ -569,7 +600,9 @@ int fmi_seed_map_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64
 int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uint64_t nlines,
                      uint64_t *d_sink, float *ms, uint64_t *lines_done, fmi_stream s) {
   if (!d_buf || !d_sink || !ms || !lines_done) return fail(FMI_EINVAL, "NULL argument");
-  if (line_bytes != 64 && line_bytes != 128 && line_bytes != 256) return fail(FMI_EINVAL, "line_bytes must be 64, 128 or 256");
+  if (line_bytes != 16 && line_bytes != 32 && line_bytes != 64 && line_bytes != 128 &&
+      line_bytes != 256)
+    return fail(FMI_EINVAL, "line_bytes must be 16, 32, 64, 128 or 256");
   if (bytes < (64ull << 20)) return fail(FMI_EINVAL, "buffer must be >= 64 MiB");
   int cus = 0;
   int rc = cu_count(&cus);
@@ -583,6 +616,23 @@ int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uin
   rc = end_timed(st, e, "gather launch");
   if (rc) return rc;
   *lines_done = fmi::gather_loads_done(nlines, (uint32_t)cus * 16u);
+  return fmi_last_kernel_ms(ms);
+}
+
+int fmi_copy_bench(const void *d_src, void *d_dst, uint64_t bytes, float *ms, fmi_stream s) {
+  if (!d_src || !d_dst || !ms) return fail(FMI_EINVAL, "NULL argument");
+  if (bytes == 0 || bytes % 16) return fail(FMI_EINVAL, "bytes must be a positive multiple of 16");
+  if ((reinterpret_cast<uintptr_t>(d_src) | reinterpret_cast<uintptr_t>(d_dst)) & 15u)
+    return fail(FMI_EINVAL, "buffers must be 16-byte aligned");
+  int cus = 0;
+  int rc = cu_count(&cus);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_copy(d_src, d_dst, bytes, (uint32_t)cus * 8u, st);
+  rc = end_timed(st, e, "copy launch");
+  if (rc) return rc;
   return fmi_last_kernel_ms(ms);
 }
 

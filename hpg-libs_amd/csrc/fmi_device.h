@@ -165,4 +165,33 @@ __device__ __forceinline__ uint32_t sa_value(const DevDir &D, uint32_t ratio, ui
   return (uint32_t)(v % ((uint64_t)n + 1));
 }
 
+// getRcompValue (search/csafm.h:237-272): start from the ISA sample of the
+// next sampled text position at or after m (or of position 0, walking
+// around the '$' row, when there is none) and step LF back to m.
+__device__ __forceinline__ uint32_t isa_value(const DevDir &D, uint32_t ratio, uint32_t n,
+                                              uint32_t m) {
+  const uint64_t siz = (uint64_t)n + 1;
+  uint64_t i = (ratio - (m % ratio)) % ratio;
+  const uint64_t k = (uint64_t)m + i;
+  uint32_t j;
+  if (k < siz) {
+    j = D.R[k / ratio];
+  } else {
+    j = D.R[0];
+    i = siz - m;
+  }
+  while (i) {
+    if (j == D.dollar) {
+      j = 0;
+    } else {
+      const uint32_t b = j / kBlockRows, t = j - b * kBlockRows;
+      const Blk B = load_block(D.blk, b);
+      const uint32_t c = code_at(B, t);
+      j = pick4(D.C, c) + occ_blk(B, D.dollar_blk, D.dollar_off, b, c, t) + 1u;
+    }
+    --i;
+  }
+  return j;
+}
+
 }  // namespace fmi

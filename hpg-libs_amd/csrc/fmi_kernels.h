@@ -32,6 +32,8 @@ hipError_t launch_compact(const uint32_t *counts, const fmi_hit *hits, uint32_t 
 hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
                          uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
                          hipStream_t s);
+hipError_t launch_isa(const DevIndex &ix, uint32_t dsel, const uint64_t *tpos, uint64_t count,
+                      uint64_t *rows, hipStream_t s);
 hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_t *counts,
                           const uint64_t *offsets, uint32_t nreads, uint32_t cap,
                           uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
@@ -39,5 +41,7 @@ hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_
 hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
                          unsigned long long *sink, uint32_t grid, hipStream_t s);
 uint64_t gather_loads_done(uint64_t nloads, uint32_t grid);
+// bytes / 16 words copied from src to dst (both 16-byte aligned).
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t grid, hipStream_t s);
 
 }  // namespace fmi

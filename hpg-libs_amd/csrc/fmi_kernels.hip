@@ -475,6 +475,30 @@ __global__ __launch_bounds__(256) void locate_kernel(LocateParams P) {
 }
 
 // --------------------------------------------------------------------------
+// inverse locate: getRcompValue (search/csafm.h:237-272)
+// --------------------------------------------------------------------------
+
+struct IsaParams {
+  DevIndex ix;
+  uint32_t dsel;
+  uint64_t count;
+  const uint64_t *tpos;
+  uint64_t *rows;
+};
+
+__global__ __launch_bounds__(256) void isa_kernel(IsaParams P) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= P.count) return;
+  const uint64_t m = P.tpos[g];
+  // Text positions beyond n are not in the index.
+  if (m > (uint64_t)P.ix.n) {
+    P.rows[g] = ~0ull;
+    return;
+  }
+  P.rows[g] = isa_value(dir_of(P.ix, P.dsel), P.ix.ratio, P.ix.n, (uint32_t)m);
+}
+
+// --------------------------------------------------------------------------
 // position resolution (bwt.c:2158-2166, binsearch search/io.h:43-64)
 // --------------------------------------------------------------------------
 
@@ -640,6 +664,16 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint4 *buf, uint64_t 
   if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);   // keeps the loads live
 }
 
+// Streaming copy of n16 16-byte words, grid-stride: a kernel of known HBM
+// traffic (n16*16 bytes read and written) that calibrates the profiler's
+// fetch/write counters and measures the streaming bandwidth.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, uint64_t n16) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
 // --------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------
@@ -727,6 +761,15 @@ hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, co
   return hipGetLastError();
 }
 
+hipError_t launch_isa(const DevIndex &ix, uint32_t dsel, const uint64_t *tpos, uint64_t count,
+                      uint64_t *rows, hipStream_t s) {
+  IsaParams P{ix, dsel, count, tpos, rows};
+  const uint64_t grid = (count + 255) / 256;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(isa_kernel, dim3((uint32_t)grid), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
 hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_t *counts,
                           const uint64_t *offsets, uint32_t nreads, uint32_t cap,
                           uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
@@ -749,11 +792,21 @@ hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, u
   per_thread = (per_thread + 3) & ~3ull;
   const uint4 *b = (const uint4 *)buf;
   switch (line_bytes) {
+    case 16: hipLaunchKernelGGL(gather_kernel<1>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
+    case 32: hipLaunchKernelGGL(gather_kernel<2>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
     case 64: hipLaunchKernelGGL(gather_kernel<4>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
     case 128: hipLaunchKernelGGL(gather_kernel<8>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
     case 256: hipLaunchKernelGGL(gather_kernel<16>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t grid, hipStream_t s) {
+  const uint64_t n16 = bytes / 16;
+  if (n16 == 0 || grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst,
+                     n16);
   return hipGetLastError();
 }
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
     "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
-    "fmi_seed_map_batch",
+    "fmi_seed_map_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
     "fmi_server_get_stats", "fmi_server_destroy",
 ]
@@ -107,6 +107,9 @@ def _bind(lib):
         "fmi_get_tuning": ([C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)], _i),
         "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
+        "fmi_inverse_locate_batch": ([_p, _i, _p, _u64, _p, _p], _i),
+        "fmi_encode_bases": ([_p, _u64, _p], _i),
+        "fmi_copy_bench": ([_p, _p, _u64, C.POINTER(C.c_float), _p], _i),
         "fmi_gather_bench": ([_p, _u64, _u32, _u64, _p, C.POINTER(C.c_float), C.POINTER(_u64),
                               _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
@@ -442,6 +445,32 @@ def locate_batch(ix, k, l, cap, forward=0, stream=None):
     return dp.download(np.uint64, n * cap, stream).reshape(n, cap), dn.download(np.uint32, n, stream)
 
 
+def inverse_locate_batch(ix, text_pos, forward=0, stream=None):
+    """fmi_inverse_locate_batch: the BWT row of each text position (get_ISA)."""
+    t = np.ascontiguousarray(text_pos, dtype=np.uint64)
+    n = len(t)
+    if n == 0:
+        return np.zeros(0, np.uint64)
+    dt = DeviceBuffer.from_array(t, stream)
+    dr = DeviceBuffer(n * 8)
+    check(lib().fmi_inverse_locate_batch(ix.h, forward, dt.ptr, n, dr.ptr, stream))
+    sync(stream)
+    return dr.download(np.uint64, n, stream)
+
+
+def encode_bases(text):
+    """fmi_encode_bases: ASCII bases (str, bytes or uint8 array) to codes
+    0..3 for A/C/G/T in either case and 4 for every other byte."""
+    if isinstance(text, str):
+        text = text.encode("latin-1")
+    src = np.frombuffer(bytes(text), np.uint8) if not isinstance(text, np.ndarray) else \
+        np.ascontiguousarray(text, dtype=np.uint8)
+    dst = np.empty(len(src), np.uint8)
+    if len(src):
+        check(lib().fmi_encode_bases(src.ctypes.data, len(src), dst.ctypes.data))
+    return dst
+
+
 def resolve_hits(ix, batch, out, row_cap, stream=None):
     n, cap = batch.nreads, out.cap
     tot = max(2 * n * cap * row_cap, 1)
@@ -557,6 +586,23 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def copy_bench(nbytes=2 << 30, reps=3, stream=None):
+    """Streaming copy of nbytes between two HBM buffers (fmi_copy_bench);
+    returns the mean kernel ms over reps launches after one warm-up."""
+    src = DeviceBuffer(nbytes)
+    dst = DeviceBuffer(nbytes)
+    src.zero(stream)
+    ms = C.c_float(0)
+    total = 0.0
+    for r in range(reps + 1):
+        check(lib().fmi_copy_bench(src.ptr, dst.ptr, nbytes, C.byref(ms), stream))
+        if r:
+            total += float(ms.value)
+    src.free()
+    dst.free()
+    return total / reps
 
 
 def gather_bench(nbytes=8 << 30, line_bytes=64, nlines=1 << 28, stream=None):

@@ -211,6 +211,17 @@ int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k,
                      const int64_t *d_l, uint32_t nintervals, uint32_t cap,
                      uint64_t *d_pos, uint32_t *d_nrows, fmi_stream s);
 
+/* get_ISA (search/runtime.h:88, getRcompValue search/csafm.h:237-272): the
+ * BWT row of text position d_text_pos[i] (0..n) of the BACKWARD (forward =
+ * 0) or FORWARD (forward = 1) index; positions beyond n give row UINT64_MAX. */
+int fmi_inverse_locate_batch(const fmi_index *ix, int forward, const uint64_t *d_text_pos,
+                             uint64_t count, uint64_t *d_rows, fmi_stream s);
+
+/* encode_bases (bwt_commons.c:99-102) with the alphabet "ACGT": A/C/G/T in
+ * either case become 0..3; every other byte becomes 4 (N; the reference
+ * leaves those codes unset, bwt_commons.c:60-91).  Host function. */
+int fmi_encode_bases(const char *src, uint64_t n, uint8_t *dst);
+
 /* Position resolution of bwt.c:2158-2166 for 1-error records: for each hit
  * (as stored by fmi_search1_batch: hits[(2*i+s)*cap ...], counts) and each
  * SA row j of [k, l] (at most row_cap rows per hit), key =
@@ -318,12 +329,17 @@ int fmi_server_get_stats(fmi_server *srv, fmi_server_stats *stats);
 int fmi_server_destroy(fmi_server *srv);
 
 /* ------------------------------------------------------ measurement */
-/* Random gather of line_bytes-sized aligned lines over d_buf (bytes >= 64
- * MiB), at least nlines loads: the random-access roofline of the path.
+/* Random gather of line_bytes-sized (16, 32, 64, 128 or 256) aligned lines
+ * over d_buf (bytes >= 64 MiB), at least nlines loads: the random-access roofline of the path.
  * *ms is the kernel time and *lines_done the loads performed. */
 int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes,
                      uint64_t nlines, uint64_t *d_sink, float *ms, uint64_t *lines_done,
                      fmi_stream s);
+/* Streaming copy of bytes (a multiple of 16) from d_src to d_dst: a kernel
+ * whose HBM traffic is known (bytes read, bytes written), to calibrate
+ * hardware fetch/write counters and to measure the streaming bandwidth.
+ * *ms is the kernel time.  No reference counterpart (measurement only). */
+int fmi_copy_bench(const void *d_src, void *d_dst, uint64_t bytes, float *ms, fmi_stream s);
 /* Counts the distinct 64-byte occurrence blocks one fmi_search1_batch with
  * the same arguments fetches (the algorithmic bytes of the launch) and the
  * LF steps it takes. */

@@ -73,3 +73,18 @@ def test_read_validation():
     with pytest.raises(ValueError):
         hpgfm.validate_reads(np.zeros(2000, np.uint8), np.array([0, 1500], np.uint64))
     hpgfm.validate_reads(reads, np.array([0, 5, 10], np.uint64))
+
+
+def test_encode_bases():
+    """encode_bases (bwt_commons.c:99-102) over the table that
+    bwt_init_replace_table (bwt_commons.c:58-89) builds for "ACGT": both cases
+    of each base map to its index; the bytes the reference leaves unset map
+    to 4, the N code of every search entry point."""
+    allb = np.arange(256, dtype=np.uint8)
+    got = hpgfm.encode_bases(allb)
+    want = np.full(256, 4, np.uint8)
+    for i, c in enumerate("ACGT"):
+        want[ord(c)] = want[ord(c.lower())] = i
+    assert np.array_equal(got, want)
+    assert list(hpgfm.encode_bases("AcGtNx")) == [0, 1, 2, 3, 4, 4]
+    assert len(hpgfm.encode_bases(b"")) == 0

@@ -122,6 +122,27 @@ def test_locate_golden(gold):
         assert list(pos[i, :len(want)]) == want
 
 
+def test_inverse_locate_golden(gold):
+    """get_ISA (runtime.h:88) against the golden vectors of the compiled
+    reference core, every text position against the oracle, ISA(SA(i)) = i
+    through both kernels, and positions beyond n."""
+    hp, g, ix, batch = gold
+    pos = g["sa_rows"].astype(np.uint64)
+    for fwd, key in ((0, "isa_bwd"), (1, "isa_fwd")):
+        assert np.array_equal(hp.inverse_locate_batch(ix, pos, forward=fwd), g[key])
+    n = ix.size_sa - 1
+    o = fo.Oracle(g["genome"], ratio=int(g["ratio"]))
+    allpos = np.arange(n + 1, dtype=np.uint64)
+    for fwd in (0, 1):
+        rows = hp.inverse_locate_batch(ix, allpos, forward=fwd)
+        assert [int(x) for x in rows] == [o.get_isa(fwd, m) for m in range(n + 1)]
+        sa, nrows = hp.locate_batch(ix, rows.astype(np.int64), rows.astype(np.int64), cap=1,
+                                    forward=fwd)
+        assert np.array_equal(sa[:, 0], allpos)
+    out = hp.inverse_locate_batch(ix, np.array([n + 1, 1 << 40, 0], np.uint64))
+    assert list(out[:2]) == [np.uint64(2**64 - 1)] * 2 and int(out[2]) == o.get_isa(0, 0)
+
+
 def test_resolve_hits(gpu_engine):
     """Position resolution of bwt.c:2158-2166 against the same formula on the
     oracle's get_SA and binsearch."""
