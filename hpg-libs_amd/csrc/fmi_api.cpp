@@ -77,6 +77,32 @@ int workspace_for(hipStream_t s, Workspace *out) {
   return FMI_OK;
 }
 
+// Per-(device, stream) scratch that grows to the largest request made on
+// it; freed only when a larger one replaces it (after the stream drains).
+std::unordered_map<std::string, std::pair<void *, uint64_t>> g_scratch;
+
+int scratch_for(hipStream_t s, uint64_t bytes, void **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  char key[64];
+  snprintf(key, sizeof key, "%d:%p", dev, (void *)s);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto &slot = g_scratch[key];
+  if (slot.second < bytes) {
+    if (slot.first) {
+      HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+      HIP_TRY(hipFree(slot.first), "hipFree(scratch)");
+      slot = {nullptr, 0};
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(FMI_ENOMEM, "device scratch of " + std::to_string(bytes) + " bytes");
+    slot = {p, bytes};
+  }
+  *out = slot.first;
+  return FMI_OK;
+}
+
 int cu_count(int *cus) {
   static int cached[64] = {0};
   int dev = 0;
@@ -483,6 +509,36 @@ int fmi_search1_batch(const fmi_index *ix, int mode, int strand_mask, const uint
                       uint32_t *d_counts, fmi_stream s) {
   return search1_common(ix, mode, strand_mask, d_reads, d_offsets, nreads, cap, d_hits, d_counts,
                         s, true);
+}
+
+int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
+                             const uint8_t *d_reads, const uint64_t *d_offsets,
+                             const int16_t *d_bounds, uint32_t nreads, uint32_t cap,
+                             fmi_hit *d_hits, uint32_t *d_counts, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (mode != FMI_MODE_EDIT && mode != FMI_MODE_MISMATCH) return fail(FMI_EINVAL, "mode must be EDIT or MISMATCH");
+  if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "strand_mask must be 1, 2 or 3");
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_counts) return fail(FMI_EINVAL, "NULL buffer");
+  if (cap > 0 && !d_hits) return fail(FMI_EINVAL, "d_hits is NULL with cap > 0");
+  if (nreads > (1u << 30)) return fail(FMI_ERANGE, "more than 2^30 reads in one batch");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  // the scratch the vectors need: 16 B * (bases + 3 * reads)
+  uint64_t bases = 0;
+  HIP_TRY(hipMemcpyAsync(&bases, d_offsets + nreads, sizeof bases, hipMemcpyDeviceToHost, st),
+          "read offsets[nreads]");
+  HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (bases > (1ull << 40)) return fail(FMI_ERANGE, "offsets[nreads] out of range");
+  void *scratch = nullptr;
+  rc = scratch_for(st, 16ull * (bases + 3ull * nreads), &scratch);
+  if (rc) return rc;
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_vhelper(ix->dev, mode == FMI_MODE_MISMATCH, (uint32_t)strand_mask,
+                                     d_reads, d_offsets, d_bounds, nreads, cap, d_hits, d_counts,
+                                     scratch, st);
+  return end_timed(st, e, "search1 vector launch");
 }
 
 int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,

@@ -29,6 +29,12 @@ uint64_t compact_scratch_bytes(uint32_t nslots);
 hipError_t launch_compact(const uint32_t *counts, const fmi_hit *hits, uint32_t nslots,
                           uint32_t cap, fmi_hit *out, unsigned long long *total, void *scratch,
                           hipStream_t s);
+// BWSearch1VectorHelper on both strands of every read; scratch holds
+// 16 * (offsets[nreads] + 3 * nreads) bytes.
+hipError_t launch_vhelper(const DevIndex &ix, bool mm_only, uint32_t strand_mask,
+                          const uint8_t *reads, const uint64_t *offsets, const int16_t *bounds,
+                          uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
+                          void *scratch, hipStream_t s);
 hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
                          uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
                          hipStream_t s);

@@ -41,6 +41,7 @@ EXPORTS = [
     "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
+    "fmi_search1_vector_batch",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
     "fmi_server_get_stats", "fmi_server_destroy",
 ]
@@ -110,6 +111,7 @@ def _bind(lib):
         "fmi_inverse_locate_batch": ([_p, _i, _p, _u64, _p, _p], _i),
         "fmi_encode_bases": ([_p, _u64, _p], _i),
         "fmi_copy_bench": ([_p, _p, _u64, C.POINTER(C.c_float), _p], _i),
+        "fmi_search1_vector_batch": ([_p, _i, _i, _p, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_gather_bench": ([_p, _u64, _u32, _u64, _p, C.POINTER(C.c_float), C.POINTER(_u64),
                               _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
@@ -421,6 +423,29 @@ def search1_batch(ix, batch, cap=64, mode=MODE_MISMATCH, strand_mask=3, plain=Fa
     c = out.counts(stream).reshape(batch.nreads, 2)
     h = out.hits(stream)
     out.free()
+    return c, h
+
+
+def search1_vector_batch(ix, batch, cap=64, mode=MODE_EDIT, strand_mask=3, bounds=None,
+                         stream=None):
+    """fmi_search1_vector_batch (BWSearch1VectorHelper per read and strand);
+    bounds: None or int16 [nreads, 2] windows relative to each read.
+    Returns (counts[n, 2], hits[n, 2, cap]) on the host."""
+    out = Search1Output(batch.nreads, cap)
+    db = None
+    if bounds is not None:
+        b = np.ascontiguousarray(bounds, dtype=np.int16).reshape(batch.nreads, 2)
+        db = DeviceBuffer.from_array(b, stream)
+    check(lib().fmi_search1_vector_batch(ix.h, mode, strand_mask, batch.d_reads.ptr,
+                                         batch.d_offsets.ptr, db.ptr if db else None,
+                                         batch.nreads, cap, out.d_hits.ptr, out.d_counts.ptr,
+                                         stream))
+    sync(stream)
+    c = out.counts(stream).reshape(batch.nreads, 2)
+    h = out.hits(stream)
+    out.free()
+    if db:
+        db.free()
     return c, h
 
 

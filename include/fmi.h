@@ -194,6 +194,21 @@ int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
                             uint32_t nreads, uint32_t cap, fmi_hit *d_hits,
                             uint32_t *d_counts, fmi_stream s);
 
+/* The mapping layer's 1-error search: BWSearch1VectorHelper
+ * (search/search.c:1491-1709) driven by the interval vectors of
+ * BWExactSearchVectorBackward/Forward, per strand as __bwt_map_inexact_read
+ * calls it (bwt.c:1940-1950, 1965-1972; + strand backward/forward, - strand
+ * forward_rev/backward_rev).  d_bounds is NULL (the whole read, start 0) or
+ * int16 [start, end] per read, relative to the read, as bwt_map_inexact_seed
+ * passes a seed window (bwt.c:1447-1479).  Records and capacity as
+ * fmi_search1_batch; windows shorter than 3 bases (or outside the read) give
+ * no records (the reference reads outside its vectors there; the mapping
+ * layer calls it for reads of 5 bases or more). */
+int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
+                             const uint8_t *d_reads, const uint64_t *d_offsets,
+                             const int16_t *d_bounds, uint32_t nreads, uint32_t cap,
+                             fmi_hit *d_hits, uint32_t *d_counts, fmi_stream s);
+
 /* Tuning of the persistent launch (the autotuner writes it): workgroups of
  * 256 lanes per CU, tasks a wavefront takes from the queue per atomic, and
  * the number of idle lanes (1..64) at which a wavefront refills them.  0

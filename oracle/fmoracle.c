@@ -459,6 +459,174 @@ uint32_t fmo_search1(const fmo_index *ix, int strand, int mismatch_only,
   return s.cnt;
 }
 
+/* BWSearch1VectorHelper, search.c:1491-1709, driven by the four interval
+ * vectors of bwt.c:1940-1950 (or bwt.c:1447-1457 for a seed [start, end]).
+ * One result r is reused for every record, so its error fields carry over
+ * from one record to the next exactly as the reference's do: only the fields
+ * each step assigns change. */
+typedef struct {
+  int64_t k, l;
+  int16_t pos;
+  uint8_t dir, nmm, kind, base;
+  int16_t err_pos;
+} vh_state;
+
+static void vh_add(s1_sink *s, const vh_state *r, int16_t start, int16_t end) {
+  if (s->mm_only && r->nmm && r->kind != FMO_MISMATCH) return;
+  fmo_hit h;
+  memset(&h, 0, sizeof(h));
+  h.k = r->k; h.l = r->l; h.start = start; h.end = end; h.pos = r->pos;
+  h.dir = r->dir; h.num_mismatches = r->nmm;
+  if (r->nmm) { h.err_kind = r->kind; h.err_base = r->base; h.err_pos = r->err_pos; }
+  sink_add(s, &h);
+}
+
+/* BWExactSearchBackward/Forward on the state (search.h:9-65). */
+static void vh_exact(const fmo_dir *d, int comp, int dir, const uint8_t *W, int16_t start,
+                     int16_t end, vh_state *r) {
+  fmo_hit t;
+  memset(&t, 0, sizeof(t));
+  t.k = r->k; t.l = r->l; t.pos = r->pos; t.start = start; t.end = end;
+  exact_dir(d, comp, dir, W, &t, NULL);
+  r->k = t.k; r->l = t.l; r->pos = t.pos;
+}
+
+uint32_t fmo_search1_vector(const fmo_index *ix, int strand, int mismatch_only,
+                            const uint8_t *W, int16_t start, int16_t end, fmo_hit *out,
+                            uint32_t cap) {
+  /* + strand: vectors on (backward, forward), bwt.c:1940-1944,1967;
+   * - strand: (forward_rev, backward_rev), bwt.c:1946-1950,1971. */
+  const int vb = strand ? FMO_VIEW_FORWARD_REV : FMO_VIEW_BACKWARD;
+  const int vf = strand ? FMO_VIEW_BACKWARD_REV : FMO_VIEW_FORWARD;
+  const fmo_dir *db = view_dir(ix, vb), *df = view_dir(ix, vf);
+  const int cb = view_comp(vb), cf = view_comp(vf);
+  s1_sink s = {W, out, cap, 0, mismatch_only, (uint8_t)strand};
+  if (end - start < 2) return 0;   /* the helper reads outside its vectors below 3 bases */
+
+  const int16_t n = (int16_t)(end - start);
+  const int64_t lsa = (int64_t)fmo_size_sa(ix) - 1;
+  int64_t *vk = (int64_t *)malloc(4 * (size_t)(n + 1) * sizeof(int64_t));
+  int64_t *vl = vk + (n + 1), *vki = vl + (n + 1), *vli = vki + (n + 1);
+  fmo_exact_vector(ix, vb, 0, W, start, end, 0, lsa, vk, vl);
+  fmo_exact_vector(ix, vf, 1, W, start, end, 0, lsa, vki, vli);
+
+  int16_t half = (int16_t)(n / 2), i, j;
+  vh_state r;
+  memset(&r, 0, sizeof(r));
+  r.dir = 0;
+  if (vk[0] <= vl[0]) { r.k = vk[0]; r.l = vl[0]; r.pos = -1; vh_add(&s, &r, start, end); }
+  r.nmm = 1; r.kind = FMO_MATCH; r.base = 0xFF; r.err_pos = start;
+
+  int64_t k, l, ka, la, size, size_last;
+  /* error at W[start]: the suffix interval W[start+1..end] */
+  size = vl[0] - vk[0];
+  size_last = size;
+  k = vk[1]; l = vl[1];
+  size = l - k;
+  if (size != size_last) {
+    r.k = k; r.l = l; r.pos = -1;
+    r.kind = FMO_DELETION; r.base = 0xFF; r.err_pos = start;
+    vh_add(&s, &r, start, end);
+    for (unsigned b = 0; b < 4; b++) {
+      lf(db, cb, k, l, b, &ka, &la);
+      if (ka > la) continue;
+      const uint8_t bw = W[start];
+      if (b != bw) {
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = FMO_MISMATCH; r.base = (uint8_t)b;
+        vh_add(&s, &r, start, end);
+      }
+      lf(db, cb, ka, la, bw, &ka, &la);
+      if (ka <= la) {
+        /* modify_last_mismatch3(&r, 3, INSERTION, b), search.c:1556: kind 3,
+         * base INSERTION, position b */
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = 3; r.base = FMO_INSERTION; r.err_pos = (int16_t)b;
+        vh_add(&s, &r, start, end);
+      }
+    }
+  }
+  for (i = (int16_t)(start + 2), j = 2; j <= half; i++, j++) {
+    size_last = size;
+    k = vk[j]; l = vl[j];
+    size = l - k;
+    if (size == size_last) continue;
+    r.k = k; r.l = l; r.pos = (int16_t)(i - 2);
+    r.kind = FMO_DELETION; r.base = 0xFF; r.err_pos = (int16_t)(i - 1);
+    vh_exact(db, cb, 0, W, start, end, &r);
+    if (r.k <= r.l) vh_add(&s, &r, start, end);
+    for (unsigned b = 0; b < 4; b++) {
+      lf(db, cb, k, l, b, &ka, &la);
+      if (ka > la) continue;
+      r.k = ka; r.l = la; r.pos = (int16_t)(i - 1);
+      r.kind = FMO_INSERTION; r.base = (uint8_t)b;
+      vh_exact(db, cb, 0, W, start, end, &r);
+      if (r.k <= r.l) vh_add(&s, &r, start, end);
+      if (b != W[i - 1]) {
+        r.k = ka; r.l = la; r.pos = (int16_t)(i - 2);
+        r.kind = FMO_MISMATCH;
+        vh_exact(db, cb, 0, W, start, end, &r);
+        if (r.k <= r.l) vh_add(&s, &r, start, end);
+      }
+    }
+  }
+
+  half--;
+  size = vli[n] - vki[n];
+  r.dir = 1;
+  /* error at W[end]: the prefix interval W[start..end-1] */
+  size_last = size;
+  k = vki[n - 1]; l = vli[n - 1];
+  size = l - k;
+  if (size != size_last) {
+    r.k = k; r.l = l; r.pos = -1;
+    r.kind = FMO_DELETION; r.base = 0xFF; r.err_pos = end;
+    vh_add(&s, &r, start, end);
+    for (unsigned b = 0; b < 4; b++) {
+      lf(df, cf, k, l, b, &ka, &la);
+      if (ka > la) continue;
+      const uint8_t bw = W[end];
+      if (b != bw) {
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = FMO_MISMATCH; r.base = (uint8_t)b;
+        vh_add(&s, &r, start, end);
+      }
+      lf(df, cf, ka, la, bw, &ka, &la);
+      if (ka <= la) {
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = FMO_INSERTION; r.base = (uint8_t)b;
+        vh_add(&s, &r, start, end);
+      }
+    }
+  }
+  for (i = (int16_t)(end - 2), j = (int16_t)(n - 2); j >= half; i--, j--) {
+    size_last = size;
+    k = vki[j]; l = vli[j];
+    size = l - k;
+    if (size == size_last) continue;
+    r.k = k; r.l = l; r.pos = (int16_t)(i + 2);
+    r.kind = FMO_DELETION; r.base = 0xFF; r.err_pos = (int16_t)(i + 1);
+    vh_exact(df, cf, 1, W, start, end, &r);
+    if (r.k <= r.l) vh_add(&s, &r, start, end);
+    for (unsigned b = 0; b < 4; b++) {
+      lf(df, cf, k, l, b, &ka, &la);
+      if (ka > la) continue;
+      r.k = ka; r.l = la; r.pos = (int16_t)(i + 1);
+      r.kind = FMO_INSERTION; r.base = (uint8_t)b;
+      vh_exact(df, cf, 1, W, start, end, &r);
+      if (r.k <= r.l) vh_add(&s, &r, start, end);
+      if (b != W[i + 1]) {
+        r.k = ka; r.l = la; r.pos = (int16_t)(i + 2);
+        r.kind = FMO_MISMATCH;
+        vh_exact(df, cf, 1, W, start, end, &r);
+        if (r.k <= r.l) vh_add(&s, &r, start, end);
+      }
+    }
+  }
+  free(vk);
+  return s.cnt;
+}
+
 uint64_t fmo_seed_map(const fmo_index *ix, const uint8_t *W, uint32_t len, uint32_t seed_len,
                       uint32_t stride, int32_t filter, const uint64_t *koff,
                       const uint64_t *kstart, uint64_t kcount, uint32_t read,

@@ -128,6 +128,16 @@ uint32_t fmo_search1(const fmo_index *ix, int strand, int mismatch_only,
                      const uint8_t *W, int16_t len, fmo_hit *out, uint32_t cap,
                      fmo_hit *anchors);
 
+/* BWSearch1VectorHelper (search.c:1491-1709) over W[start..end] on one
+ * strand, with the interval vectors bwt.c:1940-1950 computes (+ strand:
+ * backward/forward, - strand: forward_rev/backward_rev).  Records as
+ * fmo_search1; mismatch_only keeps the exact record and MISMATCH records.
+ * end - start < 2 gives no records (the reference reads outside its vectors
+ * there; the mapping layer calls it for reads of 5 bases or more). */
+uint32_t fmo_search1_vector(const fmo_index *ix, int strand, int mismatch_only,
+                            const uint8_t *W, int16_t start, int16_t end, fmo_hit *out,
+                            uint32_t cap);
+
 /* ---- seed mapping ------------------------------------------------------ */
 /* One region of bwt_map_exact_seed (same layout as fmi_seed_region). */
 typedef struct {

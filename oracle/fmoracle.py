@@ -83,6 +83,9 @@ def _load_oracle():
     lib.fmo_exact_vector.restype = C.c_int16
     lib.fmo_search1.argtypes = [_p, C.c_int, C.c_int, _p, C.c_int16, _p, C.c_uint32, _p]
     lib.fmo_search1.restype = C.c_uint32
+    lib.fmo_search1_vector.argtypes = [_p, C.c_int, C.c_int, _p, C.c_int16, C.c_int16, _p,
+                                       C.c_uint32]
+    lib.fmo_search1_vector.restype = C.c_uint32
     lib.fmo_seed_map.argtypes = [_p, _p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _p, _p,
                                  C.c_uint64, C.c_uint32, _p, C.c_uint64, _p]
     lib.fmo_seed_map.restype = C.c_uint64
@@ -210,6 +213,15 @@ class Oracle:
         res = out[:min(n, cap)].copy()
         return (int(n), res, an) if anchors else (int(n), res)
 
+    def search1_vector(self, strand, W, start=0, end=None, cap=1 << 14, mismatch_only=False):
+        """BWSearch1VectorHelper over W[start..end] on one strand: (count, records)."""
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        end = len(W) - 1 if end is None else end
+        out = np.zeros(max(cap, 1), HIT_DTYPE)
+        n = self.lib.fmo_search1_vector(self.ref, strand, int(mismatch_only), _ptr(W), start, end,
+                                        _ptr(out), cap)
+        return int(n), out[:min(n, cap)].copy()
+
     def seed_map(self, W, seed_len, stride, filter_seed_mappings, koff, kstart, read=0):
         """bwt_map_exact_seed over the tiling of one read: (regions, info[nseeds, 6])."""
         W = np.ascontiguousarray(W, dtype=np.uint8)
@@ -294,6 +306,9 @@ class RefCore:
         lib.ref_search1.argtypes = [_p, C.c_int, _p, C.c_int16, C.c_uint32, _p, _p,
                                     C.POINTER(C.c_int)]
         lib.ref_search1.restype = C.c_uint32
+        lib.ref_search1_vector.argtypes = [_p, C.c_int, _p, C.c_int16, C.c_int16, C.c_uint32, _p,
+                                           C.POINTER(C.c_int)]
+        lib.ref_search1_vector.restype = C.c_uint32
         lib.ref_get_sa.argtypes = [_p, C.c_int, C.c_uint64]
         lib.ref_get_sa.restype = C.c_uint64
         lib.ref_get_isa.argtypes = [_p, C.c_int, C.c_uint64]
@@ -385,6 +400,15 @@ class RefCore:
                                  C.byref(full))
         res = out[:n].copy()
         return (int(n), res, bool(full.value), an) if anchors else (int(n), res, bool(full.value))
+
+    def search1_vector(self, strand, W, start=0, end=None, cap=1 << 14):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        end = len(W) - 1 if end is None else end
+        out = np.zeros(max(cap, 1), HIT_DTYPE)
+        full = C.c_int(0)
+        n = self.lib.ref_search1_vector(self.h, strand, _ptr(W), start, end, cap, _ptr(out),
+                                        C.byref(full))
+        return int(n), out[:n].copy(), bool(full.value)
 
     def get_sa(self, fwd, m):
         return int(self.lib.ref_get_sa(self.h, fwd, m))

@@ -258,6 +258,40 @@ uint32_t ref_search1(void *h, int strand, const uint8_t *W, int16_t len, uint32_
   return nres;
 }
 
+/* The mapping layer's 1-error search of W[start..end] on one strand: the
+ * interval vectors, then BWSearch1VectorHelper, as bwt.c:1940-1950 +
+ * 1965-1972 (start = 0) and bwt.c:1447-1457 + 1465-1479 (a seed). */
+uint32_t ref_search1_vector(void *h, int strand, const uint8_t *W, int16_t start, int16_t end,
+                            uint32_t cap, ref_hit *out, int *full) {
+  ref_ctx *c = (ref_ctx *)h;
+  const int n1 = end - start + 1;
+  intmax_t *v = (intmax_t *)malloc(4 * (size_t)n1 * sizeof(intmax_t));
+  intmax_t *a_k = v, *a_l = v + n1, *b_k = v + 2 * n1, *b_l = v + 3 * n1;
+  results_list rl;
+  new_results_list(&rl, cap ? cap : 1);
+  rl.max_results = cap;
+  rl.num_results = 0;
+  bool f;
+  const intmax_t lsa = size_SA(&c->backward) - 1;
+  if (strand == 0) {
+    BWExactSearchVectorBackward((uint8_t *)W, start, end, 0, lsa, a_k, a_l, &c->backward);
+    BWExactSearchVectorForward((uint8_t *)W, start, end, 0, lsa, b_k, b_l, &c->forward);
+    f = BWSearch1VectorHelper((uint8_t *)W, start, end, a_k, a_l, b_k, b_l, &c->backward,
+                              &c->forward, &rl, 4);
+  } else {
+    BWExactSearchVectorBackward((uint8_t *)W, start, end, 0, lsa, a_k, a_l, &c->forward_rev);
+    BWExactSearchVectorForward((uint8_t *)W, start, end, 0, lsa, b_k, b_l, &c->backward_rev);
+    f = BWSearch1VectorHelper((uint8_t *)W, start, end, a_k, a_l, b_k, b_l, &c->forward_rev,
+                              &c->backward_rev, &rl, 4);
+  }
+  for (uintmax_t i = 0; i < rl.num_results; i++) to_hit(&rl.list[i], &out[i], (uint8_t)strand);
+  if (full) *full = f ? 1 : 0;
+  uint32_t nres = (uint32_t)rl.num_results;
+  free(rl.list);
+  free(v);
+  return nres;
+}
+
 uint64_t ref_get_sa(void *h, int fwd, uint64_t m) {
   ref_ctx *c = (ref_ctx *)h;
   return get_SA((SA_TYPE)m, fwd ? &c->forward : &c->backward);

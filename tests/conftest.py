@@ -29,10 +29,11 @@ def golden_repeats():
     return load_golden("golden_repeats.npz")
 
 
-def split_golden_hits(g):
-    """Per-(read, strand) record lists of a golden set."""
-    counts = g["s1_counts"]
-    flat = g["s1_hits"]
+def split_golden_hits(g, prefix="s1"):
+    """Per-(read, strand) record lists of a golden set (prefix s1: BWSearch1CPU,
+    s1v: BWSearch1VectorHelper on whole reads, s1w: on the s1w_bounds windows)."""
+    counts = g[f"{prefix}_counts"]
+    flat = g[f"{prefix}_hits"]
     out = {}
     o = 0
     for i in range(counts.shape[0]):

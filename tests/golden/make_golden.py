@@ -16,6 +16,10 @@ inputs and outputs as data:
     sa_rows, sa_bwd, sa_fwd, isa_bwd, isa_fwd   get_SA / get_ISA samples
     arrays_sha        SHA-256 of the reference index arrays (C, C1, desp,
                       count, S, R for both directions)
+    s1v_counts/hits   BWSearch1VectorHelper over the interval vectors of each
+                      whole read (bwt.c:1940-1972), per strand
+    s1w_bounds        one window [start, end] per read (a seed, bwt.c:1447)
+    s1w_counts/hits   the same search over that window
 
   golden_repeats.npz
     the same fields for 400 reads of 0..60 bp on 8 kbp made of 40 mutated
@@ -96,6 +100,34 @@ def write(path, g, reads, offsets, rng):
             hits.append(h)
     s1_hits = np.concatenate(hits) if hits else np.zeros(0, fo.HIT_DTYPE)
 
+    # BWSearch1VectorHelper: whole reads, then one window per read (own rng,
+    # so the fields above do not move).  Windows under 3 bases are not run:
+    # the reference reads outside its vectors there; they hold no records.
+    wrng = np.random.default_rng(int(len(g)) + 7)
+    bounds = np.zeros((n, 2), np.int16)
+    v_counts = np.zeros((n, 2), np.uint32)
+    w_counts = np.zeros((n, 2), np.uint32)
+    v_hits, w_hits = [], []
+    for i in range(n):
+        W = reads[offsets[i]:offsets[i + 1]]
+        L = len(W)
+        if L >= 3:
+            a = int(wrng.integers(0, L - 2))
+            b = int(wrng.integers(a + 2, L))
+        else:
+            a, b = 0, L - 1
+        bounds[i] = (a, b)
+        for s in (0, 1):
+            for (st, en, cnts, lst) in ((0, L - 1, v_counts, v_hits), (a, b, w_counts, w_hits)):
+                if en - st < 2:
+                    continue
+                cnt, h, full = ref.search1_vector(s, W, st, en, cap=CAP)
+                assert not full, "raise CAP"
+                cnts[i, s] = cnt
+                lst.append(h)
+    s1v_hits = np.concatenate(v_hits) if v_hits else np.zeros(0, fo.HIT_DTYPE)
+    s1w_hits = np.concatenate(w_hits) if w_hits else np.zeros(0, fo.HIT_DTYPE)
+
     ex = {}
     lsa = ref.size_sa - 1
     for view in range(4):
@@ -128,7 +160,8 @@ def write(path, g, reads, offsets, rng):
         arrays_sha=np.array(sha), bwt_sha=np.array(hashlib.sha256(bwt.tobytes()).hexdigest()),
         bwt_dollar=np.uint64(dollar),
         bwti_sha=np.array(hashlib.sha256(bwti.tobytes()).hexdigest()),
-        bwti_dollar=np.uint64(dollari), **ex)
+        bwti_dollar=np.uint64(dollari), s1v_counts=v_counts, s1v_hits=s1v_hits,
+        s1w_bounds=bounds, s1w_counts=w_counts, s1w_hits=s1w_hits, **ex)
     print("wrote", path, os.path.getsize(path), "bytes;", int(counts.sum()), "records")
 
 
