@@ -258,6 +258,28 @@ def main():
         extras["plain_one_launch_reads_per_s"] = round(args.reads / (tp.mean_ms / 1000.0))
         extras["persistent_vs_plain"] = round(tp.mean_ms / kernel_ms, 3)
         log(f"plain one-thread-per-task launch: {tp.mean_ms:.3f} ms")
+        # the reference's GPU design re-created: reference occurrence layout,
+        # 1000-byte read stride, one thread per read and strand, synchronous
+        # copies (fmi_plain_*); same records
+        plain = hpgfm.PlainSearch(ix)
+        pc, _, _, _ = plain.search1(reads, offs, cap=args.cap, mode=mode)
+        pk, pt = [], []
+        for _ in range(2):
+            _, _, mk, mt = plain.search1(reads, offs, cap=args.cap, mode=mode)
+            pk.append(mk)
+            pt.append(mt)
+        mk, mt = min(pk), min(pt)
+        extras["plain_reference_design"] = {
+            "kernel_ms": round(mk, 3), "end_to_end_ms": round(mt, 3),
+            "kernel_reads_per_s": round(args.reads / (mk / 1000.0)),
+            "end_to_end_reads_per_s": round(args.reads / (mt / 1000.0)),
+            "layout_device_bytes": plain.device_bytes,
+            "kernel_speedup_of_engine": round(mk / kernel_ms, 2),
+            "same_counts": bool(np.array_equal(pc, out.counts().reshape(-1, 2))),
+        }
+        plain.close()
+        log(f"plain reference design: kernels {mk:.2f} ms, end to end {mt:.2f} ms "
+            f"(engine kernel {mk / kernel_ms:.1f}x faster)")
         eout = hpgfm.Search1Output(batch.nreads, args.cap)
         hpgfm.search1_launch(ix, batch, eout, mode=hpgfm.MODE_EDIT)
         hpgfm.sync()
@@ -376,6 +398,10 @@ def main():
         res["served_p50_ms"] = at200["p50_ms"]
         res["served_p99_ms"] = at200["p99_ms"]
         res["served"] = served
+        prd = extras.get("plain_reference_design")
+        if prd:
+            best = max(r["reads_per_s"] for r in served["runs"])
+            prd["served_speedup_end_to_end"] = round(best / prd["end_to_end_reads_per_s"], 2)
     res.update(extras)
     out.free()
     batch.free()

@@ -675,6 +675,56 @@ int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uin
   return fmi_last_kernel_ms(ms);
 }
 
+int fmi_plain_create(const fmi_index *ix, fmi_plain **out) {
+  if (!out) return fail(FMI_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = need_device(ix);
+  if (rc) return rc;
+  std::string err;
+  fmi::RefPlain *rp = nullptr;
+  rc = fmi::refplain_create(ix, &rp, &err);
+  if (rc) return fail(rc, err);
+  *out = reinterpret_cast<fmi_plain *>(rp);
+  return FMI_OK;
+}
+
+int fmi_plain_search1(fmi_plain *p, int mode, const uint8_t *reads, const uint64_t *offsets,
+                      uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
+                      double *ms_kernel, double *ms_total) {
+  if (!p || !ms_kernel || !ms_total) return fail(FMI_EINVAL, "NULL argument");
+  if (mode != FMI_MODE_EDIT && mode != FMI_MODE_MISMATCH) return fail(FMI_EINVAL, "bad mode");
+  *ms_kernel = *ms_total = 0;
+  if (nreads == 0) return FMI_OK;
+  if (!reads || !offsets || !counts || (cap && !hits)) return fail(FMI_EINVAL, "NULL buffer");
+  if (nreads > (1u << 26)) return fail(FMI_ERANGE, "more than 2^26 reads in one batch");
+  std::string err;
+  int rc = fmi::refplain_search1(reinterpret_cast<fmi::RefPlain *>(p), mode == FMI_MODE_MISMATCH,
+                                 reads, offsets, nreads, cap, hits, counts, ms_kernel, ms_total,
+                                 &err);
+  return rc ? fail(rc, err) : FMI_OK;
+}
+
+uint64_t fmi_plain_words(const fmi_plain *p) {
+  return fmi::refplain_words(reinterpret_cast<const fmi::RefPlain *>(p));
+}
+
+int fmi_plain_get_layout(const fmi_plain *p, int forward, uint32_t *desp, uint64_t *count) {
+  if (!p || !desp || !count) return fail(FMI_EINVAL, "NULL argument");
+  std::string err;
+  int rc = fmi::refplain_layout(reinterpret_cast<const fmi::RefPlain *>(p), forward, desp, count,
+                                &err);
+  return rc ? fail(rc, err) : FMI_OK;
+}
+
+uint64_t fmi_plain_device_bytes(const fmi_plain *p) {
+  return fmi::refplain_device_bytes(reinterpret_cast<const fmi::RefPlain *>(p));
+}
+
+int fmi_plain_free(fmi_plain *p) {
+  fmi::refplain_free(reinterpret_cast<fmi::RefPlain *>(p));
+  return FMI_OK;
+}
+
 int fmi_copy_bench(const void *d_src, void *d_dst, uint64_t bytes, float *ms, fmi_stream s) {
   if (!d_src || !d_dst || !ms) return fail(FMI_EINVAL, "NULL argument");
   if (bytes == 0 || bytes % 16) return fail(FMI_EINVAL, "bytes must be a positive multiple of 16");

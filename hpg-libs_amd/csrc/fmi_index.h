@@ -43,6 +43,17 @@ int seed_map(const fmi_index *ix, const uint8_t *reads, const uint64_t *offsets,
              uint64_t *nrows, void *s, std::string *err);
 int save_index(const fmi_index *ix, const char *path, std::string *err);
 int load_index(const char *path, fmi_index *ix, std::string *err);
+// The reference's GPU design on this device (fmi_refplain.hip).
+struct RefPlain;
+int refplain_create(const fmi_index *ix, RefPlain **out, std::string *err);
+void refplain_free(RefPlain *rp);
+int refplain_search1(RefPlain *rp, int mm_only, const uint8_t *reads, const uint64_t *offsets,
+                     uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
+                     double *ms_kernel, double *ms_total, std::string *err);
+uint64_t refplain_device_bytes(const RefPlain *rp);
+uint64_t refplain_words(const RefPlain *rp);
+int refplain_layout(const RefPlain *rp, int forward, uint32_t *desp, uint64_t *count,
+                    std::string *err);
 // Symbol of BWT row r (0xFF for the '$' row).
 uint32_t host_bwt_sym(const HostDir &d, uint64_t r);
 

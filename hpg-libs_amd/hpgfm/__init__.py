@@ -41,7 +41,8 @@ EXPORTS = [
     "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
-    "fmi_search1_vector_batch",
+    "fmi_search1_vector_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
+    "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
     "fmi_server_get_stats", "fmi_server_destroy",
 ]
@@ -112,6 +113,13 @@ def _bind(lib):
         "fmi_encode_bases": ([_p, _u64, _p], _i),
         "fmi_copy_bench": ([_p, _p, _u64, C.POINTER(C.c_float), _p], _i),
         "fmi_search1_vector_batch": ([_p, _i, _i, _p, _p, _p, _u32, _u32, _p, _p, _p], _i),
+        "fmi_plain_create": ([_p, C.POINTER(_p)], _i),
+        "fmi_plain_search1": ([_p, _i, _p, _p, _u32, _u32, _p, _p, C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)], _i),
+        "fmi_plain_words": ([_p], _u64),
+        "fmi_plain_get_layout": ([_p, _i, _p, _p], _i),
+        "fmi_plain_device_bytes": ([_p], _u64),
+        "fmi_plain_free": ([_p], _i),
         "fmi_gather_bench": ([_p, _u64, _u32, _u64, _p, C.POINTER(C.c_float), C.POINTER(_u64),
                               _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
@@ -447,6 +455,50 @@ def search1_vector_batch(ix, batch, cap=64, mode=MODE_EDIT, strand_mask=3, bound
     if db:
         db.free()
     return c, h
+
+
+class PlainSearch:
+    """The reference's GPU design re-created (fmi_plain_*): the comparison
+    point of the engine, not a product path."""
+
+    def __init__(self, ix):
+        h = C.c_void_p()
+        check(lib().fmi_plain_create(ix.h, C.byref(h)))
+        self.h = h
+        self.words = int(lib().fmi_plain_words(h))
+        self.device_bytes = int(lib().fmi_plain_device_bytes(h))
+
+    def search1(self, reads, offsets, cap=32, mode=MODE_MISMATCH):
+        """Host in, host out: (counts[n, 2], hits[n, 2, cap], ms_kernel, ms_total)."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = len(offsets) - 1
+        counts = np.zeros(max(2 * n, 1), np.uint32)
+        hits = np.zeros(max(2 * n * cap, 1), HIT_DTYPE)
+        mk, mt = C.c_double(0), C.c_double(0)
+        check(lib().fmi_plain_search1(self.h, mode, reads.ctypes.data if len(reads) else None,
+                                      offsets.ctypes.data, n, cap, hits.ctypes.data,
+                                      counts.ctypes.data, C.byref(mk), C.byref(mt)))
+        return (counts[:2 * n].reshape(n, 2), hits[:2 * n * cap].reshape(n, 2, cap),
+                float(mk.value), float(mt.value))
+
+    def layout(self, forward):
+        """(desp[4, words] uint32, count[4, words] uint64) of one direction."""
+        d = np.zeros((4, self.words), np.uint32)
+        c = np.zeros((4, self.words), np.uint64)
+        check(lib().fmi_plain_get_layout(self.h, forward, d.ctypes.data, c.ctypes.data))
+        return d, c
+
+    def close(self):
+        if self.h:
+            lib().fmi_plain_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def search1_count_blocks(ix, batch, mode=MODE_MISMATCH, strand_mask=3):

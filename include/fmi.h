@@ -343,6 +343,31 @@ int fmi_server_get_stats(fmi_server *srv, fmi_server_stats *stats);
 /* Finishes queued and in-flight work, then frees everything. */
 int fmi_server_destroy(fmi_server *srv);
 
+/* ------------------------------------------- the plain comparison point */
+/* The reference's GPU design (gpu/gpu.cu:118-203 glue, gpu.cu:333-451
+ * kernels), re-created on this device and extended to the 1-error search:
+ * the reference's occurrence layout (per-symbol desp[4][m] checkpoints and
+ * count[4][m] bitmaps behind device pointer tables, search/csafm.h:51-81,
+ * get_O csafm.h:166-181), reads padded to a 1000-byte stride, one thread per
+ * read, one launch per strand, synchronous copies in and out.  Results equal
+ * fmi_search1_batch's record for record.  Kept to measure the engine
+ * against; it is not a product path. */
+typedef struct fmi_plain fmi_plain;
+/* Builds the reference layout on the device from an index on the device. */
+int fmi_plain_create(const fmi_index *ix, fmi_plain **out);
+/* One batch from HOST buffers: copy in, both launches, copy out.  counts and
+ * hits as fmi_search1_batch (hits: nreads * 2 * cap records).  *ms_kernel is
+ * the two launches, *ms_total the whole call. */
+int fmi_plain_search1(fmi_plain *p, int mode, const uint8_t *reads, const uint64_t *offsets,
+                      uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
+                      double *ms_kernel, double *ms_total);
+/* The layout of one direction: desp[b * words + w], count[b * words + w]
+ * with words = ceil((n + 2) / 64) (calculate_O, search/preprocess.c:74-130). */
+uint64_t fmi_plain_words(const fmi_plain *p);
+int fmi_plain_get_layout(const fmi_plain *p, int forward, uint32_t *desp, uint64_t *count);
+uint64_t fmi_plain_device_bytes(const fmi_plain *p);
+int fmi_plain_free(fmi_plain *p);
+
 /* ------------------------------------------------------ measurement */
 /* Random gather of line_bytes-sized (16, 32, 64, 128 or 256) aligned lines
  * over d_buf (bytes >= 64 MiB), at least nlines loads: the random-access roofline of the path.
