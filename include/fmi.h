@@ -99,8 +99,9 @@ int fmi_index_build(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_inde
  * (no fmi_index_to_device needed); host copies are kept for saving.  Needs
  * about 30 bytes of free device memory per base (3.1 Gbp: ~95 GB). */
 int fmi_index_build_gpu(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_index **out);
-/* Writes / reads the single-file index (see DESIGN: versioned header,
- * occurrence blocks, samples, karyotype).  Replaces the reference's index
+/* Writes / reads the single-file index ("HPGFMI01" magic and format
+ * version, then per direction the '$' row, C, the occurrence blocks and the
+ * SA / ISA samples, then the karyotype, with an FNV-1a checksum).  Replaces the reference's index
  * directory reader load_bwt_index (search/runtime.h:90-124). */
 int fmi_index_save(const fmi_index *ix, const char *path);
 int fmi_index_load(const char *path, fmi_index **out);
