@@ -198,7 +198,7 @@ int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
 /* The mapping layer's 1-error search: BWSearch1VectorHelper
  * (search/search.c:1491-1709) driven by the interval vectors of
  * BWExactSearchVectorBackward/Forward, per strand as __bwt_map_inexact_read
- * calls it (bwt.c:1940-1950, 1965-1972; + strand backward/forward, - strand
+ * calls it (bwt.c:1940-1950, 1994-2003; + strand backward/forward, - strand
  * forward_rev/backward_rev).  d_bounds is NULL (the whole read, start 0) or
  * int16 [start, end] per read, relative to the read, as bwt_map_inexact_seed
  * passes a seed window (bwt.c:1447-1479).  Records and capacity as
@@ -301,7 +301,8 @@ int fmi_seed_map_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64
  * (device_budget_bytes) has room, coalesces up to max_batch_reads reads per
  * launch, keeps up to max_inflight launches in flight on separate streams
  * (requests that arrive meanwhile join the next launch as memory frees up),
- * and scatters each request's records back to the buffers given at submit.
+ * compacts the stored records on the device and copies back only those;
+ * fmi_server_wait / fmi_server_fetch hand each request its slice.
  * Each request gets exactly the records fmi_search1_batch gives for its reads
  * alone. */
 typedef struct fmi_server fmi_server;

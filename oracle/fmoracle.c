@@ -494,8 +494,8 @@ static void vh_exact(const fmo_dir *d, int comp, int dir, const uint8_t *W, int1
 uint32_t fmo_search1_vector(const fmo_index *ix, int strand, int mismatch_only,
                             const uint8_t *W, int16_t start, int16_t end, fmo_hit *out,
                             uint32_t cap) {
-  /* + strand: vectors on (backward, forward), bwt.c:1940-1944,1967;
-   * - strand: (forward_rev, backward_rev), bwt.c:1946-1950,1971. */
+  /* + strand: vectors on (backward, forward), bwt.c:1940-1944,1996;
+   * - strand: (forward_rev, backward_rev), bwt.c:1946-1950,2000. */
   const int vb = strand ? FMO_VIEW_FORWARD_REV : FMO_VIEW_BACKWARD;
   const int vf = strand ? FMO_VIEW_BACKWARD_REV : FMO_VIEW_FORWARD;
   const fmo_dir *db = view_dir(ix, vb), *df = view_dir(ix, vf);

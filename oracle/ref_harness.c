@@ -260,7 +260,7 @@ uint32_t ref_search1(void *h, int strand, const uint8_t *W, int16_t len, uint32_
 
 /* The mapping layer's 1-error search of W[start..end] on one strand: the
  * interval vectors, then BWSearch1VectorHelper, as bwt.c:1940-1950 +
- * 1965-1972 (start = 0) and bwt.c:1447-1457 + 1465-1479 (a seed). */
+ * 1994-2003 (start = 0) and bwt.c:1447-1457 + 1465-1479 (a seed). */
 uint32_t ref_search1_vector(void *h, int strand, const uint8_t *W, int16_t start, int16_t end,
                             uint32_t cap, ref_hit *out, int *full) {
   ref_ctx *c = (ref_ctx *)h;

@@ -17,7 +17,7 @@ inputs and outputs as data:
     arrays_sha        SHA-256 of the reference index arrays (C, C1, desp,
                       count, S, R for both directions)
     s1v_counts/hits   BWSearch1VectorHelper over the interval vectors of each
-                      whole read (bwt.c:1940-1972), per strand
+                      whole read (bwt.c:1940-1950, 1994-2003), per strand
     s1w_bounds        one window [start, end] per read (a seed, bwt.c:1447)
     s1w_counts/hits   the same search over that window
 

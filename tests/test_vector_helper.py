@@ -1,6 +1,6 @@
 """The mapping layer's 1-error search, BWSearch1VectorHelper
 (search/search.c:1491-1709) over the interval vectors of
-BWExactSearchVectorBackward/Forward (bwt.c:1940-1972; seed windows
+BWExactSearchVectorBackward/Forward (bwt.c:1940-1950, 1994-2003; seed windows
 bwt.c:1447-1479).
 
 CPU: the oracle against the golden vectors of the compiled reference (whole
