@@ -99,13 +99,15 @@ def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000
 # Measured HBM traffic of one timed launch, from the committed rocprofv3
 # FETCH_SIZE / WRITE_SIZE summaries (scaled by the copy-kernel calibration).
 TRAFFIC_FILES = {
-    ("c2", 1_000_000, 150): "search1_mismatch_64Mbp_1Mx150bp_pmc_traffic.json",
-    ("t3g", 2_000_000, 150): "search1_mismatch_3100Mbp_2Mx150bp_pmc_traffic.json",
+    ("c2", 1_000_000, 150, False): "search1_mismatch_64Mbp_1Mx150bp_pmc_traffic.json",
+    ("t3g", 2_000_000, 150, False): "search1_mismatch_3100Mbp_2Mx150bp_pmc_traffic.json",
+    ("c2", 1_000_000, 150, True): "search1_textmode_64Mbp_1Mx150bp_pmc_traffic.json",
+    ("t3g", 2_000_000, 150, True): "search1_textmode_3100Mbp_2Mx150bp_pmc_traffic.json",
 }
 
 
-def measured_traffic(preset, reads, read_len):
-    name = TRAFFIC_FILES.get((preset, reads, read_len))
+def measured_traffic(preset, reads, read_len, text_mode):
+    name = TRAFFIC_FILES.get((preset, reads, read_len, text_mode))
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
         return None
@@ -134,6 +136,8 @@ def main():
                     help="keep the library's default launch shape")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="length of each serving run (0 skips serving)")
+    ap.add_argument("--no-expand", action="store_true",
+                    help="search the compact index only (no full SA / ISA / text on the device)")
     ap.add_argument("--tuning", default=None,
                     help="wg_per_cu,chunk,refill_min of the persistent launch (skips the "
                          "autotuner)")
@@ -187,8 +191,31 @@ def main():
     mode = hpgfm.MODE_MISMATCH
     out = hpgfm.Search1Output(batch.nreads, args.cap)
 
-    # algorithmic bytes of one launch: distinct 64-byte blocks per LF query
-    blocks, lf_steps = hpgfm.search1_count_blocks(ix, batch, mode=mode)
+    block_mode = None
+    if not args.no_expand:
+        if not args.no_extras:
+            # the same search on the compact index alone (occurrence blocks
+            # for every step), default launch shape, for comparison
+            hpgfm.search1_launch(ix, batch, out, mode=mode)
+            hpgfm.sync()
+            with hpgfm.TimingRegion() as tb:
+                for _ in range(3):
+                    hpgfm.search1_launch(ix, batch, out, mode=mode)
+                hpgfm.sync()
+            block_mode = {"kernel_ms": round(tb.mean_ms, 4),
+                          "reads_per_s": round(args.reads / (tb.mean_ms / 1000.0))}
+            log(f"block mode (compact index): {tb.mean_ms:.3f} ms")
+        t0 = time.time()
+        ix.expand()
+        hpgfm.sync()
+        log(f"index expanded (full SA / ISA of both directions + 2-bit text) in "
+            f"{time.time() - t0:.2f}s, {ix.device_bytes / 1e9:.2f} GB on device")
+
+    # algorithmic bytes of one launch: distinct 64-byte occurrence blocks per
+    # LF query, 64 bytes per SA / ISA entry read, 8 bytes (two text words)
+    # per text run of up to 16 bases
+    blocks, lf_steps, lookups, tsteps = hpgfm.search1_count_work(ix, batch, cap=args.cap,
+                                                                   mode=mode)
 
     tuning = None
     if args.tuning:
@@ -217,8 +244,8 @@ def main():
     reads_per_s = args.reads / (wall / args.steps)
     counts = out.counts().reshape(-1, 2)
     stored = int(np.minimum(counts, args.cap).sum())
-    alg_bytes = (blocks * 64 + int(offs[-1]) + 8 * (args.reads + 1) + stored * 32 +
-                 4 * 2 * args.reads)
+    alg_bytes = (blocks * 64 + lookups * 64 + tsteps * 8 + int(offs[-1]) +
+                 8 * (args.reads + 1) + stored * 32 + 4 * 2 * args.reads)
     kernel_ms = tr.mean_ms
     achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
     log(f"mismatch mode: {ms_per_step:.3f} ms/step wall, kernel {kernel_ms:.3f} ms "
@@ -399,7 +426,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": measured_traffic(args.preset, args.reads, args.read_len),
+            "traffic": measured_traffic(args.preset, args.reads, args.read_len,
+                                        not args.no_expand),
         },
         "cpu_baseline": cpu,
         "kernel": "fmi::search1_kernel<true, true, false>",
@@ -408,6 +436,10 @@ def main():
         "lf_queries_per_launch": lf_steps,
         "blocks_per_lf_query": round(blocks / max(lf_steps, 1), 4),
         "lf_queries_per_read": round(lf_steps / args.reads, 2),
+        "text_mode": not args.no_expand,
+        "sa_isa_reads_per_launch": lookups,
+        "text_runs_per_launch": tsteps,
+        "block_mode": block_mode,
         "index_device_bytes": ix.device_bytes,
         "index_cache_resident": ix.device_bytes < 256 * 2 ** 20,
         "tuning": tuning,

@@ -277,6 +277,7 @@ void fmi_index_free(fmi_index *ix) {
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(ix->device);
     (void)hipFree(ix->d_mem);
+    (void)hipFree(ix->d_full);
     (void)hipFree(ix->d_koffset);
     (void)hipFree(ix->d_kstart);
     if (cur >= 0) (void)hipSetDevice(cur);
@@ -292,7 +293,59 @@ uint64_t fmi_index_device_bytes(const fmi_index *ix) {
   uint64_t b = 0;
   for (int d = 0; d < 2; ++d)
     b += ix->h[d].nblocks * sizeof(fmi::OccBlock) + 2 * ((ix->h[d].nsamp * 4 + 255) & ~255ull);
-  return b;
+  return b + ix->full_bytes;
+}
+
+int fmi_index_expand(fmi_index *ix) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (ix->d_full) return FMI_OK;
+  if (ix->ratio == 0) return fail(FMI_ESTATE, "the index has no samples");
+  const uint64_t n1 = ix->n + 1;
+  const uint64_t arr = (n1 * 4 + 255) & ~255ull;
+  const uint64_t tbytes = (((ix->n + 15) / 16) * 4 + 255) & ~255ull;
+  const uint64_t bytes = 4 * arr + tbytes;
+  void *mem = nullptr;
+  hipError_t e = hipMalloc(&mem, bytes);
+  if (e != hipSuccess)
+    return fail(FMI_ENOMEM, "hipMalloc(full SA / ISA / text, " + std::to_string(bytes) + " bytes)");
+  char *p = static_cast<char *>(mem);
+  uint32_t *sa[2], *isa[2];
+  for (int d = 0; d < 2; ++d) {
+    sa[d] = reinterpret_cast<uint32_t *>(p); p += arr;
+    isa[d] = reinterpret_cast<uint32_t *>(p); p += arr;
+  }
+  uint32_t *text = reinterpret_cast<uint32_t *>(p);
+  for (int d = 0; d < 2 && e == hipSuccess; ++d)
+    e = fmi::launch_expand(ix->dev.d[d], (uint32_t)ix->n, ix->ratio, sa[d], isa[d], nullptr);
+  if (e == hipSuccess) e = fmi::launch_pack_text(ix->dev.d[0], (uint32_t)ix->n, isa[0], text, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(mem);
+    return hip_fail(e, "expand launch");
+  }
+  for (int d = 0; d < 2; ++d) {
+    ix->dev.d[d].sa = sa[d];
+    ix->dev.d[d].isa = isa[d];
+  }
+  ix->dev.text = text;
+  ix->d_full = mem;
+  ix->full_bytes = bytes;
+  return FMI_OK;
+}
+
+int fmi_index_get_full(const fmi_index *ix, int forward, uint32_t *sa, uint32_t *isa,
+                       uint32_t *text) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (!ix->d_full) return fail(FMI_ESTATE, "call fmi_index_expand first");
+  const int d = forward ? 1 : 0;
+  const uint64_t n1 = ix->n + 1;
+  if (sa) HIP_TRY(hipMemcpy(sa, ix->dev.d[d].sa, n1 * 4, hipMemcpyDeviceToHost), "copy SA");
+  if (isa) HIP_TRY(hipMemcpy(isa, ix->dev.d[d].isa, n1 * 4, hipMemcpyDeviceToHost), "copy ISA");
+  if (text)
+    HIP_TRY(hipMemcpy(text, ix->dev.text, ((ix->n + 15) / 16) * 4, hipMemcpyDeviceToHost), "copy text");
+  return FMI_OK;
 }
 
 int fmi_index_to_device(fmi_index *ix) {
@@ -742,17 +795,17 @@ int fmi_copy_bench(const void *d_src, void *d_dst, uint64_t bytes, float *ms, fm
   return fmi_last_kernel_ms(ms);
 }
 
-int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
-                             const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
-                             uint64_t *blocks, uint64_t *lf_steps) {
+int fmi_search1_count_work(const fmi_index *ix, int mode, int strand_mask,
+                           const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                           uint32_t cap, uint64_t *work) {
   int rc = need_device(ix);
   if (rc) return rc;
-  if (!blocks || !lf_steps) return fail(FMI_EINVAL, "NULL argument");
+  if (!work) return fail(FMI_EINVAL, "NULL argument");
   if (mode != FMI_MODE_EDIT && mode != FMI_MODE_MISMATCH) return fail(FMI_EINVAL, "bad mode");
   if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "bad strand_mask");
-  *blocks = 0;
-  *lf_steps = 0;
+  for (int i = 0; i < 4; ++i) work[i] = 0;
   if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets) return fail(FMI_EINVAL, "NULL buffer");
   hipStream_t st = nullptr;
   Workspace w;
   rc = workspace_for(st, &w);
@@ -763,19 +816,29 @@ int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
   rc = cu_count(&cus);
   if (rc) { (void)hipFree(counts); return rc; }
   HIP_TRY(fmi::launch_zero(w.queue, 1, st), "zero queue");
-  HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 4, st), "zero stats");
+  HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 8, st), "zero stats");
   hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, true, true,
-                                     (uint32_t)strand_mask, d_reads, d_offsets, nreads, 0, nullptr,
+                                     (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap, nullptr,
                                      counts, w.queue, w.stats, (uint32_t)cus * kDefaultWgPerCu,
                                      kDefaultChunk, kDefaultRefill, st);
   if (e != hipSuccess) { (void)hipFree(counts); return hip_fail(e, "count launch"); }
-  unsigned long long st2[2] = {0, 0};
-  e = hipMemcpy(st2, w.stats, 16, hipMemcpyDeviceToHost);
+  unsigned long long st4[4] = {0, 0, 0, 0};
+  e = hipMemcpy(st4, w.stats, 32, hipMemcpyDeviceToHost);
   (void)hipFree(counts);
   if (e != hipSuccess) return hip_fail(e, "copy stats");
-  *blocks = st2[0];
-  *lf_steps = st2[1];
+  for (int i = 0; i < 4; ++i) work[i] = st4[i];
   return FMI_OK;
+}
+
+int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
+                             const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                             uint64_t *blocks, uint64_t *lf_steps) {
+  if (!blocks || !lf_steps) return fail(FMI_EINVAL, "NULL argument");
+  uint64_t work[4];
+  int rc = fmi_search1_count_work(ix, mode, strand_mask, d_reads, d_offsets, nreads, 0, work);
+  *blocks = work[0];
+  *lf_steps = work[1];
+  return rc;
 }
 
 }  // extern "C"

@@ -23,6 +23,8 @@ struct fmi_index {
   uint64_t *d_koffset = nullptr;  // karyotype offsets (count + 1)
   uint64_t *d_kstart = nullptr;
   uint64_t device_bytes = 0;
+  void *d_full = nullptr;         // full SA / ISA of both directions + 2-bit text
+  uint64_t full_bytes = 0;
   fmi::DevIndex dev{};
 
   ~fmi_index();

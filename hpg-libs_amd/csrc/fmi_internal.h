@@ -40,6 +40,8 @@ struct DevDir {
   const OccBlock *blk;   // nblocks
   const uint32_t *S;     // S[i / ratio] = SA[i] for rows i % ratio == 0
   const uint32_t *R;     // R[j / ratio] = ISA[j] for text positions j % ratio == 0
+  const uint32_t *sa;    // full SA (n + 1), or NULL: see fmi_index_expand
+  const uint32_t *isa;   // full ISA (n + 1), or NULL
   uint32_t C[4];         // # symbols < b
   uint32_t dollar;       // row holding '$'
   uint32_t dollar_blk;   // dollar / 192
@@ -49,6 +51,8 @@ struct DevDir {
 
 struct DevIndex {
   DevDir d[2];           // 0: BWT of X ("backward"), 1: BWT of reverse X ("forward")
+  const uint32_t *text;  // X, 2 bits per base, 16 bases per word (base j: word j/16,
+                         // bits 2(j%16)..+1), or NULL; set with the full SA / ISA
   uint32_t n;            // text length; rows are 0..n
   uint32_t ratio;        // SA / ISA sample rate
 };

@@ -38,6 +38,12 @@ hipError_t launch_vhelper(const DevIndex &ix, bool mm_only, uint32_t strand_mask
 hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
                          uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
                          hipStream_t s);
+// Full SA / ISA of one direction from its samples (sa, isa: n + 1 entries).
+hipError_t launch_expand(const DevDir &D, uint32_t n, uint32_t ratio, uint32_t *sa, uint32_t *isa,
+                         hipStream_t s);
+// X packed 2 bits per base ((n + 15) / 16 words) from the backward direction.
+hipError_t launch_pack_text(const DevDir &D, uint32_t n, const uint32_t *isa, uint32_t *text,
+                            hipStream_t s);
 hipError_t launch_isa(const DevIndex &ix, uint32_t dsel, const uint64_t *tpos, uint64_t count,
                       uint64_t *rows, hipStream_t s);
 hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_t *counts,

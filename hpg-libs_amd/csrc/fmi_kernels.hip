@@ -137,7 +137,8 @@ struct S1Params {
   fmi_hit *hits;
   uint32_t *counts;
   uint32_t *queue;                 // work counter (persistent launch)
-  unsigned long long *stats;       // [0] blocks, [1] LF queries (count mode)
+  unsigned long long *stats;       // count mode: [0] blocks, [1] LF queries,
+                                   // [2] SA / ISA reads, [3] text steps
   uint32_t ntasks, strand_mask, cap, chunk;
   uint32_t refill_min;             // idle lanes before a wave takes new tasks
 };
@@ -150,10 +151,18 @@ struct S1Lane {
   uint32_t ck[4], cl[4];           // children of the SCAN node, physical symbol order
   const OccBlock *blk;             // the current pass's direction
   uint32_t C[4], dollar_blk, dollar_off;
+  // Text mode (expanded index): a single-row interval is held as the text
+  // position q of its suffix in the pass's direction; kTqNone = rows in k, l.
+  uint32_t tq, tqn, cq;            // node, child of W[i], continuation
+  uint32_t tdir;                   // 0: X read right to left, 1: left to right
+  const uint32_t *sa, *isa;        // the pass's direction (NULL: no text mode)
+  uint32_t nlook, ntext;           // SA / ISA reads and text steps (count mode)
   fmi_hit *out;
 };
 
-template <bool MM_ONLY>
+constexpr uint32_t kTqNone = 0xFFFFFFFFu;
+
+template <bool MM_ONLY, bool COUNT>
 struct S1 {
   const S1Params &P;
   S1Lane &L;
@@ -164,9 +173,25 @@ struct S1 {
     return L.pass ? (p <= L.len / 2 - 1) : (p >= L.len / 2);
   }
 
-  __device__ __forceinline__ void emit(uint32_t k, uint32_t l, uint8_t nmm, uint8_t kind,
-                                       uint8_t base, int16_t err_pos) {
+  // Physical symbol of the text base that precedes suffix q (q >= 1) in the
+  // pass's direction: X[q-1] for the BWT of X, X[n-q] for the BWT of
+  // reverse X.
+  __device__ __forceinline__ uint32_t text_sym(uint32_t q) const {
+    const uint32_t x = L.tdir ? P.ix.n - q : q - 1u;
+    return (P.ix.text[x >> 4] >> ((x & 15u) << 1)) & 3u;
+  }
+
+  // Row of the single-row interval whose suffix starts at q (count mode
+  // counts the read the stored record would make).
+  __device__ __forceinline__ uint32_t row_of(uint32_t q) {
+    if (COUNT) { L.nlook++; return 0u; }
+    return L.isa[q];
+  }
+
+  __device__ __forceinline__ void emit(uint32_t k, uint32_t l, uint32_t q, uint8_t nmm,
+                                       uint8_t kind, uint8_t base, int16_t err_pos) {
     if (L.nout < P.cap) {
+      if (q != kTqNone) k = l = row_of(q);
       int16_t start, end, pos;
       if (nmm == 0) {               // the exact record of pass 0
         start = 0; end = (int16_t)(L.len - 1); pos = -1;
@@ -175,8 +200,9 @@ struct S1 {
       } else {                      // bound_result(half+1, end), pos = end+1
         start = (int16_t)(L.len / 2); end = (int16_t)(L.len - 1); pos = (int16_t)L.len;
       }
-      store_hit(L.out + L.nout, k, l, start, pos, end, err_pos, (uint8_t)L.pass, nmm, kind, base,
-                (uint8_t)L.strand);
+      if (!COUNT)
+        store_hit(L.out + L.nout, k, l, start, pos, end, err_pos, (uint8_t)L.pass, nmm, kind,
+                  base, (uint8_t)L.strand);
     }
     L.nout++;
   }
@@ -214,6 +240,10 @@ struct S1 {
     for (int b = 0; b < 4; ++b) L.C[b] = second ? D1.C[b] : D0.C[b];
     L.dollar_blk = second ? D1.dollar_blk : D0.dollar_blk;
     L.dollar_off = second ? D1.dollar_off : D0.dollar_off;
+    L.sa = P.ix.text ? (second ? D1.sa : D0.sa) : nullptr;
+    L.isa = second ? D1.isa : D0.isa;
+    L.tdir = second ? 1u : 0u;
+    L.tq = L.tqn = L.cq = kTqNone;
     L.comp = L.strand;
     L.k = 0;
     L.l = P.ix.n;
@@ -225,40 +255,44 @@ struct S1 {
   __device__ __forceinline__ void enter_scan() {
     L.kn = L.k;
     L.ln = L.l;
+    L.tqn = L.tq;
     L.i = L.pass ? L.len / 2 : L.len / 2 - 1;
     scan_next();
   }
 
   __device__ __forceinline__ void scan_next() {
     if (!in_read(L.i)) {
-      if (L.pass == 0 && L.kn <= L.ln) emit(L.kn, L.ln, 0, 0, 0, 0);
+      if (L.pass == 0 && L.kn <= L.ln) emit(L.kn, L.ln, L.tqn, 0, 0, 0, 0);
       end_pass();
       return;
     }
     if (L.kn > L.ln) { end_pass(); return; }
     L.k = L.kn;
     L.l = L.ln;
+    L.tq = L.tqn;
     L.phase = PH_SCAN;
   }
 
   // Next branch of the current node, from L.bi on (0: deletion; 1 + 2b:
   // insertion of b; 2 + 2b: mismatch to b).  Branches whose continuation is
-  // already past the end of the read succeed at once.
+  // already past the end of the read succeed at once.  In text mode the
+  // node's children are its suffix extended by one text base.
   __device__ __forceinline__ void branch_next() {
     while (L.bi < 9) {
       const uint32_t bi = L.bi;
       bool take = false;
       if (bi == 0) {
-        if (!MM_ONLY) { L.qk = L.k; L.ql = L.l; L.cp = L.i + step(); take = true; }
+        if (!MM_ONLY) { L.qk = L.k; L.ql = L.l; L.cq = L.tq; L.cp = L.i + step(); take = true; }
       } else {
         const uint32_t b = (bi - 1) >> 1;
         const uint32_t ps = L.comp ? 3u - b : b;
         const uint32_t cb = pick4(L.ck, ps), lb = pick4(L.cl, ps);
         const bool nonempty = cb <= lb;
+        const uint32_t cq = L.tq == kTqNone ? kTqNone : L.tq - 1u;
         if (bi & 1u) {
-          if (!MM_ONLY && nonempty) { L.qk = cb; L.ql = lb; L.cp = L.i; take = true; }
+          if (!MM_ONLY && nonempty) { L.qk = cb; L.ql = lb; L.cq = cq; L.cp = L.i; take = true; }
         } else if (nonempty && b != L.wsym) {
-          L.qk = cb; L.ql = lb; L.cp = L.i + step(); take = true;
+          L.qk = cb; L.ql = lb; L.cq = cq; L.cp = L.i + step(); take = true;
         }
       }
       if (take) {
@@ -273,9 +307,9 @@ struct S1 {
 
   __device__ __forceinline__ void emit_branch() {
     const uint32_t bi = L.bi;
-    if (bi == 0) emit(L.qk, L.ql, 1, FMI_DELETION, 0xFF, (int16_t)L.i);
-    else emit(L.qk, L.ql, 1, (bi & 1u) ? FMI_INSERTION : FMI_MISMATCH, (uint8_t)((bi - 1) >> 1),
-              (int16_t)L.i);
+    if (bi == 0) emit(L.qk, L.ql, L.cq, 1, FMI_DELETION, 0xFF, (int16_t)L.i);
+    else emit(L.qk, L.ql, L.cq, 1, (bi & 1u) ? FMI_INSERTION : FMI_MISMATCH,
+              (uint8_t)((bi - 1) >> 1), (int16_t)L.i);
   }
 
   __device__ __forceinline__ void start(uint32_t task) {
@@ -288,12 +322,127 @@ struct S1 {
     start_strand(strand);
   }
 
-  // One LF query and the transition it drives.
-  template <bool COUNT>
+  // Number of leading read bases W[p], W[p+d], ... (d = the pass's step, at
+  // most lim <= 16, lim <= q) that equal the text bases preceding suffix q
+  // in the pass's direction (X[q-1], X[q-2], ... or X[n-q], X[n-q+1], ...),
+  // compared 16 at a time as 2-bit groups.
+  __device__ __forceinline__ uint32_t text_run(uint32_t q, int32_t p, uint32_t lim) const {
+    if (lim == 0u) return 0u;
+    const int32_t d = step();
+    // the read: 2-bit physical symbols, and a mask of its N bases
+    uint32_t r2 = 0u, bad = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < 16u; ++i) {
+      const uint32_t sym = L.W[p + (int32_t)(i < lim ? i : 0u) * d];
+      const uint32_t ps = (L.comp ? (sym ^ 3u) : sym) & 3u;
+      r2 |= ps << (2u * i);
+      bad |= (sym > 3u ? 1u : 0u) << (2u * i);
+    }
+    // the text: 16 bases in the order they are compared
+    const uint32_t nwords = (P.ix.n + 15u) >> 4;
+    uint32_t t2;
+    if (L.tdir) {                    // X[x0], X[x0+1], ... with x0 = n - q
+      const uint32_t x0 = P.ix.n - q, w = x0 >> 4;
+      const uint64_t lo = P.ix.text[w];
+      const uint64_t hi = w + 1u < nwords ? (uint64_t)P.ix.text[w + 1u] : 0ull;
+      t2 = (uint32_t)(((hi << 32) | lo) >> ((x0 & 15u) << 1));
+    } else {                         // X[x0], X[x0-1], ... with x0 = q - 1
+      const int64_t b = (int64_t)q - 16;   // X[b .. b+15] = X[x0-15 .. x0]
+      uint32_t win;
+      if (b >= 0) {
+        const uint32_t w = (uint32_t)b >> 4;
+        const uint64_t lo = P.ix.text[w];
+        const uint64_t hi = w + 1u < nwords ? (uint64_t)P.ix.text[w + 1u] : 0ull;
+        win = (uint32_t)(((hi << 32) | lo) >> (((uint32_t)b & 15u) << 1));
+      } else {
+        win = (uint32_t)((uint64_t)P.ix.text[0] << (2u * (uint32_t)(-b)));
+      }
+      // reverse the 16 groups: bit reversal, then swap the bits of each pair
+      const uint32_t v = __builtin_bitreverse32(win);
+      t2 = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+    }
+    const uint32_t x = r2 ^ t2;
+    uint32_t miss = ((x | (x >> 1)) & 0x55555555u) | bad;
+    if (lim < 16u) miss |= 1u << (2u * lim);
+    return miss ? (uint32_t)__builtin_ctz(miss) >> 1 : 16u;
+  }
+
+  // One step on a text-mode interval: up to 16 read bases against the text
+  // that precedes the suffix (no occurrence block).  A single-row interval
+  // has one non-empty child, the one of the text base, so a SCAN node
+  // branches exactly where the read's base differs from the text; the
+  // matching positions before it are passed over at once.
+  __device__ __forceinline__ void text_step(bool cont, uint32_t q) {
+    const int32_t p = cont ? L.cp : L.i;
+    const int32_t d = step();
+    if (COUNT) L.ntext++;
+    if (L.phase == PH_SCAN) {
+      const uint32_t rem = (uint32_t)(d < 0 ? p + 1 : L.len - p);
+      const uint32_t lim = min(min(rem, 16u), q);
+      const uint32_t m = text_run(q, p, lim);
+      if (m > 0u) {
+        L.kn = 0; L.ln = 0; L.tqn = q - m;
+        L.i += (int32_t)m * d;
+        scan_next();
+        return;
+      }
+      const uint32_t sym = L.W[p];
+      const uint32_t tsym = q >= 1u ? text_sym(q) : 4u;
+#pragma unroll
+      for (uint32_t s = 0; s < 4; ++s) {
+        L.ck[s] = s == tsym ? 0u : 1u;
+        L.cl[s] = 0u;
+      }
+      L.wsym = sym;
+      L.kn = P.ix.n + 1; L.ln = P.ix.n; L.tqn = kTqNone;
+      L.bi = 0;
+      branch_next();
+      return;
+    }
+    uint32_t rem;
+    if (!cont) rem = (uint32_t)(L.pass ? L.len / 2 - p : p - L.len / 2 + 1);
+    else rem = (uint32_t)(d < 0 ? p + 1 : L.len - p);
+    const uint32_t lim = min(min(rem, 16u), q);
+    const uint32_t m = text_run(q, p, lim);
+    const bool died = m < lim || (m == q && m < rem);
+    if (!cont) {                     // PH_EXACT
+      if (died) { end_pass(); return; }
+      L.tq = q - m;
+      L.k = 0; L.l = 0;
+      L.i += (int32_t)m * d;
+      if (in_read(L.i) && in_exact(L.i)) return;
+      enter_scan();
+      return;
+    }
+    if (died) {                      // continuation died
+      L.bi++;
+      branch_next();
+      return;
+    }
+    L.cq = q - m;
+    L.cp += (int32_t)m * d;
+    if (in_read(L.cp)) return;
+    emit_branch();
+    L.bi++;
+    branch_next();
+  }
+
+  // One LF query (or one text step, or the SA read that enters text mode)
+  // and the transition it drives.
   __device__ __forceinline__ void advance(unsigned long long &nblk_acc, unsigned long long &nq_acc) {
     const bool cont = L.phase == PH_CONT;
+    const uint32_t tq = cont ? L.cq : L.tq;
+    if (tq != kTqNone) { text_step(cont, tq); return; }
     const uint32_t ak = cont ? L.qk : L.k;
     const uint32_t al = cont ? L.ql : L.l;
+    if (ak == al && L.sa != nullptr) {
+      // a single row: continue on the text from its suffix
+      if (COUNT) L.nlook++;
+      const uint32_t q = L.sa[ak];
+      if (cont) L.cq = q;
+      else L.tq = q;
+      return;
+    }
     const int32_t p = cont ? L.cp : L.i;
     const uint32_t sym = L.W[p];
     const uint32_t m2 = al + 1;
@@ -337,6 +486,7 @@ struct S1 {
         L.kn = P.ix.n + 1;
         L.ln = P.ix.n;
       }
+      L.tqn = kTqNone;
       if (size_node == (int64_t)L.ln - (int64_t)L.kn) {
         L.i += step();
         scan_next();
@@ -388,7 +538,9 @@ template <bool PERSISTENT, bool MM_ONLY, bool COUNT>
 __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
   S1Lane L;
   L.phase = PH_IDLE;
-  S1<MM_ONLY> m{P, L};
+  L.nlook = 0;
+  L.ntext = 0;
+  S1<MM_ONLY, COUNT> m{P, L};
   unsigned long long nblk = 0, nq = 0;
   const uint32_t lane = lane_id();
 
@@ -397,7 +549,7 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
     if (t < P.ntasks) m.start(t);
     while (L.phase != PH_IDLE) {
       if (L.phase == PH_NEXT_STRAND) m.start_strand(1u);
-      else m.template advance<COUNT>(nblk, nq);
+      else m.advance(nblk, nq);
     }
   } else {
     // Wavefront-level work queue: the wave owns tasks [wq_next, wq_end) and
@@ -430,12 +582,14 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
       }
       if (__ballot(L.phase != PH_IDLE) == 0ull) break;
       if (L.phase == PH_NEXT_STRAND) m.start_strand(1u);
-      else if (L.phase != PH_IDLE) m.template advance<COUNT>(nblk, nq);
+      else if (L.phase != PH_IDLE) m.advance(nblk, nq);
     }
   }
   if (COUNT) {
     atomicAdd(&P.stats[0], nblk);
     atomicAdd(&P.stats[1], nq);
+    atomicAdd(&P.stats[2], (unsigned long long)L.nlook);
+    atomicAdd(&P.stats[3], (unsigned long long)L.ntext);
   }
 }
 
@@ -752,6 +906,59 @@ __global__ __launch_bounds__(256) void isa_kernel(IsaParams P) {
 }
 
 // --------------------------------------------------------------------------
+// full SA / ISA and 2-bit text from the samples (fmi_index_expand)
+// --------------------------------------------------------------------------
+// Walk w starts at the ISA sample of text position w * ratio and steps LF
+// back over the ratio - 1 positions below it (position 0 alone for w = 0);
+// one more walk starts at row 0 (the suffix "$" at position n) and covers
+// the positions above the last sample.  Every position, and so every row, is
+// written exactly once.
+
+__global__ __launch_bounds__(256) void expand_walk_kernel(DevDir D, uint32_t n, uint32_t ratio,
+                                                          uint32_t nsamp, uint32_t *sa,
+                                                          uint32_t *isa) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > nsamp) return;
+  uint32_t pos, row, cnt;
+  if (w < nsamp) {
+    pos = w * ratio;
+    row = D.R[w];
+    cnt = w == 0 ? 1u : ratio;
+  } else {
+    pos = n;
+    row = 0;
+    cnt = n - (nsamp - 1) * ratio;
+  }
+  for (uint32_t t = 0; t < cnt; ++t) {
+    isa[pos] = row;
+    sa[row] = pos;
+    if (t + 1 == cnt) break;
+    // row of suffix pos - 1: its BWT symbol is X[pos - 1] (pos >= 1 here)
+    const uint32_t j = row / kBlockRows, o = row - j * kBlockRows;
+    const Blk B = load_block(D.blk, j);
+    const uint32_t c = code_at(B, o);
+    row = pick4(D.C, c) + occ_blk(B, D.dollar_blk, D.dollar_off, j, c, o) + 1u;
+    --pos;
+  }
+}
+
+// X[j] is the BWT symbol of the row of suffix j + 1.
+__global__ __launch_bounds__(256) void pack_text_kernel(DevDir D, uint32_t n, const uint32_t *isa,
+                                                        uint32_t *text, uint32_t nwords) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwords) return;
+  uint32_t word = 0;
+  for (uint32_t t = 0; t < 16u; ++t) {
+    const uint32_t x = w * 16u + t;
+    if (x >= n) break;
+    const uint32_t row = isa[x + 1];
+    const uint32_t j = row / kBlockRows;
+    word |= code_at(load_block(D.blk, j), row - j * kBlockRows) << (2u * t);
+  }
+  text[w] = word;
+}
+
+// --------------------------------------------------------------------------
 // position resolution (bwt.c:2158-2166, binsearch search/io.h:43-64)
 // --------------------------------------------------------------------------
 
@@ -1023,6 +1230,23 @@ hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, co
   const uint64_t grid = (total + 255) / 256;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(locate_kernel, dim3((uint32_t)grid), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand(const DevDir &D, uint32_t n, uint32_t ratio, uint32_t *sa, uint32_t *isa,
+                         hipStream_t s) {
+  const uint32_t nsamp = n / ratio + 1;
+  const uint32_t grid = (nsamp + 1 + 255) / 256;
+  hipLaunchKernelGGL(expand_walk_kernel, dim3(grid), dim3(256), 0, s, D, n, ratio, nsamp, sa, isa);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_text(const DevDir &D, uint32_t n, const uint32_t *isa, uint32_t *text,
+                            hipStream_t s) {
+  const uint32_t nwords = (n + 15) / 16;
+  if (nwords == 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_text_kernel, dim3((nwords + 255) / 256), dim3(256), 0, s, D, n, isa, text,
+                     nwords);
   return hipGetLastError();
 }
 

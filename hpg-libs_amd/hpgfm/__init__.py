@@ -41,7 +41,8 @@ EXPORTS = [
     "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
-    "fmi_search1_vector_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
+    "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_get_full",
+    "fmi_search1_count_work", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
     "fmi_server_get_stats", "fmi_server_destroy",
@@ -113,6 +114,9 @@ def _bind(lib):
         "fmi_encode_bases": ([_p, _u64, _p], _i),
         "fmi_copy_bench": ([_p, _p, _u64, C.POINTER(C.c_float), _p], _i),
         "fmi_search1_vector_batch": ([_p, _i, _i, _p, _p, _p, _u32, _u32, _p, _p, _p], _i),
+        "fmi_index_expand": ([_p], _i),
+        "fmi_search1_count_work": ([_p, _i, _i, _p, _p, _u32, _u32, _p], _i),
+        "fmi_index_get_full": ([_p, _i, _p, _p, _p], _i),
         "fmi_plain_create": ([_p, C.POINTER(_p)], _i),
         "fmi_plain_search1": ([_p, _i, _p, _p, _u32, _u32, _p, _p, C.POINTER(C.c_double),
                                C.POINTER(C.c_double)], _i),
@@ -330,6 +334,21 @@ class Index:
         check(lib().fmi_index_to_device(self.h))
         return self
 
+    def expand(self):
+        """fmi_index_expand: full SA / ISA of both directions and the 2-bit
+        text on the device, for the text-comparison continuation."""
+        check(lib().fmi_index_expand(self.h))
+        return self
+
+    def full_arrays(self, forward=0):
+        """(sa, isa, text words) of one direction after expand()."""
+        n1 = self.n + 1
+        sa = np.empty(n1, np.uint32)
+        isa = np.empty(n1, np.uint32)
+        text = np.empty(max((self.n + 15) // 16, 1), np.uint32)
+        check(lib().fmi_index_get_full(self.h, forward, _ptr(sa), _ptr(isa), _ptr(text)))
+        return sa, isa, text[:(self.n + 15) // 16]
+
     def bwt(self, forward=0):
         out = np.empty(max(self.n, 1), np.uint8)
         d = C.c_uint64()
@@ -499,6 +518,14 @@ class PlainSearch:
             self.close()
         except Exception:
             pass
+
+
+def search1_count_work(ix, batch, cap=32, mode=MODE_MISMATCH, strand_mask=3):
+    """(blocks, LF queries, SA/ISA reads, text steps) of one fmi_search1_batch."""
+    w = np.zeros(4, np.uint64)
+    check(lib().fmi_search1_count_work(ix.h, mode, strand_mask, batch.d_reads.ptr,
+                                       batch.d_offsets.ptr, batch.nreads, cap, _ptr(w)))
+    return tuple(int(x) for x in w)
 
 
 def search1_count_blocks(ix, batch, mode=MODE_MISMATCH, strand_mask=3):

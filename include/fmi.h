@@ -106,6 +106,19 @@ int fmi_index_build_gpu(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_
 int fmi_index_save(const fmi_index *ix, const char *path);
 int fmi_index_load(const char *path, fmi_index **out);
 void fmi_index_free(fmi_index *ix);
+/* Keeps the full suffix array and its inverse of both directions and the text
+ * (2 bits per base) on the device, 16 bytes per base + n / 4 (3.1 Gbp:
+ * ~50 GB), derived from the samples by LF walks on the GPU.  The 1-error
+ * searches then continue every single-row interval by comparing the read
+ * with the text at that row's suffix (one SA read to enter, one ISA read per
+ * record) instead of one occurrence-block load per base; the records are
+ * the same.  No reference counterpart (it trades HBM for dependent loads). */
+int fmi_index_expand(fmi_index *ix);
+/* Copies the expanded arrays of one direction (NULL skips one): sa, isa of
+ * n + 1 entries; text (X, 16 bases per word, base j in bits 2(j%16)) of
+ * (n + 15) / 16 words. */
+int fmi_index_get_full(const fmi_index *ix, int forward, uint32_t *sa, uint32_t *isa,
+                       uint32_t *text);
 /* size_SA (search/runtime.h:86): n + 1. */
 uint64_t fmi_index_size_sa(const fmi_index *ix);
 uint32_t fmi_index_sa_ratio(const fmi_index *ix);
@@ -388,6 +401,13 @@ int fmi_copy_bench(const void *d_src, void *d_dst, uint64_t bytes, float *ms, fm
 int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
                              const uint8_t *d_reads, const uint64_t *d_offsets,
                              uint32_t nreads, uint64_t *blocks, uint64_t *lf_steps);
+/* The same launch's whole work at capacity cap: work[0] occurrence blocks,
+ * work[1] LF queries, work[2] SA / ISA entries read (text mode, including
+ * the ISA read of every stored record), work[3] text runs (each compares
+ * up to 16 read bases with two 32-bit text words). */
+int fmi_search1_count_work(const fmi_index *ix, int mode, int strand_mask,
+                           const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                           uint32_t cap, uint64_t *work);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,86 @@
+"""Text mode (fmi_index_expand): the full SA / ISA of both directions and the
+2-bit text, derived on the GPU from the samples, and the 1-error searches
+that continue single-row intervals on the text.  The arrays must be the
+oracle's SA / ISA at every row and the genome itself; every search must give
+the golden lists and exactly the records of the unexpanded engine."""
+import numpy as np
+import pytest
+
+import fmoracle as fo
+from conftest import split_golden_hits
+from hpgfm import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def unpack_text(words, n):
+    sh = (np.arange(16, dtype=np.uint32) * 2)[None, :]
+    return ((words[:, None] >> sh) & 3).astype(np.uint8).reshape(-1)[:n]
+
+
+@pytest.fixture(scope="module", params=["golden_small", "golden_repeats"])
+def tgold(request, gpu_engine):
+    g = request.getfixturevalue(request.param)
+    ix = gpu_engine.Index.build(g["genome"], int(g["ratio"])).to_device()
+    base = ix.device_bytes
+    ix.expand()
+    assert ix.device_bytes > base
+    batch = gpu_engine.ReadBatch(g["reads"], g["offsets"])
+    yield gpu_engine, g, ix, batch
+    batch.free()
+    ix.close()
+
+
+def test_expanded_arrays(tgold):
+    hp, g, ix, batch = tgold
+    o = fo.Oracle(g["genome"], ratio=int(g["ratio"]))
+    n = len(g["genome"])
+    for fwd in (0, 1):
+        sa, isa, text = ix.full_arrays(fwd)
+        assert [int(x) for x in sa] == [o.get_sa(fwd, m) for m in range(n + 1)]
+        assert np.array_equal(isa[sa], np.arange(n + 1, dtype=np.uint32))
+        assert np.array_equal(unpack_text(text, n), g["genome"])
+
+
+@pytest.mark.parametrize("plain", [False, True])
+@pytest.mark.parametrize("mode", ["edit", "mismatch"])
+def test_textmode_search1_golden(tgold, plain, mode):
+    hp, g, ix, batch = tgold
+    cap = int(g["cap"])
+    m = hp.MODE_EDIT if mode == "edit" else hp.MODE_MISMATCH
+    counts, hits = hp.search1_batch(ix, batch, cap=cap, mode=m, plain=plain)
+    for (i, s), w in split_golden_hits(g).items():
+        if mode == "mismatch":
+            w = w[(w["num_mismatches"] == 0) | (w["err_kind"] == fo.MISMATCH)]
+        assert int(counts[i, s]) == len(w), (i, s)
+        assert fo.hits_equal(hits[i, s, :min(len(w), cap)], w[:cap]), (i, s)
+
+
+def test_textmode_equals_block_mode(gpu_engine):
+    """Reads with N bases, ragged lengths, dense repeats: the expanded and
+    the plain index give identical counts and records, at a capacity that
+    truncates, on both strands and each strand alone."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(600_000, seed=91, repeat_frac=0.4, fam_len=(300, 3000),
+                           divergence=(0.0, 0.05), copies_per_family=12)
+    a = hp.Index.build(g, 16).to_device()
+    b = hp.Index.build(g, 16).to_device().expand()
+    lens = np.random.default_rng(92).integers(0, 251, 4000)
+    reads, offs, _ = synth.reads(g, 4000, seed=93, sub_rate=0.015, n_rate=0.002, lengths=lens)
+    batch = hp.ReadBatch(reads, offs)
+    for m in (hp.MODE_EDIT, hp.MODE_MISMATCH):
+        for mask in (3, 1, 2):
+            c1, h1 = hp.search1_batch(ix=a, batch=batch, cap=6, mode=m, strand_mask=mask)
+            c2, h2 = hp.search1_batch(ix=b, batch=batch, cap=6, mode=m, strand_mask=mask)
+            assert np.array_equal(c1, c2), (m, mask)
+            n = np.minimum(c1, 6)
+            for i in range(len(lens)):
+                for s in (0, 1):
+                    assert fo.hits_equal(h1[i, s, :n[i, s]], h2[i, s, :n[i, s]]), (m, mask, i, s)
+    wa = hp.search1_count_work(a, batch, cap=6)
+    wb = hp.search1_count_work(b, batch, cap=6)
+    assert wa[2] == 0 and wa[3] == 0
+    assert wb[0] < wa[0] and wb[2] > 0 and wb[3] > 0
+    batch.free()
+    a.close()
+    b.close()
