@@ -149,15 +149,11 @@ struct S1Lane {
   int32_t len, i, cp;
   uint32_t k, l, kn, ln, qk, ql, wsym;
   uint32_t ck[4], cl[4];           // children of the SCAN node, physical symbol order
-  const OccBlock *blk;             // the current pass's direction
-  uint32_t C[4], dollar_blk, dollar_off;
+  uint32_t dsel;                   // the pass's direction: 0 BWT of X, 1 of reverse X
   // Text mode (expanded index): a single-row interval is held as the text
   // position q of its suffix in the pass's direction; kTqNone = rows in k, l.
   uint32_t tq, tqn, cq;            // node, child of W[i], continuation
-  uint32_t tdir;                   // 0: X read right to left, 1: left to right
-  const uint32_t *sa, *isa;        // the pass's direction (NULL: no text mode)
   uint32_t nlook, ntext;           // SA / ISA reads and text steps (count mode)
-  fmi_hit *out;
 };
 
 constexpr uint32_t kTqNone = 0xFFFFFFFFu;
@@ -173,11 +169,21 @@ struct S1 {
     return L.pass ? (p <= L.len / 2 - 1) : (p >= L.len / 2);
   }
 
+  // C[s] of the pass's direction (s may differ per lane).
+  __device__ __forceinline__ uint32_t c_of(uint32_t s) const {
+    const uint32_t a0 = L.dsel ? P.ix.d[1].C[0] : P.ix.d[0].C[0];
+    const uint32_t a1 = L.dsel ? P.ix.d[1].C[1] : P.ix.d[0].C[1];
+    const uint32_t a2 = L.dsel ? P.ix.d[1].C[2] : P.ix.d[0].C[2];
+    const uint32_t a3 = L.dsel ? P.ix.d[1].C[3] : P.ix.d[0].C[3];
+    const uint32_t lo = (s & 1u) ? a1 : a0, hi = (s & 1u) ? a3 : a2;
+    return (s & 2u) ? hi : lo;
+  }
+
   // Physical symbol of the text base that precedes suffix q (q >= 1) in the
   // pass's direction: X[q-1] for the BWT of X, X[n-q] for the BWT of
   // reverse X.
   __device__ __forceinline__ uint32_t text_sym(uint32_t q) const {
-    const uint32_t x = L.tdir ? P.ix.n - q : q - 1u;
+    const uint32_t x = L.dsel ? P.ix.n - q : q - 1u;
     return (P.ix.text[x >> 4] >> ((x & 15u) << 1)) & 3u;
   }
 
@@ -185,7 +191,7 @@ struct S1 {
   // counts the read the stored record would make).
   __device__ __forceinline__ uint32_t row_of(uint32_t q) {
     if (COUNT) { L.nlook++; return 0u; }
-    return L.isa[q];
+    return (L.dsel ? P.ix.d[1].isa : P.ix.d[0].isa)[q];
   }
 
   __device__ __forceinline__ void emit(uint32_t k, uint32_t l, uint32_t q, uint8_t nmm,
@@ -201,7 +207,8 @@ struct S1 {
         start = (int16_t)(L.len / 2); end = (int16_t)(L.len - 1); pos = (int16_t)L.len;
       }
       if (!COUNT)
-        store_hit(L.out + L.nout, k, l, start, pos, end, err_pos, (uint8_t)L.pass, nmm, kind,
+        store_hit(P.hits + (size_t)(2 * L.read + L.strand) * P.cap + L.nout, k, l, start, pos,
+                  end, err_pos, (uint8_t)L.pass, nmm, kind,
                   base, (uint8_t)L.strand);
     }
     L.nout++;
@@ -220,7 +227,6 @@ struct S1 {
 
   __device__ __forceinline__ void start_strand(uint32_t strand) {
     L.strand = strand;
-    L.out = P.hits + (size_t)(2 * L.read + strand) * P.cap;
     L.nout = 0;
     enter_pass(0);
   }
@@ -234,15 +240,7 @@ struct S1 {
     L.pass = pass;
     // strand 0: (backward, forward); strand 1: (forward_rev, backward_rev)
     const bool second = (pass ^ L.strand) != 0u;
-    const DevDir &D0 = P.ix.d[0], &D1 = P.ix.d[1];
-    L.blk = second ? D1.blk : D0.blk;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) L.C[b] = second ? D1.C[b] : D0.C[b];
-    L.dollar_blk = second ? D1.dollar_blk : D0.dollar_blk;
-    L.dollar_off = second ? D1.dollar_off : D0.dollar_off;
-    L.sa = P.ix.text ? (second ? D1.sa : D0.sa) : nullptr;
-    L.isa = second ? D1.isa : D0.isa;
-    L.tdir = second ? 1u : 0u;
+    L.dsel = second ? 1u : 0u;
     L.tq = L.tqn = L.cq = kTqNone;
     L.comp = L.strand;
     L.k = 0;
@@ -341,7 +339,7 @@ struct S1 {
     // the text: 16 bases in the order they are compared
     const uint32_t nwords = (P.ix.n + 15u) >> 4;
     uint32_t t2;
-    if (L.tdir) {                    // X[x0], X[x0+1], ... with x0 = n - q
+    if (L.dsel) {                    // X[x0], X[x0+1], ... with x0 = n - q
       const uint32_t x0 = P.ix.n - q, w = x0 >> 4;
       const uint64_t lo = P.ix.text[w];
       const uint64_t hi = w + 1u < nwords ? (uint64_t)P.ix.text[w + 1u] : 0ull;
@@ -435,10 +433,10 @@ struct S1 {
     if (tq != kTqNone) { text_step(cont, tq); return; }
     const uint32_t ak = cont ? L.qk : L.k;
     const uint32_t al = cont ? L.ql : L.l;
-    if (ak == al && L.sa != nullptr) {
+    if (ak == al && P.ix.text != nullptr) {
       // a single row: continue on the text from its suffix
       if (COUNT) L.nlook++;
-      const uint32_t q = L.sa[ak];
+      const uint32_t q = (L.dsel ? P.ix.d[1].sa : P.ix.d[0].sa)[ak];
       if (cont) L.cq = q;
       else L.tq = q;
       return;
@@ -452,21 +450,24 @@ struct S1 {
     // [BWT[k] == s]); otherwise the block of l+1 too when it differs.
     const bool single = ak == al;
     const bool two = !single && j2 != j1;
-    const Blk B1 = load_block(L.blk, j1);
+    const OccBlock *blk = L.dsel ? P.ix.d[1].blk : P.ix.d[0].blk;
+    const uint32_t dollar_blk = L.dsel ? P.ix.d[1].dollar_blk : P.ix.d[0].dollar_blk;
+    const uint32_t dollar_off = L.dsel ? P.ix.d[1].dollar_off : P.ix.d[0].dollar_off;
+    const Blk B1 = load_block(blk, j1);
     Blk B2;
-    if (two) B2 = load_block(L.blk, j2);
+    if (two) B2 = load_block(blk, j2);
     else B2 = B1;
     if (COUNT) { nblk_acc += two ? 2u : 1u; nq_acc += 1u; }
-    const uint32_t dfix1 = (j1 == L.dollar_blk && L.dollar_off < t1) ? 1u : 0u;
-    const uint32_t dfix2 = (j2 == L.dollar_blk && L.dollar_off < t2) ? 1u : 0u;
-    const uint32_t srow = single ? sym_at(B1, L.dollar_blk, L.dollar_off, j1, t1) : 4u;
+    const uint32_t dfix1 = (j1 == dollar_blk && dollar_off < t1) ? 1u : 0u;
+    const uint32_t dfix2 = (j2 == dollar_blk && dollar_off < t2) ? 1u : 0u;
+    const uint32_t srow = single ? sym_at(B1, dollar_blk, dollar_off, j1, t1) : 4u;
 
     if (L.phase == PH_SCAN) {
       uint32_t r1[4], r2[4];
       rank4(B1, t1, r1);
       r1[0] -= dfix1;
 #pragma unroll
-      for (uint32_t s = 0; s < 4; ++s) L.ck[s] = L.C[s] + 1u + pick4(B1.c, s) + r1[s];
+      for (uint32_t s = 0; s < 4; ++s) L.ck[s] = c_of(s) + 1u + pick4(B1.c, s) + r1[s];
       if (single) {
 #pragma unroll
         for (uint32_t s = 0; s < 4; ++s) L.cl[s] = L.ck[s] - 1u + (srow == s ? 1u : 0u);
@@ -474,7 +475,7 @@ struct S1 {
         rank4(B2, t2, r2);
         r2[0] -= dfix2;
 #pragma unroll
-        for (uint32_t s = 0; s < 4; ++s) L.cl[s] = L.C[s] + pick4(B2.c, s) + r2[s];
+        for (uint32_t s = 0; s < 4; ++s) L.cl[s] = c_of(s) + pick4(B2.c, s) + r2[s];
       }
       L.wsym = sym;
       const int64_t size_node = (int64_t)L.l - (int64_t)L.k;
@@ -500,7 +501,7 @@ struct S1 {
     uint32_t rk, rl;
     if (sym < 4u) {
       const uint32_t s = L.comp ? 3u - sym : sym;
-      const uint32_t c = pick4(L.C, s);
+      const uint32_t c = c_of(s);
       rk = c + 1u + pick4(B1.c, s) + rank1(B1, s, t1) - (s == 0u ? dfix1 : 0u);
       if (single) rl = rk - 1u + (srow == s ? 1u : 0u);
       else rl = c + pick4(B2.c, s) + rank1(B2, s, t2) - (s == 0u ? dfix2 : 0u);
