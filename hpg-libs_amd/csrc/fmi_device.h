@@ -165,6 +165,55 @@ __device__ __forceinline__ uint32_t sa_value(const DevDir &D, uint32_t ratio, ui
   return (uint32_t)(v % ((uint64_t)n + 1));
 }
 
+// --------------------------------------------------------------------------
+// text mode (expanded index)
+// --------------------------------------------------------------------------
+
+// Number of leading read bases W[p], W[p+d], ... (at most lim <= 16, lim <=
+// q) that equal the text bases preceding suffix q of direction dsel in LF
+// order (X[q-1], X[q-2], ... for the BWT of X; X[n-q], X[n-q+1], ... for the
+// BWT of reverse X), the read's bases complemented when comp is set.  The 16
+// bases are compared at once as 2-bit groups; an N base never matches.
+__device__ __forceinline__ uint32_t text_match(const DevIndex &ix, uint32_t dsel, uint32_t comp,
+                                               uint32_t q, const uint8_t *W, int32_t p, int32_t d,
+                                               uint32_t lim) {
+  if (lim == 0u) return 0u;
+  uint32_t r2 = 0u, bad = 0u;
+#pragma unroll
+  for (uint32_t i = 0; i < 16u; ++i) {
+    const uint32_t sym = W[p + (int32_t)(i < lim ? i : 0u) * d];
+    const uint32_t ps = (comp ? (sym ^ 3u) : sym) & 3u;
+    r2 |= ps << (2u * i);
+    bad |= (sym > 3u ? 1u : 0u) << (2u * i);
+  }
+  const uint32_t nwords = (ix.n + 15u) >> 4;
+  uint32_t t2;
+  if (dsel) {                        // X[x0], X[x0+1], ... with x0 = n - q
+    const uint32_t x0 = ix.n - q, w = x0 >> 4;
+    const uint64_t lo = ix.text[w];
+    const uint64_t hi = w + 1u < nwords ? (uint64_t)ix.text[w + 1u] : 0ull;
+    t2 = (uint32_t)(((hi << 32) | lo) >> ((x0 & 15u) << 1));
+  } else {                           // X[x0], X[x0-1], ... with x0 = q - 1
+    const int64_t b = (int64_t)q - 16;   // X[b .. b+15] = X[x0-15 .. x0]
+    uint32_t win;
+    if (b >= 0) {
+      const uint32_t w = (uint32_t)b >> 4;
+      const uint64_t lo = ix.text[w];
+      const uint64_t hi = w + 1u < nwords ? (uint64_t)ix.text[w + 1u] : 0ull;
+      win = (uint32_t)(((hi << 32) | lo) >> (((uint32_t)b & 15u) << 1));
+    } else {
+      win = (uint32_t)((uint64_t)ix.text[0] << (2u * (uint32_t)(-b)));
+    }
+    // reverse the 16 groups: bit reversal, then swap the bits of each pair
+    const uint32_t v = __builtin_bitreverse32(win);
+    t2 = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+  }
+  const uint32_t x = r2 ^ t2;
+  uint32_t miss = ((x | (x >> 1)) & 0x55555555u) | bad;
+  if (lim < 16u) miss |= 1u << (2u * lim);
+  return miss ? (uint32_t)__builtin_ctz(miss) >> 1 : 16u;
+}
+
 // getRcompValue (search/csafm.h:237-272): start from the ISA sample of the
 // next sampled text position at or after m (or of position 0, walking
 // around the '$' row, when there is none) and step LF back to m.

@@ -30,6 +30,10 @@ struct ExactParams {
   fmi_hit *anchor;
 };
 
+// With an expanded index, a search whose interval has narrowed to one row
+// goes on against the text at that row's suffix, 16 bases at a time; where
+// it dies, one LF step from the last live row gives the reference's empty
+// interval exactly.
 __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= P.nreads) return;
@@ -37,26 +41,40 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   const uint8_t *W = P.reads + P.offsets[r];
   const int32_t len = (int32_t)(P.offsets[r + 1] - P.offsets[r]);
   const uint32_t n = P.ix.n;
+  const int32_t d = P.direction ? 1 : -1;
   uint32_t k = 0, l = 0, k2 = 0, l2 = n, nb;
-  int32_t i;
-  if (P.direction == 0) {
-    for (i = len - 1; i >= 0; --i) {
-      const uint32_t sym = W[i];
-      if (sym > 3u) { k2 = n + 1; l2 = n; break; }
-      lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
-      if (k2 > l2) break;
-      k = k2; l = l2;
+  int32_t i = P.direction ? 0 : len - 1;
+  for (; P.direction ? (i <= len - 1) : (i >= 0); i += d) {
+    const uint32_t sym = W[i];
+    if (sym > 3u) { k2 = n + 1; l2 = n; break; }
+    if (k2 == l2 && P.ix.text != nullptr) {
+      // text mode from the single row k2 on
+      uint32_t q = D.sa[k2];
+      uint32_t rem = (uint32_t)(d < 0 ? i + 1 : len - i);
+      bool died = false;
+      while (rem > 0u) {
+        const uint32_t lim = min(min(rem, 16u), q);
+        const uint32_t m = text_match(P.ix, P.dsel, P.comp, q, W, i, d, lim);
+        q -= m;
+        rem -= m;
+        i += (int32_t)m * d;
+        if (rem > 0u && (m < lim || q == 0u || m < 16u)) { died = true; break; }
+      }
+      k = l = D.isa[q];
+      if (!died) { k2 = l2 = k; break; }
+      const uint32_t s2 = W[i];
+      if (s2 > 3u) { k2 = n + 1; l2 = n; }
+      else lf_one(D, k, l, P.comp ? 3u - s2 : s2, k2, l2, nb);
+      break;
     }
+    lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+    if (k2 > l2) break;
+    k = k2; l = l2;
+  }
+  if (P.direction == 0) {
     if (P.anchor)
       store_hit(P.anchor + r, k, l, (int16_t)(i + 1), (int16_t)i, (int16_t)(len - 1), 0, 0, 0, 0, 0, 0);
   } else {
-    for (i = 0; i <= len - 1; ++i) {
-      const uint32_t sym = W[i];
-      if (sym > 3u) { k2 = n + 1; l2 = n; break; }
-      lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
-      if (k2 > l2) break;
-      k = k2; l = l2;
-    }
     if (P.anchor)
       store_hit(P.anchor + r, k, l, 0, (int16_t)i, (int16_t)(i - 1), 0, 1, 0, 0, 0, 0);
   }
@@ -320,49 +338,8 @@ struct S1 {
     start_strand(strand);
   }
 
-  // Number of leading read bases W[p], W[p+d], ... (d = the pass's step, at
-  // most lim <= 16, lim <= q) that equal the text bases preceding suffix q
-  // in the pass's direction (X[q-1], X[q-2], ... or X[n-q], X[n-q+1], ...),
-  // compared 16 at a time as 2-bit groups.
   __device__ __forceinline__ uint32_t text_run(uint32_t q, int32_t p, uint32_t lim) const {
-    if (lim == 0u) return 0u;
-    const int32_t d = step();
-    // the read: 2-bit physical symbols, and a mask of its N bases
-    uint32_t r2 = 0u, bad = 0u;
-#pragma unroll
-    for (uint32_t i = 0; i < 16u; ++i) {
-      const uint32_t sym = L.W[p + (int32_t)(i < lim ? i : 0u) * d];
-      const uint32_t ps = (L.comp ? (sym ^ 3u) : sym) & 3u;
-      r2 |= ps << (2u * i);
-      bad |= (sym > 3u ? 1u : 0u) << (2u * i);
-    }
-    // the text: 16 bases in the order they are compared
-    const uint32_t nwords = (P.ix.n + 15u) >> 4;
-    uint32_t t2;
-    if (L.dsel) {                    // X[x0], X[x0+1], ... with x0 = n - q
-      const uint32_t x0 = P.ix.n - q, w = x0 >> 4;
-      const uint64_t lo = P.ix.text[w];
-      const uint64_t hi = w + 1u < nwords ? (uint64_t)P.ix.text[w + 1u] : 0ull;
-      t2 = (uint32_t)(((hi << 32) | lo) >> ((x0 & 15u) << 1));
-    } else {                         // X[x0], X[x0-1], ... with x0 = q - 1
-      const int64_t b = (int64_t)q - 16;   // X[b .. b+15] = X[x0-15 .. x0]
-      uint32_t win;
-      if (b >= 0) {
-        const uint32_t w = (uint32_t)b >> 4;
-        const uint64_t lo = P.ix.text[w];
-        const uint64_t hi = w + 1u < nwords ? (uint64_t)P.ix.text[w + 1u] : 0ull;
-        win = (uint32_t)(((hi << 32) | lo) >> (((uint32_t)b & 15u) << 1));
-      } else {
-        win = (uint32_t)((uint64_t)P.ix.text[0] << (2u * (uint32_t)(-b)));
-      }
-      // reverse the 16 groups: bit reversal, then swap the bits of each pair
-      const uint32_t v = __builtin_bitreverse32(win);
-      t2 = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
-    }
-    const uint32_t x = r2 ^ t2;
-    uint32_t miss = ((x | (x >> 1)) & 0x55555555u) | bad;
-    if (lim < 16u) miss |= 1u << (2u * lim);
-    return miss ? (uint32_t)__builtin_ctz(miss) >> 1 : 16u;
+    return text_match(P.ix, L.dsel, L.comp, q, L.W, p, step(), lim);
   }
 
   // One step on a text-mode interval: up to 16 read bases against the text

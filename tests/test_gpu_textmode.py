@@ -84,3 +84,42 @@ def test_textmode_equals_block_mode(gpu_engine):
     batch.free()
     a.close()
     b.close()
+
+
+def test_textmode_exact_golden(tgold):
+    """BWExactSearchBackward/Forward on the four views with anchors, the
+    expanded index: the final interval (empty ones included), the position
+    where the search stopped and the anchor equal the golden vectors."""
+    hp, g, ix, batch = tgold
+    for view in range(4):
+        for d in (0, 1):
+            rec, anc = hp.exact_batch(ix, batch, view, d, anchors=True)
+            want = g[f"ex_v{view}_d{d}"]
+            wa = g[f"ex_anchor_v{view}_d{d}"]
+            for key in ("k", "l", "pos"):
+                assert np.array_equal(rec[key], want[key]), (view, d, key)
+            for key in ("k", "l", "start", "pos", "end", "dir"):
+                assert np.array_equal(anc[key], wa[key]), (view, d, key)
+
+
+def test_textmode_exact_with_n(gpu_engine):
+    """Reads with N bases and ragged lengths: the expanded index gives the
+    same exact records and anchors as the compact one on every view."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(300_000, seed=95, repeat_frac=0.3, fam_len=(300, 3000))
+    a = hp.Index.build(g, 16).to_device()
+    b = hp.Index.build(g, 16).to_device().expand()
+    lens = np.random.default_rng(96).integers(0, 251, 3000)
+    reads, offs, _ = synth.reads(g, 3000, seed=97, sub_rate=0.01, n_rate=0.003, lengths=lens)
+    batch = hp.ReadBatch(reads, offs)
+    for view in range(4):
+        for d in (0, 1):
+            r1, a1 = hp.exact_batch(a, batch, view, d, anchors=True)
+            r2, a2 = hp.exact_batch(b, batch, view, d, anchors=True)
+            for key in ("k", "l", "pos"):
+                assert np.array_equal(r1[key], r2[key]), (view, d, key)
+            for key in ("k", "l", "start", "pos", "end"):
+                assert np.array_equal(a1[key], a2[key]), (view, d, key)
+    batch.free()
+    a.close()
+    b.close()
