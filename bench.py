@@ -179,6 +179,10 @@ def main():
     log(f"index built on the GPU in {time.time() - t0:.1f}s "
         f"({ix.device_bytes / 1e6:.1f} MB on device)")
     if args.workload == "seeds":
+        if not args.no_expand:
+            t0 = time.time()
+            ix.expand()
+            log(f"index expanded in {time.time() - t0:.2f}s ({ix.device_bytes / 1e9:.2f} GB)")
         bench_seeds(args, pre, g, ix, hpgfm, synth)
         return
 
@@ -394,6 +398,11 @@ def main():
             served["runs"].append(r)
             log(f"served at {rate} req/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
                 f"{r['reads_per_s'] / 1e6:.2f} M reads/s over {r['requests']} requests")
+        sat = serve.run_closed(srv, pool, nrequests=400, window=16)
+        sat.pop("server", None)
+        served["saturated"] = sat
+        log(f"served closed loop (16 requests in flight): {sat['reads_per_s'] / 1e6:.2f} M reads/s, "
+            f"p50 {sat['p50_ms']} ms, p99 {sat['p99_ms']} ms")
         served["server_stats"] = srv.stats()
         srv.close()
 
@@ -451,7 +460,8 @@ def main():
         res["served"] = served
         prd = extras.get("plain_reference_design")
         if prd:
-            best = max(r["reads_per_s"] for r in served["runs"])
+            best = max([r["reads_per_s"] for r in served["runs"]] +
+                       [served["saturated"]["reads_per_s"]])
             prd["served_speedup_end_to_end"] = round(best / prd["end_to_end_reads_per_s"], 2)
     res.update(extras)
     out.free()

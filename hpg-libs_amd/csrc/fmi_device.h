@@ -149,6 +149,7 @@ __device__ __forceinline__ void store_hit(fmi_hit *dst, int64_t k, int64_t l, in
 
 __device__ __forceinline__ uint32_t sa_value(const DevDir &D, uint32_t ratio, uint32_t n,
                                              uint32_t row) {
+  if (D.sa) return D.sa[row];      // expanded index: the full suffix array
   uint32_t i = row, steps = 0;
   while (i % ratio) {
     if (i == D.dollar) {
@@ -219,6 +220,7 @@ __device__ __forceinline__ uint32_t text_match(const DevIndex &ix, uint32_t dsel
 // around the '$' row, when there is none) and step LF back to m.
 __device__ __forceinline__ uint32_t isa_value(const DevDir &D, uint32_t ratio, uint32_t n,
                                               uint32_t m) {
+  if (D.isa) return D.isa[m];      // expanded index: the full inverse
   const uint64_t siz = (uint64_t)n + 1;
   uint64_t i = (ratio - (m % ratio)) % ratio;
   const uint64_t k = (uint64_t)m + i;

@@ -86,3 +86,34 @@ def run(server, pool, rate, duration_s, req_reads=(1, 50_000), seed=9, keep=Fals
     if keep:
         out["kept"] = kept
     return out
+
+
+def run_closed(server, pool, nrequests, window=16, req_reads=(1, 50_000), seed=11):
+    """Closed loop: keeps `window` requests in flight until `nrequests` have
+    been served, so the server runs at its capacity without an unbounded
+    backlog.  Returns reads/s and the latency distribution."""
+    reads, offs = pool
+    npool = len(offs) - 1
+    rng = np.random.default_rng(seed)
+    sizes = np.minimum(rng.integers(req_reads[0], req_reads[1] + 1, nrequests), npool)
+    starts = rng.integers(0, npool - sizes + 1)
+    lat = []
+    inflight = []
+    t0 = time.perf_counter()
+    nxt = 0
+    while nxt < nrequests or inflight:
+        while nxt < nrequests and len(inflight) < window:
+            a, b = int(starts[nxt]), int(starts[nxt] + sizes[nxt])
+            o = offs[a:b + 1]
+            inflight.append(server.submit(reads[int(o[0]):int(o[-1])], o - o[0]))
+            nxt += 1
+        _, _, ms = server.wait(inflight.pop(0))
+        lat.append(ms)
+    wall = time.perf_counter() - t0
+    lat_a = np.array(lat, dtype=np.float64)
+    total = int(sizes.sum())
+    return {"window": window, "requests": nrequests, "reads": total, "wall_s": round(wall, 3),
+            "reads_per_s": round(total / wall) if wall > 0 else 0,
+            "p50_ms": round(float(np.percentile(lat_a, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_a, 99)), 3),
+            "server": server.stats()}

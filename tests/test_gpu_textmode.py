@@ -123,3 +123,48 @@ def test_textmode_exact_with_n(gpu_engine):
     batch.free()
     a.close()
     b.close()
+
+
+def test_textmode_locate_and_inverse(tgold):
+    """get_SA / get_ISA read the full arrays on an expanded index: the golden
+    vectors, and locate over whole intervals."""
+    hp, g, ix, batch = tgold
+    rows = g["sa_rows"].astype(np.int64)
+    for fwd, ks, ki in ((0, "sa_bwd", "isa_bwd"), (1, "sa_fwd", "isa_fwd")):
+        pos, nrows = hp.locate_batch(ix, rows, rows, cap=1, forward=fwd)
+        assert np.all(nrows == 1) and np.array_equal(pos[:, 0], g[ks])
+        assert np.array_equal(hp.inverse_locate_batch(ix, rows.astype(np.uint64), forward=fwd), g[ki])
+    o = fo.Oracle(g["genome"], ratio=int(g["ratio"]))
+    k = np.array([0, 5, 100, 7, 3], np.int64)
+    l = np.array([40, 4, 103, 7, 2000], np.int64)
+    pos, nrows = hp.locate_batch(ix, k, l, cap=64, forward=0)
+    for i in range(len(k)):
+        want = [o.get_sa(0, m) for m in range(k[i], min(l[i], k[i] + 63) + 1)] if k[i] <= l[i] else []
+        assert int(nrows[i]) == len(want) and list(pos[i, :len(want)]) == want
+
+
+def test_textmode_seed_map(gpu_engine):
+    """Seed mapping on an expanded index equals the compact index's, region
+    for region (the located positions come from the full suffix array)."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(400_000, seed=98, repeat_frac=0.3, fam_len=(300, 2000),
+                           divergence=(0.0, 0.05))
+    cuts = np.array([0, 120_000, 260_000], np.uint64)
+    lens = np.diff(np.concatenate([cuts, [len(g)]])).astype(np.uint64)
+    starts = np.full(3, 500, np.uint64)
+    out = []
+    for expand in (False, True):
+        ix = hp.Index.build(g, 16)
+        ix.set_karyotype(starts, starts + lens - 1)
+        ix.to_device()
+        if expand:
+            ix.expand()
+        reads, offs, _ = synth.reads(g, 1200, length=120, seed=99, sub_rate=0.01)
+        batch = hp.ReadBatch(reads, offs)
+        out.append(hp.seed_map_batch(ix, batch, 18, 5, 1000))
+        batch.free()
+        ix.close()
+    (n1, r1, g1, i1), (n2, r2, g2, i2) = out
+    assert n1 == n2 and r1 == r2 and r1 > 0
+    for key in g1.dtype.names:
+        assert np.array_equal(g1[key], g2[key]), key
