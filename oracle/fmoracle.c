@@ -15,6 +15,7 @@
  */
 #include "fmoracle.h"
 
+#include <stdbool.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -759,4 +760,549 @@ double fmo_exact_batch(const fmo_index *ix, const uint8_t *reads,
     out_kl[4 * i + 2] = r.k; out_kl[4 * i + 3] = r.l;
   }
   return now_s() - t0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* k-error search: BWSearchCPU, search/search.c:1171-1405, and its helpers   */
+/* search/search.c:3-1169, search/search.h:66-198                           */
+/* ------------------------------------------------------------------------ */
+
+/* One view of the pair the search runs on (its `backward` / `forward`
+ * arguments). */
+typedef struct { const fmo_dir *d; int comp; } kview;
+
+static inline void klf(kview v, int64_t k, int64_t l, unsigned b, int64_t *ko, int64_t *lo) {
+  lf(v.d, v.comp, k, l, b, ko, lo);
+}
+
+/* add_result (results.h:93-103): copy into the list, or return true from
+ * the enclosing function when it is full. */
+#define KADD(r, L)                                          \
+  do {                                                      \
+    if ((L)->n < (L)->cap) (L)->list[(L)->n++] = *(r);      \
+    else return true;                                       \
+  } while (0)
+
+/* add_mismatch / modify_last_mismatch* (results.h:59-83) */
+static inline void k_add_mm(fmo_hitk *r, uint8_t kind, uint8_t base, int16_t pos) {
+  if (r->num_mismatches < FMO_KMAX) {
+    r->err_kind[r->num_mismatches] = kind;
+    r->err_base[r->num_mismatches] = base;
+    r->err_pos[r->num_mismatches] = pos;
+  }
+  r->num_mismatches++;
+}
+static inline void k_mod2(fmo_hitk *r, uint8_t kind, uint8_t base) {
+  if (r->num_mismatches - 1 >= FMO_KMAX) return;
+  r->err_kind[r->num_mismatches - 1] = kind;
+  r->err_base[r->num_mismatches - 1] = base;
+}
+static inline void k_mod1(fmo_hitk *r, uint8_t kind) {
+  if (r->num_mismatches - 1 >= FMO_KMAX) return;
+  r->err_kind[r->num_mismatches - 1] = kind;
+}
+static inline void k_change(fmo_hitk *r, int64_t k, int64_t l, int16_t pos) { r->k = k; r->l = l; r->pos = pos; }
+
+/* BWExactSearchBackward / Forward (search.h:9-65) on a k-record */
+static void k_exact(kview v, int dir, const uint8_t *W, fmo_hitk *r, fmo_hitk *anchor) {
+  fmo_hit t;
+  memset(&t, 0, sizeof(t));
+  t.k = r->k; t.l = r->l; t.start = r->start; t.pos = r->pos; t.end = r->end;
+  fmo_hit a;
+  exact_dir(v.d, v.comp, dir, W, &t, anchor ? &a : NULL);
+  r->k = t.k; r->l = t.l; r->pos = t.pos;
+  if (anchor) {
+    memset(anchor, 0, sizeof(*anchor));
+    anchor->dir = a.dir; anchor->k = a.k; anchor->l = a.l;
+    anchor->start = a.start; anchor->pos = a.pos; anchor->end = a.end;
+  }
+}
+
+/* BWExactFinalResultBackward (search.h:66-111): the block filter, then an
+ * exact extension to start-1; only a surviving record is changed and kept. */
+static bool k_final1_bwd(const uint8_t *W, kview v, fmo_hitk *r, klist *fin, int16_t bs, int16_t lb) {
+  int16_t start = r->start, pos = r->pos;
+  int64_t k = r->k, l = r->l;
+  int16_t cb = (int16_t)(pos / bs);
+  if ((cb < lb) || (pos == start - 1)) {
+  } else if (cb > lb) {
+    return false;
+  } else if (!((pos + 1) % bs)) {
+    return false;
+  }
+  for (int16_t i = pos; i >= start; i--) {
+    klf(v, k, l, W[i], &k, &l);
+    if (k > l) break;
+  }
+  if (k <= l) {
+    k_change(r, k, l, (int16_t)(start - 1));
+    KADD(r, fin);
+  }
+  return false;
+}
+
+/* BWExactFinalResultForward (search.h:113-158) */
+static bool k_final1_fwd(const uint8_t *W, kview v, fmo_hitk *r, klist *fin, int16_t bs, int16_t lb) {
+  int16_t pos = r->pos, end = r->end;
+  int64_t k = r->k, l = r->l;
+  int16_t cb = (int16_t)(pos / bs);
+  if ((cb > lb) || (pos == end + 1)) {
+  } else if (cb < lb) {
+    return false;
+  } else if (!(pos % bs)) {
+    return false;
+  }
+  for (int16_t i = pos; i <= end; i++) {
+    klf(v, k, l, W[i], &k, &l);
+    if (k > l) break;
+  }
+  if (k <= l) {
+    k_change(r, k, l, (int16_t)(end + 1));
+    KADD(r, fin);
+  }
+  return false;
+}
+
+/* BWExactFinalResultsBackward / Forward (search.c:3-114) */
+static bool k_finals(const uint8_t *W, kview v, int dir, klist *prev, klist *out, int16_t bs, int16_t lb) {
+  for (uint64_t ii = 0; ii < prev->n; ii++) {
+    fmo_hitk *r = &prev->list[ii];
+    int16_t start = r->start, pos = r->pos, end = r->end;
+    int64_t k = r->k, l = r->l;
+    int16_t cb = (int16_t)(pos / bs);
+    if (dir == 0) {
+      if ((cb < lb) || (pos == start - 1)) {
+      } else if (cb > lb) {
+        continue;
+      } else if (!((pos + 1) % bs)) {
+        continue;
+      }
+      for (int16_t i = pos; i >= start; i--) {
+        klf(v, k, l, W[i], &k, &l);
+        if (k > l) break;
+      }
+      if (k <= l) { k_change(r, k, l, (int16_t)(start - 1)); KADD(r, out); }
+    } else {
+      if ((cb > lb) || (pos == end + 1)) {
+      } else if (cb < lb) {
+        continue;
+      } else if (!(pos % bs)) {
+        continue;
+      }
+      for (int16_t i = pos; i <= end; i++) {
+        klf(v, k, l, W[i], &k, &l);
+        if (k > l) break;
+      }
+      if (k <= l) { k_change(r, k, l, (int16_t)(end + 1)); KADD(r, out); }
+    }
+  }
+  prev->n = 0;
+  return false;
+}
+
+/* The branch step of one record at its pos, shared by the partial
+ * (BWBranchPartialResults*, search.c:648-1000; each branch record is kept)
+ * and final (BWBranchFinalResults*, search.c:116-469; each branch record is
+ * extended to the end by k_final1_*) variants.  The record itself is
+ * changed as the reference changes it, field by field. */
+static bool k_branch(const uint8_t *W, kview v, int dir, int final, fmo_hitk *r, klist *out,
+                     int16_t bs, int16_t lb) {
+  const int16_t pos = r->pos;
+  const int16_t step = dir ? 1 : -1;
+  int16_t last_err_pos;
+  uint8_t last_err_kind, last_err_base;
+  int rnm = (int)r->num_mismatches - 1;
+  if (rnm > -1) {
+    last_err_pos = r->err_pos[rnm]; last_err_kind = r->err_kind[rnm]; last_err_base = r->err_base[rnm];
+  } else {
+    last_err_pos = -10; last_err_kind = 0; last_err_base = 0xFF;
+  }
+  int64_t k = r->k, l = r->l, ka, la;
+  int no_previous = 1;
+#define KEMIT()                                                              \
+  do {                                                                       \
+    if (final) {                                                             \
+      if (dir ? k_final1_fwd(W, v, r, out, bs, lb) : k_final1_bwd(W, v, r, out, bs, lb)) return true; \
+    } else {                                                                 \
+      KADD(r, out);                                                          \
+    }                                                                        \
+  } while (0)
+  k_add_mm(r, FMO_DELETION, 0xFF, pos);
+  if (last_err_pos == pos - step) {            /* previous MISMATCH or DELETION */
+    if (last_err_kind == FMO_MISMATCH) {
+      if (W[pos] != last_err_base) { k_change(r, k, l, (int16_t)(pos + step)); KEMIT(); }
+      for (unsigned b = 0; b < 4; b++) {
+        klf(v, k, l, b, &ka, &la);
+        if (ka > la) continue;
+        if (b != W[pos]) {
+          if (b != W[last_err_pos]) {
+            k_change(r, ka, la, pos); k_mod2(r, FMO_INSERTION, (uint8_t)b); KEMIT();
+          }
+          k_change(r, ka, la, (int16_t)(pos + step)); k_mod2(r, FMO_MISMATCH, (uint8_t)b); KEMIT();
+        }
+      }
+      no_previous = 0;
+    } else if (last_err_kind == FMO_DELETION) {
+      k_change(r, k, l, (int16_t)(pos + step)); KEMIT();
+      for (unsigned b = 0; b < 4; b++) {
+        klf(v, k, l, b, &ka, &la);
+        if (ka > la) continue;
+        if (b != W[pos] && b != W[last_err_pos]) {
+          k_change(r, ka, la, (int16_t)(pos + step)); k_mod2(r, FMO_MISMATCH, (uint8_t)b); KEMIT();
+        }
+      }
+      no_previous = 0;
+    }
+  } else if (last_err_pos == pos) {            /* previous INSERTION */
+    for (unsigned b = 0; b < 4; b++) {
+      klf(v, k, l, b, &ka, &la);
+      if (ka > la) continue;
+      if (b != W[pos]) {
+        k_change(r, ka, la, pos); k_mod2(r, FMO_INSERTION, (uint8_t)b); KEMIT();
+        if (W[pos] != last_err_base) {
+          r->pos = (int16_t)(pos + step); k_mod1(r, FMO_MISMATCH); KEMIT();
+        }
+      }
+    }
+    no_previous = 0;
+  }
+  if (no_previous) {                           /* previous MATCH */
+    k_change(r, k, l, (int16_t)(pos + step)); KEMIT();
+    for (unsigned b = 0; b < 4; b++) {
+      klf(v, k, l, b, &ka, &la);
+      if (ka > la) continue;
+      if (b != W[pos]) {
+        k_change(r, ka, la, pos); k_mod2(r, FMO_INSERTION, (uint8_t)b); KEMIT();
+        r->pos = (int16_t)(pos + step); k_mod1(r, FMO_MISMATCH); KEMIT();
+      }
+    }
+  }
+#undef KEMIT
+  return false;
+}
+
+static bool k_branches(const uint8_t *W, kview v, int dir, int final, klist *prev, klist *out,
+                       int16_t bs, int16_t lb) {
+  for (uint64_t ii = 0; ii < prev->n; ii++) {
+    fmo_hitk *r = &prev->list[ii];
+    if (r->dir != dir) continue;
+    if (dir == 0 ? (r->pos < r->start) : (r->pos > r->end)) {
+      /* nothing left to branch on: kept (partial) or extended (final) */
+      if (final) {
+        if (dir == 0) { if (k_final1_bwd(W, v, r, out, bs, lb)) return true; }
+        else KADD(r, out);
+      } else {
+        KADD(r, out);
+      }
+      continue;
+    }
+    if (k_branch(W, v, dir, final, r, out, bs, lb)) return true;
+  }
+  prev->n = 0;
+  return false;
+}
+
+/* BWExactPartialResultsBackward / Forward (search.c:471-646) */
+static bool k_partials(const uint8_t *W, const uint8_t *D, uint8_t max_errors, kview v, int dir,
+                       klist *prev, klist *next, klist *next_i, int16_t bs, int16_t lb) {
+  for (uint64_t ii = 0; ii < prev->n; ii++) {
+    fmo_hitk *r = &prev->list[ii];
+    int16_t start = r->start, pos = r->pos, end = r->end;
+    uint8_t nm = r->num_mismatches;
+    int64_t kn = r->k, ln = r->l, k = 0, l = 0;
+    int64_t res, res_next = ln - kn;
+    int16_t cb = (int16_t)(pos / bs), lbp;
+    uint8_t eb = (uint8_t)(max_errors - nm);
+    int complete;
+    if (dir == 0) {
+      if ((cb < lb) || (pos == start - 1)) { lbp = start; complete = 1; }
+      else if (cb > lb) {
+        lbp = (int16_t)(((pos + 1) % bs) ? (cb - 1) * bs : cb * bs);
+        complete = 0;
+      } else if ((pos + 1) % bs) { lbp = start; complete = 1; }
+      else { lbp = (int16_t)(cb * bs); complete = 0; }
+      for (int16_t i = pos; i >= lbp; i--) {
+        k = kn; l = ln;
+        if (k > l) break;
+        klf(v, k, l, W[i], &kn, &ln);
+        res = res_next; res_next = ln - kn;
+        if (res == res_next || eb < D[i]) continue;
+        k_change(r, k, l, i);
+        KADD(r, next);
+      }
+      if (complete && kn <= ln) { k_change(r, kn, ln, (int16_t)(start - 1)); KADD(r, next_i); }
+    } else {
+      if ((cb > lb) || (pos == end + 1)) { lbp = end; complete = 1; }
+      else if (cb < lb) {
+        lbp = (int16_t)((pos % bs) ? (cb + 2) * bs - 1 : (cb + 1) * bs - 1);
+        complete = 0;
+      } else if (pos % bs) { lbp = end; complete = 1; }
+      else { lbp = (int16_t)((cb + 1) * bs - 1); complete = 0; }
+      for (int16_t i = pos; i <= lbp; i++) {
+        k = kn; l = ln;
+        if (k > l) break;
+        klf(v, k, l, W[i], &kn, &ln);
+        res = res_next; res_next = ln - kn;
+        if (res == res_next || eb < D[i]) continue;
+        k_change(r, k, l, i);
+        KADD(r, next);
+      }
+      if (complete && kn <= ln) { k_change(r, kn, ln, (int16_t)(end + 1)); KADD(r, next_i); }
+    }
+  }
+  prev->n = 0;
+  return false;
+}
+
+/* change_direction (search.h:160-198): the interval of the same occurrences
+ * on the other index, as the min / max ISA of every row's mirrored suffix. */
+static void k_change_dir(const fmo_index *ix, kview b, kview f, fmo_hitk *r) {
+  const int64_t siz = (int64_t)b.d->n + 1;
+  int64_t ki = siz - 1, li = 0;
+  int16_t err_offset = 0;
+  for (uint8_t rr = 0; rr < r->num_mismatches && rr < FMO_KMAX; rr++) {
+    if (r->err_kind[rr] == FMO_DELETION) err_offset--;
+    else if (r->err_kind[rr] == FMO_INSERTION) err_offset++;
+  }
+  const int fb = b.d == &ix->fwd, ff = f.d == &ix->fwd;
+  for (int64_t i = r->k; i <= r->l; i++) {
+    int64_t aux = siz - (int64_t)fmo_get_sa(ix, fb, (uint64_t)i) - (r->end - r->start + 2) - err_offset;
+    /* a position outside the text has no ISA (getRcompValue would walk
+     * ~2^32 steps there); such a row is left out */
+    if (aux < 0 || aux >= siz) continue;
+    int64_t aux2 = (int64_t)fmo_get_isa(ix, ff, (uint64_t)aux);
+    if (aux2 < ki) ki = aux2;
+    if (aux2 > li) li = aux2;
+  }
+  r->k = ki;
+  r->l = li;
+}
+
+/* BWChangeDirectionBackward / Forward (search.c:1002-1043): from the end of
+ * the list back, every record of the list's direction that reached its end
+ * turns to the other direction. */
+static void k_turn(const fmo_index *ix, kview b, kview f, klist *L, int to_fwd, int16_t bound) {
+  for (uint64_t ii = L->n; ii > 0; ii--) {
+    fmo_hitk *r = &L->list[ii - 1];
+    if (to_fwd) {
+      if (r->dir == 1) break;
+      if (r->pos == r->start - 1) {
+        k_change_dir(ix, b, f, r);
+        r->pos = (int16_t)(r->end + 1); r->end = bound; r->dir = 1;
+      }
+    } else {
+      if (r->dir == 0) break;
+      if (r->pos == r->end + 1) {
+        k_change_dir(ix, b, f, r);
+        r->pos = (int16_t)(r->start - 1); r->start = bound; r->dir = 0;
+      }
+    }
+  }
+}
+
+/* calculateDBackward / Forward (search.c:1045-1169): lower bounds on the
+ * errors needed in each suffix / prefix, with the reference's 8-bit types. */
+static void k_calc_d(uint8_t *D, const uint8_t *W, int64_t nW, kview b, kview f, const fmo_hitk *r,
+                     int backward) {
+  int64_t k, l;
+  uint8_t z = 0;
+  int8_t last;
+  const int64_t lb = (int64_t)b.d->n, lf_ = (int64_t)f.d->n;
+  k = r->k; l = r->l;
+  if (backward) {
+    for (int16_t i = r->pos; i >= 0; i--) {
+      klf(b, k, l, W[i], &k, &l);
+      if (k > l) { k = 0; l = lb; z++; }
+      D[i] = z;
+    }
+    if (r->end == (int16_t)(nW - 1)) {
+      last = (int8_t)D[0];
+    } else {
+      k = 0; l = lb;
+      for (int64_t i = r->end + 1; i < nW; i++) {
+        klf(f, k, l, W[i], &k, &l);
+        if (k > l) { k = 0; l = lf_; z++; }
+        D[i] = z;
+      }
+      last = (int8_t)D[nW - 1];
+      for (int64_t i = r->end + 1; i < nW; i++) D[i] = (uint8_t)(last - D[i]);
+    }
+    for (int16_t i = r->pos; i >= 0; i--) D[i] = (uint8_t)(last - D[i]);
+  } else {
+    for (int64_t i = r->pos; i < nW; i++) {
+      klf(f, k, l, W[i], &k, &l);
+      if (k > l) { k = 0; l = lf_; z++; }
+      D[i] = z;
+    }
+    if (r->start == 0) {
+      last = (int8_t)D[nW - 1];
+    } else {
+      k = 0; l = lb;
+      for (int16_t i = (int16_t)(r->start - 1); i >= 0; i--) {
+        klf(b, k, l, W[i], &k, &l);
+        if (k > l) { k = 0; l = lb; z++; }
+        D[i] = z;
+      }
+      last = (int8_t)D[0];
+      for (int16_t i = (int16_t)(r->start - 1); i >= 0; i--) D[i] = (uint8_t)(last - D[i]);
+    }
+    for (int64_t i = r->pos; i < nW; i++) D[i] = (uint8_t)(last - D[i]);
+  }
+}
+
+static bool k_search(const fmo_index *ix, const uint8_t *W, int64_t nW, kview B, kview F,
+                     klist *prev, klist *next, klist *prev_i, klist *next_i, klist *fin,
+                     klist *anc, int16_t fs, int type, uint8_t *D) {
+  fmo_hitk r, ra;
+  const int16_t fragments = (int16_t)(nW / fs);
+  int16_t half = (int16_t)(fragments / 2);
+  if (fragments % 2) half++;
+  int err_count, flow;
+  const int64_t lsa = (int64_t)B.d->n;     /* size_SA - 1 */
+  if (anc) anc->n = 0;
+
+  if (fragments <= 1) {
+    memset(&r, 0, sizeof(r));
+    if (type == 0) {
+      r.dir = 1; k_change(&r, 0, lsa, 0); r.start = 0; r.end = (int16_t)(nW - 1);
+      k_exact(F, 1, W, &r, anc ? &ra : NULL);
+    } else {
+      r.dir = 0; k_change(&r, 0, lsa, (int16_t)(nW - 1)); r.start = 0; r.end = (int16_t)(nW - 1);
+      k_exact(B, 0, W, &r, anc ? &ra : NULL);
+    }
+    if (anc) KADD(&ra, anc);
+    if (r.k <= r.l) KADD(&r, fin);
+    return false;
+  }
+
+  /* blocks half-1 .. 1: exact on the forward index, then both ways */
+  for (int16_t i = (int16_t)(half - 1); i > 0; i--) {
+    flow = 1;
+    err_count = fragments - 1;
+    prev->n = 0; prev_i->n = 0; next->n = 0; next_i->n = 0;
+    memset(&r, 0, sizeof(r));
+    r.dir = 1; k_change(&r, 0, lsa, (int16_t)(fs * i));
+    r.start = (int16_t)(fs * i); r.end = (int16_t)(fs * (i + 1) - 1);
+    k_exact(F, 1, W, &r, anc ? &ra : NULL);
+    if (anc) KADD(&ra, anc);
+    r.end = (int16_t)(nW - 1);
+    if (r.k <= r.l) {
+      k_calc_d(D, W, nW, B, F, &r, 0);
+      KADD(&r, prev);
+      while (err_count > 0) {
+        if (k_partials(W, D, (uint8_t)(fragments - 1), F, 1, prev, next, prev_i, fs, (int16_t)(half - 1))) { flow = 0; break; }
+        k_turn(ix, F, B, prev_i, 0, 0);
+        if (k_partials(W, D, (uint8_t)(fragments - 1), B, 0, prev_i, next_i, fin, fs, (int16_t)(half - 1))) { flow = 0; break; }
+        if (err_count == 1) break;
+        if (k_branches(W, F, 1, 0, next, prev, 0, 0)) { flow = 0; break; }
+        if (k_branches(W, B, 0, 0, next_i, prev_i, 0, 0)) { flow = 0; break; }
+        err_count--;
+      }
+      if (flow) {
+        k_branches(W, F, 1, 1, next, prev_i, fs, (int16_t)(half - 1));
+        k_turn(ix, F, B, prev_i, 0, 0);
+        k_finals(W, B, 0, prev_i, fin, fs, (int16_t)(half - 1));
+        k_branches(W, B, 0, 1, next_i, fin, fs, (int16_t)(half - 1));
+      }
+    }
+  }
+
+  /* block 0 */
+  flow = 1;
+  err_count = fragments - 1;
+  prev->n = 0; prev_i->n = 0; next->n = 0; next_i->n = 0;
+  memset(&r, 0, sizeof(r));
+  r.dir = 1; k_change(&r, 0, lsa, 0); r.start = 0; r.end = (int16_t)(fs - 1);
+  k_exact(F, 1, W, &r, anc ? &ra : NULL);
+  if (anc) KADD(&ra, anc);
+  r.end = (int16_t)(nW - 1);
+  if (r.k <= r.l) {
+    k_calc_d(D, W, nW, B, F, &r, 0);
+    KADD(&r, prev);
+    while (err_count > 0) {
+      if (k_partials(W, D, (uint8_t)(fragments - 1), F, 1, prev, next, fin, fs, (int16_t)(half - 1))) { flow = 0; break; }
+      if (err_count == 1) break;
+      if (k_branches(W, F, 1, 0, next, prev, 0, 0)) { flow = 0; break; }
+      err_count--;
+    }
+    if (flow) k_branches(W, F, 1, 1, next, fin, fs, (int16_t)(half - 1));
+  }
+
+  /* blocks half .. fragments-2: exact on the backward index, then both ways */
+  for (int16_t i = half; i < fragments - 1; i++) {
+    flow = 1;
+    err_count = fragments - 1;
+    prev->n = 0; prev_i->n = 0; next->n = 0; next_i->n = 0;
+    memset(&r, 0, sizeof(r));
+    r.dir = 0; k_change(&r, 0, lsa, (int16_t)(fs * (i + 1) - 1));
+    r.start = (int16_t)(fs * i); r.end = (int16_t)(fs * (i + 1) - 1);
+    k_exact(B, 0, W, &r, anc ? &ra : NULL);
+    if (anc) KADD(&ra, anc);
+    r.start = 0;
+    if (r.k <= r.l) {
+      k_calc_d(D, W, nW, B, F, &r, 1);
+      KADD(&r, prev);
+      while (err_count > 0) {
+        if (k_partials(W, D, (uint8_t)(fragments - 1), B, 0, prev, next, prev_i, fs, 0)) { flow = 0; break; }
+        k_turn(ix, B, F, prev_i, 1, (int16_t)(nW - 1));
+        if (k_partials(W, D, (uint8_t)(fragments - 1), F, 1, prev_i, next_i, fin, fs, 0)) { flow = 0; break; }
+        if (err_count == 1) break;
+        if (k_branches(W, B, 0, 0, next, prev, 0, 0)) { flow = 0; break; }
+        if (k_branches(W, F, 1, 0, next_i, prev_i, 0, 0)) { flow = 0; break; }
+        err_count--;
+      }
+      if (flow) {
+        k_branches(W, B, 0, 1, next, prev_i, fs, 0);
+        k_turn(ix, B, F, prev_i, 1, (int16_t)(nW - 1));
+        k_finals(W, F, 1, prev_i, fin, fs, 0);
+        k_branches(W, F, 1, 1, next_i, fin, fs, 0);
+      }
+    }
+  }
+
+  /* block fragments-1 (the last block takes the remainder) */
+  flow = 1;
+  err_count = fragments - 1;
+  prev->n = 0; prev_i->n = 0; next->n = 0; next_i->n = 0;
+  memset(&r, 0, sizeof(r));
+  r.dir = 0; k_change(&r, 0, lsa, (int16_t)(nW - 1));
+  r.start = (int16_t)(fs * (fragments - 1)); r.end = (int16_t)(nW - 1);
+  k_exact(B, 0, W, &r, anc ? &ra : NULL);
+  if (anc) KADD(&ra, anc);
+  r.start = 0;
+  if (r.k <= r.l) {
+    k_calc_d(D, W, nW, B, F, &r, 1);
+    KADD(&r, prev);
+    while (err_count > 0) {
+      if (k_partials(W, D, (uint8_t)(fragments - 1), B, 0, prev, next, fin, fs, 0)) { flow = 0; break; }
+      if (err_count == 1) break;
+      if (k_branches(W, B, 0, 0, next, prev, 0, 0)) { flow = 0; break; }
+      err_count--;
+    }
+    if (flow) k_branches(W, B, 0, 1, next, fin, fs, 0);
+  }
+  return false;
+}
+
+int fmo_search_k(const fmo_index *ix, int strand, const uint8_t *W, int16_t len, int16_t fragsize,
+                 uint32_t list_cap, uint32_t anchor_cap, fmo_hitk *final_out, uint32_t *nfinal,
+                 fmo_hitk *anchors_out, uint32_t *nanchors) {
+  /* bwt_map_inexact_read_2 (bwt.c:2430-2470): + strand (backward, forward,
+   * type 1); - strand (forward_rev, backward_rev, type 0) */
+  kview B, F;
+  B.d = strand ? &ix->fwd : &ix->bwd; B.comp = strand;
+  F.d = strand ? &ix->bwd : &ix->fwd; F.comp = strand;
+  klist L[5], A;
+  fmo_hitk *mem = (fmo_hitk *)calloc((size_t)5 * (list_cap ? list_cap : 1), sizeof(fmo_hitk));
+  uint8_t *D = (uint8_t *)calloc((size_t)(len > 0 ? len : 1), 1);
+  for (int i = 0; i < 5; i++) { L[i].list = mem + (size_t)i * (list_cap ? list_cap : 1); L[i].n = 0; L[i].cap = list_cap; }
+  A.list = anchors_out; A.n = 0; A.cap = anchor_cap;
+  bool full = k_search(ix, W, len, B, F, &L[0], &L[1], &L[2], &L[3], &L[4], anchors_out ? &A : NULL,
+                       fragsize, strand ? 0 : 1, D);
+  for (uint64_t i = 0; i < L[4].n; i++) final_out[i] = L[4].list[i];
+  *nfinal = (uint32_t)L[4].n;
+  if (nanchors) *nanchors = (uint32_t)A.n;
+  free(mem);
+  free(D);
+  return full ? 1 : 0;
 }

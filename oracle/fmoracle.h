@@ -138,6 +138,31 @@ uint32_t fmo_search1_vector(const fmo_index *ix, int strand, int mismatch_only,
                             const uint8_t *W, int16_t start, int16_t end, fmo_hit *out,
                             uint32_t cap);
 
+/* ---- k-error search ---------------------------------------------------- */
+/* A `result` (search/results.h:13-21) with room for FMO_KMAX error records.
+ * 64 bytes; entries past num_mismatches are not part of the record. */
+#define FMO_KMAX 8
+typedef struct {
+  int64_t k, l;
+  int16_t start, pos, end;
+  uint8_t dir, num_mismatches;
+  uint8_t err_kind[FMO_KMAX], err_base[FMO_KMAX];
+  int16_t err_pos[FMO_KMAX];
+  uint8_t pad[8];
+} fmo_hitk;
+
+typedef struct { fmo_hitk *list; uint64_t n, cap; } klist;
+
+/* BWSearchCPU (search.c:1171-1405) on one strand of W, as
+ * bwt_map_inexact_read_2 calls it (bwt.c:2430-2470): fragments = len /
+ * fragsize, up to fragments - 1 errors; the five working lists and the final
+ * list hold list_cap records, the anchor list anchor_cap.  Copies the final
+ * list and the anchors out; returns 1 when the search stopped because the
+ * anchor list was full (the reference's return value), else 0. */
+int fmo_search_k(const fmo_index *ix, int strand, const uint8_t *W, int16_t len, int16_t fragsize,
+                 uint32_t list_cap, uint32_t anchor_cap, fmo_hitk *final_out, uint32_t *nfinal,
+                 fmo_hitk *anchors_out, uint32_t *nanchors);
+
 /* ---- seed mapping ------------------------------------------------------ */
 /* One region of bwt_map_exact_seed (same layout as fmi_seed_region). */
 typedef struct {

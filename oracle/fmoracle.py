@@ -37,6 +37,29 @@ _p = C.c_void_p
 _u8p = C.POINTER(C.c_uint8)
 
 
+HITK_DTYPE = np.dtype([
+    ("k", "<i8"), ("l", "<i8"), ("start", "<i2"), ("pos", "<i2"), ("end", "<i2"),
+    ("dir", "u1"), ("num_mismatches", "u1"), ("err_kind", "u1", (8,)), ("err_base", "u1", (8,)),
+    ("err_pos", "<i2", (8,)), ("pad", "u1", (8,)),
+])
+assert HITK_DTYPE.itemsize == 64
+
+
+def hitsk_equal(a, b):
+    """k-error records equal field by field, error entries up to num_mismatches."""
+    if len(a) != len(b):
+        return False
+    for x, y in zip(a, b):
+        for key in ("k", "l", "start", "pos", "end", "dir", "num_mismatches"):
+            if x[key] != y[key]:
+                return False
+        m = min(int(x["num_mismatches"]), 8)
+        for key in ("err_kind", "err_base", "err_pos"):
+            if not np.array_equal(x[key][:m], y[key][:m]):
+                return False
+    return True
+
+
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
@@ -86,6 +109,9 @@ def _load_oracle():
     lib.fmo_search1_vector.argtypes = [_p, C.c_int, C.c_int, _p, C.c_int16, C.c_int16, _p,
                                        C.c_uint32]
     lib.fmo_search1_vector.restype = C.c_uint32
+    lib.fmo_search_k.argtypes = [_p, C.c_int, _p, C.c_int16, C.c_int16, C.c_uint32, C.c_uint32,
+                                 _p, C.POINTER(C.c_uint32), _p, C.POINTER(C.c_uint32)]
+    lib.fmo_search_k.restype = C.c_int
     lib.fmo_seed_map.argtypes = [_p, _p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _p, _p,
                                  C.c_uint64, C.c_uint32, _p, C.c_uint64, _p]
     lib.fmo_seed_map.restype = C.c_uint64
@@ -222,6 +248,16 @@ class Oracle:
                                         _ptr(out), cap)
         return int(n), out[:min(n, cap)].copy()
 
+    def search_k(self, strand, W, fragsize, list_cap=2000, anchor_cap=16):
+        """BWSearchCPU on one strand: (full, final records, anchors)."""
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        fin = np.zeros(max(list_cap, 1), HITK_DTYPE)
+        anc = np.zeros(max(anchor_cap, 1), HITK_DTYPE)
+        nf, na = C.c_uint32(0), C.c_uint32(0)
+        full = self.lib.fmo_search_k(self.ref, strand, _ptr(W), len(W), fragsize, list_cap,
+                                     anchor_cap, _ptr(fin), C.byref(nf), _ptr(anc), C.byref(na))
+        return bool(full), fin[:nf.value].copy(), anc[:na.value].copy()
+
     def seed_map(self, W, seed_len, stride, filter_seed_mappings, koff, kstart, read=0):
         """bwt_map_exact_seed over the tiling of one read: (regions, info[nseeds, 6])."""
         W = np.ascontiguousarray(W, dtype=np.uint8)
@@ -309,6 +345,10 @@ class RefCore:
         lib.ref_search1_vector.argtypes = [_p, C.c_int, _p, C.c_int16, C.c_int16, C.c_uint32, _p,
                                            C.POINTER(C.c_int)]
         lib.ref_search1_vector.restype = C.c_uint32
+        lib.ref_search_k.argtypes = [_p, C.c_int, _p, C.c_int16, C.c_int16, C.c_uint32,
+                                     C.c_uint32, _p, C.POINTER(C.c_uint32), _p,
+                                     C.POINTER(C.c_uint32)]
+        lib.ref_search_k.restype = C.c_int
         lib.ref_get_sa.argtypes = [_p, C.c_int, C.c_uint64]
         lib.ref_get_sa.restype = C.c_uint64
         lib.ref_get_isa.argtypes = [_p, C.c_int, C.c_uint64]
@@ -409,6 +449,15 @@ class RefCore:
         n = self.lib.ref_search1_vector(self.h, strand, _ptr(W), start, end, cap, _ptr(out),
                                         C.byref(full))
         return int(n), out[:n].copy(), bool(full.value)
+
+    def search_k(self, strand, W, fragsize, list_cap=2000, anchor_cap=16):
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        fin = np.zeros(max(list_cap, 1), HITK_DTYPE)
+        anc = np.zeros(max(anchor_cap, 1), HITK_DTYPE)
+        nf, na = C.c_uint32(0), C.c_uint32(0)
+        full = self.lib.ref_search_k(self.h, strand, _ptr(W), len(W), fragsize, list_cap,
+                                     anchor_cap, _ptr(fin), C.byref(nf), _ptr(anc), C.byref(na))
+        return bool(full), fin[:nf.value].copy(), anc[:na.value].copy()
 
     def get_sa(self, fwd, m):
         return int(self.lib.ref_get_sa(self.h, fwd, m))

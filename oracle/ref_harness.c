@@ -292,6 +292,62 @@ uint32_t ref_search1_vector(void *h, int strand, const uint8_t *W, int16_t start
   return nres;
 }
 
+/* BWSearchCPU (search.c:1171) on one strand, as bwt_map_inexact_read_2
+ * calls it (bwt.c:2430-2470): lists of list_cap records, anchors of
+ * anchor_cap.  Records are copied as ref_hitk (up to 8 error records). */
+typedef struct {
+  int64_t k, l;
+  int16_t start, pos, end;
+  uint8_t dir, num_mismatches;
+  uint8_t err_kind[8], err_base[8];
+  int16_t err_pos[8];
+  uint8_t pad[8];
+} ref_hitk;
+
+static void to_hitk(const result *r, ref_hitk *h) {
+  memset(h, 0, sizeof(*h));
+  h->k = r->k; h->l = r->l;
+  h->start = r->start; h->pos = r->pos; h->end = r->end;
+  h->dir = r->dir ? 1 : 0;
+  h->num_mismatches = r->num_mismatches;
+  for (int i = 0; i < r->num_mismatches && i < 8; i++) {
+    h->err_kind[i] = r->err_kind[i];
+    h->err_base[i] = r->err_base[i];
+    h->err_pos[i] = r->err_pos[i];
+  }
+}
+
+int ref_search_k(void *h, int strand, const uint8_t *W, int16_t len, int16_t fragsize,
+                 uint32_t list_cap, uint32_t anchor_cap, ref_hitk *final_out, uint32_t *nfinal,
+                 ref_hitk *anchors_out, uint32_t *nanchors) {
+  ref_ctx *c = (ref_ctx *)h;
+  results_list L[5], A;
+  for (int i = 0; i < 5; i++) {
+    new_results_list(&L[i], list_cap ? list_cap : 1);
+    L[i].max_results = list_cap;
+    L[i].num_results = 0;
+  }
+  new_results_list(&A, anchor_cap ? anchor_cap : 1);
+  A.max_results = anchor_cap;
+  A.num_results = 0;
+  bool f;
+  if (strand == 0)
+    f = BWSearchCPU((uint8_t *)W, (uint64_t)len, &c->backward, &c->forward, &L[0], &L[1], &L[2],
+                    &L[3], &L[4], anchors_out ? &A : NULL, fragsize, 1, 4);
+  else
+    f = BWSearchCPU((uint8_t *)W, (uint64_t)len, &c->forward_rev, &c->backward_rev, &L[0], &L[1],
+                    &L[2], &L[3], &L[4], anchors_out ? &A : NULL, fragsize, 0, 4);
+  for (uintmax_t i = 0; i < L[4].num_results; i++) to_hitk(&L[4].list[i], &final_out[i]);
+  *nfinal = (uint32_t)L[4].num_results;
+  if (anchors_out) {
+    for (uintmax_t i = 0; i < A.num_results; i++) to_hitk(&A.list[i], &anchors_out[i]);
+  }
+  if (nanchors) *nanchors = anchors_out ? (uint32_t)A.num_results : 0;
+  for (int i = 0; i < 5; i++) free(L[i].list);
+  free(A.list);
+  return f ? 1 : 0;
+}
+
 uint64_t ref_get_sa(void *h, int fwd, uint64_t m) {
   ref_ctx *c = (ref_ctx *)h;
   return get_SA((SA_TYPE)m, fwd ? &c->forward : &c->backward);

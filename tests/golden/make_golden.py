@@ -20,6 +20,10 @@ inputs and outputs as data:
                       whole read (bwt.c:1940-1950, 1994-2003), per strand
     s1w_bounds        one window [start, end] per read (a seed, bwt.c:1447)
     s1w_counts/hits   the same search over that window
+    sk_*              BWSearchCPU (search.c:1171) per read and strand with
+                      fragsize max(len // 3, 8), lists of sk_cap records and
+                      anchor lists of 16: the final list, the anchors, the
+                      return value (1: the anchor list filled up)
 
   golden_repeats.npz
     the same fields for 400 reads of 0..60 bp on 8 kbp made of 40 mutated
@@ -43,6 +47,7 @@ from hpgfm import synth  # noqa: E402
 OUT = os.path.join(ROOT, "tests", "golden", "golden_small.npz")
 RATIO = 16
 CAP = 512
+KCAP = 300
 
 
 def arrays_sha(arrs):
@@ -128,6 +133,27 @@ def write(path, g, reads, offsets, rng):
     s1v_hits = np.concatenate(v_hits) if v_hits else np.zeros(0, fo.HIT_DTYPE)
     s1w_hits = np.concatenate(w_hits) if w_hits else np.zeros(0, fo.HIT_DTYPE)
 
+    # BWSearchCPU with fragsize max(len // 3, 8) (up to 2 errors on most
+    # reads), lists of KCAP records and anchor lists of 16
+    k_fs = np.zeros(n, np.int16)
+    k_counts = np.zeros((n, 2), np.uint32)
+    k_acounts = np.zeros((n, 2), np.uint32)
+    k_full = np.zeros((n, 2), np.uint8)
+    k_hits, k_anch = [], []
+    for i in range(n):
+        W = reads[offsets[i]:offsets[i + 1]]
+        fs = max(len(W) // 3, 8)
+        k_fs[i] = fs
+        for s in (0, 1):
+            full, fin, anc = ref.search_k(s, W, fs, KCAP, 16)
+            k_counts[i, s] = len(fin)
+            k_acounts[i, s] = len(anc)
+            k_full[i, s] = full
+            k_hits.append(fin)
+            k_anch.append(anc)
+    sk_hits = np.concatenate(k_hits) if k_hits else np.zeros(0, fo.HITK_DTYPE)
+    sk_anch = np.concatenate(k_anch) if k_anch else np.zeros(0, fo.HITK_DTYPE)
+
     ex = {}
     lsa = ref.size_sa - 1
     for view in range(4):
@@ -161,7 +187,9 @@ def write(path, g, reads, offsets, rng):
         bwt_dollar=np.uint64(dollar),
         bwti_sha=np.array(hashlib.sha256(bwti.tobytes()).hexdigest()),
         bwti_dollar=np.uint64(dollari), s1v_counts=v_counts, s1v_hits=s1v_hits,
-        s1w_bounds=bounds, s1w_counts=w_counts, s1w_hits=s1w_hits, **ex)
+        s1w_bounds=bounds, s1w_counts=w_counts, s1w_hits=s1w_hits, sk_fragsize=k_fs,
+        sk_counts=k_counts, sk_hits=sk_hits, sk_anchor_counts=k_acounts, sk_anchors=sk_anch,
+        sk_full=k_full, sk_cap=np.uint32(KCAP), **ex)
     print("wrote", path, os.path.getsize(path), "bytes;", int(counts.sum()), "records")
 
 
