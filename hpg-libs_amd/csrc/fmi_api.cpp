@@ -4,6 +4,7 @@
 // events on the stream it runs on.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -592,6 +593,42 @@ int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
                                      d_reads, d_offsets, d_bounds, nreads, cap, d_hits, d_counts,
                                      scratch, st);
   return end_timed(st, e, "search1 vector launch");
+}
+
+int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
+                       const uint64_t *d_offsets, const int16_t *d_fragsize, uint32_t nreads,
+                       uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final,
+                       uint32_t *d_nfinal, fmi_hitk *d_anchors, uint32_t *d_nanchors,
+                       uint8_t *d_full, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "strand_mask must be 1, 2 or 3");
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_fragsize || !d_final || !d_nfinal || !d_full)
+    return fail(FMI_EINVAL, "NULL buffer");
+  if (d_anchors && !d_nanchors) return fail(FMI_EINVAL, "d_nanchors is NULL");
+  if (list_cap == 0 || list_cap > 65536) return fail(FMI_EINVAL, "list_cap must be 1..65536");
+  if (d_anchors && (anchor_cap == 0 || anchor_cap > 4096))
+    return fail(FMI_EINVAL, "anchor_cap must be 1..4096");
+  if (nreads > (1u << 30)) return fail(FMI_ERANGE, "more than 2^30 reads in one batch");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  Workspace w;
+  rc = workspace_for(st, &w);
+  if (rc) return rc;
+  // lanes: enough for the tasks, scratch within 4 GiB
+  const uint64_t per_lane = fmi::search_k_scratch_bytes(1, list_cap);
+  uint64_t lanes = std::min<uint64_t>((2ull * nreads + 255) & ~255ull, 256ull * 1024);
+  while (lanes > 256 && lanes * per_lane > (4ull << 30)) lanes -= 256;
+  void *scratch = nullptr;
+  rc = scratch_for(st, fmi::search_k_scratch_bytes((uint32_t)lanes, list_cap), &scratch);
+  if (rc) return rc;
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_search_k(ix->dev, d_reads, d_offsets, d_fragsize, nreads,
+                                      (uint32_t)strand_mask, list_cap, anchor_cap, d_final,
+                                      d_nfinal, d_anchors, d_nanchors, d_full, scratch,
+                                      (uint32_t)lanes, w.queue, st);
+  return end_timed(st, e, "k-error search launch");
 }
 
 int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,

@@ -50,6 +50,14 @@ hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_
                           const uint64_t *offsets, uint32_t nreads, uint32_t cap,
                           uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
                           uint32_t kcount, int32_t *chrom, uint64_t *coord, hipStream_t s);
+// BWSearchCPU per (read, strand) task (fmi_search_k.hip); scratch holds
+// search_k_scratch_bytes(lanes, list_cap); lanes is a multiple of 256.
+uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap);
+hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint64_t *offsets,
+                           const int16_t *fragsize, uint32_t nreads, uint32_t strand_mask,
+                           uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
+                           fmi_hitk *anchors, uint32_t *nanchors, uint8_t *full, void *scratch,
+                           uint32_t lanes, uint32_t *counter, hipStream_t s);
 hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
                          unsigned long long *sink, uint32_t grid, hipStream_t s);
 uint64_t gather_loads_done(uint64_t nloads, uint32_t grid);

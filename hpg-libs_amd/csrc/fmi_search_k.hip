@@ -1,0 +1,656 @@
+// fmi_search_k.hip -- the k-error search BWSearchCPU (search/search.c:
+// 1171-1405) on the GPU: the read is cut into len / fragsize fragments and
+// searched with up to fragments - 1 errors from every fragment in turn,
+// breadth-first over bounded result lists (partial exact extension with the
+// D lower bounds, branch steps, changes of direction through SA / ISA, final
+// extension), exactly as the reference orders, truncates and updates them.
+//
+// One lane runs one (read, strand) task at a time and takes the next from a
+// device counter.  Its four working lists and D live in a scratch slice of
+// the lane; the final list and the anchors are written straight to the
+// caller's output slots of the task.  The control flow is the reference's,
+// so lanes diverge freely: this path is about identical results for the
+// k = 2 reads of a mixed batch, not about occupancy.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fmi.h"
+#include "fmi_device.h"
+#include "fmi_internal.h"
+#include "fmi_kernels.h"
+
+namespace fmi {
+
+namespace {
+
+struct KList {
+  fmi_hitk *list;
+  uint32_t n, cap;
+};
+
+struct KView {
+  const DevDir *d;
+  uint32_t comp;
+};
+
+struct KCtx {
+  const DevIndex *ix;
+  KView B, F;                        // the search's `backward` / `forward`
+  const uint8_t *W;
+  int32_t nW;
+  uint8_t *D;
+};
+
+// BWiteration (runtime.h:38-42) on a view; N (code > 3) gives (n + 1, n).
+// Only live intervals reach it; a dead one is passed through unchanged
+// rather than read past the blocks.
+__device__ __forceinline__ void klf(const KCtx &c, const KView &v, int64_t k, int64_t l, uint32_t b,
+                                    int64_t &ko, int64_t &lo) {
+  const uint32_t n = c.ix->n;
+  if (b > 3u) { ko = (int64_t)n + 1; lo = n; return; }
+  if (k > l) { ko = k; lo = l; return; }
+  uint32_t a, e, nb;
+  lf_one(*v.d, (uint32_t)k, (uint32_t)l, v.comp ? 3u - b : b, a, e, nb);
+  ko = a;
+  lo = e;
+}
+
+// add_result (results.h:93-103): keep, or return true from the caller.
+#define KADD(r, L)                                                   \
+  do {                                                               \
+    if ((L).n < (L).cap) (L).list[(L).n++] = (r);                    \
+    else return true;                                                \
+  } while (0)
+
+__device__ __forceinline__ void k_add_mm(fmi_hitk &r, uint8_t kind, uint8_t base, int16_t pos) {
+  if (r.num_mismatches < FMI_KMAX) {
+    r.err_kind[r.num_mismatches] = kind;
+    r.err_base[r.num_mismatches] = base;
+    r.err_pos[r.num_mismatches] = pos;
+  }
+  r.num_mismatches++;
+}
+__device__ __forceinline__ void k_mod2(fmi_hitk &r, uint8_t kind, uint8_t base) {
+  if (r.num_mismatches - 1 >= FMI_KMAX) return;
+  r.err_kind[r.num_mismatches - 1] = kind;
+  r.err_base[r.num_mismatches - 1] = base;
+}
+__device__ __forceinline__ void k_mod1(fmi_hitk &r, uint8_t kind) {
+  if (r.num_mismatches - 1 >= FMI_KMAX) return;
+  r.err_kind[r.num_mismatches - 1] = kind;
+}
+__device__ __forceinline__ void k_change(fmi_hitk &r, int64_t k, int64_t l, int32_t pos) {
+  r.k = k;
+  r.l = l;
+  r.pos = (int16_t)pos;
+}
+
+__device__ __forceinline__ void k_clear(fmi_hitk &r) {
+  r.k = 0; r.l = 0; r.start = 0; r.pos = 0; r.end = 0; r.dir = 0; r.num_mismatches = 0;
+}
+
+// BWExactSearchBackward / Forward (search.h:9-65) with the anchor.
+__device__ void k_exact(const KCtx &c, const KView &v, int dir, fmi_hitk &r, fmi_hitk *anchor) {
+  int64_t k = 0, l = 0, k2 = r.k, l2 = r.l;
+  int32_t i;
+  if (dir == 0) {
+    for (i = r.pos; i >= r.start; --i) {
+      klf(c, v, k2, l2, c.W[i], k2, l2);
+      if (k2 > l2) break;
+      k = k2; l = l2;
+    }
+    if (anchor) {
+      k_clear(*anchor);
+      anchor->dir = 0; anchor->start = (int16_t)(i + 1); anchor->end = r.pos;
+      anchor->k = k; anchor->l = l; anchor->pos = (int16_t)i;
+    }
+  } else {
+    for (i = r.pos; i <= r.end; ++i) {
+      klf(c, v, k2, l2, c.W[i], k2, l2);
+      if (k2 > l2) break;
+      k = k2; l = l2;
+    }
+    if (anchor) {
+      k_clear(*anchor);
+      anchor->dir = 1; anchor->start = r.pos; anchor->end = (int16_t)(i - 1);
+      anchor->k = k; anchor->l = l; anchor->pos = (int16_t)i;
+    }
+  }
+  r.k = k2; r.l = l2; r.pos = (int16_t)i;
+}
+
+// BWExactFinalResultBackward / Forward (search.h:66-158): the block filter,
+// then an exact extension to the end; only a surviving record is changed.
+__device__ bool k_final1(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KList &fin,
+                         int32_t bs, int32_t lb) {
+  const int32_t start = r.start, pos = r.pos, end = r.end;
+  int64_t k = r.k, l = r.l;
+  const int32_t cb = pos / bs;
+  if (dir == 0) {
+    if ((cb < lb) || (pos == start - 1)) {
+    } else if (cb > lb) {
+      return false;
+    } else if (!((pos + 1) % bs)) {
+      return false;
+    }
+    for (int32_t i = pos; i >= start; --i) {
+      klf(c, v, k, l, c.W[i], k, l);
+      if (k > l) break;
+    }
+    if (k <= l) { k_change(r, k, l, start - 1); KADD(r, fin); }
+  } else {
+    if ((cb > lb) || (pos == end + 1)) {
+    } else if (cb < lb) {
+      return false;
+    } else if (!(pos % bs)) {
+      return false;
+    }
+    for (int32_t i = pos; i <= end; ++i) {
+      klf(c, v, k, l, c.W[i], k, l);
+      if (k > l) break;
+    }
+    if (k <= l) { k_change(r, k, l, end + 1); KADD(r, fin); }
+  }
+  return false;
+}
+
+// BWExactFinalResultsBackward / Forward (search.c:3-114)
+__device__ bool k_finals(const KCtx &c, const KView &v, int dir, KList &prev, KList &out,
+                         int32_t bs, int32_t lb) {
+  for (uint32_t ii = 0; ii < prev.n; ++ii) {
+    fmi_hitk &r = prev.list[ii];
+    const int32_t start = r.start, pos = r.pos, end = r.end;
+    int64_t k = r.k, l = r.l;
+    const int32_t cb = pos / bs;
+    if (dir == 0) {
+      if ((cb < lb) || (pos == start - 1)) {
+      } else if (cb > lb) {
+        continue;
+      } else if (!((pos + 1) % bs)) {
+        continue;
+      }
+      for (int32_t i = pos; i >= start; --i) {
+        klf(c, v, k, l, c.W[i], k, l);
+        if (k > l) break;
+      }
+      if (k <= l) { k_change(r, k, l, start - 1); KADD(r, out); }
+    } else {
+      if ((cb > lb) || (pos == end + 1)) {
+      } else if (cb < lb) {
+        continue;
+      } else if (!(pos % bs)) {
+        continue;
+      }
+      for (int32_t i = pos; i <= end; ++i) {
+        klf(c, v, k, l, c.W[i], k, l);
+        if (k > l) break;
+      }
+      if (k <= l) { k_change(r, k, l, end + 1); KADD(r, out); }
+    }
+  }
+  prev.n = 0;
+  return false;
+}
+
+// One record's branch step at its pos (BWBranchPartialResults*, search.c:
+// 648-1000, and BWBranchFinalResults*, search.c:116-469): deletion, then
+// insertion and mismatch of each base, as the previous error allows; the
+// record is updated field by field as the reference updates it.
+template <bool FINAL>
+__device__ bool k_branch(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KList &out,
+                         int32_t bs, int32_t lb) {
+  const int32_t pos = r.pos;
+  const int32_t step = dir ? 1 : -1;
+  int32_t last_err_pos;
+  uint32_t last_err_kind, last_err_base;
+  const int32_t rnm = (int32_t)r.num_mismatches - 1;
+  if (rnm > -1 && rnm < FMI_KMAX) {
+    last_err_pos = r.err_pos[rnm]; last_err_kind = r.err_kind[rnm]; last_err_base = r.err_base[rnm];
+  } else {
+    last_err_pos = -10; last_err_kind = 0; last_err_base = 0xFFu;
+  }
+  const int64_t k = r.k, l = r.l;
+  int64_t ka, la;
+  bool no_previous = true;
+  const uint32_t wp = c.W[pos];
+#define KEMIT()                                                          \
+  do {                                                                   \
+    if (FINAL) {                                                         \
+      if (k_final1(c, v, dir, r, out, bs, lb)) return true;              \
+    } else {                                                             \
+      KADD(r, out);                                                      \
+    }                                                                    \
+  } while (0)
+  k_add_mm(r, FMI_DELETION, 0xFF, (int16_t)pos);
+  if (last_err_pos == pos - step) {
+    if (last_err_kind == FMI_MISMATCH) {
+      if (wp != last_err_base) { k_change(r, k, l, pos + step); KEMIT(); }
+      const uint32_t wl = c.W[last_err_pos];
+      for (uint32_t b = 0; b < 4u; ++b) {
+        klf(c, v, k, l, b, ka, la);
+        if (ka > la) continue;
+        if (b != wp) {
+          if (b != wl) { k_change(r, ka, la, pos); k_mod2(r, FMI_INSERTION, (uint8_t)b); KEMIT(); }
+          k_change(r, ka, la, pos + step); k_mod2(r, FMI_MISMATCH, (uint8_t)b); KEMIT();
+        }
+      }
+      no_previous = false;
+    } else if (last_err_kind == FMI_DELETION) {
+      k_change(r, k, l, pos + step); KEMIT();
+      const uint32_t wl = c.W[last_err_pos];
+      for (uint32_t b = 0; b < 4u; ++b) {
+        klf(c, v, k, l, b, ka, la);
+        if (ka > la) continue;
+        if (b != wp && b != wl) {
+          k_change(r, ka, la, pos + step); k_mod2(r, FMI_MISMATCH, (uint8_t)b); KEMIT();
+        }
+      }
+      no_previous = false;
+    }
+  } else if (last_err_pos == pos) {
+    for (uint32_t b = 0; b < 4u; ++b) {
+      klf(c, v, k, l, b, ka, la);
+      if (ka > la) continue;
+      if (b != wp) {
+        k_change(r, ka, la, pos); k_mod2(r, FMI_INSERTION, (uint8_t)b); KEMIT();
+        if (wp != last_err_base) { r.pos = (int16_t)(pos + step); k_mod1(r, FMI_MISMATCH); KEMIT(); }
+      }
+    }
+    no_previous = false;
+  }
+  if (no_previous) {
+    k_change(r, k, l, pos + step); KEMIT();
+    for (uint32_t b = 0; b < 4u; ++b) {
+      klf(c, v, k, l, b, ka, la);
+      if (ka > la) continue;
+      if (b != wp) {
+        k_change(r, ka, la, pos); k_mod2(r, FMI_INSERTION, (uint8_t)b); KEMIT();
+        r.pos = (int16_t)(pos + step); k_mod1(r, FMI_MISMATCH); KEMIT();
+      }
+    }
+  }
+#undef KEMIT
+  return false;
+}
+
+template <bool FINAL>
+__device__ bool k_branches(const KCtx &c, const KView &v, int dir, KList &prev, KList &out,
+                           int32_t bs, int32_t lb) {
+  for (uint32_t ii = 0; ii < prev.n; ++ii) {
+    fmi_hitk &r = prev.list[ii];
+    if (r.dir != dir) continue;
+    if (dir == 0 ? (r.pos < r.start) : (r.pos > r.end)) {
+      if (FINAL && dir == 0) {
+        if (k_final1(c, v, 0, r, out, bs, lb)) return true;
+      } else {
+        KADD(r, out);
+      }
+      continue;
+    }
+    if (k_branch<FINAL>(c, v, dir, r, out, bs, lb)) return true;
+  }
+  prev.n = 0;
+  return false;
+}
+
+// BWExactPartialResultsBackward / Forward (search.c:471-646)
+__device__ bool k_partials(const KCtx &c, uint8_t max_errors, const KView &v, int dir, KList &prev,
+                           KList &next, KList &next_i, int32_t bs, int32_t lb) {
+  for (uint32_t ii = 0; ii < prev.n; ++ii) {
+    fmi_hitk &r = prev.list[ii];
+    const int32_t start = r.start, pos = r.pos, end = r.end;
+    int64_t kn = r.k, ln = r.l, k = 0, l = 0;
+    int64_t res, res_next = ln - kn;
+    const int32_t cb = pos / bs;
+    const uint8_t eb = (uint8_t)(max_errors - r.num_mismatches);
+    int32_t lbp;
+    bool complete;
+    if (dir == 0) {
+      if ((cb < lb) || (pos == start - 1)) { lbp = start; complete = true; }
+      else if (cb > lb) { lbp = ((pos + 1) % bs) ? (cb - 1) * bs : cb * bs; complete = false; }
+      else if ((pos + 1) % bs) { lbp = start; complete = true; }
+      else { lbp = cb * bs; complete = false; }
+      for (int32_t i = pos; i >= lbp; --i) {
+        k = kn; l = ln;
+        if (k > l) break;
+        klf(c, v, k, l, c.W[i], kn, ln);
+        res = res_next; res_next = ln - kn;
+        if (res == res_next || eb < c.D[i]) continue;
+        k_change(r, k, l, i);
+        KADD(r, next);
+      }
+      if (complete && kn <= ln) { k_change(r, kn, ln, start - 1); KADD(r, next_i); }
+    } else {
+      if ((cb > lb) || (pos == end + 1)) { lbp = end; complete = true; }
+      else if (cb < lb) { lbp = (pos % bs) ? (cb + 2) * bs - 1 : (cb + 1) * bs - 1; complete = false; }
+      else if (pos % bs) { lbp = end; complete = true; }
+      else { lbp = (cb + 1) * bs - 1; complete = false; }
+      for (int32_t i = pos; i <= lbp; ++i) {
+        k = kn; l = ln;
+        if (k > l) break;
+        klf(c, v, k, l, c.W[i], kn, ln);
+        res = res_next; res_next = ln - kn;
+        if (res == res_next || eb < c.D[i]) continue;
+        k_change(r, k, l, i);
+        KADD(r, next);
+      }
+      if (complete && kn <= ln) { k_change(r, kn, ln, end + 1); KADD(r, next_i); }
+    }
+  }
+  prev.n = 0;
+  return false;
+}
+
+// change_direction (search.h:160-198): min / max ISA on the other index of
+// every row's mirrored suffix; a position outside the text is left out.
+__device__ void k_change_dir(const KCtx &c, const KView &b, const KView &f, fmi_hitk &r) {
+  const int64_t siz = (int64_t)c.ix->n + 1;
+  int64_t ki = siz - 1, li = 0;
+  int32_t err_offset = 0;
+  for (uint32_t rr = 0; rr < r.num_mismatches && rr < FMI_KMAX; ++rr) {
+    if (r.err_kind[rr] == FMI_DELETION) err_offset--;
+    else if (r.err_kind[rr] == FMI_INSERTION) err_offset++;
+  }
+  for (int64_t i = r.k; i <= r.l; ++i) {
+    const int64_t aux = siz - (int64_t)sa_value(*b.d, c.ix->ratio, c.ix->n, (uint32_t)i) -
+                        (r.end - r.start + 2) - err_offset;
+    if (aux < 0 || aux >= siz) continue;
+    const int64_t aux2 = isa_value(*f.d, c.ix->ratio, c.ix->n, (uint32_t)aux);
+    if (aux2 < ki) ki = aux2;
+    if (aux2 > li) li = aux2;
+  }
+  r.k = ki;
+  r.l = li;
+}
+
+// BWChangeDirectionBackward / Forward (search.c:1002-1043)
+__device__ void k_turn(const KCtx &c, const KView &b, const KView &f, KList &L, bool to_fwd,
+                       int32_t bound) {
+  for (uint32_t ii = L.n; ii > 0; --ii) {
+    fmi_hitk &r = L.list[ii - 1];
+    if (to_fwd) {
+      if (r.dir == 1) break;
+      if (r.pos == r.start - 1) {
+        k_change_dir(c, b, f, r);
+        r.pos = (int16_t)(r.end + 1); r.end = (int16_t)bound; r.dir = 1;
+      }
+    } else {
+      if (r.dir == 0) break;
+      if (r.pos == r.end + 1) {
+        k_change_dir(c, b, f, r);
+        r.pos = (int16_t)(r.start - 1); r.start = (int16_t)bound; r.dir = 0;
+      }
+    }
+  }
+}
+
+// calculateDBackward / Forward (search.c:1045-1169), with the reference's
+// 8-bit counter and int8 `last`.
+__device__ void k_calc_d(const KCtx &c, const fmi_hitk &r, bool backward) {
+  const int64_t nmax = c.ix->n;
+  int64_t k = r.k, l = r.l;
+  uint8_t z = 0;
+  int8_t last;
+  uint8_t *D = c.D;
+  const int32_t nW = c.nW;
+  if (backward) {
+    for (int32_t i = r.pos; i >= 0; --i) {
+      klf(c, c.B, k, l, c.W[i], k, l);
+      if (k > l) { k = 0; l = nmax; z++; }
+      D[i] = z;
+    }
+    if (r.end == (int16_t)(nW - 1)) {
+      last = (int8_t)D[0];
+    } else {
+      k = 0; l = nmax;
+      for (int32_t i = r.end + 1; i < nW; ++i) {
+        klf(c, c.F, k, l, c.W[i], k, l);
+        if (k > l) { k = 0; l = nmax; z++; }
+        D[i] = z;
+      }
+      last = (int8_t)D[nW - 1];
+      for (int32_t i = r.end + 1; i < nW; ++i) D[i] = (uint8_t)(last - D[i]);
+    }
+    for (int32_t i = r.pos; i >= 0; --i) D[i] = (uint8_t)(last - D[i]);
+  } else {
+    for (int32_t i = r.pos; i < nW; ++i) {
+      klf(c, c.F, k, l, c.W[i], k, l);
+      if (k > l) { k = 0; l = nmax; z++; }
+      D[i] = z;
+    }
+    if (r.start == 0) {
+      last = (int8_t)D[nW - 1];
+    } else {
+      k = 0; l = nmax;
+      for (int32_t i = r.start - 1; i >= 0; --i) {
+        klf(c, c.B, k, l, c.W[i], k, l);
+        if (k > l) { k = 0; l = nmax; z++; }
+        D[i] = z;
+      }
+      last = (int8_t)D[0];
+      for (int32_t i = r.start - 1; i >= 0; --i) D[i] = (uint8_t)(last - D[i]);
+    }
+    for (int32_t i = r.pos; i < nW; ++i) D[i] = (uint8_t)(last - D[i]);
+  }
+}
+
+// BWSearchCPU (search.c:1171-1405) for one strand of one read.
+__device__ bool k_search(const KCtx &c, KList &prev, KList &next, KList &prev_i, KList &next_i,
+                         KList &fin, KList &anc, int32_t fs, int type) {
+  fmi_hitk r, ra;
+  const int32_t nW = c.nW;
+  const int32_t fragments = nW / fs;
+  int32_t half = fragments / 2;
+  if (fragments % 2) half++;
+  const int64_t lsa = c.ix->n;
+  const uint8_t maxe = (uint8_t)(fragments - 1);
+  anc.n = 0;
+
+  if (fragments <= 1) {
+    k_clear(r);
+    if (type == 0) {
+      r.dir = 1; k_change(r, 0, lsa, 0); r.start = 0; r.end = (int16_t)(nW - 1);
+      k_exact(c, c.F, 1, r, &ra);
+    } else {
+      r.dir = 0; k_change(r, 0, lsa, nW - 1); r.start = 0; r.end = (int16_t)(nW - 1);
+      k_exact(c, c.B, 0, r, &ra);
+    }
+    if (anc.list) KADD(ra, anc);
+    if (r.k <= r.l) KADD(r, fin);
+    return false;
+  }
+
+  for (int32_t i = half - 1; i > 0; --i) {
+    bool flow = true;
+    int32_t err_count = fragments - 1;
+    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
+    k_clear(r);
+    r.dir = 1; k_change(r, 0, lsa, fs * i); r.start = (int16_t)(fs * i); r.end = (int16_t)(fs * (i + 1) - 1);
+    k_exact(c, c.F, 1, r, &ra);
+    if (anc.list) KADD(ra, anc);
+    r.end = (int16_t)(nW - 1);
+    if (r.k <= r.l) {
+      k_calc_d(c, r, false);
+      KADD(r, prev);
+      while (err_count > 0) {
+        if (k_partials(c, maxe, c.F, 1, prev, next, prev_i, fs, half - 1)) { flow = false; break; }
+        k_turn(c, c.F, c.B, prev_i, false, 0);
+        if (k_partials(c, maxe, c.B, 0, prev_i, next_i, fin, fs, half - 1)) { flow = false; break; }
+        if (err_count == 1) break;
+        if (k_branches<false>(c, c.F, 1, next, prev, 0, 0)) { flow = false; break; }
+        if (k_branches<false>(c, c.B, 0, next_i, prev_i, 0, 0)) { flow = false; break; }
+        err_count--;
+      }
+      if (flow) {
+        k_branches<true>(c, c.F, 1, next, prev_i, fs, half - 1);
+        k_turn(c, c.F, c.B, prev_i, false, 0);
+        k_finals(c, c.B, 0, prev_i, fin, fs, half - 1);
+        k_branches<true>(c, c.B, 0, next_i, fin, fs, half - 1);
+      }
+    }
+  }
+
+  {
+    bool flow = true;
+    int32_t err_count = fragments - 1;
+    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
+    k_clear(r);
+    r.dir = 1; k_change(r, 0, lsa, 0); r.start = 0; r.end = (int16_t)(fs - 1);
+    k_exact(c, c.F, 1, r, &ra);
+    if (anc.list) KADD(ra, anc);
+    r.end = (int16_t)(nW - 1);
+    if (r.k <= r.l) {
+      k_calc_d(c, r, false);
+      KADD(r, prev);
+      while (err_count > 0) {
+        if (k_partials(c, maxe, c.F, 1, prev, next, fin, fs, half - 1)) { flow = false; break; }
+        if (err_count == 1) break;
+        if (k_branches<false>(c, c.F, 1, next, prev, 0, 0)) { flow = false; break; }
+        err_count--;
+      }
+      if (flow) k_branches<true>(c, c.F, 1, next, fin, fs, half - 1);
+    }
+  }
+
+  for (int32_t i = half; i < fragments - 1; ++i) {
+    bool flow = true;
+    int32_t err_count = fragments - 1;
+    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
+    k_clear(r);
+    r.dir = 0; k_change(r, 0, lsa, fs * (i + 1) - 1); r.start = (int16_t)(fs * i); r.end = (int16_t)(fs * (i + 1) - 1);
+    k_exact(c, c.B, 0, r, &ra);
+    if (anc.list) KADD(ra, anc);
+    r.start = 0;
+    if (r.k <= r.l) {
+      k_calc_d(c, r, true);
+      KADD(r, prev);
+      while (err_count > 0) {
+        if (k_partials(c, maxe, c.B, 0, prev, next, prev_i, fs, 0)) { flow = false; break; }
+        k_turn(c, c.B, c.F, prev_i, true, nW - 1);
+        if (k_partials(c, maxe, c.F, 1, prev_i, next_i, fin, fs, 0)) { flow = false; break; }
+        if (err_count == 1) break;
+        if (k_branches<false>(c, c.B, 0, next, prev, 0, 0)) { flow = false; break; }
+        if (k_branches<false>(c, c.F, 1, next_i, prev_i, 0, 0)) { flow = false; break; }
+        err_count--;
+      }
+      if (flow) {
+        k_branches<true>(c, c.B, 0, next, prev_i, fs, 0);
+        k_turn(c, c.B, c.F, prev_i, true, nW - 1);
+        k_finals(c, c.F, 1, prev_i, fin, fs, 0);
+        k_branches<true>(c, c.F, 1, next_i, fin, fs, 0);
+      }
+    }
+  }
+
+  {
+    bool flow = true;
+    int32_t err_count = fragments - 1;
+    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
+    k_clear(r);
+    r.dir = 0; k_change(r, 0, lsa, nW - 1); r.start = (int16_t)(fs * (fragments - 1)); r.end = (int16_t)(nW - 1);
+    k_exact(c, c.B, 0, r, &ra);
+    if (anc.list) KADD(ra, anc);
+    r.start = 0;
+    if (r.k <= r.l) {
+      k_calc_d(c, r, true);
+      KADD(r, prev);
+      while (err_count > 0) {
+        if (k_partials(c, maxe, c.B, 0, prev, next, fin, fs, 0)) { flow = false; break; }
+        if (err_count == 1) break;
+        if (k_branches<false>(c, c.B, 0, next, prev, 0, 0)) { flow = false; break; }
+        err_count--;
+      }
+      if (flow) k_branches<true>(c, c.B, 0, next, fin, fs, 0);
+    }
+  }
+  return false;
+}
+
+#undef KADD
+
+struct KParams {
+  DevIndex ix;
+  const uint8_t *reads;
+  const uint64_t *offsets;
+  const int16_t *fragsize;
+  fmi_hitk *fin, *anchors;
+  uint32_t *nfin, *nanchors;
+  uint8_t *full;
+  fmi_hitk *scratch;                 // per lane: 4 lists of list_cap records
+  uint8_t *dscratch;                 // per lane: FMI_MAX_READ_LEN bytes of D
+  uint32_t *counter;
+  uint32_t ntasks, list_cap, anchor_cap, strand_mask;
+};
+
+__global__ __launch_bounds__(256) void search_k_kernel(KParams P) {
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  fmi_hitk *sl = P.scratch + lane * 4ull * P.list_cap;
+  while (true) {
+    const uint32_t t = atomicAdd(P.counter, 1u);
+    if (t >= P.ntasks) break;
+    const uint32_t rd = t >> 1, strand = t & 1u;
+    const uint64_t off = P.offsets[rd];
+    const int32_t len = (int32_t)(P.offsets[rd + 1] - off);
+    const int32_t fs = P.fragsize[rd];
+    P.nfin[t] = 0;
+    if (P.anchors) P.nanchors[t] = 0;
+    if (!(P.strand_mask & (1u << strand))) { P.full[t] = 0; continue; }
+    if (fs <= 0 || len > (int32_t)FMI_MAX_READ_LEN || len / fs - 1 > FMI_KMAX) {
+      P.full[t] = 2;                 // not searched: fragment size out of range
+      continue;
+    }
+    KCtx c;
+    c.ix = &P.ix;
+    // + strand: (backward, forward); - strand: (forward_rev, backward_rev)
+    c.B = KView{strand ? &P.ix.d[1] : &P.ix.d[0], strand};
+    c.F = KView{strand ? &P.ix.d[0] : &P.ix.d[1], strand};
+    c.W = P.reads + off;
+    c.nW = len;
+    c.D = P.dscratch + lane * (uint64_t)FMI_MAX_READ_LEN;
+    KList prev{sl, 0, P.list_cap}, next{sl + P.list_cap, 0, P.list_cap};
+    KList prev_i{sl + 2 * P.list_cap, 0, P.list_cap}, next_i{sl + 3 * P.list_cap, 0, P.list_cap};
+    KList fin{P.fin + (uint64_t)t * P.list_cap, 0, P.list_cap};
+    KList anc{P.anchors ? P.anchors + (uint64_t)t * P.anchor_cap : nullptr, 0, P.anchor_cap};
+    const bool f = k_search(c, prev, next, prev_i, next_i, fin, anc, fs, strand ? 0 : 1);
+    P.nfin[t] = fin.n;
+    if (P.anchors) P.nanchors[t] = anc.n;
+    P.full[t] = f ? 1 : 0;
+  }
+}
+
+}  // namespace
+
+uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap) {
+  return (uint64_t)lanes * (4ull * list_cap * sizeof(fmi_hitk) + FMI_MAX_READ_LEN);
+}
+
+hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint64_t *offsets,
+                           const int16_t *fragsize, uint32_t nreads, uint32_t strand_mask,
+                           uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
+                           fmi_hitk *anchors, uint32_t *nanchors, uint8_t *full, void *scratch,
+                           uint32_t lanes, uint32_t *counter, hipStream_t s) {
+  KParams P;
+  P.ix = ix;
+  P.reads = reads;
+  P.offsets = offsets;
+  P.fragsize = fragsize;
+  P.fin = fin;
+  P.anchors = anchors;
+  P.nfin = nfin;
+  P.nanchors = nanchors;
+  P.full = full;
+  P.scratch = static_cast<fmi_hitk *>(scratch);
+  P.dscratch = static_cast<uint8_t *>(scratch) + (uint64_t)lanes * 4ull * list_cap * sizeof(fmi_hitk);
+  P.counter = counter;
+  P.ntasks = 2u * nreads;
+  P.list_cap = list_cap;
+  P.anchor_cap = anchor_cap;
+  P.strand_mask = strand_mask;
+  hipError_t e = launch_zero(counter, 1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(search_k_kernel, dim3(lanes / 256), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+}  // namespace fmi

@@ -22,6 +22,14 @@ HIT_DTYPE = np.dtype([
 ])
 assert HIT_DTYPE.itemsize == 32
 
+# fmi_hitk (include/fmi.h), 64 bytes
+HITK_DTYPE = np.dtype([
+    ("k", "<i8"), ("l", "<i8"), ("start", "<i2"), ("pos", "<i2"), ("end", "<i2"),
+    ("dir", "u1"), ("num_mismatches", "u1"), ("err_kind", "u1", (8,)), ("err_base", "u1", (8,)),
+    ("err_pos", "<i2", (8,)), ("pad", "u1", (8,)),
+])
+assert HITK_DTYPE.itemsize == 64
+
 OK = 0
 MATCH, DELETION, MISMATCH, INSERTION = 0, 1, 2, 3
 VIEW_BACKWARD, VIEW_FORWARD, VIEW_BACKWARD_REV, VIEW_FORWARD_REV = 0, 1, 2, 3
@@ -42,7 +50,7 @@ EXPORTS = [
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_get_full",
-    "fmi_search1_count_work", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
+    "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
     "fmi_server_get_stats", "fmi_server_destroy",
@@ -116,6 +124,8 @@ def _bind(lib):
         "fmi_search1_vector_batch": ([_p, _i, _i, _p, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_index_expand": ([_p], _i),
         "fmi_search1_count_work": ([_p, _i, _i, _p, _p, _u32, _u32, _p], _i),
+        "fmi_search_k_batch": ([_p, _i, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p, _p, _p, _p],
+                               _i),
         "fmi_index_get_full": ([_p, _i, _p, _p, _p], _i),
         "fmi_plain_create": ([_p, C.POINTER(_p)], _i),
         "fmi_plain_search1": ([_p, _i, _p, _p, _u32, _u32, _p, _p, C.POINTER(C.c_double),
@@ -526,6 +536,55 @@ def search1_count_work(ix, batch, cap=32, mode=MODE_MISMATCH, strand_mask=3):
     check(lib().fmi_search1_count_work(ix.h, mode, strand_mask, batch.d_reads.ptr,
                                        batch.d_offsets.ptr, batch.nreads, cap, _ptr(w)))
     return tuple(int(x) for x in w)
+
+
+class SearchKResult:
+    """Host copies of one fmi_search_k_batch: full[n, 2] (1: the anchor list
+    filled up, 2: fragment size out of range), final(i, s), anchors(i, s)."""
+
+    def __init__(self, nfin, fin, nanc, anc, full, list_cap, anchor_cap):
+        self.nfin, self.fin, self.nanc, self.anc = nfin, fin, nanc, anc
+        self.full, self.list_cap, self.anchor_cap = full, list_cap, anchor_cap
+
+    def final(self, i, s):
+        t = 2 * i + s
+        return self.fin[t * self.list_cap:t * self.list_cap + int(self.nfin[t])]
+
+    def anchors(self, i, s):
+        t = 2 * i + s
+        return self.anc[t * self.anchor_cap:t * self.anchor_cap + int(self.nanc[t])]
+
+
+def search_k_launch(ix, batch, d_fragsize, list_cap, anchor_cap, d_fin, d_nfin, d_anc, d_nanc,
+                    d_full, strand_mask=3, stream=None):
+    check(lib().fmi_search_k_batch(ix.h, strand_mask, batch.d_reads.ptr, batch.d_offsets.ptr,
+                                   d_fragsize.ptr, batch.nreads, list_cap, anchor_cap, d_fin.ptr,
+                                   d_nfin.ptr, d_anc.ptr if d_anc else None,
+                                   d_nanc.ptr if d_nanc else None, d_full.ptr, stream))
+
+
+def search_k_batch(ix, batch, fragsize, list_cap=300, anchor_cap=16, strand_mask=3, stream=None):
+    """fmi_search_k_batch (BWSearchCPU per read and strand); fragsize: int16
+    per read.  Returns a SearchKResult on the host."""
+    n = batch.nreads
+    fs = DeviceBuffer.from_array(np.ascontiguousarray(fragsize, dtype=np.int16), stream)
+    d_fin = DeviceBuffer(max(2 * n * list_cap, 1) * 64)
+    d_nfin = DeviceBuffer(max(2 * n, 1) * 4)
+    d_anc = DeviceBuffer(max(2 * n * anchor_cap, 1) * 64)
+    d_nanc = DeviceBuffer(max(2 * n, 1) * 4)
+    d_full = DeviceBuffer(max(2 * n, 1))
+    search_k_launch(ix, batch, fs, list_cap, anchor_cap, d_fin, d_nfin, d_anc, d_nanc, d_full,
+                    strand_mask, stream)
+    sync(stream)
+    res = SearchKResult(d_nfin.download(np.uint32, 2 * n, stream),
+                        d_fin.download(HITK_DTYPE, 2 * n * list_cap, stream),
+                        d_nanc.download(np.uint32, 2 * n, stream),
+                        d_anc.download(HITK_DTYPE, 2 * n * anchor_cap, stream),
+                        d_full.download(np.uint8, 2 * n, stream).reshape(n, 2),
+                        list_cap, anchor_cap)
+    for b in (fs, d_fin, d_nfin, d_anc, d_nanc, d_full):
+        b.free()
+    return res
 
 
 def search1_count_blocks(ix, batch, mode=MODE_MISMATCH, strand_mask=3):

@@ -79,6 +79,19 @@ typedef struct fmi_hit {
   uint8_t strand, pad[3];
 } fmi_hit;
 
+/* A `result` of the k-error search (search/results.h:13-21) with room for
+ * FMI_KMAX error records; entries past num_mismatches are not part of the
+ * record.  64 bytes. */
+#define FMI_KMAX 8
+typedef struct fmi_hitk {
+  int64_t k, l;
+  int16_t start, pos, end;
+  uint8_t dir, num_mismatches;
+  uint8_t err_kind[FMI_KMAX], err_base[FMI_KMAX];
+  int16_t err_pos[FMI_KMAX];
+  uint8_t pad[8];
+} fmi_hitk;
+
 typedef struct fmi_index fmi_index;   /* opaque: host arrays + device copy */
 typedef void *fmi_stream;             /* a hipStream_t; NULL = default stream */
 
@@ -357,6 +370,26 @@ int fmi_server_fetch(fmi_server *srv, uint64_t request_id, uint32_t *counts, fmi
 int fmi_server_get_stats(fmi_server *srv, fmi_server_stats *stats);
 /* Finishes queued and in-flight work, then frees everything. */
 int fmi_server_destroy(fmi_server *srv);
+
+/* ------------------------------------------------- k-error search */
+/* BWSearchCPU (search/search.c:1171-1405, helpers search.c:3-1169 and
+ * search.h:66-198) on each strand of every read, as bwt_map_inexact_read_2
+ * calls it (bwt.c:2430-2470; + strand backward/forward, - strand
+ * forward_rev/backward_rev): read i is cut into len / d_fragsize[i]
+ * fragments and searched with up to fragments - 1 errors (at most FMI_KMAX).
+ * The four working lists and the final list hold list_cap records, the
+ * anchor list anchor_cap (d_anchors may be NULL: no anchor list, as the
+ * reference's NULL rl_anchor).  Per task t = 2 * read + strand:
+ * d_final[t * list_cap ..] and d_nfinal[t] (the final list),
+ * d_anchors[t * anchor_cap ..] and d_nanchors[t], d_full[t] = the
+ * reference's return value (1: stopped on a full anchor list), or 2 when the
+ * fragment size was out of range and the task was not searched.  Truncation
+ * is the reference's: a full list ends the step that filled it. */
+int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
+                       const uint64_t *d_offsets, const int16_t *d_fragsize, uint32_t nreads,
+                       uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final,
+                       uint32_t *d_nfinal, fmi_hitk *d_anchors, uint32_t *d_nanchors,
+                       uint8_t *d_full, fmi_stream s);
 
 /* ------------------------------------------- the plain comparison point */
 /* The reference's GPU design (gpu/gpu.cu:118-203 glue, gpu.cu:333-451
