@@ -43,9 +43,165 @@ def cpu_threads():
 # 3.1 Gbp target point (45% repeats at 1-20% divergence, 0.1% N runs).
 PRESETS = {
     "c2": dict(genome_mbp=64.0, repeat_frac=0.2, div=(0.02, 0.15), n_frac=0.0, reads=1_000_000),
+    "c3": dict(genome_mbp=256.0, repeat_frac=0.2, div=(0.02, 0.15), n_frac=0.0, reads=4_000_000),
     "t3g": dict(genome_mbp=3100.0, repeat_frac=0.45, div=(0.01, 0.20), n_frac=0.001,
                 reads=2_000_000),
 }
+
+
+def subset_reads(reads, offs, idx):
+    """The reads idx (in that order) as a new (codes, offsets) batch."""
+    o64 = offs.astype(np.int64)
+    ln = o64[1:][idx] - o64[:-1][idx]
+    so = np.concatenate([[0], np.cumsum(ln)]).astype(np.uint64)
+    gather = np.repeat(o64[:-1][idx] - so[:-1].astype(np.int64), ln) + np.arange(int(so[-1]))
+    return reads[gather], so
+
+
+def n_free(reads, offs):
+    """Indexes of the reads that hold no N (code > 3)."""
+    csum = np.concatenate([[0], np.cumsum(reads > 3, dtype=np.int64)])
+    o64 = offs.astype(np.int64)
+    return np.nonzero(csum[o64[1:]] - csum[o64[:-1]] == 0)[0]
+
+
+def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
+    """Mixed error budgets in one batch: reads of U-shaped log-normal lengths
+    (median 120, clipped to 50-250 bp), 0.5% N bases, 1% substitutions; budget
+    0 (50%): exact search of both strands (bwt_map_exact_batch's pairing,
+    bwt.c:1233-1237); budget 1 (35%): the 1-error search BWSearch1CPU (edit
+    mode); budget 2 (15%): the k-error search BWSearchCPU with fragsize
+    len / 3.  A step is the launches of the three groups over the batch."""
+    rng = np.random.default_rng(3)
+    n = args.reads
+    lens = synth.lognormal_lengths(rng, n, median=120, lo=50, hi=250)
+    t0 = time.time()
+    reads, offs, _ = synth.reads(g, n, seed=4, sub_rate=args.sub_rate, n_rate=0.005,
+                                 lengths=lens, both_strands=True)
+    budget = rng.choice(3, size=n, p=[0.5, 0.35, 0.15])
+    groups = []
+    for b in range(3):
+        idx = np.nonzero(budget == b)[0]
+        rs, os_ = subset_reads(reads, offs, idx)
+        groups.append((idx, rs, os_, hpgfm.ReadBatch(rs, os_)))
+    log(f"{n} reads ({[len(x[0]) for x in groups]} per budget 0/1/2) resident in "
+        f"{time.time() - t0:.1f}s")
+    b0, b1, b2 = (x[3] for x in groups)
+    ex_out = [hpgfm.DeviceBuffer(max(b0.nreads, 1) * 32) for _ in range(2)]
+    s1 = hpgfm.Search1Output(b1.nreads, args.cap)
+    fs2 = np.maximum(np.diff(groups[2][2]).astype(np.int64) // 3, 1).astype(np.int16)
+    d_fs = hpgfm.DeviceBuffer.from_array(fs2)
+    n2 = b2.nreads
+    d_fin = hpgfm.DeviceBuffer(max(2 * n2 * list_cap, 1) * 64)
+    d_nfin = hpgfm.DeviceBuffer(max(2 * n2, 1) * 4)
+    d_anc = hpgfm.DeviceBuffer(max(2 * n2 * anchor_cap, 1) * 64)
+    d_nanc = hpgfm.DeviceBuffer(max(2 * n2, 1) * 4)
+    d_full = hpgfm.DeviceBuffer(max(2 * n2, 1))
+    L = hpgfm.lib()
+
+    def run_exact():
+        hpgfm.check(L.fmi_exact_batch(ix.h, hpgfm.VIEW_BACKWARD, 0, b0.d_reads.ptr,
+                                      b0.d_offsets.ptr, b0.nreads, ex_out[0].ptr, None, None))
+        hpgfm.check(L.fmi_exact_batch(ix.h, hpgfm.VIEW_BACKWARD_REV, 1, b0.d_reads.ptr,
+                                      b0.d_offsets.ptr, b0.nreads, ex_out[1].ptr, None, None))
+
+    def run_s1():
+        hpgfm.search1_launch(ix, b1, s1, mode=hpgfm.MODE_EDIT)
+
+    def run_k():
+        hpgfm.search_k_launch(ix, b2, d_fs, list_cap, anchor_cap, d_fin, d_nfin, d_anc, d_nanc,
+                              d_full)
+
+    parts = {}
+    for name, fn in (("exact", run_exact), ("search1", run_s1), ("search_k", run_k)):
+        fn()
+        hpgfm.sync()
+        reps = max(1, args.steps // 2)
+        with hpgfm.TimingRegion() as tr:
+            for _ in range(reps):
+                fn()
+            hpgfm.sync()
+        parts[name] = round(tr.sum_ms / reps, 3)
+        log(f"  {name}: {parts[name]} ms per step")
+    for _ in range(args.warmup):
+        run_exact(); run_s1(); run_k()
+    hpgfm.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_exact(); run_s1(); run_k()
+    hpgfm.sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    log(f"mixed budgets: {wall * 1000:.2f} ms per step, {n / wall / 1e6:.2f} M reads/s")
+
+    cpu = None
+    extras = {}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fmoracle as fo
+        threads = cpu_threads()
+        o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
+        ref = fo.RefCore(oracle=o) if fo.ref_available() else None
+        kind = "reference" if ref else "port"
+        core = ref or o
+        per_read, same, sample = [], True, []
+        for b, (idx, rs, os_, _) in enumerate(groups):
+            free = n_free(rs, os_)
+            m = min(len(free), (20000, 4000, 400)[b])
+            srs, sos = subset_reads(rs, os_, free[:m])
+            if b == 0:
+                t, kl = core.exact_batch(srs, sos, threads=threads)
+                g0 = ex_out[0].download(hpgfm.HIT_DTYPE, b0.nreads)[free[:m]]
+                g1 = ex_out[1].download(hpgfm.HIT_DTYPE, b0.nreads)[free[:m]]
+                same &= bool(np.array_equal(kl[:, 0], g0["k"]) and np.array_equal(kl[:, 1], g0["l"])
+                             and np.array_equal(kl[:, 2], g1["k"]) and np.array_equal(kl[:, 3], g1["l"]))
+            elif b == 1:
+                if ref:
+                    t, cc = core.search1_batch(srs, sos, cap=args.cap, threads=threads)
+                else:
+                    t, cc, _ = core.search1_batch(srs, sos, cap=args.cap, threads=threads,
+                                                  mismatch_only=False, keep_hits=False)
+                gc = s1.counts().reshape(-1, 2)[free[:m]].reshape(-1)
+                same &= bool(np.array_equal(np.minimum(gc, args.cap), np.minimum(cc, args.cap)))
+            else:
+                if not ref:
+                    break
+                t, cc = core.search_k_batch(srs, sos, fs2[free[:m]], list_cap, anchor_cap,
+                                            threads=threads)
+                gk = d_nfin.download(np.uint32, 2 * n2).reshape(-1, 2)[free[:m]].reshape(-1)
+                same &= bool(np.array_equal(gk, cc))
+            per_read.append(t / max(m, 1))
+            sample.append(m)
+        if len(per_read) == 3:
+            frac = [len(x[0]) / n for x in groups]
+            cpu_rps = 1.0 / sum(f * t for f, t in zip(frac, per_read))
+            cpu = {"value": round(cpu_rps), "unit": "reads/s", "cores": threads, "kind": kind,
+                   "sample": f"N-free reads of each budget group ({sample[0]} / {sample[1]} / "
+                             f"{sample[2]}): the reference's exact search of both strands, "
+                             f"BWSearch1CPU and BWSearchCPU (fragsize len/3, lists of "
+                             f"{list_cap}), OpenMP over reads as bwt.c:1134; reads/s weighted "
+                             f"by the group fractions"}
+            extras["parity_vs_cpu_sample"] = same
+            log(f"CPU baseline ({kind}, {threads} threads): {cpu_rps:.0f} reads/s; "
+                f"GPU == reference on the samples: {same}")
+    res = {
+        "metric": "reads/s, mixed error budgets 0/1/2 (exact / BWSearch1CPU / BWSearchCPU)",
+        "value": round(n / wall), "unit": "reads/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1000, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"mixed budgets, {args.genome_mbp:g} Mbp synthetic reference "
+                               f"({pre['repeat_frac'] * 100:g}% repeats), {n} reads of "
+                               f"50-250 bp (log-normal, median 120), 0.5% N, budgets 0/1/2 at "
+                               f"50/35/15%", "preset": args.preset, "list_cap": list_cap},
+        "cpu_baseline": cpu, "ms_per_group": parts,
+        "reads_per_group": [len(x[0]) for x in groups],
+    }
+    res.update(extras)
+    for x in groups:
+        x[3].free()
+    for b in ex_out + [d_fs, d_fin, d_nfin, d_anc, d_nanc, d_full]:
+        b.free()
+    s1.free()
+    print(json.dumps(res), flush=True)
 
 
 def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000):
@@ -146,9 +302,10 @@ def main():
                          "WRITE_SIZE counters: a 1 GiB streaming copy (fmi_copy_bench) and "
                          "2^24-line random gathers of 64 and 128 bytes over a buffer the size "
                          "of the index and over 8 GiB")
-    ap.add_argument("--workload", choices=["search1", "seeds"], default="search1",
+    ap.add_argument("--workload", choices=["search1", "seeds", "mixed"], default="search1",
                     help="search1: the 1-mismatch search (the metric); seeds: exact seed mapping "
-                         "(18-mers at stride 5, <=1000 positions per seed)")
+                         "(18-mers at stride 5, <=1000 positions per seed); mixed: error "
+                         "budgets 0/1/2 in one batch (use with --preset c3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -178,6 +335,11 @@ def main():
     ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
     log(f"index built on the GPU in {time.time() - t0:.1f}s "
         f"({ix.device_bytes / 1e6:.1f} MB on device)")
+    if args.workload == "mixed":
+        if not args.no_expand:
+            ix.expand()
+        bench_mixed(args, pre, g, ix, hpgfm, synth)
+        return
     if args.workload == "seeds":
         if not args.no_expand:
             t0 = time.time()

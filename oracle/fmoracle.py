@@ -349,6 +349,9 @@ class RefCore:
                                      C.c_uint32, _p, C.POINTER(C.c_uint32), _p,
                                      C.POINTER(C.c_uint32)]
         lib.ref_search_k.restype = C.c_int
+        lib.ref_search_k_batch.argtypes = [_p, _p, _p, _p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                           _p, C.c_int]
+        lib.ref_search_k_batch.restype = C.c_double
         lib.ref_get_sa.argtypes = [_p, C.c_int, C.c_uint64]
         lib.ref_get_sa.restype = C.c_uint64
         lib.ref_get_isa.argtypes = [_p, C.c_int, C.c_uint64]
@@ -458,6 +461,17 @@ class RefCore:
         full = self.lib.ref_search_k(self.h, strand, _ptr(W), len(W), fragsize, list_cap,
                                      anchor_cap, _ptr(fin), C.byref(nf), _ptr(anc), C.byref(na))
         return bool(full), fin[:nf.value].copy(), anc[:na.value].copy()
+
+    def search_k_batch(self, reads, offsets, fragsize, list_cap=128, anchor_cap=16, threads=0):
+        """(seconds, final-list sizes[2n]) of BWSearchCPU over a batch."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        fragsize = np.ascontiguousarray(fragsize, dtype=np.int16)
+        n = len(offsets) - 1
+        out = np.zeros(max(2 * n, 1), np.uint32)
+        t = self.lib.ref_search_k_batch(self.h, _ptr(reads), _ptr(offsets), _ptr(fragsize), n,
+                                        list_cap, anchor_cap, _ptr(out), threads)
+        return float(t), out[:2 * n]
 
     def get_sa(self, fwd, m):
         return int(self.lib.ref_get_sa(self.h, fwd, m))

@@ -402,6 +402,45 @@ double ref_search1_batch(void *h, const uint8_t *reads, const uint64_t *offsets,
   return now_s() - t0;
 }
 
+/* CPU baseline for the k-error search: BWSearchCPU on both strands of every
+ * read with its own fragment size, parallel over reads as bwt.c:1134; each
+ * thread owns its lists (bwt_map_inexact_read_2 allocates them per read). */
+double ref_search_k_batch(void *h, const uint8_t *reads, const uint64_t *offsets,
+                          const int16_t *fragsize, uint64_t nreads, uint32_t list_cap,
+                          uint32_t anchor_cap, uint32_t *nfinal, int threads) {
+  ref_ctx *c = (ref_ctx *)h;
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+  double t0 = now_s();
+#pragma omp parallel
+  {
+    results_list L[5], A;
+    for (int i = 0; i < 5; i++) { new_results_list(&L[i], list_cap); }
+    new_results_list(&A, anchor_cap);
+#pragma omp for schedule(dynamic, chunk)
+    for (long i = 0; i < (long)nreads; i++) {
+      uint8_t *W = (uint8_t *)reads + offsets[i];
+      uint64_t len = offsets[i + 1] - offsets[i];
+      for (int s = 0; s < 2; s++) {
+        for (int j = 0; j < 5; j++) L[j].num_results = 0;
+        A.num_results = 0;
+        if (s == 0)
+          BWSearchCPU(W, len, &c->backward, &c->forward, &L[0], &L[1], &L[2], &L[3], &L[4], &A,
+                      fragsize[i], 1, 4);
+        else
+          BWSearchCPU(W, len, &c->forward_rev, &c->backward_rev, &L[0], &L[1], &L[2], &L[3],
+                      &L[4], &A, fragsize[i], 0, 4);
+        nfinal[2 * i + s] = (uint32_t)L[4].num_results;
+      }
+    }
+    for (int i = 0; i < 5; i++) free(L[i].list);
+    free(A.list);
+  }
+  return now_s() - t0;
+}
+
 /* CPU baseline for exact search: + strand backward on `backward`, - strand
  * forward on `backward_rev` (bwt.c:1222-1238). */
 double ref_exact_batch(void *h, const uint8_t *reads, const uint64_t *offsets,
