@@ -139,9 +139,17 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import fmoracle as fo
         threads = cpu_threads()
+        t0 = time.time()
         o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
+        # BWSearchCPU's change of direction reads SA / ISA: sample the
+        # expanded index's full arrays at the index's own ratio
+        sa0, isa0, _ = ix.full_arrays(0)
+        sa1, isa1, _ = ix.full_arrays(1)
+        o.set_samples(ix.sa_ratio, sa0, isa0, sa1, isa1)
+        del sa0, isa0, sa1, isa1
         ref = fo.RefCore(oracle=o) if fo.ref_available() else None
         kind = "reference" if ref else "port"
+        log(f"CPU index ({kind}, SA ratio {ix.sa_ratio}) ready in {time.time() - t0:.1f}s")
         core = ref or o
         per_read, same, sample = [], True, []
         for b, (idx, rs, os_, _) in enumerate(groups):
@@ -336,8 +344,7 @@ def main():
     log(f"index built on the GPU in {time.time() - t0:.1f}s "
         f"({ix.device_bytes / 1e6:.1f} MB on device)")
     if args.workload == "mixed":
-        if not args.no_expand:
-            ix.expand()
+        ix.expand()
         bench_mixed(args, pre, g, ix, hpgfm, synth)
         return
     if args.workload == "seeds":

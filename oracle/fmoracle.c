@@ -193,6 +193,21 @@ out:
 
 void fmo_index_free(fmo_index *ix) { fmo_dir_free(&ix->bwd); fmo_dir_free(&ix->fwd); }
 
+int fmo_dir_set_samples(fmo_index *ix, int fwd, uint32_t ratio, const uint32_t *S,
+                        const uint32_t *R, uint64_t nsamp) {
+  fmo_dir *d = fwd ? &ix->fwd : &ix->bwd;
+  uint64_t n1 = d->n + 1;
+  if (ratio == 0 || nsamp != n1 / ratio + (n1 % ratio ? 1 : 0)) return -1;
+  uint32_t *s2 = (uint32_t *)malloc(nsamp * sizeof(uint32_t));
+  uint32_t *r2 = (uint32_t *)malloc(nsamp * sizeof(uint32_t));
+  if (!s2 || !r2) { free(s2); free(r2); return -1; }
+  memcpy(s2, S, nsamp * sizeof(uint32_t));
+  memcpy(r2, R, nsamp * sizeof(uint32_t));
+  free(d->S); free(d->R);
+  d->S = s2; d->R = r2; d->nsamp = nsamp; d->ratio = ratio;
+  return 0;
+}
+
 uint64_t fmo_dir_words(const fmo_index *ix, int fwd) { return fwd ? ix->fwd.words : ix->bwd.words; }
 
 void fmo_dir_export(const fmo_index *ix, int fwd, uint32_t *C, uint32_t *C1,

@@ -92,6 +92,11 @@ int fmo_index_from_bwts(fmo_index *ix, const uint8_t *bwt, uint64_t dollar,
                         const uint8_t *bwt_rev, uint64_t dollar_rev, uint64_t n,
                         uint32_t ratio);
 void fmo_index_free(fmo_index *ix);
+/* Installs S/R samples computed elsewhere (S[i] = SA[i * ratio],
+ * R[i] = ISA[i * ratio], the layout calculate_S_and_R writes) into one
+ * direction of an index built with ratio 0. */
+int fmo_dir_set_samples(fmo_index *ix, int fwd, uint32_t ratio, const uint32_t *S,
+                        const uint32_t *R, uint64_t nsamp);
 
 /* Raw access for tests: copies the reference-layout arrays out. */
 uint64_t fmo_dir_words(const fmo_index *ix, int fwd);

@@ -85,6 +85,7 @@ def _load_oracle():
     lib.fmo_bwt_from_sa.argtypes = [_p, C.c_uint64, _p, _p, C.POINTER(C.c_uint64)]
     lib.fmo_index_build_naive.argtypes = [_p, _p, C.c_uint64, C.c_uint32]
     lib.fmo_index_from_bwts.argtypes = [_p, _p, C.c_uint64, _p, C.c_uint64, C.c_uint64, C.c_uint32]
+    lib.fmo_dir_set_samples.argtypes = [_p, C.c_int, C.c_uint32, _p, _p, C.c_uint64]
     lib.fmo_index_free.argtypes = [_p]
     lib.fmo_dir_words.argtypes = [_p, C.c_int]
     lib.fmo_dir_words.restype = C.c_uint64
@@ -182,6 +183,17 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+    def set_samples(self, ratio, sa_bwd, isa_bwd, sa_fwd, isa_fwd):
+        """Samples every ratio-th entry of full SA / ISA arrays (uint32, n + 1,
+        one pair per direction) into S and R, instead of the LF walk of
+        calculate_S_and_R (preprocess.c:7-42) that ratio > 0 runs at build."""
+        for fwd, sa, isa in ((0, sa_bwd, isa_bwd), (1, sa_fwd, isa_fwd)):
+            S = np.ascontiguousarray(sa[::ratio], dtype=np.uint32)
+            R = np.ascontiguousarray(isa[::ratio], dtype=np.uint32)
+            if self.lib.fmo_dir_set_samples(self.ref, fwd, ratio, _ptr(S), _ptr(R), len(S)):
+                raise ValueError("sample arrays do not fit the index")
+        self.ratio = ratio
 
     @property
     def size_sa(self):

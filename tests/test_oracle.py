@@ -83,6 +83,28 @@ def test_golden_sa_isa(golden):
         assert o.get_isa(1, m) == int(g["isa_fwd"][j])
 
 
+def test_set_samples_equals_lf_walk():
+    """S/R sampled from full SA / ISA arrays (the brute-force suffix sort
+    here) equal the ones the build-time LF walk writes."""
+    g, _, _ = synth.genome(3000, seed=31, repeat_frac=0.3, fam_len=(100, 400))
+    walk = fo.Oracle(g, ratio=8)
+    o = fo.Oracle(g, ratio=0)
+    full = []
+    for t in (g, g[::-1]):
+        sa = np.array(py_suffix_array(t), np.uint32)
+        isa = np.empty_like(sa)
+        isa[sa] = np.arange(len(sa), dtype=np.uint32)
+        full += [sa, isa]
+    o.set_samples(8, *full)
+    for fwd in (0, 1):
+        a, b = o.arrays(fwd), walk.arrays(fwd)
+        assert np.array_equal(a["S"], b["S"]) and np.array_equal(a["R"], b["R"])
+    for m in range(0, o.size_sa, 97):
+        assert o.get_sa(0, m) == walk.get_sa(0, m) and o.get_isa(1, m) == walk.get_isa(1, m)
+    with pytest.raises(ValueError):
+        o.set_samples(8, full[0][:-8], *full[1:])
+
+
 # ---------------------------------------------------------------- brute force
 
 def py_suffix_array(text):
