@@ -43,6 +43,7 @@ def cpu_threads():
 # 3.1 Gbp target point (45% repeats at 1-20% divergence, 0.1% N runs).
 PRESETS = {
     "c2": dict(genome_mbp=64.0, repeat_frac=0.2, div=(0.02, 0.15), n_frac=0.0, reads=1_000_000),
+    "c1": dict(genome_mbp=1.0, repeat_frac=0.0, div=(0.0, 0.0), n_frac=0.0, reads=100_000),
     "c3": dict(genome_mbp=256.0, repeat_frac=0.2, div=(0.02, 0.15), n_frac=0.0, reads=4_000_000),
     "t3g": dict(genome_mbp=3100.0, repeat_frac=0.45, div=(0.01, 0.20), n_frac=0.001,
                 reads=2_000_000),
@@ -212,6 +213,92 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     print(json.dumps(res), flush=True)
 
 
+def bench_exact(args, pre, g, ix, hpgfm, synth, read_len=100, cap=64):
+    """Exact search + locate: reads of 100 bp, 90% copied from random
+    positions and strands without errors and 10% random; per read the
+    backward search of the + strand and the forward search on backward_rev
+    for the - strand (bwt.c:1233-1237), then SA(j) of the backward index for
+    the first 64 rows of each interval (bwt.c:1271-1277).  The reference's
+    CPU path runs the same searches in the same run."""
+    n = args.reads
+    t0 = time.time()
+    reads, offs, _ = synth.reads(g, n, length=read_len, seed=2, sub_rate=0.0, random_frac=0.1)
+    batch = hpgfm.ReadBatch(reads, offs)
+    log(f"{n} reads resident in {time.time() - t0:.1f}s")
+    L = hpgfm.lib()
+    outs = [hpgfm.DeviceBuffer(max(n, 1) * 32) for _ in range(2)]
+    pos = [hpgfm.DeviceBuffer(max(n * cap, 1) * 8) for _ in range(2)]
+    nrows = [hpgfm.DeviceBuffer(max(n, 1) * 4) for _ in range(2)]
+
+    def step():
+        for s, view in ((0, hpgfm.VIEW_BACKWARD), (1, hpgfm.VIEW_BACKWARD_REV)):
+            hpgfm.check(L.fmi_exact_batch(ix.h, view, s, batch.d_reads.ptr, batch.d_offsets.ptr,
+                                          n, outs[s].ptr, None, None))
+            hpgfm.check(L.fmi_locate_hits(ix.h, 0, outs[s].ptr, n, cap, pos[s].ptr, nrows[s].ptr,
+                                          None))
+
+    for _ in range(args.warmup):
+        step()
+    hpgfm.sync()
+    with hpgfm.TimingRegion() as tr:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        hpgfm.sync()
+        wall = (time.perf_counter() - t0) / args.steps
+    kern = tr.sum_ms / args.steps
+    log(f"exact + locate: {wall * 1000:.3f} ms per step (kernels {kern:.3f} ms), "
+        f"{n / wall / 1e6:.2f} M reads/s")
+    g_rows = np.stack([nrows[s].download(np.uint32, n) for s in (0, 1)], axis=1)
+    g_sums = np.zeros((n, 2), np.uint64)
+    for s in (0, 1):
+        p = pos[s].download(np.uint64, n * cap).reshape(n, cap)
+        mask = np.arange(cap)[None, :] < g_rows[:, s][:, None]
+        g_sums[:, s] = np.where(mask, p, 0).sum(axis=1, dtype=np.uint64)
+
+    cpu = None
+    extras = {"rows_located": int(g_rows.sum())}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fmoracle as fo
+        threads = cpu_threads()
+        o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
+        sa0, isa0, _ = ix.full_arrays(0)
+        sa1, isa1, _ = ix.full_arrays(1)
+        o.set_samples(ix.sa_ratio, sa0, isa0, sa1, isa1)
+        ref = fo.RefCore(oracle=o) if fo.ref_available() else None
+        kind = "reference" if ref else "port"
+        core = ref or o
+        tcpu, c_rows, c_sums = core.exact_locate_batch(reads, offs, cap=cap, threads=threads)
+        t1, _, _ = core.exact_locate_batch(reads, offs, cap=cap, threads=1)
+        same = bool(np.array_equal(c_rows, g_rows) and np.array_equal(c_sums, g_sums))
+        cpu = {"value": round(n / tcpu), "unit": "reads/s", "cores": threads, "kind": kind,
+               "sample": f"all {n} reads: BWExactSearchBackward (+ strand) and "
+                         f"BWExactSearchForward on backward_rev (- strand), then get_SA over "
+                         f"the first {cap} rows (SA ratio {ix.sa_ratio}), OpenMP over reads as "
+                         f"bwt.c:1134, -O2 -fopenmp -DFM_COMP_64"}
+        extras["cpu_one_thread_reads_per_s"] = round(n / t1)
+        extras["parity_vs_cpu"] = same
+        log(f"CPU ({kind}): {n / tcpu:.0f} reads/s on {threads} threads, {n / t1:.0f} on one; "
+            f"GPU == CPU: {same}")
+    res = {
+        "metric": "reads/s, exact search of both strands + locate",
+        "value": round(n / wall), "unit": "reads/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1000, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"exact search + locate, {args.genome_mbp:g} Mbp synthetic "
+                               f"reference (i.i.d., 41% GC), {n} reads of {read_len} bp (90% "
+                               f"exact copies of either strand, 10% random), <= {cap} rows "
+                               f"per interval", "preset": args.preset},
+        "kernel_ms_per_step": round(kern, 4), "cpu_baseline": cpu,
+    }
+    res.update(extras)
+    batch.free()
+    for b in outs + pos + nrows:
+        b.free()
+    print(json.dumps(res), flush=True)
+
+
 def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000):
     """Exact seed mapping: every read cut into seed_len-mers at `stride`, both
     orientations, up to `filt` SA positions resolved per seed (the
@@ -310,10 +397,12 @@ def main():
                          "WRITE_SIZE counters: a 1 GiB streaming copy (fmi_copy_bench) and "
                          "2^24-line random gathers of 64 and 128 bytes over a buffer the size "
                          "of the index and over 8 GiB")
-    ap.add_argument("--workload", choices=["search1", "seeds", "mixed"], default="search1",
+    ap.add_argument("--workload", choices=["search1", "seeds", "mixed", "exact"],
+                    default="search1",
                     help="search1: the 1-mismatch search (the metric); seeds: exact seed mapping "
                          "(18-mers at stride 5, <=1000 positions per seed); mixed: error "
-                         "budgets 0/1/2 in one batch (use with --preset c3)")
+                         "budgets 0/1/2 in one batch (use with --preset c3); exact: exact "
+                         "search + locate with the reference's CPU path (use with --preset c1)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -343,6 +432,10 @@ def main():
     ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
     log(f"index built on the GPU in {time.time() - t0:.1f}s "
         f"({ix.device_bytes / 1e6:.1f} MB on device)")
+    if args.workload == "exact":
+        ix.expand()
+        bench_exact(args, pre, g, ix, hpgfm, synth)
+        return
     if args.workload == "mixed":
         ix.expand()
         bench_mixed(args, pre, g, ix, hpgfm, synth)

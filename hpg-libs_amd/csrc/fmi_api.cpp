@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -667,8 +668,28 @@ int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k, const
   hipStream_t st = static_cast<hipStream_t>(s);
   rc = begin_timed(st);
   if (rc) return rc;
-  hipError_t e = fmi::launch_locate(ix->dev, forward ? 1u : 0u, d_k, d_l, nintervals, cap, d_pos,
+  hipError_t e = fmi::launch_locate(ix->dev, forward ? 1u : 0u, d_k, d_l, 1, nintervals, cap, d_pos,
                                     d_nrows, st);
+  return end_timed(st, e, "locate launch");
+}
+
+int fmi_locate_hits(const fmi_index *ix, int forward, const fmi_hit *d_hits, uint32_t count,
+                    uint32_t cap, uint64_t *d_pos, uint32_t *d_nrows, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (count == 0 || cap == 0) return FMI_OK;
+  if (!d_hits || !d_pos || !d_nrows) return fail(FMI_EINVAL, "NULL buffer");
+  if ((uint64_t)count * cap > (1ull << 36)) return fail(FMI_ERANGE, "count * cap too large");
+  static_assert(sizeof(fmi_hit) % sizeof(int64_t) == 0 && offsetof(fmi_hit, k) == 0 &&
+                    offsetof(fmi_hit, l) == 8,
+                "fmi_hit layout");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  const int64_t *base = reinterpret_cast<const int64_t *>(d_hits);
+  hipError_t e = fmi::launch_locate(ix->dev, forward ? 1u : 0u, base, base + 1,
+                                    (uint32_t)(sizeof(fmi_hit) / sizeof(int64_t)), count, cap,
+                                    d_pos, d_nrows, st);
   return end_timed(st, e, "locate launch");
 }
 

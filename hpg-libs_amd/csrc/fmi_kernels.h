@@ -36,8 +36,8 @@ hipError_t launch_vhelper(const DevIndex &ix, bool mm_only, uint32_t strand_mask
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
                           void *scratch, hipStream_t s);
 hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
-                         uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
-                         hipStream_t s);
+                         uint32_t stride, uint32_t nintervals, uint32_t cap, uint64_t *pos,
+                         uint32_t *nrows, hipStream_t s);
 // Full SA / ISA of one direction from its samples (sa, isa: n + 1 entries).
 hipError_t launch_expand(const DevDir &D, uint32_t n, uint32_t ratio, uint32_t *sa, uint32_t *isa,
                          hipStream_t s);

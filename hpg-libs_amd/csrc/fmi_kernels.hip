@@ -837,7 +837,7 @@ __global__ __launch_bounds__(256) void vhelper_kernel(VhParams P) {
 
 struct LocateParams {
   DevIndex ix;
-  uint32_t dsel, nintervals, cap;
+  uint32_t dsel, nintervals, cap, stride;   // stride: int64 words between intervals
   const int64_t *k, *l;
   uint64_t *pos;
   uint32_t *nrows;
@@ -848,7 +848,7 @@ __global__ __launch_bounds__(256) void locate_kernel(LocateParams P) {
   const uint64_t iv = g / P.cap;
   const uint32_t j = (uint32_t)(g - iv * P.cap);
   if (iv >= P.nintervals) return;
-  const int64_t k = P.k[iv], l = P.l[iv];
+  const int64_t k = P.k[iv * P.stride], l = P.l[iv * P.stride];
   // Rows outside 0..n are not in the index: such an interval yields no rows.
   const bool valid = k >= 0 && l <= (int64_t)P.ix.n;
   const int64_t cnt = (valid && k <= l) ? (l - k + 1) : 0;
@@ -1201,9 +1201,9 @@ hipError_t launch_vhelper(const DevIndex &ix, bool mm_only, uint32_t strand_mask
 }
 
 hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
-                         uint32_t nintervals, uint32_t cap, uint64_t *pos, uint32_t *nrows,
-                         hipStream_t s) {
-  LocateParams P{ix, dsel, nintervals, cap, k, l, pos, nrows};
+                         uint32_t stride, uint32_t nintervals, uint32_t cap, uint64_t *pos,
+                         uint32_t *nrows, hipStream_t s) {
+  LocateParams P{ix, dsel, nintervals, cap, stride, k, l, pos, nrows};
   const uint64_t total = (uint64_t)nintervals * cap;
   const uint64_t grid = (total + 255) / 256;
   if (grid == 0) return hipSuccess;

@@ -46,7 +46,8 @@ EXPORTS = [
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
     "fmi_stream_destroy", "fmi_stream_sync", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
-    "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_resolve_hits", "fmi_gather_bench",
+    "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits",
+    "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_get_full",
@@ -117,6 +118,7 @@ def _bind(lib):
         "fmi_set_tuning": ([_u32, _u32, _u32], _i),
         "fmi_get_tuning": ([C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)], _i),
         "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
+        "fmi_locate_hits": ([_p, _i, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
         "fmi_inverse_locate_batch": ([_p, _i, _p, _u64, _p, _p], _i),
         "fmi_encode_bases": ([_p, _u64, _p], _i),
@@ -606,6 +608,20 @@ def locate_batch(ix, k, l, cap, forward=0, stream=None):
     check(lib().fmi_locate_batch(ix.h, forward, dk.ptr, dl.ptr, n, cap, dp.ptr, dn.ptr, stream))
     sync(stream)
     return dp.download(np.uint64, n * cap, stream).reshape(n, cap), dn.download(np.uint32, n, stream)
+
+
+def locate_hits(ix, d_hits, count, cap, forward=0, stream=None):
+    """fmi_locate_hits over `count` records already on the device (a
+    DeviceBuffer of fmi_hit); returns (positions[count, cap], rows[count])."""
+    dp = DeviceBuffer(max(count * cap, 1) * 8)
+    dn = DeviceBuffer(max(count, 1) * 4)
+    check(lib().fmi_locate_hits(ix.h, forward, d_hits.ptr, count, cap, dp.ptr, dn.ptr, stream))
+    sync(stream)
+    out = dp.download(np.uint64, count * cap, stream).reshape(count, cap), \
+        dn.download(np.uint32, count, stream)
+    dp.free()
+    dn.free()
+    return out
 
 
 def inverse_locate_batch(ix, text_pos, forward=0, stream=None):

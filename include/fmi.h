@@ -253,6 +253,14 @@ int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k,
                      const int64_t *d_l, uint32_t nintervals, uint32_t cap,
                      uint64_t *d_pos, uint32_t *d_nrows, fmi_stream s);
 
+/* The same rows for the interval of each record d_hits[i] (k, l of an
+ * fmi_exact_batch or fmi_search1_batch record): the get_SA loop over [k, l]
+ * that follows an exact search in bwt_map_exact_seq (bwt.c:927-952) and
+ * bwt_map_exact_seed (bwt.c:1271-1277), without splitting the records into
+ * k and l arrays first. */
+int fmi_locate_hits(const fmi_index *ix, int forward, const fmi_hit *d_hits, uint32_t count,
+                    uint32_t cap, uint64_t *d_pos, uint32_t *d_nrows, fmi_stream s);
+
 /* get_ISA (search/runtime.h:88, getRcompValue search/csafm.h:237-272): the
  * BWT row of text position d_text_pos[i] (0..n) of the BACKWARD (forward =
  * 0) or FORWARD (forward = 1) index; positions beyond n give row UINT64_MAX. */

@@ -777,6 +777,42 @@ double fmo_exact_batch(const fmo_index *ix, const uint8_t *reads,
   return now_s() - t0;
 }
 
+/* fmo_exact_batch followed by get_SA(j, backward) over the first cap rows
+ * of each interval (bwt.c:1271-1277): rows located and their position sum,
+ * per read and strand. */
+double fmo_exact_locate_batch(const fmo_index *ix, const uint8_t *reads,
+                              const uint64_t *offsets, uint64_t nreads, uint32_t cap,
+                              uint32_t *nrows, uint64_t *possum, int threads) {
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+#endif
+  double t0 = now_s();
+  int64_t lsa = (int64_t)fmo_size_sa(ix) - 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, chunk)
+#endif
+  for (long i = 0; i < (long)nreads; i++) {
+    const uint8_t *W = reads + offsets[i];
+    int16_t len = (int16_t)(offsets[i + 1] - offsets[i]);
+    for (int s = 0; s < 2; s++) {
+      fmo_hit r;
+      memset(&r, 0, sizeof(r));
+      r.k = 0; r.l = lsa; r.start = 0; r.end = (int16_t)(len - 1);
+      r.pos = s ? 0 : (int16_t)(len - 1);
+      fmo_exact(ix, s ? FMO_VIEW_BACKWARD_REV : FMO_VIEW_BACKWARD, s, W, &r, NULL);
+      uint32_t m = 0;
+      uint64_t sum = 0;
+      for (int64_t j = r.k; j <= r.l && m < cap; j++, m++) sum += fmo_get_sa(ix, 0, (uint64_t)j);
+      nrows[2 * i + s] = m;
+      possum[2 * i + s] = sum;
+    }
+  }
+  return now_s() - t0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* k-error search: BWSearchCPU, search/search.c:1171-1405, and its helpers   */
 /* search/search.c:3-1169, search/search.h:66-198                           */

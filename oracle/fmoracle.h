@@ -206,6 +206,11 @@ double fmo_search1_batch(const fmo_index *ix, const uint8_t *reads,
 double fmo_exact_batch(const fmo_index *ix, const uint8_t *reads,
                        const uint64_t *offsets, uint64_t nreads, int64_t *out_kl,
                        int threads);
+/* The same searches, then get_SA(j, backward) over the first cap rows of
+ * each interval: nrows / possum[2*i + strand]. */
+double fmo_exact_locate_batch(const fmo_index *ix, const uint8_t *reads,
+                              const uint64_t *offsets, uint64_t nreads, uint32_t cap,
+                              uint32_t *nrows, uint64_t *possum, int threads);
 
 #ifdef __cplusplus
 }

@@ -122,6 +122,8 @@ def _load_oracle():
     lib.fmo_search1_batch.restype = C.c_double
     lib.fmo_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
     lib.fmo_exact_batch.restype = C.c_double
+    lib.fmo_exact_locate_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, _p, _p, C.c_int]
+    lib.fmo_exact_locate_batch.restype = C.c_double
     return lib
 
 
@@ -304,6 +306,18 @@ class Oracle:
         t = self.lib.fmo_exact_batch(self.ref, _ptr(reads), _ptr(offsets), nr, _ptr(out), threads)
         return t, out.reshape(nr, 4)
 
+    def exact_locate_batch(self, reads, offsets, cap=64, threads=0):
+        """(seconds, rows[n, 2], position sums[n, 2]): exact search of both
+        strands, then get_SA over the first cap rows of each interval."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = len(offsets) - 1
+        rows = np.zeros(max(2 * n, 1), np.uint32)
+        sums = np.zeros(max(2 * n, 1), np.uint64)
+        t = self.lib.fmo_exact_locate_batch(self.ref, _ptr(reads), _ptr(offsets), n, cap,
+                                            _ptr(rows), _ptr(sums), threads)
+        return float(t), rows[:2 * n].reshape(n, 2), sums[:2 * n].reshape(n, 2)
+
 
 def binsearch(array, key):
     array = np.ascontiguousarray(array, dtype=np.uint64)
@@ -374,6 +388,8 @@ class RefCore:
         lib.ref_search1_batch.restype = C.c_double
         lib.ref_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
         lib.ref_exact_batch.restype = C.c_double
+        lib.ref_exact_locate_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, _p, _p, C.c_int]
+        lib.ref_exact_locate_batch.restype = C.c_double
         self.lib = lib
         self._tmp = None
         if oracle is not None:
@@ -473,6 +489,18 @@ class RefCore:
         full = self.lib.ref_search_k(self.h, strand, _ptr(W), len(W), fragsize, list_cap,
                                      anchor_cap, _ptr(fin), C.byref(nf), _ptr(anc), C.byref(na))
         return bool(full), fin[:nf.value].copy(), anc[:na.value].copy()
+
+    def exact_locate_batch(self, reads, offsets, cap=64, threads=0):
+        """(seconds, rows[n, 2], position sums[n, 2]) of the exact search of
+        both strands followed by get_SA over the first cap rows."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = len(offsets) - 1
+        rows = np.zeros(max(2 * n, 1), np.uint32)
+        sums = np.zeros(max(2 * n, 1), np.uint64)
+        t = self.lib.ref_exact_locate_batch(self.h, _ptr(reads), _ptr(offsets), n, cap, _ptr(rows),
+                                            _ptr(sums), threads)
+        return float(t), rows[:2 * n].reshape(n, 2), sums[:2 * n].reshape(n, 2)
 
     def search_k_batch(self, reads, offsets, fragsize, list_cap=128, anchor_cap=16, threads=0):
         """(seconds, final-list sizes[2n]) of BWSearchCPU over a batch."""

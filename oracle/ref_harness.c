@@ -441,6 +441,44 @@ double ref_search_k_batch(void *h, const uint8_t *reads, const uint64_t *offsets
   return now_s() - t0;
 }
 
+/* CPU baseline for exact search + locate: both strands as ref_exact_batch,
+ * then get_SA(j, backward) over the first `cap` rows of each interval (the
+ * loop after the search in bwt_map_exact_seed, bwt.c:1271-1277).  Per read
+ * and strand: rows located and the sum of their positions. */
+double ref_exact_locate_batch(void *h, const uint8_t *reads, const uint64_t *offsets,
+                              uint64_t nreads, uint32_t cap, uint32_t *nrows, uint64_t *possum,
+                              int threads) {
+  ref_ctx *c = (ref_ctx *)h;
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+  double t0 = now_s();
+#pragma omp parallel for schedule(dynamic, chunk)
+  for (long i = 0; i < (long)nreads; i++) {
+    uint8_t *W = (uint8_t *)reads + offsets[i];
+    int16_t len = (int16_t)(offsets[i + 1] - offsets[i]);
+    for (int s = 0; s < 2; s++) {
+      result r;
+      bound_result(&r, 0, len - 1);
+      if (s == 0) {
+        change_result(&r, 0, size_SA(&c->backward) - 1, len - 1);
+        BWExactSearchBackward(W, &c->backward, &r, NULL);
+      } else {
+        change_result(&r, 0, size_SA(&c->backward) - 1, 0);
+        BWExactSearchForward(W, &c->backward_rev, &r, NULL);
+      }
+      uint32_t m = 0;
+      uint64_t sum = 0;
+      for (intmax_t j = r.k; j <= r.l && m < cap; j++, m++)
+        sum += (uint64_t)get_SA((SA_TYPE)j, &c->backward);
+      nrows[2 * i + s] = m;
+      possum[2 * i + s] = sum;
+    }
+  }
+  return now_s() - t0;
+}
+
 /* CPU baseline for exact search: + strand backward on `backward`, - strand
  * forward on `backward_rev` (bwt.c:1222-1238). */
 double ref_exact_batch(void *h, const uint8_t *reads, const uint64_t *offsets,

@@ -122,6 +122,29 @@ def test_locate_golden(gold):
         assert list(pos[i, :len(want)]) == want
 
 
+def test_locate_hits_after_exact(gold):
+    """fmi_locate_hits reads k, l straight from exact-search records: the
+    rows of each read's interval equal the oracle's SA over [k, l] (capped),
+    and a dead interval (k > l, or the (n + 1, n) of an N read) has none."""
+    hp, g, ix, batch = gold
+    n = batch.nreads
+    o = fo.Oracle(g["genome"], ratio=int(g["ratio"]))
+    for view, fwd in ((hp.VIEW_BACKWARD, 0), (hp.VIEW_FORWARD, 1)):
+        out = hp.DeviceBuffer(max(n, 1) * 32)
+        hp.check(hp.lib().fmi_exact_batch(ix.h, view, 0 if fwd == 0 else 1, batch.d_reads.ptr,
+                                          batch.d_offsets.ptr, n, out.ptr, None, None))
+        hp.sync()
+        rec = out.download(hp.HIT_DTYPE, n)
+        cap = 16
+        pos, nrows = hp.locate_hits(ix, out, n, cap, forward=fwd)
+        out.free()
+        for i in range(n):
+            k, l = int(rec["k"][i]), int(rec["l"][i])
+            want = [o.get_sa(fwd, m) for m in range(k, min(l, k + cap - 1) + 1)] if k <= l else []
+            assert int(nrows[i]) == len(want), (view, i)
+            assert list(pos[i, :len(want)]) == want, (view, i)
+
+
 def test_inverse_locate_golden(gold):
     """get_ISA (runtime.h:88) against the golden vectors of the compiled
     reference core, every text position against the oracle, ISA(SA(i)) = i

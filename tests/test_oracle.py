@@ -105,6 +105,28 @@ def test_set_samples_equals_lf_walk():
         o.set_samples(8, full[0][:-8], *full[1:])
 
 
+def test_exact_locate_batch_closed_form():
+    """Exact search of both strands + get_SA: a read copied from strand s of
+    a unique text at position p has one row on strand s, at p, and none on
+    the other; random reads have none; the compiled reference agrees."""
+    g, _, _ = synth.genome(200_000, seed=1)
+    o = fo.Oracle(g, ratio=16)
+    reads, offs, truth = synth.reads(g, 2000, length=100, seed=2, random_frac=0.1)
+    _, rows, sums = o.exact_locate_batch(reads, offs, cap=8)
+    for i in range(len(offs) - 1):
+        if truth["is_random"][i]:
+            assert rows[i].sum() == 0, i
+            continue
+        s = int(truth["strand"][i])
+        assert rows[i, s] == 1 and int(sums[i, s]) == int(truth["pos"][i]), i
+        assert rows[i, 1 - s] == 0, i
+    if fo.ref_available():
+        r = fo.RefCore(oracle=o)
+        _, rr, rs = r.exact_locate_batch(reads, offs, cap=8)
+        assert np.array_equal(rr, rows) and np.array_equal(rs, sums)
+        r.close()
+
+
 # ---------------------------------------------------------------- brute force
 
 def py_suffix_array(text):
