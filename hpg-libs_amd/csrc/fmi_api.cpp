@@ -272,6 +272,18 @@ int fmi_index_load(const char *path, fmi_index **out) {
   return FMI_OK;
 }
 
+int fmi_index_import_reference(const char *dir, fmi_index **out) {
+  if (!dir || !out) return fail(FMI_EINVAL, "NULL argument");
+  *out = nullptr;
+  fmi_index *ix = new (std::nothrow) fmi_index();
+  if (!ix) return fail(FMI_ENOMEM, "out of host memory");
+  std::string err;
+  int rc = fmi::import_reference_dir(dir, ix, &err);
+  if (rc) { delete ix; return fail(rc, err); }
+  *out = ix;
+  return FMI_OK;
+}
+
 void fmi_index_free(fmi_index *ix) {
   if (!ix) return;
   if (ix->on_device) {
@@ -422,7 +434,28 @@ int fmi_index_set_karyotype(fmi_index *ix, uint64_t count, const uint64_t *start
   ix->kstart.assign(start, start + count);
   ix->kend.assign(end, end + count);
   ix->koffset = off;
+  ix->knames.assign(count, std::string());
   return FMI_OK;
+}
+
+int fmi_index_set_contig_names(fmi_index *ix, uint64_t count, const char *const *names) {
+  if (!ix || (count && !names)) return fail(FMI_EINVAL, "NULL argument");
+  if (count != ix->kstart.size()) return fail(FMI_EINVAL, "count differs from the karyotype's");
+  std::vector<std::string> v(count);
+  for (uint64_t c = 0; c < count; ++c) {
+    if (!names[c]) return fail(FMI_EINVAL, "NULL name");
+    v[c] = names[c];
+    if (v[c].size() > 4096) return fail(FMI_ERANGE, "contig name longer than 4096 bytes");
+  }
+  ix->knames = std::move(v);
+  return FMI_OK;
+}
+
+uint64_t fmi_index_contig_count(const fmi_index *ix) { return ix ? ix->kstart.size() : 0; }
+
+const char *fmi_index_contig_name(const fmi_index *ix, uint64_t c) {
+  if (!ix || c >= ix->kstart.size()) return nullptr;
+  return c < ix->knames.size() ? ix->knames[c].c_str() : "";
 }
 
 int fmi_device_count(int *count) {

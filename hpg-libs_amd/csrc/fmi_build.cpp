@@ -169,6 +169,7 @@ int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix, 
   ix->kstart.assign(1, 0);
   ix->kend.assign(1, n ? n - 1 : 0);
   ix->koffset = {0, n};
+  ix->knames.assign(1, std::string());
   return FMI_OK;
 }
 
@@ -178,13 +179,14 @@ int build_index(const uint8_t *text, uint64_t n, uint32_t ratio, fmi_index *ix, 
 //   uint64 n; uint32 ratio; uint32 reserved;
 //   per direction: uint64 dollar; uint32 C[4]; uint64 nblocks; uint64 nsamp;
 //   uint64 kcount; uint64 kstart[kcount]; uint64 kend[kcount];
+//   (version 3) per contig: uint32 length; char name[length];
 //   per direction: OccBlock blk[nblocks]; uint32 S[nsamp]; uint32 R[nsamp];
 //   uint64 checksum (FNV-1a 64 over everything before it).
 namespace {
 
 const char kMagic[8] = {'H', 'P', 'G', 'F', 'M', 'I', '0', '1'};
-// 2: bit-plane occurrence blocks (1 held 2-bit packed codes)
-const uint32_t kFormatVersion = 2;
+// 2: bit-plane occurrence blocks (1 held 2-bit packed codes); 3: + contig names
+const uint32_t kFormatVersion = 3;
 
 struct Fnv {
   uint64_t h = 1469598103934665603ull;
@@ -239,6 +241,12 @@ int save_index(const fmi_index *ix, const char *path, std::string *err) {
   w.put(&kc, 8);
   w.put(ix->kstart.data(), kc * 8);
   w.put(ix->kend.data(), kc * 8);
+  for (uint64_t c = 0; c < kc; ++c) {
+    const std::string nm = c < ix->knames.size() ? ix->knames[c] : std::string();
+    uint32_t len = (uint32_t)nm.size();
+    w.put(&len, 4);
+    w.put(nm.data(), len);
+  }
   for (int d = 0; d < 2; ++d) {
     w.put(ix->h[d].blk, ix->h[d].nblocks * sizeof(OccBlock));
     w.put(ix->h[d].S, ix->h[d].nsamp * 4);
@@ -260,7 +268,8 @@ int load_index(const char *path, fmi_index *ix, std::string *err) {
   r.get(magic, 8);
   r.get(&version, 4);
   r.get(&brows, 4);
-  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || version != kFormatVersion || brows != kBlockRows) {
+  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || (version != kFormatVersion && version != 2) ||
+      brows != kBlockRows) {
     fclose(f);
     *err = "not an index file of this version";
     return FMI_EFORMAT;
@@ -288,6 +297,14 @@ int load_index(const char *path, fmi_index *ix, std::string *err) {
   ix->kend.resize(kc);
   r.get(ix->kstart.data(), kc * 8);
   r.get(ix->kend.data(), kc * 8);
+  ix->knames.assign(kc, std::string());
+  for (uint64_t c = 0; version >= 3 && c < kc && r.ok; ++c) {
+    uint32_t len = 0;
+    r.get(&len, 4);
+    if (!r.ok || len > 4096) { fclose(f); *err = "index contig names are inconsistent"; return FMI_EFORMAT; }
+    ix->knames[c].resize(len);
+    r.get(&ix->knames[c][0], len);
+  }
   for (int d = 0; d < 2; ++d) {
     ix->h[d].blk = alloc_array<OccBlock>(ix->h[d].nblocks);
     ix->h[d].S = alloc_array<uint32_t>(ix->h[d].nsamp);

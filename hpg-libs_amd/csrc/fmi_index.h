@@ -15,6 +15,7 @@ struct fmi_index {
   std::vector<uint64_t> kstart;   // karyotype: start coordinate of contig c
   std::vector<uint64_t> kend;     //            end coordinate of contig c
   std::vector<uint64_t> koffset;  //            text offset of contig c (count+1)
+  std::vector<std::string> knames;  //          name of contig c (may be empty)
 
   // device copy
   bool on_device = false;
@@ -44,6 +45,8 @@ int seed_map(const fmi_index *ix, const uint8_t *reads, const uint64_t *offsets,
              uint64_t info_cap, fmi_seed_region *regions, uint64_t region_cap, uint64_t *nseeds,
              uint64_t *nrows, void *s, std::string *err);
 int save_index(const fmi_index *ix, const char *path, std::string *err);
+// Reads the reference's index directory (fmi_import.cpp).
+int import_reference_dir(const char *dir, fmi_index *ix, std::string *err);
 int load_index(const char *path, fmi_index *ix, std::string *err);
 // The reference's GPU design on this device (fmi_refplain.hip).
 struct RefPlain;

@@ -40,7 +40,9 @@ MAX_READ_LEN = 1000
 # Every symbol include/fmi.h declares; tests check the library exports them.
 EXPORTS = [
     "fmi_status_str", "fmi_last_error", "fmi_index_build", "fmi_index_build_gpu", "fmi_index_save",
-    "fmi_index_load", "fmi_index_free", "fmi_index_size_sa", "fmi_index_sa_ratio",
+    "fmi_index_load", "fmi_index_import_reference", "fmi_index_free", "fmi_index_size_sa",
+    "fmi_index_sa_ratio", "fmi_index_set_contig_names", "fmi_index_contig_count",
+    "fmi_index_contig_name",
     "fmi_index_device_bytes", "fmi_index_to_device", "fmi_index_get_bwt",
     "fmi_index_set_karyotype", "fmi_device_count", "fmi_device_alloc", "fmi_device_free",
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
@@ -91,6 +93,10 @@ def _bind(lib):
         "fmi_index_build_gpu": ([_p, _u64, _u32, C.POINTER(_p)], _i),
         "fmi_index_save": ([_p, C.c_char_p], _i),
         "fmi_index_load": ([C.c_char_p, C.POINTER(_p)], _i),
+        "fmi_index_import_reference": ([C.c_char_p, C.POINTER(_p)], _i),
+        "fmi_index_set_contig_names": ([_p, _u64, _p], _i),
+        "fmi_index_contig_count": ([_p], _u64),
+        "fmi_index_contig_name": ([_p, _u64], C.c_char_p),
         "fmi_index_free": ([_p], None),
         "fmi_index_size_sa": ([_p], _u64),
         "fmi_index_sa_ratio": ([_p], _u32),
@@ -312,8 +318,25 @@ class Index:
         check(lib().fmi_index_load(os.fsencode(path), C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def import_reference(cls, directory):
+        """fmi_index_import_reference: the index directory the reference's
+        builder writes (C, C1, O/Oi, S/R, Si/Ri, config.txt, index)."""
+        h = C.c_void_p()
+        check(lib().fmi_index_import_reference(os.fsencode(directory), C.byref(h)))
+        return cls(h)
+
     def save(self, path):
         check(lib().fmi_index_save(self.h, os.fsencode(path)))
+
+    def contig_names(self):
+        L = lib()
+        return [L.fmi_index_contig_name(self.h, c).decode()
+                for c in range(int(L.fmi_index_contig_count(self.h)))]
+
+    def set_contig_names(self, names):
+        arr = (C.c_char_p * max(len(names), 1))(*[x.encode() for x in names])
+        check(lib().fmi_index_set_contig_names(self.h, len(names), arr))
 
     def close(self):
         if self.h is not None:

@@ -114,10 +114,22 @@ int fmi_index_build(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_inde
 int fmi_index_build_gpu(const uint8_t *text, uint64_t n, uint32_t sa_ratio, fmi_index **out);
 /* Writes / reads the single-file index ("HPGFMI01" magic and format
  * version, then per direction the '$' row, C, the occurrence blocks and the
- * SA / ISA samples, then the karyotype, with an FNV-1a checksum).  Replaces the reference's index
+ * SA / ISA samples, then the karyotype and contig names, with an FNV-1a
+ * checksum; files of format 2, without names, still load).  Replaces the reference's index
  * directory reader load_bwt_index (search/runtime.h:90-124). */
 int fmi_index_save(const fmi_index *ix, const char *path);
 int fmi_index_load(const char *path, fmi_index **out);
+/* Reads an index directory written by the reference's builder
+ * (bwt_generate_index_files, bwt.c:744-819) the way bwt_index_new
+ * (bwt.c:682-712) does: config.txt (read_config, search/io.c:336-358),
+ * C / C1, O / Oi (desp + count), S / R and Si / Ri (load_bwt_index,
+ * search/runtime.h:90-124; read_vector, read_comp_vector, read_comp_matrix,
+ * search/csafm.c:121-250) and the karyotype file `index` (load_exome_file,
+ * search/io.c:26-81).  The occurrence bitmaps are re-laid out as this
+ * engine's blocks; desp, C, C1 and the sample headers are checked against
+ * them.  FMI_EIO when a file is missing, FMI_EFORMAT when the files are
+ * inconsistent, use another alphabet than ACGT or a duplicated strand. */
+int fmi_index_import_reference(const char *dir, fmi_index **out);
 void fmi_index_free(fmi_index *ix);
 /* Keeps the full suffix array and its inverse of both directions and the text
  * (2 bits per base) on the device, 16 bytes per base + n / 4 (3.1 Gbp:
@@ -147,6 +159,12 @@ int fmi_index_get_bwt(const fmi_index *ix, int forward, uint8_t *out_n_bytes, ui
  * offset[c+1] = offset[c] + end[c] - start[c] + 1). */
 int fmi_index_set_karyotype(fmi_index *ix, uint64_t count, const uint64_t *start,
                             const uint64_t *end);
+/* Contig names (the `>name` of each karyotype line, search/io.c:47-56);
+ * set_karyotype clears them. */
+int fmi_index_set_contig_names(fmi_index *ix, uint64_t count, const char *const *names);
+uint64_t fmi_index_contig_count(const fmi_index *ix);
+/* Name of contig c ("" when unnamed), NULL past the end; valid while ix is. */
+const char *fmi_index_contig_name(const fmi_index *ix, uint64_t c);
 
 /* ------------------------------------------------------ device plumbing */
 int fmi_device_count(int *count);

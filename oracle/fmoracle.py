@@ -388,6 +388,7 @@ class RefCore:
         lib.ref_search1_batch.restype = C.c_double
         lib.ref_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
         lib.ref_exact_batch.restype = C.c_double
+        lib.ref_save_dir.argtypes = [_p, C.c_char_p, C.c_uint64, _p, _p]
         lib.ref_exact_locate_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, _p, _p, C.c_int]
         lib.ref_exact_locate_batch.restype = C.c_double
         self.lib = lib
@@ -489,6 +490,17 @@ class RefCore:
         full = self.lib.ref_search_k(self.h, strand, _ptr(W), len(W), fragsize, list_cap,
                                      anchor_cap, _ptr(fin), C.byref(nf), _ptr(anc), C.byref(na))
         return bool(full), fin[:nf.value].copy(), anc[:na.value].copy()
+
+    def save_dir(self, path, starts=None, ends=None):
+        """Writes the reference's index directory (its own writers) to path;
+        starts / ends: the karyotype (default: one contig over the text)."""
+        if starts is None:
+            starts, ends = [0], [self.n - 1]
+        st = np.ascontiguousarray(starts, dtype=np.uint64)
+        en = np.ascontiguousarray(ends, dtype=np.uint64)
+        os.makedirs(path, exist_ok=True)
+        if self.lib.ref_save_dir(self.h, path.encode(), len(st), _ptr(st), _ptr(en)) != 0:
+            raise ValueError("too many contigs")
 
     def exact_locate_batch(self, reads, offsets, cap=64, threads=0):
         """(seconds, rows[n, 2], position sums[n, 2]) of the exact search of

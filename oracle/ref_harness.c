@@ -95,6 +95,37 @@ void *ref_build(const uint8_t *text, uint64_t n, uint32_t ratio, const char *wor
   return c;
 }
 
+/* Writes the index directory bwt_generate_index_files (bwt.c:744-819)
+ * would write, with the reference's own writers: config.txt, C, C1, O, S,
+ * R, Oi, Si, Ri and the karyotype file `index` (contigs c of the text:
+ * names chr1.., start[c], end[c]).  save_exome_file leaves its stream open
+ * (search/io.c:83-103), so the streams are flushed here. */
+int ref_save_dir(void *h, const char *dir, uint64_t count, const uint64_t *start,
+                 const uint64_t *end) {
+  ref_ctx *c = (ref_ctx *)h;
+  save_config("ACGT", false, dir);
+  save_vector(&c->backward.C, dir, "C");
+  save_vector(&c->backward.C1, dir, "C1");
+  save_comp_matrix(&c->backward.O, dir, "O");
+  save_comp_vector(&c->backward.S, dir, "S");
+  save_comp_vector(&c->backward.R, dir, "R");
+  save_comp_matrix(&c->forward.O, dir, "Oi");
+  save_comp_vector(&c->forward.S, dir, "Si");
+  save_comp_vector(&c->forward.R, dir, "Ri");
+  exome *ex = exome_new();
+  if (count > (uint64_t)ex->max_chromosomes) { exome_free(ex); return -1; }
+  for (uint64_t i = 0; i < count; i++) {
+    snprintf(ex->chromosome + i * IDMAX, IDMAX, "chr%llu", (unsigned long long)(i + 1));
+    ex->start[i] = start[i];
+    ex->end[i] = end[i];
+  }
+  ex->size = count;
+  save_exome_file(ex, false, dir);
+  fflush(NULL);
+  exome_free(ex);
+  return 0;
+}
+
 /* Wraps index arrays the caller owns (the reference layout: C, C1, per-symbol
  * desp/count, S, R) into the reference's bwt_index structs, without running
  * the reference builder.  Used for large texts, where dbwt and the serial
