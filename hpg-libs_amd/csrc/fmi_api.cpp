@@ -797,6 +797,30 @@ int fmi_seed_map_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64
   return rc ? fail(rc, err) : FMI_OK;
 }
 
+int fmi_map_inexact_seeds_batch(const fmi_index *ix, const uint8_t *d_reads,
+                                const uint64_t *d_offsets, uint32_t nreads, uint32_t seed_size,
+                                uint32_t min_seed_size, uint32_t rec_cap, uint32_t region_cap,
+                                uint64_t windows_cap, fmi_inexact_region *d_regions,
+                                uint32_t *d_nregions, uint8_t *d_status, uint64_t *nwindows,
+                                fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (!nwindows) return fail(FMI_EINVAL, "NULL argument");
+  *nwindows = 0;
+  if (seed_size == 0 || seed_size > FMI_MAX_READ_LEN || min_seed_size == 0)
+    return fail(FMI_EINVAL, "seed_size must be 1..1000 and min_seed_size >= 1");
+  if (rec_cap == 0 || rec_cap > 20000 || region_cap == 0 || region_cap > 65536)
+    return fail(FMI_EINVAL, "rec_cap must be 1..20000 and region_cap 1..65536");
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_regions || !d_nregions || !d_status)
+    return fail(FMI_EINVAL, "NULL buffer");
+  std::string err;
+  rc = fmi::map_inexact_seeds(ix, d_reads, d_offsets, nreads, seed_size, min_seed_size, rec_cap,
+                              region_cap, windows_cap, d_regions, d_nregions, d_status, nwindows,
+                              s, &err);
+  return rc ? fail(rc, err) : FMI_OK;
+}
+
 int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uint64_t nlines,
                      uint64_t *d_sink, float *ms, uint64_t *lines_done, fmi_stream s) {
   if (!d_buf || !d_sink || !ms || !lines_done) return fail(FMI_EINVAL, "NULL argument");

@@ -44,6 +44,11 @@ int seed_map(const fmi_index *ix, const uint8_t *reads, const uint64_t *offsets,
              uint32_t seed_len, uint32_t stride, uint32_t filter, fmi_seed_info *info,
              uint64_t info_cap, fmi_seed_region *regions, uint64_t region_cap, uint64_t *nseeds,
              uint64_t *nrows, void *s, std::string *err);
+int map_inexact_seeds(const fmi_index *ix, const uint8_t *reads, const uint64_t *offsets,
+                      uint32_t nreads, uint32_t seed_size, uint32_t min_seed_size,
+                      uint32_t rec_cap, uint32_t region_cap, uint64_t windows_cap,
+                      fmi_inexact_region *regions, uint32_t *nregions, uint8_t *status,
+                      uint64_t *nwindows, void *s, std::string *err);
 int save_index(const fmi_index *ix, const char *path, std::string *err);
 // Reads the reference's index directory (fmi_import.cpp).
 int import_reference_dir(const char *dir, fmi_index *ix, std::string *err);

@@ -23,6 +23,7 @@
 #include "fmi_device.h"
 #include "fmi_index.h"
 #include "fmi_internal.h"
+#include "fmi_vhelper.h"
 
 namespace fmi {
 
@@ -164,6 +165,184 @@ __global__ __launch_bounds__(256) void seed_rows_kernel(RowParams P) {
   }
 }
 
+// ---------------------------------------------------------------- inexact seeds
+// bwt_map_inexact_seeds_seq (bwt.c:3262-3302) + bwt_map_inexact_seed
+// (bwt.c:1401-1644):
+//   1. windows per read (len / seed_size, plus the remainder when it holds
+//      at least min_seed_size bases), exclusive scan -> window numbering
+//   2. per chunk of windows: one lane per (window, strand) runs the vector
+//      helper (vh_task) into a record slot of rec_cap records
+//   3. one lane per window turns both strands' records into regions in the
+//      caller's slots, + strand first because final_pos carries over, then
+//      dedups and reverses each strand's list in place.
+
+__global__ void win_count_kernel(const uint64_t *offsets, uint32_t nreads, uint32_t seed_size,
+                                 uint32_t min_seed_size, uint64_t *cnt) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= nreads;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t c = 0;
+    if (r < nreads) {
+      const uint64_t len = offsets[r + 1] - offsets[r];
+      c = len / seed_size + (len % seed_size >= min_seed_size ? 1 : 0);
+    }
+    cnt[r] = c;
+  }
+}
+
+struct WinGeom {
+  uint64_t read, off;
+  int32_t len, start, end, w;
+};
+
+__device__ __forceinline__ WinGeom win_geom(const uint64_t *offsets, const uint64_t *win_base,
+                                            uint32_t nreads, uint32_t seed_size, uint64_t g) {
+  WinGeom G;
+  G.read = owner(win_base, nreads, g);
+  G.off = offsets[G.read];
+  G.len = (int32_t)(offsets[G.read + 1] - G.off);
+  G.w = (int32_t)(g - win_base[G.read]);
+  const int32_t num_seeds = G.len / (int32_t)seed_size;
+  G.start = G.w * (int32_t)seed_size;
+  G.end = G.w < num_seeds ? G.start + (int32_t)seed_size - 1 : G.len - 1;
+  return G;
+}
+
+struct WinParams {
+  DevIndex ix;
+  const uint8_t *reads;
+  const uint64_t *offsets;
+  const uint64_t *win_base;    // nreads + 1
+  uint64_t g0, nwin;           // windows [g0, g0 + nwin) of this chunk
+  uint32_t nreads, seed_size, rec_cap;
+  fmi_hit *recs;               // 2 * nwin * rec_cap
+  uint32_t *rcount;            // 2 * nwin
+  uint2 *vscratch;             // 2 * nwin * (seed_size + 3)
+};
+
+__global__ __launch_bounds__(256) void inexact_window_kernel(WinParams P) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * P.nwin;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const WinGeom G = win_geom(P.offsets, P.win_base, P.nreads, P.seed_size, P.g0 + (t >> 1));
+    const uint32_t strand = (uint32_t)(t & 1u);
+    if (G.end - G.start < 2) { P.rcount[t] = 0; continue; }
+    P.rcount[t] = vh_task(P.ix, P.reads + G.off, G.start, G.end, strand, 0u,
+                          P.vscratch + t * (uint64_t)(P.seed_size + 3),
+                          P.recs + t * (uint64_t)P.rec_cap, P.rec_cap);
+  }
+}
+
+struct RegParams {
+  DevIndex ix;
+  const uint64_t *offsets;
+  const uint64_t *win_base;
+  uint64_t g0, nwin;
+  uint32_t nreads, seed_size, rec_cap, region_cap;
+  const fmi_hit *recs;
+  const uint32_t *rcount;
+  const uint64_t *koff;        // kcount + 1
+  const uint64_t *kstart;
+  uint32_t kcount;
+  fmi_inexact_region *out;     // caller's slots, indexed by the global window
+  uint32_t *nreg;
+  uint8_t *status;
+};
+
+__global__ __launch_bounds__(256) void inexact_region_kernel(RegParams P) {
+  const uint64_t size_sa = (uint64_t)P.ix.n + 1;
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < P.nwin;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t g = P.g0 + q;
+    const WinGeom G = win_geom(P.offsets, P.win_base, P.nreads, P.seed_size, g);
+    const int64_t len = (int64_t)G.end - G.start + 1;
+    int32_t final_pos = 0, final_err;
+    for (uint32_t s = 0; s < 2; ++s) {
+      const uint32_t type = 1u - s;               // bwt.c:1460: type 1 (+) first
+      const uint64_t slot = 2 * g + s, lslot = 2 * q + s;
+      fmi_inexact_region *out = P.out + slot * P.region_cap;
+      const uint32_t cnt = P.rcount[lslot];
+      uint8_t st = cnt > P.rec_cap ? 1 : 0;
+      const uint32_t nrec = cnt < P.rec_cap ? cnt : P.rec_cap;
+      const uint32_t aux_s = type ? (uint32_t)G.start : (uint32_t)(G.len - G.end - 1);
+      const uint32_t aux_e = type ? (uint32_t)G.end : (uint32_t)(G.len - G.start - 1);
+      uint32_t m = 0;
+      for (uint32_t ii = 0; ii < nrec; ++ii) {
+        const fmi_hit r = P.recs[lslot * P.rec_cap + ii];
+        const uint32_t error = r.num_mismatches ? r.err_kind : 0u;
+        int64_t len_calc = len;
+        if (error == FMI_DELETION) len_calc--;
+        else if (error == FMI_INSERTION) len_calc++;
+        const int32_t pos = r.err_pos;
+        if (error == FMI_DELETION || pos == 0 || pos == len - 1) {
+          final_pos = type ? pos : (int32_t)(len - 1 - pos);
+        } else if (error == FMI_INSERTION) {
+          if (type) final_pos = r.dir ? pos : pos + 1;
+          else final_pos = r.dir ? (int32_t)(len - pos) : (int32_t)(len - pos - 1);
+        }
+        if (error == FMI_MISMATCH) final_err = -1;
+        else if (error == FMI_DELETION) final_err = 1;      // SEED_INSERTION
+        else final_err = 2;                                 // SEED_DELETION
+        final_err += G.start;
+        const int64_t k0 = r.k;
+        const int64_t l0 = (r.l - r.k + 1) < 50 ? r.l : r.k + 1;   // bwt.c:1539-1547
+        const uint32_t direction = type ? r.dir : !r.dir;
+        for (int64_t j = k0; j <= l0; ++j) {
+          const uint64_t key =
+              direction ? size_sa - sa_value(P.ix.d[1], P.ix.ratio, P.ix.n, (uint32_t)j) -
+                              (uint64_t)len_calc - 1
+                        : sa_value(P.ix.d[0], P.ix.ratio, P.ix.n, (uint32_t)j);
+          uint64_t lo = 0, size = P.kcount;        // binsearch, search/io.h:43-64
+          while (size > 0) {
+            const uint64_t w = size / 2;
+            if (P.koff[lo + w] <= key) { lo += w + 1; size -= w + 1; }
+            else size = w;
+          }
+          if (!(lo >= 1 && key + (uint64_t)len_calc <= P.koff[lo])) continue;
+          if (m >= P.region_cap) { st |= 2; continue; }
+          fmi_inexact_region R;
+          R.start = P.kstart[lo - 1] + (key - P.koff[lo - 1]) + 1;
+          R.end = R.start + (uint64_t)len - 1;
+          R.chrom_idx = (uint32_t)lo;
+          R.read = (uint32_t)G.read;
+          R.seq_start = (uint16_t)aux_s;
+          R.seq_end = (uint16_t)aux_e;
+          R.seq_len = (uint16_t)G.len;
+          R.seed_id = (uint16_t)G.w;
+          R.pos_err = final_pos;
+          R.type_err = final_err;
+          R.strand = (uint8_t)s;
+          for (int b = 0; b < 7; ++b) R.pad[b] = 0;
+          out[m++] = R;
+        }
+      }
+      // dedup within 10 bp (bwt.c:1593-1615), the mark kept in pad[0]
+      for (int32_t a1 = (int32_t)m - 1; a1 >= 1; --a1) {
+        if (out[a1].pad[0]) continue;
+        const uint64_t s1 = out[a1].start;
+        const uint32_t c1 = out[a1].chrom_idx;
+        for (int32_t a2 = a1 - 1; a2 >= 0; --a2) {
+          const int32_t d = (int32_t)(uint32_t)(s1 - out[a2].start);
+          const int64_t dist = d < 0 ? -(int64_t)d : (int64_t)d;
+          if (c1 == out[a2].chrom_idx && dist < 10 && !out[a2].pad[0]) out[a2].pad[0] = 1;
+        }
+      }
+      // kept regions, handed over from the last to the first (bwt.c:1616-1626)
+      uint32_t kept = 0;
+      for (uint32_t i = 0; i < m; ++i) {
+        if (out[i].pad[0]) continue;
+        fmi_inexact_region R = out[i];
+        out[kept++] = R;
+      }
+      for (uint32_t i = 0; i < kept / 2; ++i) {
+        const fmi_inexact_region a = out[i];
+        out[i] = out[kept - 1 - i];
+        out[kept - 1 - i] = a;
+      }
+      P.nreg[slot] = kept;
+      P.status[slot] = st;
+    }
+  }
+}
+
 inline dim3 grid_for(uint64_t m) {
   uint64_t g = (m + 255) / 256;
   if (g > 65536ull * 4) g = 65536ull * 4;
@@ -250,6 +429,73 @@ done:
   (void)hipFree(rows);
   (void)hipFree(rbase);
   (void)hipFree(kl);
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int map_inexact_seeds(const fmi_index *ix, const uint8_t *reads, const uint64_t *offsets,
+                      uint32_t nreads, uint32_t seed_size, uint32_t min_seed_size,
+                      uint32_t rec_cap, uint32_t region_cap, uint64_t windows_cap,
+                      fmi_inexact_region *regions, uint32_t *nregions, uint8_t *status,
+                      uint64_t *nwindows, void *stream, std::string *err) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = FMI_OK;
+  uint64_t *cnt = nullptr, *base = nullptr;
+  fmi_hit *recs = nullptr;
+  uint32_t *rcount = nullptr;
+  uint2 *vscr = nullptr;
+  void *tmp = nullptr;
+  size_t need = 0;
+  uint64_t nwin = 0, chunk = 0;
+  *nwindows = 0;
+  SEED_TRY(hipMalloc(&cnt, (uint64_t)(nreads + 1) * 8));
+  SEED_TRY(hipMalloc(&base, (uint64_t)(nreads + 1) * 8));
+  win_count_kernel<<<grid_for(nreads + 1), 256, 0, s>>>(offsets, nreads, seed_size,
+                                                        min_seed_size, cnt);
+  SEED_TRY(hipGetLastError());
+  SEED_TRY(rocprim::exclusive_scan(nullptr, need, cnt, base, (uint64_t)0, (uint64_t)nreads + 1,
+                                   rocprim::plus<uint64_t>(), s));
+  SEED_TRY(hipMalloc(&tmp, need));
+  SEED_TRY(rocprim::exclusive_scan(tmp, need, cnt, base, (uint64_t)0, (uint64_t)nreads + 1,
+                                   rocprim::plus<uint64_t>(), s));
+  SEED_TRY(hipMemcpyAsync(&nwin, base + nreads, 8, hipMemcpyDeviceToHost, s));
+  SEED_TRY(hipStreamSynchronize(s));
+  *nwindows = nwin;
+  if (nwin > windows_cap) {
+    *err = "the batch has " + std::to_string(nwin) + " windows, more than windows_cap";
+    rc = FMI_ERANGE;
+    goto done;
+  }
+  if (nwin == 0) goto done;
+  // chunks of windows bounded to ~2 GiB of record scratch
+  chunk = (2ull << 30) / (2ull * rec_cap * sizeof(fmi_hit));
+  if (chunk < 1) chunk = 1;
+  if (chunk > nwin) chunk = nwin;
+  SEED_TRY(hipMalloc(&recs, 2 * chunk * rec_cap * sizeof(fmi_hit)));
+  SEED_TRY(hipMalloc(&rcount, 2 * chunk * 4));
+  SEED_TRY(hipMalloc(&vscr, 2 * chunk * (uint64_t)(seed_size + 3) * sizeof(uint2)));
+  for (uint64_t g0 = 0; g0 < nwin; g0 += chunk) {
+    const uint64_t m = nwin - g0 < chunk ? nwin - g0 : chunk;
+    WinParams W{ix->dev, reads, offsets, base, g0, m, nreads, seed_size, rec_cap, recs, rcount,
+                vscr};
+    inexact_window_kernel<<<grid_for(2 * m), 256, 0, s>>>(W);
+    SEED_TRY(hipGetLastError());
+    RegParams R{ix->dev, offsets, base, g0, m, nreads, seed_size, rec_cap, region_cap, recs,
+                rcount, ix->d_koffset, ix->d_kstart, (uint32_t)ix->kstart.size(), regions,
+                nregions, status};
+    inexact_region_kernel<<<grid_for(m), 256, 0, s>>>(R);
+    SEED_TRY(hipGetLastError());
+  }
+done:
+  if (rc == FMI_OK) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { *err = hipGetErrorString(e); rc = FMI_EDEVICE; }
+  }
+  (void)hipFree(cnt);
+  (void)hipFree(base);
+  (void)hipFree(recs);
+  (void)hipFree(rcount);
+  (void)hipFree(vscr);
   (void)hipFree(tmp);
   return rc;
 }

@@ -1,0 +1,244 @@
+// fmi_vhelper.h -- BWSearch1VectorHelper (search/search.c:1491-1709) for one
+// window of one read on one strand, shared by the vector-driven 1-error
+// search (fmi_search1_vector_batch) and the inexact seed mapping
+// (fmi_map_inexact_seeds_batch).
+//
+// The helper walks the backward vector from the left end of the window and
+// the forward vector from the right end, the reverse of the order the
+// vectors are built in, so the lane first writes the parts it reads
+// (entries 0..n/2 of the backward vector and n/2-1..n of the forward one,
+// n + 3 intervals) to a scratch slice.  One record state is carried from
+// record to record, as the reference reuses one result: each step assigns
+// only the fields the reference assigns, so the error fields a record
+// inherits (a mismatch after the first block's insertion keeps position b)
+// are the same.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fmi.h"
+#include "fmi_device.h"
+#include "fmi_internal.h"
+
+namespace fmi {
+
+struct VhView {
+  const DevDir *D;
+  uint32_t comp, n;
+  __device__ __forceinline__ void lf(uint32_t k, uint32_t l, uint32_t b, uint32_t &ko,
+                                     uint32_t &lo) const {
+    if (b > 3u) { ko = n + 1u; lo = n; return; }   // N never occurs
+    uint32_t nb;
+    lf_one(*D, k, l, comp ? 3u - b : b, ko, lo, nb);
+  }
+};
+
+struct VhRec {
+  uint32_t k, l;
+  int32_t pos, err_pos;
+  uint32_t dir, nmm, kind, base;
+};
+
+struct VhSink {
+  fmi_hit *out;
+  uint32_t cap, cnt, mm_only, strand;
+  int16_t start, end;
+  __device__ __forceinline__ void add(const VhRec &r) {
+    if (mm_only && r.nmm && r.kind != FMI_MISMATCH) return;
+    if (cnt < cap)
+      store_hit(out + cnt, (int64_t)r.k, (int64_t)r.l, start, (int16_t)r.pos, end,
+                r.nmm ? (int16_t)r.err_pos : (int16_t)0, (uint8_t)r.dir, (uint8_t)r.nmm,
+                r.nmm ? (uint8_t)r.kind : (uint8_t)0, r.nmm ? (uint8_t)r.base : (uint8_t)0,
+                (uint8_t)strand);
+    ++cnt;
+  }
+};
+
+// BWExactSearchBackward (dir 0) / Forward (dir 1) on the record state.
+__device__ __forceinline__ void vh_exact(const VhView &V, const uint8_t *W, int32_t start,
+                                         int32_t end, uint32_t dir, VhRec &r) {
+  uint32_t k2 = r.k, l2 = r.l;
+  int32_t i = r.pos;
+  if (dir == 0) {
+    for (; i >= start; --i) {
+      V.lf(k2, l2, W[i], k2, l2);
+      if (k2 > l2) break;
+    }
+  } else {
+    for (; i <= end; ++i) {
+      V.lf(k2, l2, W[i], k2, l2);
+      if (k2 > l2) break;
+    }
+  }
+  r.k = k2; r.l = l2; r.pos = i;
+}
+
+// The helper over W[start..end] (end - start >= 2) on strand 0 (backward,
+// forward) or 1 (forward_rev, backward_rev); vb: n + 3 intervals of scratch.
+// Stores at most cap records at out; returns the count of all of them.
+__device__ inline uint32_t vh_task(const DevIndex &ix, const uint8_t *W, int32_t start,
+                                   int32_t end, uint32_t strand, uint32_t mm_only, uint2 *vb,
+                                   fmi_hit *out, uint32_t cap) {
+  const uint32_t n_sa = ix.n;
+  // + strand: backward / forward; - strand: forward_rev / backward_rev
+  const VhView B{strand ? &ix.d[1] : &ix.d[0], strand, n_sa};
+  const VhView F{strand ? &ix.d[0] : &ix.d[1], strand, n_sa};
+  const int32_t n = end - start;
+  int32_t half = n / 2;
+  uint2 *vf = vb + (half + 1);     // forward vector entries half-1 .. n
+
+  // BWExactSearchVectorBackward over W[start..end]: entry j = W[start+j..end]
+  {
+    uint32_t k = 0, l = n_sa;
+    bool dead = false;
+    for (int32_t j = n; j >= 0; --j) {
+      if (!dead) {
+        B.lf(k, l, W[start + j], k, l);
+        dead = k > l;
+      }
+      if (j <= half) vb[j] = make_uint2(k, l);
+    }
+  }
+  // BWExactSearchVectorForward: entry j = W[start..start+j]
+  {
+    uint32_t k = 0, l = n_sa;
+    bool dead = false;
+    for (int32_t j = 0; j <= n; ++j) {
+      if (!dead) {
+        F.lf(k, l, W[start + j], k, l);
+        dead = k > l;
+      }
+      if (j >= half - 1) vf[j - (half - 1)] = make_uint2(k, l);
+    }
+  }
+
+  VhSink S{out, cap, 0u, mm_only, strand, (int16_t)start, (int16_t)end};
+  VhRec r{0, 0, -1, 0, 0, 0, 0, 0};
+  uint2 v = vb[0];
+  if (v.x <= v.y) { r.k = v.x; r.l = v.y; r.pos = -1; S.add(r); }
+  r.nmm = 1; r.kind = FMI_MATCH; r.base = 0xFFu; r.err_pos = start;
+
+  uint32_t k, l, ka, la;
+  int64_t size = (int64_t)v.y - (int64_t)v.x, size_last;
+  // an error at W[start]
+  size_last = size;
+  v = vb[1];
+  k = v.x; l = v.y;
+  size = (int64_t)l - (int64_t)k;
+  if (size != size_last) {
+    r.k = k; r.l = l; r.pos = -1;
+    r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = start;
+    S.add(r);
+    const uint32_t bw = W[start];
+    for (uint32_t b = 0; b < 4u; ++b) {
+      B.lf(k, l, b, ka, la);
+      if (ka > la) continue;
+      if (b != bw) {
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = FMI_MISMATCH; r.base = b;
+        S.add(r);
+      }
+      B.lf(ka, la, bw, ka, la);
+      if (ka <= la) {
+        // search.c:1556 stores kind 3, base INSERTION (3), position b
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = 3u; r.base = FMI_INSERTION; r.err_pos = (int32_t)b;
+        S.add(r);
+      }
+    }
+  }
+  for (int32_t i = start + 2, j = 2; j <= half; ++i, ++j) {
+    size_last = size;
+    v = vb[j];
+    k = v.x; l = v.y;
+    size = (int64_t)l - (int64_t)k;
+    if (size == size_last) continue;
+    r.k = k; r.l = l; r.pos = i - 2;
+    r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = i - 1;
+    vh_exact(B, W, start, end, 0, r);
+    if (r.k <= r.l) S.add(r);
+    for (uint32_t b = 0; b < 4u; ++b) {
+      B.lf(k, l, b, ka, la);
+      if (ka > la) continue;
+      if (!mm_only) {
+        r.k = ka; r.l = la; r.pos = i - 1;
+        r.kind = FMI_INSERTION; r.base = b;
+        vh_exact(B, W, start, end, 0, r);
+        if (r.k <= r.l) S.add(r);
+      } else {
+        r.kind = FMI_INSERTION; r.base = b;
+      }
+      if (b != W[i - 1]) {
+        r.k = ka; r.l = la; r.pos = i - 2;
+        r.kind = FMI_MISMATCH;
+        vh_exact(B, W, start, end, 0, r);
+        if (r.k <= r.l) S.add(r);
+      }
+    }
+  }
+
+  half -= 1;
+  v = vf[n - half];                // entry n
+  size = (int64_t)v.y - (int64_t)v.x;
+  r.dir = 1;
+  // an error at W[end]
+  size_last = size;
+  v = vf[n - 1 - half];            // entry n - 1
+  k = v.x; l = v.y;
+  size = (int64_t)l - (int64_t)k;
+  if (size != size_last) {
+    r.k = k; r.l = l; r.pos = -1;
+    r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = end;
+    S.add(r);
+    const uint32_t bw = W[end];
+    for (uint32_t b = 0; b < 4u; ++b) {
+      F.lf(k, l, b, ka, la);
+      if (ka > la) continue;
+      if (b != bw) {
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = FMI_MISMATCH; r.base = b;
+        S.add(r);
+      }
+      F.lf(ka, la, bw, ka, la);
+      if (ka <= la) {
+        r.k = ka; r.l = la; r.pos = -1;
+        r.kind = FMI_INSERTION; r.base = b;
+        S.add(r);
+      }
+    }
+  }
+  for (int32_t i = end - 2, j = n - 2; j >= half; --i, --j) {
+    size_last = size;
+    v = vf[j - half];
+    k = v.x; l = v.y;
+    size = (int64_t)l - (int64_t)k;
+    if (size == size_last) continue;
+    r.k = k; r.l = l; r.pos = i + 2;
+    r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = i + 1;
+    vh_exact(F, W, start, end, 1, r);
+    if (r.k <= r.l) S.add(r);
+    for (uint32_t b = 0; b < 4u; ++b) {
+      F.lf(k, l, b, ka, la);
+      if (ka > la) continue;
+      if (!mm_only) {
+        r.k = ka; r.l = la; r.pos = i + 1;
+        r.kind = FMI_INSERTION; r.base = b;
+        vh_exact(F, W, start, end, 1, r);
+        if (r.k <= r.l) S.add(r);
+      } else {
+        r.kind = FMI_INSERTION; r.base = b;
+      }
+      if (b != W[i + 1]) {
+        r.k = ka; r.l = la; r.pos = i + 2;
+        r.kind = FMI_MISMATCH;
+        vh_exact(F, W, start, end, 1, r);
+        if (r.k <= r.l) S.add(r);
+      }
+    }
+  }
+  return S.cnt;
+}
+
+}  // namespace fmi

@@ -51,7 +51,7 @@ EXPORTS = [
     "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits",
     "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
-    "fmi_seed_map_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
+    "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_get_full",
     "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
@@ -146,6 +146,8 @@ def _bind(lib):
                               _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
                                       C.POINTER(_u64)], _i),
+        "fmi_map_inexact_seeds_batch": ([_p, _p, _p, _u32, _u32, _u32, _u32, _u32, _u64, _p,
+                                         _p, _p, C.POINTER(_u64), _p], _i),
         "fmi_seed_map_batch": ([_p, _p, _p, _u32, _u32, _u32, _u32, _p, _u64, _p, _u64,
                                 C.POINTER(_u64), C.POINTER(_u64), _p], _i),
         "fmi_server_create": ([_p, C.POINTER(ServerConfig), C.POINTER(_p)], _i),
@@ -418,6 +420,7 @@ class ReadBatch:
         self.total = int(offsets[-1])
         self.d_reads = DeviceBuffer.from_array(reads if len(reads) else np.zeros(1, np.uint8), stream)
         self.d_offsets = DeviceBuffer.from_array(offsets, stream)
+        self.offsets = offsets
         self.lengths = np.diff(offsets.astype(np.int64))
 
     def free(self):
@@ -696,6 +699,58 @@ SEED_INFO_DTYPE = np.dtype([
     ("rows_plus", "<u4"), ("rows_minus", "<u4"),
 ])
 assert SEED_REGION_DTYPE.itemsize == 32 and SEED_INFO_DTYPE.itemsize == 40
+
+
+INEXACT_REGION_DTYPE = np.dtype([
+    ("start", "<u8"), ("end", "<u8"), ("chrom_idx", "<u4"), ("read", "<u4"),
+    ("seq_start", "<u2"), ("seq_end", "<u2"), ("seq_len", "<u2"), ("seed_id", "<u2"),
+    ("pos_err", "<i4"), ("type_err", "<i4"), ("strand", "u1"), ("pad", "u1", (7,)),
+])
+assert INEXACT_REGION_DTYPE.itemsize == 48
+
+
+def windows_of(offsets, seed_size, min_seed_size):
+    """Windows per read of bwt_map_inexact_seeds_seq's tiling (bwt.c:3270-3296)."""
+    lens = np.diff(np.asarray(offsets, dtype=np.int64))
+    return lens // seed_size + (lens % seed_size >= min_seed_size)
+
+
+class InexactSeeds:
+    """Host copies of one fmi_map_inexact_seeds_batch: regions(w, s) of
+    window w (batch numbering) and strand s, status[w, s], window_base[r]
+    (first window of read r)."""
+
+    def __init__(self, regs, nreg, status, region_cap, window_base):
+        self.regs, self.nreg, self.status = regs, nreg, status
+        self.region_cap, self.window_base = region_cap, window_base
+
+    def regions(self, w, s):
+        t = 2 * w + s
+        return self.regs[t * self.region_cap:t * self.region_cap + int(self.nreg[t])]
+
+
+def map_inexact_seeds_batch(ix, batch, seed_size, min_seed_size, rec_cap=256, region_cap=64,
+                            stream=None):
+    """fmi_map_inexact_seeds_batch over a ReadBatch; returns InexactSeeds."""
+    nw = windows_of(batch.offsets, seed_size, min_seed_size)
+    total = int(nw.sum())
+    regs = DeviceBuffer(max(2 * total * region_cap, 1) * 48)
+    nreg = DeviceBuffer(max(2 * total, 1) * 4)
+    st = DeviceBuffer(max(2 * total, 1))
+    got = C.c_uint64()
+    check(lib().fmi_map_inexact_seeds_batch(ix.h, batch.d_reads.ptr, batch.d_offsets.ptr,
+                                            batch.nreads, seed_size, min_seed_size, rec_cap,
+                                            region_cap, total, regs.ptr, nreg.ptr, st.ptr,
+                                            C.byref(got), stream))
+    assert got.value == total
+    sync(stream)
+    out = InexactSeeds(regs.download(INEXACT_REGION_DTYPE, 2 * total * region_cap, stream),
+                       nreg.download(np.uint32, 2 * total, stream),
+                       st.download(np.uint8, 2 * total, stream).reshape(-1, 2), region_cap,
+                       np.concatenate([[0], np.cumsum(nw)]).astype(np.int64))
+    for b in (regs, nreg, st):
+        b.free()
+    return out
 
 
 def seed_map_batch(ix, batch, seed_len, stride, filter_seed_mappings=1000, region_cap=None,

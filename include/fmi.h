@@ -344,6 +344,49 @@ int fmi_seed_map_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64
                        uint64_t seed_info_cap, fmi_seed_region *d_regions, uint64_t region_cap,
                        uint64_t *nseeds, uint64_t *nrows, fmi_stream s);
 
+/* One region of bwt_map_inexact_seed (bwt.c:1401-1644; region_t,
+ * bwt.h:186-198): an SA row of a 1-error record of a seed window that lies
+ * inside one contig.  48 bytes. */
+typedef struct fmi_inexact_region {
+  uint64_t start;      /* start_mapping = start[idx-1] + key - offset[idx-1] + 1 (bwt.c:1573) */
+  uint64_t end;        /* start + len - 1, len = seq_end - seq_start + 1 of the window */
+  uint32_t chrom_idx;  /* idx as region_bwt_new receives it: binsearch result (contig + 1) */
+  uint32_t read;       /* read index in the batch */
+  uint16_t seq_start;  /* aux_seq_start: on the read (+) or its reverse complement (-) */
+  uint16_t seq_end;    /* aux_seq_end */
+  uint16_t seq_len;    /* read length */
+  uint16_t seed_id;    /* window index in the read's tiling */
+  int32_t pos_err;     /* final_pos (bwt.c:1508-1525; carried over where not assigned) */
+  int32_t type_err;    /* final_err + seq_start: -1 mismatch, 1 SEED_INSERTION, 2 SEED_DELETION */
+  uint8_t strand;      /* !type: 0 for +, 1 for - */
+  uint8_t pad[7];
+} fmi_inexact_region;
+
+/* bwt_map_inexact_seeds_seq (bwt.c:3262-3302) over a batch: each read is
+ * cut into windows of seed_size bases at 0, seed_size, ... plus the
+ * remainder when it holds at least min_seed_size bases; each window runs
+ * bwt_map_inexact_seed (bwt.c:1401-1644): BWSearch1VectorHelper on the +
+ * strand (backward, forward) then the - strand (forward_rev, backward_rev),
+ * one region per SA row of each record (all rows when l - k + 1 < 50, else
+ * rows k and k + 1) whose occurrence lies inside one contig, the
+ * within-10-bp dedup of each strand's list (bwt.c:1593-1615) and its
+ * hand-over from last to first (bwt.c:1616-1626).  Window w of the batch
+ * (reads in order, windows in read order) and strand s (0: +) own slot
+ * t = 2 w + s: at most region_cap regions at d_regions[t * region_cap],
+ * their count in d_nregions[t], and d_status[t] bit 0 when the strand had
+ * more than rec_cap records (the reference keeps 20000), bit 1 when it had
+ * more than region_cap regions before the dedup; a slot with a status bit
+ * set is not the reference's list.  *nwindows gets the window count;
+ * FMI_ERANGE (nothing written) when it exceeds windows_cap.  A window of
+ * fewer than 3 bases has no records.  Synchronises the stream and
+ * allocates its scratch (about 64 * rec_cap bytes per window, in chunks). */
+int fmi_map_inexact_seeds_batch(const fmi_index *ix, const uint8_t *d_reads,
+                                const uint64_t *d_offsets, uint32_t nreads, uint32_t seed_size,
+                                uint32_t min_seed_size, uint32_t rec_cap, uint32_t region_cap,
+                                uint64_t windows_cap, fmi_inexact_region *d_regions,
+                                uint32_t *d_nregions, uint8_t *d_status, uint64_t *nwindows,
+                                fmi_stream s);
+
 /* ----------------------------------------------------------- serving */
 /* A request loop over a device-resident index (no counterpart in the
  * reference, whose closest analog is the bounded work queue of

@@ -693,6 +693,116 @@ uint64_t fmo_seed_map(const fmo_index *ix, const uint8_t *W, uint32_t len, uint3
   return nrow;
 }
 
+/* bwt_map_inexact_seed (bwt.c:1401-1644) for the window [ss, se] of W:
+ * writes both strands' region lists; *final_pos carries across records and
+ * strands as the reference's local does (it is only assigned for some
+ * error kinds, bwt.c:1508-1525). */
+static void inexact_seed(const fmo_index *ix, const uint8_t *W, uint32_t seq_len, uint32_t ss,
+                         uint32_t se, uint32_t seed_id, const uint64_t *koff,
+                         const uint64_t *kstart, uint64_t kcount, uint32_t read,
+                         uint32_t rec_cap, uint32_t region_cap, fmo_hit *recs,
+                         fmo_inexact_region *out2, uint32_t *nreg2, uint8_t *status2) {
+  const int64_t len = (int64_t)se - (int64_t)ss + 1;
+  const uint64_t size_sa = fmo_size_sa(ix);
+  const int max_mappings = 50;                                 /* bwt.c:1409 */
+  int final_pos = 0, final_err;
+  for (int type = 1; type >= 0; type--) {
+    const int s = !type;                                        /* + strand first */
+    fmo_inexact_region *out = out2 + (uint64_t)s * region_cap;
+    uint8_t st = 0;
+    const uint32_t cnt = fmo_search1_vector(ix, s, 0, W, (int16_t)ss, (int16_t)se, recs, rec_cap);
+    if (cnt > rec_cap) st |= 1;
+    const uint32_t nrec = cnt < rec_cap ? cnt : rec_cap;
+    const uint32_t aux_s = type ? ss : seq_len - se - 1;
+    const uint32_t aux_e = type ? se : seq_len - ss - 1;
+    uint32_t m = 0;                                             /* regions in tmp list */
+    for (uint32_t ii = 0; ii < nrec; ii++) {
+      const fmo_hit *r = &recs[ii];
+      const int error = r->num_mismatches ? r->err_kind : 0;
+      int64_t len_calc = len;
+      if (error == FMO_DELETION) len_calc--;
+      else if (error == FMO_INSERTION) len_calc++;
+      const int pos = r->err_pos;
+      if (error == FMO_DELETION || pos == 0 || pos == len - 1) {
+        final_pos = type ? pos : (int)(len - 1 - pos);
+      } else if (error == FMO_INSERTION) {
+        if (type) final_pos = r->dir ? pos : pos + 1;
+        else final_pos = r->dir ? (int)(len - pos) : (int)(len - pos - 1);
+      }
+      if (error == FMO_MISMATCH) final_err = -1;
+      else if (error == FMO_DELETION) final_err = 1;             /* SEED_INSERTION */
+      else final_err = 2;                                        /* SEED_DELETION */
+      final_err += (int)ss;
+      int64_t k0 = r->k, l0 = r->l;
+      if (!((r->l - r->k + 1) < max_mappings)) l0 = r->k + 1;    /* LIMIT EXCEEDED */
+      const int direction = type ? r->dir : !r->dir;
+      for (int64_t j = k0; j <= l0; j++) {
+        const uint64_t key = direction
+            ? size_sa - fmo_get_sa(ix, 1, (uint64_t)j) - (uint64_t)len_calc - 1
+            : fmo_get_sa(ix, 0, (uint64_t)j);
+        const uint64_t idx = fmo_binsearch(koff, kcount, key);
+        if (!(idx >= 1 && key + (uint64_t)len_calc <= koff[idx])) continue;
+        if (m >= region_cap) { st |= 2; m++; continue; }
+        fmo_inexact_region g;
+        memset(&g, 0, sizeof(g));
+        g.start = kstart[idx - 1] + (key - koff[idx - 1]) + 1;
+        g.end = g.start + (uint64_t)len - 1;
+        g.chrom_idx = (uint32_t)idx; g.read = read;
+        g.seq_start = (uint16_t)aux_s; g.seq_end = (uint16_t)aux_e;
+        g.seq_len = (uint16_t)seq_len; g.seed_id = (uint16_t)seed_id;
+        g.pos_err = final_pos; g.type_err = final_err;
+        g.strand = (uint8_t)(!type);
+        out[m++] = g;
+      }
+    }
+    if (m > region_cap) m = region_cap;
+    /* dedup within 10 bp (bwt.c:1593-1615): an earlier region of the same
+     * contig closer than 10 to a later one that is kept is dropped; the
+     * distance is abs() of the 32-bit truncated difference */
+    uint8_t del[m ? m : 1];
+    memset(del, 0, sizeof(del));
+    for (int a1 = (int)m - 1; a1 >= 1; a1--) {
+      if (del[a1]) continue;
+      for (int a2 = a1 - 1; a2 >= 0; a2--) {
+        int32_t d = (int32_t)(uint32_t)(out[a1].start - out[a2].start);
+        int64_t dist = d < 0 ? -(int64_t)d : d;
+        if (out[a1].chrom_idx == out[a2].chrom_idx && dist < 10 && !del[a2]) del[a2] = 1;
+      }
+    }
+    /* the kept regions are handed over from the last to the first
+     * (bwt.c:1616-1626) */
+    uint32_t kept = 0;
+    for (uint32_t i = 0; i < m; i++)
+      if (!del[i]) out[kept++] = out[i];
+    for (uint32_t i = 0; i < kept / 2; i++) {
+      fmo_inexact_region t = out[i];
+      out[i] = out[kept - 1 - i];
+      out[kept - 1 - i] = t;
+    }
+    nreg2[s] = kept;
+    status2[s] = st;
+  }
+}
+
+uint32_t fmo_map_inexact_seeds(const fmo_index *ix, const uint8_t *W, uint32_t len,
+                               uint32_t seed_size, uint32_t min_seed_size, const uint64_t *koff,
+                               const uint64_t *kstart, uint64_t kcount, uint32_t read,
+                               uint32_t rec_cap, uint32_t region_cap, fmo_inexact_region *out,
+                               uint32_t *nregions, uint8_t *status, uint32_t wcap) {
+  /* bwt_map_inexact_seeds_seq, bwt.c:3262-3302 */
+  const uint32_t num_seeds = len / seed_size;
+  const uint32_t nwin = num_seeds + (len % seed_size >= min_seed_size ? 1 : 0);
+  fmo_hit *recs = (fmo_hit *)malloc((size_t)(rec_cap ? rec_cap : 1) * sizeof(fmo_hit));
+  for (uint32_t w = 0; w < nwin && w < wcap; w++) {
+    const uint32_t ss = w * seed_size;
+    const uint32_t se = w < num_seeds ? ss + seed_size - 1 : len - 1;
+    inexact_seed(ix, W, len, ss, se, w, koff, kstart, kcount, read, rec_cap, region_cap, recs,
+                 out + (uint64_t)2 * w * region_cap, nregions + 2 * w, status + 2 * w);
+  }
+  free(recs);
+  return nwin;
+}
+
 uint64_t fmo_binsearch(const uint64_t *array, uint64_t size, uint64_t key) {
   /* binsearch, search/io.h:43-64: first index whose value is > key. */
   if (!array) return 0;

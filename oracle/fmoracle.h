@@ -189,6 +189,32 @@ uint64_t fmo_seed_map(const fmo_index *ix, const uint8_t *W, uint32_t len, uint3
                       const uint64_t *kstart, uint64_t kcount, uint32_t read,
                       fmo_seed_region *out, uint64_t cap, int64_t *info);
 
+/* One region of bwt_map_inexact_seed (same layout as fmi_inexact_region). */
+typedef struct {
+  uint64_t start, end;
+  uint32_t chrom_idx, read;
+  uint16_t seq_start, seq_end, seq_len, seed_id;
+  int32_t pos_err, type_err;
+  uint8_t strand, pad[7];
+} fmo_inexact_region;
+
+/* bwt_map_inexact_seeds_seq (bwt.c:3262-3302) for one read: windows of
+ * seed_size bases at 0, seed_size, ... and the remainder when it holds at
+ * least min_seed_size bases; per window bwt_map_inexact_seed (bwt.c:
+ * 1401-1644): the 1-error vector helper on both strands (+ then -), a
+ * region per kept SA row (at most 50 rows, else rows k and k + 1), the
+ * within-10-bp dedup of each strand's list and its reversed hand-over.
+ * Regions go to out[(2 * w + s) * region_cap ...], their count to
+ * nregions[2 * w + s] and status[2 * w + s] gets bit 0 when the strand's
+ * records exceeded rec_cap, bit 1 when its regions exceeded region_cap
+ * (the stored list is then not the reference's).  Returns the window
+ * count; at most wcap windows are written. */
+uint32_t fmo_map_inexact_seeds(const fmo_index *ix, const uint8_t *W, uint32_t len,
+                               uint32_t seed_size, uint32_t min_seed_size, const uint64_t *koff,
+                               const uint64_t *kstart, uint64_t kcount, uint32_t read,
+                               uint32_t rec_cap, uint32_t region_cap, fmo_inexact_region *out,
+                               uint32_t *nregions, uint8_t *status, uint32_t wcap);
+
 /* ---- position resolution ---------------------------------------------- */
 /* binsearch, search/io.h:43-64: upper_bound of key in array[0..size). */
 uint64_t fmo_binsearch(const uint64_t *array, uint64_t size, uint64_t key);

@@ -30,6 +30,12 @@ SEED_REGION_DTYPE = np.dtype([
     ("valid", "u1"),
 ])
 
+INEXACT_REGION_DTYPE = np.dtype([
+    ("start", "<u8"), ("end", "<u8"), ("chrom_idx", "<u4"), ("read", "<u4"),
+    ("seq_start", "<u2"), ("seq_end", "<u2"), ("seq_len", "<u2"), ("seed_id", "<u2"),
+    ("pos_err", "<i4"), ("type_err", "<i4"), ("strand", "u1"), ("pad", "u1", (7,)),
+])
+
 MATCH, DELETION, MISMATCH, INSERTION = 0, 1, 2, 3
 VIEW_BACKWARD, VIEW_FORWARD, VIEW_BACKWARD_REV, VIEW_FORWARD_REV = 0, 1, 2, 3
 
@@ -116,6 +122,10 @@ def _load_oracle():
     lib.fmo_seed_map.argtypes = [_p, _p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _p, _p,
                                  C.c_uint64, C.c_uint32, _p, C.c_uint64, _p]
     lib.fmo_seed_map.restype = C.c_uint64
+    lib.fmo_map_inexact_seeds.argtypes = [_p, _p, C.c_uint32, C.c_uint32, C.c_uint32, _p, _p,
+                                          C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _p, _p,
+                                          _p, C.c_uint32]
+    lib.fmo_map_inexact_seeds.restype = C.c_uint32
     lib.fmo_binsearch.argtypes = [_p, C.c_uint64, C.c_uint64]
     lib.fmo_binsearch.restype = C.c_uint64
     lib.fmo_search1_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_int, C.c_uint32, _p, _p, C.c_int]
@@ -286,6 +296,25 @@ class Oracle:
         self.lib.fmo_seed_map(self.ref, _ptr(W), L, seed_len, stride, filter_seed_mappings,
                               _ptr(koff), _ptr(kstart), len(kstart), read, _ptr(out), n, None)
         return out[:int(n)], info[:ns]
+
+    def map_inexact_seeds(self, W, seed_size, min_seed_size, koff, kstart, read=0, rec_cap=256,
+                          region_cap=64):
+        """bwt_map_inexact_seeds_seq on one read: a list of (regions, status)
+        per window and strand (+ then -), in the reference's list order."""
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        koff = np.ascontiguousarray(koff, dtype=np.uint64)
+        kstart = np.ascontiguousarray(kstart, dtype=np.uint64)
+        L = len(W)
+        nw = L // seed_size + (1 if L % seed_size >= min_seed_size else 0)
+        out = np.zeros(max(2 * nw * region_cap, 1), INEXACT_REGION_DTYPE)
+        cnt = np.zeros(max(2 * nw, 1), np.uint32)
+        st = np.zeros(max(2 * nw, 1), np.uint8)
+        got = self.lib.fmo_map_inexact_seeds(self.ref, _ptr(W), L, seed_size, min_seed_size,
+                                             _ptr(koff), _ptr(kstart), len(kstart), read, rec_cap,
+                                             region_cap, _ptr(out), _ptr(cnt), _ptr(st), nw)
+        assert got == nw
+        return [(out[t * region_cap:t * region_cap + int(cnt[t])], int(st[t]))
+                for t in range(2 * nw)]
 
     def search1_batch(self, reads, offsets, cap=64, mismatch_only=False, threads=0,
                       keep_hits=True):
