@@ -12,6 +12,7 @@ outputs resident in HBM.
 stdout: one JSON line.  stderr: progress.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -299,6 +300,88 @@ def bench_exact(args, pre, g, ix, hpgfm, synth, read_len=100, cap=64):
     print(json.dumps(res), flush=True)
 
 
+def bench_inexact_seeds(args, pre, g, ix, hpgfm, synth, seed_size=18, min_seed_size=16,
+                        rec_cap=256, region_cap=64):
+    """Inexact seed mapping (bwt_map_inexact_seeds_seq, bwt.c:3262-3302): reads
+    cut into 18-base windows (the remainder when >= 16), per window the
+    1-error vector helper on both strands, regions of <= 50 rows per record,
+    the 10-bp dedup.  One step = one fmi_map_inexact_seeds_batch call (it
+    allocates its scratch and synchronises, as a caller sees it)."""
+    n = args.reads
+    t0 = time.time()
+    reads, offs, _ = synth.reads(g, n, length=args.read_len, seed=2, sub_rate=args.sub_rate)
+    batch = hpgfm.ReadBatch(reads, offs)
+    nwin = int(hpgfm.windows_of(offs, seed_size, min_seed_size).sum())
+    log(f"{n} reads ({nwin} windows) resident in {time.time() - t0:.1f}s")
+    L = hpgfm.lib()
+    regs = hpgfm.DeviceBuffer(2 * nwin * region_cap * 48)
+    nreg = hpgfm.DeviceBuffer(2 * nwin * 4)
+    st = hpgfm.DeviceBuffer(2 * nwin)
+    got = C.c_uint64()
+
+    def step():
+        hpgfm.check(L.fmi_map_inexact_seeds_batch(ix.h, batch.d_reads.ptr, batch.d_offsets.ptr,
+                                                  n, seed_size, min_seed_size, rec_cap,
+                                                  region_cap, nwin, regs.ptr, nreg.ptr, st.ptr,
+                                                  C.byref(got), None))
+
+    for _ in range(args.warmup):
+        step()
+    hpgfm.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    hpgfm.sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    counts = nreg.download(np.uint32, 2 * nwin)
+    status = st.download(np.uint8, 2 * nwin)
+    kept = int(counts.sum())
+    log(f"inexact seeds: {wall * 1000:.2f} ms per batch, {n / wall / 1e6:.2f} M reads/s, "
+        f"{kept} regions, {int((status != 0).sum())} slots over capacity")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import fmoracle as fo
+        threads = cpu_threads()
+        o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
+        sa0, isa0, _ = ix.full_arrays(0)
+        sa1, isa1, _ = ix.full_arrays(1)
+        o.set_samples(ix.sa_ratio, sa0, isa0, sa1, isa1)
+        del sa0, isa0, sa1, isa1
+        m = min(n, 40000)
+        koff = np.array([0, ix.n], np.uint64)
+        kst = np.zeros(1, np.uint64)
+        tcpu, ccpu = o.map_inexact_seeds_batch(reads[:int(offs[m])], offs[:m + 1], seed_size,
+                                               min_seed_size, koff, kst, rec_cap, region_cap,
+                                               threads=threads)
+        mw = int(hpgfm.windows_of(offs[:m + 1], seed_size, min_seed_size).sum())
+        same = int(counts[:2 * mw].sum()) == ccpu
+        cpu = {"value": round(m / tcpu), "unit": "reads/s", "cores": threads, "kind": "port",
+               "sample": f"the first {m} reads: the oracle's restatement of "
+                         f"bwt_map_inexact_seeds_seq (the reference's bwt.c does not build "
+                         f"here), SA ratio {ix.sa_ratio}, OpenMP over reads"}
+        log(f"CPU (port): {m / tcpu:.0f} reads/s on {threads} threads; region totals equal: {same}")
+    res = {
+        "metric": "reads/s, inexact seed mapping (18-base windows, <= 1 error)",
+        "value": round(n / wall), "unit": "reads/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1000, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"inexact seed mapping, {args.genome_mbp:g} Mbp synthetic "
+                               f"reference, {n} reads of {args.read_len} bp, "
+                               f"{args.sub_rate * 100:g}% substitutions, windows of {seed_size} "
+                               f"(remainder >= {min_seed_size}), {rec_cap} records and "
+                               f"{region_cap} regions per window and strand",
+                   "preset": args.preset, "windows": nwin},
+        "regions": kept, "windows_per_s": round(nwin / wall), "cpu_baseline": cpu,
+    }
+    if cpu:
+        res["cpu_region_total_equal"] = same
+    batch.free()
+    for b in (regs, nreg, st):
+        b.free()
+    print(json.dumps(res), flush=True)
+
+
 def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000):
     """Exact seed mapping: every read cut into seed_len-mers at `stride`, both
     orientations, up to `filt` SA positions resolved per seed (the
@@ -397,12 +480,13 @@ def main():
                          "WRITE_SIZE counters: a 1 GiB streaming copy (fmi_copy_bench) and "
                          "2^24-line random gathers of 64 and 128 bytes over a buffer the size "
                          "of the index and over 8 GiB")
-    ap.add_argument("--workload", choices=["search1", "seeds", "mixed", "exact"],
+    ap.add_argument("--workload", choices=["search1", "seeds", "mixed", "exact", "inexact-seeds"],
                     default="search1",
                     help="search1: the 1-mismatch search (the metric); seeds: exact seed mapping "
                          "(18-mers at stride 5, <=1000 positions per seed); mixed: error "
                          "budgets 0/1/2 in one batch (use with --preset c3); exact: exact "
-                         "search + locate with the reference's CPU path (use with --preset c1)")
+                         "search + locate with the reference's CPU path (use with --preset c1); "
+                         "inexact-seeds: bwt_map_inexact_seeds_seq over 18-base windows")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -432,6 +516,11 @@ def main():
     ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
     log(f"index built on the GPU in {time.time() - t0:.1f}s "
         f"({ix.device_bytes / 1e6:.1f} MB on device)")
+    if args.workload == "inexact-seeds":
+        if not args.no_expand:
+            ix.expand()
+        bench_inexact_seeds(args, pre, g, ix, hpgfm, synth)
+        return
     if args.workload == "exact":
         ix.expand()
         bench_exact(args, pre, g, ix, hpgfm, synth)

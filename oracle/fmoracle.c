@@ -923,6 +923,44 @@ double fmo_exact_locate_batch(const fmo_index *ix, const uint8_t *reads,
   return now_s() - t0;
 }
 
+double fmo_map_inexact_seeds_batch(const fmo_index *ix, const uint8_t *reads,
+                                   const uint64_t *offsets, uint64_t nreads, uint32_t seed_size,
+                                   uint32_t min_seed_size, const uint64_t *koff,
+                                   const uint64_t *kstart, uint64_t kcount, uint32_t rec_cap,
+                                   uint32_t region_cap, uint64_t *nregions_total, int threads) {
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+#endif
+  double t0 = now_s();
+  uint64_t total = 0;
+#ifdef _OPENMP
+#pragma omp parallel reduction(+ : total)
+#endif
+  {
+    uint32_t wmax = FMO_MAX_LEN / (seed_size ? seed_size : 1) + 1;
+    fmo_inexact_region *out =
+        (fmo_inexact_region *)malloc((size_t)2 * wmax * region_cap * sizeof(fmo_inexact_region));
+    uint32_t *nreg = (uint32_t *)malloc((size_t)2 * wmax * sizeof(uint32_t));
+    uint8_t *st = (uint8_t *)malloc((size_t)2 * wmax);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, chunk)
+#endif
+    for (long i = 0; i < (long)nreads; i++) {
+      const uint32_t len = (uint32_t)(offsets[i + 1] - offsets[i]);
+      const uint32_t nw = fmo_map_inexact_seeds(ix, reads + offsets[i], len, seed_size,
+                                                min_seed_size, koff, kstart, kcount, (uint32_t)i,
+                                                rec_cap, region_cap, out, nreg, st, wmax);
+      for (uint32_t t = 0; t < 2 * nw && t < 2 * wmax; t++) total += nreg[t];
+    }
+    free(out); free(nreg); free(st);
+  }
+  *nregions_total = total;
+  return now_s() - t0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* k-error search: BWSearchCPU, search/search.c:1171-1405, and its helpers   */
 /* search/search.c:3-1169, search/search.h:66-198                           */

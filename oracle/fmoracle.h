@@ -34,6 +34,7 @@ extern "C" {
 #endif
 
 /* Error kinds, same codes as bwt_commons.h:15-18. */
+#define FMO_MAX_LEN 1000   /* MAXLINE, bwt_commons.h:11 */
 #define FMO_MATCH     0
 #define FMO_DELETION  1
 #define FMO_MISMATCH  2
@@ -214,6 +215,14 @@ uint32_t fmo_map_inexact_seeds(const fmo_index *ix, const uint8_t *W, uint32_t l
                                const uint64_t *kstart, uint64_t kcount, uint32_t read,
                                uint32_t rec_cap, uint32_t region_cap, fmo_inexact_region *out,
                                uint32_t *nregions, uint8_t *status, uint32_t wcap);
+
+/* fmo_map_inexact_seeds over a batch (reads of at most FMO_MAX_LEN bases),
+ * OpenMP over reads; *nregions_total gets the regions kept.  Returns seconds. */
+double fmo_map_inexact_seeds_batch(const fmo_index *ix, const uint8_t *reads,
+                                   const uint64_t *offsets, uint64_t nreads, uint32_t seed_size,
+                                   uint32_t min_seed_size, const uint64_t *koff,
+                                   const uint64_t *kstart, uint64_t kcount, uint32_t rec_cap,
+                                   uint32_t region_cap, uint64_t *nregions_total, int threads);
 
 /* ---- position resolution ---------------------------------------------- */
 /* binsearch, search/io.h:43-64: upper_bound of key in array[0..size). */

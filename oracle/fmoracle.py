@@ -126,6 +126,10 @@ def _load_oracle():
                                           C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _p, _p,
                                           _p, C.c_uint32]
     lib.fmo_map_inexact_seeds.restype = C.c_uint32
+    lib.fmo_map_inexact_seeds_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                                _p, _p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                                C.POINTER(C.c_uint64), C.c_int]
+    lib.fmo_map_inexact_seeds_batch.restype = C.c_double
     lib.fmo_binsearch.argtypes = [_p, C.c_uint64, C.c_uint64]
     lib.fmo_binsearch.restype = C.c_uint64
     lib.fmo_search1_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_int, C.c_uint32, _p, _p, C.c_int]
@@ -315,6 +319,20 @@ class Oracle:
         assert got == nw
         return [(out[t * region_cap:t * region_cap + int(cnt[t])], int(st[t]))
                 for t in range(2 * nw)]
+
+    def map_inexact_seeds_batch(self, reads, offsets, seed_size, min_seed_size, koff, kstart,
+                                rec_cap=256, region_cap=64, threads=0):
+        """(seconds, regions kept) of map_inexact_seeds over a batch."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        koff = np.ascontiguousarray(koff, dtype=np.uint64)
+        kstart = np.ascontiguousarray(kstart, dtype=np.uint64)
+        tot = C.c_uint64()
+        t = self.lib.fmo_map_inexact_seeds_batch(self.ref, _ptr(reads), _ptr(offsets),
+                                                 len(offsets) - 1, seed_size, min_seed_size,
+                                                 _ptr(koff), _ptr(kstart), len(kstart), rec_cap,
+                                                 region_cap, C.byref(tot), threads)
+        return float(t), int(tot.value)
 
     def search1_batch(self, reads, offsets, cap=64, mismatch_only=False, threads=0,
                       keep_hits=True):
