@@ -16,6 +16,7 @@
 
 #include <rocprim/rocprim.hpp>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 
@@ -170,11 +171,11 @@ __global__ __launch_bounds__(256) void seed_rows_kernel(RowParams P) {
 // (bwt.c:1401-1644):
 //   1. windows per read (len / seed_size, plus the remainder when it holds
 //      at least min_seed_size bases), exclusive scan -> window numbering
-//   2. per chunk of windows: one lane per (window, strand) runs the vector
-//      helper (vh_task) into a record slot of rec_cap records
-//   3. one lane per window turns both strands' records into regions in the
-//      caller's slots, + strand first because final_pos carries over, then
-//      dedups and reverses each strand's list in place.
+//   2. one lane per window (persistent grid): the vector helper on the +
+//      strand then the - strand (final_pos carries over between them), each
+//      record turned into regions as it is produced (the reference makes the
+//      records first and then walks them in the same order), straight into
+//      the caller's slot; then the slot's dedup and reversal in place.
 
 __global__ void win_count_kernel(const uint64_t *offsets, uint32_t nreads, uint32_t seed_size,
                                  uint32_t min_seed_size, uint64_t *cnt) {
@@ -189,131 +190,103 @@ __global__ void win_count_kernel(const uint64_t *offsets, uint32_t nreads, uint3
   }
 }
 
-struct WinGeom {
-  uint64_t read, off;
-  int32_t len, start, end, w;
-};
-
-__device__ __forceinline__ WinGeom win_geom(const uint64_t *offsets, const uint64_t *win_base,
-                                            uint32_t nreads, uint32_t seed_size, uint64_t g) {
-  WinGeom G;
-  G.read = owner(win_base, nreads, g);
-  G.off = offsets[G.read];
-  G.len = (int32_t)(offsets[G.read + 1] - G.off);
-  G.w = (int32_t)(g - win_base[G.read]);
-  const int32_t num_seeds = G.len / (int32_t)seed_size;
-  G.start = G.w * (int32_t)seed_size;
-  G.end = G.w < num_seeds ? G.start + (int32_t)seed_size - 1 : G.len - 1;
-  return G;
-}
-
-struct WinParams {
+struct InexactParams {
   DevIndex ix;
   const uint8_t *reads;
   const uint64_t *offsets;
   const uint64_t *win_base;    // nreads + 1
-  uint64_t g0, nwin;           // windows [g0, g0 + nwin) of this chunk
-  uint32_t nreads, seed_size, rec_cap;
-  fmi_hit *recs;               // 2 * nwin * rec_cap
-  uint32_t *rcount;            // 2 * nwin
-  uint2 *vscratch;             // 2 * nwin * (seed_size + 3)
-};
-
-__global__ __launch_bounds__(256) void inexact_window_kernel(WinParams P) {
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * P.nwin;
-       t += (uint64_t)gridDim.x * blockDim.x) {
-    const WinGeom G = win_geom(P.offsets, P.win_base, P.nreads, P.seed_size, P.g0 + (t >> 1));
-    const uint32_t strand = (uint32_t)(t & 1u);
-    if (G.end - G.start < 2) { P.rcount[t] = 0; continue; }
-    P.rcount[t] = vh_task(P.ix, P.reads + G.off, G.start, G.end, strand, 0u,
-                          P.vscratch + t * (uint64_t)(P.seed_size + 3),
-                          P.recs + t * (uint64_t)P.rec_cap, P.rec_cap);
-  }
-}
-
-struct RegParams {
-  DevIndex ix;
-  const uint64_t *offsets;
-  const uint64_t *win_base;
-  uint64_t g0, nwin;
+  uint64_t nwin;
   uint32_t nreads, seed_size, rec_cap, region_cap;
-  const fmi_hit *recs;
-  const uint32_t *rcount;
   const uint64_t *koff;        // kcount + 1
   const uint64_t *kstart;
   uint32_t kcount;
-  fmi_inexact_region *out;     // caller's slots, indexed by the global window
+  uint2 *vscratch;             // lanes * (seed_size + 3)
+  fmi_inexact_region *out;     // 2 * nwin * region_cap
   uint32_t *nreg;
   uint8_t *status;
 };
 
-__global__ __launch_bounds__(256) void inexact_region_kernel(RegParams P) {
-  const uint64_t size_sa = (uint64_t)P.ix.n + 1;
-  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < P.nwin;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t g = P.g0 + q;
-    const WinGeom G = win_geom(P.offsets, P.win_base, P.nreads, P.seed_size, g);
-    const int64_t len = (int64_t)G.end - G.start + 1;
-    int32_t final_pos = 0, final_err;
-    for (uint32_t s = 0; s < 2; ++s) {
-      const uint32_t type = 1u - s;               // bwt.c:1460: type 1 (+) first
-      const uint64_t slot = 2 * g + s, lslot = 2 * q + s;
-      fmi_inexact_region *out = P.out + slot * P.region_cap;
-      const uint32_t cnt = P.rcount[lslot];
-      uint8_t st = cnt > P.rec_cap ? 1 : 0;
-      const uint32_t nrec = cnt < P.rec_cap ? cnt : P.rec_cap;
-      const uint32_t aux_s = type ? (uint32_t)G.start : (uint32_t)(G.len - G.end - 1);
-      const uint32_t aux_e = type ? (uint32_t)G.end : (uint32_t)(G.len - G.start - 1);
-      uint32_t m = 0;
-      for (uint32_t ii = 0; ii < nrec; ++ii) {
-        const fmi_hit r = P.recs[lslot * P.rec_cap + ii];
-        const uint32_t error = r.num_mismatches ? r.err_kind : 0u;
-        int64_t len_calc = len;
-        if (error == FMI_DELETION) len_calc--;
-        else if (error == FMI_INSERTION) len_calc++;
-        const int32_t pos = r.err_pos;
-        if (error == FMI_DELETION || pos == 0 || pos == len - 1) {
-          final_pos = type ? pos : (int32_t)(len - 1 - pos);
-        } else if (error == FMI_INSERTION) {
-          if (type) final_pos = r.dir ? pos : pos + 1;
-          else final_pos = r.dir ? (int32_t)(len - pos) : (int32_t)(len - pos - 1);
-        }
-        if (error == FMI_MISMATCH) final_err = -1;
-        else if (error == FMI_DELETION) final_err = 1;      // SEED_INSERTION
-        else final_err = 2;                                 // SEED_DELETION
-        final_err += G.start;
-        const int64_t k0 = r.k;
-        const int64_t l0 = (r.l - r.k + 1) < 50 ? r.l : r.k + 1;   // bwt.c:1539-1547
-        const uint32_t direction = type ? r.dir : !r.dir;
-        for (int64_t j = k0; j <= l0; ++j) {
-          const uint64_t key =
-              direction ? size_sa - sa_value(P.ix.d[1], P.ix.ratio, P.ix.n, (uint32_t)j) -
-                              (uint64_t)len_calc - 1
-                        : sa_value(P.ix.d[0], P.ix.ratio, P.ix.n, (uint32_t)j);
-          uint64_t lo = 0, size = P.kcount;        // binsearch, search/io.h:43-64
-          while (size > 0) {
-            const uint64_t w = size / 2;
-            if (P.koff[lo + w] <= key) { lo += w + 1; size -= w + 1; }
-            else size = w;
-          }
-          if (!(lo >= 1 && key + (uint64_t)len_calc <= P.koff[lo])) continue;
-          if (m >= P.region_cap) { st |= 2; continue; }
-          fmi_inexact_region R;
-          R.start = P.kstart[lo - 1] + (key - P.koff[lo - 1]) + 1;
-          R.end = R.start + (uint64_t)len - 1;
-          R.chrom_idx = (uint32_t)lo;
-          R.read = (uint32_t)G.read;
-          R.seq_start = (uint16_t)aux_s;
-          R.seq_end = (uint16_t)aux_e;
-          R.seq_len = (uint16_t)G.len;
-          R.seed_id = (uint16_t)G.w;
-          R.pos_err = final_pos;
-          R.type_err = final_err;
-          R.strand = (uint8_t)s;
-          for (int b = 0; b < 7; ++b) R.pad[b] = 0;
-          out[m++] = R;
-        }
+// Regions of one strand's records (bwt.c:1490-1588), in record order.
+struct RegionSink {
+  const InexactParams *P;
+  fmi_inexact_region *out;
+  uint32_t m, nrec, type, read, aux_s, aux_e, seq_len, seed_id;
+  int32_t start, len;
+  int32_t *final_pos;
+  uint8_t st;
+  __device__ void add(const VhRec &r) {
+    if (nrec++ >= P->rec_cap) { st |= 1; return; }   // the reference's list holds 20000
+    // a record as the list stores it: no error fields without an error
+    const uint32_t error = r.nmm ? r.kind : 0u;
+    const int32_t pos = r.nmm ? r.err_pos : 0;
+    int64_t len_calc = len;
+    if (error == FMI_DELETION) len_calc--;
+    else if (error == FMI_INSERTION) len_calc++;
+    if (error == FMI_DELETION || pos == 0 || pos == len - 1) {
+      *final_pos = type ? pos : len - 1 - pos;
+    } else if (error == FMI_INSERTION) {
+      if (type) *final_pos = r.dir ? pos : pos + 1;
+      else *final_pos = r.dir ? len - pos : len - pos - 1;
+    }
+    int32_t final_err = error == FMI_MISMATCH ? -1 : (error == FMI_DELETION ? 1 : 2);
+    final_err += start;                    // -1 mismatch, SEED_INSERTION 1, SEED_DELETION 2
+    const int64_t k = r.k, l = r.l;
+    const int64_t l0 = (l - k + 1) < 50 ? l : k + 1;   // bwt.c:1539-1547
+    const uint32_t direction = type ? r.dir : !r.dir;
+    const uint64_t size_sa = (uint64_t)P->ix.n + 1;
+    for (int64_t j = k; j <= l0; ++j) {
+      const uint64_t key =
+          direction ? size_sa - sa_value(P->ix.d[1], P->ix.ratio, P->ix.n, (uint32_t)j) -
+                          (uint64_t)len_calc - 1
+                    : sa_value(P->ix.d[0], P->ix.ratio, P->ix.n, (uint32_t)j);
+      uint64_t lo = 0, size = P->kcount;    // binsearch, search/io.h:43-64
+      while (size > 0) {
+        const uint64_t w = size / 2;
+        if (P->koff[lo + w] <= key) { lo += w + 1; size -= w + 1; }
+        else size = w;
       }
+      if (!(lo >= 1 && key + (uint64_t)len_calc <= P->koff[lo])) continue;
+      if (m >= P->region_cap) { st |= 2; continue; }
+      fmi_inexact_region R;
+      R.start = P->kstart[lo - 1] + (key - P->koff[lo - 1]) + 1;
+      R.end = R.start + (uint64_t)len - 1;
+      R.chrom_idx = (uint32_t)lo;
+      R.read = read;
+      R.seq_start = (uint16_t)aux_s;
+      R.seq_end = (uint16_t)aux_e;
+      R.seq_len = (uint16_t)seq_len;
+      R.seed_id = (uint16_t)seed_id;
+      R.pos_err = *final_pos;
+      R.type_err = final_err;
+      R.strand = (uint8_t)(1u - type);
+      for (int b = 0; b < 7; ++b) R.pad[b] = 0;
+      out[m++] = R;
+    }
+  }
+};
+
+__global__ __launch_bounds__(256) void inexact_seed_kernel(InexactParams P) {
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint2 *vb = P.vscratch + lane * (uint64_t)(P.seed_size + 3);
+  for (uint64_t g = lane; g < P.nwin; g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t rd = owner(P.win_base, P.nreads, g);
+    const uint64_t off = P.offsets[rd];
+    const int32_t len = (int32_t)(P.offsets[rd + 1] - off);
+    const int32_t w = (int32_t)(g - P.win_base[rd]);
+    const int32_t num_seeds = len / (int32_t)P.seed_size;
+    const int32_t ws = w * (int32_t)P.seed_size;
+    const int32_t we = w < num_seeds ? ws + (int32_t)P.seed_size - 1 : len - 1;
+    int32_t final_pos = 0;
+    for (uint32_t s = 0; s < 2; ++s) {
+      const uint32_t type = 1u - s;          // bwt.c:1460: type 1 (+) first
+      const uint64_t slot = 2 * g + s;
+      RegionSink S{&P, P.out + slot * P.region_cap, 0u, 0u, type, (uint32_t)rd,
+                   type ? (uint32_t)ws : (uint32_t)(len - we - 1),
+                   type ? (uint32_t)we : (uint32_t)(len - ws - 1), (uint32_t)len, (uint32_t)w,
+                   ws, we - ws + 1, &final_pos, 0};
+      if (we - ws >= 2) vh_run(P.ix, P.reads + off, ws, we, s, 0u, vb, S);
+      fmi_inexact_region *out = S.out;
+      const uint32_t m = S.m;
       // dedup within 10 bp (bwt.c:1593-1615), the mark kept in pad[0]
       for (int32_t a1 = (int32_t)m - 1; a1 >= 1; --a1) {
         if (out[a1].pad[0]) continue;
@@ -329,16 +302,16 @@ __global__ __launch_bounds__(256) void inexact_region_kernel(RegParams P) {
       uint32_t kept = 0;
       for (uint32_t i = 0; i < m; ++i) {
         if (out[i].pad[0]) continue;
-        fmi_inexact_region R = out[i];
+        const fmi_inexact_region R = out[i];
         out[kept++] = R;
       }
       for (uint32_t i = 0; i < kept / 2; ++i) {
-        const fmi_inexact_region a = out[i];
+        const fmi_inexact_region x = out[i];
         out[i] = out[kept - 1 - i];
-        out[kept - 1 - i] = a;
+        out[kept - 1 - i] = x;
       }
       P.nreg[slot] = kept;
-      P.status[slot] = st;
+      P.status[slot] = S.st;
     }
   }
 }
@@ -441,12 +414,10 @@ int map_inexact_seeds(const fmi_index *ix, const uint8_t *reads, const uint64_t 
   hipStream_t s = static_cast<hipStream_t>(stream);
   int rc = FMI_OK;
   uint64_t *cnt = nullptr, *base = nullptr;
-  fmi_hit *recs = nullptr;
-  uint32_t *rcount = nullptr;
   uint2 *vscr = nullptr;
   void *tmp = nullptr;
   size_t need = 0;
-  uint64_t nwin = 0, chunk = 0;
+  uint64_t nwin = 0;
   *nwindows = 0;
   SEED_TRY(hipMalloc(&cnt, (uint64_t)(nreads + 1) * 8));
   SEED_TRY(hipMalloc(&base, (uint64_t)(nreads + 1) * 8));
@@ -467,23 +438,16 @@ int map_inexact_seeds(const fmi_index *ix, const uint8_t *reads, const uint64_t 
     goto done;
   }
   if (nwin == 0) goto done;
-  // chunks of windows bounded to ~2 GiB of record scratch
-  chunk = (2ull << 30) / (2ull * rec_cap * sizeof(fmi_hit));
-  if (chunk < 1) chunk = 1;
-  if (chunk > nwin) chunk = nwin;
-  SEED_TRY(hipMalloc(&recs, 2 * chunk * rec_cap * sizeof(fmi_hit)));
-  SEED_TRY(hipMalloc(&rcount, 2 * chunk * 4));
-  SEED_TRY(hipMalloc(&vscr, 2 * chunk * (uint64_t)(seed_size + 3) * sizeof(uint2)));
-  for (uint64_t g0 = 0; g0 < nwin; g0 += chunk) {
-    const uint64_t m = nwin - g0 < chunk ? nwin - g0 : chunk;
-    WinParams W{ix->dev, reads, offsets, base, g0, m, nreads, seed_size, rec_cap, recs, rcount,
-                vscr};
-    inexact_window_kernel<<<grid_for(2 * m), 256, 0, s>>>(W);
-    SEED_TRY(hipGetLastError());
-    RegParams R{ix->dev, offsets, base, g0, m, nreads, seed_size, rec_cap, region_cap, recs,
-                rcount, ix->d_koffset, ix->d_kstart, (uint32_t)ix->kstart.size(), regions,
-                nregions, status};
-    inexact_region_kernel<<<grid_for(m), 256, 0, s>>>(R);
+  {
+    uint64_t lanes = (nwin + 255) / 256 * 256;
+    const uint64_t per = (uint64_t)(seed_size + 3) * sizeof(uint2);
+    const uint64_t max_lanes = std::min<uint64_t>(1ull << 19, ((1ull << 30) / per) / 256 * 256);
+    if (lanes > max_lanes) lanes = max_lanes;
+    SEED_TRY(hipMalloc(&vscr, lanes * per));
+    InexactParams P{ix->dev, reads, offsets, base, nwin, nreads, seed_size, rec_cap, region_cap,
+                    ix->d_koffset, ix->d_kstart, (uint32_t)ix->kstart.size(), vscr, regions,
+                    nregions, status};
+    inexact_seed_kernel<<<dim3((uint32_t)(lanes / 256)), 256, 0, s>>>(P);
     SEED_TRY(hipGetLastError());
   }
 done:
@@ -493,8 +457,6 @@ done:
   }
   (void)hipFree(cnt);
   (void)hipFree(base);
-  (void)hipFree(recs);
-  (void)hipFree(rcount);
   (void)hipFree(vscr);
   (void)hipFree(tmp);
   return rc;

@@ -77,10 +77,10 @@ __device__ __forceinline__ void vh_exact(const VhView &V, const uint8_t *W, int3
 
 // The helper over W[start..end] (end - start >= 2) on strand 0 (backward,
 // forward) or 1 (forward_rev, backward_rev); vb: n + 3 intervals of scratch.
-// Stores at most cap records at out; returns the count of all of them.
-__device__ inline uint32_t vh_task(const DevIndex &ix, const uint8_t *W, int32_t start,
-                                   int32_t end, uint32_t strand, uint32_t mm_only, uint2 *vb,
-                                   fmi_hit *out, uint32_t cap) {
+// Every record goes to S.add(r) in the reference's order.
+template <class Sink>
+__device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t start, int32_t end,
+                              uint32_t strand, uint32_t mm_only, uint2 *vb, Sink &S) {
   const uint32_t n_sa = ix.n;
   // + strand: backward / forward; - strand: forward_rev / backward_rev
   const VhView B{strand ? &ix.d[1] : &ix.d[0], strand, n_sa};
@@ -114,7 +114,6 @@ __device__ inline uint32_t vh_task(const DevIndex &ix, const uint8_t *W, int32_t
     }
   }
 
-  VhSink S{out, cap, 0u, mm_only, strand, (int16_t)start, (int16_t)end};
   VhRec r{0, 0, -1, 0, 0, 0, 0, 0};
   uint2 v = vb[0];
   if (v.x <= v.y) { r.k = v.x; r.l = v.y; r.pos = -1; S.add(r); }
@@ -238,6 +237,14 @@ __device__ inline uint32_t vh_task(const DevIndex &ix, const uint8_t *W, int32_t
       }
     }
   }
+}
+
+// vh_run storing at most cap records at out; returns the count of all of them.
+__device__ inline uint32_t vh_task(const DevIndex &ix, const uint8_t *W, int32_t start,
+                                   int32_t end, uint32_t strand, uint32_t mm_only, uint2 *vb,
+                                   fmi_hit *out, uint32_t cap) {
+  VhSink S{out, cap, 0u, mm_only, strand, (int16_t)start, (int16_t)end};
+  vh_run(ix, W, start, end, strand, mm_only, vb, S);
   return S.cnt;
 }
 

@@ -379,7 +379,8 @@ typedef struct fmi_inexact_region {
  * set is not the reference's list.  *nwindows gets the window count;
  * FMI_ERANGE (nothing written) when it exceeds windows_cap.  A window of
  * fewer than 3 bases has no records.  Synchronises the stream and
- * allocates its scratch (about 64 * rec_cap bytes per window, in chunks). */
+ * allocates its scratch (the window numbering and at most 1 GiB of interval
+ * vectors). */
 int fmi_map_inexact_seeds_batch(const fmi_index *ix, const uint8_t *d_reads,
                                 const uint64_t *d_offsets, uint32_t nreads, uint32_t seed_size,
                                 uint32_t min_seed_size, uint32_t rec_cap, uint32_t region_cap,
