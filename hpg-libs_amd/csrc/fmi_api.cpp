@@ -757,9 +757,11 @@ int fmi_encode_bases(const char *src, uint64_t n, uint8_t *dst) {
   return FMI_OK;
 }
 
-int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
-                     const uint64_t *d_offsets, uint32_t nreads, uint32_t cap, uint32_t row_cap,
-                     int32_t *d_chrom, uint64_t *d_coord, fmi_stream s) {
+namespace {
+
+int resolve_common(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
+                   const uint64_t *d_offsets, uint32_t nreads, uint32_t cap, uint32_t row_cap,
+                   int32_t *d_chrom, uint64_t *d_coord, uint32_t record_len, fmi_stream s) {
   int rc = need_device(ix);
   if (rc) return rc;
   if (nreads == 0 || cap == 0 || row_cap == 0) return FMI_OK;
@@ -770,8 +772,24 @@ int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t 
   if (rc) return rc;
   hipError_t e = fmi::launch_resolve(ix->dev, d_hits, d_counts, d_offsets, nreads, cap, row_cap,
                                      ix->d_koffset, ix->d_kstart, (uint32_t)ix->kstart.size(),
-                                     d_chrom, d_coord, st);
+                                     record_len, d_chrom, d_coord, st);
   return end_timed(st, e, "resolve launch");
+}
+
+}  // namespace
+
+int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
+                     const uint64_t *d_offsets, uint32_t nreads, uint32_t cap, uint32_t row_cap,
+                     int32_t *d_chrom, uint64_t *d_coord, fmi_stream s) {
+  return resolve_common(ix, d_hits, d_counts, d_offsets, nreads, cap, row_cap, d_chrom, d_coord,
+                        0u, s);
+}
+
+int fmi_resolve_hits_text(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
+                          const uint64_t *d_offsets, uint32_t nreads, uint32_t cap,
+                          uint32_t row_cap, int32_t *d_chrom, uint64_t *d_coord, fmi_stream s) {
+  return resolve_common(ix, d_hits, d_counts, d_offsets, nreads, cap, row_cap, d_chrom, d_coord,
+                        1u, s);
 }
 
 int fmi_seed_map_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64_t *d_offsets,
@@ -819,6 +837,27 @@ int fmi_map_inexact_seeds_batch(const fmi_index *ix, const uint8_t *d_reads,
                               region_cap, windows_cap, d_regions, d_nregions, d_status, nwindows,
                               s, &err);
   return rc ? fail(rc, err) : FMI_OK;
+}
+
+int fmi_format_results(const fmi_index *ix, uint64_t read_index, int strand, const char *mapping,
+                       uint32_t nW, const fmi_hit *hits, uint32_t nhits, const int32_t *chrom,
+                       const uint64_t *coord, uint32_t row_cap, char *buf, size_t buf_size,
+                       size_t *len, uint64_t *rows_missing) {
+  if (!ix || !len || !rows_missing || (nW && !mapping)) return fail(FMI_EINVAL, "NULL argument");
+  if (nhits && (!hits || !chrom || !coord || row_cap == 0))
+    return fail(FMI_EINVAL, "NULL records or rows");
+  if (buf_size && !buf) return fail(FMI_EINVAL, "buf is NULL with buf_size > 0");
+  std::string text;
+  int rc = fmi::format_results(ix, read_index, strand, mapping ? mapping : "", nW, hits, nhits,
+                               chrom, coord, row_cap, &text, rows_missing);
+  if (rc) return fail(rc, "format_results");
+  *len = text.size();
+  if (buf_size) {
+    const size_t n = text.size() < buf_size ? text.size() : buf_size - 1;
+    std::memcpy(buf, text.data(), n);
+    buf[n] = 0;
+  }
+  return FMI_OK;
 }
 
 int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uint64_t nlines,

@@ -49,6 +49,10 @@ int map_inexact_seeds(const fmi_index *ix, const uint8_t *reads, const uint64_t 
                       uint32_t rec_cap, uint32_t region_cap, uint64_t windows_cap,
                       fmi_inexact_region *regions, uint32_t *nregions, uint8_t *status,
                       uint64_t *nwindows, void *s, std::string *err);
+int format_results(const fmi_index *ix, uint64_t read_index, int strand, const char *mapping,
+                   uint32_t nW, const fmi_hit *hits, uint32_t nhits, const int32_t *chrom,
+                   const uint64_t *coord, uint32_t row_cap, std::string *out,
+                   uint64_t *rows_missing);
 int save_index(const fmi_index *ix, const char *path, std::string *err);
 // Reads the reference's index directory (fmi_import.cpp).
 int import_reference_dir(const char *dir, fmi_index *ix, std::string *err);

@@ -49,7 +49,7 @@ hipError_t launch_isa(const DevIndex &ix, uint32_t dsel, const uint64_t *tpos, u
 hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_t *counts,
                           const uint64_t *offsets, uint32_t nreads, uint32_t cap,
                           uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
-                          uint32_t kcount, int32_t *chrom, uint64_t *coord, hipStream_t s);
+                          uint32_t kcount, uint32_t record_len, int32_t *chrom, uint64_t *coord, hipStream_t s);
 // BWSearchCPU per (read, strand) task (fmi_search_k.hip); scratch holds
 // search_k_scratch_bytes(lanes, list_cap); lanes is a multiple of 256.
 uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap);

@@ -736,6 +736,7 @@ struct ResolveParams {
   const uint64_t *kstart;
   uint32_t nslots, cap, row_cap;
   uint32_t kcount;
+  uint32_t record_len;      // 0: the read's length (bwt.c:2031-2036); 1: end - start + 1
   int32_t *chrom;
   uint64_t *coord;
 };
@@ -755,7 +756,10 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveParams P) {
   const int64_t nrows = (valid && hit.k <= hit.l) ? hit.l - hit.k + 1 : 0;
   if ((int64_t)j >= nrows) { P.chrom[o] = -1; P.coord[o] = 0; return; }
   const uint32_t read = (uint32_t)(slot >> 1), strand = (uint32_t)(slot & 1u);
-  const uint64_t len = P.offsets[read + 1] - P.offsets[read];
+  // bwt.c:2031-2036 measure the read; manage_single_result measures the
+  // record's own window (enW = end - start + 1, search/results.h:146-152)
+  const uint64_t len = P.record_len ? (uint64_t)(int64_t)(hit.end - hit.start + 1)
+                                    : P.offsets[read + 1] - P.offsets[read];
   uint64_t len_calc = len;
   if (hit.num_mismatches) {
     if (hit.err_kind == FMI_DELETION) len_calc--;
@@ -1027,9 +1031,10 @@ hipError_t launch_isa(const DevIndex &ix, uint32_t dsel, const uint64_t *tpos, u
 hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_t *counts,
                           const uint64_t *offsets, uint32_t nreads, uint32_t cap,
                           uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
-                          uint32_t kcount, int32_t *chrom, uint64_t *coord, hipStream_t s) {
+                          uint32_t kcount, uint32_t record_len, int32_t *chrom, uint64_t *coord,
+                          hipStream_t s) {
   ResolveParams P{ix, hits, counts, offsets, koff, kstart, 2u * nreads, cap, row_cap, kcount,
-                  chrom, coord};
+                  record_len, chrom, coord};
   const uint64_t total = (uint64_t)2 * nreads * cap * row_cap;
   const uint64_t grid = (total + 255) / 256;
   if (grid == 0) return hipSuccess;

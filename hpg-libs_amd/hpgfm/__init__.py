@@ -48,10 +48,10 @@ EXPORTS = [
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
     "fmi_stream_destroy", "fmi_stream_sync", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
-    "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits",
+    "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
     "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
-    "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
+    "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_get_full",
     "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
@@ -126,6 +126,7 @@ def _bind(lib):
         "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_locate_hits": ([_p, _i, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
+        "fmi_resolve_hits_text": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
         "fmi_inverse_locate_batch": ([_p, _i, _p, _u64, _p, _p], _i),
         "fmi_encode_bases": ([_p, _u64, _p], _i),
         "fmi_copy_bench": ([_p, _p, _u64, C.POINTER(C.c_float), _p], _i),
@@ -146,6 +147,8 @@ def _bind(lib):
                               _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
                                       C.POINTER(_u64)], _i),
+        "fmi_format_results": ([_p, _u64, _i, C.c_char_p, _u32, _p, _u32, _p, _p, _u32, _p,
+                                C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(_u64)], _i),
         "fmi_map_inexact_seeds_batch": ([_p, _p, _p, _u32, _u32, _u32, _u32, _u32, _u64, _p,
                                          _p, _p, C.POINTER(_u64), _p], _i),
         "fmi_seed_map_batch": ([_p, _p, _p, _u32, _u32, _u32, _u32, _p, _u64, _p, _u64,
@@ -663,6 +666,28 @@ def inverse_locate_batch(ix, text_pos, forward=0, stream=None):
     return dr.download(np.uint64, n, stream)
 
 
+def format_results(ix, read_index, strand, mapping, hits, chrom, coord):
+    """fmi_format_results for one (read, strand) list: hits (HIT_DTYPE),
+    chrom / coord [len(hits), row_cap] as resolve_hits gives them for the
+    list.  Returns (text, rows beyond row_cap)."""
+    hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+    chrom = np.ascontiguousarray(chrom, dtype=np.int32)
+    coord = np.ascontiguousarray(coord, dtype=np.uint64)
+    row_cap = chrom.shape[1] if chrom.ndim == 2 and chrom.shape[1] else 1
+    mapping = bytes(mapping)
+    n = C.c_size_t()
+    miss = C.c_uint64()
+    L = lib()
+    check(L.fmi_format_results(ix.h, read_index, strand, mapping, len(mapping), _ptr(hits),
+                               len(hits), _ptr(chrom), _ptr(coord), row_cap, None, 0,
+                               C.byref(n), C.byref(miss)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(L.fmi_format_results(ix.h, read_index, strand, mapping, len(mapping), _ptr(hits),
+                               len(hits), _ptr(chrom), _ptr(coord), row_cap, buf, n.value + 1,
+                               C.byref(n), C.byref(miss)))
+    return buf.raw[:n.value].decode(), int(miss.value)
+
+
 def encode_bases(text):
     """fmi_encode_bases: ASCII bases (str, bytes or uint8 array) to codes
     0..3 for A/C/G/T in either case and 4 for every other byte."""
@@ -676,17 +701,23 @@ def encode_bases(text):
     return dst
 
 
-def resolve_hits(ix, batch, out, row_cap, stream=None):
+def resolve_hits(ix, batch, out, row_cap, text_rule=False, stream=None):
+    """fmi_resolve_hits (or fmi_resolve_hits_text with text_rule): chrom and
+    coord [nreads, 2, cap, row_cap] on the host."""
     n, cap = batch.nreads, out.cap
     tot = max(2 * n * cap * row_cap, 1)
     dc = DeviceBuffer(tot * 4)
     dd = DeviceBuffer(tot * 8)
-    check(lib().fmi_resolve_hits(ix.h, out.d_hits.ptr, out.d_counts.ptr, batch.d_offsets.ptr, n,
-                                 cap, row_cap, dc.ptr, dd.ptr, stream))
+    fn = lib().fmi_resolve_hits_text if text_rule else lib().fmi_resolve_hits
+    check(fn(ix.h, out.d_hits.ptr, out.d_counts.ptr, batch.d_offsets.ptr, n, cap, row_cap, dc.ptr,
+             dd.ptr, stream))
     sync(stream)
     shape = (n, 2, cap, row_cap)
-    return (dc.download(np.int32, 2 * n * cap * row_cap, stream).reshape(shape),
-            dd.download(np.uint64, 2 * n * cap * row_cap, stream).reshape(shape))
+    res = (dc.download(np.int32, 2 * n * cap * row_cap, stream).reshape(shape),
+           dd.download(np.uint64, 2 * n * cap * row_cap, stream).reshape(shape))
+    dc.free()
+    dd.free()
+    return res
 
 
 SEED_REGION_DTYPE = np.dtype([

@@ -303,6 +303,33 @@ int fmi_encode_bases(const char *src, uint64_t n, uint8_t *dst);
 int fmi_resolve_hits(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
                      const uint64_t *d_offsets, uint32_t nreads, uint32_t cap,
                      uint32_t row_cap, int32_t *d_chrom, uint64_t *d_coord, fmi_stream s);
+/* The same resolution with the length manage_single_result uses
+ * (search/results.h:146-165): enW = end - start + 1 of the record, minus 1
+ * for a DELETION, plus 1 for an INSERTION, in place of the read's length.
+ * A record of the second pass of BWSearch1CPU carries start = half, so its
+ * key moves by half against fmi_resolve_hits; these are the rows the
+ * reference's text output prints (fmi_format_results). */
+int fmi_resolve_hits_text(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,
+                          const uint64_t *d_offsets, uint32_t nreads, uint32_t cap,
+                          uint32_t row_cap, int32_t *d_chrom, uint64_t *d_coord, fmi_stream s);
+
+/* The reference's text lines for one (read, strand) record list
+ * (write_results, search/results.c:3-43; manage_single_result,
+ * search/results.h:125-187): records whose (k, l) repeats an earlier
+ * record's are skipped; every other record's SA rows that lie inside a
+ * contig give "read_<read_index> <+|-> <contig name> <coordinate>
+ * <errors><mask> <read text>\n", the mask "_<pos>d", "_<pos>m<base>" or
+ * "_<pos>i<base>".  hits are the list's records on the host (as
+ * fmi_search1_batch stores them), chrom / coord their rows as
+ * fmi_resolve_hits_text writes them (row_cap per record, chrom -1 for a row
+ * that crosses a contig end), mapping the read text (its first nW characters are
+ * printed).  Writes at most buf_size - 1 bytes and a NUL to buf (buf may be
+ * NULL with buf_size 0), the full length to *len and the rows beyond
+ * row_cap (not printed) to *rows_missing.  Host function. */
+int fmi_format_results(const fmi_index *ix, uint64_t read_index, int strand, const char *mapping,
+                       uint32_t nW, const fmi_hit *hits, uint32_t nhits, const int32_t *chrom,
+                       const uint64_t *coord, uint32_t row_cap, char *buf, size_t buf_size,
+                       size_t *len, uint64_t *rows_missing);
 
 /* ------------------------------------------------------ seed mapping */
 /* One region of bwt_map_exact_seed (bwt.c:1200-1294): an occurrence of a

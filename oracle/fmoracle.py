@@ -436,6 +436,9 @@ class RefCore:
         lib.ref_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
         lib.ref_exact_batch.restype = C.c_double
         lib.ref_save_dir.argtypes = [_p, C.c_char_p, C.c_uint64, _p, _p]
+        lib.ref_write_results.argtypes = [_p, C.c_int, _p, C.c_char_p, C.c_int16, C.c_uint64,
+                                          C.c_uint32, C.c_uint64, _p, _p, C.c_char_p, C.c_size_t]
+        lib.ref_write_results.restype = C.c_size_t
         lib.ref_exact_locate_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, _p, _p, C.c_int]
         lib.ref_exact_locate_batch.restype = C.c_double
         self.lib = lib
@@ -537,6 +540,21 @@ class RefCore:
         full = self.lib.ref_search_k(self.h, strand, _ptr(W), len(W), fragsize, list_cap,
                                      anchor_cap, _ptr(fin), C.byref(nf), _ptr(anc), C.byref(na))
         return bool(full), fin[:nf.value].copy(), anc[:na.value].copy()
+
+    def write_results(self, strand, W, mapping, read_index, cap, starts, ends):
+        """The reference's write_results text for BWSearch1CPU on one strand
+        of W (codes) with the read text `mapping` (bytes)."""
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        st = np.ascontiguousarray(starts, dtype=np.uint64)
+        en = np.ascontiguousarray(ends, dtype=np.uint64)
+        size = 1 << 16
+        while True:
+            buf = C.create_string_buffer(size)
+            n = self.lib.ref_write_results(self.h, strand, _ptr(W), mapping, len(W), read_index,
+                                           cap, len(st), _ptr(st), _ptr(en), buf, size)
+            if n < size:
+                return buf.raw[:n].decode()
+            size = n + 1
 
     def save_dir(self, path, starts=None, ends=None):
         """Writes the reference's index directory (its own writers) to path;

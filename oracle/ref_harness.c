@@ -289,6 +289,50 @@ uint32_t ref_search1(void *h, int strand, const uint8_t *W, int16_t len, uint32_
   return nres;
 }
 
+/* The reference's text output of one read's 1-error search: BWSearch1CPU on
+ * one strand (as ref_search1, list capacity cap), then write_results
+ * (search/results.c:3-43) with contigs chr1.. of the given start/end, the
+ * read text `mapping` and read_index.  Writes at most outcap - 1 bytes of
+ * text to out (NUL-terminated) and returns the full length. */
+size_t ref_write_results(void *h, int strand, const uint8_t *W, const char *mapping, int16_t len,
+                         uint64_t read_index, uint32_t cap, uint64_t kcount,
+                         const uint64_t *kstart, const uint64_t *kend, char *out, size_t outcap) {
+  ref_ctx *c = (ref_ctx *)h;
+  results_list rl;
+  new_results_list(&rl, cap ? cap : 1);
+  rl.max_results = cap;
+  rl.read_index = read_index;
+  result res;
+  bound_result(&res, 0, len - 1);
+  change_result(&res, 0, size_SA(&c->backward) - 1, 0);
+  if (strand == 0)
+    BWSearch1CPU((uint8_t *)W, &c->backward, &c->forward, &res, &rl, NULL, 4);
+  else
+    BWSearch1CPU((uint8_t *)W, &c->forward_rev, &c->backward_rev, &res, &rl, NULL, 4);
+  exome *ex = exome_new();
+  ex->offset[0] = 0;
+  for (uint64_t i = 0; i < kcount; i++) {
+    snprintf(ex->chromosome + i * IDMAX, IDMAX, "chr%llu", (unsigned long long)(i + 1));
+    ex->start[i] = kstart[i];
+    ex->end[i] = kend[i];
+    ex->offset[i + 1] = ex->offset[i] + (kend[i] - kstart[i] + 1);
+  }
+  ex->size = kcount;
+  intmax_t *kk = (intmax_t *)malloc((rl.num_results + 1) * sizeof(intmax_t));
+  intmax_t *ll = (intmax_t *)malloc((rl.num_results + 1) * sizeof(intmax_t));
+  char *buf = NULL;
+  size_t blen = 0;
+  FILE *fp = open_memstream(&buf, &blen);
+  write_results(&rl, kk, ll, ex, &c->backward, &c->forward, (char *)mapping, (uintmax_t)len,
+                strand == 0 ? 1 : 0, fp, &c->cfg);
+  fclose(fp);
+  size_t n = blen < outcap ? blen : (outcap ? outcap - 1 : 0);
+  if (outcap) { memcpy(out, buf, n); out[n] = 0; }
+  free(buf); free(kk); free(ll); free(rl.list);
+  exome_free(ex);
+  return blen;
+}
+
 /* The mapping layer's 1-error search of W[start..end] on one strand: the
  * interval vectors, then BWSearch1VectorHelper, as bwt.c:1940-1950 +
  * 1994-2003 (start = 0) and bwt.c:1447-1457 + 1465-1479 (a seed). */

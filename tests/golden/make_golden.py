@@ -24,6 +24,14 @@ inputs and outputs as data:
                       fragsize max(len // 3, 8), lists of sk_cap records and
                       anchor lists of 16: the final list, the anchors, the
                       return value (1: the anchor list filled up)
+    wr_blob, wr_off   write_results' text lines (search/results.c:3-43) for
+                      BWSearch1CPU's list (cap 512) with the contigs
+                      wr_kstart / wr_kend named chr1.., the read's ACGT text:
+                      the text of (read i, strand s) is
+                      wr_blob[wr_off[2i+s]:wr_off[2i+s+1]]; only for reads
+                      of 12..99 bp (wr_done): the reference's 6-byte mask
+                      buffer (results.h:134) overflows for error positions
+                      of 3 digits, and shorter reads print thousands of rows
 
   golden_repeats.npz
     the same fields for 400 reads of 0..60 bp on 8 kbp made of 40 mutated
@@ -154,6 +162,29 @@ def write(path, g, reads, offsets, rng):
     sk_hits = np.concatenate(k_hits) if k_hits else np.zeros(0, fo.HITK_DTYPE)
     sk_anch = np.concatenate(k_anch) if k_anch else np.zeros(0, fo.HITK_DTYPE)
 
+    # write_results text with a karyotype of 2-3 contigs
+    G = len(g)
+    if G >= 30000:
+        klen = [G // 3, G // 3 + 4000, G - 2 * (G // 3) - 4000]
+        kst = [0, 100, 0]
+    else:
+        klen = [G // 2, G - G // 2]
+        kst = [0, 0]
+    kstart = np.array(kst, np.uint64)
+    kend = kstart + np.array(klen, np.uint64) - 1
+    wr_parts = []
+    wr_done = np.zeros(n, np.uint8)
+    for i in range(n):
+        W = reads[offsets[i]:offsets[i + 1]]
+        ok = 12 <= len(W) < 100
+        wr_done[i] = ok
+        mapping = "".join("ACGT"[int(x)] for x in W).encode()
+        for s in (0, 1):
+            wr_parts.append(ref.write_results(s, W, mapping, i, CAP, kstart, kend).encode()
+                            if ok else b"")
+    wr_off = np.concatenate([[0], np.cumsum([len(x) for x in wr_parts])]).astype(np.uint64)
+    wr_blob = np.frombuffer(b"".join(wr_parts), np.uint8)
+
     ex = {}
     lsa = ref.size_sa - 1
     for view in range(4):
@@ -189,7 +220,8 @@ def write(path, g, reads, offsets, rng):
         bwti_dollar=np.uint64(dollari), s1v_counts=v_counts, s1v_hits=s1v_hits,
         s1w_bounds=bounds, s1w_counts=w_counts, s1w_hits=s1w_hits, sk_fragsize=k_fs,
         sk_counts=k_counts, sk_hits=sk_hits, sk_anchor_counts=k_acounts, sk_anchors=sk_anch,
-        sk_full=k_full, sk_cap=np.uint32(KCAP), **ex)
+        sk_full=k_full, sk_cap=np.uint32(KCAP), wr_kstart=kstart, wr_kend=kend, wr_blob=wr_blob,
+        wr_off=wr_off, wr_done=wr_done, **ex)
     print("wrote", path, os.path.getsize(path), "bytes;", int(counts.sum()), "records")
 
 
