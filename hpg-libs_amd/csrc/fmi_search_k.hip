@@ -31,7 +31,7 @@ struct KList {
 
 struct KView {
   const DevDir *d;
-  uint32_t comp;
+  uint32_t comp, dsel;               // dsel: 0 BWT of X, 1 BWT of reverse X
 };
 
 struct KCtx {
@@ -40,7 +40,36 @@ struct KCtx {
   const uint8_t *W;
   int32_t nW;
   uint8_t *D;
+  bool text;                         // expanded index: single rows step on the text
 };
+
+// On an expanded index a one-row interval is stepped on the text: the LF
+// steps of W[i], W[i + step], ... (count positions) from the row whose
+// suffix is q keep one row while the read's bases equal the text bases
+// before q (in the view's LF order).  Returns the positions matched, the
+// suffix after them in q_end, and whether a step inside count fails (a base
+// differs or is N, or the text start is reached: the '$' row has no
+// successor), where klf would give an empty interval.
+__device__ __forceinline__ int32_t k_run(const KCtx &c, const KView &v, uint32_t q, int32_t i,
+                                         int32_t step, int32_t count, uint32_t &q_end,
+                                         bool &dies) {
+  int32_t done = 0;
+  dies = false;
+  while (done < count) {
+    uint32_t lim = (uint32_t)(count - done) < 16u ? (uint32_t)(count - done) : 16u;
+    if (lim > q) lim = q;
+    if (lim == 0u) { dies = true; break; }
+    const uint32_t m = text_match(*c.ix, v.dsel, v.comp, q, c.W, i + done * step, step, lim);
+    done += (int32_t)m;
+    q -= m;
+    if (m < lim) { dies = true; break; }
+  }
+  q_end = q;
+  return done;
+}
+
+__device__ __forceinline__ uint32_t k_sa(const KView &v, int64_t row) { return v.d->sa[row]; }
+__device__ __forceinline__ uint32_t k_isa(const KView &v, uint32_t q) { return v.d->isa[q]; }
 
 // BWiteration (runtime.h:38-42) on a view; N (code > 3) gives (n + 1, n).
 // Only live intervals reach it; a dead one is passed through unchanged
@@ -90,12 +119,23 @@ __device__ __forceinline__ void k_clear(fmi_hitk &r) {
   r.k = 0; r.l = 0; r.start = 0; r.pos = 0; r.end = 0; r.dir = 0; r.num_mismatches = 0;
 }
 
-// BWExactSearchBackward / Forward (search.h:9-65) with the anchor.
+// BWExactSearchBackward / Forward (search.h:9-65) with the anchor.  In a
+// text run the empty interval where the search dies is not formed: the
+// callers only keep a live result, and the anchor holds the last live one.
 __device__ void k_exact(const KCtx &c, const KView &v, int dir, fmi_hitk &r, fmi_hitk *anchor) {
   int64_t k = 0, l = 0, k2 = r.k, l2 = r.l;
   int32_t i;
   if (dir == 0) {
     for (i = r.pos; i >= r.start; --i) {
+      if (c.text && k2 == l2) {
+        uint32_t q2;
+        bool dies;
+        const int32_t done = k_run(c, v, k_sa(v, k2), i, -1, i - r.start + 1, q2, dies);
+        if (done) { k = l = k2 = l2 = k_isa(v, q2); }
+        i -= done;
+        if (dies) { k2 = (int64_t)c.ix->n + 1; l2 = c.ix->n; }
+        break;
+      }
       klf(c, v, k2, l2, c.W[i], k2, l2);
       if (k2 > l2) break;
       k = k2; l = l2;
@@ -107,6 +147,15 @@ __device__ void k_exact(const KCtx &c, const KView &v, int dir, fmi_hitk &r, fmi
     }
   } else {
     for (i = r.pos; i <= r.end; ++i) {
+      if (c.text && k2 == l2) {
+        uint32_t q2;
+        bool dies;
+        const int32_t done = k_run(c, v, k_sa(v, k2), i, 1, r.end - i + 1, q2, dies);
+        if (done) { k = l = k2 = l2 = k_isa(v, q2); }
+        i += done;
+        if (dies) { k2 = (int64_t)c.ix->n + 1; l2 = c.ix->n; }
+        break;
+      }
       klf(c, v, k2, l2, c.W[i], k2, l2);
       if (k2 > l2) break;
       k = k2; l = l2;
@@ -118,6 +167,25 @@ __device__ void k_exact(const KCtx &c, const KView &v, int dir, fmi_hitk &r, fmi
     }
   }
   r.k = k2; r.l = l2; r.pos = (int16_t)i;
+}
+
+// The exact extension of the final helpers: klf over W[i], W[i + step], ...
+// (count positions) until the interval empties; a one-row interval runs on
+// the text and its row is read back only when it survives.
+__device__ __forceinline__ void k_extend(const KCtx &c, const KView &v, int64_t &k, int64_t &l,
+                                         int32_t i, int32_t step, int32_t count) {
+  for (; count > 0; i += step, --count) {
+    if (c.text && k == l) {
+      uint32_t q2;
+      bool dies;
+      const int32_t done = k_run(c, v, k_sa(v, k), i, step, count, q2, dies);
+      if (dies) { k = (int64_t)c.ix->n + 1; l = c.ix->n; }
+      else if (done) { k = l = k_isa(v, q2); }
+      return;
+    }
+    klf(c, v, k, l, c.W[i], k, l);
+    if (k > l) return;
+  }
 }
 
 // BWExactFinalResultBackward / Forward (search.h:66-158): the block filter,
@@ -134,10 +202,7 @@ __device__ bool k_final1(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KL
     } else if (!((pos + 1) % bs)) {
       return false;
     }
-    for (int32_t i = pos; i >= start; --i) {
-      klf(c, v, k, l, c.W[i], k, l);
-      if (k > l) break;
-    }
+    k_extend(c, v, k, l, pos, -1, pos - start + 1);
     if (k <= l) { k_change(r, k, l, start - 1); KADD(r, fin); }
   } else {
     if ((cb > lb) || (pos == end + 1)) {
@@ -146,10 +211,7 @@ __device__ bool k_final1(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KL
     } else if (!(pos % bs)) {
       return false;
     }
-    for (int32_t i = pos; i <= end; ++i) {
-      klf(c, v, k, l, c.W[i], k, l);
-      if (k > l) break;
-    }
+    k_extend(c, v, k, l, pos, 1, end - pos + 1);
     if (k <= l) { k_change(r, k, l, end + 1); KADD(r, fin); }
   }
   return false;
@@ -170,10 +232,7 @@ __device__ bool k_finals(const KCtx &c, const KView &v, int dir, KList &prev, KL
       } else if (!((pos + 1) % bs)) {
         continue;
       }
-      for (int32_t i = pos; i >= start; --i) {
-        klf(c, v, k, l, c.W[i], k, l);
-        if (k > l) break;
-      }
+      k_extend(c, v, k, l, pos, -1, pos - start + 1);
       if (k <= l) { k_change(r, k, l, start - 1); KADD(r, out); }
     } else {
       if ((cb > lb) || (pos == end + 1)) {
@@ -182,10 +241,7 @@ __device__ bool k_finals(const KCtx &c, const KView &v, int dir, KList &prev, KL
       } else if (!(pos % bs)) {
         continue;
       }
-      for (int32_t i = pos; i <= end; ++i) {
-        klf(c, v, k, l, c.W[i], k, l);
-        if (k > l) break;
-      }
+      k_extend(c, v, k, l, pos, 1, end - pos + 1);
       if (k <= l) { k_change(r, k, l, end + 1); KADD(r, out); }
     }
   }
@@ -314,6 +370,24 @@ __device__ bool k_partials(const KCtx &c, uint8_t max_errors, const KView &v, in
       for (int32_t i = pos; i >= lbp; --i) {
         k = kn; l = ln;
         if (k > l) break;
+        if (c.text && k == l) {
+          // one row: no size change (no record) until the step that empties it
+          uint32_t q2;
+          bool dies;
+          const int32_t done = k_run(c, v, k_sa(v, k), i, -1, i - lbp + 1, q2, dies);
+          i -= done;
+          if (!dies) {
+            if (done) kn = ln = k_isa(v, q2);
+            break;
+          }
+          res_next = -1;                     // the step at i empties the row
+          kn = (int64_t)c.ix->n + 1; ln = c.ix->n;
+          if (eb < c.D[i]) break;
+          if (done) k = l = k_isa(v, q2);
+          k_change(r, k, l, i);
+          KADD(r, next);
+          break;
+        }
         klf(c, v, k, l, c.W[i], kn, ln);
         res = res_next; res_next = ln - kn;
         if (res == res_next || eb < c.D[i]) continue;
@@ -329,6 +403,23 @@ __device__ bool k_partials(const KCtx &c, uint8_t max_errors, const KView &v, in
       for (int32_t i = pos; i <= lbp; ++i) {
         k = kn; l = ln;
         if (k > l) break;
+        if (c.text && k == l) {
+          uint32_t q2;
+          bool dies;
+          const int32_t done = k_run(c, v, k_sa(v, k), i, 1, lbp - i + 1, q2, dies);
+          i += done;
+          if (!dies) {
+            if (done) kn = ln = k_isa(v, q2);
+            break;
+          }
+          res_next = -1;
+          kn = (int64_t)c.ix->n + 1; ln = c.ix->n;
+          if (eb < c.D[i]) break;
+          if (done) k = l = k_isa(v, q2);
+          k_change(r, k, l, i);
+          KADD(r, next);
+          break;
+        }
         klf(c, v, k, l, c.W[i], kn, ln);
         res = res_next; res_next = ln - kn;
         if (res == res_next || eb < c.D[i]) continue;
@@ -387,47 +478,59 @@ __device__ void k_turn(const KCtx &c, const KView &b, const KView &f, KList &L, 
 
 // calculateDBackward / Forward (search.c:1045-1169), with the reference's
 // 8-bit counter and int8 `last`.
+// One walk of calculateD*: klf over W[i], W[i + step], ... (count
+// positions), restarting from the whole index where the interval empties
+// and counting the restarts in z; D[i] = z.  One-row stretches run on the
+// text; the interval left at the end is not used by the callers.
+__device__ __forceinline__ void k_dwalk(const KCtx &c, const KView &v, int64_t k, int64_t l,
+                                        int32_t i, int32_t step, int32_t count, uint8_t &z) {
+  const int64_t nmax = c.ix->n;
+  uint8_t *D = c.D;
+  while (count > 0) {
+    if (c.text && k == l) {
+      uint32_t q2;
+      bool dies;
+      const int32_t done = k_run(c, v, k_sa(v, k), i, step, count, q2, dies);
+      for (int32_t t = 0; t < done; ++t) D[i + t * step] = z;
+      i += done * step;
+      count -= done;
+      if (!dies) return;
+      k = 0; l = nmax; z++;                // the step at i empties the row
+      D[i] = z;
+      i += step;
+      --count;
+      continue;
+    }
+    klf(c, v, k, l, c.W[i], k, l);
+    if (k > l) { k = 0; l = nmax; z++; }
+    D[i] = z;
+    i += step;
+    --count;
+  }
+}
+
 __device__ void k_calc_d(const KCtx &c, const fmi_hitk &r, bool backward) {
   const int64_t nmax = c.ix->n;
-  int64_t k = r.k, l = r.l;
   uint8_t z = 0;
   int8_t last;
   uint8_t *D = c.D;
   const int32_t nW = c.nW;
   if (backward) {
-    for (int32_t i = r.pos; i >= 0; --i) {
-      klf(c, c.B, k, l, c.W[i], k, l);
-      if (k > l) { k = 0; l = nmax; z++; }
-      D[i] = z;
-    }
+    k_dwalk(c, c.B, r.k, r.l, r.pos, -1, r.pos + 1, z);
     if (r.end == (int16_t)(nW - 1)) {
       last = (int8_t)D[0];
     } else {
-      k = 0; l = nmax;
-      for (int32_t i = r.end + 1; i < nW; ++i) {
-        klf(c, c.F, k, l, c.W[i], k, l);
-        if (k > l) { k = 0; l = nmax; z++; }
-        D[i] = z;
-      }
+      k_dwalk(c, c.F, 0, nmax, r.end + 1, 1, nW - (r.end + 1), z);
       last = (int8_t)D[nW - 1];
       for (int32_t i = r.end + 1; i < nW; ++i) D[i] = (uint8_t)(last - D[i]);
     }
     for (int32_t i = r.pos; i >= 0; --i) D[i] = (uint8_t)(last - D[i]);
   } else {
-    for (int32_t i = r.pos; i < nW; ++i) {
-      klf(c, c.F, k, l, c.W[i], k, l);
-      if (k > l) { k = 0; l = nmax; z++; }
-      D[i] = z;
-    }
+    k_dwalk(c, c.F, r.k, r.l, r.pos, 1, nW - r.pos, z);
     if (r.start == 0) {
       last = (int8_t)D[nW - 1];
     } else {
-      k = 0; l = nmax;
-      for (int32_t i = r.start - 1; i >= 0; --i) {
-        klf(c, c.B, k, l, c.W[i], k, l);
-        if (k > l) { k = 0; l = nmax; z++; }
-        D[i] = z;
-      }
+      k_dwalk(c, c.B, 0, nmax, r.start - 1, -1, r.start, z);
       last = (int8_t)D[0];
       for (int32_t i = r.start - 1; i >= 0; --i) D[i] = (uint8_t)(last - D[i]);
     }
@@ -603,8 +706,9 @@ __global__ __launch_bounds__(256) void search_k_kernel(KParams P) {
     KCtx c;
     c.ix = &P.ix;
     // + strand: (backward, forward); - strand: (forward_rev, backward_rev)
-    c.B = KView{strand ? &P.ix.d[1] : &P.ix.d[0], strand};
-    c.F = KView{strand ? &P.ix.d[0] : &P.ix.d[1], strand};
+    c.B = KView{strand ? &P.ix.d[1] : &P.ix.d[0], strand, strand ? 1u : 0u};
+    c.F = KView{strand ? &P.ix.d[0] : &P.ix.d[1], strand, strand ? 0u : 1u};
+    c.text = P.ix.text != nullptr && P.ix.d[0].sa != nullptr && P.ix.d[1].sa != nullptr;
     c.W = P.reads + off;
     c.nW = len;
     c.D = P.dscratch + lane * (uint64_t)FMI_MAX_READ_LEN;

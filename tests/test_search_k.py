@@ -168,12 +168,16 @@ def test_gpu_search_k_golden(kgpu, expanded):
 
 
 @pytest.mark.gpu
-def test_gpu_search_k_vs_oracle(gpu_engine):
-    """Repeats, N bases, fragments of 2-5, list capacities that truncate."""
+@pytest.mark.parametrize("expanded", [False, True])
+def test_gpu_search_k_vs_oracle(gpu_engine, expanded):
+    """Repeats, N bases, fragments of 2-5, list capacities that truncate;
+    on the expanded index one-row stretches run on the text."""
     hp = gpu_engine
     g, _, _ = synth.genome(200_000, seed=75, repeat_frac=0.4, fam_len=(300, 2000),
                            divergence=(0.0, 0.05))
     ix = hp.Index.build(g, 16).to_device()
+    if expanded:
+        ix.expand()
     o = fo.Oracle(g, ratio=16)
     lens = np.random.default_rng(76).integers(8, 160, 800)
     reads, offs, _ = synth.reads(g, 800, seed=77, sub_rate=0.03, n_rate=0.003, lengths=lens)
