@@ -122,7 +122,7 @@ __device__ __forceinline__ void k_clear(fmi_hitk &r) {
 // BWExactSearchBackward / Forward (search.h:9-65) with the anchor.  In a
 // text run the empty interval where the search dies is not formed: the
 // callers only keep a live result, and the anchor holds the last live one.
-__device__ void k_exact(const KCtx &c, const KView &v, int dir, fmi_hitk &r, fmi_hitk *anchor) {
+__device__ __forceinline__ void k_exact(const KCtx &c, const KView &v, int dir, fmi_hitk &r, fmi_hitk *anchor) {
   int64_t k = 0, l = 0, k2 = r.k, l2 = r.l;
   int32_t i;
   if (dir == 0) {
@@ -190,7 +190,7 @@ __device__ __forceinline__ void k_extend(const KCtx &c, const KView &v, int64_t 
 
 // BWExactFinalResultBackward / Forward (search.h:66-158): the block filter,
 // then an exact extension to the end; only a surviving record is changed.
-__device__ bool k_final1(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KList &fin,
+__device__ __forceinline__ bool k_final1(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KList &fin,
                          int32_t bs, int32_t lb) {
   const int32_t start = r.start, pos = r.pos, end = r.end;
   int64_t k = r.k, l = r.l;
@@ -218,7 +218,7 @@ __device__ bool k_final1(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KL
 }
 
 // BWExactFinalResultsBackward / Forward (search.c:3-114)
-__device__ bool k_finals(const KCtx &c, const KView &v, int dir, KList &prev, KList &out,
+__device__ __forceinline__ bool k_finals(const KCtx &c, const KView &v, int dir, KList &prev, KList &out,
                          int32_t bs, int32_t lb) {
   for (uint32_t ii = 0; ii < prev.n; ++ii) {
     fmi_hitk &r = prev.list[ii];
@@ -254,7 +254,7 @@ __device__ bool k_finals(const KCtx &c, const KView &v, int dir, KList &prev, KL
 // insertion and mismatch of each base, as the previous error allows; the
 // record is updated field by field as the reference updates it.
 template <bool FINAL>
-__device__ bool k_branch(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KList &out,
+__device__ __forceinline__ bool k_branch(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KList &out,
                          int32_t bs, int32_t lb) {
   const int32_t pos = r.pos;
   const int32_t step = dir ? 1 : -1;
@@ -331,7 +331,7 @@ __device__ bool k_branch(const KCtx &c, const KView &v, int dir, fmi_hitk &r, KL
 }
 
 template <bool FINAL>
-__device__ bool k_branches(const KCtx &c, const KView &v, int dir, KList &prev, KList &out,
+__device__ __forceinline__ bool k_branches(const KCtx &c, const KView &v, int dir, KList &prev, KList &out,
                            int32_t bs, int32_t lb) {
   for (uint32_t ii = 0; ii < prev.n; ++ii) {
     fmi_hitk &r = prev.list[ii];
@@ -351,7 +351,7 @@ __device__ bool k_branches(const KCtx &c, const KView &v, int dir, KList &prev, 
 }
 
 // BWExactPartialResultsBackward / Forward (search.c:471-646)
-__device__ bool k_partials(const KCtx &c, uint8_t max_errors, const KView &v, int dir, KList &prev,
+__device__ __forceinline__ bool k_partials(const KCtx &c, uint8_t max_errors, const KView &v, int dir, KList &prev,
                            KList &next, KList &next_i, int32_t bs, int32_t lb) {
   for (uint32_t ii = 0; ii < prev.n; ++ii) {
     fmi_hitk &r = prev.list[ii];
@@ -435,7 +435,7 @@ __device__ bool k_partials(const KCtx &c, uint8_t max_errors, const KView &v, in
 
 // change_direction (search.h:160-198): min / max ISA on the other index of
 // every row's mirrored suffix; a position outside the text is left out.
-__device__ void k_change_dir(const KCtx &c, const KView &b, const KView &f, fmi_hitk &r) {
+__device__ __forceinline__ void k_change_dir(const KCtx &c, const KView &b, const KView &f, fmi_hitk &r) {
   const int64_t siz = (int64_t)c.ix->n + 1;
   int64_t ki = siz - 1, li = 0;
   int32_t err_offset = 0;
@@ -456,7 +456,7 @@ __device__ void k_change_dir(const KCtx &c, const KView &b, const KView &f, fmi_
 }
 
 // BWChangeDirectionBackward / Forward (search.c:1002-1043)
-__device__ void k_turn(const KCtx &c, const KView &b, const KView &f, KList &L, bool to_fwd,
+__device__ __forceinline__ void k_turn(const KCtx &c, const KView &b, const KView &f, KList &L, bool to_fwd,
                        int32_t bound) {
   for (uint32_t ii = L.n; ii > 0; --ii) {
     fmi_hitk &r = L.list[ii - 1];
@@ -509,7 +509,7 @@ __device__ __forceinline__ void k_dwalk(const KCtx &c, const KView &v, int64_t k
   }
 }
 
-__device__ void k_calc_d(const KCtx &c, const fmi_hitk &r, bool backward) {
+__device__ __forceinline__ void k_calc_d(const KCtx &c, const fmi_hitk &r, bool backward) {
   const int64_t nmax = c.ix->n;
   uint8_t z = 0;
   int8_t last;
@@ -538,8 +538,67 @@ __device__ void k_calc_d(const KCtx &c, const fmi_hitk &r, bool backward) {
   }
 }
 
+// One pass of BWSearchCPU (search.c:1171-1405): the exact search of
+// fragment i, then up to fragments - 1 rounds of partial extension and
+// branching and the final step.  A forward pass (the middle fragments
+// i = half - 1 .. 1 and fragment 0, search.c:1214-1296) searches on
+// `forward` left to right and, for a middle fragment, turns its finished
+// partials to `backward`; a backward pass (fragments half .. fragments - 2 and
+// the last, search.c:1298-1401) mirrors it.  Each pass starts from empty
+// working lists; the final list and the anchors carry over.
+__device__ __forceinline__ bool k_pass(const KCtx &c, KList &prev, KList &next, KList &prev_i,
+                                       KList &next_i, KList &fin, KList &anc, int32_t fs,
+                                       int32_t fragments, int32_t half, bool fwd, int32_t i) {
+  const int32_t nW = c.nW;
+  const int64_t lsa = c.ix->n;
+  const uint8_t maxe = (uint8_t)(fragments - 1);
+  const bool two = fwd ? i > 0 : i < fragments - 1;   // a middle fragment
+  const KView &X = fwd ? c.F : c.B;
+  const KView &Y = fwd ? c.B : c.F;
+  const int dx = fwd ? 1 : 0, dy = 1 - dx;
+  const int32_t lb = fwd ? half - 1 : 0;
+  const int32_t bound = fwd ? 0 : nW - 1;
+  fmi_hitk r, ra;
+  bool flow = true;
+  int32_t err_count = fragments - 1;
+  prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
+  k_clear(r);
+  r.dir = (uint8_t)dx;
+  r.start = (int16_t)(fs * i);
+  r.end = (int16_t)(fwd || i < fragments - 1 ? fs * (i + 1) - 1 : nW - 1);
+  k_change(r, 0, lsa, fwd ? r.start : r.end);
+  k_exact(c, X, dx, r, &ra);
+  if (anc.list) KADD(ra, anc);
+  if (fwd) r.end = (int16_t)(nW - 1);
+  else r.start = 0;
+  if (r.k > r.l) return false;
+  k_calc_d(c, r, !fwd);
+  KADD(r, prev);
+  KList &out1 = two ? prev_i : fin;          // where X's complete partials go
+  while (err_count > 0) {
+    if (k_partials(c, maxe, X, dx, prev, next, out1, fs, lb)) { flow = false; break; }
+    if (two) {
+      k_turn(c, X, Y, prev_i, !fwd, bound);
+      if (k_partials(c, maxe, Y, dy, prev_i, next_i, fin, fs, lb)) { flow = false; break; }
+    }
+    if (err_count == 1) break;
+    if (k_branches<false>(c, X, dx, next, prev, 0, 0)) { flow = false; break; }
+    if (two && k_branches<false>(c, Y, dy, next_i, prev_i, 0, 0)) { flow = false; break; }
+    err_count--;
+  }
+  if (flow) {
+    k_branches<true>(c, X, dx, next, out1, fs, lb);
+    if (two) {
+      k_turn(c, X, Y, prev_i, !fwd, bound);
+      k_finals(c, Y, dy, prev_i, fin, fs, lb);
+      k_branches<true>(c, Y, dy, next_i, fin, fs, lb);
+    }
+  }
+  return false;
+}
+
 // BWSearchCPU (search.c:1171-1405) for one strand of one read.
-__device__ bool k_search(const KCtx &c, KList &prev, KList &next, KList &prev_i, KList &next_i,
+__device__ __forceinline__ bool k_search(const KCtx &c, KList &prev, KList &next, KList &prev_i, KList &next_i,
                          KList &fin, KList &anc, int32_t fs, int type) {
   fmi_hitk r, ra;
   const int32_t nW = c.nW;
@@ -547,7 +606,6 @@ __device__ bool k_search(const KCtx &c, KList &prev, KList &next, KList &prev_i,
   int32_t half = fragments / 2;
   if (fragments % 2) half++;
   const int64_t lsa = c.ix->n;
-  const uint8_t maxe = (uint8_t)(fragments - 1);
   anc.n = 0;
 
   if (fragments <= 1) {
@@ -564,108 +622,12 @@ __device__ bool k_search(const KCtx &c, KList &prev, KList &next, KList &prev_i,
     return false;
   }
 
-  for (int32_t i = half - 1; i > 0; --i) {
-    bool flow = true;
-    int32_t err_count = fragments - 1;
-    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
-    k_clear(r);
-    r.dir = 1; k_change(r, 0, lsa, fs * i); r.start = (int16_t)(fs * i); r.end = (int16_t)(fs * (i + 1) - 1);
-    k_exact(c, c.F, 1, r, &ra);
-    if (anc.list) KADD(ra, anc);
-    r.end = (int16_t)(nW - 1);
-    if (r.k <= r.l) {
-      k_calc_d(c, r, false);
-      KADD(r, prev);
-      while (err_count > 0) {
-        if (k_partials(c, maxe, c.F, 1, prev, next, prev_i, fs, half - 1)) { flow = false; break; }
-        k_turn(c, c.F, c.B, prev_i, false, 0);
-        if (k_partials(c, maxe, c.B, 0, prev_i, next_i, fin, fs, half - 1)) { flow = false; break; }
-        if (err_count == 1) break;
-        if (k_branches<false>(c, c.F, 1, next, prev, 0, 0)) { flow = false; break; }
-        if (k_branches<false>(c, c.B, 0, next_i, prev_i, 0, 0)) { flow = false; break; }
-        err_count--;
-      }
-      if (flow) {
-        k_branches<true>(c, c.F, 1, next, prev_i, fs, half - 1);
-        k_turn(c, c.F, c.B, prev_i, false, 0);
-        k_finals(c, c.B, 0, prev_i, fin, fs, half - 1);
-        k_branches<true>(c, c.B, 0, next_i, fin, fs, half - 1);
-      }
-    }
-  }
-
-  {
-    bool flow = true;
-    int32_t err_count = fragments - 1;
-    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
-    k_clear(r);
-    r.dir = 1; k_change(r, 0, lsa, 0); r.start = 0; r.end = (int16_t)(fs - 1);
-    k_exact(c, c.F, 1, r, &ra);
-    if (anc.list) KADD(ra, anc);
-    r.end = (int16_t)(nW - 1);
-    if (r.k <= r.l) {
-      k_calc_d(c, r, false);
-      KADD(r, prev);
-      while (err_count > 0) {
-        if (k_partials(c, maxe, c.F, 1, prev, next, fin, fs, half - 1)) { flow = false; break; }
-        if (err_count == 1) break;
-        if (k_branches<false>(c, c.F, 1, next, prev, 0, 0)) { flow = false; break; }
-        err_count--;
-      }
-      if (flow) k_branches<true>(c, c.F, 1, next, fin, fs, half - 1);
-    }
-  }
-
-  for (int32_t i = half; i < fragments - 1; ++i) {
-    bool flow = true;
-    int32_t err_count = fragments - 1;
-    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
-    k_clear(r);
-    r.dir = 0; k_change(r, 0, lsa, fs * (i + 1) - 1); r.start = (int16_t)(fs * i); r.end = (int16_t)(fs * (i + 1) - 1);
-    k_exact(c, c.B, 0, r, &ra);
-    if (anc.list) KADD(ra, anc);
-    r.start = 0;
-    if (r.k <= r.l) {
-      k_calc_d(c, r, true);
-      KADD(r, prev);
-      while (err_count > 0) {
-        if (k_partials(c, maxe, c.B, 0, prev, next, prev_i, fs, 0)) { flow = false; break; }
-        k_turn(c, c.B, c.F, prev_i, true, nW - 1);
-        if (k_partials(c, maxe, c.F, 1, prev_i, next_i, fin, fs, 0)) { flow = false; break; }
-        if (err_count == 1) break;
-        if (k_branches<false>(c, c.B, 0, next, prev, 0, 0)) { flow = false; break; }
-        if (k_branches<false>(c, c.F, 1, next_i, prev_i, 0, 0)) { flow = false; break; }
-        err_count--;
-      }
-      if (flow) {
-        k_branches<true>(c, c.B, 0, next, prev_i, fs, 0);
-        k_turn(c, c.B, c.F, prev_i, true, nW - 1);
-        k_finals(c, c.F, 1, prev_i, fin, fs, 0);
-        k_branches<true>(c, c.F, 1, next_i, fin, fs, 0);
-      }
-    }
-  }
-
-  {
-    bool flow = true;
-    int32_t err_count = fragments - 1;
-    prev.n = 0; prev_i.n = 0; next.n = 0; next_i.n = 0;
-    k_clear(r);
-    r.dir = 0; k_change(r, 0, lsa, nW - 1); r.start = (int16_t)(fs * (fragments - 1)); r.end = (int16_t)(nW - 1);
-    k_exact(c, c.B, 0, r, &ra);
-    if (anc.list) KADD(ra, anc);
-    r.start = 0;
-    if (r.k <= r.l) {
-      k_calc_d(c, r, true);
-      KADD(r, prev);
-      while (err_count > 0) {
-        if (k_partials(c, maxe, c.B, 0, prev, next, fin, fs, 0)) { flow = false; break; }
-        if (err_count == 1) break;
-        if (k_branches<false>(c, c.B, 0, next, prev, 0, 0)) { flow = false; break; }
-        err_count--;
-      }
-      if (flow) k_branches<true>(c, c.B, 0, next, fin, fs, 0);
-    }
+  // passes in the reference's order: fragments half-1 .. 1 forward, then
+  // fragment 0, then fragments half .. fragments-2 backward, then the last
+  for (int32_t b = 0; b < fragments; ++b) {
+    const bool fwd = b < half;
+    const int32_t i = b < half - 1 ? half - 1 - b : (b == half - 1 ? 0 : b);
+    if (k_pass(c, prev, next, prev_i, next_i, fin, anc, fs, fragments, half, fwd, i)) return true;
   }
   return false;
 }
@@ -686,7 +648,7 @@ struct KParams {
   uint32_t ntasks, list_cap, anchor_cap, strand_mask;
 };
 
-__global__ __launch_bounds__(256) void search_k_kernel(KParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void search_k_kernel(KParams P) {
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   fmi_hitk *sl = P.scratch + lane * 4ull * P.list_cap;
   while (true) {
