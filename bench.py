@@ -432,16 +432,18 @@ def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000
 
 # Measured HBM traffic of one timed launch, from the committed rocprofv3
 # FETCH_SIZE / WRITE_SIZE summaries (scaled by the copy-kernel calibration).
+# (preset, reads, read length, text mode, k-mer table k) -> calibrated
+# FETCH + WRITE summary of that search1 launch under profiles/
 TRAFFIC_FILES = {
-    ("c2", 1_000_000, 150, False): "search1_mismatch_64Mbp_1Mx150bp_pmc_traffic.json",
-    ("t3g", 2_000_000, 150, False): "search1_mismatch_3100Mbp_2Mx150bp_pmc_traffic.json",
-    ("c2", 1_000_000, 150, True): "search1_textmode_64Mbp_1Mx150bp_pmc_traffic.json",
-    ("t3g", 2_000_000, 150, True): "search1_textmode_3100Mbp_2Mx150bp_pmc_traffic.json",
+    ("c2", 1_000_000, 150, False, 0): "search1_mismatch_64Mbp_1Mx150bp_pmc_traffic.json",
+    ("t3g", 2_000_000, 150, False, 0): "search1_mismatch_3100Mbp_2Mx150bp_pmc_traffic.json",
+    ("c2", 1_000_000, 150, True, 0): "search1_textmode_64Mbp_1Mx150bp_pmc_traffic.json",
+    ("t3g", 2_000_000, 150, True, 0): "search1_textmode_3100Mbp_2Mx150bp_pmc_traffic.json",
 }
 
 
-def measured_traffic(preset, reads, read_len, text_mode):
-    name = TRAFFIC_FILES.get((preset, reads, read_len, text_mode))
+def measured_traffic(preset, reads, read_len, text_mode, kmer):
+    name = TRAFFIC_FILES.get((preset, reads, read_len, text_mode, kmer))
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
         return None
@@ -461,6 +463,8 @@ def main():
     ap.add_argument("--sub-rate", type=float, default=0.01)
     ap.add_argument("--sa-ratio", type=int, default=16)
     ap.add_argument("--cap", type=int, default=32)
+    ap.add_argument("--kmer", type=int, default=12,
+                    help="k of the k-mer interval table for the 1-error search (0: none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reads", type=int, default=0,
                     help="CPU baseline sample size (0: all reads, capped by time)")
@@ -566,11 +570,17 @@ def main():
         log(f"index expanded (full SA / ISA of both directions + 2-bit text) in "
             f"{time.time() - t0:.2f}s, {ix.device_bytes / 1e9:.2f} GB on device")
 
+    if args.kmer:
+        t0 = time.time()
+        ix.kmer_table(args.kmer)
+        log(f"k-mer interval table (k = {args.kmer}) built in {time.time() - t0:.2f}s, "
+            f"{ix.device_bytes / 1e9:.2f} GB on device")
+
     # algorithmic bytes of one launch: distinct 64-byte occurrence blocks per
     # LF query, 64 bytes per SA / ISA entry read, 8 bytes (two text words)
-    # per text run of up to 16 bases
-    blocks, lf_steps, lookups, tsteps = hpgfm.search1_count_work(ix, batch, cap=args.cap,
-                                                                   mode=mode)
+    # per text run of up to 16 bases, 8 bytes per k-mer table entry
+    blocks, lf_steps, lookups, tsteps, kmers = hpgfm.search1_count_work(ix, batch, cap=args.cap,
+                                                                          mode=mode)
 
     tuning = None
     if args.tuning:
@@ -599,7 +609,7 @@ def main():
     reads_per_s = args.reads / (wall / args.steps)
     counts = out.counts().reshape(-1, 2)
     stored = int(np.minimum(counts, args.cap).sum())
-    alg_bytes = (blocks * 64 + lookups * 64 + tsteps * 8 + int(offs[-1]) +
+    alg_bytes = (blocks * 64 + lookups * 64 + tsteps * 8 + kmers * 8 + int(offs[-1]) +
                  8 * (args.reads + 1) + stored * 32 + 4 * 2 * args.reads)
     kernel_ms = tr.mean_ms
     achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
@@ -781,13 +791,14 @@ def main():
             "preset": args.preset,
             "genome_bp": n_bp, "reads": args.reads, "read_len": args.read_len,
             "mode": "mismatch", "cap": args.cap, "sa_ratio": args.sa_ratio,
+            "kmer_table_k": args.kmer, "text_mode": not args.no_expand,
             "parallelism": "single GPU, persistent work-queue launch",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": measured_traffic(args.preset, args.reads, args.read_len,
-                                        not args.no_expand),
+                                        not args.no_expand, args.kmer),
         },
         "cpu_baseline": cpu,
         "kernel": "fmi::search1_kernel<true, true, false>",
@@ -799,6 +810,7 @@ def main():
         "text_mode": not args.no_expand,
         "sa_isa_reads_per_launch": lookups,
         "text_runs_per_launch": tsteps,
+        "kmer_lookups_per_launch": kmers,
         "block_mode": block_mode,
         "index_device_bytes": ix.device_bytes,
         "index_cache_resident": ix.device_bytes < 256 * 2 ** 20,

@@ -292,6 +292,7 @@ void fmi_index_free(fmi_index *ix) {
     (void)hipSetDevice(ix->device);
     (void)hipFree(ix->d_mem);
     (void)hipFree(ix->d_full);
+    (void)hipFree(ix->d_kmer);
     (void)hipFree(ix->d_koffset);
     (void)hipFree(ix->d_kstart);
     if (cur >= 0) (void)hipSetDevice(cur);
@@ -307,7 +308,7 @@ uint64_t fmi_index_device_bytes(const fmi_index *ix) {
   uint64_t b = 0;
   for (int d = 0; d < 2; ++d)
     b += ix->h[d].nblocks * sizeof(fmi::OccBlock) + 2 * ((ix->h[d].nsamp * 4 + 255) & ~255ull);
-  return b + ix->full_bytes;
+  return b + ix->full_bytes + ix->kmer_bytes;
 }
 
 int fmi_index_expand(fmi_index *ix) {
@@ -345,6 +346,51 @@ int fmi_index_expand(fmi_index *ix) {
   ix->dev.text = text;
   ix->d_full = mem;
   ix->full_bytes = bytes;
+  return FMI_OK;
+}
+
+int fmi_index_kmer_table(fmi_index *ix, uint32_t k) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (k != 0 && (k < 4 || k > 13)) return fail(FMI_EINVAL, "k must be 0 or 4..13");
+  if (ix->d_kmer) {
+    HIP_TRY(hipDeviceSynchronize(), "drain before freeing the table");
+    (void)hipFree(ix->d_kmer);
+    ix->d_kmer = nullptr;
+    ix->kmer_bytes = 0;
+    ix->dev.kmer[0] = ix->dev.kmer[1] = nullptr;
+    ix->dev.kmer_k = 0;
+  }
+  if (k == 0) return FMI_OK;
+  const uint64_t per = (1ull << (2 * k)) * sizeof(uint64_t);
+  void *mem = nullptr;
+  hipError_t e = hipMalloc(&mem, 2 * per);
+  if (e != hipSuccess) return fail(FMI_ENOMEM, "hipMalloc(k-mer table, " + std::to_string(2 * per) + " bytes)");
+  uint64_t *t[2] = {static_cast<uint64_t *>(mem),
+                    reinterpret_cast<uint64_t *>(static_cast<char *>(mem) + per)};
+  for (int d = 0; d < 2 && e == hipSuccess; ++d)
+    e = fmi::launch_kmer_table(ix->dev.d[d], (uint32_t)ix->n, k, t[d], nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(mem);
+    return hip_fail(e, "k-mer table launch");
+  }
+  ix->d_kmer = mem;
+  ix->kmer_bytes = 2 * per;
+  ix->dev.kmer[0] = t[0];
+  ix->dev.kmer[1] = t[1];
+  ix->dev.kmer_k = k;
+  return FMI_OK;
+}
+
+int fmi_index_get_kmer_table(const fmi_index *ix, int forward, uint64_t *out) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (!out) return fail(FMI_EINVAL, "NULL argument");
+  if (!ix->d_kmer) return fail(FMI_ESTATE, "call fmi_index_kmer_table first");
+  const uint64_t per = ix->kmer_bytes / 2;
+  HIP_TRY(hipMemcpy(out, ix->dev.kmer[forward ? 1 : 0], per, hipMemcpyDeviceToHost),
+          "copy k-mer table");
   return FMI_OK;
 }
 
@@ -957,7 +1003,7 @@ int fmi_search1_count_work(const fmi_index *ix, int mode, int strand_mask,
   if (!work) return fail(FMI_EINVAL, "NULL argument");
   if (mode != FMI_MODE_EDIT && mode != FMI_MODE_MISMATCH) return fail(FMI_EINVAL, "bad mode");
   if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "bad strand_mask");
-  for (int i = 0; i < 4; ++i) work[i] = 0;
+  for (int i = 0; i < 5; ++i) work[i] = 0;
   if (nreads == 0) return FMI_OK;
   if (!d_reads || !d_offsets) return fail(FMI_EINVAL, "NULL buffer");
   hipStream_t st = nullptr;
@@ -970,17 +1016,17 @@ int fmi_search1_count_work(const fmi_index *ix, int mode, int strand_mask,
   rc = cu_count(&cus);
   if (rc) { (void)hipFree(counts); return rc; }
   HIP_TRY(fmi::launch_zero(w.queue, 1, st), "zero queue");
-  HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 8, st), "zero stats");
+  HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 10, st), "zero stats");
   hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, true, true,
                                      (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap, nullptr,
                                      counts, w.queue, w.stats, (uint32_t)cus * kDefaultWgPerCu,
                                      kDefaultChunk, kDefaultRefill, st);
   if (e != hipSuccess) { (void)hipFree(counts); return hip_fail(e, "count launch"); }
-  unsigned long long st4[4] = {0, 0, 0, 0};
-  e = hipMemcpy(st4, w.stats, 32, hipMemcpyDeviceToHost);
+  unsigned long long st5[5] = {0, 0, 0, 0, 0};
+  e = hipMemcpy(st5, w.stats, 40, hipMemcpyDeviceToHost);
   (void)hipFree(counts);
   if (e != hipSuccess) return hip_fail(e, "copy stats");
-  for (int i = 0; i < 4; ++i) work[i] = st4[i];
+  for (int i = 0; i < 5; ++i) work[i] = st5[i];
   return FMI_OK;
 }
 
@@ -988,7 +1034,7 @@ int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
                              const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
                              uint64_t *blocks, uint64_t *lf_steps) {
   if (!blocks || !lf_steps) return fail(FMI_EINVAL, "NULL argument");
-  uint64_t work[4];
+  uint64_t work[5];
   int rc = fmi_search1_count_work(ix, mode, strand_mask, d_reads, d_offsets, nreads, 0, work);
   *blocks = work[0];
   *lf_steps = work[1];

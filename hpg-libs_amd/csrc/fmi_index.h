@@ -26,6 +26,8 @@ struct fmi_index {
   uint64_t device_bytes = 0;
   void *d_full = nullptr;         // full SA / ISA of both directions + 2-bit text
   uint64_t full_bytes = 0;
+  void *d_kmer = nullptr;         // k-mer interval tables of both directions
+  uint64_t kmer_bytes = 0;
   fmi::DevIndex dev{};
 
   ~fmi_index();

@@ -55,6 +55,11 @@ struct DevIndex {
                          // bits 2(j%16)..+1), or NULL; set with the full SA / ISA
   uint32_t n;            // text length; rows are 0..n
   uint32_t ratio;        // SA / ISA sample rate
+  // Interval (k | l << 32) of every string of kmer_k symbols per direction
+  // (index: symbol j fed to LF at bits 2j..2j+1), or NULL; kmer_k = 0
+  // without a table.
+  const uint64_t *kmer[2];
+  uint32_t kmer_k;
 };
 
 // Host-side view of one direction.

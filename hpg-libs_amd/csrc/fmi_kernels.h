@@ -42,6 +42,7 @@ hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, co
 hipError_t launch_expand(const DevDir &D, uint32_t n, uint32_t ratio, uint32_t *sa, uint32_t *isa,
                          hipStream_t s);
 // X packed 2 bits per base ((n + 15) / 16 words) from the backward direction.
+hipError_t launch_kmer_table(const DevDir &D, uint32_t n, uint32_t k, uint64_t *out, hipStream_t s);
 hipError_t launch_pack_text(const DevDir &D, uint32_t n, const uint32_t *isa, uint32_t *text,
                             hipStream_t s);
 hipError_t launch_isa(const DevIndex &ix, uint32_t dsel, const uint64_t *tpos, uint64_t count,

@@ -8,6 +8,7 @@
 // bounds, so a 4-way branch costs the same fetch as a single step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "../../include/fmi.h"
@@ -157,7 +158,8 @@ struct S1Params {
   uint32_t *counts;
   uint32_t *queue;                 // work counter (persistent launch)
   unsigned long long *stats;       // count mode: [0] blocks, [1] LF queries,
-                                   // [2] SA / ISA reads, [3] text steps
+                                   // [2] SA / ISA reads, [3] text steps,
+                                   // [4] k-mer table lookups
   uint32_t ntasks, strand_mask, cap, chunk;
   uint32_t refill_min;             // idle lanes before a wave takes new tasks
 };
@@ -172,7 +174,7 @@ struct S1Lane {
   // Text mode (expanded index): a single-row interval is held as the text
   // position q of its suffix in the pass's direction; kTqNone = rows in k, l.
   uint32_t tq, tqn, cq;            // node, child of W[i], continuation
-  uint32_t nlook, ntext;           // SA / ISA reads and text steps (count mode)
+  uint32_t nlook, ntext, nkmer;    // SA / ISA reads, text steps, table lookups (count mode)
 };
 
 constexpr uint32_t kTqNone = 0xFFFFFFFFu;
@@ -403,14 +405,37 @@ struct S1 {
     branch_next();
   }
 
-  // One LF query (or one text step, or the SA read that enters text mode)
-  // and the transition it drives.
+  // One LF query (or one text step, or the SA read that enters text mode,
+  // or one k-mer table lookup) and the transition it drives.
   __device__ __forceinline__ void advance(unsigned long long &nblk_acc, unsigned long long &nq_acc) {
     const bool cont = L.phase == PH_CONT;
     const uint32_t tq = cont ? L.cq : L.tq;
     if (tq != kTqNone) { text_step(cont, tq); return; }
     const uint32_t ak = cont ? L.qk : L.k;
     const uint32_t al = cont ? L.ql : L.l;
+    uint32_t rk, rl;
+    int32_t adv = 1;
+    // (0, n) in EXACT is the phase's start (every LF step gives k >= 1): its
+    // first kmer_k steps are one table entry, the interval of those read
+    // bases (complemented on the - strand, as the LF steps take them); an N
+    // among them empties the interval as its LF step would.
+    if (L.phase == PH_EXACT && P.ix.kmer_k && ak == 0u && al == P.ix.n &&
+        (L.pass ? L.len / 2 - L.i : L.i - L.len / 2 + 1) >= (int32_t)P.ix.kmer_k) {
+      const uint32_t K = P.ix.kmer_k;
+      const int32_t d = step();
+      uint32_t code = 0u, bad = 0u;
+      for (uint32_t j = 0; j < K; ++j) {
+        const uint32_t s = L.W[L.i + (int32_t)j * d];
+        bad |= s > 3u ? 1u : 0u;
+        code |= (s & 3u) << (2u * j);
+      }
+      if (L.comp) code ^= (1u << (2u * K)) - 1u;
+      if (COUNT) L.nkmer++;
+      const uint64_t e = bad ? 1ull : (L.dsel ? P.ix.kmer[1] : P.ix.kmer[0])[code];
+      rk = (uint32_t)e;
+      rl = (uint32_t)(e >> 32);
+      adv = (int32_t)K;
+    } else {
     if (ak == al && P.ix.text != nullptr) {
       // a single row: continue on the text from its suffix
       if (COUNT) L.nlook++;
@@ -476,7 +501,6 @@ struct S1 {
       return;
     }
 
-    uint32_t rk, rl;
     if (sym < 4u) {
       const uint32_t s = L.comp ? 3u - sym : sym;
       const uint32_t c = c_of(s);
@@ -487,10 +511,11 @@ struct S1 {
       rk = P.ix.n + 1;
       rl = P.ix.n;
     }
+    }
     if (!cont) {                     // PH_EXACT
       if (rk > rl) { end_pass(); return; }
       L.k = rk; L.l = rl;
-      L.i += step();
+      L.i += adv * step();
       if (in_read(L.i) && in_exact(L.i)) return;
       enter_scan();
       return;
@@ -519,6 +544,7 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
   L.phase = PH_IDLE;
   L.nlook = 0;
   L.ntext = 0;
+  L.nkmer = 0;
   S1<MM_ONLY, COUNT> m{P, L};
   unsigned long long nblk = 0, nq = 0;
   const uint32_t lane = lane_id();
@@ -569,6 +595,7 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
     atomicAdd(&P.stats[1], nq);
     atomicAdd(&P.stats[2], (unsigned long long)L.nlook);
     atomicAdd(&P.stats[3], (unsigned long long)L.ntext);
+    atomicAdd(&P.stats[4], (unsigned long long)L.nkmer);
   }
 }
 
@@ -614,6 +641,27 @@ __global__ __launch_bounds__(256) void vhelper_kernel(VhParams P) {
   uint2 *vb = P.scratch + 2 * (off + 3ull * rd) + (uint64_t)strand * (uint64_t)(len + 3);
   P.counts[t] = vh_task(P.ix, W, start, end, strand, P.mm_only, vb,
                         P.hits + (uint64_t)t * P.cap, P.cap);
+}
+
+// --------------------------------------------------------------------------
+// k-mer interval table
+// --------------------------------------------------------------------------
+// The interval of every string of k symbols fed to LF from the whole index
+// (symbol j at bits 2j..2j+1 of the entry's index), so an exact search can
+// take its first k steps in one read.  Entry = k | l << 32; an empty
+// interval is stored as (1, 0).
+
+__global__ __launch_bounds__(256) void kmer_table_kernel(DevDir D, uint32_t n, uint32_t k,
+                                                         uint64_t count, uint64_t *out) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < count;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0u, hi = n, nb;
+    for (uint32_t j = 0; j < k; ++j) {
+      lf_one(D, lo, hi, (uint32_t)(c >> (2u * j)) & 3u, lo, hi, nb);
+      if (lo > hi) break;
+    }
+    out[c] = lo <= hi ? ((uint64_t)hi << 32 | lo) : 1ull;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -1007,6 +1055,13 @@ hipError_t launch_expand(const DevDir &D, uint32_t n, uint32_t ratio, uint32_t *
   const uint32_t nsamp = n / ratio + 1;
   const uint32_t grid = (nsamp + 1 + 255) / 256;
   hipLaunchKernelGGL(expand_walk_kernel, dim3(grid), dim3(256), 0, s, D, n, ratio, nsamp, sa, isa);
+  return hipGetLastError();
+}
+
+hipError_t launch_kmer_table(const DevDir &D, uint32_t n, uint32_t k, uint64_t *out, hipStream_t s) {
+  const uint64_t count = 1ull << (2 * k);
+  const uint64_t grid = std::min<uint64_t>((count + 255) / 256, 65536ull);
+  hipLaunchKernelGGL(kmer_table_kernel, dim3((uint32_t)grid), dim3(256), 0, s, D, n, k, count, out);
   return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ EXPORTS = [
     "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
-    "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_get_full",
+    "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_kmer_table", "fmi_index_get_kmer_table",
+    "fmi_index_get_full",
     "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
@@ -132,6 +133,8 @@ def _bind(lib):
         "fmi_copy_bench": ([_p, _p, _u64, C.POINTER(C.c_float), _p], _i),
         "fmi_search1_vector_batch": ([_p, _i, _i, _p, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_index_expand": ([_p], _i),
+        "fmi_index_kmer_table": ([_p, _u32], _i),
+        "fmi_index_get_kmer_table": ([_p, _i, _p], _i),
         "fmi_search1_count_work": ([_p, _i, _i, _p, _p, _u32, _u32, _p], _i),
         "fmi_search_k_batch": ([_p, _i, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p, _p, _p, _p],
                                _i),
@@ -380,6 +383,20 @@ class Index:
         check(lib().fmi_index_expand(self.h))
         return self
 
+    def kmer_table(self, k):
+        """fmi_index_kmer_table: the interval of every k-mer of both
+        directions on the device (k = 0 frees it)."""
+        check(lib().fmi_index_kmer_table(self.h, k))
+        self._kmer_k = k
+        return self
+
+    def kmer_entries(self, forward=0):
+        """One direction's k-mer table on the host (uint64 k | l << 32)."""
+        k = self._kmer_k
+        out = np.empty(4 ** k, np.uint64)
+        check(lib().fmi_index_get_kmer_table(self.h, forward, _ptr(out)))
+        return out
+
     def full_arrays(self, forward=0):
         """(sa, isa, text words) of one direction after expand()."""
         n1 = self.n + 1
@@ -562,8 +579,9 @@ class PlainSearch:
 
 
 def search1_count_work(ix, batch, cap=32, mode=MODE_MISMATCH, strand_mask=3):
-    """(blocks, LF queries, SA/ISA reads, text steps) of one fmi_search1_batch."""
-    w = np.zeros(4, np.uint64)
+    """(blocks, LF queries, SA/ISA reads, text steps, k-mer table lookups) of
+    one fmi_search1_batch."""
+    w = np.zeros(5, np.uint64)
     check(lib().fmi_search1_count_work(ix.h, mode, strand_mask, batch.d_reads.ptr,
                                        batch.d_offsets.ptr, batch.nreads, cap, _ptr(w)))
     return tuple(int(x) for x in w)

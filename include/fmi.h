@@ -139,6 +139,18 @@ void fmi_index_free(fmi_index *ix);
  * record) instead of one occurrence-block load per base; the records are
  * the same.  No reference counterpart (it trades HBM for dependent loads). */
 int fmi_index_expand(fmi_index *ix);
+/* A table of the interval of every string of k bases in each direction
+ * (4^k entries of 8 bytes per direction: k = 11 takes 64 MiB, k = 12 256
+ * MiB), built on the device from the occurrence blocks.  fmi_search1_batch
+ * then takes the first k LF steps of each exact phase in one read of the
+ * table; the records are the same.  k = 0 frees the table; otherwise k is
+ * 4..13.  No reference counterpart (it trades HBM for the widest, most
+ * expensive LF steps of every search). */
+int fmi_index_kmer_table(fmi_index *ix, uint32_t k);
+/* Copies one direction's table (4^k entries, k | l << 32, an empty
+ * interval as (1, 0); entry index: base j of the string at bits 2j..2j+1,
+ * in the order LF takes them) to the host; FMI_ESTATE without a table. */
+int fmi_index_get_kmer_table(const fmi_index *ix, int forward, uint64_t *out);
 /* Copies the expanded arrays of one direction (NULL skips one): sa, isa of
  * n + 1 entries; text (X, 16 bases per word, base j in bits 2(j%16)) of
  * (n + 15) / 16 words. */
@@ -531,10 +543,11 @@ int fmi_copy_bench(const void *d_src, void *d_dst, uint64_t bytes, float *ms, fm
 int fmi_search1_count_blocks(const fmi_index *ix, int mode, int strand_mask,
                              const uint8_t *d_reads, const uint64_t *d_offsets,
                              uint32_t nreads, uint64_t *blocks, uint64_t *lf_steps);
-/* The same launch's whole work at capacity cap: work[0] occurrence blocks,
- * work[1] LF queries, work[2] SA / ISA entries read (text mode, including
- * the ISA read of every stored record), work[3] text runs (each compares
- * up to 16 read bases with two 32-bit text words). */
+/* The same launch's whole work at capacity cap (work holds 5 counters):
+ * work[0] occurrence blocks, work[1] LF queries, work[2] SA / ISA entries
+ * read (text mode, including the ISA read of every stored record), work[3]
+ * text runs (each compares up to 16 read bases with two 32-bit text words),
+ * work[4] k-mer table lookups (8 bytes each). */
 int fmi_search1_count_work(const fmi_index *ix, int mode, int strand_mask,
                            const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
                            uint32_t cap, uint64_t *work);

@@ -1,0 +1,84 @@
+"""The k-mer interval table (fmi_index_kmer_table): every entry equals the
+oracle's exact search of that k-mer from the whole index, and the 1-error
+search that takes the first k steps of each exact phase from it keeps the
+golden records of the compiled reference core and the oracle's (N bases,
+reads too short for a lookup, both modes, compact and expanded index)."""
+import numpy as np
+import pytest
+
+import fmoracle as fo
+from conftest import split_golden_hits
+from hpgfm import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=["golden_small", "golden_repeats"])
+def kgold(request, gpu_engine):
+    g = request.getfixturevalue(request.param)
+    ix = gpu_engine.Index.build(g["genome"], int(g["ratio"])).to_device()
+    batch = gpu_engine.ReadBatch(g["reads"], g["offsets"])
+    yield gpu_engine, g, ix, batch
+    batch.free()
+    ix.close()
+
+
+@pytest.mark.parametrize("expanded", [False, True])
+@pytest.mark.parametrize("k", [4, 7])
+def test_kmer_golden_search1(kgold, k, expanded):
+    hp, g, ix, batch = kgold
+    if expanded:
+        ix.expand()
+    ix.kmer_table(k)
+    want = split_golden_hits(g)
+    cap = int(g["cap"])
+    counts, hits = hp.search1_batch(ix, batch, cap=cap, mode=hp.MODE_EDIT)
+    assert np.array_equal(counts, g["s1_counts"])
+    for i in range(batch.nreads):
+        for s in (0, 1):
+            c = int(counts[i, s])
+            assert fo.hits_equal(hits[i, s, :c], want[(i, s)]), (k, i, s)
+    w = hp.search1_count_work(ix, batch, cap=cap)
+    assert w[4] > 0
+    ix.kmer_table(0)
+    w0 = hp.search1_count_work(ix, batch, cap=cap)
+    assert w0[4] == 0 and w0[1] > w[1]
+
+
+def test_kmer_entries_and_oracle_with_n(gpu_engine):
+    hp = gpu_engine
+    g, _, _ = synth.genome(150_000, seed=61, repeat_frac=0.3, fam_len=(200, 1500),
+                           divergence=(0.0, 0.04))
+    ix = hp.Index.build(g, 16).to_device()
+    o = fo.Oracle(g, ratio=16)
+    k = 6
+    ix.kmer_table(k)
+    tables = [ix.kmer_entries(0), ix.kmer_entries(1)]
+    assert len(tables[0]) == 4 ** k
+    # entries: the exact search of the k-mer in feed order, per direction
+    rng = np.random.default_rng(62)
+    lsa = o.size_sa - 1
+    for code in rng.integers(0, 4 ** k, 200):
+        syms = np.array([(int(code) >> (2 * j)) & 3 for j in range(k)], np.uint8)
+        # direction 0 (BWT of X) takes W[k-1], W[k-2], ... : W = reversed feed
+        r0 = o.exact(fo.VIEW_BACKWARD, 0, syms[::-1].copy(), 0, lsa, 0, k - 1, k - 1)
+        # direction 1 (BWT of reverse X) takes W[0], W[1], ...
+        r1 = o.exact(fo.VIEW_FORWARD, 1, syms, 0, lsa, 0, 0, k - 1)
+        for fwd, r in ((0, r0), (1, r1)):
+            e = int(tables[fwd][int(code)])
+            want = (int(r["k"]), int(r["l"])) if r["k"] <= r["l"] else (1, 0)
+            assert (e & 0xFFFFFFFF, e >> 32) == want, (fwd, int(code))
+    lens = rng.integers(1, 40, 500)
+    lens[:20] = rng.integers(1, 2 * k, 20)        # too short for a lookup
+    reads, offs, _ = synth.reads(g, 500, seed=63, sub_rate=0.02, n_rate=0.01, lengths=lens)
+    batch = hp.ReadBatch(reads, offs)
+    for mode, mm in ((hp.MODE_EDIT, False), (hp.MODE_MISMATCH, True)):
+        counts, hits = hp.search1_batch(ix, batch, cap=64, mode=mode)
+        for i in range(batch.nreads):
+            W = reads[offs[i]:offs[i + 1]]
+            for s in (0, 1):
+                n, want = o.search1(s, W, cap=64, mismatch_only=mm)
+                assert int(counts[i, s]) == n, (mode, i, s)
+                assert fo.hits_equal(hits[i, s, :min(n, 64)], want), (mode, i, s)
+    batch.free()
+    ix.close()
