@@ -32,10 +32,13 @@ struct ExactParams {
   fmi_hit *anchor;
 };
 
-// With an expanded index, a search whose interval has narrowed to one row
-// goes on against the text at that row's suffix, 16 bases at a time; where
-// it dies, one LF step from the last live row gives the reference's empty
-// interval exactly.
+// With a k-mer table the first k steps are one read of it when all k bases
+// are A/C/G/T and their interval is not empty (otherwise the steps are taken
+// one by one, so the position and interval where the search dies are the
+// reference's).  With an expanded index, a search whose interval has
+// narrowed to one row goes on against the text at that row's suffix, 16
+// bases at a time; where it dies, one LF step from the last live row gives
+// the reference's empty interval exactly.
 __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= P.nreads) return;
@@ -46,6 +49,24 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   const int32_t d = P.direction ? 1 : -1;
   uint32_t k = 0, l = 0, k2 = 0, l2 = n, nb;
   int32_t i = P.direction ? 0 : len - 1;
+  const uint32_t K = P.ix.kmer_k;
+  if (K && len >= (int32_t)K) {
+    uint32_t code = 0u, bad = 0u;
+    for (uint32_t j = 0; j < K; ++j) {
+      const uint32_t s = W[i + (int32_t)j * d];
+      bad |= s > 3u ? 1u : 0u;
+      code |= (s & 3u) << (2u * j);
+    }
+    if (P.comp) code ^= (1u << (2u * K)) - 1u;
+    if (!bad) {
+      const uint64_t e = P.ix.kmer[P.dsel][code];
+      if ((uint32_t)e <= (uint32_t)(e >> 32)) {
+        k = k2 = (uint32_t)e;
+        l = l2 = (uint32_t)(e >> 32);
+        i += (int32_t)K * d;
+      }
+    }
+  }
   for (; P.direction ? (i <= len - 1) : (i >= 0); i += d) {
     const uint32_t sym = W[i];
     if (sym > 3u) { k2 = n + 1; l2 = n; break; }

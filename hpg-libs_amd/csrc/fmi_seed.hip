@@ -77,17 +77,43 @@ __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
     const uint64_t s = (g - P.seed_base[r]) * P.stride;
     const uint8_t *W = P.reads + P.offsets[r] + s;
     uint32_t nb;
+    // The first k steps of each search from the k-mer table when its k bases
+    // are A/C/G/T and their interval is not empty (else step by step, so a
+    // dead interval is the one the LF steps give).
+    const uint32_t K = P.ix.kmer_k;
+    const bool use_k = K && P.seed_len >= K;
     // + strand: BWExactSearchBackward over W[0 .. seed_len-1] on BACKWARD
     uint32_t kp = 0, lp = n;
-    for (int32_t i = (int32_t)P.seed_len - 1; i >= 0; --i) {
+    int32_t ip = (int32_t)P.seed_len - 1;
+    // - strand: BWExactSearchForward on BACKWARD_REV (complemented symbols)
+    uint32_t km = 0, lm = n;
+    uint32_t im = 0;
+    if (use_k) {
+      uint32_t cp = 0u, cm = 0u, bad = 0u;
+      for (uint32_t j = 0; j < K; ++j) {
+        const uint32_t sp = W[ip - (int32_t)j], sm = W[j];
+        bad |= (sp > 3u || sm > 3u) ? 1u : 0u;
+        cp |= (sp & 3u) << (2u * j);
+        cm |= (sm & 3u) << (2u * j);
+      }
+      cm ^= (1u << (2u * K)) - 1u;
+      if (!bad) {
+        const uint64_t ep = P.ix.kmer[0][cp], em = P.ix.kmer[0][cm];
+        if ((uint32_t)ep <= (uint32_t)(ep >> 32)) {
+          kp = (uint32_t)ep; lp = (uint32_t)(ep >> 32); ip -= (int32_t)K;
+        }
+        if ((uint32_t)em <= (uint32_t)(em >> 32)) {
+          km = (uint32_t)em; lm = (uint32_t)(em >> 32); im = K;
+        }
+      }
+    }
+    for (int32_t i = ip; i >= 0; --i) {
       const uint32_t sym = W[i];
       if (sym > 3u) { kp = n + 1; lp = n; break; }
       lf_one(D, kp, lp, sym, kp, lp, nb);
       if (kp > lp) break;
     }
-    // - strand: BWExactSearchForward on BACKWARD_REV (complemented symbols)
-    uint32_t km = 0, lm = n;
-    for (uint32_t i = 0; i < P.seed_len; ++i) {
+    for (uint32_t i = im; i < P.seed_len; ++i) {
       const uint32_t sym = W[i];
       if (sym > 3u) { km = n + 1; lm = n; break; }
       lf_one(D, km, lm, 3u - sym, km, lm, nb);

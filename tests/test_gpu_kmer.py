@@ -1,8 +1,11 @@
 """The k-mer interval table (fmi_index_kmer_table): every entry equals the
-oracle's exact search of that k-mer from the whole index, and the 1-error
-search that takes the first k steps of each exact phase from it keeps the
-golden records of the compiled reference core and the oracle's (N bases,
-reads too short for a lookup, both modes, compact and expanded index)."""
+oracle's exact search of that k-mer from the whole index; the 1-error search
+that takes the first k steps of each exact phase from it keeps the golden
+records of the compiled reference core and the oracle's (N bases, reads too
+short for a lookup, both modes, compact and expanded index); the exact
+search keeps the golden intervals, stop positions and anchors (dead ones
+included) on all four views; the seed mapping gives the same regions and
+per-seed intervals with and without the table."""
 import numpy as np
 import pytest
 
@@ -80,5 +83,44 @@ def test_kmer_entries_and_oracle_with_n(gpu_engine):
                 n, want = o.search1(s, W, cap=64, mismatch_only=mm)
                 assert int(counts[i, s]) == n, (mode, i, s)
                 assert fo.hits_equal(hits[i, s, :min(n, 64)], want), (mode, i, s)
+    batch.free()
+    ix.close()
+
+
+@pytest.mark.parametrize("expanded", [False, True])
+def test_kmer_exact_golden(kgold, expanded):
+    hp, g, ix, batch = kgold
+    if expanded:
+        ix.expand()
+    for k in (4, 7):
+        ix.kmer_table(k)
+        for view in range(4):
+            for d in (0, 1):
+                rec, anc = hp.exact_batch(ix, batch, view, d, anchors=True)
+                want = g[f"ex_v{view}_d{d}"]
+                wa = g[f"ex_anchor_v{view}_d{d}"]
+                for key in ("k", "l", "pos"):
+                    assert np.array_equal(rec[key], want[key]), (k, view, d, key)
+                for key in ("k", "l", "start", "pos", "end", "dir"):
+                    assert np.array_equal(anc[key], wa[key]), (k, view, d, key)
+    ix.kmer_table(0)
+
+
+def test_kmer_seed_map_unchanged(gpu_engine):
+    hp = gpu_engine
+    g, _, _ = synth.genome(300_000, seed=66, repeat_frac=0.4, fam_len=(200, 2000),
+                           divergence=(0.0, 0.03))
+    ix = hp.Index.build(g, 16)
+    ix.set_karyotype([0, 0], [149_999, 149_999])
+    ix.to_device()
+    reads, offs, _ = synth.reads(g, 600, length=100, seed=67, sub_rate=0.02, n_rate=0.005)
+    batch = hp.ReadBatch(reads, offs)
+    base = hp.seed_map_batch(ix, batch, 18, 5, 1000)
+    for k in (6, 12):
+        ix.kmer_table(k)
+        got = hp.seed_map_batch(ix, batch, 18, 5, 1000)
+        assert got[0] == base[0] and got[1] == base[1]
+        assert got[2].tobytes() == base[2].tobytes(), k
+        assert got[3].tobytes() == base[3].tobytes(), k
     batch.free()
     ix.close()
