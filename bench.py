@@ -14,6 +14,7 @@ stdout: one JSON line.  stderr: progress.
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -201,7 +202,8 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
         "config": {"workload": f"mixed budgets, {args.genome_mbp:g} Mbp synthetic reference "
                                f"({pre['repeat_frac'] * 100:g}% repeats), {n} reads of "
                                f"50-250 bp (log-normal, median 120), 0.5% N, budgets 0/1/2 at "
-                               f"50/35/15%", "preset": args.preset, "list_cap": list_cap},
+                               f"50/35/15%", "preset": args.preset, "list_cap": list_cap,
+                   "kmer_table_k": args.kmer},
         "cpu_baseline": cpu, "ms_per_group": parts,
         "reads_per_group": [len(x[0]) for x in groups],
     }
@@ -290,7 +292,8 @@ def bench_exact(args, pre, g, ix, hpgfm, synth, read_len=100, cap=64):
         "config": {"workload": f"exact search + locate, {args.genome_mbp:g} Mbp synthetic "
                                f"reference (i.i.d., 41% GC), {n} reads of {read_len} bp (90% "
                                f"exact copies of either strand, 10% random), <= {cap} rows "
-                               f"per interval", "preset": args.preset},
+                               f"per interval", "preset": args.preset,
+                   "kmer_table_k": args.kmer},
         "kernel_ms_per_step": round(kern, 4), "cpu_baseline": cpu,
     }
     res.update(extras)
@@ -371,7 +374,7 @@ def bench_inexact_seeds(args, pre, g, ix, hpgfm, synth, seed_size=18, min_seed_s
                                f"{args.sub_rate * 100:g}% substitutions, windows of {seed_size} "
                                f"(remainder >= {min_seed_size}), {rec_cap} records and "
                                f"{region_cap} regions per window and strand",
-                   "preset": args.preset, "windows": nwin},
+                   "preset": args.preset, "windows": nwin, "kmer_table_k": args.kmer},
         "regions": kept, "windows_per_s": round(nwin / wall), "cpu_baseline": cpu,
     }
     if cpu:
@@ -421,6 +424,7 @@ def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000
                                f"{read_len} bp cut into {seed_len}-mers at stride {stride}, both "
                                f"orientations, <= {filt} positions per seed",
                    "preset": args.preset, "seed_len": seed_len, "stride": stride,
+                   "kmer_table_k": args.kmer,
                    "filter_seed_mappings": filt},
         "reads_per_s": round(args.reads / wall), "positions_per_s": round(nr / wall),
         "seeds": ns, "rows": nr, "regions": valid,
@@ -439,6 +443,7 @@ TRAFFIC_FILES = {
     ("t3g", 2_000_000, 150, False, 0): "search1_mismatch_3100Mbp_2Mx150bp_pmc_traffic.json",
     ("c2", 1_000_000, 150, True, 0): "search1_textmode_64Mbp_1Mx150bp_pmc_traffic.json",
     ("t3g", 2_000_000, 150, True, 0): "search1_textmode_3100Mbp_2Mx150bp_pmc_traffic.json",
+    ("c2", 1_000_000, 150, True, 12): "search1_textmode_kmer12_64Mbp_1Mx150bp_pmc_traffic.json",
 }
 
 
@@ -463,8 +468,9 @@ def main():
     ap.add_argument("--sub-rate", type=float, default=0.01)
     ap.add_argument("--sa-ratio", type=int, default=16)
     ap.add_argument("--cap", type=int, default=32)
-    ap.add_argument("--kmer", type=int, default=12,
-                    help="k of the k-mer interval table for the 1-error search (0: none)")
+    ap.add_argument("--kmer", type=int, default=-1,
+                    help="k of the k-mer interval table (0: none; default: floor(log4 n), "
+                         "4..13)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reads", type=int, default=0,
                     help="CPU baseline sample size (0: all reads, capped by time)")
@@ -512,6 +518,8 @@ def main():
     if args.reads is None:
         args.reads = pre["reads"]
     n_bp = int(args.genome_mbp * 1_000_000)
+    if args.kmer < 0:
+        args.kmer = min(13, max(4, int(math.log(max(n_bp, 4), 4))))
     t0 = time.time()
     g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=pre["repeat_frac"],
                            fam_len=(300, 6000), divergence=pre["div"], n_frac=pre["n_frac"])
@@ -520,25 +528,18 @@ def main():
     ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
     log(f"index built on the GPU in {time.time() - t0:.1f}s "
         f"({ix.device_bytes / 1e6:.1f} MB on device)")
-    if args.workload == "inexact-seeds":
-        if not args.no_expand:
-            ix.expand()
-        bench_inexact_seeds(args, pre, g, ix, hpgfm, synth)
-        return
-    if args.workload == "exact":
-        ix.expand()
-        bench_exact(args, pre, g, ix, hpgfm, synth)
-        return
-    if args.workload == "mixed":
-        ix.expand()
-        bench_mixed(args, pre, g, ix, hpgfm, synth)
-        return
-    if args.workload == "seeds":
-        if not args.no_expand:
+    if args.workload != "search1":
+        if args.workload in ("exact", "mixed") or not args.no_expand:
             t0 = time.time()
             ix.expand()
             log(f"index expanded in {time.time() - t0:.2f}s ({ix.device_bytes / 1e9:.2f} GB)")
-        bench_seeds(args, pre, g, ix, hpgfm, synth)
+        if args.kmer:
+            t0 = time.time()
+            ix.kmer_table(args.kmer)
+            log(f"k-mer interval table (k = {args.kmer}) built in {time.time() - t0:.2f}s")
+        run = {"inexact-seeds": bench_inexact_seeds, "exact": bench_exact, "mixed": bench_mixed,
+               "seeds": bench_seeds}[args.workload]
+        run(args, pre, g, ix, hpgfm, synth)
         return
 
     t0 = time.time()
