@@ -577,9 +577,10 @@ def main():
         log(f"k-mer interval table (k = {args.kmer}) built in {time.time() - t0:.2f}s, "
             f"{ix.device_bytes / 1e9:.2f} GB on device")
 
-    # algorithmic bytes of one launch: distinct 64-byte occurrence blocks per
-    # LF query, 64 bytes per SA / ISA entry read, 8 bytes (two text words)
-    # per text run of up to 16 bases, 8 bytes per k-mer table entry
+    # algorithmic bytes of one launch, in distinct 64-byte lines per random
+    # access: the occurrence blocks of each LF query, one line per SA / ISA
+    # entry read, per text run of up to 16 bases (two adjacent 32-bit words)
+    # and per k-mer table entry; plus the reads, offsets, records and counts
     blocks, lf_steps, lookups, tsteps, kmers = hpgfm.search1_count_work(ix, batch, cap=args.cap,
                                                                           mode=mode)
 
@@ -610,7 +611,7 @@ def main():
     reads_per_s = args.reads / (wall / args.steps)
     counts = out.counts().reshape(-1, 2)
     stored = int(np.minimum(counts, args.cap).sum())
-    alg_bytes = (blocks * 64 + lookups * 64 + tsteps * 8 + kmers * 8 + int(offs[-1]) +
+    alg_bytes = (blocks * 64 + lookups * 64 + tsteps * 64 + kmers * 64 + int(offs[-1]) +
                  8 * (args.reads + 1) + stored * 32 + 4 * 2 * args.reads)
     kernel_ms = tr.mean_ms
     achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
