@@ -34,6 +34,26 @@ __device__ __forceinline__ Blk load_block(const OccBlock *base, uint32_t j) {
   return B;
 }
 
+// The block held by four 16-byte loads of it.
+__device__ __forceinline__ Blk blk_of4(const uint4 &a, const uint4 &b, const uint4 &c,
+                                       const uint4 &d) {
+  Blk B;
+  B.c[0] = a.x; B.c[1] = a.y; B.c[2] = a.z; B.c[3] = a.w;
+  B.lo[0] = ((uint64_t)b.y << 32) | b.x;
+  B.lo[1] = ((uint64_t)b.w << 32) | b.z;
+  B.lo[2] = ((uint64_t)c.y << 32) | c.x;
+  B.hi[0] = ((uint64_t)c.w << 32) | c.z;
+  B.hi[1] = ((uint64_t)d.y << 32) | d.x;
+  B.hi[2] = ((uint64_t)d.w << 32) | d.z;
+  return B;
+}
+
+// Word i (0..3) of a 16-byte load.
+__device__ __forceinline__ uint32_t pick4u(const uint4 &v, uint32_t i) {
+  const uint32_t lo = (i & 1u) ? v.y : v.x, hi = (i & 1u) ? v.w : v.z;
+  return (i & 2u) ? hi : lo;
+}
+
 // Masks selecting the first t (0..192) rows of the block in each plane word.
 __device__ __forceinline__ void row_masks(uint32_t t, uint64_t &m0, uint64_t &m1, uint64_t &m2) {
   const uint32_t r = t & 63u, q = t >> 6;

@@ -284,6 +284,7 @@ struct S1 {
     const bool second = (pass ^ L.strand) != 0u;
     L.dsel = second ? 1u : 0u;
     L.tq = L.tqn = L.cq = kTqNone;
+    L.bi = 0u;                       // no k-mer entry index prepared
     L.comp = L.strand;
     L.k = 0;
     L.l = P.ix.n;
@@ -362,37 +363,58 @@ struct S1 {
     start_strand(strand);
   }
 
-  __device__ __forceinline__ uint32_t text_run(uint32_t q, int32_t p, uint32_t lim) const {
-    return text_match(P.ix, L.dsel, L.comp, q, L.W, p, step(), lim);
+  // The 16 text bases that precede suffix q in the pass's direction, in
+  // feed order as 2-bit groups (physical symbols), from the text words lo32
+  // and hi32 that fetch() read for q (hi32 = 0 past the last word).
+  __device__ __forceinline__ uint32_t text_window(uint32_t q, uint32_t lo32, uint32_t hi32) const {
+    if (L.dsel) {                    // X[x0], X[x0+1], ... with x0 = n - q
+      const uint32_t x0 = P.ix.n - q;
+      return (uint32_t)((((uint64_t)hi32 << 32) | lo32) >> ((x0 & 15u) << 1));
+    }
+    const int64_t b = (int64_t)q - 16;   // X[b .. b+15] = X[q-16 .. q-1]
+    uint32_t win;
+    if (b >= 0) win = (uint32_t)((((uint64_t)hi32 << 32) | lo32) >> (((uint32_t)b & 15u) << 1));
+    else win = lo32 << (2u * (uint32_t)(-b));
+    // reverse the 16 groups: bit reversal, then swap the bits of each pair
+    const uint32_t v = __builtin_bitreverse32(win);
+    return ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
   }
 
-  // One step on a text-mode interval: up to 16 read bases against the text
-  // that precedes the suffix (no occurrence block).  A single-row interval
-  // has one non-empty child, the one of the text base, so a SCAN node
-  // branches exactly where the read's base differs from the text; the
+  // First mismatch (0..lim) of the read window r2 (feed order, complemented
+  // on the - strand) against the text window t2; bad marks N bases (bit 2i).
+  __device__ __forceinline__ static uint32_t run_length(uint32_t t2, uint32_t r2, uint32_t bad,
+                                                        uint32_t lim) {
+    if (lim == 0u) return 0u;
+    const uint32_t x = r2 ^ t2;
+    uint32_t miss = ((x | (x >> 1)) & 0x55555555u) | bad;
+    if (lim < 16u) miss |= 1u << (2u * lim);
+    return miss ? (uint32_t)__builtin_ctz(miss) >> 1 : 16u;
+  }
+
+  // The transition of one text step on a text-mode interval (suffix q): m of
+  // lim read bases matched the text that precedes the suffix (no occurrence
+  // block); tsym is the text base before q (4 at q = 0).  A single-row
+  // interval has one non-empty child, the one of the text base, so a SCAN
+  // node branches exactly where the read's base differs from the text; the
   // matching positions before it are passed over at once.
-  __device__ __forceinline__ void text_step(bool cont, uint32_t q) {
+  __device__ __forceinline__ void text_apply(bool cont, uint32_t q, uint32_t lim, uint32_t m,
+                                             uint32_t r0, uint32_t tsym) {
     const int32_t p = cont ? L.cp : L.i;
     const int32_t d = step();
     if (COUNT) L.ntext++;
     if (L.phase == PH_SCAN) {
-      const uint32_t rem = (uint32_t)(d < 0 ? p + 1 : L.len - p);
-      const uint32_t lim = min(min(rem, 16u), q);
-      const uint32_t m = text_run(q, p, lim);
       if (m > 0u) {
         L.kn = 0; L.ln = 0; L.tqn = q - m;
         L.i += (int32_t)m * d;
         scan_next();
         return;
       }
-      const uint32_t sym = L.W[p];
-      const uint32_t tsym = q >= 1u ? text_sym(q) : 4u;
 #pragma unroll
       for (uint32_t s = 0; s < 4; ++s) {
         L.ck[s] = s == tsym ? 0u : 1u;
         L.cl[s] = 0u;
       }
-      L.wsym = sym;
+      L.wsym = r0;
       L.kn = P.ix.n + 1; L.ln = P.ix.n; L.tqn = kTqNone;
       L.bi = 0;
       branch_next();
@@ -401,8 +423,6 @@ struct S1 {
     uint32_t rem;
     if (!cont) rem = (uint32_t)(L.pass ? L.len / 2 - p : p - L.len / 2 + 1);
     else rem = (uint32_t)(d < 0 ? p + 1 : L.len - p);
-    const uint32_t lim = min(min(rem, 16u), q);
-    const uint32_t m = text_run(q, p, lim);
     const bool died = m < lim || (m == q && m < rem);
     if (!cont) {                     // PH_EXACT
       if (died) { end_pass(); return; }
@@ -426,61 +446,205 @@ struct S1 {
     branch_next();
   }
 
-  // One LF query (or one text step, or the SA read that enters text mode,
-  // or one k-mer table lookup) and the transition it drives.
+  // What one iteration of a lane reads.
+  enum : uint32_t { F_TEXT = 0, F_KPREP = 1, F_KLOOK = 2, F_SA = 3, F_LF = 4 };
+
+  // One iteration of the lane's state machine: one LF query, one text step,
+  // the SA read that enters text mode, or one k-mer table lookup (in two
+  // iterations: the read bases, then the entry), and the transition it
+  // drives.  Every lane first works out the addresses of what it reads and
+  // all lanes issue their loads before the first wait, so a wavefront whose
+  // lanes are in different kinds of step waits for one memory round trip per
+  // iteration, not one per kind; the transitions, ALU work, follow.
+  //
+  // The k-mer lookup: (0, n) in EXACT is the phase's start (every LF step
+  // gives k >= 1); its first kmer_k steps are one table entry, the interval
+  // of those read bases (complemented on the - strand, as the LF steps take
+  // them); an N among them empties the interval as its LF step would.  The
+  // prepared entry index is kept in bi (unused in EXACT) with kKReady set.
+  static constexpr uint32_t kKReady = 0x80000000u;
+
   __device__ __forceinline__ void advance(unsigned long long &nblk_acc, unsigned long long &nq_acc) {
     const bool cont = L.phase == PH_CONT;
     const uint32_t tq = cont ? L.cq : L.tq;
-    if (tq != kTqNone) { text_step(cont, tq); return; }
     const uint32_t ak = cont ? L.qk : L.k;
     const uint32_t al = cont ? L.ql : L.l;
-    uint32_t rk, rl;
-    int32_t adv = 1;
-    // (0, n) in EXACT is the phase's start (every LF step gives k >= 1): its
-    // first kmer_k steps are one table entry, the interval of those read
-    // bases (complemented on the - strand, as the LF steps take them); an N
-    // among them empties the interval as its LF step would.
-    if (L.phase == PH_EXACT && P.ix.kmer_k && ak == 0u && al == P.ix.n &&
-        (L.pass ? L.len / 2 - L.i : L.i - L.len / 2 + 1) >= (int32_t)P.ix.kmer_k) {
-      const uint32_t K = P.ix.kmer_k;
-      const int32_t d = step();
-      uint32_t code = 0u, bad = 0u;
-      for (uint32_t j = 0; j < K; ++j) {
-        const uint32_t s = L.W[L.i + (int32_t)j * d];
-        bad |= s > 3u ? 1u : 0u;
-        code |= (s & 3u) << (2u * j);
+    const int32_t p = cont ? L.cp : L.i;
+    const int32_t d = step();
+    const uint32_t K = P.ix.kmer_k;
+    const uint32_t *text = P.ix.text;
+
+    // ---- what to read (ALU only)
+    uint32_t kind, rlim = 1u, lim = 0u;
+    const uint32_t *a0 = nullptr;    // the word the kind reads (not F_LF, F_KPREP)
+    bool whi = false, two = false;
+    uint32_t j1 = 0u, j2 = 0u;
+    if (tq != kTqNone) {
+      kind = F_TEXT;
+      uint32_t rem;
+      if (L.phase == PH_EXACT) rem = (uint32_t)(L.pass ? L.len / 2 - p : p - L.len / 2 + 1);
+      else rem = (uint32_t)(d < 0 ? p + 1 : L.len - p);
+      lim = min(min(rem, 16u), tq);
+      rlim = max(lim, 1u);
+      const uint32_t nwords = (P.ix.n + 15u) >> 4;
+      uint32_t w;
+      if (L.dsel) {
+        w = min((P.ix.n - tq) >> 4, nwords - 1u);   // q = 0 (lim = 0) is clamped
+      } else {
+        const int64_t b = (int64_t)tq - 16;
+        w = b >= 0 ? (uint32_t)b >> 4 : 0u;
       }
-      if (L.comp) code ^= (1u << (2u * K)) - 1u;
-      if (COUNT) L.nkmer++;
-      const uint64_t e = bad ? 1ull : (L.dsel ? P.ix.kmer[1] : P.ix.kmer[0])[code];
-      rk = (uint32_t)e;
-      rl = (uint32_t)(e >> 32);
-      adv = (int32_t)K;
+      whi = w + 1u < nwords;
+      a0 = text + w;
+    } else if (L.phase == PH_EXACT && K && ak == 0u && al == P.ix.n &&
+               (L.pass ? L.len / 2 - L.i : L.i - L.len / 2 + 1) >= (int32_t)K) {
+      if (L.bi & kKReady) {
+        kind = F_KLOOK;
+        const uint64_t *tab = L.dsel ? P.ix.kmer[1] : P.ix.kmer[0];
+        a0 = reinterpret_cast<const uint32_t *>(tab + (L.bi & ~kKReady));
+      } else {
+        kind = F_KPREP;
+        rlim = K;
+      }
+    } else if (ak == al && text != nullptr) {
+      kind = F_SA;                   // a single row: continue on the text from its suffix
+      a0 = (L.dsel ? P.ix.d[1].sa : P.ix.d[0].sa) + ak;
     } else {
-    if (ak == al && P.ix.text != nullptr) {
-      // a single row: continue on the text from its suffix
+      kind = F_LF;
+      j1 = ak / kBlockRows;
+      j2 = (al + 1u) / kBlockRows;
+      // A size-1 interval needs only row k's block (Occ(s, k+1) = Occ(s, k) +
+      // [BWT[k] == s]); otherwise the block of l+1 too when it differs.
+      two = ak != al && j2 != j1;
+    }
+    // The read bases W[p], W[p+d], ... (rlim of them) lie in two aligned
+    // 16-byte chunks c0 < c1 (c0 = c1 or c1 = c0 when one of them holds
+    // none of those bases: no chunk without a read base of the lane is
+    // read); W[p] is byte wp of the 32.
+    const uint8_t *pa = L.W + p;
+    const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(pa) & 15u);
+    const uint8_t *pal = pa - mis;
+    const uint8_t *c0, *c1;
+    uint32_t wp;
+    if (d > 0) {
+      c0 = pal;
+      c1 = mis + rlim - 1u >= 16u ? pal + 16 : pal;
+      wp = mis;
+    } else {
+      c1 = pal;
+      c0 = mis < rlim - 1u ? pal - 16 : pal;
+      wp = 16u + mis;
+    }
+    // First line: the occupancy block of row k, or the aligned 16 bytes that
+    // hold the word(s) the kind reads (both text words when the second is in
+    // the next 16 bytes), the rest pointing at the same 16 bytes.  Second
+    // line: the block of row l+1, or the first line again.
+    const uint8_t *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;
+    if (kind == F_LF) {
+      const OccBlock *blk = L.dsel ? P.ix.d[1].blk : P.ix.d[0].blk;
+      f0 = reinterpret_cast<const uint8_t *>(blk + j1);
+      g0 = reinterpret_cast<const uint8_t *>(blk + (two ? j2 : j1));
+      f1 = f0 + 16; f2 = f0 + 32; f3 = f0 + 48;
+      g1 = g0 + 16; g2 = g0 + 32; g3 = g0 + 48;
+    } else {
+      const uint8_t *w0 = kind == F_KPREP ? c1 : reinterpret_cast<const uint8_t *>(a0);
+      const uint32_t wm = (uint32_t)(reinterpret_cast<uintptr_t>(w0) & 15u);
+      f0 = w0 - wm;
+      f1 = (kind == F_TEXT && whi && wm == 12u) ? f0 + 16 : f0;
+      f2 = f3 = g0 = g1 = g2 = g3 = f0;
+    }
+
+    // ---- the loads: one batch, no value used before the last is issued
+    const uint4 rc0 = *reinterpret_cast<const uint4 *>(c0);
+    const uint4 rc1 = *reinterpret_cast<const uint4 *>(c1);
+    const uint4 fx0 = *reinterpret_cast<const uint4 *>(f0);
+    const uint4 fx1 = *reinterpret_cast<const uint4 *>(f1);
+    const uint4 fx2 = *reinterpret_cast<const uint4 *>(f2);
+    const uint4 fx3 = *reinterpret_cast<const uint4 *>(f3);
+    const uint4 gx0 = *reinterpret_cast<const uint4 *>(g0);
+    const uint4 gx1 = *reinterpret_cast<const uint4 *>(g1);
+    const uint4 gx2 = *reinterpret_cast<const uint4 *>(g2);
+    const uint4 gx3 = *reinterpret_cast<const uint4 *>(g3);
+
+    // ---- the read window: 16 bases from W[p] in feed order as 2-bit
+    // groups (rw, physical symbols) and their N flags (rb, bit i)
+    uint64_t codes = 0ull;
+    uint32_t nflags = 0u;
+    {
+      const uint32_t v[8] = {rc0.x, rc0.y, rc0.z, rc0.w, rc1.x, rc1.y, rc1.z, rc1.w};
+#pragma unroll
+      for (uint32_t j = 0; j < 8u; ++j) {
+        uint32_t t = v[j] & 0x03030303u;            // 4 codes -> 8 bits
+        t = (t | (t >> 6)) & 0x000F000Fu;
+        t = (t | (t >> 12)) & 0xFFu;
+        uint32_t z = (v[j] >> 2) & 0x3F3F3F3Fu;     // a byte > 3 -> its bit 0
+        z = z | (z >> 3);
+        z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
+        z = (z | (z >> 7)) & 0x00030003u;
+        z = (z | (z >> 14)) & 0xFu;
+        codes |= (uint64_t)t << (8u * j);
+        nflags |= z << (4u * j);
+      }
+    }
+    uint32_t rw, rb;
+    if (d > 0) {
+      rw = (uint32_t)(codes >> (2u * wp));
+      rb = (nflags >> wp) & 0xFFFFu;
+    } else {
+      const uint32_t s0 = wp - 15u;                 // W[p-15 .. p] = bytes s0 .. wp
+      const uint32_t v = __builtin_bitreverse32((uint32_t)(codes >> (2u * s0)));
+      rw = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+      rb = __builtin_bitreverse32((nflags >> s0) & 0xFFFFu) >> 16;
+    }
+    const uint32_t r0 = (rb & 1u) ? 4u : (rw & 3u);   // W[p] (4 for any N)
+
+    // ---- the transitions
+    if (kind == F_TEXT) {
+      const uint32_t sel = (uint32_t)(reinterpret_cast<uintptr_t>(a0) & 15u) >> 2;
+      const uint32_t wlo = pick4u(fx0, sel);
+      const uint32_t whi32 = !whi ? 0u : (sel == 3u ? fx1.x : pick4u(fx0, sel + 1u));
+      const uint32_t t2 = text_window(tq, wlo, whi32);
+      uint32_t spread = rb;          // N flag of base i -> bit 2i
+      spread = (spread | (spread << 8)) & 0x00FF00FFu;
+      spread = (spread | (spread << 4)) & 0x0F0F0F0Fu;
+      spread = (spread | (spread << 2)) & 0x33333333u;
+      spread = (spread | (spread << 1)) & 0x55555555u;
+      const uint32_t m = run_length(t2, L.comp ? ~rw : rw, spread, lim);
+      text_apply(cont, tq, lim, m, r0, tq >= 1u ? (t2 & 3u) : 4u);
+      return;
+    }
+    if (kind == F_SA) {
       if (COUNT) L.nlook++;
-      const uint32_t q = (L.dsel ? P.ix.d[1].sa : P.ix.d[0].sa)[ak];
+      const uint32_t q = pick4u(fx0, (uint32_t)(reinterpret_cast<uintptr_t>(a0) & 15u) >> 2);
       if (cont) L.cq = q;
       else L.tq = q;
       return;
     }
-    const int32_t p = cont ? L.cp : L.i;
-    const uint32_t sym = L.W[p];
+    uint32_t rk, rl;
+    int32_t adv = 1;
+    if (kind == F_KPREP) {
+      const uint32_t kmask = (1u << (2u * K)) - 1u;
+      const uint32_t code = (L.comp ? ~rw : rw) & kmask;
+      if (COUNT) L.nkmer++;
+      if ((rb & ((1u << K) - 1u)) == 0u) { L.bi = code | kKReady; return; }
+      end_pass();                    // an N: the interval is empty
+      return;
+    }
+    if (kind == F_KLOOK) {
+      const bool hi = (reinterpret_cast<uintptr_t>(a0) & 8u) != 0u;
+      rk = hi ? fx0.z : fx0.x;
+      rl = hi ? fx0.w : fx0.y;
+      adv = (int32_t)K;
+      L.bi = 0u;
+    } else {
+    const uint32_t sym = r0;
     const uint32_t m2 = al + 1;
-    const uint32_t j1 = ak / kBlockRows, j2 = m2 / kBlockRows;
     const uint32_t t1 = ak - j1 * kBlockRows, t2 = m2 - j2 * kBlockRows;
-    // A size-1 interval needs only row k's block (Occ(s, k+1) = Occ(s, k) +
-    // [BWT[k] == s]); otherwise the block of l+1 too when it differs.
     const bool single = ak == al;
-    const bool two = !single && j2 != j1;
-    const OccBlock *blk = L.dsel ? P.ix.d[1].blk : P.ix.d[0].blk;
     const uint32_t dollar_blk = L.dsel ? P.ix.d[1].dollar_blk : P.ix.d[0].dollar_blk;
     const uint32_t dollar_off = L.dsel ? P.ix.d[1].dollar_off : P.ix.d[0].dollar_off;
-    const Blk B1 = load_block(blk, j1);
-    Blk B2;
-    if (two) B2 = load_block(blk, j2);
-    else B2 = B1;
+    const Blk B1 = blk_of4(fx0, fx1, fx2, fx3);
+    const Blk B2 = blk_of4(gx0, gx1, gx2, gx3);
     if (COUNT) { nblk_acc += two ? 2u : 1u; nq_acc += 1u; }
     const uint32_t dfix1 = (j1 == dollar_blk && dollar_off < t1) ? 1u : 0u;
     const uint32_t dfix2 = (j2 == dollar_blk && dollar_off < t2) ? 1u : 0u;
