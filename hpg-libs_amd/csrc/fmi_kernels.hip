@@ -199,6 +199,7 @@ struct S1Lane {
 };
 
 constexpr uint32_t kTqNone = 0xFFFFFFFFu;
+constexpr uint32_t kTextRowTag = 0x80000000u;   // k's high word: k, l hold q
 
 template <bool MM_ONLY, bool COUNT>
 struct S1 {
@@ -229,17 +230,19 @@ struct S1 {
     return (P.ix.text[x >> 4] >> ((x & 15u) << 1)) & 3u;
   }
 
-  // Row of the single-row interval whose suffix starts at q (count mode
-  // counts the read the stored record would make).
-  __device__ __forceinline__ uint32_t row_of(uint32_t q) {
-    if (COUNT) { L.nlook++; return 0u; }
-    return (L.dsel ? P.ix.d[1].isa : P.ix.d[0].isa)[q];
-  }
-
   __device__ __forceinline__ void emit(uint32_t k, uint32_t l, uint32_t q, uint8_t nmm,
                                        uint8_t kind, uint8_t base, int16_t err_pos) {
     if (L.nout < P.cap) {
-      if (q != kTqNone) k = l = row_of(q);
+      // A single-row text-mode interval is stored as its suffix's text
+      // position, tagged in k's high word; text_rows_kernel turns it into
+      // the row (ISA[q]) after the search, so no lane waits for that read
+      // here (count mode counts it).
+      uint64_t kk = k, ll = l;
+      if (q != kTqNone) {
+        if (COUNT) L.nlook++;
+        kk = ((uint64_t)kTextRowTag << 32) | q;
+        ll = q;
+      }
       int16_t start, end, pos;
       if (nmm == 0) {               // the exact record of pass 0
         start = 0; end = (int16_t)(L.len - 1); pos = -1;
@@ -249,7 +252,8 @@ struct S1 {
         start = (int16_t)(L.len / 2); end = (int16_t)(L.len - 1); pos = (int16_t)L.len;
       }
       if (!COUNT)
-        store_hit(P.hits + (size_t)(2 * L.read + L.strand) * P.cap + L.nout, k, l, start, pos,
+        store_hit(P.hits + (size_t)(2 * L.read + L.strand) * P.cap + L.nout, (int64_t)kk,
+                  (int64_t)ll, start, pos,
                   end, err_pos, (uint8_t)L.pass, nmm, kind,
                   base, (uint8_t)L.strand);
     }
@@ -784,6 +788,26 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
   }
 }
 
+// The rows of the records search1_kernel stored as text positions (k's high
+// word kTextRowTag, l = q): k = l = ISA[q] in the record's direction (dir ^
+// strand: the pass's direction).  One thread per (read, strand) slot.
+__global__ __launch_bounds__(256) void text_rows_kernel(DevIndex ix, fmi_hit *hits,
+                                                        const uint32_t *counts, uint32_t nslots,
+                                                        uint32_t cap) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslots) return;
+  const uint32_t c = min(counts[t], cap);
+  uint4 *h = reinterpret_cast<uint4 *>(hits + (size_t)t * cap);
+  for (uint32_t j = 0; j < c; ++j) {
+    const uint4 a = h[2 * j];
+    if (a.y != kTextRowTag) continue;
+    const uint4 b = h[2 * j + 1];
+    const uint32_t dsel = (b.z & 0xFFu) ^ (b.w & 0xFFu);
+    const uint32_t row = (dsel ? ix.d[1].isa : ix.d[0].isa)[a.x];
+    h[2 * j] = make_uint4(row, 0u, row, 0u);
+  }
+}
+
 __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1182,6 +1206,11 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
     const dim3 g((ntasks + 255) / 256), b(256);
     if (mm_only) hipLaunchKernelGGL((search1_kernel<false, true, false>), g, b, 0, s, P);
     else hipLaunchKernelGGL((search1_kernel<false, false, false>), g, b, 0, s, P);
+  }
+  if (!count && ix.text != nullptr && cap > 0) {
+    const uint64_t nslots = 2ull * ntasks;
+    hipLaunchKernelGGL(text_rows_kernel, dim3((uint32_t)((nslots + 255) / 256)), dim3(256), 0, s,
+                       ix, hits, counts, (uint32_t)nslots, cap);
   }
   return hipGetLastError();
 }
