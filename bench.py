@@ -469,8 +469,8 @@ def main():
     ap.add_argument("--sa-ratio", type=int, default=16)
     ap.add_argument("--cap", type=int, default=32)
     ap.add_argument("--kmer", type=int, default=-1,
-                    help="k of the k-mer interval table (0: none; default: floor(log4 n), "
-                         "4..13)")
+                    help="k of the k-mer interval table (0: none; default: the smallest k "
+                         "with 4^k >= n, 4..15)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reads", type=int, default=0,
                     help="CPU baseline sample size (0: all reads, capped by time)")
@@ -519,7 +519,11 @@ def main():
         args.reads = pre["reads"]
     n_bp = int(args.genome_mbp * 1_000_000)
     if args.kmer < 0:
-        args.kmer = min(13, max(4, int(math.log(max(n_bp, 4), 4))))
+        # the smallest k with 4^k >= n (most k-mers of the text then name
+        # one row or a few), at most 15 (8 GiB per direction)
+        args.kmer = 4
+        while args.kmer < 15 and 4 ** args.kmer < n_bp:
+            args.kmer += 1
     t0 = time.time()
     g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=pre["repeat_frac"],
                            fam_len=(300, 6000), divergence=pre["div"], n_frac=pre["n_frac"])

@@ -352,7 +352,7 @@ int fmi_index_expand(fmi_index *ix) {
 int fmi_index_kmer_table(fmi_index *ix, uint32_t k) {
   int rc = need_device(ix);
   if (rc) return rc;
-  if (k != 0 && (k < 4 || k > 13)) return fail(FMI_EINVAL, "k must be 0 or 4..13");
+  if (k != 0 && (k < 4 || k > 15)) return fail(FMI_EINVAL, "k must be 0 or 4..15");
   if (ix->d_kmer) {
     HIP_TRY(hipDeviceSynchronize(), "drain before freeing the table");
     (void)hipFree(ix->d_kmer);

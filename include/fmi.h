@@ -140,11 +140,11 @@ void fmi_index_free(fmi_index *ix);
  * the same.  No reference counterpart (it trades HBM for dependent loads). */
 int fmi_index_expand(fmi_index *ix);
 /* A table of the interval of every string of k bases in each direction
- * (4^k entries of 8 bytes per direction: k = 11 takes 64 MiB, k = 12 256
- * MiB), built on the device from the occurrence blocks.  fmi_search1_batch
- * then takes the first k LF steps of each exact phase in one read of the
- * table; the records are the same.  k = 0 frees the table; otherwise k is
- * 4..13.  No reference counterpart (it trades HBM for the widest, most
+ * (4^k entries of 8 bytes per direction: k = 12 takes 128 MiB, k = 13 512
+ * MiB, k = 15 8 GiB), built on the device from the occurrence blocks.
+ * fmi_search1_batch then takes the first k LF steps of each exact phase in
+ * one read of the table; the records are the same.  k = 0 frees the table;
+ * otherwise k is 4..15.  No reference counterpart (it trades HBM for the widest, most
  * expensive LF steps of every search). */
 int fmi_index_kmer_table(fmi_index *ix, uint32_t k);
 /* Copies one direction's table (4^k entries, k | l << 32, an empty

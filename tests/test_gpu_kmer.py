@@ -27,7 +27,7 @@ def kgold(request, gpu_engine):
 
 
 @pytest.mark.parametrize("expanded", [False, True])
-@pytest.mark.parametrize("k", [4, 7])
+@pytest.mark.parametrize("k", [4, 7, 15])
 def test_kmer_golden_search1(kgold, k, expanded):
     hp, g, ix, batch = kgold
     if expanded:
