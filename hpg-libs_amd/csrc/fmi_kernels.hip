@@ -367,32 +367,40 @@ struct S1 {
     start_strand(strand);
   }
 
-  // The 16 text bases that precede suffix q in the pass's direction, in
-  // feed order as 2-bit groups (physical symbols), from the text words lo32
-  // and hi32 that fetch() read for q (hi32 = 0 past the last word).
-  __device__ __forceinline__ uint32_t text_window(uint32_t q, uint32_t lo32, uint32_t hi32) const {
+  // The 32 text bases that precede suffix q in the pass's direction, in
+  // feed order as 2-bit groups (physical symbols), from the three text words
+  // t0, t1, t2 from word w on that advance() read for q (0 past the last
+  // word): w = (n - q) / 16 for the BWT of reverse X, (q - 32) / 16 (0 when
+  // q < 32) for the BWT of X.
+  __device__ __forceinline__ uint64_t text_window(uint32_t q, uint32_t t0, uint32_t t1,
+                                                  uint32_t t2) const {
+    const uint64_t lo = ((uint64_t)t1 << 32) | t0;
     if (L.dsel) {                    // X[x0], X[x0+1], ... with x0 = n - q
-      const uint32_t x0 = P.ix.n - q;
-      return (uint32_t)((((uint64_t)hi32 << 32) | lo32) >> ((x0 & 15u) << 1));
+      const uint32_t sh = ((P.ix.n - q) & 15u) << 1;
+      return sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
     }
-    const int64_t b = (int64_t)q - 16;   // X[b .. b+15] = X[q-16 .. q-1]
-    uint32_t win;
-    if (b >= 0) win = (uint32_t)((((uint64_t)hi32 << 32) | lo32) >> (((uint32_t)b & 15u) << 1));
-    else win = lo32 << (2u * (uint32_t)(-b));
-    // reverse the 16 groups: bit reversal, then swap the bits of each pair
-    const uint32_t v = __builtin_bitreverse32(win);
-    return ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+    const int64_t b = (int64_t)q - 32;   // X[b .. b+31] = X[q-32 .. q-1]
+    uint64_t win;
+    if (b >= 0) {
+      const uint32_t sh = ((uint32_t)b & 15u) << 1;
+      win = sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
+    } else {                         // q < 32: X[0 .. q-1] end the window
+      win = lo << min(2u * (uint32_t)(-b), 62u);
+    }
+    // reverse the 32 groups: bit reversal, then swap the bits of each pair
+    const uint64_t v = __builtin_bitreverse64(win);
+    return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
   }
 
-  // First mismatch (0..lim) of the read window r2 (feed order, complemented
-  // on the - strand) against the text window t2; bad marks N bases (bit 2i).
-  __device__ __forceinline__ static uint32_t run_length(uint32_t t2, uint32_t r2, uint32_t bad,
+  // First mismatch (0..lim) of the read window r (feed order, complemented
+  // on the - strand) against the text window t; bad marks N bases (bit 2i).
+  __device__ __forceinline__ static uint32_t run_length(uint64_t t, uint64_t r, uint64_t bad,
                                                         uint32_t lim) {
     if (lim == 0u) return 0u;
-    const uint32_t x = r2 ^ t2;
-    uint32_t miss = ((x | (x >> 1)) & 0x55555555u) | bad;
-    if (lim < 16u) miss |= 1u << (2u * lim);
-    return miss ? (uint32_t)__builtin_ctz(miss) >> 1 : 16u;
+    const uint64_t x = r ^ t;
+    uint64_t miss = ((x | (x >> 1)) & 0x5555555555555555ull) | bad;
+    if (lim < 32u) miss |= 1ull << (2u * lim);
+    return miss ? (uint32_t)__builtin_ctzll(miss) >> 1 : 32u;
   }
 
   // The transition of one text step on a text-mode interval (suffix q): m of
@@ -481,24 +489,26 @@ struct S1 {
     // ---- what to read (ALU only)
     uint32_t kind, rlim = 1u, lim = 0u;
     const uint32_t *a0 = nullptr;    // the word the kind reads (not F_LF, F_KPREP)
-    bool whi = false, two = false;
+    bool two = false, v1 = false, v2 = false, next = false;
     uint32_t j1 = 0u, j2 = 0u;
     if (tq != kTqNone) {
       kind = F_TEXT;
       uint32_t rem;
       if (L.phase == PH_EXACT) rem = (uint32_t)(L.pass ? L.len / 2 - p : p - L.len / 2 + 1);
       else rem = (uint32_t)(d < 0 ? p + 1 : L.len - p);
-      lim = min(min(rem, 16u), tq);
+      lim = min(min(rem, 32u), tq);
       rlim = max(lim, 1u);
       const uint32_t nwords = (P.ix.n + 15u) >> 4;
       uint32_t w;
       if (L.dsel) {
         w = min((P.ix.n - tq) >> 4, nwords - 1u);   // q = 0 (lim = 0) is clamped
       } else {
-        const int64_t b = (int64_t)tq - 16;
+        const int64_t b = (int64_t)tq - 32;
         w = b >= 0 ? (uint32_t)b >> 4 : 0u;
       }
-      whi = w + 1u < nwords;
+      v1 = w + 1u < nwords;
+      v2 = w + 2u < nwords;
+      next = (w & 3u) >= 2u && w + 4u - (w & 3u) < nwords;   // words in the next 16 bytes
       a0 = text + w;
     } else if (L.phase == PH_EXACT && K && ak == 0u && al == P.ix.n &&
                (L.pass ? L.len / 2 - L.i : L.i - L.len / 2 + 1) >= (int32_t)K) {
@@ -521,28 +531,30 @@ struct S1 {
       // [BWT[k] == s]); otherwise the block of l+1 too when it differs.
       two = ak != al && j2 != j1;
     }
-    // The read bases W[p], W[p+d], ... (rlim of them) lie in two aligned
-    // 16-byte chunks c0 < c1 (c0 = c1 or c1 = c0 when one of them holds
-    // none of those bases: no chunk without a read base of the lane is
-    // read); W[p] is byte wp of the 32.
+    // The read bases W[p], W[p+d], ... (rlim <= 32 of them) lie in three
+    // aligned 16-byte chunks c0 < c1 < c2; a chunk that holds none of them
+    // is replaced by the one of W[p] (no chunk without a read base of the
+    // lane is read).  W[p] is byte wp of the 48.
     const uint8_t *pa = L.W + p;
     const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(pa) & 15u);
     const uint8_t *pal = pa - mis;
-    const uint8_t *c0, *c1;
+    const uint8_t *c0, *c1, *c2;
     uint32_t wp;
     if (d > 0) {
       c0 = pal;
       c1 = mis + rlim - 1u >= 16u ? pal + 16 : pal;
+      c2 = mis + rlim - 1u >= 32u ? pal + 32 : pal;
       wp = mis;
     } else {
-      c1 = pal;
-      c0 = mis < rlim - 1u ? pal - 16 : pal;
-      wp = 16u + mis;
+      c2 = pal;
+      c1 = rlim - 1u > mis ? pal - 16 : pal;
+      c0 = rlim - 1u > mis + 16u ? pal - 32 : pal;
+      wp = 32u + mis;
     }
     // First line: the occupancy block of row k, or the aligned 16 bytes that
-    // hold the word(s) the kind reads (both text words when the second is in
-    // the next 16 bytes), the rest pointing at the same 16 bytes.  Second
-    // line: the block of row l+1, or the first line again.
+    // hold the word(s) the kind reads (and the next 16 bytes when text words
+    // continue there), the rest pointing at the same 16 bytes.  Second line:
+    // the block of row l+1, or the first line again.
     const uint8_t *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;
     if (kind == F_LF) {
       const OccBlock *blk = L.dsel ? P.ix.d[1].blk : P.ix.d[0].blk;
@@ -551,16 +563,17 @@ struct S1 {
       f1 = f0 + 16; f2 = f0 + 32; f3 = f0 + 48;
       g1 = g0 + 16; g2 = g0 + 32; g3 = g0 + 48;
     } else {
-      const uint8_t *w0 = kind == F_KPREP ? c1 : reinterpret_cast<const uint8_t *>(a0);
+      const uint8_t *w0 = kind == F_KPREP ? c2 : reinterpret_cast<const uint8_t *>(a0);
       const uint32_t wm = (uint32_t)(reinterpret_cast<uintptr_t>(w0) & 15u);
       f0 = w0 - wm;
-      f1 = (kind == F_TEXT && whi && wm == 12u) ? f0 + 16 : f0;
+      f1 = next ? f0 + 16 : f0;
       f2 = f3 = g0 = g1 = g2 = g3 = f0;
     }
 
     // ---- the loads: one batch, no value used before the last is issued
     const uint4 rc0 = *reinterpret_cast<const uint4 *>(c0);
     const uint4 rc1 = *reinterpret_cast<const uint4 *>(c1);
+    const uint4 rc2 = *reinterpret_cast<const uint4 *>(c2);
     const uint4 fx0 = *reinterpret_cast<const uint4 *>(f0);
     const uint4 fx1 = *reinterpret_cast<const uint4 *>(f1);
     const uint4 fx2 = *reinterpret_cast<const uint4 *>(f2);
@@ -570,14 +583,15 @@ struct S1 {
     const uint4 gx2 = *reinterpret_cast<const uint4 *>(g2);
     const uint4 gx3 = *reinterpret_cast<const uint4 *>(g3);
 
-    // ---- the read window: 16 bases from W[p] in feed order as 2-bit
+    // ---- the read window: 32 bases from W[p] in feed order as 2-bit
     // groups (rw, physical symbols) and their N flags (rb, bit i)
-    uint64_t codes = 0ull;
-    uint32_t nflags = 0u;
+    uint64_t clo = 0ull, nfl = 0ull;
+    uint32_t chi = 0u;
     {
-      const uint32_t v[8] = {rc0.x, rc0.y, rc0.z, rc0.w, rc1.x, rc1.y, rc1.z, rc1.w};
+      const uint32_t v[12] = {rc0.x, rc0.y, rc0.z, rc0.w, rc1.x, rc1.y,
+                              rc1.z, rc1.w, rc2.x, rc2.y, rc2.z, rc2.w};
 #pragma unroll
-      for (uint32_t j = 0; j < 8u; ++j) {
+      for (uint32_t j = 0; j < 12u; ++j) {
         uint32_t t = v[j] & 0x03030303u;            // 4 codes -> 8 bits
         t = (t | (t >> 6)) & 0x000F000Fu;
         t = (t | (t >> 12)) & 0xFFu;
@@ -586,35 +600,41 @@ struct S1 {
         z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
         z = (z | (z >> 7)) & 0x00030003u;
         z = (z | (z >> 14)) & 0xFu;
-        codes |= (uint64_t)t << (8u * j);
-        nflags |= z << (4u * j);
+        if (j < 8u) clo |= (uint64_t)t << (8u * j);
+        else chi |= t << (8u * (j - 8u));
+        nfl |= (uint64_t)z << (4u * j);
       }
     }
-    uint32_t rw, rb;
-    if (d > 0) {
-      rw = (uint32_t)(codes >> (2u * wp));
-      rb = (nflags >> wp) & 0xFFFFu;
-    } else {
-      const uint32_t s0 = wp - 15u;                 // W[p-15 .. p] = bytes s0 .. wp
-      const uint32_t v = __builtin_bitreverse32((uint32_t)(codes >> (2u * s0)));
-      rw = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
-      rb = __builtin_bitreverse32((nflags >> s0) & 0xFFFFu) >> 16;
+    uint64_t rw;
+    uint32_t rb;
+    if (d > 0) {                     // bytes wp .. wp+31
+      const uint32_t sh = 2u * wp;
+      rw = sh ? (clo >> sh) | ((uint64_t)chi << (64u - sh)) : clo;
+      rb = (uint32_t)(nfl >> wp);
+    } else {                         // W[p-31 .. p] = bytes wp-31 .. wp
+      const uint32_t s0 = wp - 31u, sh = 2u * s0;    // s0 >= 1
+      const uint64_t win = (clo >> sh) | ((uint64_t)chi << (64u - sh));
+      const uint64_t v = __builtin_bitreverse64(win);
+      rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+      rb = __builtin_bitreverse32((uint32_t)(nfl >> s0));
     }
-    const uint32_t r0 = (rb & 1u) ? 4u : (rw & 3u);   // W[p] (4 for any N)
+    const uint32_t r0 = (rb & 1u) ? 4u : (uint32_t)(rw & 3u);   // W[p] (4 for any N)
 
     // ---- the transitions
     if (kind == F_TEXT) {
       const uint32_t sel = (uint32_t)(reinterpret_cast<uintptr_t>(a0) & 15u) >> 2;
-      const uint32_t wlo = pick4u(fx0, sel);
-      const uint32_t whi32 = !whi ? 0u : (sel == 3u ? fx1.x : pick4u(fx0, sel + 1u));
-      const uint32_t t2 = text_window(tq, wlo, whi32);
-      uint32_t spread = rb;          // N flag of base i -> bit 2i
-      spread = (spread | (spread << 8)) & 0x00FF00FFu;
-      spread = (spread | (spread << 4)) & 0x0F0F0F0Fu;
-      spread = (spread | (spread << 2)) & 0x33333333u;
-      spread = (spread | (spread << 1)) & 0x55555555u;
-      const uint32_t m = run_length(t2, L.comp ? ~rw : rw, spread, lim);
-      text_apply(cont, tq, lim, m, r0, tq >= 1u ? (t2 & 3u) : 4u);
+      const uint32_t t0 = pick4u(fx0, sel);
+      const uint32_t t1 = !v1 ? 0u : (sel == 3u ? fx1.x : pick4u(fx0, sel + 1u));
+      const uint32_t t2w = !v2 ? 0u : (sel >= 2u ? pick4u(fx1, sel - 2u) : pick4u(fx0, sel + 2u));
+      const uint64_t tw = text_window(tq, t0, t1, t2w);
+      uint64_t spread = rb;          // N flag of base i -> bit 2i
+      spread = (spread | (spread << 16)) & 0x0000FFFF0000FFFFull;
+      spread = (spread | (spread << 8)) & 0x00FF00FF00FF00FFull;
+      spread = (spread | (spread << 4)) & 0x0F0F0F0F0F0F0F0Full;
+      spread = (spread | (spread << 2)) & 0x3333333333333333ull;
+      spread = (spread | (spread << 1)) & 0x5555555555555555ull;
+      const uint32_t m = run_length(tw, L.comp ? ~rw : rw, spread, lim);
+      text_apply(cont, tq, lim, m, r0, tq >= 1u ? (uint32_t)(tw & 3u) : 4u);
       return;
     }
     if (kind == F_SA) {
@@ -628,7 +648,7 @@ struct S1 {
     int32_t adv = 1;
     if (kind == F_KPREP) {
       const uint32_t kmask = (1u << (2u * K)) - 1u;
-      const uint32_t code = (L.comp ? ~rw : rw) & kmask;
+      const uint32_t code = (uint32_t)(L.comp ? ~rw : rw) & kmask;
       if (COUNT) L.nkmer++;
       if ((rb & ((1u << K) - 1u)) == 0u) { L.bi = code | kKReady; return; }
       end_pass();                    // an N: the interval is empty
