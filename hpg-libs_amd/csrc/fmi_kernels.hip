@@ -195,11 +195,45 @@ struct S1Lane {
   // Text mode (expanded index): a single-row interval is held as the text
   // position q of its suffix in the pass's direction; kTqNone = rows in k, l.
   uint32_t tq, tqn, cq;            // node, child of W[i], continuation
+  uint32_t kc[2];                  // k-mer table index of each pass's first k bases
+                                   // (+ strand; kKReady unless unknown or with an N)
   uint32_t nlook, ntext, nkmer;    // SA / ISA reads, text steps, table lookups (count mode)
 };
 
 constexpr uint32_t kTqNone = 0xFFFFFFFFu;
 constexpr uint32_t kTextRowTag = 0x80000000u;   // k's high word: k, l hold q
+
+// 16 read bases from byte wp of the 32 bytes c0, c1 (towards higher bytes
+// when up, else lower) as 2-bit groups in that order (w, physical symbols)
+// and their N flags (nb, bit i: a byte > 3).
+__device__ __forceinline__ void window16(const uint4 &c0, const uint4 &c1, uint32_t wp, bool up,
+                                         uint32_t &w, uint32_t &nb) {
+  const uint32_t v[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+  uint64_t codes = 0ull;
+  uint32_t nfl = 0u;
+#pragma unroll
+  for (uint32_t j = 0; j < 8u; ++j) {
+    uint32_t t = v[j] & 0x03030303u;              // 4 codes -> 8 bits
+    t = (t | (t >> 6)) & 0x000F000Fu;
+    t = (t | (t >> 12)) & 0xFFu;
+    uint32_t z = (v[j] >> 2) & 0x3F3F3F3Fu;       // a byte > 3 -> its bit 0
+    z = z | (z >> 3);
+    z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
+    z = (z | (z >> 7)) & 0x00030003u;
+    z = (z | (z >> 14)) & 0xFu;
+    codes |= (uint64_t)t << (8u * j);
+    nfl |= z << (4u * j);
+  }
+  if (up) {
+    w = (uint32_t)(codes >> (2u * wp));
+    nb = (nfl >> wp) & 0xFFFFu;
+  } else {
+    const uint32_t s0 = wp - 15u;                   // bytes s0 .. wp
+    const uint32_t r = __builtin_bitreverse32((uint32_t)(codes >> (2u * s0)));
+    w = ((r >> 1) & 0x55555555u) | ((r & 0x55555555u) << 1);
+    nb = __builtin_bitreverse32((nfl >> s0) & 0xFFFFu) >> 16;
+  }
+}
 
 template <bool MM_ONLY, bool COUNT>
 struct S1 {
@@ -288,8 +322,12 @@ struct S1 {
     const bool second = (pass ^ L.strand) != 0u;
     L.dsel = second ? 1u : 0u;
     L.tq = L.tqn = L.cq = kTqNone;
-    L.bi = 0u;                       // no k-mer entry index prepared
     L.comp = L.strand;
+    // the pass's k-mer table index, complemented on the - strand (0: none
+    // known; the first iteration reads the bases)
+    const uint32_t kc = pass ? L.kc[1] : L.kc[0];
+    const uint32_t kmask = (1u << (2u * P.ix.kmer_k)) - 1u;
+    L.bi = (kc & kKReady) && L.comp ? kc ^ kmask : kc;
     L.k = 0;
     L.l = P.ix.n;
     L.i = pass ? 0 : L.len - 1;
@@ -364,6 +402,30 @@ struct S1 {
     L.len = (int32_t)(P.offsets[L.read + 1] - off);
     const uint32_t strand = (P.strand_mask == 2u) ? 1u : 0u;
     if (P.strand_mask != 3u) P.counts[2 * L.read + (strand ^ 1u)] = 0;
+    // The k-mer table index of both passes' first k bases (pass 1: W[0],
+    // W[1], ...; pass 0: W[len-1], W[len-2], ...), read here with the task
+    // (two aligned 16-byte chunks at each end of the read, each holding one
+    // of those bases), so that no iteration waits for them later.
+    L.kc[0] = L.kc[1] = 0u;
+    const uint32_t K = P.ix.kmer_k;
+    if (K && L.len >= (int32_t)K) {
+      const uint8_t *a = L.W, *z = L.W + (L.len - 1);
+      const uint32_t ma = (uint32_t)(reinterpret_cast<uintptr_t>(a) & 15u);
+      const uint32_t mz = (uint32_t)(reinterpret_cast<uintptr_t>(z) & 15u);
+      const uint8_t *a0 = a - ma, *z1 = z - mz;
+      const uint8_t *a1 = ma + K - 1u >= 16u ? a0 + 16 : a0;
+      const uint8_t *z0 = K - 1u > mz ? z1 - 16 : z1;
+      const uint4 ca0 = *reinterpret_cast<const uint4 *>(a0);
+      const uint4 ca1 = *reinterpret_cast<const uint4 *>(a1);
+      const uint4 cz0 = *reinterpret_cast<const uint4 *>(z0);
+      const uint4 cz1 = *reinterpret_cast<const uint4 *>(z1);
+      const uint32_t kmask = (1u << (2u * K)) - 1u, nmask = (1u << K) - 1u;
+      uint32_t w, nb;
+      window16(ca0, ca1, ma, true, w, nb);
+      L.kc[1] = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
+      window16(cz0, cz1, 16u + mz, false, w, nb);
+      L.kc[0] = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
+    }
     start_strand(strand);
   }
 
@@ -649,8 +711,8 @@ struct S1 {
     if (kind == F_KPREP) {
       const uint32_t kmask = (1u << (2u * K)) - 1u;
       const uint32_t code = (uint32_t)(L.comp ? ~rw : rw) & kmask;
-      if (COUNT) L.nkmer++;
       if ((rb & ((1u << K) - 1u)) == 0u) { L.bi = code | kKReady; return; }
+      if (COUNT) L.nkmer++;
       end_pass();                    // an N: the interval is empty
       return;
     }
@@ -659,6 +721,7 @@ struct S1 {
       rk = hi ? fx0.z : fx0.x;
       rl = hi ? fx0.w : fx0.y;
       adv = (int32_t)K;
+      if (COUNT) L.nkmer++;
       L.bi = 0u;
     } else {
     const uint32_t sym = r0;
