@@ -804,7 +804,7 @@ def main():
                                         not args.no_expand, args.kmer),
         },
         "cpu_baseline": cpu,
-        "kernel": "fmi::search1_kernel<true, true, false>",
+        "kernel": "fmi::search1_kernel<true, true, false, %s>" % ("false" if args.no_expand else "true"),
         "kernel_ms_mean": round(kernel_ms, 4),
         "alg_bytes_per_launch": alg_bytes,
         "lf_queries_per_launch": lf_steps,

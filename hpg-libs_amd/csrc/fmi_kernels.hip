@@ -235,7 +235,7 @@ __device__ __forceinline__ void window16(const uint4 &c0, const uint4 &c1, uint3
   }
 }
 
-template <bool MM_ONLY, bool COUNT>
+template <bool MM_ONLY, bool COUNT, bool TEXT>
 struct S1 {
   const S1Params &P;
   S1Lane &L;
@@ -553,7 +553,7 @@ struct S1 {
     const uint32_t *a0 = nullptr;    // the word the kind reads (not F_LF, F_KPREP)
     bool two = false, v1 = false, v2 = false, next = false;
     uint32_t j1 = 0u, j2 = 0u;
-    if (tq != kTqNone) {
+    if (TEXT && tq != kTqNone) {
       kind = F_TEXT;
       uint32_t rem;
       if (L.phase == PH_EXACT) rem = (uint32_t)(L.pass ? L.len / 2 - p : p - L.len / 2 + 1);
@@ -582,7 +582,7 @@ struct S1 {
         kind = F_KPREP;
         rlim = K;
       }
-    } else if (ak == al && text != nullptr) {
+    } else if (TEXT && ak == al) {
       kind = F_SA;                   // a single row: continue on the text from its suffix
       a0 = (L.dsel ? P.ix.d[1].sa : P.ix.d[0].sa) + ak;
     } else {
@@ -600,6 +600,8 @@ struct S1 {
     const uint8_t *pa = L.W + p;
     const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(pa) & 15u);
     const uint8_t *pal = pa - mis;
+    // (On the compact index no step compares more than a k-mer, at most 15
+    // bases, so two chunks: c2 is c1, W[p] is byte wp of the 32.)
     const uint8_t *c0, *c1, *c2;
     uint32_t wp;
     if (d > 0) {
@@ -607,46 +609,62 @@ struct S1 {
       c1 = mis + rlim - 1u >= 16u ? pal + 16 : pal;
       c2 = mis + rlim - 1u >= 32u ? pal + 32 : pal;
       wp = mis;
-    } else {
+    } else if (TEXT) {
       c2 = pal;
       c1 = rlim - 1u > mis ? pal - 16 : pal;
       c0 = rlim - 1u > mis + 16u ? pal - 32 : pal;
       wp = 32u + mis;
+    } else {
+      c1 = c2 = pal;
+      c0 = rlim - 1u > mis ? pal - 16 : pal;
+      wp = 16u + mis;
     }
     // First line: the occupancy block of row k, or the aligned 16 bytes that
     // hold the word(s) the kind reads (and the next 16 bytes when text words
-    // continue there), the rest pointing at the same 16 bytes.  Second line:
-    // the block of row l+1, or the first line again.
-    const uint8_t *f0, *f1, *f2, *f3, *g0, *g1, *g2, *g3;
+    // continue there), the rest pointing at the same 16 bytes.  Second line
+    // (read only by the lanes whose LF query needs it): the block of row l+1.
+    const uint8_t *f0, *f1, *f2, *f3, *g0 = nullptr;
     if (kind == F_LF) {
       const OccBlock *blk = L.dsel ? P.ix.d[1].blk : P.ix.d[0].blk;
       f0 = reinterpret_cast<const uint8_t *>(blk + j1);
-      g0 = reinterpret_cast<const uint8_t *>(blk + (two ? j2 : j1));
+      g0 = reinterpret_cast<const uint8_t *>(blk + j2);
       f1 = f0 + 16; f2 = f0 + 32; f3 = f0 + 48;
-      g1 = g0 + 16; g2 = g0 + 32; g3 = g0 + 48;
     } else {
       const uint8_t *w0 = kind == F_KPREP ? c2 : reinterpret_cast<const uint8_t *>(a0);
       const uint32_t wm = (uint32_t)(reinterpret_cast<uintptr_t>(w0) & 15u);
       f0 = w0 - wm;
       f1 = next ? f0 + 16 : f0;
-      f2 = f3 = g0 = g1 = g2 = g3 = f0;
+      f2 = f3 = f0;
     }
 
-    // ---- the loads: one batch, no value used before the last is issued
+    // ---- the loads: one batch, no value used before the last is issued;
+    // the second block's loads come last, in the branch of the lanes that
+    // need it, so a wait the compiler puts there is the batch's one wait
     const uint4 rc0 = *reinterpret_cast<const uint4 *>(c0);
     const uint4 rc1 = *reinterpret_cast<const uint4 *>(c1);
-    const uint4 rc2 = *reinterpret_cast<const uint4 *>(c2);
+    uint4 rc2;
+    if (TEXT) rc2 = *reinterpret_cast<const uint4 *>(c2);
     const uint4 fx0 = *reinterpret_cast<const uint4 *>(f0);
     const uint4 fx1 = *reinterpret_cast<const uint4 *>(f1);
     const uint4 fx2 = *reinterpret_cast<const uint4 *>(f2);
     const uint4 fx3 = *reinterpret_cast<const uint4 *>(f3);
-    const uint4 gx0 = *reinterpret_cast<const uint4 *>(g0);
-    const uint4 gx1 = *reinterpret_cast<const uint4 *>(g1);
-    const uint4 gx2 = *reinterpret_cast<const uint4 *>(g2);
-    const uint4 gx3 = *reinterpret_cast<const uint4 *>(g3);
+    uint4 gx0, gx1, gx2, gx3;
+    if (two) {
+      gx0 = *reinterpret_cast<const uint4 *>(g0);
+      gx1 = *reinterpret_cast<const uint4 *>(g0 + 16);
+      gx2 = *reinterpret_cast<const uint4 *>(g0 + 32);
+      gx3 = *reinterpret_cast<const uint4 *>(g0 + 48);
+    }
 
     // ---- the read window: 32 bases from W[p] in feed order as 2-bit
     // groups (rw, physical symbols) and their N flags (rb, bit i)
+    uint64_t rw;
+    uint32_t rb;
+    if (!TEXT) {
+      uint32_t w32;
+      window16(rc0, rc1, wp, d > 0, w32, rb);
+      rw = w32;
+    } else {
     uint64_t clo = 0ull, nfl = 0ull;
     uint32_t chi = 0u;
     {
@@ -667,8 +685,6 @@ struct S1 {
         nfl |= (uint64_t)z << (4u * j);
       }
     }
-    uint64_t rw;
-    uint32_t rb;
     if (d > 0) {                     // bytes wp .. wp+31
       const uint32_t sh = 2u * wp;
       rw = sh ? (clo >> sh) | ((uint64_t)chi << (64u - sh)) : clo;
@@ -679,6 +695,7 @@ struct S1 {
       const uint64_t v = __builtin_bitreverse64(win);
       rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
       rb = __builtin_bitreverse32((uint32_t)(nfl >> s0));
+    }
     }
     const uint32_t r0 = (rb & 1u) ? 4u : (uint32_t)(rw & 3u);   // W[p] (4 for any N)
 
@@ -731,7 +748,7 @@ struct S1 {
     const uint32_t dollar_blk = L.dsel ? P.ix.d[1].dollar_blk : P.ix.d[0].dollar_blk;
     const uint32_t dollar_off = L.dsel ? P.ix.d[1].dollar_off : P.ix.d[0].dollar_off;
     const Blk B1 = blk_of4(fx0, fx1, fx2, fx3);
-    const Blk B2 = blk_of4(gx0, gx1, gx2, gx3);
+    const Blk B2 = two ? blk_of4(gx0, gx1, gx2, gx3) : B1;
     if (COUNT) { nblk_acc += two ? 2u : 1u; nq_acc += 1u; }
     const uint32_t dfix1 = (j1 == dollar_blk && dollar_off < t1) ? 1u : 0u;
     const uint32_t dfix2 = (j2 == dollar_blk && dollar_off < t2) ? 1u : 0u;
@@ -810,14 +827,14 @@ __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
-template <bool PERSISTENT, bool MM_ONLY, bool COUNT>
+template <bool PERSISTENT, bool MM_ONLY, bool COUNT, bool TEXT>
 __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
   S1Lane L;
   L.phase = PH_IDLE;
   L.nlook = 0;
   L.ntext = 0;
   L.nkmer = 0;
-  S1<MM_ONLY, COUNT> m{P, L};
+  S1<MM_ONLY, COUNT, TEXT> m{P, L};
   unsigned long long nblk = 0, nq = 0;
   const uint32_t lane = lane_id();
 
@@ -1274,21 +1291,33 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
   S1Params P{ix, reads, offsets, hits, counts, queue, stats, ntasks, strand_mask, cap, chunk,
              refill_min};
   if (ntasks == 0) return hipSuccess;
+  const bool text = ix.text != nullptr;   // expanded index: text mode
   if (persistent) {
     if (grid_persistent == 0 || chunk == 0 || refill_min == 0 || refill_min > 64)
       return hipErrorInvalidValue;
     const dim3 g(grid_persistent), b(256);
+#define FMI_S1_LAUNCH(MM, CNT)                                                             \
+  do {                                                                                     \
+    if (text) hipLaunchKernelGGL((search1_kernel<true, MM, CNT, true>), g, b, 0, s, P);    \
+    else hipLaunchKernelGGL((search1_kernel<true, MM, CNT, false>), g, b, 0, s, P);        \
+  } while (0)
     if (mm_only) {
-      if (count) hipLaunchKernelGGL((search1_kernel<true, true, true>), g, b, 0, s, P);
-      else hipLaunchKernelGGL((search1_kernel<true, true, false>), g, b, 0, s, P);
+      if (count) FMI_S1_LAUNCH(true, true);
+      else FMI_S1_LAUNCH(true, false);
     } else {
-      if (count) hipLaunchKernelGGL((search1_kernel<true, false, true>), g, b, 0, s, P);
-      else hipLaunchKernelGGL((search1_kernel<true, false, false>), g, b, 0, s, P);
+      if (count) FMI_S1_LAUNCH(false, true);
+      else FMI_S1_LAUNCH(false, false);
     }
+#undef FMI_S1_LAUNCH
   } else {
     const dim3 g((ntasks + 255) / 256), b(256);
-    if (mm_only) hipLaunchKernelGGL((search1_kernel<false, true, false>), g, b, 0, s, P);
-    else hipLaunchKernelGGL((search1_kernel<false, false, false>), g, b, 0, s, P);
+    if (mm_only) {
+      if (text) hipLaunchKernelGGL((search1_kernel<false, true, false, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((search1_kernel<false, true, false, false>), g, b, 0, s, P);
+    } else {
+      if (text) hipLaunchKernelGGL((search1_kernel<false, false, false, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((search1_kernel<false, false, false, false>), g, b, 0, s, P);
+    }
   }
   if (!count && ix.text != nullptr && cap > 0) {
     const uint64_t nslots = 2ull * ntasks;
