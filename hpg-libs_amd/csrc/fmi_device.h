@@ -144,6 +144,33 @@ __device__ __forceinline__ void lf_one(const DevDir &D, uint32_t k, uint32_t l, 
   lo = c + occ_blk(B2, D.dollar_blk, D.dollar_off, j2, s, m2 - j2 * kBlockRows);
 }
 
+// The LF step of all four physical symbols on [k, l] at once (one or two
+// block loads): ko[s], lo[s] as lf_one gives them for s.
+__device__ __forceinline__ void lf_four(const DevDir &D, uint32_t k, uint32_t l,
+                                        uint32_t (&ko)[4], uint32_t (&lo)[4]) {
+  const uint32_t m2 = l + 1;
+  const uint32_t j1 = k / kBlockRows, j2 = m2 / kBlockRows;
+  const uint32_t t1 = k - j1 * kBlockRows, t2 = m2 - j2 * kBlockRows;
+  const Blk B1 = load_block(D.blk, j1);
+  uint32_t r1[4];
+  rank4(B1, t1, r1);
+  r1[0] -= (j1 == D.dollar_blk && D.dollar_off < t1) ? 1u : 0u;
+#pragma unroll
+  for (uint32_t s = 0; s < 4u; ++s) ko[s] = pick4(D.C, s) + 1u + B1.c[s] + r1[s];
+  if (k == l) {
+    const uint32_t srow = sym_at(B1, D.dollar_blk, D.dollar_off, j1, t1);
+#pragma unroll
+    for (uint32_t s = 0; s < 4u; ++s) lo[s] = ko[s] - 1u + (srow == s ? 1u : 0u);
+    return;
+  }
+  const Blk B2 = j2 != j1 ? load_block(D.blk, j2) : B1;
+  uint32_t r2[4];
+  rank4(B2, t2, r2);
+  r2[0] -= (j2 == D.dollar_blk && D.dollar_off < t2) ? 1u : 0u;
+#pragma unroll
+  for (uint32_t s = 0; s < 4u; ++s) lo[s] = pick4(D.C, s) + B2.c[s] + r2[s];
+}
+
 __device__ __forceinline__ const DevDir &dir_of(const DevIndex &ix, uint32_t d) {
   return d ? ix.d[1] : ix.d[0];
 }

@@ -85,6 +85,29 @@ __device__ __forceinline__ void klf(const KCtx &c, const KView &v, int64_t k, in
   lo = e;
 }
 
+// The k-mer table's interval of the kmer_k bases W[i], W[i + step], ... on
+// view v (in the order its LF steps take them, complemented on the - strand),
+// when the index has a table, all of them are A/C/G/T and the interval is not
+// empty; otherwise the caller steps one base at a time, so where and how a
+// search dies stays the reference's.
+__device__ __forceinline__ bool k_table(const KCtx &c, const KView &v, int32_t i, int32_t step,
+                                        int64_t &k, int64_t &l) {
+  const uint32_t K = c.ix->kmer_k;
+  uint32_t code = 0u, bad = 0u;
+  for (uint32_t j = 0; j < K; ++j) {
+    const uint32_t s = c.W[i + (int32_t)j * step];
+    bad |= s > 3u ? 1u : 0u;
+    code |= (s & 3u) << (2u * j);
+  }
+  if (bad) return false;
+  if (v.comp) code ^= (1u << (2u * K)) - 1u;
+  const uint64_t e = c.ix->kmer[v.dsel][code];
+  if ((uint32_t)e > (uint32_t)(e >> 32)) return false;
+  k = (uint32_t)e;
+  l = (uint32_t)(e >> 32);
+  return true;
+}
+
 // add_result (results.h:93-103): keep, or return true from the caller.
 #define KADD(r, L)                                                   \
   do {                                                               \
@@ -124,9 +147,16 @@ __device__ __forceinline__ void k_clear(fmi_hitk &r) {
 // callers only keep a live result, and the anchor holds the last live one.
 __device__ __forceinline__ void k_exact(const KCtx &c, const KView &v, int dir, fmi_hitk &r, fmi_hitk *anchor) {
   int64_t k = 0, l = 0, k2 = r.k, l2 = r.l;
-  int32_t i;
+  int32_t i, i0 = r.pos;
+  // from the whole index, the first kmer_k steps can be one table read
+  const int32_t K = (int32_t)c.ix->kmer_k;
+  const bool whole = K && k2 == 0 && l2 == (int64_t)c.ix->n;
+  int64_t kt, lt;
   if (dir == 0) {
-    for (i = r.pos; i >= r.start; --i) {
+    if (whole && r.pos - r.start + 1 >= K && k_table(c, v, r.pos, -1, kt, lt)) {
+      k = k2 = kt; l = l2 = lt; i0 = r.pos - K;
+    }
+    for (i = i0; i >= r.start; --i) {
       if (c.text && k2 == l2) {
         uint32_t q2;
         bool dies;
@@ -146,7 +176,10 @@ __device__ __forceinline__ void k_exact(const KCtx &c, const KView &v, int dir, 
       anchor->k = k; anchor->l = l; anchor->pos = (int16_t)i;
     }
   } else {
-    for (i = r.pos; i <= r.end; ++i) {
+    if (whole && r.end - r.pos + 1 >= K && k_table(c, v, r.pos, 1, kt, lt)) {
+      k = k2 = kt; l = l2 = lt; i0 = r.pos + K;
+    }
+    for (i = i0; i <= r.end; ++i) {
       if (c.text && k2 == l2) {
         uint32_t q2;
         bool dies;
@@ -270,6 +303,23 @@ __device__ __forceinline__ bool k_branch(const KCtx &c, const KView &v, int dir,
   int64_t ka, la;
   bool no_previous = true;
   const uint32_t wp = c.W[pos];
+  // the children of (k, l) for b = 0..3, from one read of the block(s): the
+  // four klf calls of the loops below, which the record stores in between
+  // would otherwise make four dependent reads
+  int64_t kb[4], lb4[4];
+  if (k > l) {
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; ++b) { kb[b] = k; lb4[b] = l; }
+  } else {
+    uint32_t ko[4], lo[4];
+    lf_four(*v.d, (uint32_t)k, (uint32_t)l, ko, lo);
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; ++b) {
+      const uint32_t s = v.comp ? 3u - b : b;
+      kb[b] = pick4(ko, s);
+      lb4[b] = pick4(lo, s);
+    }
+  }
 #define KEMIT()                                                          \
   do {                                                                   \
     if (FINAL) {                                                         \
@@ -284,7 +334,7 @@ __device__ __forceinline__ bool k_branch(const KCtx &c, const KView &v, int dir,
       if (wp != last_err_base) { k_change(r, k, l, pos + step); KEMIT(); }
       const uint32_t wl = c.W[last_err_pos];
       for (uint32_t b = 0; b < 4u; ++b) {
-        klf(c, v, k, l, b, ka, la);
+        ka = kb[b]; la = lb4[b];
         if (ka > la) continue;
         if (b != wp) {
           if (b != wl) { k_change(r, ka, la, pos); k_mod2(r, FMI_INSERTION, (uint8_t)b); KEMIT(); }
@@ -296,7 +346,7 @@ __device__ __forceinline__ bool k_branch(const KCtx &c, const KView &v, int dir,
       k_change(r, k, l, pos + step); KEMIT();
       const uint32_t wl = c.W[last_err_pos];
       for (uint32_t b = 0; b < 4u; ++b) {
-        klf(c, v, k, l, b, ka, la);
+        ka = kb[b]; la = lb4[b];
         if (ka > la) continue;
         if (b != wp && b != wl) {
           k_change(r, ka, la, pos + step); k_mod2(r, FMI_MISMATCH, (uint8_t)b); KEMIT();
@@ -306,7 +356,7 @@ __device__ __forceinline__ bool k_branch(const KCtx &c, const KView &v, int dir,
     }
   } else if (last_err_pos == pos) {
     for (uint32_t b = 0; b < 4u; ++b) {
-      klf(c, v, k, l, b, ka, la);
+      ka = kb[b]; la = lb4[b];
       if (ka > la) continue;
       if (b != wp) {
         k_change(r, ka, la, pos); k_mod2(r, FMI_INSERTION, (uint8_t)b); KEMIT();
@@ -318,7 +368,7 @@ __device__ __forceinline__ bool k_branch(const KCtx &c, const KView &v, int dir,
   if (no_previous) {
     k_change(r, k, l, pos + step); KEMIT();
     for (uint32_t b = 0; b < 4u; ++b) {
-      klf(c, v, k, l, b, ka, la);
+      ka = kb[b]; la = lb4[b];
       if (ka > la) continue;
       if (b != wp) {
         k_change(r, ka, la, pos); k_mod2(r, FMI_INSERTION, (uint8_t)b); KEMIT();
@@ -485,8 +535,18 @@ __device__ __forceinline__ void k_turn(const KCtx &c, const KView &b, const KVie
 __device__ __forceinline__ void k_dwalk(const KCtx &c, const KView &v, int64_t k, int64_t l,
                                         int32_t i, int32_t step, int32_t count, uint8_t &z) {
   const int64_t nmax = c.ix->n;
+  const int32_t K = (int32_t)c.ix->kmer_k;
   uint8_t *D = c.D;
   while (count > 0) {
+    int64_t kt, lt;
+    if (K && k == 0 && l == nmax && count >= K && k_table(c, v, i, step, kt, lt)) {
+      // K steps from the whole index, none of them empty
+      for (int32_t t = 0; t < K; ++t) D[i + t * step] = z;
+      i += K * step;
+      count -= K;
+      k = kt; l = lt;
+      continue;
+    }
     if (c.text && k == l) {
       uint32_t q2;
       bool dies;

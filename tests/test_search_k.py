@@ -151,33 +151,43 @@ def kgpu(request, gpu_engine):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kmer", [0, 5])
 @pytest.mark.parametrize("expanded", [False, True])
-def test_gpu_search_k_golden(kgpu, expanded):
+def test_gpu_search_k_golden(kgpu, expanded, kmer):
+    """The golden records and anchors, with and without a k-mer table (the
+    exact searches and D walks from the whole index start with its entry)."""
     hp, g, ix, batch = kgpu
     if expanded:
         ix.expand()
-    cap = int(g["sk_cap"])
-    res = hp.search_k_batch(ix, batch, g["sk_fragsize"], list_cap=cap, anchor_cap=16)
-    fin = split(g, "sk_hits", "sk_counts")
-    anc = split(g, "sk_anchors", "sk_anchor_counts")
-    for i in range(batch.nreads):
-        for s in (0, 1):
-            assert bool(res.full[i, s]) == bool(g["sk_full"][i, s]), (i, s)
-            assert fo.hitsk_equal(res.final(i, s), fin[(i, s)]), (i, s)
-            assert fo.hitsk_equal(res.anchors(i, s), anc[(i, s)]), (i, s)
+    ix.kmer_table(kmer)
+    try:
+        cap = int(g["sk_cap"])
+        res = hp.search_k_batch(ix, batch, g["sk_fragsize"], list_cap=cap, anchor_cap=16)
+        fin = split(g, "sk_hits", "sk_counts")
+        anc = split(g, "sk_anchors", "sk_anchor_counts")
+        for i in range(batch.nreads):
+            for s in (0, 1):
+                assert bool(res.full[i, s]) == bool(g["sk_full"][i, s]), (i, s)
+                assert fo.hitsk_equal(res.final(i, s), fin[(i, s)]), (i, s)
+                assert fo.hitsk_equal(res.anchors(i, s), anc[(i, s)]), (i, s)
+    finally:
+        ix.kmer_table(0)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kmer", [0, 8])
 @pytest.mark.parametrize("expanded", [False, True])
-def test_gpu_search_k_vs_oracle(gpu_engine, expanded):
+def test_gpu_search_k_vs_oracle(gpu_engine, expanded, kmer):
     """Repeats, N bases, fragments of 2-5, list capacities that truncate;
-    on the expanded index one-row stretches run on the text."""
+    on the expanded index one-row stretches run on the text; with a k-mer
+    table, searches from the whole index start with its entry."""
     hp = gpu_engine
     g, _, _ = synth.genome(200_000, seed=75, repeat_frac=0.4, fam_len=(300, 2000),
                            divergence=(0.0, 0.05))
     ix = hp.Index.build(g, 16).to_device()
     if expanded:
         ix.expand()
+    ix.kmer_table(kmer)
     o = fo.Oracle(g, ratio=16)
     lens = np.random.default_rng(76).integers(8, 160, 800)
     reads, offs, _ = synth.reads(g, 800, seed=77, sub_rate=0.03, n_rate=0.003, lengths=lens)
