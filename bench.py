@@ -480,7 +480,7 @@ def main():
     ap.add_argument("--no-expand", action="store_true",
                     help="search the compact index only (no full SA / ISA / text on the device)")
     ap.add_argument("--tuning", default=None,
-                    help="wg_per_cu,chunk,refill_min of the persistent launch (skips the "
+                    help="wg_per_cu,chunk,refill_min[,loads] of the persistent launch (skips the "
                          "autotuner)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run kernels of known traffic for the profiler's FETCH_SIZE / "
@@ -587,13 +587,18 @@ def main():
 
     tuning = None
     if args.tuning:
-        w, c, r = (int(x) for x in args.tuning.split(","))
+        parts = [int(x) for x in args.tuning.split(",")]
+        w, c, r = parts[:3]
+        loads = parts[3] if len(parts) > 3 else hpgfm.LOADS_AUTO
         hpgfm.set_tuning(w, c, r)
-        tuning = {"wg_per_cu": w, "chunk": c, "refill_min": r, "source": "--tuning"}
+        hpgfm.set_load_mode(loads)
+        tuning = {"wg_per_cu": w, "chunk": c, "refill_min": r, "loads": loads,
+                  "source": "--tuning"}
     elif not args.no_autotune:
         from hpgfm.autotune import autotune
         best, best_ms, table = autotune(ix, batch, out, mode=mode, log=log)
         tuning = {"wg_per_cu": best[0], "chunk": best[1], "refill_min": best[2],
+                  "loads": best[3],
                   "tuned_ms": round(best_ms, 4), "points": len(table) - 1,
                   "plain_ms": round(table["plain"], 4)}
 

@@ -45,10 +45,16 @@ constexpr uint32_t kDefaultWgPerCu = 8;
 constexpr uint32_t kDefaultChunk = 64;
 constexpr uint32_t kDefaultRefill = 16;
 
+// An index whose device arrays exceed this spreads the search's accesses
+// over enough memory that issuing only the loads a step needs pays
+// (FMI_LOADS_AUTO).
+constexpr uint64_t kOwnLoadsBytes = 16ull << 30;
+
 struct Tuning {
   uint32_t wg_per_cu = 0;   // workgroups of 256 per CU; 0 = default
   uint32_t chunk = kDefaultChunk;            // tasks a wave takes per atomic
   uint32_t refill_min = kDefaultRefill;      // idle lanes before a wave refills
+  uint32_t loads = FMI_LOADS_AUTO;           // fmi_set_load_mode
 };
 Tuning g_tuning;
 std::mutex g_mu;
@@ -202,10 +208,12 @@ int search1_common(const fmi_index *ix, int mode, int strand_mask, const uint8_t
   }
   rc = begin_timed(st);
   if (rc) return rc;
+  const bool own = t.loads == FMI_LOADS_OWN ||
+                   (t.loads == FMI_LOADS_AUTO && fmi_index_device_bytes(ix) > kOwnLoadsBytes);
   hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, persistent, false,
                                      (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap,
                                      d_hits, d_counts, w.queue, w.stats, grid, t.chunk,
-                                     t.refill_min, st);
+                                     t.refill_min, own, st);
   return end_timed(st, e, "search1 launch");
 }
 
@@ -736,6 +744,20 @@ int fmi_get_tuning(uint32_t *wg_per_cu, uint32_t *chunk, uint32_t *refill_min) {
   return FMI_OK;
 }
 
+int fmi_set_load_mode(uint32_t mode) {
+  if (mode > FMI_LOADS_OWN) return fail(FMI_EINVAL, "load mode must be FMI_LOADS_AUTO, _ALL or _OWN");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuning.loads = mode;
+  return FMI_OK;
+}
+
+int fmi_get_load_mode(uint32_t *mode) {
+  if (!mode) return fail(FMI_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  *mode = g_tuning.loads;
+  return FMI_OK;
+}
+
 int fmi_locate_batch(const fmi_index *ix, int forward, const int64_t *d_k, const int64_t *d_l,
                      uint32_t nintervals, uint32_t cap, uint64_t *d_pos, uint32_t *d_nrows,
                      fmi_stream s) {
@@ -1020,7 +1042,7 @@ int fmi_search1_count_work(const fmi_index *ix, int mode, int strand_mask,
   hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, true, true,
                                      (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap, nullptr,
                                      counts, w.queue, w.stats, (uint32_t)cus * kDefaultWgPerCu,
-                                     kDefaultChunk, kDefaultRefill, st);
+                                     kDefaultChunk, kDefaultRefill, false, st);
   if (e != hipSuccess) { (void)hipFree(counts); return hip_fail(e, "count launch"); }
   unsigned long long st5[5] = {0, 0, 0, 0, 0};
   e = hipMemcpy(st5, w.stats, 40, hipMemcpyDeviceToHost);

@@ -20,7 +20,7 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
                           uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
                           uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
-                          uint32_t chunk, uint32_t refill_min, hipStream_t s);
+                          uint32_t chunk, uint32_t refill_min, bool own_loads, hipStream_t s);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t s);
 // Dense copy of the stored records of nslots (read, strand) slots, slot after
 // slot; *total (device) receives the record count.  scratch holds

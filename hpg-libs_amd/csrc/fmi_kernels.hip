@@ -203,6 +203,14 @@ struct S1Lane {
 constexpr uint32_t kTqNone = 0xFFFFFFFFu;
 constexpr uint32_t kTextRowTag = 0x80000000u;   // k's high word: k, l hold q
 
+// A value the compiler cannot see (no instruction is emitted): the
+// register is taken as written here.
+__device__ __forceinline__ uint4 opaque4() {
+  uint32_t a, b, c, d;
+  asm volatile("" : "=v"(a), "=v"(b), "=v"(c), "=v"(d));
+  return make_uint4(a, b, c, d);
+}
+
 // 16 read bases from byte wp of the 32 bytes c0, c1 (towards higher bytes
 // when up, else lower) as 2-bit groups in that order (w, physical symbols)
 // and their N flags (nb, bit i: a byte > 3).
@@ -235,7 +243,7 @@ __device__ __forceinline__ void window16(const uint4 &c0, const uint4 &c1, uint3
   }
 }
 
-template <bool MM_ONLY, bool COUNT, bool TEXT>
+template <bool MM_ONLY, bool COUNT, bool TEXT, bool OWN>
 struct S1 {
   const S1Params &P;
   S1Lane &L;
@@ -593,68 +601,76 @@ struct S1 {
       // [BWT[k] == s]); otherwise the block of l+1 too when it differs.
       two = ak != al && j2 != j1;
     }
-    // The read bases W[p], W[p+d], ... (rlim <= 32 of them) lie in three
-    // aligned 16-byte chunks c0 < c1 < c2; a chunk that holds none of them
-    // is replaced by the one of W[p] (no chunk without a read base of the
-    // lane is read).  W[p] is byte wp of the 48.
+    // The read bases W[p], W[p+d], ... (rlim <= 32 of them) lie in the
+    // aligned 16-byte chunk of W[p] and the one or two after it in the step's
+    // direction (x1, x2); a lane reads x1 / x2 only when they hold one of
+    // its bases.  On the compact index no step compares more than a k-mer
+    // (at most 15 bases), so x2 is never needed there.
     const uint8_t *pa = L.W + p;
     const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(pa) & 15u);
     const uint8_t *pal = pa - mis;
-    // (On the compact index no step compares more than a k-mer, at most 15
-    // bases, so two chunks: c2 is c1, W[p] is byte wp of the 32.)
-    const uint8_t *c0, *c1, *c2;
-    uint32_t wp;
-    if (d > 0) {
-      c0 = pal;
-      c1 = mis + rlim - 1u >= 16u ? pal + 16 : pal;
-      c2 = mis + rlim - 1u >= 32u ? pal + 32 : pal;
-      wp = mis;
-    } else if (TEXT) {
-      c2 = pal;
-      c1 = rlim - 1u > mis ? pal - 16 : pal;
-      c0 = rlim - 1u > mis + 16u ? pal - 32 : pal;
-      wp = 32u + mis;
-    } else {
-      c1 = c2 = pal;
-      c0 = rlim - 1u > mis ? pal - 16 : pal;
-      wp = 16u + mis;
-    }
-    // First line: the occupancy block of row k, or the aligned 16 bytes that
-    // hold the word(s) the kind reads (and the next 16 bytes when text words
-    // continue there), the rest pointing at the same 16 bytes.  Second line
-    // (read only by the lanes whose LF query needs it): the block of row l+1.
-    const uint8_t *f0, *f1, *f2, *f3, *g0 = nullptr;
+    const uint8_t *x1 = d > 0 ? pal + 16 : pal - 16;
+    const uint8_t *x2 = d > 0 ? pal + 32 : pal - 32;
+    const bool n1 = d > 0 ? mis + rlim - 1u >= 16u : rlim - 1u > mis;
+    const bool n2 = TEXT && (d > 0 ? mis + rlim - 1u >= 32u : rlim - 1u > mis + 16u);
+    // First line: the occupancy block of row k (four 16-byte loads), or the
+    // aligned 16 bytes that hold the word(s) the kind reads (and the next 16
+    // bytes when text words continue there).  Second line (the lanes whose
+    // LF query needs it): the block of row l+1.
+    const uint8_t *f0, *g0 = nullptr;
     if (kind == F_LF) {
       const OccBlock *blk = L.dsel ? P.ix.d[1].blk : P.ix.d[0].blk;
       f0 = reinterpret_cast<const uint8_t *>(blk + j1);
       g0 = reinterpret_cast<const uint8_t *>(blk + j2);
-      f1 = f0 + 16; f2 = f0 + 32; f3 = f0 + 48;
     } else {
-      const uint8_t *w0 = kind == F_KPREP ? c2 : reinterpret_cast<const uint8_t *>(a0);
-      const uint32_t wm = (uint32_t)(reinterpret_cast<uintptr_t>(w0) & 15u);
-      f0 = w0 - wm;
-      f1 = next ? f0 + 16 : f0;
-      f2 = f3 = f0;
+      const uint8_t *w0 = kind == F_KPREP ? pal : reinterpret_cast<const uint8_t *>(a0);
+      f0 = w0 - (reinterpret_cast<uintptr_t>(w0) & 15u);
     }
+    const bool lf = kind == F_LF;
 
-    // ---- the loads: one batch, no value used before the last is issued;
-    // the second block's loads come last, in the branch of the lanes that
-    // need it, so a wait the compiler puts there is the batch's one wait
-    const uint4 rc0 = *reinterpret_cast<const uint4 *>(c0);
-    const uint4 rc1 = *reinterpret_cast<const uint4 *>(c1);
-    uint4 rc2;
-    if (TEXT) rc2 = *reinterpret_cast<const uint4 *>(c2);
+    // ---- the loads: one batch, no value used before the last is issued.
+    // The loads only some lanes make come last, each group in the branch of
+    // its lanes; a lane that skips one holds an opaque value there (no
+    // constant for the compiler to fold into the branch, where it would
+    // wait for the load; no instruction either), and those bytes or words
+    // are never compared.
+    //
+    // Two forms of the batch, chosen at launch (results are the same): with
+    // OWN every lane issues only the loads its step needs; otherwise every
+    // lane issues the first seven, pointing those it does not need at the
+    // chunk of W[p] or the first 16 bytes it reads (more L1 requests, fewer
+    // instructions and branches — faster while the index's lines come from
+    // the caches, slower once its accesses spread over tens of GB).
+    const uint4 rp = *reinterpret_cast<const uint4 *>(pal);
     const uint4 fx0 = *reinterpret_cast<const uint4 *>(f0);
-    const uint4 fx1 = *reinterpret_cast<const uint4 *>(f1);
-    const uint4 fx2 = *reinterpret_cast<const uint4 *>(f2);
-    const uint4 fx3 = *reinterpret_cast<const uint4 *>(f3);
-    uint4 gx0, gx1, gx2, gx3;
+    uint4 fx1 = opaque4(), fx2 = opaque4(), fx3 = opaque4(), rx1 = opaque4(), rx2 = opaque4();
+    uint4 gx0 = opaque4(), gx1 = opaque4(), gx2 = opaque4(), gx3 = opaque4();
+    if (OWN) {
+      if (lf || next) fx1 = *reinterpret_cast<const uint4 *>(f0 + 16);
+      if (lf) {
+        fx2 = *reinterpret_cast<const uint4 *>(f0 + 32);
+        fx3 = *reinterpret_cast<const uint4 *>(f0 + 48);
+      }
+      if (n1) rx1 = *reinterpret_cast<const uint4 *>(x1);
+      if (n2) rx2 = *reinterpret_cast<const uint4 *>(x2);
+    } else {
+      fx1 = *reinterpret_cast<const uint4 *>(lf || next ? f0 + 16 : f0);
+      fx2 = *reinterpret_cast<const uint4 *>(lf ? f0 + 32 : f0);
+      fx3 = *reinterpret_cast<const uint4 *>(lf ? f0 + 48 : f0);
+      rx1 = *reinterpret_cast<const uint4 *>(n1 ? x1 : pal);
+      if (TEXT) rx2 = *reinterpret_cast<const uint4 *>(n2 ? x2 : pal);
+    }
     if (two) {
       gx0 = *reinterpret_cast<const uint4 *>(g0);
       gx1 = *reinterpret_cast<const uint4 *>(g0 + 16);
       gx2 = *reinterpret_cast<const uint4 *>(g0 + 32);
       gx3 = *reinterpret_cast<const uint4 *>(g0 + 48);
     }
+    // the 48 (32 on the compact index) bytes in address order; W[p] is byte wp
+    const uint4 rc0 = d > 0 ? rp : (TEXT ? rx2 : rx1);
+    const uint4 rc1 = d > 0 ? rx1 : (TEXT ? rx1 : rp);
+    const uint4 rc2 = d > 0 ? rx2 : rp;
+    const uint32_t wp = d > 0 ? mis : (TEXT ? 32u : 16u) + mis;
 
     // ---- the read window: 32 bases from W[p] in feed order as 2-bit
     // groups (rw, physical symbols) and their N flags (rb, bit i)
@@ -827,14 +843,14 @@ __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
-template <bool PERSISTENT, bool MM_ONLY, bool COUNT, bool TEXT>
+template <bool PERSISTENT, bool MM_ONLY, bool COUNT, bool TEXT, bool OWN>
 __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
   S1Lane L;
   L.phase = PH_IDLE;
   L.nlook = 0;
   L.ntext = 0;
   L.nkmer = 0;
-  S1<MM_ONLY, COUNT, TEXT> m{P, L};
+  S1<MM_ONLY, COUNT, TEXT, OWN> m{P, L};
   unsigned long long nblk = 0, nq = 0;
   const uint32_t lane = lane_id();
 
@@ -1286,7 +1302,7 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
                           uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
                           uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
-                          uint32_t chunk, uint32_t refill_min, hipStream_t s) {
+                          uint32_t chunk, uint32_t refill_min, bool own_loads, hipStream_t s) {
   const uint32_t ntasks = nreads;   // one task per read (both strands in turn)
   S1Params P{ix, reads, offsets, hits, counts, queue, stats, ntasks, strand_mask, cap, chunk,
              refill_min};
@@ -1296,10 +1312,12 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
     if (grid_persistent == 0 || chunk == 0 || refill_min == 0 || refill_min > 64)
       return hipErrorInvalidValue;
     const dim3 g(grid_persistent), b(256);
-#define FMI_S1_LAUNCH(MM, CNT)                                                             \
-  do {                                                                                     \
-    if (text) hipLaunchKernelGGL((search1_kernel<true, MM, CNT, true>), g, b, 0, s, P);    \
-    else hipLaunchKernelGGL((search1_kernel<true, MM, CNT, false>), g, b, 0, s, P);        \
+#define FMI_S1_LAUNCH(MM, CNT)                                                                \
+  do {                                                                                        \
+    if (!text) hipLaunchKernelGGL((search1_kernel<true, MM, CNT, false, false>), g, b, 0, s, P); \
+    else if (own_loads)                                                                       \
+      hipLaunchKernelGGL((search1_kernel<true, MM, CNT, true, true>), g, b, 0, s, P);         \
+    else hipLaunchKernelGGL((search1_kernel<true, MM, CNT, true, false>), g, b, 0, s, P);     \
   } while (0)
     if (mm_only) {
       if (count) FMI_S1_LAUNCH(true, true);
@@ -1312,11 +1330,11 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
   } else {
     const dim3 g((ntasks + 255) / 256), b(256);
     if (mm_only) {
-      if (text) hipLaunchKernelGGL((search1_kernel<false, true, false, true>), g, b, 0, s, P);
-      else hipLaunchKernelGGL((search1_kernel<false, true, false, false>), g, b, 0, s, P);
+      if (text) hipLaunchKernelGGL((search1_kernel<false, true, false, true, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((search1_kernel<false, true, false, false, false>), g, b, 0, s, P);
     } else {
-      if (text) hipLaunchKernelGGL((search1_kernel<false, false, false, true>), g, b, 0, s, P);
-      else hipLaunchKernelGGL((search1_kernel<false, false, false, false>), g, b, 0, s, P);
+      if (text) hipLaunchKernelGGL((search1_kernel<false, false, false, true, true>), g, b, 0, s, P);
+      else hipLaunchKernelGGL((search1_kernel<false, false, false, false, false>), g, b, 0, s, P);
     }
   }
   if (!count && ix.text != nullptr && cap > 0) {

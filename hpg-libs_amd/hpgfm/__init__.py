@@ -48,7 +48,7 @@ EXPORTS = [
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
     "fmi_stream_destroy", "fmi_stream_sync", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
-    "fmi_set_tuning", "fmi_get_tuning", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
+    "fmi_set_tuning", "fmi_get_tuning", "fmi_set_load_mode", "fmi_get_load_mode", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
     "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
@@ -124,6 +124,8 @@ def _bind(lib):
         "fmi_search1_batch_plain": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_set_tuning": ([_u32, _u32, _u32], _i),
         "fmi_get_tuning": ([C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)], _i),
+        "fmi_set_load_mode": ([_u32], _i),
+        "fmi_get_load_mode": ([C.POINTER(_u32)], _i),
         "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_locate_hits": ([_p, _i, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
@@ -297,6 +299,20 @@ def get_tuning():
     a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
     check(lib().fmi_get_tuning(C.byref(a), C.byref(b), C.byref(c)))
     return int(a.value), int(b.value), int(c.value)
+
+
+LOADS_AUTO, LOADS_ALL, LOADS_OWN = 0, 1, 2
+
+
+def set_load_mode(mode):
+    """fmi_set_load_mode: LOADS_AUTO, LOADS_ALL or LOADS_OWN."""
+    check(lib().fmi_set_load_mode(mode))
+
+
+def get_load_mode():
+    m = C.c_uint32()
+    check(lib().fmi_get_load_mode(C.byref(m)))
+    return int(m.value)
 
 
 class Index:

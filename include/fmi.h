@@ -272,6 +272,19 @@ int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
  * selects the default.  Results never depend on the tuning. */
 int fmi_set_tuning(uint32_t wg_per_cu, uint32_t chunk, uint32_t refill_min);
 int fmi_get_tuning(uint32_t *wg_per_cu, uint32_t *chunk, uint32_t *refill_min);
+/* How the text-mode 1-error search issues an iteration's loads (the
+ * autotuner writes it; results never depend on it): FMI_LOADS_ALL — every
+ * lane issues the same batch, pointing the loads its step does not need at
+ * data it reads anyway (fewer instructions); FMI_LOADS_OWN — each lane
+ * issues only the loads its step needs (fewer memory requests, faster once
+ * the index spreads over tens of GB); FMI_LOADS_AUTO (the default) — OWN for
+ * an index of more than 16 GiB on the device, else ALL.  No reference
+ * counterpart. */
+#define FMI_LOADS_AUTO 0u
+#define FMI_LOADS_ALL 1u
+#define FMI_LOADS_OWN 2u
+int fmi_set_load_mode(uint32_t mode);
+int fmi_get_load_mode(uint32_t *mode);
 
 /* ------------------------------------------------------------ locate */
 /* get_SA (search/runtime.h:87, getScompValue search/csafm.h:208-235) for

@@ -62,6 +62,11 @@ def test_tuning_range():
     hpgfm.set_tuning(0, 0, 0)
     wg, chunk, refill = hpgfm.get_tuning()
     assert wg > 0 and chunk > 0 and 1 <= refill <= 64
+    with pytest.raises(hpgfm.FmiError):
+        hpgfm.set_load_mode(3)
+    for m in (hpgfm.LOADS_OWN, hpgfm.LOADS_ALL, hpgfm.LOADS_AUTO):
+        hpgfm.set_load_mode(m)
+        assert hpgfm.get_load_mode() == m
 
 
 def test_read_validation():

@@ -59,7 +59,8 @@ def test_textmode_search1_golden(tgold, plain, mode):
 def test_textmode_equals_block_mode(gpu_engine):
     """Reads with N bases, ragged lengths, dense repeats: the expanded and
     the plain index give identical counts and records, at a capacity that
-    truncates, on both strands and each strand alone."""
+    truncates, on both strands and each strand alone, with either way of
+    issuing an iteration's loads (fmi_set_load_mode)."""
     hp = gpu_engine
     g, _, _ = synth.genome(600_000, seed=91, repeat_frac=0.4, fam_len=(300, 3000),
                            divergence=(0.0, 0.05), copies_per_family=12)
@@ -68,15 +69,21 @@ def test_textmode_equals_block_mode(gpu_engine):
     lens = np.random.default_rng(92).integers(0, 251, 4000)
     reads, offs, _ = synth.reads(g, 4000, seed=93, sub_rate=0.015, n_rate=0.002, lengths=lens)
     batch = hp.ReadBatch(reads, offs)
-    for m in (hp.MODE_EDIT, hp.MODE_MISMATCH):
-        for mask in (3, 1, 2):
-            c1, h1 = hp.search1_batch(ix=a, batch=batch, cap=6, mode=m, strand_mask=mask)
-            c2, h2 = hp.search1_batch(ix=b, batch=batch, cap=6, mode=m, strand_mask=mask)
-            assert np.array_equal(c1, c2), (m, mask)
-            n = np.minimum(c1, 6)
-            for i in range(len(lens)):
-                for s in (0, 1):
-                    assert fo.hits_equal(h1[i, s, :n[i, s]], h2[i, s, :n[i, s]]), (m, mask, i, s)
+    try:
+        for lm in (hp.LOADS_ALL, hp.LOADS_OWN):
+            hp.set_load_mode(lm)
+            for m in (hp.MODE_EDIT, hp.MODE_MISMATCH):
+                for mask in (3, 1, 2):
+                    c1, h1 = hp.search1_batch(ix=a, batch=batch, cap=6, mode=m, strand_mask=mask)
+                    c2, h2 = hp.search1_batch(ix=b, batch=batch, cap=6, mode=m, strand_mask=mask)
+                    assert np.array_equal(c1, c2), (lm, m, mask)
+                    n = np.minimum(c1, 6)
+                    for i in range(len(lens)):
+                        for s in (0, 1):
+                            assert fo.hits_equal(h1[i, s, :n[i, s]], h2[i, s, :n[i, s]]), \
+                                (lm, m, mask, i, s)
+    finally:
+        hp.set_load_mode(hp.LOADS_AUTO)
     wa = hp.search1_count_work(a, batch, cap=6)
     wb = hp.search1_count_work(b, batch, cap=6)
     assert wa[2] == 0 and wa[3] == 0
