@@ -403,21 +403,19 @@ struct S1 {
               (uint8_t)((bi - 1) >> 1), (int16_t)L.i);
   }
 
-  __device__ __forceinline__ void start(uint32_t task) {
-    L.read = task;
-    const uint64_t off = P.offsets[L.read];
-    L.W = P.reads + off;
-    L.len = (int32_t)(P.offsets[L.read + 1] - off);
-    const uint32_t strand = (P.strand_mask == 2u) ? 1u : 0u;
-    if (P.strand_mask != 3u) P.counts[2 * L.read + (strand ^ 1u)] = 0;
-    // The k-mer table index of both passes' first k bases (pass 1: W[0],
-    // W[1], ...; pass 0: W[len-1], W[len-2], ...), read here with the task
-    // (two aligned 16-byte chunks at each end of the read, each holding one
-    // of those bases), so that no iteration waits for them later.
-    L.kc[0] = L.kc[1] = 0u;
+  // What a lane needs to start a task: the read's offset and length and the
+  // k-mer table index of both passes' first k bases (pass 1: W[0], W[1],
+  // ...; pass 0: W[len-1], W[len-2], ...; kKReady unless there is no table,
+  // the read is shorter than k or an N is among them), from two aligned
+  // 16-byte chunks at each end of the read, each holding one of those bases.
+  __device__ __forceinline__ void task_info(uint32_t task, uint64_t &off, uint32_t &len,
+                                            uint32_t &kc0, uint32_t &kc1) const {
+    off = P.offsets[task];
+    len = (uint32_t)(P.offsets[task + 1] - off);
+    kc0 = kc1 = 0u;
     const uint32_t K = P.ix.kmer_k;
-    if (K && L.len >= (int32_t)K) {
-      const uint8_t *a = L.W, *z = L.W + (L.len - 1);
+    if (K && len >= K) {
+      const uint8_t *a = P.reads + off, *z = P.reads + off + (len - 1u);
       const uint32_t ma = (uint32_t)(reinterpret_cast<uintptr_t>(a) & 15u);
       const uint32_t mz = (uint32_t)(reinterpret_cast<uintptr_t>(z) & 15u);
       const uint8_t *a0 = a - ma, *z1 = z - mz;
@@ -430,11 +428,29 @@ struct S1 {
       const uint32_t kmask = (1u << (2u * K)) - 1u, nmask = (1u << K) - 1u;
       uint32_t w, nb;
       window16(ca0, ca1, ma, true, w, nb);
-      L.kc[1] = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
+      kc1 = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
       window16(cz0, cz1, 16u + mz, false, w, nb);
-      L.kc[0] = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
+      kc0 = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
     }
+  }
+
+  __device__ __forceinline__ void start_with(uint32_t task, uint64_t off, uint32_t len,
+                                             uint32_t kc0, uint32_t kc1) {
+    L.read = task;
+    L.W = P.reads + off;
+    L.len = (int32_t)len;
+    L.kc[0] = kc0;
+    L.kc[1] = kc1;
+    const uint32_t strand = (P.strand_mask == 2u) ? 1u : 0u;
+    if (P.strand_mask != 3u) P.counts[2 * L.read + (strand ^ 1u)] = 0;
     start_strand(strand);
+  }
+
+  __device__ __forceinline__ void start(uint32_t task) {
+    uint64_t off;
+    uint32_t len, kc0, kc1;
+    task_info(task, off, len, kc0, kc1);
+    start_with(task, off, len, kc0, kc1);
   }
 
   // The 32 text bases that precede suffix q in the pass's direction, in
@@ -867,7 +883,15 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
     // popcount); a new chunk is taken with one atomic when it runs dry.
     // Lanes are refilled only once refill_min of them are idle (or the whole
     // wave is), so the task set-up runs for many lanes at a time.
-    uint32_t wq_next = 0, wq_end = 0;
+    //
+    // The next (up to) 64 tasks of the chunk are read ahead, one per lane
+    // (their offsets and k-mer indexes, task_info): lane i holds task
+    // wbase + i, and a lane that takes a task gets it from its holder with
+    // register shuffles, so a refill waits for no memory except once per 64
+    // tasks.
+    uint32_t wq_next = 0, wq_end = 0, wbase = 0, wend = 0;
+    uint64_t h_off = 0;
+    uint32_t h_len = 0, h_kc0 = 0, h_kc1 = 0;
     bool more = true;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     while (true) {
@@ -883,10 +907,22 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
           wq_next = base;
           wq_end = min(base + P.chunk, P.ntasks);
         }
+        if (wq_next >= wend) {       // read the next 64 tasks ahead
+          wbase = wq_next;
+          wend = min(wbase + 64u, wq_end);
+          if (wbase + lane < wend) m.task_info(wbase + lane, h_off, h_len, h_kc0, h_kc1);
+        }
         const uint32_t nidle = (uint32_t)__popcll(idle);
         const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
-        const uint32_t avail = wq_end - wq_next;
-        if (L.phase == PH_IDLE && rank < avail) m.start(wq_next + rank);
+        const uint32_t avail = wend - wq_next;
+        const int src = (int)((wq_next - wbase + rank) & 63u);   // the holder of task wq_next + rank
+        const uint32_t off_lo = __shfl((uint32_t)h_off, src);
+        const uint32_t off_hi = __shfl((uint32_t)(h_off >> 32), src);
+        const uint32_t len = __shfl(h_len, src);
+        const uint32_t kc0 = __shfl(h_kc0, src);
+        const uint32_t kc1 = __shfl(h_kc1, src);
+        if (L.phase == PH_IDLE && rank < avail)
+          m.start_with(wq_next + rank, ((uint64_t)off_hi << 32) | off_lo, len, kc0, kc1);
         wq_next += min(avail, nidle);
         idle = __ballot(L.phase == PH_IDLE);
       }
