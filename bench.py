@@ -466,8 +466,8 @@ def main():
     ap.add_argument("--sa-ratio", type=int, default=16)
     ap.add_argument("--cap", type=int, default=32)
     ap.add_argument("--kmer", type=int, default=-1,
-                    help="k of the k-mer interval table (0: none; default: the smallest k "
-                         "with 4^k >= n, 4..15)")
+                    help="k of the k-mer interval table (0: none; default: two more than "
+                         "the smallest k with 4^k >= n, at most 15)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reads", type=int, default=0,
                     help="CPU baseline sample size (0: all reads, capped by time)")
@@ -516,11 +516,13 @@ def main():
         args.reads = pre["reads"]
     n_bp = int(args.genome_mbp * 1_000_000)
     if args.kmer < 0:
-        # the smallest k with 4^k >= n (most k-mers of the text then name
-        # one row or a few), at most 15 (8 GiB per direction)
-        args.kmer = 4
-        while args.kmer < 15 and 4 ** args.kmer < n_bp:
-            args.kmer += 1
+        # two bases past the smallest k with 4^k >= n (a k-mer of the text
+        # then names one row, and a k-mer absent from it ends a search at
+        # the lookup), at most 15 (8 GiB per direction)
+        k = 4
+        while 4 ** k < n_bp:
+            k += 1
+        args.kmer = min(15, k + 2)
     t0 = time.time()
     g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=pre["repeat_frac"],
                            fam_len=(300, 6000), divergence=pre["div"], n_frac=pre["n_frac"])
