@@ -439,7 +439,7 @@ def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000
 # (preset, reads, read length, text mode, k-mer table k) -> calibrated
 # FETCH + WRITE summary of that search1 launch under profiles/
 TRAFFIC_FILES = {
-    ("c2", 1_000_000, 150, True, 13): "search1_textmode_kmer13_64Mbp_1Mx150bp_pmc_traffic.json",
+    ("c2", 1_000_000, 150, True, 15): "search1_textmode_kmer15_64Mbp_1Mx150bp_pmc_traffic.json",
     ("t3g", 2_000_000, 150, True, 15): "search1_textmode_kmer15_3100Mbp_2Mx150bp_pmc_traffic.json",
 }
 

@@ -43,6 +43,9 @@ def test_kmer_golden_search1(kgold, k, expanded):
             assert fo.hits_equal(hits[i, s, :c], want[(i, s)]), (k, i, s)
     w = hp.search1_count_work(ix, batch, cap=cap)
     assert w[4] > 0
+    for bad in (3, 16):                  # k is 0 or 4..15
+        with pytest.raises(hp.FmiError):
+            ix.kmer_table(bad)
     ix.kmer_table(0)
     w0 = hp.search1_count_work(ix, batch, cap=cap)
     assert w0[4] == 0 and w0[1] > w[1]
