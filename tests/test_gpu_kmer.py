@@ -51,6 +51,30 @@ def test_kmer_golden_search1(kgold, k, expanded):
     assert w0[4] == 0 and w0[1] > w[1]
 
 
+@pytest.mark.parametrize("name", ["golden_small", "golden_repeats"])
+def test_kmer_table_before_expand(gpu_engine, name, request):
+    """A table built on the compact index, then the index expanded: the
+    golden 1-error records, and the same again after the table is rebuilt
+    on the expanded index."""
+    hp = gpu_engine
+    g = request.getfixturevalue(name)
+    ix = hp.Index.build(g["genome"], int(g["ratio"])).to_device()
+    batch = hp.ReadBatch(g["reads"], g["offsets"])
+    want = split_golden_hits(g)
+    cap = int(g["cap"])
+    ix.kmer_table(6)
+    ix.expand()
+    for _ in range(2):
+        counts, hits = hp.search1_batch(ix, batch, cap=cap, mode=hp.MODE_EDIT)
+        assert np.array_equal(counts, g["s1_counts"])
+        for i in range(batch.nreads):
+            for s in (0, 1):
+                assert fo.hits_equal(hits[i, s, :int(counts[i, s])], want[(i, s)]), (i, s)
+        ix.kmer_table(7)
+    batch.free()
+    ix.close()
+
+
 def test_kmer_entries_and_oracle_with_n(gpu_engine):
     hp = gpu_engine
     g, _, _ = synth.genome(150_000, seed=61, repeat_frac=0.3, fam_len=(200, 1500),
