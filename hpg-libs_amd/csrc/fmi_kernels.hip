@@ -702,19 +702,26 @@ struct S1 {
     {
       const uint32_t v[12] = {rc0.x, rc0.y, rc0.z, rc0.w, rc1.x, rc1.y,
                               rc1.z, rc1.w, rc2.x, rc2.y, rc2.z, rc2.w};
+      uint32_t any = 0u;                            // some byte > 3 (an N)
 #pragma unroll
       for (uint32_t j = 0; j < 12u; ++j) {
         uint32_t t = v[j] & 0x03030303u;            // 4 codes -> 8 bits
         t = (t | (t >> 6)) & 0x000F000Fu;
         t = (t | (t >> 12)) & 0xFFu;
-        uint32_t z = (v[j] >> 2) & 0x3F3F3F3Fu;     // a byte > 3 -> its bit 0
-        z = z | (z >> 3);
-        z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
-        z = (z | (z >> 7)) & 0x00030003u;
-        z = (z | (z >> 14)) & 0xFu;
         if (j < 8u) clo |= (uint64_t)t << (8u * j);
         else chi |= t << (8u * (j - 8u));
-        nfl |= (uint64_t)z << (4u * j);
+        any |= v[j] & 0xFCFCFCFCu;
+      }
+      if (any) {                                    // the N flags, only where needed
+#pragma unroll
+        for (uint32_t j = 0; j < 12u; ++j) {
+          uint32_t z = (v[j] >> 2) & 0x3F3F3F3Fu;   // a byte > 3 -> its bit 0
+          z = z | (z >> 3);
+          z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
+          z = (z | (z >> 7)) & 0x00030003u;
+          z = (z | (z >> 14)) & 0xFu;
+          nfl |= (uint64_t)z << (4u * j);
+        }
       }
     }
     if (d > 0) {                     // bytes wp .. wp+31
