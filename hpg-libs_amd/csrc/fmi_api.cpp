@@ -131,7 +131,16 @@ thread_local size_t g_used = 0;
 
 thread_local hipEvent_t g_single0 = nullptr, g_single1 = nullptr;
 
+// A stream being captured into a graph (fmi_graph_begin) records no timing
+// events: the launches are not run then.
+bool capturing(hipStream_t s) {
+  if (!s) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 int begin_timed(hipStream_t s) {
+  if (capturing(s)) return FMI_OK;
   if (g_region) {
     if (g_used == g_pool.size()) {
       hipEvent_t a, b;
@@ -156,6 +165,7 @@ int begin_timed(hipStream_t s) {
 
 int end_timed(hipStream_t s, hipError_t launch_err, const char *what) {
   if (launch_err != hipSuccess) return hip_fail(launch_err, what);
+  if (capturing(s)) return FMI_OK;
   HIP_TRY(hipEventRecord(g_ev1, s), "hipEventRecord");
   g_timed = true;
   return FMI_OK;
@@ -571,6 +581,39 @@ int fmi_stream_destroy(fmi_stream s) {
 
 int fmi_stream_sync(fmi_stream s) {
   HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(s)), "hipStreamSynchronize");
+  return FMI_OK;
+}
+
+int fmi_graph_begin(fmi_stream s) {
+  if (!s) return fail(FMI_EINVAL, "capture needs a stream of its own (fmi_stream_create)");
+  HIP_TRY(hipStreamBeginCapture(static_cast<hipStream_t>(s), hipStreamCaptureModeRelaxed),
+          "hipStreamBeginCapture");
+  return FMI_OK;
+}
+
+int fmi_graph_end(fmi_stream s, fmi_graph *g) {
+  if (!s || !g) return fail(FMI_EINVAL, "NULL argument");
+  *g = nullptr;
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamEndCapture(static_cast<hipStream_t>(s), &graph), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  *g = exec;
+  return FMI_OK;
+}
+
+int fmi_graph_launch(fmi_graph g, fmi_stream s) {
+  if (!g) return fail(FMI_EINVAL, "NULL graph");
+  HIP_TRY(hipGraphLaunch(static_cast<hipGraphExec_t>(g), static_cast<hipStream_t>(s)),
+          "hipGraphLaunch");
+  return FMI_OK;
+}
+
+int fmi_graph_destroy(fmi_graph g) {
+  if (!g) return FMI_OK;
+  HIP_TRY(hipGraphExecDestroy(static_cast<hipGraphExec_t>(g)), "hipGraphExecDestroy");
   return FMI_OK;
 }
 

@@ -46,7 +46,8 @@ EXPORTS = [
     "fmi_index_device_bytes", "fmi_index_to_device", "fmi_index_get_bwt",
     "fmi_index_set_karyotype", "fmi_device_count", "fmi_device_alloc", "fmi_device_free",
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
-    "fmi_stream_destroy", "fmi_stream_sync", "fmi_last_kernel_ms", "fmi_exact_batch",
+    "fmi_stream_destroy", "fmi_stream_sync", "fmi_graph_begin", "fmi_graph_end",
+    "fmi_graph_launch", "fmi_graph_destroy", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
     "fmi_set_tuning", "fmi_get_tuning", "fmi_set_load_mode", "fmi_get_load_mode", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
     "fmi_gather_bench",
@@ -114,6 +115,10 @@ def _bind(lib):
         "fmi_stream_create": ([C.POINTER(_p)], _i),
         "fmi_stream_destroy": ([_p], _i),
         "fmi_stream_sync": ([_p], _i),
+        "fmi_graph_begin": ([C.c_void_p], _i),
+        "fmi_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], _i),
+        "fmi_graph_launch": ([C.c_void_p, C.c_void_p], _i),
+        "fmi_graph_destroy": ([C.c_void_p], _i),
         "fmi_last_kernel_ms": ([C.POINTER(C.c_float)], _i),
         "fmi_timing_begin": ([], _i),
         "fmi_timing_end": ([C.POINTER(_u32), C.POINTER(C.c_double), C.POINTER(C.c_float),
@@ -260,6 +265,40 @@ class Stream:
 
 def sync(stream=None):
     check(lib().fmi_stream_sync(stream))
+
+
+class Graph:
+    """A chain of launches captured once on a Stream and replayed with one
+    call (fmi_graph_*):
+
+        with hpgfm.Graph(stream) as g:
+            hpgfm.search1_launch(ix, batch, out, stream=stream.handle)
+        g.launch()            # the whole chain again, on the same buffers
+    """
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.handle = None
+
+    def __enter__(self):
+        check(lib().fmi_graph_begin(self.stream.handle))
+        return self
+
+    def __exit__(self, *exc):
+        h = C.c_void_p()
+        rc = lib().fmi_graph_end(self.stream.handle, C.byref(h))
+        if exc[0] is None:
+            check(rc)
+            self.handle = h
+        return False
+
+    def launch(self, stream=None):
+        check(lib().fmi_graph_launch(self.handle, (stream or self.stream).handle))
+
+    def close(self):
+        if self.handle is not None:
+            lib().fmi_graph_destroy(self.handle)
+            self.handle = None
 
 
 def last_kernel_ms():

@@ -188,6 +188,20 @@ int fmi_memset_d(void *d_dst, int value, size_t bytes, fmi_stream s);
 int fmi_stream_create(fmi_stream *s);
 int fmi_stream_destroy(fmi_stream s);
 int fmi_stream_sync(fmi_stream s);
+/* Launch chains as HIP graphs: between fmi_graph_begin and fmi_graph_end
+ * the batch calls made on stream s (one created with fmi_stream_create) are
+ * recorded, not run; fmi_graph_launch replays the whole chain on a stream
+ * with one call.  Each call must have run once on that stream before the
+ * capture (it sizes the stream's work counters and scratch then), and a
+ * replay reads and writes the buffers the calls named — their contents may
+ * change between replays, not their addresses or sizes.  The library's
+ * counter resets are kernels, never memset nodes.  No reference
+ * counterpart (the reference launches synchronously, gpu.cu:118-203). */
+typedef void *fmi_graph;
+int fmi_graph_begin(fmi_stream s);
+int fmi_graph_end(fmi_stream s, fmi_graph *g);
+int fmi_graph_launch(fmi_graph g, fmi_stream s);
+int fmi_graph_destroy(fmi_graph g);
 /* Milliseconds the last kernel launched by the calling thread took on the
  * GPU (HIP events around that launch; valid after fmi_stream_sync). */
 int fmi_last_kernel_ms(float *ms);
