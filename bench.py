@@ -102,18 +102,18 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     d_full = hpgfm.DeviceBuffer(max(2 * n2, 1))
     L = hpgfm.lib()
 
-    def run_exact():
+    def run_exact(st=None):
         hpgfm.check(L.fmi_exact_batch(ix.h, hpgfm.VIEW_BACKWARD, 0, b0.d_reads.ptr,
-                                      b0.d_offsets.ptr, b0.nreads, ex_out[0].ptr, None, None))
+                                      b0.d_offsets.ptr, b0.nreads, ex_out[0].ptr, None, st))
         hpgfm.check(L.fmi_exact_batch(ix.h, hpgfm.VIEW_BACKWARD_REV, 1, b0.d_reads.ptr,
-                                      b0.d_offsets.ptr, b0.nreads, ex_out[1].ptr, None, None))
+                                      b0.d_offsets.ptr, b0.nreads, ex_out[1].ptr, None, st))
 
-    def run_s1():
-        hpgfm.search1_launch(ix, b1, s1, mode=hpgfm.MODE_EDIT)
+    def run_s1(st=None):
+        hpgfm.search1_launch(ix, b1, s1, mode=hpgfm.MODE_EDIT, stream=st)
 
-    def run_k():
+    def run_k(st=None):
         hpgfm.search_k_launch(ix, b2, d_fs, list_cap, anchor_cap, d_fin, d_nfin, d_anc, d_nanc,
-                              d_full)
+                              d_full, stream=st)
 
     parts = {}
     for name, fn in (("exact", run_exact), ("search1", run_s1), ("search_k", run_k)):
@@ -133,8 +133,29 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     for _ in range(args.steps):
         run_exact(); run_s1(); run_k()
     hpgfm.sync()
+    eager = (time.perf_counter() - t0) / args.steps
+    log(f"mixed budgets, launched one by one: {eager * 1000:.2f} ms per step, "
+        f"{n / eager / 1e6:.2f} M reads/s")
+    # The timed step: the same launches captured once as a graph (fmi_graph_*)
+    # on a stream of their own and replayed with one call per step.
+    st = hpgfm.Stream()
+    run_exact(st.handle); run_s1(st.handle); run_k(st.handle)   # sizes its scratch
+    st.sync()
+    graph = hpgfm.Graph(st)
+    with graph:
+        run_exact(st.handle); run_s1(st.handle); run_k(st.handle)
+    for _ in range(args.warmup):
+        graph.launch()
+    st.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    st.sync()
     wall = (time.perf_counter() - t0) / args.steps
-    log(f"mixed budgets: {wall * 1000:.2f} ms per step, {n / wall / 1e6:.2f} M reads/s")
+    graph.close()
+    st.close()
+    log(f"mixed budgets, one graph per step: {wall * 1000:.2f} ms per step, "
+        f"{n / wall / 1e6:.2f} M reads/s")
 
     cpu = None
     extras = {}
@@ -205,6 +226,8 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
                                f"50/35/15%", "preset": args.preset, "list_cap": list_cap,
                    "kmer_table_k": args.kmer},
         "cpu_baseline": cpu, "ms_per_group": parts,
+        "launch": "one HIP graph per step (the three groups' launches, fmi_graph_*)",
+        "ms_per_step_launched_one_by_one": round(eager * 1000, 3),
         "reads_per_group": [len(x[0]) for x in groups],
     }
     res.update(extras)
