@@ -69,6 +69,18 @@ def test_tuning_range():
         assert hpgfm.get_load_mode() == m
 
 
+def test_graph_arguments():
+    """The graph entry points reject a missing stream or graph before any
+    HIP call (no GPU needed); destroying no graph is a no-op."""
+    import ctypes as C
+    L = hpgfm.lib()
+    assert L.fmi_graph_begin(None) == -1
+    g = C.c_void_p()
+    assert L.fmi_graph_end(None, C.byref(g)) == -1
+    assert L.fmi_graph_launch(None, None) == -1
+    assert L.fmi_graph_destroy(None) == 0
+
+
 def test_read_validation():
     reads = np.zeros(10, np.uint8)
     with pytest.raises(ValueError):
