@@ -171,6 +171,48 @@ __device__ __forceinline__ void lf_four(const DevDir &D, uint32_t k, uint32_t l,
   for (uint32_t s = 0; s < 4u; ++s) lo[s] = pick4(D.C, s) + B2.c[s] + r2[s];
 }
 
+// A value the compiler cannot see (no instruction is emitted): the
+// register is taken as written here.
+__device__ __forceinline__ uint4 opaque4() {
+  uint32_t a, b, c, d;
+  asm volatile("" : "=v"(a), "=v"(b), "=v"(c), "=v"(d));
+  return make_uint4(a, b, c, d);
+}
+
+// The LF step of the read base at wp on the live interval [k, k <= l]: the
+// block(s) and the base are read in one batch (the block addresses do not
+// depend on the base), so a step waits for memory once.  Returns the base;
+// for a base <= 3 ko, lo are the step of the physical symbol (comp ? 3 - b :
+// b) as lf_one gives them; for an N the caller empties the interval.
+__device__ __forceinline__ uint32_t lf_read(const DevDir &D, uint32_t k, uint32_t l,
+                                            const uint8_t *wp, uint32_t comp, uint32_t &ko,
+                                            uint32_t &lo) {
+  const uint32_t m2 = l + 1;
+  const uint32_t j1 = k / kBlockRows, j2 = m2 / kBlockRows;
+  const uint32_t t1 = k - j1 * kBlockRows, t2 = m2 - j2 * kBlockRows;
+  const bool two = k != l && j2 != j1;
+  const uint4 *q1 = reinterpret_cast<const uint4 *>(D.blk + j1);
+  const uint4 a0 = q1[0], a1 = q1[1], a2 = q1[2], a3 = q1[3];
+  const uint32_t sym = *wp;
+  uint4 b0 = opaque4(), b1 = opaque4(), b2 = opaque4(), b3 = opaque4();
+  if (two) {
+    const uint4 *q2 = reinterpret_cast<const uint4 *>(D.blk + j2);
+    b0 = q2[0]; b1 = q2[1]; b2 = q2[2]; b3 = q2[3];
+  }
+  if (sym > 3u) return sym;
+  const Blk B1 = blk_of4(a0, a1, a2, a3);
+  const uint32_t s = comp ? 3u - sym : sym;
+  const uint32_t c = pick4(D.C, s);
+  ko = c + 1u + occ_blk(B1, D.dollar_blk, D.dollar_off, j1, s, t1);
+  if (k == l) {
+    lo = ko - 1u + (sym_at(B1, D.dollar_blk, D.dollar_off, j1, t1) == s ? 1u : 0u);
+  } else {
+    const Blk B2 = two ? blk_of4(b0, b1, b2, b3) : B1;
+    lo = c + occ_blk(B2, D.dollar_blk, D.dollar_off, j2, s, t2);
+  }
+  return sym;
+}
+
 __device__ __forceinline__ const DevDir &dir_of(const DevIndex &ix, uint32_t d) {
   return d ? ix.d[1] : ix.d[0];
 }

@@ -68,9 +68,8 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
     }
   }
   for (; P.direction ? (i <= len - 1) : (i >= 0); i += d) {
-    const uint32_t sym = W[i];
-    if (sym > 3u) { k2 = n + 1; l2 = n; break; }
     if (k2 == l2 && P.ix.text != nullptr) {
+      if (W[i] > 3u) { k2 = n + 1; l2 = n; break; }
       // text mode from the single row k2 on
       uint32_t q = D.sa[k2];
       uint32_t rem = (uint32_t)(d < 0 ? i + 1 : len - i);
@@ -90,7 +89,9 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
       else lf_one(D, k, l, P.comp ? 3u - s2 : s2, k2, l2, nb);
       break;
     }
-    lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+    uint32_t kn, ln;
+    if (lf_read(D, k2, l2, W + i, P.comp, kn, ln) > 3u) { k2 = n + 1; l2 = n; break; }
+    k2 = kn; l2 = ln;
     if (k2 > l2) break;
     k = k2; l = l2;
   }
@@ -124,13 +125,13 @@ __global__ __launch_bounds__(256) void vector_kernel(VectorParams P) {
   const uint32_t n = P.ix.n;
   int64_t *vk = P.vk + off, *vl = P.vl + off;
   int32_t good = 0;
-  uint32_t k2 = 0, l2 = n, nb;
+  uint32_t k2 = 0, l2 = n;
   if (P.direction == 0) {
     int32_t i = len - 1, j = len - 1;
     for (; i >= 0; --i, --j) {
-      const uint32_t sym = W[i];
-      if (sym > 3u) { k2 = n + 1; l2 = n; }
-      else lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+      uint32_t kn, ln;
+      if (lf_read(D, k2, l2, W + i, P.comp, kn, ln) > 3u) { k2 = n + 1; l2 = n; }
+      else { k2 = kn; l2 = ln; }
       vk[j] = k2; vl[j] = l2;
       if (k2 > l2) { --i; --j; break; }
       ++good;
@@ -139,9 +140,9 @@ __global__ __launch_bounds__(256) void vector_kernel(VectorParams P) {
   } else {
     int32_t i = 0, j = 0;
     for (; i <= len - 1; ++i, ++j) {
-      const uint32_t sym = W[i];
-      if (sym > 3u) { k2 = n + 1; l2 = n; }
-      else lf_one(D, k2, l2, P.comp ? 3u - sym : sym, k2, l2, nb);
+      uint32_t kn, ln;
+      if (lf_read(D, k2, l2, W + i, P.comp, kn, ln) > 3u) { k2 = n + 1; l2 = n; }
+      else { k2 = kn; l2 = ln; }
       vk[j] = k2; vl[j] = l2;
       if (k2 > l2) { ++i; ++j; break; }
       ++good;
@@ -202,14 +203,6 @@ struct S1Lane {
 
 constexpr uint32_t kTqNone = 0xFFFFFFFFu;
 constexpr uint32_t kTextRowTag = 0x80000000u;   // k's high word: k, l hold q
-
-// A value the compiler cannot see (no instruction is emitted): the
-// register is taken as written here.
-__device__ __forceinline__ uint4 opaque4() {
-  uint32_t a, b, c, d;
-  asm volatile("" : "=v"(a), "=v"(b), "=v"(c), "=v"(d));
-  return make_uint4(a, b, c, d);
-}
 
 // 16 read bases from byte wp of the 32 bytes c0, c1 (towards higher bytes
 // when up, else lower) as 2-bit groups in that order (w, physical symbols)

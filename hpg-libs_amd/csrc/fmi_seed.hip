@@ -76,7 +76,6 @@ __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
     const uint64_t r = owner(P.seed_base, P.nreads, g);
     const uint64_t s = (g - P.seed_base[r]) * P.stride;
     const uint8_t *W = P.reads + P.offsets[r] + s;
-    uint32_t nb;
     // The first k steps of each search from the k-mer table when its k bases
     // are A/C/G/T and their interval is not empty (else step by step, so a
     // dead interval is the one the LF steps give).
@@ -108,15 +107,15 @@ __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
       }
     }
     for (int32_t i = ip; i >= 0; --i) {
-      const uint32_t sym = W[i];
-      if (sym > 3u) { kp = n + 1; lp = n; break; }
-      lf_one(D, kp, lp, sym, kp, lp, nb);
+      uint32_t kn, ln;
+      if (lf_read(D, kp, lp, W + i, 0u, kn, ln) > 3u) { kp = n + 1; lp = n; break; }
+      kp = kn; lp = ln;
       if (kp > lp) break;
     }
     for (uint32_t i = im; i < P.seed_len; ++i) {
-      const uint32_t sym = W[i];
-      if (sym > 3u) { km = n + 1; lm = n; break; }
-      lf_one(D, km, lm, 3u - sym, km, lm, nb);
+      uint32_t kn, ln;
+      if (lf_read(D, km, lm, W + i, 1u, kn, ln) > 3u) { km = n + 1; lm = n; break; }
+      km = kn; lm = ln;
       if (km > lm) break;
     }
     // cumulative filter with the reference's int arithmetic
