@@ -33,6 +33,15 @@ struct VhView {
     uint32_t nb;
     lf_one(*D, k, l, comp ? 3u - b : b, ko, lo, nb);
   }
+  // The same step on a live interval with the base still in memory: the
+  // base and the block(s) are read in one batch.
+  __device__ __forceinline__ void lfw(uint32_t k, uint32_t l, const uint8_t *wp, uint32_t &ko,
+                                      uint32_t &lo) const {
+    uint32_t a, b;
+    if (lf_read(*D, k, l, wp, comp, a, b) > 3u) { ko = n + 1u; lo = n; return; }
+    ko = a;
+    lo = b;
+  }
 };
 
 struct VhRec {
@@ -95,7 +104,7 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     bool dead = false;
     for (int32_t j = n; j >= 0; --j) {
       if (!dead) {
-        B.lf(k, l, W[start + j], k, l);
+        B.lfw(k, l, W + start + j, k, l);
         dead = k > l;
       }
       if (j <= half) vb[j] = make_uint2(k, l);
@@ -107,7 +116,7 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     bool dead = false;
     for (int32_t j = 0; j <= n; ++j) {
       if (!dead) {
-        F.lf(k, l, W[start + j], k, l);
+        F.lfw(k, l, W + start + j, k, l);
         dead = k > l;
       }
       if (j >= half - 1) vf[j - (half - 1)] = make_uint2(k, l);
