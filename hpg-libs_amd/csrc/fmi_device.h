@@ -179,11 +179,11 @@ __device__ __forceinline__ uint4 opaque4() {
   return make_uint4(a, b, c, d);
 }
 
-// The LF step of the read base at wp on the live interval [k, k <= l]: the
-// block(s) and the base are read in one batch (the block addresses do not
-// depend on the base), so a step waits for memory once.  Returns the base;
-// for a base <= 3 ko, lo are the step of the physical symbol (comp ? 3 - b :
-// b) as lf_one gives them; for an N the caller empties the interval.
+// The LF step of the read base at wp on [k, l]: the block(s) and the base
+// are read in one batch (the block addresses do not depend on the base), so
+// a step waits for memory once.  Returns the base; for a base <= 3 ko, lo
+// are the step of the physical symbol (comp ? 3 - b : b) exactly as lf_one
+// gives them (for any k, l); for an N the caller empties the interval.
 __device__ __forceinline__ uint32_t lf_read(const DevDir &D, uint32_t k, uint32_t l,
                                             const uint8_t *wp, uint32_t comp, uint32_t &ko,
                                             uint32_t &lo) {

@@ -33,8 +33,21 @@ struct VhView {
     uint32_t nb;
     lf_one(*D, k, l, comp ? 3u - b : b, ko, lo, nb);
   }
-  // The same step on a live interval with the base still in memory: the
-  // base and the block(s) are read in one batch.
+  // lf for b = 0..3 on one interval from one read of its block(s):
+  // ko[b], lo[b] as lf(k, l, b) gives them, for any k, l.
+  __device__ __forceinline__ void lf4(uint32_t k, uint32_t l, uint32_t (&ko)[4],
+                                      uint32_t (&lo)[4]) const {
+    uint32_t pk[4], pl[4];
+    lf_four(*D, k, l, pk, pl);
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; ++b) {
+      const uint32_t s = comp ? 3u - b : b;
+      ko[b] = pick4(pk, s);
+      lo[b] = pick4(pl, s);
+    }
+  }
+  // The same step with the base still in memory: the base and the block(s)
+  // are read in one batch (lf_read computes what lf_one does, for any k, l).
   __device__ __forceinline__ void lfw(uint32_t k, uint32_t l, const uint8_t *wp, uint32_t &ko,
                                       uint32_t &lo) const {
     uint32_t a, b;
@@ -72,12 +85,12 @@ __device__ __forceinline__ void vh_exact(const VhView &V, const uint8_t *W, int3
   int32_t i = r.pos;
   if (dir == 0) {
     for (; i >= start; --i) {
-      V.lf(k2, l2, W[i], k2, l2);
+      V.lfw(k2, l2, W + i, k2, l2);
       if (k2 > l2) break;
     }
   } else {
     for (; i <= end; ++i) {
-      V.lf(k2, l2, W[i], k2, l2);
+      V.lfw(k2, l2, W + i, k2, l2);
       if (k2 > l2) break;
     }
   }
@@ -140,8 +153,10 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = start;
     S.add(r);
     const uint32_t bw = W[start];
+    uint32_t kb4[4], lb4[4];
+    B.lf4(k, l, kb4, lb4);
     for (uint32_t b = 0; b < 4u; ++b) {
-      B.lf(k, l, b, ka, la);
+      ka = pick4(kb4, b); la = pick4(lb4, b);
       if (ka > la) continue;
       if (b != bw) {
         r.k = ka; r.l = la; r.pos = -1;
@@ -167,8 +182,10 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = i - 1;
     vh_exact(B, W, start, end, 0, r);
     if (r.k <= r.l) S.add(r);
+    uint32_t kb4[4], lb4[4];
+    B.lf4(k, l, kb4, lb4);
     for (uint32_t b = 0; b < 4u; ++b) {
-      B.lf(k, l, b, ka, la);
+      ka = pick4(kb4, b); la = pick4(lb4, b);
       if (ka > la) continue;
       if (!mm_only) {
         r.k = ka; r.l = la; r.pos = i - 1;
@@ -201,8 +218,10 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = end;
     S.add(r);
     const uint32_t bw = W[end];
+    uint32_t kb4[4], lb4[4];
+    F.lf4(k, l, kb4, lb4);
     for (uint32_t b = 0; b < 4u; ++b) {
-      F.lf(k, l, b, ka, la);
+      ka = pick4(kb4, b); la = pick4(lb4, b);
       if (ka > la) continue;
       if (b != bw) {
         r.k = ka; r.l = la; r.pos = -1;
@@ -227,8 +246,10 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = i + 1;
     vh_exact(F, W, start, end, 1, r);
     if (r.k <= r.l) S.add(r);
+    uint32_t kb4[4], lb4[4];
+    F.lf4(k, l, kb4, lb4);
     for (uint32_t b = 0; b < 4u; ++b) {
-      F.lf(k, l, b, ka, la);
+      ka = pick4(kb4, b); la = pick4(lb4, b);
       if (ka > la) continue;
       if (!mm_only) {
         r.k = ka; r.l = la; r.pos = i + 1;
