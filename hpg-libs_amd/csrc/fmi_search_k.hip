@@ -108,6 +108,27 @@ __device__ __forceinline__ bool k_table(const KCtx &c, const KView &v, int32_t i
   return true;
 }
 
+// klf of the read base at wp: on a live interval the base and the block(s)
+// are read in one batch (lf_read); a dead one reads only the base (an N still
+// gives (n + 1, n), as klf does).
+__device__ __forceinline__ void klfw(const KCtx &c, const KView &v, int64_t k, int64_t l,
+                                     const uint8_t *wp, int64_t &ko, int64_t &lo) {
+  const uint32_t n = c.ix->n;
+  if (k > l) {
+    if (*wp > 3u) { ko = (int64_t)n + 1; lo = n; }
+    else { ko = k; lo = l; }
+    return;
+  }
+  uint32_t a, e;
+  if (lf_read(*v.d, (uint32_t)k, (uint32_t)l, wp, v.comp, a, e) > 3u) {
+    ko = (int64_t)n + 1;
+    lo = n;
+    return;
+  }
+  ko = a;
+  lo = e;
+}
+
 // add_result (results.h:93-103): keep, or return true from the caller.
 #define KADD(r, L)                                                   \
   do {                                                               \
@@ -166,7 +187,7 @@ __device__ __forceinline__ void k_exact(const KCtx &c, const KView &v, int dir, 
         if (dies) { k2 = (int64_t)c.ix->n + 1; l2 = c.ix->n; }
         break;
       }
-      klf(c, v, k2, l2, c.W[i], k2, l2);
+      klfw(c, v, k2, l2, c.W + i, k2, l2);
       if (k2 > l2) break;
       k = k2; l = l2;
     }
@@ -189,7 +210,7 @@ __device__ __forceinline__ void k_exact(const KCtx &c, const KView &v, int dir, 
         if (dies) { k2 = (int64_t)c.ix->n + 1; l2 = c.ix->n; }
         break;
       }
-      klf(c, v, k2, l2, c.W[i], k2, l2);
+      klfw(c, v, k2, l2, c.W + i, k2, l2);
       if (k2 > l2) break;
       k = k2; l = l2;
     }
@@ -216,7 +237,7 @@ __device__ __forceinline__ void k_extend(const KCtx &c, const KView &v, int64_t 
       else if (done) { k = l = k_isa(v, q2); }
       return;
     }
-    klf(c, v, k, l, c.W[i], k, l);
+    klfw(c, v, k, l, c.W + i, k, l);
     if (k > l) return;
   }
 }
@@ -438,7 +459,7 @@ __device__ __forceinline__ bool k_partials(const KCtx &c, uint8_t max_errors, co
           KADD(r, next);
           break;
         }
-        klf(c, v, k, l, c.W[i], kn, ln);
+        klfw(c, v, k, l, c.W + i, kn, ln);
         res = res_next; res_next = ln - kn;
         if (res == res_next || eb < c.D[i]) continue;
         k_change(r, k, l, i);
@@ -470,7 +491,7 @@ __device__ __forceinline__ bool k_partials(const KCtx &c, uint8_t max_errors, co
           KADD(r, next);
           break;
         }
-        klf(c, v, k, l, c.W[i], kn, ln);
+        klfw(c, v, k, l, c.W + i, kn, ln);
         res = res_next; res_next = ln - kn;
         if (res == res_next || eb < c.D[i]) continue;
         k_change(r, k, l, i);
@@ -561,7 +582,7 @@ __device__ __forceinline__ void k_dwalk(const KCtx &c, const KView &v, int64_t k
       --count;
       continue;
     }
-    klf(c, v, k, l, c.W[i], k, l);
+    klfw(c, v, k, l, c.W + i, k, l);
     if (k > l) { k = 0; l = nmax; z++; }
     D[i] = z;
     i += step;
