@@ -69,7 +69,8 @@ def same_records(a, b):
     cb, hb = b
     if not np.array_equal(ca, cb):
         return False
-    m = np.arange(CAP)[None, None, :] < np.minimum(ca, CAP)[:, :, None]
+    cap = ha.shape[2]
+    m = np.arange(cap)[None, None, :] < np.minimum(ca, cap)[:, :, None]
     x, y = ha[m], hb[m]
     for key in ("k", "l", "start", "pos", "end", "dir", "num_mismatches", "strand"):
         if not np.array_equal(x[key], y[key]):

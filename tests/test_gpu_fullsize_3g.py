@@ -1,0 +1,107 @@
+"""The 1-mismatch search on a 3.1 Gbp synthetic reference (41% GC, 45% repeat
+families at 1-20% divergence, 0.1% N runs) with 2M reads of 150 bp from both
+strands at 1% substitutions: the index size where SA values, rows and text
+positions pass 2^31 and the index no longer fits the caches.
+
+The whole batch is held to the size-independent properties of the 64 Mbp
+test: the compact index, the expanded index and the expanded index with the
+k = 15 table, under both load modes, give the same counts and records; every
+read drawn without a substitution or an N has an exact record on its strand
+whose interval holds the row of the drawn position, the row here found by
+the engine's own get_ISA (fmi_inverse_locate_batch, runtime.h:88) on the
+compact index, which walks LF from the sampled ISA."""
+import numpy as np
+import pytest
+
+from hpgfm import synth
+from test_gpu_fullsize import same_records
+
+pytestmark = pytest.mark.gpu
+
+NBP = 3_100_000_000
+NREADS = 2_000_000
+LEN = 150
+CAP = 16
+
+
+def say(msg):
+    print(f"[fullsize 3.1 Gbp] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def huge(gpu_engine):
+    hp = gpu_engine
+    say(f"genome {NBP} bp, {NREADS} x {LEN} bp reads")
+    g, _, _ = synth.genome(NBP, seed=1, gc=0.41, repeat_frac=0.45, fam_len=(300, 6000),
+                           divergence=(0.01, 0.20), n_frac=0.001)
+    reads, offs, truth = synth.reads(g, NREADS, length=LEN, seed=2, sub_rate=0.01)
+    say("index")
+    ix = hp.Index.build_gpu(synth.replace_n(g, seed=1), 16)
+    batch = hp.ReadBatch(reads, offs)
+    out = hp.Search1Output(NREADS, CAP)
+    hp.search1_launch(ix, batch, out, mode=hp.MODE_MISMATCH)
+    hp.sync()
+    compact = (out.counts().reshape(-1, 2), out.hits())
+    state = dict(hp=hp, g=g, reads=reads, truth=truth, ix=ix, batch=batch, out=out,
+                 compact=compact)
+    yield state
+    hp.set_load_mode(hp.LOADS_AUTO)
+    out.free()
+    batch.free()
+    ix.close()
+
+
+def test_3g_planted_positions(huge):
+    hp, g, ix = huge["hp"], huge["g"], huge["ix"]
+    say("planted positions")
+    pos = huge["truth"]["pos"].astype(np.int64)
+    strand = huge["truth"]["strand"].astype(np.int64)
+    mat = huge["reads"].reshape(NREADS, LEN)
+    clean = np.zeros(NREADS, bool)
+    for lo in range(0, NREADS, 100_000):
+        hi = min(lo + 100_000, NREADS)
+        src = g[pos[lo:hi, None] + np.arange(LEN)[None, :]]
+        rc = strand[lo:hi] == 1
+        src[rc] = (3 - src[rc])[:, ::-1]
+        m = mat[lo:hi]
+        clean[lo:hi] = np.all(src == m, axis=1) & np.all(m < 4, axis=1)
+    counts, hits = huge["compact"]
+    idx = np.nonzero(clean)[0]
+    c = counts[idx, strand[idx]]
+    assert len(idx) > 300_000 and np.all(c >= 1)
+    idx = idx[c <= CAP]
+    assert len(idx) > 0.7 * clean.sum()
+    rec = hits[idx, strand[idx], :]
+    valid = np.arange(CAP)[None, :] < counts[idx, strand[idx]][:, None]
+    exact = valid & (rec["num_mismatches"] == 0)
+    assert np.all(exact.any(axis=1))
+    assert int(pos[idx].max()) >= 1 << 31
+    n1 = NBP + 1
+    lc = rec["end"].astype(np.int64) - rec["start"].astype(np.int64) + 1
+    p = np.broadcast_to(pos[idx][:, None], rec.shape)
+    # BWT of the text: SA[row] = pos; of the reversed text: SA[row] = n1 - lc - 1 - pos
+    direction = np.where(strand[idx][:, None] == 1, rec["dir"] == 0, rec["dir"] == 1)
+    row = np.full(rec.shape, -1, np.int64)
+    for fwd in (0, 1):
+        sel = exact & (direction == bool(fwd))
+        q = (n1 - lc - 1 - p)[sel] if fwd else p[sel]
+        row[sel] = hp.inverse_locate_batch(ix, q.astype(np.uint64), forward=fwd).astype(np.int64)
+    found = exact & (rec["k"] <= row) & (row <= rec["l"])
+    assert np.all(found.any(axis=1))
+
+
+def test_3g_modes_agree(huge):
+    hp, ix = huge["hp"], huge["ix"]
+    say("expanding")
+    ix.expand()
+    for k in (0, 15):
+        ix.kmer_table(k)
+        for loads in (hp.LOADS_ALL, hp.LOADS_OWN):
+            say(f"expanded index, k-mer table k={k}, load mode {loads}")
+            hp.set_load_mode(loads)
+            hp.search1_launch(ix, huge["batch"], huge["out"], mode=hp.MODE_MISMATCH)
+            hp.sync()
+            got = (huge["out"].counts().reshape(-1, 2), huge["out"].hits())
+            assert same_records(huge["compact"], got), (k, loads)
+    hp.set_load_mode(hp.LOADS_AUTO)
+    ix.kmer_table(0)
