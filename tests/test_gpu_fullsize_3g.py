@@ -90,6 +90,54 @@ def test_3g_planted_positions(huge):
     assert np.all(found.any(axis=1))
 
 
+def test_3g_seed_regions_planted(huge):
+    """Exact seeds (bwt_map_exact_seed, bwt.c:1200-1294) of 100K reads of 250
+    bp, 18 bases at stride 5, on the compact index: every seed without a
+    substitution or an N yields, on its read's strand, the region of the
+    position it was cut from (1-based, one contig from coordinate 0), unless
+    that strand passed the 1000-row filter, in which case the strand has no
+    region for the seed at all."""
+    hp, g, ix = huge["hp"], huge["g"], huge["ix"]
+    say("exact seeds")
+    n, L, sl, st = 100_000, 250, 18, 5
+    reads, offs, truth = synth.reads(g, n, length=L, seed=4, sub_rate=0.01)
+    batch = hp.ReadBatch(reads, offs)
+    try:
+        nseeds, nrows, regs, _ = hp.seed_map_batch(ix, batch, sl, st, 1000)
+    finally:
+        batch.free()
+    assert len(regs) == nrows
+    pos = truth["pos"].astype(np.int64)
+    strand = truth["strand"].astype(np.int64)
+    src = g[pos[:, None] + np.arange(L)[None, :]]
+    rc = strand == 1
+    src[rc] = (3 - src[rc])[:, ::-1]
+    mat = reads.reshape(n, L)
+    bad = np.concatenate([np.zeros((n, 1), np.int32),
+                          np.cumsum((src != mat) | (mat > 3), axis=1, dtype=np.int32)], axis=1)
+    s0 = np.arange(0, L - sl + 1, st)
+    clean = (bad[:, s0 + sl] - bad[:, s0]) == 0                     # [n, seeds]
+    ri, q = np.nonzero(clean)
+    off = s0[q]
+    tpos = np.where(strand[ri] == 0, pos[ri] + off, pos[ri] + L - off - sl)
+    assert int(tpos.max()) >= 1 << 31
+
+    def key(read, seed, s, start):
+        return ((read.astype(np.uint64) << np.uint64(39)) | (seed.astype(np.uint64) << np.uint64(33))
+                | (s.astype(np.uint64) << np.uint64(32)) | start.astype(np.uint64))
+
+    v = regs[regs["valid"] == 1]
+    have = key(v["read"], v["seed_id"], v["strand"], v["start"])
+    want = key(ri, q, strand[ri], tpos + 1)
+    hit = np.isin(want, have)
+    groups = np.unique(have >> np.uint64(32))
+    # a missing planted region only where the seed's strand has no region at all
+    assert not np.any(np.isin(want[~hit] >> np.uint64(32), groups))
+    assert hit.mean() > 0.9, hit.mean()
+    ends = v["end"][np.isin(have, want)] - v["start"][np.isin(have, want)]
+    assert np.all(ends == sl - 1)
+
+
 def test_3g_modes_agree(huge):
     hp, ix = huge["hp"], huge["ix"]
     say("expanding")
