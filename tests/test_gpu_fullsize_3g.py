@@ -9,10 +9,13 @@ k = 15 table, under both load modes, give the same counts and records; every
 read drawn without a substitution or an N has an exact record on its strand
 whose interval holds the row of the drawn position, the row here found by
 the engine's own get_ISA (fmi_inverse_locate_batch, runtime.h:88) on the
-compact index, which walks LF from the sampled ISA."""
+compact index, which walks LF from the sampled ISA.  On the same index: the
+exact seed regions of planted seeds, and the serving loop's requests under
+Poisson load against the batch call alone."""
 import numpy as np
 import pytest
 
+import fmoracle as fo
 from hpgfm import synth
 from test_gpu_fullsize import same_records
 
@@ -152,4 +155,39 @@ def test_3g_modes_agree(huge):
             got = (huge["out"].counts().reshape(-1, 2), huge["out"].hits())
             assert same_records(huge["compact"], got), (k, loads)
     hp.set_load_mode(hp.LOADS_AUTO)
+    ix.kmer_table(0)
+
+
+def test_3g_served_equal_alone(huge):
+    """The serving configuration: the 3.1 Gbp index resident (expanded, k = 15
+    table), Poisson arrivals at 200 requests/s of 1-50K reads of 50-250 bp
+    with 1% substitutions, at most 4 launches in flight under a 2 GiB
+    budget; every request's counts and records equal those of
+    fmi_search1_batch on its reads alone."""
+    hp, ix = huge["hp"], huge["ix"]
+    from hpgfm import serve
+    say("serving")
+    ix.expand()
+    ix.kmer_table(15)
+    pool = serve.read_pool(huge["g"], 1_000_000, seed=5)
+    srv = hp.Server(ix, budget_bytes=2 << 30, max_inflight=4, max_batch_reads=1 << 20, cap=CAP,
+                    mode=hp.MODE_MISMATCH)
+    try:
+        res = serve.run(srv, pool, rate=200, duration_s=1.0, seed=200, keep=True)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert res["requests"] > 100 and st["requests_done"] == res["requests"]
+    reads, offs = pool
+    say(f"{res['requests']} requests served; each against the batch call alone")
+    for start, size, c, recs in res["kept"]:
+        o = offs[start:start + size + 1]
+        batch = hp.ReadBatch(reads[int(o[0]):int(o[-1])], o - o[0])
+        try:
+            c2, h2 = hp.search1_batch(ix, batch, cap=CAP, mode=hp.MODE_MISMATCH)
+        finally:
+            batch.free()
+        assert np.array_equal(c, c2)
+        alone = h2[np.arange(CAP)[None, None, :] < np.minimum(c2, CAP)[:, :, None]]
+        assert fo.hits_equal(recs, alone)
     ix.kmer_table(0)
