@@ -16,7 +16,9 @@ it to the oracle:
   position it was drawn at (ISA of the position, or of its mirror in the
   reversed text for a forward-direction record);
 - 2000 of the reads give the oracle's BWSearch1CPU records, mismatch-only and
-  edit mode, bit for bit (search/search.c:1876-2138)."""
+  edit mode, bit for bit (search/search.c:1876-2138);
+- the inexact seed mapping of 200K of the reads finds every 18-base window
+  with at most one substitution within 10 bp of where it was cut."""
 import os
 
 import numpy as np
@@ -161,3 +163,46 @@ def test_fullsize_sample_vs_oracle(big, compact):
         sub.free()
     _, oc, oh = o.search1_batch(rs, os_, cap=CAP, mismatch_only=False, threads=threads)
     assert same_records(ge, (oc.reshape(-1, 2), oh.reshape(-1, 2, CAP)))
+
+
+def test_fullsize_inexact_seeds_planted(big):
+    """Inexact seeds (bwt_map_inexact_seeds_seq, bwt.c:3262-3302) of the first
+    200K reads, 18-base windows: every window with at most one substitution
+    has, on its read's strand, a region within 10 bp of the 1-based start it
+    was cut from (the one contig starts at coordinate 0), and an error-free
+    window exactly one, unless its list overflowed (status set)."""
+    hp, g, ix = big["hp"], big["g"], big["ix"]
+    say("inexact seeds")
+    n, ss_, ms, cap = 200_000, 18, 16, 16
+    reads, offs = big["reads"][:n * LEN], big["offs"][:n + 1]
+    pos = big["truth"]["pos"][:n].astype(np.int64)
+    strand = big["truth"]["strand"][:n].astype(np.int64)
+    batch = hp.ReadBatch(reads, offs)
+    try:
+        res = hp.map_inexact_seeds_batch(ix, batch, ss_, ms, region_cap=cap)
+    finally:
+        batch.free()
+    nw = LEN // ss_                                       # 8 windows; the 6-base rest < 16
+    assert np.array_equal(np.diff(res.window_base), np.full(n, nw))
+    src = g[pos[:, None] + np.arange(LEN)[None, :]]
+    rc = strand == 1
+    src[rc] = (3 - src[rc])[:, ::-1]
+    diff = np.concatenate([np.zeros((n, 1), np.int32),
+                           np.cumsum(src != reads.reshape(n, LEN), axis=1, dtype=np.int32)], axis=1)
+    st = np.arange(nw) * ss_
+    errs = (diff[:, st + ss_] - diff[:, st]).reshape(-1)            # [n * nw]
+    t0 = np.where(strand[:, None] == 0, pos[:, None] + st[None, :],
+                  pos[:, None] + (LEN - st[None, :] - ss_)).reshape(-1)
+    s = np.repeat(strand, nw)
+    w = np.arange(n * nw)
+    regs = res.regs.reshape(n * nw, 2, cap)[w, s]
+    valid = np.arange(cap)[None, :] < np.minimum(res.nreg.reshape(n * nw, 2)[w, s], cap)[:, None]
+    near = valid & (regs["chrom_idx"] == 1) & \
+        (np.abs(regs["start"].astype(np.int64) - (t0 + 1)[:, None]) < 10)
+    ok = res.status[w, s] == 0
+    assert ok.mean() > 0.8
+    one = ok & (errs <= 1)
+    assert one.sum() > 0.6 * n * nw
+    assert np.all(near[one].any(axis=1))
+    zero = ok & (errs == 0)
+    assert np.all(near[zero].sum(axis=1) == 1)
