@@ -10,6 +10,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -59,55 +60,85 @@ struct Tuning {
 Tuning g_tuning;
 std::mutex g_mu;
 
-// Per-(device, stream) work counters for the persistent launch, so that
-// launches on different streams never share a queue.
+// Per-(device, stream) state: the persistent launches' work counters and
+// the scratch of the vector helper and the k-error search, so that launches
+// on different streams never share a queue or a scratch slice.  A graph
+// captured on a stream holds these addresses: while one is alive
+// (fmi_graph_end .. fmi_graph_destroy) the scratch may not move, and none of
+// it may be created while the stream is capturing.
+struct StreamState {
+  uint32_t *queue = nullptr;              // 256 B: queue counter, then stats
+  unsigned long long *stats = nullptr;
+  void *scratch = nullptr;
+  uint64_t scratch_bytes = 0;
+  uint32_t graphs = 0;                    // live graphs captured on the stream
+};
+std::unordered_map<std::string, StreamState> g_streams;
+std::unordered_map<void *, std::string> g_graph_stream;   // graph -> stream key
+
+int stream_key(hipStream_t s, std::string *key) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  char buf[64];
+  snprintf(buf, sizeof buf, "%d:%p", dev, (void *)s);
+  *key = buf;
+  return FMI_OK;
+}
+
+bool capturing(hipStream_t s);
+
 struct Workspace {
   uint32_t *queue = nullptr;
   unsigned long long *stats = nullptr;
 };
-std::unordered_map<std::string, Workspace> g_ws;
 
 int workspace_for(hipStream_t s, Workspace *out) {
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
-  char key[64];
-  snprintf(key, sizeof key, "%d:%p", dev, (void *)s);
+  std::string key;
+  int rc = stream_key(s, &key);
+  if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_mu);
-  auto it = g_ws.find(key);
-  if (it != g_ws.end()) { *out = it->second; return FMI_OK; }
-  Workspace w;
-  void *p = nullptr;
-  HIP_TRY(hipMalloc(&p, 256), "hipMalloc(workspace)");
-  w.queue = static_cast<uint32_t *>(p);
-  w.stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
-  g_ws[key] = w;
-  *out = w;
+  StreamState &st = g_streams[key];
+  if (!st.queue) {
+    if (capturing(s))
+      return fail(FMI_ESTATE, "run the call once on this stream before capturing it");
+    void *p = nullptr;
+    HIP_TRY(hipMalloc(&p, 256), "hipMalloc(workspace)");
+    st.queue = static_cast<uint32_t *>(p);
+    st.stats = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
+  }
+  out->queue = st.queue;
+  out->stats = st.stats;
   return FMI_OK;
 }
 
-// Per-(device, stream) scratch that grows to the largest request made on
-// it; freed only when a larger one replaces it (after the stream drains).
-std::unordered_map<std::string, std::pair<void *, uint64_t>> g_scratch;
-
+// Scratch that grows to the largest request made on the stream; an older,
+// smaller one is freed after the stream drains.  Growing is refused while
+// the stream captures or a graph captured on it is alive.
 int scratch_for(hipStream_t s, uint64_t bytes, void **out) {
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
-  char key[64];
-  snprintf(key, sizeof key, "%d:%p", dev, (void *)s);
+  std::string key;
+  int rc = stream_key(s, &key);
+  if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_mu);
-  auto &slot = g_scratch[key];
-  if (slot.second < bytes) {
-    if (slot.first) {
+  StreamState &st = g_streams[key];
+  if (st.scratch_bytes < bytes) {
+    if (capturing(s) || st.graphs)
+      return fail(FMI_ESTATE, "the call needs " + std::to_string(bytes) +
+                                  " bytes of stream scratch, more than the " +
+                                  std::to_string(st.scratch_bytes) +
+                                  " a graph captured on this stream holds");
+    if (st.scratch) {
       HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
-      HIP_TRY(hipFree(slot.first), "hipFree(scratch)");
-      slot = {nullptr, 0};
+      HIP_TRY(hipFree(st.scratch), "hipFree(scratch)");
+      st.scratch = nullptr;
+      st.scratch_bytes = 0;
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) return fail(FMI_ENOMEM, "device scratch of " + std::to_string(bytes) + " bytes");
-    slot = {p, bytes};
+    st.scratch = p;
+    st.scratch_bytes = bytes;
   }
-  *out = slot.first;
+  *out = st.scratch;
   return FMI_OK;
 }
 
@@ -229,6 +260,30 @@ int search1_common(const fmi_index *ix, int mode, int strand_mask, const uint8_t
 
 }  // namespace
 
+namespace fmi {
+
+int release_stream_state(hipStream_t s) {
+  std::string key;
+  int rc = stream_key(s, &key);
+  if (rc) return rc;
+  StreamState st;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_streams.find(key);
+    if (it == g_streams.end()) return FMI_OK;
+    if (it->second.graphs)
+      return fail(FMI_ESTATE, "destroy the graphs captured on this stream first");
+    st = it->second;
+    g_streams.erase(it);
+  }
+  HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (st.queue) HIP_TRY(hipFree(st.queue), "hipFree(workspace)");
+  if (st.scratch) HIP_TRY(hipFree(st.scratch), "hipFree(scratch)");
+  return FMI_OK;
+}
+
+}  // namespace fmi
+
 extern "C" {
 
 const char *fmi_status_str(int status) {
@@ -319,6 +374,16 @@ void fmi_index_free(fmi_index *ix) {
 }
 
 uint64_t fmi_index_size_sa(const fmi_index *ix) { return ix ? ix->n + 1 : 0; }
+
+uint64_t fmi_index_sample_count(const fmi_index *ix) { return ix ? ix->h[0].nsamp : 0; }
+
+int fmi_index_get_samples(const fmi_index *ix, int forward, uint32_t *S, uint32_t *R) {
+  if (!ix) return fail(FMI_EINVAL, "index is NULL");
+  const fmi::HostDir &h = ix->h[forward ? 1 : 0];
+  if (S) std::memcpy(S, h.S, h.nsamp * 4);
+  if (R) std::memcpy(R, h.R, h.nsamp * 4);
+  return FMI_OK;
+}
 uint32_t fmi_index_sa_ratio(const fmi_index *ix) { return ix ? ix->ratio : 0; }
 
 uint64_t fmi_index_device_bytes(const fmi_index *ix) {
@@ -476,11 +541,24 @@ int fmi_index_to_device(fmi_index *ix) {
 int fmi_index_get_bwt(const fmi_index *ix, int forward, uint8_t *out, uint64_t *dollar) {
   if (!ix || !out || !dollar) return fail(FMI_EINVAL, "NULL argument");
   const fmi::HostDir &h = ix->h[forward ? 1 : 0];
-  uint64_t o = 0;
-  for (uint64_t r = 0; r <= ix->n; ++r) {
-    if (r == h.dollar) continue;
-    out[o++] = (uint8_t)fmi::host_bwt_sym(h, r);
-  }
+  // rows [0, dollar) land at out[r], rows past the '$' row at out[r - 1];
+  // threads take contiguous row ranges
+  const uint64_t rows = ix->n + 1;
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = std::max(1u, std::min(nt, 16u));
+  if (rows < (1u << 20)) nt = 1;
+  const uint64_t per = (rows + nt - 1) / nt;
+  auto work = [&](unsigned t) {
+    const uint64_t r0 = t * per, r1 = std::min(rows, r0 + per);
+    for (uint64_t r = r0; r < r1; ++r) {
+      if (r == h.dollar) continue;
+      out[r < h.dollar ? r : r - 1] = (uint8_t)fmi::host_bwt_sym_raw(h, r);
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
   *dollar = h.dollar;
   return FMI_OK;
 }
@@ -562,7 +640,22 @@ int fmi_memcpy_d2h(void *dst, const void *d_src, size_t bytes, fmi_stream s) {
 int fmi_memset_d(void *d_dst, int value, size_t bytes, fmi_stream s) {
   if (!bytes) return FMI_OK;
   if (!d_dst) return fail(FMI_EINVAL, "NULL argument");
-  HIP_TRY(hipMemsetAsync(d_dst, value, bytes, static_cast<hipStream_t>(s)), "hipMemsetAsync");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  // no memset nodes in a graph: a fill kernel while the stream captures
+  if (capturing(st)) {
+    HIP_TRY(fmi::launch_fill(d_dst, (uint8_t)value, bytes, st), "fill launch");
+    return FMI_OK;
+  }
+  HIP_TRY(hipMemsetAsync(d_dst, value, bytes, st), "hipMemsetAsync");
+  return FMI_OK;
+}
+
+int fmi_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(FMI_EINVAL, "NULL argument");
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+  *free_bytes = f;
+  *total_bytes = t;
   return FMI_OK;
 }
 
@@ -575,7 +668,11 @@ int fmi_stream_create(fmi_stream *s) {
 }
 
 int fmi_stream_destroy(fmi_stream s) {
-  HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(s)), "hipStreamDestroy");
+  if (!s) return fail(FMI_EINVAL, "the default stream cannot be destroyed");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  int rc = fmi::release_stream_state(st);
+  if (rc) return rc;
+  HIP_TRY(hipStreamDestroy(st), "hipStreamDestroy");
   return FMI_OK;
 }
 
@@ -594,12 +691,22 @@ int fmi_graph_begin(fmi_stream s) {
 int fmi_graph_end(fmi_stream s, fmi_graph *g) {
   if (!s || !g) return fail(FMI_EINVAL, "NULL argument");
   *g = nullptr;
+  hipStream_t st = static_cast<hipStream_t>(s);
+  std::string key;
+  int rc = stream_key(st, &key);
+  if (rc) return rc;
   hipGraph_t graph = nullptr;
-  HIP_TRY(hipStreamEndCapture(static_cast<hipStream_t>(s), &graph), "hipStreamEndCapture");
+  HIP_TRY(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
   hipGraphExec_t exec = nullptr;
   const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  {
+    // the stream's counters and scratch are now baked into the graph
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_streams[key].graphs++;
+    g_graph_stream[exec] = key;
+  }
   *g = exec;
   return FMI_OK;
 }
@@ -614,6 +721,13 @@ int fmi_graph_launch(fmi_graph g, fmi_stream s) {
 int fmi_graph_destroy(fmi_graph g) {
   if (!g) return FMI_OK;
   HIP_TRY(hipGraphExecDestroy(static_cast<hipGraphExec_t>(g)), "hipGraphExecDestroy");
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_graph_stream.find(g);
+  if (it != g_graph_stream.end()) {
+    auto st = g_streams.find(it->second);
+    if (st != g_streams.end() && st->second.graphs) st->second.graphs--;
+    g_graph_stream.erase(it);
+  }
   return FMI_OK;
 }
 
@@ -709,20 +823,16 @@ int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
   if (cap > 0 && !d_hits) return fail(FMI_EINVAL, "d_hits is NULL with cap > 0");
   if (nreads > (1u << 30)) return fail(FMI_ERANGE, "more than 2^30 reads in one batch");
   hipStream_t st = static_cast<hipStream_t>(s);
-  // the scratch the vectors need: 16 B * (bases + 3 * reads)
-  uint64_t bases = 0;
-  HIP_TRY(hipMemcpyAsync(&bases, d_offsets + nreads, sizeof bases, hipMemcpyDeviceToHost, st),
-          "read offsets[nreads]");
-  HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
-  if (bases > (1ull << 40)) return fail(FMI_ERANGE, "offsets[nreads] out of range");
+  // per-lane vector scratch: sized by the lane count, no read of the offsets
+  const uint32_t lanes = fmi::vhelper_lanes(nreads);
   void *scratch = nullptr;
-  rc = scratch_for(st, 16ull * (bases + 3ull * nreads), &scratch);
+  rc = scratch_for(st, fmi::vhelper_scratch_bytes(lanes), &scratch);
   if (rc) return rc;
   rc = begin_timed(st);
   if (rc) return rc;
   hipError_t e = fmi::launch_vhelper(ix->dev, mode == FMI_MODE_MISMATCH, (uint32_t)strand_mask,
                                      d_reads, d_offsets, d_bounds, nreads, cap, d_hits, d_counts,
-                                     scratch, st);
+                                     scratch, lanes, st);
   return end_timed(st, e, "search1 vector launch");
 }
 

@@ -7,6 +7,7 @@
 // computes the suffix array of each direction with SA-IS and writes the
 // interleaved 64-byte occurrence blocks and the samples straight from it;
 // the sampled values are the same (S[i/r] = SA[i], R[j/r] = ISA[j]).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -134,6 +135,12 @@ void free_dir(HostDir &d) {
 
 }  // namespace
 
+uint32_t host_bwt_sym_raw(const HostDir &d, uint64_t r) {
+  const OccBlock &B = d.blk[r / kBlockRows];
+  uint32_t t = (uint32_t)(r % kBlockRows);
+  return (uint32_t)((B.lo[t >> 6] >> (t & 63)) & 1u) | ((uint32_t)((B.hi[t >> 6] >> (t & 63)) & 1u) << 1);
+}
+
 uint32_t host_bwt_sym(const HostDir &d, uint64_t r) {
   if (r == d.dollar) return 0xFFu;
   const OccBlock &B = d.blk[r / kBlockRows];
@@ -217,6 +224,63 @@ struct Reader {
     sum.add(p, n);
   }
 };
+
+// The contents of one direction read from a file: every block's counts
+// follow from the previous block's and its rows, the totals agree with C,
+// the '$' row is stored as code 0, and every S / R sample names a row or a
+// text position of the index (0..n).
+bool check_direction(const HostDir &h, uint64_t n, uint32_t ratio, std::string *err) {
+  const uint64_t rows = n + 1;
+  uint64_t run[4] = {0, 0, 0, 0};
+  if (host_bwt_sym_raw(h, h.dollar) != 0) {
+    *err = "index '$' row is not stored as code 0";
+    return false;
+  }
+  for (uint64_t j = 0; j < h.nblocks; ++j) {
+    const OccBlock &B = h.blk[j];
+    for (int b = 0; b < 4; ++b) {
+      if (B.cnt[b] != run[b]) {
+        *err = "index occurrence counts are inconsistent at block " + std::to_string(j);
+        return false;
+      }
+    }
+    const uint64_t r0 = j * kBlockRows;
+    if (r0 >= rows) continue;
+    const uint32_t m = (uint32_t)std::min<uint64_t>(kBlockRows, rows - r0);
+    for (uint32_t w = 0; w < 3; ++w) {
+      const uint32_t lo_t = w * 64;
+      if (lo_t >= m) break;
+      const uint32_t nb = std::min<uint32_t>(64, m - lo_t);
+      const uint64_t mask = nb == 64 ? ~0ull : ((1ull << nb) - 1);
+      const uint64_t lo = B.lo[w] & mask, hi = B.hi[w] & mask;
+      run[1] += (uint64_t)__builtin_popcountll(lo & ~hi);
+      run[2] += (uint64_t)__builtin_popcountll(~lo & hi & mask);
+      run[3] += (uint64_t)__builtin_popcountll(lo & hi);
+      run[0] += (uint64_t)__builtin_popcountll(~lo & ~hi & mask);
+    }
+    if (h.dollar >= r0 && h.dollar < r0 + m) run[0]--;   // the '$' row counts for nothing
+  }
+  uint64_t acc = 0;
+  for (int b = 0; b < 4; ++b) {
+    if (h.C[b] != acc) {
+      *err = "index C does not match the occurrence counts";
+      return false;
+    }
+    acc += run[b];
+  }
+  if (acc != n) {
+    *err = "index occurrence counts do not add up to the text length";
+    return false;
+  }
+  for (uint64_t i = 0; i < h.nsamp; ++i) {
+    if (h.S[i] > n || h.R[i] > n) {
+      *err = "index S / R sample " + std::to_string(i) + " lies outside 0..n";
+      return false;
+    }
+  }
+  (void)ratio;
+  return true;
+}
 
 }  // namespace
 
@@ -319,8 +383,23 @@ int load_index(const char *path, fmi_index *ix, std::string *err) {
   fclose(f);
   if (!ok) { *err = "index file is truncated"; return FMI_EFORMAT; }
   if (got != want) { *err = "index checksum mismatch"; return FMI_EFORMAT; }
+  // A file with a valid checksum can still be wrong inside; the kernels
+  // index with these values, so check every one they trust.
+  for (int d = 0; d < 2; ++d) {
+    if (!check_direction(ix->h[d], ix->n, ix->ratio, err)) return FMI_EFORMAT;
+  }
   ix->koffset.assign(kc + 1, 0);
-  for (uint64_t c = 0; c < kc; ++c) ix->koffset[c + 1] = ix->koffset[c] + (ix->kend[c] - ix->kstart[c] + 1);
+  for (uint64_t c = 0; c < kc; ++c) {
+    if (ix->kend[c] + 1 < ix->kstart[c] || ix->kend[c] - ix->kstart[c] + 1 > ix->n) {
+      *err = "index karyotype has a contig that ends before it starts";
+      return FMI_EFORMAT;
+    }
+    ix->koffset[c + 1] = ix->koffset[c] + (ix->kend[c] - ix->kstart[c] + 1);
+  }
+  if (kc && ix->koffset[kc] != ix->n) {
+    *err = "index karyotype does not add up to the text length";
+    return FMI_EFORMAT;
+  }
   return FMI_OK;
 }
 

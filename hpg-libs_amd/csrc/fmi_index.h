@@ -72,5 +72,7 @@ int refplain_layout(const RefPlain *rp, int forward, uint32_t *desp, uint64_t *c
                     std::string *err);
 // Symbol of BWT row r (0xFF for the '$' row).
 uint32_t host_bwt_sym(const HostDir &d, uint64_t r);
+// The 2-bit code stored at row r (the '$' row holds 0).
+uint32_t host_bwt_sym_raw(const HostDir &d, uint64_t r);
 
 }  // namespace fmi

@@ -22,6 +22,12 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
                           uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
                           uint32_t chunk, uint32_t refill_min, bool own_loads, hipStream_t s);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t s);
+// Byte fill of bytes bytes with v (a kernel, so a captured graph holds no
+// memset node).
+hipError_t launch_fill(void *p, uint8_t v, uint64_t bytes, hipStream_t s);
+// Frees the work counters and scratch the library keeps for stream s after
+// draining it (fmi_api.cpp); FMI_ESTATE while a graph captured on s lives.
+int release_stream_state(hipStream_t s);
 // Dense copy of the stored records of nslots (read, strand) slots, slot after
 // slot; *total (device) receives the record count.  scratch holds
 // compact_scratch_bytes(nslots) bytes.
@@ -29,12 +35,16 @@ uint64_t compact_scratch_bytes(uint32_t nslots);
 hipError_t launch_compact(const uint32_t *counts, const fmi_hit *hits, uint32_t nslots,
                           uint32_t cap, fmi_hit *out, unsigned long long *total, void *scratch,
                           hipStream_t s);
-// BWSearch1VectorHelper on both strands of every read; scratch holds
-// 16 * (offsets[nreads] + 3 * nreads) bytes.
+// BWSearch1VectorHelper on both strands of every read, `lanes` lanes (a
+// multiple of 256, vhelper_lanes) striding over the tasks; scratch holds
+// vhelper_scratch_bytes(lanes) bytes.
+constexpr uint32_t kVhelperMaxLanes = 64u * 1024u;
+uint32_t vhelper_lanes(uint32_t nreads);
+uint64_t vhelper_scratch_bytes(uint32_t lanes);
 hipError_t launch_vhelper(const DevIndex &ix, bool mm_only, uint32_t strand_mask,
                           const uint8_t *reads, const uint64_t *offsets, const int16_t *bounds,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
-                          void *scratch, hipStream_t s);
+                          void *scratch, uint32_t lanes, hipStream_t s);
 hipError_t launch_locate(const DevIndex &ix, uint32_t dsel, const int64_t *k, const int64_t *l,
                          uint32_t stride, uint32_t nintervals, uint32_t cap, uint64_t *pos,
                          uint32_t *nrows, hipStream_t s);

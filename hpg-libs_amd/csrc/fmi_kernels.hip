@@ -965,15 +965,24 @@ __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
+// Byte fill (fmi_memset_d while a stream captures: no memset graph nodes).
+__global__ void fill_u8_kernel(uint8_t *p, uint8_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 // --------------------------------------------------------------------------
 // mapping-layer 1-error search: BWSearch1VectorHelper (search/search.c:
 // 1491-1709) over the interval vectors of bwt.c:1940-1950 (or, for a seed
 // window, bwt.c:1447-1457)
 // --------------------------------------------------------------------------
 //
-// One lane per (read, strand) runs the helper (vh_task, fmi_vhelper.h) over
-// the read or its window; its vector scratch slice is n + 3 intervals at
-// the read's own offset.
+// One lane per (read, strand) task at a time runs the helper (vh_task,
+// fmi_vhelper.h) over the read or its window; the lanes stride over the
+// tasks, and each owns a vector scratch slice of kMaxReadLen + 3 intervals,
+// so the scratch is sized by the lane count alone (no read of the offsets
+// on the host).
 
 struct VhParams {
   DevIndex ix;
@@ -982,26 +991,30 @@ struct VhParams {
   const int16_t *bounds;    // NULL or [start, end] per read, relative to the read
   fmi_hit *hits;
   uint32_t *counts;
-  uint2 *scratch;           // 2 * (offsets[nreads] + 3 * nreads) intervals
+  uint2 *scratch;           // lanes * (kMaxReadLen + 3) intervals
   uint32_t nreads, cap, strand_mask, mm_only;
 };
 
 __global__ __launch_bounds__(256) void vhelper_kernel(VhParams P) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2u * P.nreads) return;
-  const uint32_t rd = t >> 1, strand = t & 1u;
-  const uint64_t off = P.offsets[rd];
-  const int32_t len = (int32_t)(P.offsets[rd + 1] - off);
-  const uint8_t *W = P.reads + off;
-  int32_t start = 0, end = len - 1;
-  if (P.bounds) { start = P.bounds[2 * rd]; end = P.bounds[2 * rd + 1]; }
-  if (!(P.strand_mask & (1u << strand)) || start < 0 || end >= len || end - start < 2) {
-    P.counts[t] = 0;
-    return;
+  const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lanes = gridDim.x * blockDim.x;
+  uint2 *vb = P.scratch + (uint64_t)lane * (kMaxReadLen + 3u);
+  for (uint32_t t = lane; t < 2u * P.nreads; t += lanes) {
+    const uint32_t rd = t >> 1, strand = t & 1u;
+    const uint64_t off = P.offsets[rd];
+    const uint64_t len64 = P.offsets[rd + 1] - off;
+    const int32_t len = len64 > kMaxReadLen ? -1 : (int32_t)len64;
+    const uint8_t *W = P.reads + off;
+    int32_t start = 0, end = len - 1;
+    if (P.bounds) { start = P.bounds[2 * rd]; end = P.bounds[2 * rd + 1]; }
+    if (!(P.strand_mask & (1u << strand)) || len < 0 || start < 0 || end >= len ||
+        end - start < 2) {
+      P.counts[t] = 0;
+      continue;
+    }
+    P.counts[t] = vh_task(P.ix, W, start, end, strand, P.mm_only, vb,
+                          P.hits + (uint64_t)t * P.cap, P.cap);
   }
-  uint2 *vb = P.scratch + 2 * (off + 3ull * rd) + (uint64_t)strand * (uint64_t)(len + 3);
-  P.counts[t] = vh_task(P.ix, W, start, end, strand, P.mm_only, vb,
-                        P.hits + (uint64_t)t * P.cap, P.cap);
 }
 
 // --------------------------------------------------------------------------
@@ -1401,21 +1414,37 @@ hipError_t launch_compact(const uint32_t *counts, const fmi_hit *hits, uint32_t 
   return hipGetLastError();
 }
 
+hipError_t launch_fill(void *p, uint8_t v, uint64_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  const uint64_t grid = std::min<uint64_t>((bytes + 255) / 256, 65536);
+  hipLaunchKernelGGL(fill_u8_kernel, dim3((uint32_t)grid), dim3(256), 0, s,
+                     static_cast<uint8_t *>(p), v, bytes);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 
+uint32_t vhelper_lanes(uint32_t nreads) {
+  const uint64_t want = (2ull * nreads + 255) / 256 * 256;
+  return (uint32_t)std::min<uint64_t>(want, kVhelperMaxLanes);
+}
+
+uint64_t vhelper_scratch_bytes(uint32_t lanes) {
+  return (uint64_t)lanes * (kMaxReadLen + 3u) * sizeof(uint2);
+}
+
 hipError_t launch_vhelper(const DevIndex &ix, bool mm_only, uint32_t strand_mask,
                           const uint8_t *reads, const uint64_t *offsets, const int16_t *bounds,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
-                          void *scratch, hipStream_t s) {
+                          void *scratch, uint32_t lanes, hipStream_t s) {
   VhParams P{ix, reads, offsets, bounds, hits, counts, static_cast<uint2 *>(scratch), nreads, cap,
              strand_mask, mm_only ? 1u : 0u};
-  const uint64_t grid = (2ull * nreads + 255) / 256;
-  if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL(vhelper_kernel, dim3((uint32_t)grid), dim3(256), 0, s, P);
+  if (nreads == 0 || lanes == 0) return hipSuccess;
+  hipLaunchKernelGGL(vhelper_kernel, dim3(lanes / 256), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 

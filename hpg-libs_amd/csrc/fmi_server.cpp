@@ -445,7 +445,10 @@ int fmi_server_destroy(fmi_server *srv) {
   }
   srv->cv_submit.notify_all();
   if (srv->thr.joinable()) srv->thr.join();
-  for (auto s : srv->streams) (void)hipStreamDestroy(s);
+  for (auto s : srv->streams) {
+    (void)fmi::release_stream_state(s);   // the search's counters on that stream
+    (void)hipStreamDestroy(s);
+  }
   for (auto e : srv->events) (void)hipEventDestroy(e);
   if (srv->h_stage) (void)hipHostFree(srv->h_stage);
   if (srv->d_arena) (void)hipFree(srv->d_arena);

@@ -97,6 +97,10 @@ __device__ __forceinline__ void vh_exact(const VhView &V, const uint8_t *W, int3
   r.k = k2; r.l = l2; r.pos = i;
 }
 
+template <class Sink>
+__device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t start, int32_t end,
+                                 uint32_t strand, uint32_t mm_only, const uint2 *vb, Sink &S);
+
 // The helper over W[start..end] (end - start >= 2) on strand 0 (backward,
 // forward) or 1 (forward_rev, backward_rev); vb: n + 3 intervals of scratch.
 // Every record goes to S.add(r) in the reference's order.
@@ -135,6 +139,20 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
       if (j >= half - 1) vf[j - (half - 1)] = make_uint2(k, l);
     }
   }
+  vh_replay(ix, W, start, end, strand, mm_only, vb, S);
+}
+
+// The helper's record sequence from the vectors in vb (entries 0..n/2 of the
+// backward vector, then entries n/2-1..n of the forward vector).
+template <class Sink>
+__device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t start, int32_t end,
+                                 uint32_t strand, uint32_t mm_only, const uint2 *vb, Sink &S) {
+  const uint32_t n_sa = ix.n;
+  const VhView B{strand ? &ix.d[1] : &ix.d[0], strand, n_sa};
+  const VhView F{strand ? &ix.d[0] : &ix.d[1], strand, n_sa};
+  const int32_t n = end - start;
+  int32_t half = n / 2;
+  const uint2 *vf = vb + (half + 1);
 
   VhRec r{0, 0, -1, 0, 0, 0, 0, 0};
   uint2 v = vb[0];

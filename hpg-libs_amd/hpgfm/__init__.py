@@ -45,16 +45,17 @@ EXPORTS = [
     "fmi_index_contig_name",
     "fmi_index_device_bytes", "fmi_index_to_device", "fmi_index_get_bwt",
     "fmi_index_set_karyotype", "fmi_device_count", "fmi_device_alloc", "fmi_device_free",
-    "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_stream_create",
+    "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_device_mem_info", "fmi_stream_create",
     "fmi_stream_destroy", "fmi_stream_sync", "fmi_graph_begin", "fmi_graph_end",
     "fmi_graph_launch", "fmi_graph_destroy", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
+    "fmi_search1_anchors_batch",
     "fmi_set_tuning", "fmi_get_tuning", "fmi_set_load_mode", "fmi_get_load_mode", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
     "fmi_gather_bench",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_kmer_table", "fmi_index_get_kmer_table",
-    "fmi_index_get_full",
+    "fmi_index_get_full", "fmi_index_sample_count", "fmi_index_get_samples",
     "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
@@ -112,6 +113,7 @@ def _bind(lib):
         "fmi_memcpy_h2d": ([_p, _p, C.c_size_t, _p], _i),
         "fmi_memcpy_d2h": ([_p, _p, C.c_size_t, _p], _i),
         "fmi_memset_d": ([_p, _i, C.c_size_t, _p], _i),
+        "fmi_device_mem_info": ([C.POINTER(_u64), C.POINTER(_u64)], _i),
         "fmi_stream_create": ([C.POINTER(_p)], _i),
         "fmi_stream_destroy": ([_p], _i),
         "fmi_stream_sync": ([_p], _i),
@@ -127,6 +129,7 @@ def _bind(lib):
         "fmi_exact_vector_batch": ([_p, _i, _i, _p, _p, _u32, _p, _p, _p, _p], _i),
         "fmi_search1_batch": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_search1_batch_plain": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
+        "fmi_search1_anchors_batch": ([_p, _i, _p, _p, _u32, _p, _p], _i),
         "fmi_set_tuning": ([_u32, _u32, _u32], _i),
         "fmi_get_tuning": ([C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)], _i),
         "fmi_set_load_mode": ([_u32], _i),
@@ -146,6 +149,8 @@ def _bind(lib):
         "fmi_search_k_batch": ([_p, _i, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p, _p, _p, _p],
                                _i),
         "fmi_index_get_full": ([_p, _i, _p, _p, _p], _i),
+        "fmi_index_sample_count": ([_p], _u64),
+        "fmi_index_get_samples": ([_p, _i, _p, _p], _i),
         "fmi_plain_create": ([_p, C.POINTER(_p)], _i),
         "fmi_plain_search1": ([_p, _i, _p, _p, _u32, _u32, _p, _p, C.POINTER(C.c_double),
                                C.POINTER(C.c_double)], _i),
@@ -196,6 +201,13 @@ def check(rc):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def mem_info():
+    """(free, total) bytes of the current device."""
+    f, t = C.c_uint64(), C.c_uint64()
+    check(lib().fmi_device_mem_info(C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
 
 
 def device_count():
@@ -258,8 +270,11 @@ class Stream:
         check(lib().fmi_stream_sync(self.handle))
 
     def close(self):
+        """Drains the stream and frees it with the library's counters and
+        scratch for it; raises (FMI_ESTATE) while a graph captured on it
+        lives."""
         if self.handle is not None:
-            lib().fmi_stream_destroy(self.handle)
+            check(lib().fmi_stream_destroy(self.handle))
             self.handle = None
 
 
@@ -461,6 +476,14 @@ class Index:
         check(lib().fmi_index_get_full(self.h, forward, _ptr(sa), _ptr(isa), _ptr(text)))
         return sa, isa, text[:(self.n + 15) // 16]
 
+    def samples(self, forward=0):
+        """(S, R): the sampled SA and ISA of one direction (S.vec / R.vec)."""
+        m = int(lib().fmi_index_sample_count(self.h))
+        S = np.empty(m, np.uint32)
+        R = np.empty(m, np.uint32)
+        check(lib().fmi_index_get_samples(self.h, forward, _ptr(S), _ptr(R)))
+        return S, R
+
     def bwt(self, forward=0):
         out = np.empty(max(self.n, 1), np.uint8)
         d = C.c_uint64()
@@ -564,6 +587,20 @@ def search1_batch(ix, batch, cap=64, mode=MODE_MISMATCH, strand_mask=3, plain=Fa
     h = out.hits(stream)
     out.free()
     return c, h
+
+
+def search1_anchors_batch(ix, batch, strand_mask=3, stream=None):
+    """fmi_search1_anchors_batch: BWSearch1CPU's two anchors per read and
+    strand, as a host array [nreads, 2 strands, 2 passes] of HIT_DTYPE."""
+    n = batch.nreads
+    d = DeviceBuffer(max(4 * n, 1) * 32)
+    d.zero(stream)
+    check(lib().fmi_search1_anchors_batch(ix.h, strand_mask, batch.d_reads.ptr,
+                                          batch.d_offsets.ptr, n, d.ptr, stream))
+    sync(stream)
+    out = d.download(HIT_DTYPE, 4 * n, stream).reshape(n, 2, 2)
+    d.free()
+    return out
 
 
 def search1_vector_batch(ix, batch, cap=64, mode=MODE_EDIT, strand_mask=3, bounds=None,

@@ -41,10 +41,10 @@ def replace_n(codes, seed=0):
     return out
 
 
-def genome(n, seed=1, gc=0.41, repeat_frac=0.0, fam_len=(300, 6000),
-           divergence=(0.02, 0.15), n_frac=0.0, n_run=(50, 5000), contigs=1,
-           copies_per_family=20):
-    """Returns (codes uint8[n], contig_starts uint64[c], contig_names list)."""
+def _genome_v1(n, seed, gc, repeat_frac, fam_len, divergence, n_frac, n_run, contigs,
+               copies_per_family):
+    """The first generator (float draws, one copy at a time): kept so that
+    tests/golden/make_golden.py reproduces the committed fixtures."""
     rng = np.random.default_rng(seed)
     g = iid_bases(rng, n, gc)
     if repeat_frac > 0 and n > fam_len[1] * 4:
@@ -65,6 +65,94 @@ def genome(n, seed=1, gc=0.41, repeat_frac=0.0, fam_len=(300, 6000),
                     cp = (3 - cp[::-1]).astype(np.uint8)
                 g[p:p + L] = cp
                 covered += L
+    if n_frac > 0:
+        target = int(n_frac * n)
+        covered = 0
+        while covered < target:
+            L = int(rng.integers(n_run[0], n_run[1] + 1))
+            p = int(rng.integers(0, max(1, n - L)))
+            g[p:p + L] = N
+            covered += L
+    if contigs > 1:
+        cuts = set()
+        while len(cuts) < contigs - 1:
+            cuts.add(int(rng.integers(1000, n - 1000)))
+        starts = np.concatenate([[0], np.sort(np.array(sorted(cuts), dtype=np.int64))]).astype(np.uint64)
+    else:
+        starts = np.zeros(1, np.uint64)
+    names = [f"chr{i + 1}" for i in range(len(starts))]
+    return g, starts, names
+
+
+def _base_lut(gc):
+    """Base of every 16-bit draw: P(A) = P(T) = (1 - gc) / 2, P(C) = P(G) = gc / 2
+    to within 2^-16."""
+    at = (1.0 - gc) / 2.0
+    cuts = np.array([at, 0.5, 0.5 + gc / 2.0]) * 65536.0
+    return np.searchsorted(cuts, np.arange(65536) + 0.5, side="right").astype(np.uint8)
+
+
+def iid_bases_par(seed, n, gc=0.41, chunk=1 << 26, threads=None):
+    """n i.i.d. bases at GC fraction gc from 16-bit draws; chunk i has its own
+    generator spawned from (seed, i), so the output does not depend on how
+    many threads fill it."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    lut = _base_lut(gc)
+    out = np.empty(n, dtype=np.uint8)
+    nch = (n + chunk - 1) // chunk
+    kids = np.random.SeedSequence([int(seed), 0x5EED]).spawn(max(nch, 1))
+
+    def fill(i):
+        a = i * chunk
+        b = min(n, a + chunk)
+        r = np.random.Generator(np.random.PCG64(kids[i]))
+        np.take(lut, r.integers(0, 65536, size=b - a, dtype=np.uint16), out=out[a:b])
+
+    if threads is None:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    if nch <= 1 or threads == 1:
+        for i in range(nch):
+            fill(i)
+    else:
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(fill, range(nch)))
+    return out
+
+
+def genome(n, seed=1, gc=0.41, repeat_frac=0.0, fam_len=(300, 6000),
+           divergence=(0.02, 0.15), n_frac=0.0, n_run=(50, 5000), contigs=1,
+           copies_per_family=20, method="v2"):
+    """Returns (codes uint8[n], contig_starts uint64[c], contig_names list).
+
+    Repeat families: a random consensus of fam_len bases, copies_per_family
+    mutated copies (substitutions at a per-copy divergence drawn from
+    `divergence`, half of them reverse-complemented) pasted at random
+    positions until repeat_frac of the genome is covered.  method "v1" is
+    the first, slower generator (the golden fixtures' genomes)."""
+    if method == "v1":
+        return _genome_v1(n, seed, gc, repeat_frac, fam_len, divergence, n_frac, n_run, contigs,
+                          copies_per_family)
+    g = iid_bases_par(seed, n, gc)
+    rng = np.random.default_rng([int(seed), 1])
+    lut = _base_lut(gc)
+    if repeat_frac > 0 and n > fam_len[1] * 4:
+        target = int(repeat_frac * n)
+        covered = 0
+        while covered < target:
+            L = int(rng.integers(fam_len[0], fam_len[1] + 1))
+            cons = lut[rng.integers(0, 65536, size=L, dtype=np.uint16)]
+            k = int(min(copies_per_family, -(-(target - covered) // L)))
+            ps = rng.integers(0, n - L, size=k)
+            thr = (rng.uniform(divergence[0], divergence[1], size=k) * 65536.0).astype(np.uint32)
+            cp = np.broadcast_to(cons, (k, L)).copy()
+            m = rng.integers(0, 65536, size=(k, L), dtype=np.uint16) < thr[:, None]
+            cp[m] = (cp[m] + rng.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) % 4
+            flip = rng.random(k) < 0.5
+            cp[flip] = (3 - cp[flip])[:, ::-1]
+            for j in range(k):
+                g[ps[j]:ps[j] + L] = cp[j]
+            covered += k * L
     if n_frac > 0:
         target = int(n_frac * n)
         covered = 0

@@ -156,6 +156,13 @@ int fmi_index_get_kmer_table(const fmi_index *ix, int forward, uint64_t *out);
  * (n + 15) / 16 words. */
 int fmi_index_get_full(const fmi_index *ix, int forward, uint32_t *sa, uint32_t *isa,
                        uint32_t *text);
+/* The sampled suffix array and inverse suffix array of one direction, as
+ * the reference's S.vec / R.vec (Si / Ri for the forward direction) hold
+ * them (calculate_S_and_R, search/preprocess.c:7-42): S[i] = SA[i * ratio],
+ * R[j] = ISA[j * ratio], i, j < nsamp = ceil((n + 1) / ratio) entries each
+ * (either pointer may be NULL). */
+uint64_t fmi_index_sample_count(const fmi_index *ix);
+int fmi_index_get_samples(const fmi_index *ix, int forward, uint32_t *S, uint32_t *R);
 /* size_SA (search/runtime.h:86): n + 1. */
 uint64_t fmi_index_size_sa(const fmi_index *ix);
 uint32_t fmi_index_sa_ratio(const fmi_index *ix);
@@ -184,8 +191,15 @@ int fmi_device_alloc(void **d_ptr, size_t bytes);
 int fmi_device_free(void *d_ptr);
 int fmi_memcpy_h2d(void *d_dst, const void *src, size_t bytes, fmi_stream s);
 int fmi_memcpy_d2h(void *dst, const void *d_src, size_t bytes, fmi_stream s);
+/* On a stream that is being captured this records a fill kernel, not a
+ * memset node. */
 int fmi_memset_d(void *d_dst, int value, size_t bytes, fmi_stream s);
+/* Free and total bytes of the current device (hipMemGetInfo). */
+int fmi_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
 int fmi_stream_create(fmi_stream *s);
+/* Drains the stream, frees the work counters and scratch the library keeps
+ * for it, then destroys it; FMI_ESTATE (nothing freed) while a graph
+ * captured on it has not been destroyed. */
 int fmi_stream_destroy(fmi_stream s);
 int fmi_stream_sync(fmi_stream s);
 /* Launch chains as HIP graphs: between fmi_graph_begin and fmi_graph_end
@@ -195,7 +209,10 @@ int fmi_stream_sync(fmi_stream s);
  * capture (it sizes the stream's work counters and scratch then), and a
  * replay reads and writes the buffers the calls named — their contents may
  * change between replays, not their addresses or sizes.  The library's
- * counter resets are kernels, never memset nodes.  No reference
+ * counter resets are kernels, never memset nodes.  While a graph captured on
+ * a stream lives, a call on that stream that needs more scratch than the
+ * stream holds returns FMI_ESTATE instead of moving it (the graph keeps the
+ * address), and so does one that needs scratch during the capture.  No reference
  * counterpart (the reference launches synchronously, gpu.cu:118-203). */
 typedef void *fmi_graph;
 int fmi_graph_begin(fmi_stream s);
@@ -257,6 +274,20 @@ int fmi_search1_batch(const fmi_index *ix, int mode, int strand_mask,
                       const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
                       uint32_t cap, fmi_hit *d_hits, uint32_t *d_counts, fmi_stream s);
 
+/* The anchor list BWSearch1CPU fills when it is given r_list_anchor
+ * (search/search.c:1897-1925): per read i and strand s in strand_mask,
+ * d_anchors[4*i + 2*s] is the anchor of pass 1 (BWExactSearchBackward of
+ * W[len/2 .. len-1] on the pair's first index: dir 0, start = where the
+ * extension stopped + 1, end = len - 1) and d_anchors[4*i + 2*s + 1] the
+ * anchor of pass 2 (BWExactSearchForward of W[0 .. len/2 - 1] on the second:
+ * dir 1, start 0, end = where it stopped - 1); each holds the last non-empty
+ * interval (0, 0 when the first step already empties) and pos = the
+ * position where the extension stopped.  Slots of strands not in the mask
+ * are not written. */
+int fmi_search1_anchors_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
+                              const uint64_t *d_offsets, uint32_t nreads, fmi_hit *d_anchors,
+                              fmi_stream s);
+
 /* The same results from the plain design of the reference's GPU glue
  * (gpu.cu:118-203): one thread per (read, strand), no work queue, one launch
  * per batch.  Kept as the comparison point for the persistent launch. */
@@ -272,9 +303,12 @@ int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
  * forward_rev/backward_rev).  d_bounds is NULL (the whole read, start 0) or
  * int16 [start, end] per read, relative to the read, as bwt_map_inexact_seed
  * passes a seed window (bwt.c:1447-1479).  Records and capacity as
- * fmi_search1_batch; windows shorter than 3 bases (or outside the read) give
- * no records (the reference reads outside its vectors there; the mapping
- * layer calls it for reads of 5 bases or more). */
+ * fmi_search1_batch; windows shorter than 3 bases (or outside the read) and
+ * reads longer than FMI_MAX_READ_LEN give no records (the reference reads
+ * outside its vectors there; the mapping layer calls it for reads of 5 bases
+ * or more).  Its scratch is a fixed slice per lane (at most 64 K lanes of
+ * 8 KB), so the call neither reads the offsets on the host nor synchronises
+ * and can be captured into a graph. */
 int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
                              const uint8_t *d_reads, const uint64_t *d_offsets,
                              const int16_t *d_bounds, uint32_t nreads, uint32_t cap,

@@ -1505,3 +1505,35 @@ int fmo_search_k(const fmo_index *ix, int strand, const uint8_t *W, int16_t len,
   free(D);
   return full ? 1 : 0;
 }
+
+/* fmo_search_k on both strands of every read (fragsize[i] per read), OpenMP
+ * over reads as fmo_search1_batch; task t = 2 * read + strand owns
+ * final_out[t * list_cap ..] / nfinal[t], anchors_out[t * anchor_cap ..] /
+ * nanchors[t] and full[t]. */
+double fmo_search_k_batch(const fmo_index *ix, const uint8_t *reads, const uint64_t *offsets,
+                          uint64_t nreads, const int16_t *fragsize, uint32_t list_cap,
+                          uint32_t anchor_cap, fmo_hitk *final_out, uint32_t *nfinal,
+                          fmo_hitk *anchors_out, uint32_t *nanchors, uint8_t *full, int threads) {
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+  omp_set_num_threads(threads);
+  long chunk = (long)(nreads / ((uint64_t)threads * 10));
+  if (chunk < 1) chunk = 1;
+#endif
+  double t0 = now_s();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, chunk)
+#endif
+  for (long i = 0; i < (long)nreads; i++) {
+    const uint8_t *W = reads + offsets[i];
+    int16_t len = (int16_t)(offsets[i + 1] - offsets[i]);
+    for (int s = 0; s < 2; s++) {
+      const size_t t = (size_t)i * 2 + s;
+      full[t] = (uint8_t)fmo_search_k(ix, s, W, len, fragsize[i], list_cap, anchor_cap,
+                                      final_out + t * list_cap, &nfinal[t],
+                                      anchors_out ? anchors_out + t * anchor_cap : NULL,
+                                      &nanchors[t]);
+    }
+  }
+  return now_s() - t0;
+}

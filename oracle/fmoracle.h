@@ -241,6 +241,13 @@ double fmo_search1_batch(const fmo_index *ix, const uint8_t *reads,
 double fmo_exact_batch(const fmo_index *ix, const uint8_t *reads,
                        const uint64_t *offsets, uint64_t nreads, int64_t *out_kl,
                        int threads);
+/* fmo_search_k on both strands of every read with fragsize[i]: task
+ * t = 2 * read + strand gets final_out[t * list_cap ..], nfinal[t],
+ * anchors_out[t * anchor_cap ..] (may be NULL), nanchors[t], full[t]. */
+double fmo_search_k_batch(const fmo_index *ix, const uint8_t *reads, const uint64_t *offsets,
+                          uint64_t nreads, const int16_t *fragsize, uint32_t list_cap,
+                          uint32_t anchor_cap, fmo_hitk *final_out, uint32_t *nfinal,
+                          fmo_hitk *anchors_out, uint32_t *nanchors, uint8_t *full, int threads);
 /* The same searches, then get_SA(j, backward) over the first cap rows of
  * each interval: nrows / possum[2*i + strand]. */
 double fmo_exact_locate_batch(const fmo_index *ix, const uint8_t *reads,

@@ -134,6 +134,9 @@ def _load_oracle():
     lib.fmo_binsearch.restype = C.c_uint64
     lib.fmo_search1_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_int, C.c_uint32, _p, _p, C.c_int]
     lib.fmo_search1_batch.restype = C.c_double
+    lib.fmo_search_k_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_uint32, C.c_uint32, _p,
+                                       _p, _p, _p, _p, C.c_int]
+    lib.fmo_search_k_batch.restype = C.c_double
     lib.fmo_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
     lib.fmo_exact_batch.restype = C.c_double
     lib.fmo_exact_locate_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, _p, _p, C.c_int]
@@ -207,6 +210,16 @@ class Oracle:
         for fwd, sa, isa in ((0, sa_bwd, isa_bwd), (1, sa_fwd, isa_fwd)):
             S = np.ascontiguousarray(sa[::ratio], dtype=np.uint32)
             R = np.ascontiguousarray(isa[::ratio], dtype=np.uint32)
+            if self.lib.fmo_dir_set_samples(self.ref, fwd, ratio, _ptr(S), _ptr(R), len(S)):
+                raise ValueError("sample arrays do not fit the index")
+        self.ratio = ratio
+
+    def set_sampled(self, ratio, S_bwd, R_bwd, S_fwd, R_fwd):
+        """Installs already-sampled S / R arrays (S.vec, R.vec, Si.vec, Ri.vec:
+        every ratio-th entry of SA and ISA of each direction)."""
+        for fwd, S, R in ((0, S_bwd, R_bwd), (1, S_fwd, R_fwd)):
+            S = np.ascontiguousarray(S, dtype=np.uint32)
+            R = np.ascontiguousarray(R, dtype=np.uint32)
             if self.lib.fmo_dir_set_samples(self.ref, fwd, ratio, _ptr(S), _ptr(R), len(S)):
                 raise ValueError("sample arrays do not fit the index")
         self.ratio = ratio
@@ -344,6 +357,24 @@ class Oracle:
         t = self.lib.fmo_search1_batch(self.ref, _ptr(reads), _ptr(offsets), nr,
                                        int(mismatch_only), cap, _ptr(hits), _ptr(counts), threads)
         return t, counts, hits
+
+    def search_k_batch(self, reads, offsets, fragsize, list_cap=128, anchor_cap=16, threads=0):
+        """BWSearchCPU on both strands of every read: (seconds, nfinal[2n],
+        final[2n * list_cap], nanchors[2n], anchors[2n * anchor_cap],
+        full[n, 2]) in the layout fmi_search_k_batch writes."""
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        fragsize = np.ascontiguousarray(fragsize, dtype=np.int16)
+        nr = len(offsets) - 1
+        fin = np.zeros(max(2 * nr * list_cap, 1), HITK_DTYPE)
+        anc = np.zeros(max(2 * nr * anchor_cap, 1), HITK_DTYPE)
+        nf = np.zeros(max(2 * nr, 1), np.uint32)
+        na = np.zeros(max(2 * nr, 1), np.uint32)
+        full = np.zeros(max(2 * nr, 1), np.uint8)
+        t = self.lib.fmo_search_k_batch(self.ref, _ptr(reads), _ptr(offsets), nr, _ptr(fragsize),
+                                        list_cap, anchor_cap, _ptr(fin), _ptr(nf), _ptr(anc),
+                                        _ptr(na), _ptr(full), threads)
+        return float(t), nf[:2 * nr], fin, na[:2 * nr], anc, full[:2 * nr].reshape(nr, 2)
 
     def exact_batch(self, reads, offsets, threads=0):
         reads = np.ascontiguousarray(reads, dtype=np.uint8)

@@ -81,7 +81,7 @@ def repeat_genome(seed):
 
 
 def main():
-    g, _, _ = synth.genome(64_000, seed=11, repeat_frac=0.2, fam_len=(300, 2000),
+    g, _, _ = synth.genome(64_000, seed=11, repeat_frac=0.2, fam_len=(300, 2000), method="v1",
                            divergence=(0.0, 0.08))
     rng = np.random.default_rng(12)
     lengths = rng.integers(20, 121, size=2000)

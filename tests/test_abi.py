@@ -18,8 +18,33 @@ def declared_functions():
     return sorted(set(re.findall(r"\b(fmi_[a-z0-9_]+)\s*\(", src)))
 
 
+def declared_surface():
+    """Functions include/hpg_search.h declares (the reference-named surface)."""
+    src = open(os.path.join(ROOT, "include", "hpg_search.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    lines, in_macro = [], False
+    for line in src.splitlines():
+        if line.lstrip().startswith("#") or in_macro:
+            in_macro = line.rstrip().endswith("\\")
+            continue
+        lines.append(line)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", "\n".join(lines))
+    return sorted(set(names))
+
+
 def test_header_and_binding_agree():
     assert declared_functions() == sorted(hpgfm.EXPORTS)
+
+
+def test_library_exports_the_reference_surface():
+    lib = C.CDLL(hpgfm.LIB_PATH)
+    names = declared_surface()
+    for want in ("BWExactSearchBackward", "BWExactSearchForward", "BWExactSearchVectorBackward",
+                 "BWExactSearchVectorForward", "BWSearch1CPU", "BWSearch1VectorHelper",
+                 "BWSearchCPU", "binsearch", "load_bwt_index", "free_bwt_index"):
+        assert want in names
+    missing = [f for f in names if not hasattr(lib, f)]
+    assert not missing, missing
 
 
 def test_library_exports_every_declared_symbol():

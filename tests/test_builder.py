@@ -112,6 +112,83 @@ def test_load_rejects_corrupt_and_truncated(tmp_path):
     assert e.value.code == -3
 
 
+def _fnv1a(data):
+    h = 1469598103934665603
+    for b in data:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _layout(data):
+    """Offsets of the parts of an index file (fmi_build.cpp's format 3)."""
+    import struct
+    n, = struct.unpack_from("<Q", data, 16)
+    dirs = []
+    o = 32
+    for _ in range(2):
+        dollar, = struct.unpack_from("<Q", data, o)
+        nblocks, nsamp = struct.unpack_from("<QQ", data, o + 24)
+        dirs.append({"hdr": o, "nblocks": nblocks, "nsamp": nsamp})
+        o += 40
+    kc, = struct.unpack_from("<Q", data, o)
+    kpos = o + 8
+    o = kpos + 16 * kc
+    for _ in range(kc):
+        ln, = struct.unpack_from("<I", data, o)
+        o += 4 + ln
+    for d in dirs:
+        d["blk"] = o
+        o += 64 * d["nblocks"]
+        d["S"] = o
+        o += 4 * d["nsamp"]
+        d["R"] = o
+        o += 4 * d["nsamp"]
+    assert o + 8 == len(data)
+    return n, dirs, kc, kpos
+
+
+def test_load_rejects_inconsistent_contents(tmp_path):
+    """A file whose checksum is right but whose contents are not: every value
+    the kernels index with is checked after the checksum."""
+    import struct
+    g, _, _ = synth.genome(5000, seed=4)
+    ix = hpgfm.Index.build(g, 8)
+    ix.set_karyotype([0, 0], [1999, 2999])
+    p = str(tmp_path / "x.fmi")
+    ix.save(p)
+    data = bytes(open(p, "rb").read())
+    n, dirs, kc, kpos = _layout(data)
+    assert n == 5000 and kc == 2
+
+    def patched(edit):
+        b = bytearray(data[:-8])
+        edit(b)
+        b += struct.pack("<Q", _fnv1a(b))
+        return bytes(b)
+
+    good = patched(lambda b: None)
+    assert good == data                         # the checksum is recomputed exactly
+    cases = {
+        "S sample past n": lambda b: struct.pack_into("<I", b, dirs[0]["S"] + 4 * 3, n + 5),
+        "R sample past n": lambda b: struct.pack_into("<I", b, dirs[1]["R"] + 4 * 7, 0xFFFFFFF0),
+        "block count": lambda b: struct.pack_into("<I", b, dirs[0]["blk"] + 64 * 5 + 4,
+                                                  struct.unpack_from("<I", b, dirs[0]["blk"] + 64 * 5 + 4)[0] + 1),
+        "bit plane": lambda b: b.__setitem__(dirs[1]["blk"] + 64 * 2 + 16, b[dirs[1]["blk"] + 64 * 2 + 16] ^ 0x01),
+        "C": lambda b: struct.pack_into("<I", b, dirs[0]["hdr"] + 8 + 8,
+                                        struct.unpack_from("<I", b, dirs[0]["hdr"] + 8 + 8)[0] + 1),
+        "karyotype sum": lambda b: struct.pack_into("<Q", b, kpos + 8 * kc + 8, 3000),
+        "karyotype order": lambda b: struct.pack_into("<Q", b, kpos, 2500),
+    }
+    for name, edit in cases.items():
+        q = str(tmp_path / "bad.fmi")
+        open(q, "wb").write(patched(edit))
+        with pytest.raises(hpgfm.FmiError) as e:
+            hpgfm.Index.load(q)
+        assert e.value.code == -4, name
+    open(p + ".ok", "wb").write(good)
+    hpgfm.Index.load(p + ".ok")
+
+
 def test_build_argument_errors():
     with pytest.raises(hpgfm.FmiError) as e:
         hpgfm.Index.build(np.array([0, 1, 4, 2], np.uint8), 8)      # N must be replaced

@@ -9,9 +9,19 @@ k = 15 table, under both load modes, give the same counts and records; every
 read drawn without a substitution or an N has an exact record on its strand
 whose interval holds the row of the drawn position, the row here found by
 the engine's own get_ISA (fmi_inverse_locate_batch, runtime.h:88) on the
-compact index, which walks LF from the sampled ISA.  On the same index: the
-exact seed regions of planted seeds, and the serving loop's requests under
+compact index, which walks LF from the sampled ISA.
+
+Independently of the engine: 2000 reads give the oracle's BWSearch1CPU
+records in both modes (the oracle built from this index's BWTs and samples,
+on the reference's own occurrence layout), rows past 2^31 among them; and
+error-free reads resolved to coordinates (fmi_resolve_hits, bwt.c:2158-2166)
+land on their drawn position, every resolved coordinate holding the read in
+the indexed text, coordinates past 2^31 among them.  On the same index: the
+exact seed regions of the whole 2M x 250 bp seed workload against the
+positions its seeds were cut from, and the serving loop's requests under
 Poisson load against the batch call alone."""
+import os
+
 import numpy as np
 import pytest
 
@@ -39,13 +49,14 @@ def huge(gpu_engine):
                            divergence=(0.01, 0.20), n_frac=0.001)
     reads, offs, truth = synth.reads(g, NREADS, length=LEN, seed=2, sub_rate=0.01)
     say("index")
-    ix = hp.Index.build_gpu(synth.replace_n(g, seed=1), 16)
+    text = synth.replace_n(g, seed=1)
+    ix = hp.Index.build_gpu(text, 16)
     batch = hp.ReadBatch(reads, offs)
     out = hp.Search1Output(NREADS, CAP)
     hp.search1_launch(ix, batch, out, mode=hp.MODE_MISMATCH)
     hp.sync()
     compact = (out.counts().reshape(-1, 2), out.hits())
-    state = dict(hp=hp, g=g, reads=reads, truth=truth, ix=ix, batch=batch, out=out,
+    state = dict(hp=hp, g=g, text=text, reads=reads, truth=truth, ix=ix, batch=batch, out=out,
                  compact=compact)
     yield state
     hp.set_load_mode(hp.LOADS_AUTO)
@@ -94,51 +105,150 @@ def test_3g_planted_positions(huge):
 
 
 def test_3g_seed_regions_planted(huge):
-    """Exact seeds (bwt_map_exact_seed, bwt.c:1200-1294) of 100K reads of 250
-    bp, 18 bases at stride 5, on the compact index: every seed without a
-    substitution or an N yields, on its read's strand, the region of the
-    position it was cut from (1-based, one contig from coordinate 0), unless
-    that strand passed the 1000-row filter, in which case the strand has no
-    region for the seed at all."""
+    """Exact seeds (bwt_map_exact_seed, bwt.c:1200-1294) of the whole seed
+    workload, 2M reads of 250 bp, 18 bases at stride 5 (94M seeds, one call),
+    on the compact index: every seed without a substitution or an N yields,
+    on its read's strand, the region of the position it was cut from
+    (1-based, one contig from coordinate 0), unless that strand passed the
+    1000-row filter, in which case the strand has no region for the seed at
+    all; every such region spans the seed's 18 bases minus one (the
+    reference's one-short length)."""
     hp, g, ix = huge["hp"], huge["g"], huge["ix"]
-    say("exact seeds")
-    n, L, sl, st = 100_000, 250, 18, 5
+    n, L, sl, st = 2_000_000, 250, 18, 5
+    say(f"exact seeds of {n} x {L} bp")
     reads, offs, truth = synth.reads(g, n, length=L, seed=4, sub_rate=0.01)
     batch = hp.ReadBatch(reads, offs)
     try:
-        nseeds, nrows, regs, _ = hp.seed_map_batch(ix, batch, sl, st, 1000)
+        nseeds, nrows, regs, _ = hp.seed_map_batch(ix, batch, sl, st, 1000, with_info=False)
     finally:
         batch.free()
-    assert len(regs) == nrows
+    s0 = np.arange(0, L - sl + 1, st)
+    assert nseeds == n * len(s0) and len(regs) == nrows
+    say(f"{nseeds} seeds, {nrows} regions; checking")
     pos = truth["pos"].astype(np.int64)
     strand = truth["strand"].astype(np.int64)
-    src = g[pos[:, None] + np.arange(L)[None, :]]
-    rc = strand == 1
-    src[rc] = (3 - src[rc])[:, ::-1]
     mat = reads.reshape(n, L)
-    bad = np.concatenate([np.zeros((n, 1), np.int32),
-                          np.cumsum((src != mat) | (mat > 3), axis=1, dtype=np.int32)], axis=1)
-    s0 = np.arange(0, L - sl + 1, st)
-    clean = (bad[:, s0 + sl] - bad[:, s0]) == 0                     # [n, seeds]
-    ri, q = np.nonzero(clean)
-    off = s0[q]
-    tpos = np.where(strand[ri] == 0, pos[ri] + off, pos[ri] + L - off - sl)
-    assert int(tpos.max()) >= 1 << 31
 
     def key(read, seed, s, start):
         return ((read.astype(np.uint64) << np.uint64(39)) | (seed.astype(np.uint64) << np.uint64(33))
                 | (s.astype(np.uint64) << np.uint64(32)) | start.astype(np.uint64))
 
-    v = regs[regs["valid"] == 1]
-    have = key(v["read"], v["seed_id"], v["strand"], v["start"])
-    want = key(ri, q, strand[ri], tpos + 1)
-    hit = np.isin(want, have)
-    groups = np.unique(have >> np.uint64(32))
-    # a missing planted region only where the seed's strand has no region at all
-    assert not np.any(np.isin(want[~hit] >> np.uint64(32), groups))
-    assert hit.mean() > 0.9, hit.mean()
-    ends = v["end"][np.isin(have, want)] - v["start"][np.isin(have, want)]
-    assert np.all(ends == sl - 1)
+    bounds = np.searchsorted(regs["read"], np.arange(0, n + 1, 250_000))
+    hits = misses = 0
+    top = 0
+    for c, lo in enumerate(range(0, n, 250_000)):
+        hi = min(n, lo + 250_000)
+        src = g[pos[lo:hi, None] + np.arange(L)[None, :]]
+        rc = strand[lo:hi] == 1
+        src[rc] = (3 - src[rc])[:, ::-1]
+        m = mat[lo:hi]
+        bad = np.concatenate([np.zeros((hi - lo, 1), np.int32),
+                              np.cumsum((src != m) | (m > 3), axis=1, dtype=np.int32)], axis=1)
+        clean = (bad[:, s0 + sl] - bad[:, s0]) == 0
+        ri, q = np.nonzero(clean)
+        ri = ri + lo
+        off = s0[q]
+        tpos = np.where(strand[ri] == 0, pos[ri] + off, pos[ri] + L - off - sl)
+        top = max(top, int(tpos.max()))
+        v = regs[bounds[c]:bounds[c + 1]]
+        v = v[v["valid"] == 1]
+        have = key(v["read"], v["seed_id"], v["strand"], v["start"])
+        want = key(ri, q, strand[ri], tpos + 1)
+        hit = np.isin(want, have)
+        groups = np.unique(have >> np.uint64(32))
+        # a missing planted region only where the seed's strand has no region at all
+        assert not np.any(np.isin(want[~hit] >> np.uint64(32), groups)), c
+        sel = np.isin(have, want)
+        assert np.all(v["end"][sel] - v["start"][sel] == sl - 1), c
+        hits += int(hit.sum())
+        misses += int((~hit).sum())
+    assert top >= 1 << 31
+    assert hits / (hits + misses) > 0.9, (hits, misses)
+
+
+def test_3g_sample_vs_oracle(huge):
+    """2000 reads against the oracle's BWSearch1CPU (search/search.c:1876-1933)
+    in mismatch mode (the compact index's records of the whole batch) and in
+    edit mode; the oracle is built from this index's BWTs and S / R samples
+    on the reference's own per-symbol occurrence layout."""
+    hp, ix = huge["hp"], huge["ix"]
+    say("oracle from the index's BWTs and samples")
+    o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
+    o.set_sampled(ix.sa_ratio, *ix.samples(0), *ix.samples(1))
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    pick = np.sort(np.random.default_rng(33).choice(NREADS, 2000, replace=False))
+    rs = huge["reads"].reshape(NREADS, LEN)[pick].reshape(-1)
+    os_ = (np.arange(len(pick) + 1) * LEN).astype(np.uint64)
+    counts, hits = huge["compact"]
+    _, oc, oh = o.search1_batch(rs, os_, cap=CAP, mismatch_only=True, threads=threads)
+    assert same_records((counts[pick], hits[pick]), (oc.reshape(-1, 2), oh.reshape(-1, 2, CAP)))
+    stored = np.arange(CAP)[None, None, :] < np.minimum(oc.reshape(-1, 2), CAP)[:, :, None]
+    assert int(oh.reshape(-1, 2, CAP)[stored]["l"].max()) >= 1 << 31
+    say("mismatch mode: ok")
+    sub = hp.ReadBatch(rs, os_)
+    try:
+        ge = hp.search1_batch(ix, sub, cap=CAP, mode=hp.MODE_EDIT)
+    finally:
+        sub.free()
+    _, oc, oh = o.search1_batch(rs, os_, cap=CAP, mismatch_only=False, threads=threads)
+    assert same_records(ge, (oc.reshape(-1, 2), oh.reshape(-1, 2, CAP)))
+    say("edit mode: ok")
+    o.close()
+
+
+def test_3g_planted_coordinates(huge):
+    """Error-free reads resolved to genome coordinates (fmi_resolve_hits,
+    bwt.c:2158-2166; one contig from coordinate 0): the exact record on the
+    read's strand resolves to the drawn position when its interval fits the
+    row capacity, and every coordinate it resolves to holds the read (its
+    reverse complement on the - strand) in the indexed text."""
+    hp, ix, text = huge["hp"], huge["ix"], huge["text"]
+    say("planted coordinates")
+    pos = huge["truth"]["pos"].astype(np.int64)
+    strand = huge["truth"]["strand"].astype(np.int64)
+    mat = huge["reads"].reshape(NREADS, LEN)
+    cand = np.random.default_rng(34).choice(NREADS, 60_000, replace=False)
+    src = text[pos[cand, None] + np.arange(LEN)[None, :]]
+    rc = strand[cand] == 1
+    src[rc] = (3 - src[rc])[:, ::-1]
+    m = mat[cand]
+    pick = np.sort(cand[np.all(src == m, axis=1) & np.all(m < 4, axis=1)][:20_000])
+    assert len(pick) == 20_000 and int(pos[pick].max()) >= 1 << 31
+    rs = mat[pick].reshape(-1)
+    os_ = (np.arange(len(pick) + 1) * LEN).astype(np.uint64)
+    row_cap = 16
+    sub = hp.ReadBatch(rs, os_)
+    out = hp.Search1Output(len(pick), CAP)
+    try:
+        hp.search1_launch(ix, sub, out, mode=hp.MODE_MISMATCH)
+        hp.sync()
+        chrom, coord = hp.resolve_hits(ix, sub, out, row_cap)
+        counts = out.counts().reshape(-1, 2)
+        recs = out.hits()
+    finally:
+        out.free()
+        sub.free()
+    found = 0
+    for j, i in enumerate(pick):
+        s = int(strand[i])
+        want = mat[i] if s == 0 else synth.revcomp(mat[i])
+        on_target = False
+        for h in range(min(int(counts[j, s]), CAP)):
+            r = recs[j, s, h]
+            if r["num_mismatches"]:
+                continue
+            nrow = int(r["l"] - r["k"] + 1)
+            cs = coord[j, s, h, :min(nrow, row_cap)].astype(np.int64)
+            assert np.all(chrom[j, s, h, :min(nrow, row_cap)] == 0), i
+            for c in cs:
+                assert np.array_equal(text[c:c + LEN], want), (i, c)
+            if nrow <= row_cap:
+                on_target |= bool(np.any(cs == pos[i]))
+            else:
+                on_target = True
+        assert on_target, i
+        found += 1
+    assert found == len(pick)
 
 
 def test_3g_modes_agree(huge):

@@ -82,3 +82,140 @@ def test_graph_replays_equal_eager(gpu_engine):
         other.free()
         st.close()
         ix.close()
+
+
+def _k_buffers(hp, n, list_cap, anchor_cap):
+    return [hp.DeviceBuffer(2 * n * list_cap * 64), hp.DeviceBuffer(2 * n * 4),
+            hp.DeviceBuffer(2 * n * anchor_cap * 64), hp.DeviceBuffer(2 * n * 4),
+            hp.DeviceBuffer(2 * n)]
+
+
+def test_capture_pins_stream_scratch(gpu_engine):
+    """Once a graph is captured on a stream, a call that would need more
+    scratch on that stream is refused (FMI_ESTATE) instead of moving the
+    scratch the graph writes; the replay still equals the eager records, and
+    destroying the graph lifts the pin.  A stream with a live graph cannot be
+    destroyed."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(200_000, seed=111, repeat_frac=0.2, fam_len=(300, 2000))
+    ix = hp.Index.build(g, 16).to_device()
+    reads, offs, _ = synth.reads(g, 400, length=90, seed=112, sub_rate=0.02)
+    batch = hp.ReadBatch(reads, offs)
+    n = batch.nreads
+    st = hp.Stream()
+    list_cap, anchor_cap = 16, 4
+    fs = hp.DeviceBuffer.from_array(np.full(n, 30, np.int16))
+    k_out = _k_buffers(hp, n, list_cap, anchor_cap)
+    out = hp.Search1Output(n, 8)
+    big = _k_buffers(hp, n, 1024, anchor_cap)
+
+    def chain():
+        hp.search_k_launch(ix, batch, fs, list_cap, anchor_cap, *k_out, stream=st.handle)
+        hp.lib().fmi_search1_vector_batch(ix.h, hp.MODE_EDIT, 3, batch.d_reads.ptr,
+                                          batch.d_offsets.ptr, None, n, 8, out.d_hits.ptr,
+                                          out.d_counts.ptr, st.handle)
+
+    def results():
+        st.sync()
+        return (k_out[1].download(np.uint32, 2 * n, st.handle),
+                k_out[0].download(hp.HITK_DTYPE, 2 * n * list_cap, st.handle),
+                out.counts(st.handle), out.hits(st.handle))
+
+    for b in k_out + [out.d_counts, out.d_hits]:
+        b.zero(st.handle)
+    chain()
+    want = results()
+    graph = hp.Graph(st)
+    with graph:
+        chain()
+    try:
+        with pytest.raises(hp.FmiError) as e:       # 64x the list capacity: more scratch
+            hp.search_k_launch(ix, batch, fs, 1024, anchor_cap, *big, stream=st.handle)
+        assert e.value.code == -7
+        with pytest.raises(hp.FmiError) as e:
+            st.close()
+        assert e.value.code == -7 and st.handle is not None
+        for b in k_out + [out.d_counts, out.d_hits]:
+            b.zero(st.handle)
+        graph.launch()
+        got = results()
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[2], want[2])
+        for t in range(2 * n):
+            c = min(int(want[0][t]), list_cap)
+            assert fo.hitsk_equal(got[1][t * list_cap:t * list_cap + c],
+                                  want[1][t * list_cap:t * list_cap + c]), t
+        cs = np.minimum(want[2], 8)
+        for t in range(2 * n):
+            assert fo.hits_equal(got[3].reshape(-1, 8)[t, :cs[t]], want[3].reshape(-1, 8)[t, :cs[t]]), t
+    finally:
+        graph.close()
+    # the pin is gone with the graph
+    hp.search_k_launch(ix, batch, fs, 1024, anchor_cap, *big, stream=st.handle)
+    st.sync()
+    st.close()
+    for b in k_out + big + [fs]:
+        b.free()
+    out.free()
+    batch.free()
+    ix.close()
+
+
+def test_memset_in_capture_is_a_kernel(gpu_engine):
+    """DeviceBuffer.zero (fmi_memset_d) inside a capture records a fill
+    kernel: every replay clears the buffer again after eager writes."""
+    hp = gpu_engine
+    st = hp.Stream()
+    pat = np.arange(1 << 16, dtype=np.uint32) | 1
+    buf = hp.DeviceBuffer.from_array(pat, st.handle)
+    graph = hp.Graph(st)
+    with graph:
+        hp.check(hp.lib().fmi_memset_d(buf.ptr, 0x5A, buf.nbytes - 12, st.handle))
+    try:
+        for rep in range(3):
+            buf.upload(pat, st.handle)
+            graph.launch()
+            got = buf.download(np.uint8, buf.nbytes, st.handle)
+            assert np.all(got[:-12] == 0x5A), rep
+            assert np.array_equal(got[-12:], pat.view(np.uint8)[-12:]), rep
+    finally:
+        graph.close()
+    buf.free()
+    st.close()
+
+
+def test_stream_destroy_releases_its_memory(gpu_engine):
+    """50 cycles of create a stream, run searches that allocate its counters
+    and scratch (the k-error search's lists, the vector helper's slices),
+    destroy it: the device's free memory stays flat."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(100_000, seed=121)
+    ix = hp.Index.build(g, 16).to_device()
+    reads, offs, _ = synth.reads(g, 3000, length=120, seed=122, sub_rate=0.02)
+    batch = hp.ReadBatch(reads, offs)
+    n = batch.nreads
+    fs = hp.DeviceBuffer.from_array(np.full(n, 40, np.int16))
+    k_out = _k_buffers(hp, n, 256, 4)
+    out = hp.Search1Output(n, 8)
+
+    def cycle():
+        st = hp.Stream()
+        hp.search_k_launch(ix, batch, fs, 256, 4, *k_out, stream=st.handle)
+        hp.check(hp.lib().fmi_search1_vector_batch(ix.h, hp.MODE_EDIT, 3, batch.d_reads.ptr,
+                                                   batch.d_offsets.ptr, None, n, 8, out.d_hits.ptr,
+                                                   out.d_counts.ptr, st.handle))
+        hp.search1_launch(ix, batch, out, stream=st.handle)
+        st.sync()
+        st.close()
+
+    cycle()
+    free0, _ = hp.mem_info()
+    for _ in range(50):
+        cycle()
+    free1, _ = hp.mem_info()
+    # one cycle's scratch is hundreds of MB; 50 leaked cycles would be tens of GB
+    assert free0 - free1 < (64 << 20), (free0, free1)
+    for b in k_out + [fs]:
+        b.free()
+    out.free()
+    batch.free()
+    ix.close()
