@@ -37,7 +37,7 @@ def cpu_threads():
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, n)
 
 
 # Genome / read models.  "c2" is the configuration the metric is quoted on
@@ -458,22 +458,452 @@ def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000
 
 
 # Measured HBM traffic of one timed launch, from the committed rocprofv3
-# FETCH_SIZE / WRITE_SIZE summaries (scaled by the copy-kernel calibration).
-# (preset, reads, read length, text mode, k-mer table k) -> calibrated
-# FETCH + WRITE summary of that search1 launch under profiles/
+# FETCH_SIZE / WRITE_SIZE summaries (scaled by the copy / gather calibration;
+# written by --summarize-pmc).  A summary counts only for the launch shape it
+# was profiled at: the timed launch's tuning must equal the one it records.
 TRAFFIC_FILES = {
     ("c2", 1_000_000, 150, True, 15): "search1_textmode_kmer15_64Mbp_1Mx150bp_pmc_traffic.json",
     ("t3g", 2_000_000, 150, True, 15): "search1_textmode_kmer15_3100Mbp_2Mx150bp_pmc_traffic.json",
 }
 
 
-def measured_traffic(preset, reads, read_len, text_mode, kmer):
+def measured_traffic(preset, reads, read_len, text_mode, kmer, tuning):
+    """(bytes per launch, summary name) when a committed summary was
+    profiled at this workload and this launch shape, else (None, reason)."""
     name = TRAFFIC_FILES.get((preset, reads, read_len, text_mode, kmer))
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
-        return None
+        return None, "no profile of this workload"
     with open(path) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        d = json.load(f)
+    shape = d.get("tuning")
+    mine = [tuning.get(k) for k in ("wg_per_cu", "chunk", "refill_min", "loads")] if tuning else None
+    if shape is None or mine is None or list(shape) != mine:
+        return None, f"profiled at tuning {shape}, timed at {mine}"
+    return d.get("traffic_bytes_per_launch"), name
+
+
+def summarize_pmc(fetch_dir, write_dir, out_path, tuning, workload):
+    """Writes the traffic summary of the search launch from two rocprofv3
+    --pmc runs (FETCH_SIZE, then WRITE_SIZE) of the same bench command with
+    --calibrate: the copy kernel's FETCH_SIZE calibrates 128-byte streaming
+    reads, the 64-byte gather's the search's random 64-byte reads."""
+    import csv
+    import glob
+
+    def rows(d):
+        out = []
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn) as f:
+                out.extend(csv.DictReader(f))
+        return out
+
+    def per_kernel(d, counter):
+        acc = {}
+        for r in rows(d):
+            if r.get("Counter_Name") != counter:
+                continue
+            key = (r["Kernel_Name"], int(r.get("Dispatch_Id", 0)))
+            acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
+        return acc
+
+    fetch = per_kernel(fetch_dir, "FETCH_SIZE")
+    write = per_kernel(write_dir, "WRITE_SIZE")
+
+    def pick(acc, word):
+        return [v for (k, _), v in sorted(acc.items(), key=lambda x: x[0][1]) if word in k]
+
+    s_f = pick(fetch, "search1_kernel<true, true, false")
+    s_w = pick(write, "search1_kernel<true, true, false")
+    c_f, c_w = pick(fetch, "copy_kernel"), pick(write, "copy_kernel")
+    g_f = pick(fetch, "gather_kernel<4>")
+    copy_bytes = 1 << 30
+    fetch_copy = copy_bytes / (c_f[0] * 1024)
+    write_copy = copy_bytes / (c_w[0] * 1024)
+    # the first gather of the calibration runs over an index-sized buffer
+    fetch_g64 = (1 << 24) * 64 / (g_f[0] * 1024)
+    kib_f = sum(s_f) / len(s_f)
+    kib_w = sum(s_w) / len(s_w)
+    traffic = kib_f * 1024 * fetch_g64 + kib_w * 1024 * write_copy
+    d = {"workload": workload, "tuning": tuning, "dispatches": len(s_f),
+         "fetch_size_kib_per_launch": round(kib_f), "write_size_kib_per_launch": round(kib_w),
+         "fetch_scale_64B_random": round(fetch_g64, 4), "fetch_scale_copy": round(fetch_copy, 4),
+         "write_scale_copy": round(write_copy, 4), "traffic_bytes_per_launch": round(traffic)}
+    with open(out_path, "w") as f:
+        json.dump(d, f, indent=1)
+    print(json.dumps(d))
+
+
+PROBE_GRID = {"lines_in_flight": (1, 2, 4, 8, 16), "wg_per_cu": (2, 4, 8, 16),
+              "cooperative": (False, True)}
+
+
+def probe_sweep(hpgfm, nbytes, nlines=1 << 25, grid=None):
+    """The random 64-byte gather over a buffer of nbytes at every point of the
+    grid (lines in flight per lane, workgroups per CU, per-lane or
+    cooperative quad loads): the best point is the measured random-access
+    ceiling at that footprint."""
+    grid = grid or PROBE_GRID
+    pr = hpgfm.GatherProbe(nbytes)
+    pts = []
+    try:
+        for coop in grid["cooperative"]:
+            for f in grid["lines_in_flight"]:
+                if not coop and f > 8:
+                    continue          # 16 lines x 64 B per lane does not fit the registers
+                for w in grid["wg_per_cu"]:
+                    pr.run(f, w, coop, 1 << 20)
+                    gbps, glps, ms = pr.run(f, w, coop, nlines)
+                    pts.append({"lines_in_flight": f, "wg_per_cu": w, "cooperative": coop,
+                                "GBps": round(gbps, 1), "Glines_per_s": round(glps, 2)})
+    finally:
+        pr.close()
+    best = max(pts, key=lambda p: p["GBps"])
+    return {"buffer_bytes": nbytes, "best": best, "points": pts}
+
+
+def tuning_key(preset, reads, read_len, kmer, text_mode, mode):
+    return (f"search1/{preset}/{reads}x{read_len}/k{kmer}/"
+            f"{'text' if text_mode else 'compact'}/{'mismatch' if mode == 1 else 'edit'}")
+
+
+def choose_tuning(args, hpgfm, ix, g, synth, key, mode, nreads):
+    """The launch shape of the timed search: --tuning, else the persisted
+    autotuner result for this device and workload, else (or with --retune)
+    the autotuner on a tuning batch of the workload's shape drawn with its
+    own seed, persisted for later runs."""
+    from hpgfm import autotune as at
+    if args.tuning:
+        parts = [int(x) for x in args.tuning.split(",")]
+        w, c, r = parts[:3]
+        loads = parts[3] if len(parts) > 3 else hpgfm.LOADS_AUTO
+        entry = {"wg_per_cu": w, "chunk": c, "refill_min": r, "loads": loads}
+        at.install(entry)
+        return dict(entry, source="--tuning")
+    entry = None if args.retune else at.load(key)
+    if entry:
+        at.install(entry)
+        return dict(entry, source="persisted autotuner result (hpg-libs_amd/hpgfm/tuning.json)")
+    if args.no_autotune:
+        return None
+    t0 = time.time()
+    n = min(nreads, 1_000_000)
+    treads, toffs, _ = synth.reads(g, n, length=args.read_len, seed=9, sub_rate=args.sub_rate)
+    tb = hpgfm.ReadBatch(treads, toffs)
+    tout = hpgfm.Search1Output(n, args.cap)
+    try:
+        best, best_ms, table = at.autotune(ix, tb, tout, mode=mode, log=log)
+    finally:
+        tout.free()
+        tb.free()
+    entry = {"wg_per_cu": best[0], "chunk": best[1], "refill_min": best[2], "loads": best[3],
+             "tuned_ms": round(best_ms, 4), "tuning_batch": f"{n} reads, seed 9",
+             "points": len(table) - 1, "plain_ms": round(table["plain"], 4)}
+    try:
+        at.save(key, entry, args.tuning_out or at.TUNING_FILE)
+        saved = args.tuning_out or at.TUNING_FILE
+    except OSError:
+        saved = None
+    log(f"autotuned in {time.time() - t0:.1f}s (saved to {saved})")
+    return dict(entry, source="autotuner in this run on a separate tuning batch")
+
+
+def random_footprint(ix):
+    """Bytes of the arrays the search reads at random: occurrence blocks and
+    samples, the full SA / ISA and 2-bit text, the k-mer tables."""
+    d = ix.layout_bytes()
+    d["total"] = sum(d.values())
+    return d
+
+
+def search1_point(args, hpgfm, synth, preset, extras=True, cpu_baseline=True, serve_s=0.0,
+                  probe=True):
+    """Builds the preset's genome and index, then times the 1-mismatch search
+    over its batch of reads (text mode with the k-mer table unless
+    --no-expand): the headline measurement at c2 and at the 3.1 Gbp target
+    point.  Returns the JSON object of that point."""
+    pre = PRESETS[preset]
+    mbp = args.genome_mbp if (preset == args.preset and args.genome_mbp) else pre["genome_mbp"]
+    nreads = args.reads if (preset == args.preset and args.reads) else pre["reads"]
+    n_bp = int(mbp * 1_000_000)
+    kmer = args.kmer
+    if kmer < 0:
+        # two bases past the smallest k with 4^k >= n (a k-mer of the text
+        # then names one row, and a k-mer absent from it ends a search at
+        # the lookup), at most 15 (8 GiB per direction)
+        k = 4
+        while 4 ** k < n_bp:
+            k += 1
+        kmer = min(15, k + 2)
+    t0 = time.time()
+    g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=pre["repeat_frac"],
+                           fam_len=(300, 6000), divergence=pre["div"], n_frac=pre["n_frac"])
+    log(f"[{preset}] genome {n_bp} bp in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
+    log(f"[{preset}] index built on the GPU in {time.time() - t0:.1f}s "
+        f"({ix.device_bytes / 1e6:.1f} MB on device)")
+    t0 = time.time()
+    reads, offs, _ = synth.reads(g, nreads, length=args.read_len, seed=2,
+                                 sub_rate=args.sub_rate, both_strands=True)
+    batch = hpgfm.ReadBatch(reads, offs)
+    log(f"[{preset}] {nreads} reads generated and resident in {time.time() - t0:.1f}s")
+    mode = hpgfm.MODE_MISMATCH
+    out = hpgfm.Search1Output(batch.nreads, args.cap)
+    res = {}
+    try:
+        block_mode = None
+        if not args.no_expand:
+            if extras:
+                # the same search on the compact index alone (occurrence
+                # blocks for every step), default launch shape, for comparison
+                hpgfm.search1_launch(ix, batch, out, mode=mode)
+                hpgfm.sync()
+                with hpgfm.TimingRegion() as tb:
+                    for _ in range(3):
+                        hpgfm.search1_launch(ix, batch, out, mode=mode)
+                    hpgfm.sync()
+                block_mode = {"kernel_ms": round(tb.mean_ms, 4),
+                              "reads_per_s": round(nreads / (tb.mean_ms / 1000.0))}
+                log(f"[{preset}] block mode (compact index): {tb.mean_ms:.3f} ms")
+            t0 = time.time()
+            ix.expand()
+            hpgfm.sync()
+            log(f"[{preset}] index expanded in {time.time() - t0:.2f}s, "
+                f"{ix.device_bytes / 1e9:.2f} GB on device")
+        if kmer:
+            t0 = time.time()
+            ix.kmer_table(kmer)
+            log(f"[{preset}] k-mer interval table (k = {kmer}) built in {time.time() - t0:.2f}s, "
+                f"{ix.device_bytes / 1e9:.2f} GB on device")
+        text_mode = not args.no_expand
+        # algorithmic bytes of one launch, in distinct 64-byte lines per random
+        # access: the occurrence blocks of each LF query, one line per SA / ISA
+        # entry read, per text run and per k-mer table entry; plus the reads,
+        # offsets, records and counts
+        blocks, lf_steps, lookups, tsteps, kmers = hpgfm.search1_count_work(ix, batch,
+                                                                              cap=args.cap, mode=mode)
+        key = tuning_key(preset, nreads, args.read_len, kmer, text_mode, mode)
+        tuning = choose_tuning(args, hpgfm, ix, g, synth, key, mode, nreads)
+        for _ in range(args.warmup):
+            hpgfm.search1_launch(ix, batch, out, mode=mode)
+        hpgfm.sync()
+        with hpgfm.TimingRegion() as tr:
+            hpgfm.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                hpgfm.search1_launch(ix, batch, out, mode=mode)
+            hpgfm.sync()
+            wall = time.perf_counter() - t0
+        ms_per_step = wall * 1000.0 / args.steps
+        reads_per_s = nreads / (wall / args.steps)
+        counts = out.counts().reshape(-1, 2)
+        stored = int(np.minimum(counts, args.cap).sum())
+        alg_bytes = (blocks * 64 + lookups * 64 + tsteps * 64 + kmers * 64 + int(offs[-1]) +
+                     8 * (nreads + 1) + stored * 32 + 4 * 2 * nreads)
+        kernel_ms = tr.mean_ms
+        achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
+        log(f"[{preset}] mismatch mode: {ms_per_step:.3f} ms/step wall, kernel {kernel_ms:.3f} ms "
+            f"(min {tr.min_ms:.3f}, max {tr.max_ms:.3f}), {reads_per_s / 1e6:.2f} M reads/s, "
+            f"{achieved:.0f} GB/s algorithmic")
+        traffic, traffic_src = measured_traffic(preset, nreads, args.read_len, text_mode, kmer,
+                                                tuning)
+        foot = random_footprint(ix)
+        res.update({
+            "value": round(reads_per_s), "unit": "reads/s", "ms_per_step": round(ms_per_step, 4),
+            "config": {
+                "workload": (f"1-mismatch search (BWSearch1CPU mismatch projection), "
+                             f"{mbp:g} Mbp synthetic reference (41% GC, "
+                             f"{pre['repeat_frac'] * 100:g}% repeat families 300 bp-6 kbp at "
+                             f"{pre['div'][0] * 100:g}-{pre['div'][1] * 100:g}% divergence"
+                             + (f", {pre['n_frac'] * 100:g}% N runs" if pre['n_frac'] else "")
+                             + f"), {nreads} reads x {args.read_len} bp, both strands, "
+                             f"{args.sub_rate * 100:g}% substitutions"),
+                "preset": preset, "genome_bp": n_bp, "reads": nreads,
+                "read_len": args.read_len, "mode": "mismatch", "cap": args.cap,
+                "sa_ratio": args.sa_ratio, "kmer_table_k": kmer, "text_mode": text_mode,
+                "parallelism": "single GPU, persistent work-queue launch"},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                         "traffic": traffic},
+            "traffic_source": traffic_src,
+            "kernel": "fmi::search1_kernel<true, true, false, %s>" % ("true" if text_mode else "false"),
+            "kernel_ms_mean": round(kernel_ms, 4),
+            "alg_bytes_per_launch": alg_bytes,
+            "lf_queries_per_launch": lf_steps,
+            "blocks_per_lf_query": round(blocks / max(lf_steps, 1), 4),
+            "lf_queries_per_read": round(lf_steps / nreads, 2),
+            "sa_isa_reads_per_launch": lookups, "text_runs_per_launch": tsteps,
+            "kmer_lookups_per_launch": kmers, "block_mode": block_mode,
+            "random_access_footprint_bytes": foot,
+            "index_cache_resident": foot["total"] < 256 * 2 ** 20,
+            "tuning": tuning,
+        })
+        if traffic:
+            res["traffic_over_alg"] = round(traffic / alg_bytes, 3)
+        if probe and not args.no_extras:
+            t0 = time.time()
+            try:
+                pb = probe_sweep(hpgfm, max(64 << 20, foot["total"]))
+                best = pb["best"]
+                res["random_access_probe"] = {"buffer_bytes": pb["buffer_bytes"], "best": best,
+                                              "points": len(pb["points"])}
+                res["random_access_probe_points"] = pb["points"]
+                if achieved < best["GBps"]:
+                    res["frac_of_measured_random_access"] = round(achieved / best["GBps"], 3)
+                else:
+                    res["frac_of_measured_random_access"] = None
+                    res["probe_note"] = ("the search's algorithmic rate exceeds the probe's best: "
+                                         "the probe is no ceiling here (the search's hot lines hit "
+                                         "the caches)")
+                log(f"[{preset}] random 64-B gather over {pb['buffer_bytes'] / 2**30:.2f} GiB, "
+                    f"best of {len(pb['points'])} points: {best['GBps']} GB/s "
+                    f"({best}) in {time.time() - t0:.1f}s")
+            except Exception as e:   # noqa: BLE001 - reported, the point still counts
+                res["random_access_probe"] = {"error": str(e)}
+        if extras and not args.no_extras:
+            hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
+            hpgfm.sync()
+            with hpgfm.TimingRegion() as tp:
+                for _ in range(3):
+                    hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
+                hpgfm.sync()
+            res["plain_one_launch_reads_per_s"] = round(nreads / (tp.mean_ms / 1000.0))
+            res["persistent_vs_plain"] = round(tp.mean_ms / kernel_ms, 3)
+            log(f"[{preset}] plain one-thread-per-task launch: {tp.mean_ms:.3f} ms")
+            # the reference's GPU design re-created: reference occurrence
+            # layout, 1000-byte read stride, one thread per read and strand,
+            # synchronous copies (fmi_plain_*); same records
+            plain = hpgfm.PlainSearch(ix)
+            pc, _, _, _ = plain.search1(reads, offs, cap=args.cap, mode=mode)
+            pk, pt = [], []
+            for _ in range(2):
+                _, _, mk, mt = plain.search1(reads, offs, cap=args.cap, mode=mode)
+                pk.append(mk)
+                pt.append(mt)
+            mk, mt = min(pk), min(pt)
+            res["plain_reference_design"] = {
+                "kernel_ms": round(mk, 3), "end_to_end_ms": round(mt, 3),
+                "kernel_reads_per_s": round(nreads / (mk / 1000.0)),
+                "end_to_end_reads_per_s": round(nreads / (mt / 1000.0)),
+                "layout_device_bytes": plain.device_bytes,
+                "kernel_speedup_of_engine": round(mk / kernel_ms, 2),
+                "same_counts": bool(np.array_equal(pc, counts)),
+            }
+            plain.close()
+            log(f"[{preset}] plain reference design: kernels {mk:.2f} ms, end to end {mt:.2f} ms "
+                f"(engine kernel {mk / kernel_ms:.1f}x faster)")
+        edit_counts = None
+        if extras and not args.no_extras or cpu_baseline:
+            eout = hpgfm.Search1Output(batch.nreads, args.cap)
+            hpgfm.search1_launch(ix, batch, eout, mode=hpgfm.MODE_EDIT)
+            hpgfm.sync()
+            with hpgfm.TimingRegion() as te:
+                for _ in range(3):
+                    hpgfm.search1_launch(ix, batch, eout, mode=hpgfm.MODE_EDIT)
+                hpgfm.sync()
+            res["edit_mode_reads_per_s"] = round(nreads / (te.mean_ms / 1000.0))
+            edit_counts = eout.counts().reshape(-1, 2)
+            eout.free()
+            log(f"[{preset}] edit mode (mismatch + indel records): {te.mean_ms:.3f} ms")
+        if cpu_baseline and not args.no_cpu_baseline:
+            res["cpu_baseline"], same = cpu_search1(args, ix, reads, offs, nreads, edit_counts,
+                                                    preset)
+            res["parity_counts_vs_cpu_sample"] = same
+        else:
+            res["cpu_baseline"] = None
+        if serve_s > 0:
+            try:
+                res["served"] = serve_point(args, hpgfm, ix, g, serve_s, mode, preset)
+            except Exception as e:   # noqa: BLE001 - reported, the point still counts
+                res["served"] = {"error": str(e)}
+    finally:
+        out.free()
+        batch.free()
+        ix.close()
+        hpgfm.set_tuning(0, 0, 0)
+        hpgfm.set_load_mode(hpgfm.LOADS_AUTO)
+    return res
+
+
+def cpu_search1(args, ix, reads, offs, nreads, edit_counts, preset):
+    """The reference's BWSearch1CPU on the host cores over a bounded sample of
+    the workload (about 15 s of CPU work), with the GPU's edit-mode record
+    counts compared on that sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fmoracle as fo
+    threads = cpu_threads()
+    t0 = time.time()
+    o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
+    kind = "reference" if fo.ref_available() else "port"
+    core = fo.RefCore(oracle=o) if kind == "reference" else o
+    log(f"[{preset}] CPU index ({kind}) ready in {time.time() - t0:.1f}s")
+    # The compiled reference indexes its C/O arrays with a read's code, so a
+    # read holding N (code 4) is undefined behaviour there: the sample is the
+    # workload's reads without N, in order.
+    sel = n_free(reads, offs)
+    nr = min(args.cpu_reads or nreads, len(sel))
+    probe_n = min(nr, 20000)
+    rs, os_ = subset_reads(reads, offs, sel[:probe_n])
+    if kind == "reference":
+        tprobe, _ = core.search1_batch(rs, os_, cap=args.cap, threads=threads)
+    else:
+        tprobe, _, _ = core.search1_batch(rs, os_, cap=args.cap, threads=threads, keep_hits=False)
+    nr = int(min(nr, max(probe_n, probe_n * 15.0 / max(tprobe, 1e-6))))
+    rs, os_ = subset_reads(reads, offs, sel[:nr])
+    if kind == "reference":
+        tcpu, ccpu = core.search1_batch(rs, os_, cap=args.cap, threads=threads)
+    else:
+        tcpu, ccpu, _ = core.search1_batch(rs, os_, cap=args.cap, threads=threads,
+                                           keep_hits=False)
+    cpu = {"value": round(nr / tcpu), "unit": "reads/s", "cores": threads, "kind": kind,
+           "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+           "sample": f"the first {nr} reads of the workload that hold no N ({len(sel)} of "
+                     f"{nreads} do), both strands; BWSearch1CPU (search/search.c:1876) per read "
+                     f"and strand as bwt.c:1690-1701, OpenMP over reads as bwt.c:1134 on every "
+                     f"CPU of the process's affinity, -O2 -fopenmp -DFM_COMP_64; the reference "
+                     f"has no mismatch-only mode, so this is its full 1-error search (the GPU "
+                     f"edit-mode figure is the like-for-like one)"}
+    log(f"[{preset}] CPU baseline: {nr} reads in {tcpu:.2f}s on {threads} threads "
+        f"(nproc {os.cpu_count()}) = {nr / tcpu:.0f} reads/s")
+    same = None
+    if edit_counts is not None:
+        same = bool(np.array_equal(np.minimum(edit_counts[sel[:nr]].reshape(-1), args.cap),
+                                   np.minimum(ccpu, args.cap)))
+        log(f"[{preset}] GPU edit-mode record counts == reference counts on the sample: {same}")
+    o.close()
+    return cpu, same
+
+
+def serve_point(args, hpgfm, ix, g, seconds, mode, preset):
+    """The serving loop over the resident index: Poisson arrivals at 20, 200
+    and 2000 requests/s (U[1, 50000] reads of U[50, 250] bp each), then a
+    closed loop of 16 requests in flight."""
+    from hpgfm import serve
+    t0 = time.time()
+    pool = serve.read_pool(g, 1_000_000, seed=5, sub_rate=args.sub_rate)
+    log(f"[{preset}] serving read pool (1M reads of 50-250 bp) in {time.time() - t0:.1f}s")
+    served = {"index": f"the {preset} index, resident", "requests": "Poisson arrivals, "
+              "U[1, 50000] reads of U[50, 250] bp, 1% substitutions, <=1 mismatch, both strands",
+              "runs": []}
+    srv = hpgfm.Server(ix, budget_bytes=2 << 30, max_inflight=4, max_batch_reads=1 << 20,
+                       cap=args.cap, mode=mode)
+    try:
+        for rate in (20, 200, 2000):
+            r = serve.run(srv, pool, rate=rate, duration_s=seconds, seed=rate)
+            r.pop("server", None)
+            served["runs"].append(r)
+            log(f"[{preset}] served at {rate} req/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
+                f"{r['reads_per_s'] / 1e6:.2f} M reads/s over {r['requests']} requests")
+        sat = serve.run_closed(srv, pool, nrequests=400, window=16)
+        sat.pop("server", None)
+        served["saturated"] = sat
+        log(f"[{preset}] served closed loop (16 in flight): {sat['reads_per_s'] / 1e6:.2f} M "
+            f"reads/s, p50 {sat['p50_ms']} ms, p99 {sat['p99_ms']} ms")
+        served["server_stats"] = srv.stats()
+    finally:
+        srv.close()
+    return served
 
 
 def main():
@@ -497,9 +927,17 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the gather roofline probe and the plain / edit comparisons")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the library's default launch shape")
+                    help="without a persisted tuning, keep the library's default launch shape")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the autotuner (on a separate tuning batch) even when a tuning "
+                         "is persisted, and persist its result")
+    ap.add_argument("--tuning-out", default=None,
+                    help="where the autotuner's result is persisted (default: "
+                         "hpg-libs_amd/hpgfm/tuning.json)")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
-                    help="length of each serving run (0 skips serving)")
+                    help="length of each serving run on the target-point index (0 skips serving)")
+    ap.add_argument("--no-target", action="store_true",
+                    help="skip the 3.1 Gbp target point (and the serving runs on its index)")
     ap.add_argument("--no-expand", action="store_true",
                     help="search the compact index only (no full SA / ISA / text on the device)")
     ap.add_argument("--tuning", default=None,
@@ -510,14 +948,27 @@ def main():
                          "WRITE_SIZE counters: a 1 GiB streaming copy (fmi_copy_bench) and "
                          "2^24-line random gathers of 64 and 128 bytes over a buffer the size "
                          "of the index and over 8 GiB")
-    ap.add_argument("--workload", choices=["search1", "seeds", "mixed", "exact", "inexact-seeds"],
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
+                    help="write the traffic summary of two rocprofv3 --pmc runs (FETCH_SIZE, "
+                         "WRITE_SIZE) of a --calibrate bench command and exit (no GPU)")
+    ap.add_argument("--workload", choices=["search1", "seeds", "mixed", "exact", "inexact-seeds",
+                                           "probe"],
                     default="search1",
                     help="search1: the 1-mismatch search (the metric); seeds: exact seed mapping "
                          "(18-mers at stride 5, <=1000 positions per seed); mixed: error "
                          "budgets 0/1/2 in one batch (use with --preset c3); exact: exact "
                          "search + locate with the reference's CPU path (use with --preset c1); "
-                         "inexact-seeds: bwt_map_inexact_seeds_seq over 18-base windows")
+                         "inexact-seeds: bwt_map_inexact_seeds_seq over 18-base windows; probe: "
+                         "the random 64-byte gather sweep over buffer sizes")
+    ap.add_argument("--probe-sizes-gib", default="0.0625,0.25,1,8,32,72",
+                    help="buffer sizes of --workload probe")
     args = ap.parse_args()
+
+    if args.summarize_pmc:
+        fd, wd, outp = args.summarize_pmc
+        tun = [int(x) for x in args.tuning.split(",")] if args.tuning else None
+        summarize_pmc(fd, wd, outp, tun, f"preset {args.preset}")
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -532,29 +983,40 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == "probe":
+        rows = []
+        for gib in (float(x) for x in args.probe_sizes_gib.split(",")):
+            t0 = time.time()
+            pb = probe_sweep(hpgfm, int(gib * (1 << 30)))
+            rows.append({"buffer_gib": gib, "best": pb["best"], "points": pb["points"]})
+            log(f"probe {gib} GiB: best {pb['best']} ({time.time() - t0:.1f}s)")
+        cms = hpgfm.copy_bench(2 << 30, reps=3)
+        print(json.dumps({"metric": "random 64-byte gather, best over the probe grid",
+                          "grid": PROBE_GRID, "sizes": rows,
+                          "streaming_copy_GBps": round(2 * (2 << 30) / (cms / 1e3) / 1e9, 1)}),
+              flush=True)
+        return
+
     pre = PRESETS[args.preset]
-    if args.genome_mbp is None:
-        args.genome_mbp = pre["genome_mbp"]
-    if args.reads is None:
-        args.reads = pre["reads"]
-    n_bp = int(args.genome_mbp * 1_000_000)
-    if args.kmer < 0:
-        # two bases past the smallest k with 4^k >= n (a k-mer of the text
-        # then names one row, and a k-mer absent from it ends a search at
-        # the lookup), at most 15 (8 GiB per direction)
-        k = 4
-        while 4 ** k < n_bp:
-            k += 1
-        args.kmer = min(15, k + 2)
-    t0 = time.time()
-    g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=pre["repeat_frac"],
-                           fam_len=(300, 6000), divergence=pre["div"], n_frac=pre["n_frac"])
-    log(f"genome {n_bp} bp in {time.time() - t0:.1f}s")
-    t0 = time.time()
-    ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
-    log(f"index built on the GPU in {time.time() - t0:.1f}s "
-        f"({ix.device_bytes / 1e6:.1f} MB on device)")
     if args.workload != "search1":
+        if args.genome_mbp is None:
+            args.genome_mbp = pre["genome_mbp"]
+        if args.reads is None:
+            args.reads = pre["reads"]
+        n_bp = int(args.genome_mbp * 1_000_000)
+        if args.kmer < 0:
+            k = 4
+            while 4 ** k < n_bp:
+                k += 1
+            args.kmer = min(15, k + 2)
+        t0 = time.time()
+        g, _, _ = synth.genome(n_bp, seed=1, gc=0.41, repeat_frac=pre["repeat_frac"],
+                               fam_len=(300, 6000), divergence=pre["div"], n_frac=pre["n_frac"])
+        log(f"genome {n_bp} bp in {time.time() - t0:.1f}s")
+        t0 = time.time()
+        ix = hpgfm.Index.build_gpu(synth.replace_n(g, seed=1), args.sa_ratio)
+        log(f"index built on the GPU in {time.time() - t0:.1f}s "
+            f"({ix.device_bytes / 1e6:.1f} MB on device)")
         if args.workload in ("exact", "mixed") or not args.no_expand:
             t0 = time.time()
             ix.expand()
@@ -568,300 +1030,39 @@ def main():
         run(args, pre, g, ix, hpgfm, synth)
         return
 
-    t0 = time.time()
-    reads, offs, _ = synth.reads(g, args.reads, length=args.read_len, seed=2,
-                                 sub_rate=args.sub_rate, both_strands=True)
-    batch = hpgfm.ReadBatch(reads, offs)
-    log(f"{args.reads} reads generated and resident in {time.time() - t0:.1f}s")
-
-    mode = hpgfm.MODE_MISMATCH
-    out = hpgfm.Search1Output(batch.nreads, args.cap)
-
-    block_mode = None
-    if not args.no_expand:
-        if not args.no_extras:
-            # the same search on the compact index alone (occurrence blocks
-            # for every step), default launch shape, for comparison
-            hpgfm.search1_launch(ix, batch, out, mode=mode)
-            hpgfm.sync()
-            with hpgfm.TimingRegion() as tb:
-                for _ in range(3):
-                    hpgfm.search1_launch(ix, batch, out, mode=mode)
-                hpgfm.sync()
-            block_mode = {"kernel_ms": round(tb.mean_ms, 4),
-                          "reads_per_s": round(args.reads / (tb.mean_ms / 1000.0))}
-            log(f"block mode (compact index): {tb.mean_ms:.3f} ms")
-        t0 = time.time()
-        ix.expand()
-        hpgfm.sync()
-        log(f"index expanded (full SA / ISA of both directions + 2-bit text) in "
-            f"{time.time() - t0:.2f}s, {ix.device_bytes / 1e9:.2f} GB on device")
-
-    if args.kmer:
-        t0 = time.time()
-        ix.kmer_table(args.kmer)
-        log(f"k-mer interval table (k = {args.kmer}) built in {time.time() - t0:.2f}s, "
-            f"{ix.device_bytes / 1e9:.2f} GB on device")
-
-    # algorithmic bytes of one launch, in distinct 64-byte lines per random
-    # access: the occurrence blocks of each LF query, one line per SA / ISA
-    # entry read, per text run of up to 16 bases (two adjacent 32-bit words)
-    # and per k-mer table entry; plus the reads, offsets, records and counts
-    blocks, lf_steps, lookups, tsteps, kmers = hpgfm.search1_count_work(ix, batch, cap=args.cap,
-                                                                          mode=mode)
-
-    tuning = None
-    if args.tuning:
-        parts = [int(x) for x in args.tuning.split(",")]
-        w, c, r = parts[:3]
-        loads = parts[3] if len(parts) > 3 else hpgfm.LOADS_AUTO
-        hpgfm.set_tuning(w, c, r)
-        hpgfm.set_load_mode(loads)
-        tuning = {"wg_per_cu": w, "chunk": c, "refill_min": r, "loads": loads,
-                  "source": "--tuning"}
-    elif not args.no_autotune:
-        from hpgfm.autotune import autotune
-        best, best_ms, table = autotune(ix, batch, out, mode=mode, log=log)
-        tuning = {"wg_per_cu": best[0], "chunk": best[1], "refill_min": best[2],
-                  "loads": best[3],
-                  "tuned_ms": round(best_ms, 4), "points": len(table) - 1,
-                  "plain_ms": round(table["plain"], 4)}
-
-    for _ in range(args.warmup):
-        hpgfm.search1_launch(ix, batch, out, mode=mode)
-    hpgfm.sync()
-
-    with hpgfm.TimingRegion() as tr:
-        hpgfm.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            hpgfm.search1_launch(ix, batch, out, mode=mode)
-        hpgfm.sync()
-        wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000.0 / args.steps
-    reads_per_s = args.reads / (wall / args.steps)
-    counts = out.counts().reshape(-1, 2)
-    stored = int(np.minimum(counts, args.cap).sum())
-    alg_bytes = (blocks * 64 + lookups * 64 + tsteps * 64 + kmers * 64 + int(offs[-1]) +
-                 8 * (args.reads + 1) + stored * 32 + 4 * 2 * args.reads)
-    kernel_ms = tr.mean_ms
-    achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
-    log(f"mismatch mode: {ms_per_step:.3f} ms/step wall, kernel {kernel_ms:.3f} ms "
-        f"(min {tr.min_ms:.3f}, max {tr.max_ms:.3f}), {reads_per_s / 1e6:.2f} M reads/s, "
-        f"{achieved:.0f} GB/s algorithmic")
-
-    extras = {}
-    if args.calibrate:
-        cms = hpgfm.copy_bench(1 << 30, reps=1)
-        log(f"calibration: copy 1 GiB read + 1 GiB written in {cms:.3f} ms")
-        for nbytes in (max(64 << 20, ix.device_bytes), 8 << 30):
-            for lb in (64, 128):
-                gms, done = hpgfm.gather_bench(nbytes, lb, 1 << 24)
-                log(f"calibration: gather {done} lines of {lb} B over {nbytes} B "
-                    f"({done * lb} B read) in {gms:.3f} ms")
-    if not args.no_extras:
-        cms = hpgfm.copy_bench(2 << 30, reps=3)
-        extras["streaming_copy_GBps"] = round(2 * (2 << 30) / (cms / 1000.0) / 1e9, 1)
-        log(f"streaming copy 2 GiB -> 2 GiB: {extras['streaming_copy_GBps']} GB/s (read + write)")
-        ms, done = hpgfm.gather_bench(8 << 30, 64, 1 << 28)
-        extras["random_gather_64B_GBps"] = round(done * 64 / (ms / 1000.0) / 1e9, 1)
-        log(f"random 64-byte gather over 8 GiB: {extras['random_gather_64B_GBps']} GB/s")
-        # the same probe over a buffer the size of this index's blocks + samples
-        ibytes = max(64 << 20, ix.device_bytes)
-        ms2, done2 = hpgfm.gather_bench(ibytes, 64, 1 << 28)
-        extras["random_gather_64B_index_size_GBps"] = round(done2 * 64 / (ms2 / 1000.0) / 1e9, 1)
-        extras["frac_of_measured_random_access"] = round(
-            achieved / extras["random_gather_64B_index_size_GBps"], 3)
-        log(f"random 64-byte gather over {ibytes / 2**30:.2f} GiB (index size): "
-            f"{extras['random_gather_64B_index_size_GBps']} GB/s")
-        hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
-        hpgfm.sync()
-        with hpgfm.TimingRegion() as tp:
-            for _ in range(3):
-                hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
-            hpgfm.sync()
-        extras["plain_one_launch_reads_per_s"] = round(args.reads / (tp.mean_ms / 1000.0))
-        extras["persistent_vs_plain"] = round(tp.mean_ms / kernel_ms, 3)
-        log(f"plain one-thread-per-task launch: {tp.mean_ms:.3f} ms")
-        # the reference's GPU design re-created: reference occurrence layout,
-        # 1000-byte read stride, one thread per read and strand, synchronous
-        # copies (fmi_plain_*); same records
-        plain = hpgfm.PlainSearch(ix)
-        pc, _, _, _ = plain.search1(reads, offs, cap=args.cap, mode=mode)
-        pk, pt = [], []
-        for _ in range(2):
-            _, _, mk, mt = plain.search1(reads, offs, cap=args.cap, mode=mode)
-            pk.append(mk)
-            pt.append(mt)
-        mk, mt = min(pk), min(pt)
-        extras["plain_reference_design"] = {
-            "kernel_ms": round(mk, 3), "end_to_end_ms": round(mt, 3),
-            "kernel_reads_per_s": round(args.reads / (mk / 1000.0)),
-            "end_to_end_reads_per_s": round(args.reads / (mt / 1000.0)),
-            "layout_device_bytes": plain.device_bytes,
-            "kernel_speedup_of_engine": round(mk / kernel_ms, 2),
-            "same_counts": bool(np.array_equal(pc, out.counts().reshape(-1, 2))),
-        }
-        plain.close()
-        log(f"plain reference design: kernels {mk:.2f} ms, end to end {mt:.2f} ms "
-            f"(engine kernel {mk / kernel_ms:.1f}x faster)")
-        eout = hpgfm.Search1Output(batch.nreads, args.cap)
-        hpgfm.search1_launch(ix, batch, eout, mode=hpgfm.MODE_EDIT)
-        hpgfm.sync()
-        with hpgfm.TimingRegion() as te:
-            for _ in range(3):
-                hpgfm.search1_launch(ix, batch, eout, mode=hpgfm.MODE_EDIT)
-            hpgfm.sync()
-        extras["edit_mode_reads_per_s"] = round(args.reads / (te.mean_ms / 1000.0))
-        edit_counts = eout.counts().reshape(-1, 2)
-        eout.free()
-        log(f"edit mode (mismatch + indel records): {te.mean_ms:.3f} ms")
-    else:
-        edit_counts = None
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import fmoracle as fo
-        threads = cpu_threads()
-        t0 = time.time()
-        o = fo.Oracle(bwts=(*ix.bwt(0), *ix.bwt(1)), ratio=0)
-        kind = "reference" if fo.ref_available() else "port"
-        core = fo.RefCore(oracle=o) if kind == "reference" else o
-        log(f"CPU index ({kind}) ready in {time.time() - t0:.1f}s")
-        # The compiled reference indexes its C/O arrays with a read's code,
-        # so a read holding N (code 4) is undefined behaviour there: the
-        # sample is the workload's reads without N, in order.
-        csum = np.concatenate([[0], np.cumsum(reads > 3, dtype=np.int64)])
-        o64 = offs.astype(np.int64)
-        sel = np.nonzero(csum[o64[1:]] - csum[o64[:-1]] == 0)[0]
-        lens_all = np.diff(o64)
-
-        def subset(idx):
-            ln = lens_all[idx]
-            so = np.concatenate([[0], np.cumsum(ln)]).astype(np.uint64)
-            starts = offs[:-1][idx].astype(np.int64)
-            gather = np.repeat(starts - so[:-1].astype(np.int64), ln) + np.arange(int(so[-1]))
-            return reads[gather], so
-
-        nr = min(args.cpu_reads or args.reads, len(sel))
-        probe = min(nr, 20000)
-        rs, os_ = subset(sel[:probe])
-        if kind == "reference":
-            tprobe, _ = core.search1_batch(rs, os_, cap=args.cap, threads=threads)
-        else:
-            tprobe, _, _ = core.search1_batch(rs, os_, cap=args.cap, threads=threads,
-                                              keep_hits=False)
-        # size the sample to about 15 s of CPU work
-        nr = int(min(nr, max(probe, probe * 15.0 / max(tprobe, 1e-6))))
-        rs, os_ = subset(sel[:nr])
-        if kind == "reference":
-            tcpu, ccpu = core.search1_batch(rs, os_, cap=args.cap, threads=threads)
-        else:
-            tcpu, ccpu, _ = core.search1_batch(rs, os_, cap=args.cap, threads=threads,
-                                               keep_hits=False)
-        n_free = len(sel)
-        cpu = {"value": round(nr / tcpu), "unit": "reads/s", "cores": threads, "kind": kind,
-               "sample": f"the first {nr} reads of the workload that hold no N ({n_free} of "
-                         f"{args.reads} do), both strands; BWSearch1CPU (search/search.c:1876) "
-                         f"per read and strand as bwt.c:1690-1701, OpenMP over reads as "
-                         f"bwt.c:1134, -O2 -fopenmp -DFM_COMP_64; the reference has no "
-                         f"mismatch-only mode, so this is its full 1-error search (the GPU "
-                         f"edit-mode figure is the like-for-like one)"}
-        log(f"CPU baseline: {nr} reads in {tcpu:.2f}s on {threads} threads = {nr / tcpu:.0f} reads/s")
-        if edit_counts is not None:
-            same = np.array_equal(np.minimum(edit_counts[sel[:nr]].reshape(-1), args.cap),
-                                  np.minimum(ccpu, args.cap))
-            extras["parity_counts_vs_cpu_sample"] = bool(same)
-            log(f"GPU edit-mode record counts == reference counts on the sample: {same}")
-
-    served = None
-    if args.serve_seconds > 0:
-        from hpgfm import serve
-        t0 = time.time()
-        pool = serve.read_pool(g, 1_000_000, seed=5, sub_rate=args.sub_rate)
-        log(f"serving read pool (1M reads of 50-250 bp) in {time.time() - t0:.1f}s")
-        served = {"index": "the same resident index", "requests": "Poisson arrivals, U[1, 50000] "
-                  "reads of U[50, 250] bp, 1% substitutions, <=1 mismatch, both strands",
-                  "runs": []}
-        srv = hpgfm.Server(ix, budget_bytes=2 << 30, max_inflight=4, max_batch_reads=1 << 20,
-                           cap=args.cap, mode=mode)
-        for rate in (20, 200, 2000):
-            r = serve.run(srv, pool, rate=rate, duration_s=args.serve_seconds, seed=rate)
-            r.pop("server", None)
-            served["runs"].append(r)
-            log(f"served at {rate} req/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
-                f"{r['reads_per_s'] / 1e6:.2f} M reads/s over {r['requests']} requests")
-        sat = serve.run_closed(srv, pool, nrequests=400, window=16)
-        sat.pop("server", None)
-        served["saturated"] = sat
-        log(f"served closed loop (16 requests in flight): {sat['reads_per_s'] / 1e6:.2f} M reads/s, "
-            f"p50 {sat['p50_ms']} ms, p99 {sat['p99_ms']} ms")
-        served["server_stats"] = srv.stats()
-        srv.close()
-
+    # The headline: the preset the metric is quoted on (c2), with the
+    # comparisons; then the 3.1 Gbp target point with its own roofline, CPU
+    # baseline and the serving runs on its resident index.
+    head = search1_point(args, hpgfm, synth, args.preset, extras=True, cpu_baseline=True,
+                         serve_s=0.0 if not args.no_target else args.serve_seconds)
     res = {
-        "metric": METRIC,
-        "value": round(reads_per_s),
-        "unit": "reads/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic",
-        "config": {
-            "workload": (f"1-mismatch search (BWSearch1CPU mismatch projection), "
-                         f"{args.genome_mbp:g} Mbp synthetic reference (41% GC, "
-                         f"{pre['repeat_frac'] * 100:g}% repeat families 300 bp-6 kbp at "
-                         f"{pre['div'][0] * 100:g}-{pre['div'][1] * 100:g}% divergence"
-                         + (f", {pre['n_frac'] * 100:g}% N runs" if pre['n_frac'] else "")
-                         + f"), {args.reads} reads x {args.read_len} bp, both strands, "
-                         f"{args.sub_rate * 100:g}% substitutions"),
-            "preset": args.preset,
-            "genome_bp": n_bp, "reads": args.reads, "read_len": args.read_len,
-            "mode": "mismatch", "cap": args.cap, "sa_ratio": args.sa_ratio,
-            "kmer_table_k": args.kmer, "text_mode": not args.no_expand,
-            "parallelism": "single GPU, persistent work-queue launch",
-        },
-        "roofline": {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": measured_traffic(args.preset, args.reads, args.read_len,
-                                        not args.no_expand, args.kmer),
-        },
-        "cpu_baseline": cpu,
-        "kernel": "fmi::search1_kernel<true, true, false, %s>" % ("false" if args.no_expand else "true"),
-        "kernel_ms_mean": round(kernel_ms, 4),
-        "alg_bytes_per_launch": alg_bytes,
-        "lf_queries_per_launch": lf_steps,
-        "blocks_per_lf_query": round(blocks / max(lf_steps, 1), 4),
-        "lf_queries_per_read": round(lf_steps / args.reads, 2),
-        "text_mode": not args.no_expand,
-        "sa_isa_reads_per_launch": lookups,
-        "text_runs_per_launch": tsteps,
-        "kmer_lookups_per_launch": kmers,
-        "block_mode": block_mode,
-        "index_device_bytes": ix.device_bytes,
-        "index_cache_resident": ix.device_bytes < 256 * 2 ** 20,
-        "tuning": tuning,
+        "metric": METRIC, "value": head.pop("value"), "unit": head.pop("unit"), "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head.pop("ms_per_step"),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic", "config": head.pop("config"), "roofline": head.pop("roofline"),
+        "cpu_baseline": head.pop("cpu_baseline"),
     }
-    if served:
+    res.update(head)
+    if not args.no_target and args.preset != "t3g":
+        try:
+            tgt = search1_point(args, hpgfm, synth, "t3g", extras=True, cpu_baseline=True,
+                                serve_s=args.serve_seconds)
+            res["target_3g"] = tgt
+        except Exception as e:   # noqa: BLE001 - the headline still stands
+            log(f"target point failed: {e}")
+            res["target_3g"] = {"error": str(e)}
+    served = (res.get("target_3g") or {}).get("served") or res.get("served")
+    if served and "runs" in served:
         at200 = next(r for r in served["runs"] if r["rate_req_s"] == 200)
         res["served_p50_ms"] = at200["p50_ms"]
         res["served_p99_ms"] = at200["p99_ms"]
-        res["served"] = served
-        prd = extras.get("plain_reference_design")
+        res["served_index"] = served["index"]
+        src = res.get("target_3g") if "served" in (res.get("target_3g") or {}) else res
+        prd = src.get("plain_reference_design")
         if prd:
             best = max([r["reads_per_s"] for r in served["runs"]] +
                        [served["saturated"]["reads_per_s"]])
             prd["served_speedup_end_to_end"] = round(best / prd["end_to_end_reads_per_s"], 2)
-    res.update(extras)
-    out.free()
-    batch.free()
     print(json.dumps(res), flush=True)
 
 

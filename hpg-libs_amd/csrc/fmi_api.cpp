@@ -76,6 +76,34 @@ struct StreamState {
 std::unordered_map<std::string, StreamState> g_streams;
 std::unordered_map<void *, std::string> g_graph_stream;   // graph -> stream key
 
+// The device memory of fmi_search1_batch_noalloc: the persistent launch's
+// work-queue counter (bytes 0..3) and its work counters (64..103), the same
+// layout as a stream's workspace.  One copy per GPU, held until the process
+// ends; in host code `scratch` is only a handle for hipGetSymbolAddress, the
+// kernels get the device address kept in g_scratch_addr.
+constexpr size_t kNoAllocScratchBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) on the first no-alloc call there.
+int noalloc_scratch(void **out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(FMI_EDEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return FMI_OK;
+}
+
 int stream_key(hipStream_t s, std::string *key) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
@@ -296,6 +324,8 @@ const char *fmi_status_str(int status) {
     case FMI_EDEVICE: return "device error";
     case FMI_ERANGE: return "size out of range";
     case FMI_ESTATE: return "call out of order";
+    case FMI_EAGAIN: return "not now (time-out or full queue)";
+    case FMI_ECANCELED: return "cancelled";
     default: return "unknown status";
   }
 }
@@ -392,6 +422,14 @@ uint64_t fmi_index_device_bytes(const fmi_index *ix) {
   for (int d = 0; d < 2; ++d)
     b += ix->h[d].nblocks * sizeof(fmi::OccBlock) + 2 * ((ix->h[d].nsamp * 4 + 255) & ~255ull);
   return b + ix->full_bytes + ix->kmer_bytes;
+}
+
+int fmi_index_device_layout(const fmi_index *ix, uint64_t *out) {
+  if (!ix || !out) return fail(FMI_EINVAL, "NULL argument");
+  out[0] = fmi_index_device_bytes(ix) - ix->full_bytes - ix->kmer_bytes;
+  out[1] = ix->full_bytes;
+  out[2] = ix->kmer_bytes;
+  return FMI_OK;
 }
 
 int fmi_index_expand(fmi_index *ix) {
@@ -650,6 +688,21 @@ int fmi_memset_d(void *d_dst, int value, size_t bytes, fmi_stream s) {
   return FMI_OK;
 }
 
+int fmi_device_name(char *name, size_t size) {
+  if (!name || size == 0) return fail(FMI_EINVAL, "NULL argument");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  std::snprintf(name, size, "%s:%s", prop.name, prop.gcnArchName);
+  return FMI_OK;
+}
+
+int fmi_device_cu_count(int *cus) {
+  if (!cus) return fail(FMI_EINVAL, "NULL argument");
+  return cu_count(cus);
+}
+
 int fmi_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
   if (!free_bytes || !total_bytes) return fail(FMI_EINVAL, "NULL argument");
   size_t f = 0, t = 0;
@@ -810,6 +863,48 @@ int fmi_search1_batch(const fmi_index *ix, int mode, int strand_mask, const uint
                         s, true);
 }
 
+int fmi_search1_batch_noalloc(const fmi_index *ix, int mode, int strand_mask,
+                              const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                              uint32_t cap, fmi_hit *d_hits, uint32_t *d_counts, fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (mode != FMI_MODE_EDIT && mode != FMI_MODE_MISMATCH) return fail(FMI_EINVAL, "mode must be EDIT or MISMATCH");
+  if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "strand_mask must be 1, 2 or 3");
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_counts) return fail(FMI_EINVAL, "NULL buffer");
+  if (cap > 0 && !d_hits) return fail(FMI_EINVAL, "d_hits is NULL with cap > 0");
+  if (nreads > (1u << 30)) return fail(FMI_ERANGE, "more than 2^30 reads in one batch");
+  // the launch needs the queue counter and 5 work counters: 104 bytes
+  static_assert(kNoAllocScratchBytes >= 64 + 5 * sizeof(unsigned long long),
+                "scratch holds the queue and work counters");
+  void *base = nullptr;
+  rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  uint32_t *queue = static_cast<uint32_t *>(base);
+  unsigned long long *stats =
+      reinterpret_cast<unsigned long long *>(static_cast<char *>(base) + 64);
+  hipStream_t st = static_cast<hipStream_t>(s);
+  Tuning t;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    t = g_tuning;
+  }
+  int cus = 0;
+  rc = cu_count(&cus);
+  if (rc) return rc;
+  const uint32_t grid = (uint32_t)cus * (t.wg_per_cu ? t.wg_per_cu : kDefaultWgPerCu);
+  // the counter keeps the last call's value: clear it with a kernel (no
+  // memset), on the call's stream, before the search reads it
+  HIP_TRY(fmi::launch_zero(queue, 1, st), "zero queue");
+  const bool own = t.loads == FMI_LOADS_OWN ||
+                   (t.loads == FMI_LOADS_AUTO && fmi_index_device_bytes(ix) > kOwnLoadsBytes);
+  HIP_TRY(fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, true, false,
+                              (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap, d_hits,
+                              d_counts, queue, stats, grid, t.chunk, t.refill_min, own, st),
+          "search1 launch");
+  return FMI_OK;
+}
+
 int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
                              const uint8_t *d_reads, const uint64_t *d_offsets,
                              const int16_t *d_bounds, uint32_t nreads, uint32_t cap,
@@ -825,14 +920,14 @@ int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
   hipStream_t st = static_cast<hipStream_t>(s);
   // per-lane vector scratch: sized by the lane count, no read of the offsets
   const uint32_t lanes = fmi::vhelper_lanes(nreads);
-  void *scratch = nullptr;
-  rc = scratch_for(st, fmi::vhelper_scratch_bytes(lanes), &scratch);
+  void *scr = nullptr;
+  rc = scratch_for(st, fmi::vhelper_scratch_bytes(lanes), &scr);
   if (rc) return rc;
   rc = begin_timed(st);
   if (rc) return rc;
   hipError_t e = fmi::launch_vhelper(ix->dev, mode == FMI_MODE_MISMATCH, (uint32_t)strand_mask,
                                      d_reads, d_offsets, d_bounds, nreads, cap, d_hits, d_counts,
-                                     scratch, lanes, st);
+                                     scr, lanes, st);
   return end_timed(st, e, "search1 vector launch");
 }
 
@@ -860,14 +955,14 @@ int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_re
   const uint64_t per_lane = fmi::search_k_scratch_bytes(1, list_cap);
   uint64_t lanes = std::min<uint64_t>((2ull * nreads + 255) & ~255ull, 256ull * 1024);
   while (lanes > 256 && lanes * per_lane > (4ull << 30)) lanes -= 256;
-  void *scratch = nullptr;
-  rc = scratch_for(st, fmi::search_k_scratch_bytes((uint32_t)lanes, list_cap), &scratch);
+  void *scr = nullptr;
+  rc = scratch_for(st, fmi::search_k_scratch_bytes((uint32_t)lanes, list_cap), &scr);
   if (rc) return rc;
   rc = begin_timed(st);
   if (rc) return rc;
   hipError_t e = fmi::launch_search_k(ix->dev, d_reads, d_offsets, d_fragsize, nreads,
                                       (uint32_t)strand_mask, list_cap, anchor_cap, d_final,
-                                      d_nfinal, d_anchors, d_nanchors, d_full, scratch,
+                                      d_nfinal, d_anchors, d_nanchors, d_full, scr,
                                       (uint32_t)lanes, w.queue, st);
   return end_timed(st, e, "k-error search launch");
 }
@@ -1100,6 +1195,31 @@ int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes, uin
   rc = end_timed(st, e, "gather launch");
   if (rc) return rc;
   *lines_done = fmi::gather_loads_done(nlines, (uint32_t)cus * 16u);
+  return fmi_last_kernel_ms(ms);
+}
+
+int fmi_gather_probe(const void *d_buf, uint64_t bytes, uint32_t lines_in_flight,
+                     uint32_t wg_per_cu, int cooperative, uint64_t nlines, uint64_t *d_sink,
+                     float *ms, uint64_t *lines_done, fmi_stream s) {
+  if (!d_buf || !d_sink || !ms || !lines_done) return fail(FMI_EINVAL, "NULL argument");
+  const uint32_t f = lines_in_flight;
+  if (f != 1 && f != 2 && f != 4 && f != 8 && f != 16)
+    return fail(FMI_EINVAL, "lines_in_flight must be 1, 2, 4, 8 or 16");
+  if (wg_per_cu < 1 || wg_per_cu > 64) return fail(FMI_EINVAL, "wg_per_cu must be 1..64");
+  if (bytes < (64ull << 20)) return fail(FMI_EINVAL, "buffer must be >= 64 MiB");
+  if (nlines == 0 || nlines > (1ull << 34)) return fail(FMI_EINVAL, "nlines must be 1..2^34");
+  int cus = 0;
+  int rc = cu_count(&cus);
+  if (rc) return rc;
+  const uint32_t grid = (uint32_t)cus * wg_per_cu;
+  hipStream_t st = static_cast<hipStream_t>(s);
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_gather_probe(d_buf, bytes, f, cooperative != 0, nlines,
+                                          reinterpret_cast<unsigned long long *>(d_sink), grid, st);
+  rc = end_timed(st, e, "gather probe launch");
+  if (rc) return rc;
+  *lines_done = fmi::gather_probe_lines(nlines, grid, f, cooperative != 0);
   return fmi_last_kernel_ms(ms);
 }
 

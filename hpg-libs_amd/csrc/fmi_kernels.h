@@ -72,6 +72,13 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
 hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
                          unsigned long long *sink, uint32_t grid, hipStream_t s);
 uint64_t gather_loads_done(uint64_t nloads, uint32_t grid);
+// 64-byte random lines, ilp (1, 2, 4, 8, 16) in flight per lane (coop:
+// per quad of lanes, each lane loading a quarter of the line); at least
+// nlines lines, gather_probe_lines of them exactly.
+hipError_t launch_gather_probe(const void *buf, uint64_t bytes, uint32_t ilp, bool coop,
+                               uint64_t nlines, unsigned long long *sink, uint32_t grid,
+                               hipStream_t s);
+uint64_t gather_probe_lines(uint64_t nlines, uint32_t grid, uint32_t ilp, bool coop);
 // bytes / 16 words copied from src to dst (both 16-byte aligned).
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, uint32_t grid, hipStream_t s);
 

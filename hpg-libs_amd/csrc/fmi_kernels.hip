@@ -1315,6 +1315,47 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint4 *buf, uint64_t 
   if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);   // keeps the loads live
 }
 
+// 64-byte lines at random, ILP lines in flight per lane (COOP = false: each
+// lane loads its own lines as four 16-byte pieces) or per quad of lanes
+// (COOP = true: the four lanes of a quad load one 16-byte piece each of the
+// quad's line, so one instruction fetches 16 whole lines).  per_thread
+// iterations of ILP loads; lines loaded = threads * per_thread (COOP: / 4).
+template <int ILP, bool COOP>
+__global__ __launch_bounds__(256) void gather64_kernel(const uint4 *buf, uint64_t line_mask,
+                                                       uint64_t per_thread,
+                                                       unsigned long long *sink) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t piece = COOP ? (threadIdx.x & 3u) : 0u;
+  uint64_t st = mix64((COOP ? tid >> 2 : tid) * 0x9E3779B97F4A7C15ull + 1);
+  uint32_t acc = 0;
+  for (uint64_t it = 0; it < per_thread; it += ILP) {
+    if (COOP) {
+      uint4 x[ILP];
+#pragma unroll
+      for (int u = 0; u < ILP; ++u) {
+        st = mix64(st + u + 1);
+        x[u] = buf[(st & line_mask) * 4 + piece];
+      }
+#pragma unroll
+      for (int u = 0; u < ILP; ++u) acc ^= x[u].x ^ x[u].y ^ x[u].z ^ x[u].w;
+    } else {
+      uint4 x[ILP][4];
+#pragma unroll
+      for (int u = 0; u < ILP; ++u) {
+        st = mix64(st + u + 1);
+        const uint4 *p = buf + (st & line_mask) * 4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x[u][v] = p[v];
+      }
+#pragma unroll
+      for (int u = 0; u < ILP; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc ^= x[u][v].x ^ x[u][v].y ^ x[u][v].z ^ x[u][v].w;
+    }
+  }
+  if (acc == 0x9E3779B9u) atomicAdd(sink, 1ull);   // keeps the loads live
+}
+
 // Streaming copy of n16 16-byte words, grid-stride: a kernel of known HBM
 // traffic (n16*16 bytes read and written) that calibrates the profiler's
 // fetch/write counters and measures the streaming bandwidth.
@@ -1522,6 +1563,43 @@ hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, u
     case 256: hipLaunchKernelGGL(gather_kernel<16>, dim3(grid), dim3(256), 0, s, b, nlines - 1, per_thread, sink); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// Iterations per lane of gather64_kernel for at least nlines lines.
+static uint64_t probe_per_thread(uint64_t nlines, uint32_t grid, uint32_t ilp, bool coop) {
+  const uint64_t units = (uint64_t)grid * 256 / (coop ? 4 : 1);
+  uint64_t per = (nlines + units - 1) / units;
+  return (per + ilp - 1) / ilp * ilp;
+}
+
+uint64_t gather_probe_lines(uint64_t nlines, uint32_t grid, uint32_t ilp, bool coop) {
+  return probe_per_thread(nlines, grid, ilp, coop) * ((uint64_t)grid * 256 / (coop ? 4 : 1));
+}
+
+hipError_t launch_gather_probe(const void *buf, uint64_t bytes, uint32_t ilp, bool coop,
+                               uint64_t nlines, unsigned long long *sink, uint32_t grid,
+                               hipStream_t s) {
+  uint64_t n = 1;
+  while (n * 2 <= bytes / 64) n *= 2;
+  const uint64_t per = probe_per_thread(nlines, grid, ilp, coop);
+  const uint4 *b = (const uint4 *)buf;
+#define FMI_PROBE(I, C) \
+  hipLaunchKernelGGL((gather64_kernel<I, C>), dim3(grid), dim3(256), 0, s, b, n - 1, per, sink)
+  switch (ilp * 2 + (coop ? 1 : 0)) {
+    case 2: FMI_PROBE(1, false); break;
+    case 3: FMI_PROBE(1, true); break;
+    case 4: FMI_PROBE(2, false); break;
+    case 5: FMI_PROBE(2, true); break;
+    case 8: FMI_PROBE(4, false); break;
+    case 9: FMI_PROBE(4, true); break;
+    case 16: FMI_PROBE(8, false); break;
+    case 17: FMI_PROBE(8, true); break;
+    case 32: FMI_PROBE(16, false); break;
+    case 33: FMI_PROBE(16, true); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef FMI_PROBE
   return hipGetLastError();
 }
 

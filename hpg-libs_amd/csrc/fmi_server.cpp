@@ -1,27 +1,36 @@
 // fmi_server.cpp -- continuous-batching request loop over a device-resident
 // index (fmi_server_* in include/fmi.h).
 //
-// One scheduler thread owns the device side: a device arena of
-// device_budget_bytes and a pinned host staging area of the same size share
-// one first-fit allocator, so a launch's region has the same offsets in
-// both.  A launch:
-//   1. packs the admitted requests' reads and rebased offsets into staging
-//      and copies them up;
-//   2. runs the persistent 1-error search, then compacts the stored records
-//      of every (read, strand) slot into a dense array (scan of the counts);
-//   3. copies the counts and the record total back and records an event;
-//   4. once that event has completed, copies exactly the dense records back
-//      and records a second event;
-//   5. once that one has completed, hands each request its slice.
+// Host memory is pinned and allocated once: an input staging area, where
+// fmi_server_submit copies a request's reads and offsets (their only host
+// copy) on the caller's thread, and an output mirror of the device arena.
+// One scheduler thread owns the device side; a launch:
+//   1. takes pending requests first-come first-served while their reads
+//      and results fit max_batch_reads and the device arena;
+//   2. copies each request's reads and request-relative offsets from its
+//      staging into the launch's device region (one copy each), then a
+//      rebase kernel turns the offsets into launch offsets;
+//   3. runs the persistent 1-error search and compacts the stored records of
+//      every (read, strand) slot into a dense array (a scan of the counts);
+//   4. copies the counts and the record total into the output mirror and
+//      records an event; once it has passed, the requests' input staging is
+//      released and exactly the dense records are copied back;
+//   5. once those have landed, a copy worker hands each request its counts
+//      and records and releases the launch's region.
 // Up to max_inflight launches run at once on separate (default-priority)
-// streams, so requests that arrive while others are on the GPU are admitted
-// as soon as memory and a stream free up.
+// streams.  Submission is bounded: a request that finds max_pending requests
+// queued, or no room in the input staging, is refused with FMI_EAGAIN (the
+// caller retries; the reference's producer blocks at max_num_work_items,
+// commons/workflow_scheduler.c:252-271).  A queued request can be cancelled;
+// one already on the GPU is dropped when its launch completes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -40,35 +49,45 @@ using Clock = std::chrono::steady_clock;
 
 constexpr uint64_t kAlign = 256;
 inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+constexpr uint32_t kCopyWorkers = 2;
+constexpr uint32_t kDefaultMaxPending = 4096;
 
 struct Request {
   uint64_t id = 0;
-  std::vector<uint8_t> reads;
-  std::vector<uint64_t> offsets;   // rebased to 0, nreads + 1 entries
   uint32_t nreads = 0;
-  std::vector<uint32_t> counts;    // results (2 * nreads)
-  std::vector<fmi_hit> records;    // results (dense)
+  uint64_t bases = 0;
+  uint64_t in_off = 0, in_bytes = 0;   // staging: reads, then nreads + 1 offsets
+  bool staged = false;                 // holds input staging
+  std::vector<uint32_t> counts;        // results (2 * nreads)
+  std::vector<fmi_hit> records;        // results (dense)
   Clock::time_point submit, done;
   bool finished = false;
+  bool cancelled = false;
   int status = FMI_OK;
 };
 
+// Device layout of one launch.
 struct Layout {
-  uint64_t o_reads, o_offs, o_hits, o_counts, o_total, o_compact, o_scratch, size;
+  uint64_t o_reads, o_offs, o_table, o_hits, o_counts, o_total, o_compact, o_scratch, size;
 };
 
-Layout layout_for(uint64_t bases, uint64_t nreads, uint32_t cap) {
+Layout layout_for(uint64_t bases, uint64_t nreads, uint64_t nreq, uint32_t cap) {
   const uint64_t slots = 2 * nreads;
   Layout L;
   L.o_reads = 0;
   L.o_offs = align_up(bases ? bases : 1);
-  L.o_hits = L.o_offs + align_up((nreads + 1) * 8);
+  L.o_table = L.o_offs + align_up((nreads + 1) * 8);
+  L.o_hits = L.o_table + align_up(2 * nreq * 8);
   L.o_counts = L.o_hits + align_up(slots * (uint64_t)cap * sizeof(fmi_hit));
   L.o_total = L.o_counts + align_up(slots * 4);
   L.o_compact = L.o_total + kAlign;
   L.o_scratch = L.o_compact + align_up(slots * (uint64_t)cap * sizeof(fmi_hit));
   L.size = L.o_scratch + align_up(fmi::compact_scratch_bytes((uint32_t)slots));
   return L;
+}
+
+uint64_t input_bytes(uint64_t bases, uint64_t nreads) {
+  return align_up(bases ? bases : 1) + align_up((nreads + 1) * 8);
 }
 
 // First-fit allocator over [0, size) in kAlign units.
@@ -123,6 +142,22 @@ struct Launch {
   uint64_t total = 0;
 };
 
+// Launch offsets from request-relative ones: table holds, per request, its
+// first read in the launch and the launch position of its first base.
+// Entry i belongs to the last request whose first read is <= i (a request's
+// closing entry was overwritten by the next request's opening 0).
+__global__ void rebase_kernel(uint64_t *offs, uint32_t n1, const uint64_t *table, uint32_t nreq) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n1) return;
+  uint32_t lo = 0, hi = nreq;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (table[2 * mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  offs[i] += table[2 * lo + 1];
+}
+
 }  // namespace
 
 struct fmi_server {
@@ -130,47 +165,68 @@ struct fmi_server {
   fmi_server_config cfg{};
   int device = 0;
   char *d_arena = nullptr;
-  uint8_t *h_stage = nullptr;
-  Arena arena;
+  uint8_t *h_mirror = nullptr;   // output mirror of the device arena (pinned)
+  uint8_t *h_in = nullptr;       // input staging (pinned)
+  uint64_t in_size = 0;
+  Arena arena, in_arena;
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
   std::vector<bool> slot_busy;
 
   std::mutex mu;
-  std::condition_variable cv_submit, cv_done;
+  std::condition_variable cv_submit, cv_done, cv_work;
   std::deque<std::shared_ptr<Request>> pending;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> live;
   std::vector<Launch> inflight;       // touched by the scheduler thread only
+  std::deque<std::function<void()>> work;   // result hand-out tasks
   uint64_t next_id = 1;
-  bool stop = false;
+  bool stop = false, workers_stop = false;
   fmi_server_stats stats{};
   std::string fault;
   std::thread thr;
+  std::vector<std::thread> workers;
 
   void run();
+  void worker();
   int start_launch(Launch &L);
   int fetch_records(Launch &L);
-  void finish_launch(Launch &L, int status);
+  void hand_out(Launch L, int status);
+  void release_inputs(Launch &L);
 };
 
 int fmi_server::start_launch(Launch &L) {
-  uint8_t *st = h_stage + L.off;
   const Layout &lay = L.lay;
-  uint64_t *offs = reinterpret_cast<uint64_t *>(st + lay.o_offs);
-  uint64_t base = 0;
-  uint32_t r = 0;
-  offs[0] = 0;
-  for (auto &rp : L.reqs) {
-    L.first_read.push_back(r);
-    std::memcpy(st + base, rp->reads.data(), rp->reads.size());
-    for (uint32_t i = 0; i < rp->nreads; ++i) offs[r + i + 1] = base + rp->offsets[i + 1];
-    base += rp->reads.size();
-    r += rp->nreads;
-  }
   hipStream_t s = streams[L.slot];
   char *d = d_arena + L.off;
+  // request table of the rebase kernel, staged in the output mirror
+  uint64_t *table = reinterpret_cast<uint64_t *>(h_mirror + L.off + lay.o_table);
+  uint64_t base = 0;
+  uint32_t r = 0;
+  for (size_t q = 0; q < L.reqs.size(); ++q) {
+    Request &rq = *L.reqs[q];
+    L.first_read.push_back(r);
+    table[2 * q] = r;
+    table[2 * q + 1] = base;
+    const uint8_t *src = h_in + rq.in_off;
+    if (rq.bases &&
+        hipMemcpyAsync(d + lay.o_reads + base, src, rq.bases, hipMemcpyHostToDevice, s) != hipSuccess)
+      return FMI_EDEVICE;
+    // request-relative offsets; the next request's opening entry (0)
+    // overwrites this request's closing one, the rebase kernel adds bases
+    if (hipMemcpyAsync(d + lay.o_offs + 8ull * r, src + align_up(rq.bases ? rq.bases : 1),
+                       8ull * (rq.nreads + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+      return FMI_EDEVICE;
+    base += rq.bases;
+    r += rq.nreads;
+  }
+  const uint32_t nreq = (uint32_t)L.reqs.size();
+  if (hipMemcpyAsync(d + lay.o_table, table, 16ull * nreq, hipMemcpyHostToDevice, s) != hipSuccess)
+    return FMI_EDEVICE;
+  hipLaunchKernelGGL(rebase_kernel, dim3((L.nreads + 1 + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<uint64_t *>(d + lay.o_offs), L.nreads + 1,
+                     reinterpret_cast<const uint64_t *>(d + lay.o_table), nreq);
+  if (hipGetLastError() != hipSuccess) return FMI_EDEVICE;
   const uint32_t slots = 2 * L.nreads;
-  if (hipMemcpyAsync(d, st, lay.o_hits, hipMemcpyHostToDevice, s) != hipSuccess) return FMI_EDEVICE;
   int rc = fmi_search1_batch(ix, cfg.mode, cfg.strand_mask, reinterpret_cast<const uint8_t *>(d),
                              reinterpret_cast<const uint64_t *>(d + lay.o_offs), L.nreads, cfg.cap,
                              reinterpret_cast<fmi_hit *>(d + lay.o_hits),
@@ -183,19 +239,29 @@ int fmi_server::start_launch(Launch &L) {
                           d + lay.o_scratch, s) != hipSuccess)
     return FMI_EDEVICE;
   // counts and the record total are adjacent in the layout: one copy
-  if (hipMemcpyAsync(st + lay.o_counts, d + lay.o_counts, lay.o_compact - lay.o_counts,
-                     hipMemcpyDeviceToHost, s) != hipSuccess)
+  if (hipMemcpyAsync(h_mirror + L.off + lay.o_counts, d + lay.o_counts,
+                     lay.o_compact - lay.o_counts, hipMemcpyDeviceToHost, s) != hipSuccess)
     return FMI_EDEVICE;
   if (hipEventRecord(events[L.slot], s) != hipSuccess) return FMI_EDEVICE;
   L.phase = 1;
   return FMI_OK;
 }
 
+void fmi_server::release_inputs(Launch &L) {
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto &rp : L.reqs) {
+    if (rp->staged) {
+      in_arena.release(rp->in_off, rp->in_bytes);
+      rp->staged = false;
+    }
+  }
+}
+
 int fmi_server::fetch_records(Launch &L) {
-  L.total = *reinterpret_cast<const unsigned long long *>(h_stage + L.off + L.lay.o_total);
+  L.total = *reinterpret_cast<const unsigned long long *>(h_mirror + L.off + L.lay.o_total);
   hipStream_t s = streams[L.slot];
   if (L.total &&
-      hipMemcpyAsync(h_stage + L.off + L.lay.o_compact, d_arena + L.off + L.lay.o_compact,
+      hipMemcpyAsync(h_mirror + L.off + L.lay.o_compact, d_arena + L.off + L.lay.o_compact,
                      L.total * sizeof(fmi_hit), hipMemcpyDeviceToHost, s) != hipSuccess)
     return FMI_EDEVICE;
   if (hipEventRecord(events[L.slot], s) != hipSuccess) return FMI_EDEVICE;
@@ -203,36 +269,58 @@ int fmi_server::fetch_records(Launch &L) {
   return FMI_OK;
 }
 
-void fmi_server::finish_launch(Launch &L, int status) {
-  // Hand out results outside the lock: only this thread writes them.
+// Copies each request's slice of the launch's output mirror into it (on a
+// copy worker), then releases the launch's region and stream slot.
+void fmi_server::hand_out(Launch L, int status) {
   if (status == FMI_OK) {
-    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(h_stage + L.off + L.lay.o_counts);
-    const fmi_hit *rec = reinterpret_cast<const fmi_hit *>(h_stage + L.off + L.lay.o_compact);
+    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(h_mirror + L.off + L.lay.o_counts);
+    const fmi_hit *rec = reinterpret_cast<const fmi_hit *>(h_mirror + L.off + L.lay.o_compact);
     uint64_t o = 0;
     for (size_t q = 0; q < L.reqs.size(); ++q) {
       Request &r = *L.reqs[q];
       const uint64_t s0 = 2ull * L.first_read[q], ns = 2ull * r.nreads;
-      r.counts.assign(cnt + s0, cnt + s0 + ns);
       uint64_t m = 0;
       for (uint64_t i = 0; i < ns; ++i) m += std::min<uint32_t>(cnt[s0 + i], cfg.cap);
-      r.records.assign(rec + o, rec + o + m);
+      if (!r.cancelled) {
+        r.counts.assign(cnt + s0, cnt + s0 + ns);
+        r.records.assign(rec + o, rec + o + m);
+      }
       o += m;
     }
   }
   const auto now = Clock::now();
   std::lock_guard<std::mutex> lk(mu);
   for (auto &rp : L.reqs) {
+    if (rp->staged) {
+      in_arena.release(rp->in_off, rp->in_bytes);
+      rp->staged = false;
+    }
     rp->done = now;
     rp->finished = true;
-    rp->status = status;
-    if (status == FMI_OK) {
+    rp->status = rp->cancelled ? FMI_ECANCELED : status;
+    if (status == FMI_OK && !rp->cancelled) {
       stats.requests_done++;
       stats.reads_done += rp->nreads;
     }
+    if (rp->cancelled) live.erase(rp->id);
   }
   arena.release(L.off, L.lay.size);
   slot_busy[L.slot] = false;
   cv_done.notify_all();
+  cv_submit.notify_all();   // the scheduler may admit into the freed memory
+}
+
+void fmi_server::worker() {
+  std::unique_lock<std::mutex> lk(mu);
+  while (true) {
+    cv_work.wait(lk, [&] { return workers_stop || !work.empty(); });
+    if (work.empty()) return;
+    auto task = std::move(work.front());
+    work.pop_front();
+    lk.unlock();
+    task();
+    lk.lock();
+  }
 }
 
 void fmi_server::run() {
@@ -249,13 +337,19 @@ void fmi_server::run() {
       int rc = FMI_OK;
       bool done = true;
       if (L.phase == 1) {
+        release_inputs(L);          // the reads are on the device
         rc = fetch_records(L);
         done = rc != FMI_OK;
       }
-      if (done) finish_launch(L, rc);
       lk.lock();
-      if (done) inflight.erase(inflight.begin() + (long)i);
-      else ++i;
+      if (done) {
+        Launch fin = std::move(inflight[i]);
+        inflight.erase(inflight.begin() + (long)i);
+        work.emplace_back([this, fin, rc]() mutable { hand_out(std::move(fin), rc); });
+        cv_work.notify_one();
+      } else {
+        ++i;
+      }
     }
     // admission: FIFO, as many requests as fit one launch and the arena
     while (!pending.empty() && inflight.size() < cfg.max_inflight) {
@@ -268,13 +362,13 @@ void fmi_server::run() {
       for (auto &rp : pending) {
         if (nr + rp->nreads > cfg.max_batch_reads) break;
         L.reqs.push_back(rp);
-        bases += rp->reads.size();
+        bases += rp->bases;
         nr += rp->nreads;
       }
       while (!L.reqs.empty()) {
-        L.lay = layout_for(bases, nr, cfg.cap);
+        L.lay = layout_for(bases, nr, L.reqs.size(), cfg.cap);
         if (arena.alloc(L.lay.size, &L.off)) break;
-        bases -= L.reqs.back()->reads.size();
+        bases -= L.reqs.back()->bases;
         nr -= L.reqs.back()->nreads;
         L.reqs.pop_back();
       }
@@ -293,7 +387,7 @@ void fmi_server::run() {
           std::lock_guard<std::mutex> g(mu);
           fault = fmi_last_error();
         }
-        finish_launch(L, rc);
+        hand_out(std::move(L), rc);
         lk.lock();
       } else {
         lk.lock();
@@ -321,12 +415,17 @@ int fmi_server_create(const fmi_index *ix, const fmi_server_config *cfg, fmi_ser
       cfg->max_batch_reads > (1u << 24) || cfg->cap > 4096 ||
       (uint64_t)cfg->max_batch_reads * 2 * cfg->cap >= (1ull << 32) ||
       (cfg->mode != FMI_MODE_EDIT && cfg->mode != FMI_MODE_MISMATCH) || cfg->strand_mask < 1 ||
-      cfg->strand_mask > 3 || cfg->device_budget_bytes < (1u << 20))
+      cfg->strand_mask > 3 || cfg->device_budget_bytes < (1u << 20) ||
+      cfg->max_pending > (1u << 24) ||
+      (cfg->staging_bytes && cfg->staging_bytes < (1u << 20)))
     return FMI_EINVAL;
   auto *srv = new (std::nothrow) fmi_server();
   if (!srv) return FMI_ENOMEM;
   srv->ix = ix;
   srv->cfg = *cfg;
+  if (!srv->cfg.max_pending) srv->cfg.max_pending = kDefaultMaxPending;
+  srv->in_size = cfg->staging_bytes ? cfg->staging_bytes : 2 * cfg->device_budget_bytes;
+  srv->cfg.staging_bytes = srv->in_size;
   srv->device = ix->device;
   void *d = nullptr;
   if (hipSetDevice(ix->device) != hipSuccess || hipMalloc(&d, cfg->device_budget_bytes) != hipSuccess) {
@@ -334,14 +433,18 @@ int fmi_server_create(const fmi_index *ix, const fmi_server_config *cfg, fmi_ser
     return FMI_ENOMEM;
   }
   srv->d_arena = static_cast<char *>(d);
-  void *h = nullptr;
-  if (hipHostMalloc(&h, cfg->device_budget_bytes, hipHostMallocDefault) != hipSuccess) {
+  void *h = nullptr, *hi = nullptr;
+  if (hipHostMalloc(&h, cfg->device_budget_bytes, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&hi, srv->in_size, hipHostMallocDefault) != hipSuccess) {
+    if (h) (void)hipHostFree(h);
     (void)hipFree(srv->d_arena);
     delete srv;
     return FMI_ENOMEM;
   }
-  srv->h_stage = static_cast<uint8_t *>(h);
+  srv->h_mirror = static_cast<uint8_t *>(h);
+  srv->h_in = static_cast<uint8_t *>(hi);
   srv->arena.reset(cfg->device_budget_bytes);
+  srv->in_arena.reset(srv->in_size);
   for (uint32_t s = 0; s < cfg->max_inflight; ++s) {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
@@ -354,6 +457,7 @@ int fmi_server_create(const fmi_index *ix, const fmi_server_config *cfg, fmi_ser
     srv->events.push_back(ev);
     srv->slot_busy.push_back(false);
   }
+  for (uint32_t w = 0; w < kCopyWorkers; ++w) srv->workers.emplace_back([srv] { srv->worker(); });
   srv->thr = std::thread([srv] { srv->run(); });
   *out = srv;
   return FMI_OK;
@@ -368,17 +472,37 @@ int fmi_server_submit(fmi_server *srv, const uint8_t *reads, const uint64_t *off
       return FMI_EINVAL;
   }
   const uint64_t bases = offsets[nreads] - offsets[0];
-  if (layout_for(bases, nreads, srv->cfg.cap).size > srv->cfg.device_budget_bytes) return FMI_ERANGE;
+  const uint64_t in_bytes = input_bytes(bases, nreads);
+  if (layout_for(bases, nreads, 1, srv->cfg.cap).size > srv->cfg.device_budget_bytes ||
+      in_bytes > srv->in_size)
+    return FMI_ERANGE;
   auto r = std::make_shared<Request>();
-  r->reads.assign(reads + offsets[0], reads + offsets[nreads]);
-  r->offsets.resize(nreads + 1);
-  for (uint32_t i = 0; i <= nreads; ++i) r->offsets[i] = offsets[i] - offsets[0];
   r->nreads = nreads;
-  r->submit = Clock::now();
+  r->bases = bases;
+  r->in_bytes = in_bytes;
   {
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return FMI_ESTATE;
+    if (nreads && (srv->pending.size() >= srv->cfg.max_pending ||
+                   !srv->in_arena.alloc(in_bytes, &r->in_off))) {
+      srv->stats.refused++;
+      return FMI_EAGAIN;     // backpressure: the queue or the staging is full
+    }
+    r->staged = nreads != 0;
     r->id = srv->next_id++;
+  }
+  if (nreads) {
+    // the request's only host copy: reads and request-relative offsets
+    // straight into its pinned staging, on the caller's thread
+    uint8_t *dst = srv->h_in + r->in_off;
+    std::memcpy(dst, reads + offsets[0], bases);
+    uint64_t *o = reinterpret_cast<uint64_t *>(dst + align_up(bases ? bases : 1));
+    const uint64_t o0 = offsets[0];
+    for (uint32_t i = 0; i <= nreads; ++i) o[i] = offsets[i] - o0;
+  }
+  r->submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
     srv->live[r->id] = r;
     if (nreads == 0) {
       r->finished = true;
@@ -389,6 +513,37 @@ int fmi_server_submit(fmi_server *srv, const uint8_t *reads, const uint64_t *off
     *request_id = r->id;
   }
   srv->cv_submit.notify_one();
+  return FMI_OK;
+}
+
+int fmi_server_cancel(fmi_server *srv, uint64_t request_id) {
+  if (!srv) return FMI_EINVAL;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->live.find(request_id);
+  if (it == srv->live.end()) return FMI_EINVAL;
+  std::shared_ptr<Request> r = it->second;
+  if (r->finished) {
+    // its results were ready: drop them
+    srv->live.erase(it);
+    return FMI_OK;
+  }
+  r->cancelled = true;
+  srv->stats.cancelled++;
+  auto p = std::find(srv->pending.begin(), srv->pending.end(), r);
+  if (p != srv->pending.end()) {
+    // still queued: never launched
+    srv->pending.erase(p);
+    if (r->staged) {
+      srv->in_arena.release(r->in_off, r->in_bytes);
+      r->staged = false;
+    }
+    r->finished = true;
+    r->status = FMI_ECANCELED;
+    r->done = Clock::now();
+    srv->live.erase(request_id);
+    srv->cv_done.notify_all();
+  }
+  // else on the GPU: its launch completes and the results are dropped
   return FMI_OK;
 }
 
@@ -445,12 +600,20 @@ int fmi_server_destroy(fmi_server *srv) {
   }
   srv->cv_submit.notify_all();
   if (srv->thr.joinable()) srv->thr.join();
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->workers_stop = true;
+  }
+  srv->cv_work.notify_all();
+  for (auto &w : srv->workers)
+    if (w.joinable()) w.join();
   for (auto s : srv->streams) {
     (void)fmi::release_stream_state(s);   // the search's counters on that stream
     (void)hipStreamDestroy(s);
   }
   for (auto e : srv->events) (void)hipEventDestroy(e);
-  if (srv->h_stage) (void)hipHostFree(srv->h_stage);
+  if (srv->h_mirror) (void)hipHostFree(srv->h_mirror);
+  if (srv->h_in) (void)hipHostFree(srv->h_in);
   if (srv->d_arena) (void)hipFree(srv->d_arena);
   delete srv;
   return FMI_OK;

@@ -43,15 +43,16 @@ EXPORTS = [
     "fmi_index_load", "fmi_index_import_reference", "fmi_index_free", "fmi_index_size_sa",
     "fmi_index_sa_ratio", "fmi_index_set_contig_names", "fmi_index_contig_count",
     "fmi_index_contig_name",
-    "fmi_index_device_bytes", "fmi_index_to_device", "fmi_index_get_bwt",
+    "fmi_index_device_bytes", "fmi_index_to_device", "fmi_index_get_bwt", "fmi_index_device_layout",
     "fmi_index_set_karyotype", "fmi_device_count", "fmi_device_alloc", "fmi_device_free",
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_device_mem_info", "fmi_stream_create",
+    "fmi_device_name", "fmi_device_cu_count",
     "fmi_stream_destroy", "fmi_stream_sync", "fmi_graph_begin", "fmi_graph_end",
     "fmi_graph_launch", "fmi_graph_destroy", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
-    "fmi_search1_anchors_batch",
+    "fmi_search1_anchors_batch", "fmi_search1_batch_noalloc",
     "fmi_set_tuning", "fmi_get_tuning", "fmi_set_load_mode", "fmi_get_load_mode", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
-    "fmi_gather_bench",
+    "fmi_gather_bench", "fmi_gather_probe",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_kmer_table", "fmi_index_get_kmer_table",
@@ -59,6 +60,7 @@ EXPORTS = [
     "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
+    "fmi_server_cancel",
     "fmi_server_get_stats", "fmi_server_destroy",
 ]
 
@@ -66,13 +68,18 @@ EXPORTS = [
 class ServerConfig(C.Structure):
     _fields_ = [("device_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32),
                 ("max_batch_reads", C.c_uint32), ("cap", C.c_uint32), ("mode", C.c_int),
-                ("strand_mask", C.c_int)]
+                ("strand_mask", C.c_int), ("max_pending", C.c_uint32),
+                ("staging_bytes", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_uint64), ("reads_done", C.c_uint64),
                 ("launches", C.c_uint64), ("max_batch_reads_seen", C.c_uint64),
-                ("max_inflight_seen", C.c_uint32), ("pending", C.c_uint32)]
+                ("max_inflight_seen", C.c_uint32), ("pending", C.c_uint32),
+                ("refused", C.c_uint64), ("cancelled", C.c_uint64)]
+
+
+EAGAIN, ECANCELED = -8, -9
 
 
 class FmiError(RuntimeError):
@@ -104,6 +111,7 @@ def _bind(lib):
         "fmi_index_size_sa": ([_p], _u64),
         "fmi_index_sa_ratio": ([_p], _u32),
         "fmi_index_device_bytes": ([_p], _u64),
+        "fmi_index_device_layout": ([_p, _p], _i),
         "fmi_index_to_device": ([_p], _i),
         "fmi_index_get_bwt": ([_p, _i, _p, C.POINTER(_u64)], _i),
         "fmi_index_set_karyotype": ([_p, _u64, _p, _p], _i),
@@ -114,6 +122,8 @@ def _bind(lib):
         "fmi_memcpy_d2h": ([_p, _p, C.c_size_t, _p], _i),
         "fmi_memset_d": ([_p, _i, C.c_size_t, _p], _i),
         "fmi_device_mem_info": ([C.POINTER(_u64), C.POINTER(_u64)], _i),
+        "fmi_device_name": ([C.c_char_p, C.c_size_t], _i),
+        "fmi_device_cu_count": ([C.POINTER(_i)], _i),
         "fmi_stream_create": ([C.POINTER(_p)], _i),
         "fmi_stream_destroy": ([_p], _i),
         "fmi_stream_sync": ([_p], _i),
@@ -130,6 +140,7 @@ def _bind(lib):
         "fmi_search1_batch": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_search1_batch_plain": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_search1_anchors_batch": ([_p, _i, _p, _p, _u32, _p, _p], _i),
+        "fmi_search1_batch_noalloc": ([_p, _i, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_set_tuning": ([_u32, _u32, _u32], _i),
         "fmi_get_tuning": ([C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)], _i),
         "fmi_set_load_mode": ([_u32], _i),
@@ -160,6 +171,8 @@ def _bind(lib):
         "fmi_plain_free": ([_p], _i),
         "fmi_gather_bench": ([_p, _u64, _u32, _u64, _p, C.POINTER(C.c_float), C.POINTER(_u64),
                               _p], _i),
+        "fmi_gather_probe": ([_p, _u64, _u32, _u32, _i, _u64, _p, C.POINTER(C.c_float),
+                              C.POINTER(_u64), _p], _i),
         "fmi_search1_count_blocks": ([_p, _i, _i, _p, _p, _u32, C.POINTER(_u64),
                                       C.POINTER(_u64)], _i),
         "fmi_format_results": ([_p, _u64, _i, C.c_char_p, _u32, _p, _u32, _p, _p, _u32, _p,
@@ -172,6 +185,7 @@ def _bind(lib):
         "fmi_server_submit": ([_p, _p, _p, _u32, C.POINTER(_u64)], _i),
         "fmi_server_wait": ([_p, _u64, C.c_int64, C.POINTER(C.c_double), C.POINTER(_u64)], _i),
         "fmi_server_fetch": ([_p, _u64, _p, _p], _i),
+        "fmi_server_cancel": ([_p, _u64], _i),
         "fmi_server_get_stats": ([_p, C.POINTER(ServerStats)], _i),
         "fmi_server_destroy": ([_p], _i),
     }
@@ -208,6 +222,19 @@ def mem_info():
     f, t = C.c_uint64(), C.c_uint64()
     check(lib().fmi_device_mem_info(C.byref(f), C.byref(t)))
     return int(f.value), int(t.value)
+
+
+def device_name():
+    """Marketing name of the current device as HIP reports it."""
+    buf = C.create_string_buffer(256)
+    check(lib().fmi_device_name(buf, 256))
+    return buf.value.decode(errors="replace")
+
+
+def cu_count():
+    n = C.c_int(0)
+    check(lib().fmi_device_cu_count(C.byref(n)))
+    return int(n.value)
 
 
 def device_count():
@@ -475,6 +502,13 @@ class Index:
         text = np.empty(max((self.n + 15) // 16, 1), np.uint32)
         check(lib().fmi_index_get_full(self.h, forward, _ptr(sa), _ptr(isa), _ptr(text)))
         return sa, isa, text[:(self.n + 15) // 16]
+
+    def layout_bytes(self):
+        """Device bytes by part (fmi_index_device_layout)."""
+        a = np.zeros(3, np.uint64)
+        check(lib().fmi_index_device_layout(self.h, _ptr(a)))
+        return {"blocks_and_samples": int(a[0]), "sa_isa_text": int(a[1]),
+                "kmer_tables": int(a[2])}
 
     def samples(self, forward=0):
         """(S, R): the sampled SA and ISA of one direction (S.vec / R.vec)."""
@@ -931,14 +965,16 @@ def seed_map_batch(ix, batch, seed_len, stride, filter_seed_mappings=1000, regio
 
 class Server:
     """Continuous-batching request loop over a device-resident index
-    (fmi_server_*).  submit() copies the reads and returns a request id;
-    wait(id) returns (counts[n, 2], records, latency_ms) where records holds
-    the stored records of every (read, strand) slot, slot after slot."""
+    (fmi_server_*).  submit() copies the reads into the server's staging and
+    returns a request id (retrying while the server answers FMI_EAGAIN, or
+    returning None with block=False); wait(id) returns (counts[n, 2],
+    records, latency_ms) where records holds the stored records of every
+    (read, strand) slot, slot after slot."""
 
     def __init__(self, ix, budget_bytes=1 << 30, max_inflight=4, max_batch_reads=1 << 20,
-                 cap=16, mode=MODE_MISMATCH, strand_mask=3):
+                 cap=16, mode=MODE_MISMATCH, strand_mask=3, max_pending=0, staging_bytes=0):
         self.cfg = ServerConfig(budget_bytes, max_inflight, max_batch_reads, cap, mode,
-                                strand_mask)
+                                strand_mask, max_pending, staging_bytes)
         h = C.c_void_p()
         check(lib().fmi_server_create(ix.h, C.byref(self.cfg), C.byref(h)))
         self.h = h
@@ -946,21 +982,37 @@ class Server:
         self.cap = cap
         self._n = {}
 
-    def submit(self, reads, offsets):
+    def submit(self, reads, offsets, block=True, max_wait_s=60.0):
+        import time as _t
         reads = np.ascontiguousarray(reads, dtype=np.uint8)
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         n = len(offsets) - 1
         rid = C.c_uint64()
-        check(lib().fmi_server_submit(self.h, _ptr(reads) if len(reads) else None, _ptr(offsets),
-                                      n, C.byref(rid)))
+        t0 = None
+        while True:
+            rc = lib().fmi_server_submit(self.h, _ptr(reads) if len(reads) else None,
+                                         _ptr(offsets), n, C.byref(rid))
+            if rc != EAGAIN:
+                break
+            if not block:
+                return None
+            t0 = t0 or _t.perf_counter()
+            if _t.perf_counter() - t0 > max_wait_s:
+                raise FmiError(rc, "server queue stayed full")
+            _t.sleep(5e-5)
+        check(rc)
         self._n[rid.value] = n
         return rid.value
+
+    def cancel(self, rid):
+        check(lib().fmi_server_cancel(self.h, rid))
+        self._n.pop(rid, None)
 
     def wait(self, rid, timeout_us=-1):
         lat = C.c_double()
         nrec = C.c_uint64()
         rc = lib().fmi_server_wait(self.h, rid, timeout_us, C.byref(lat), C.byref(nrec))
-        if rc == -8:
+        if rc == EAGAIN:
             return None
         check(rc)
         n = self._n.pop(rid)
@@ -1016,3 +1068,29 @@ def gather_bench(nbytes=8 << 30, line_bytes=64, nlines=1 << 28, stream=None):
     buf.free()
     sink.free()
     return float(ms.value), int(done.value)
+
+
+class GatherProbe:
+    """The random 64-byte line gather (fmi_gather_probe) over one buffer of
+    nbytes, kept allocated across sweep points."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.buf = DeviceBuffer(self.nbytes)
+        self.buf.zero()
+        self.sink = DeviceBuffer(8)
+        self.sink.zero()
+
+    def run(self, lines_in_flight, wg_per_cu, cooperative, nlines=1 << 26, stream=None):
+        """(GB/s, G lines/s, ms) of one launch."""
+        ms = C.c_float(0)
+        done = C.c_uint64(0)
+        check(lib().fmi_gather_probe(self.buf.ptr, self.nbytes, lines_in_flight, wg_per_cu,
+                                     int(bool(cooperative)), nlines, self.sink.ptr, C.byref(ms),
+                                     C.byref(done), stream))
+        sec = float(ms.value) / 1e3
+        return done.value * 64 / sec / 1e9, done.value / sec / 1e9, float(ms.value)
+
+    def close(self):
+        self.buf.free()
+        self.sink.free()

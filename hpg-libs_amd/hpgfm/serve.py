@@ -88,32 +88,52 @@ def run(server, pool, rate, duration_s, req_reads=(1, 50_000), seed=9, keep=Fals
     return out
 
 
-def run_closed(server, pool, nrequests, window=16, req_reads=(1, 50_000), seed=11):
-    """Closed loop: keeps `window` requests in flight until `nrequests` have
-    been served, so the server runs at its capacity without an unbounded
-    backlog.  Returns reads/s and the latency distribution."""
+def run_closed(server, pool, nrequests, window=16, req_reads=(1, 50_000), seed=11, clients=4):
+    """Closed loop: `clients` threads each keep window / clients requests in
+    flight until `nrequests` have been served in all, so the server runs at
+    its capacity without an unbounded backlog (each client's submits copy
+    their reads into the server's staging in parallel with the others').
+    Returns reads/s and the latency distribution."""
     reads, offs = pool
     npool = len(offs) - 1
     rng = np.random.default_rng(seed)
     sizes = np.minimum(rng.integers(req_reads[0], req_reads[1] + 1, nrequests), npool)
     starts = rng.integers(0, npool - sizes + 1)
-    lat = []
-    inflight = []
+    clients = max(1, min(clients, window))
+    lat = [[] for _ in range(clients)]
+    err = []
+
+    def client(c):
+        mine = list(range(c, nrequests, clients))
+        per = max(1, window // clients)
+        inflight = []
+        nxt = 0
+        try:
+            while nxt < len(mine) or inflight:
+                while nxt < len(mine) and len(inflight) < per:
+                    q = mine[nxt]
+                    a, b = int(starts[q]), int(starts[q] + sizes[q])
+                    o = offs[a:b + 1]
+                    inflight.append(server.submit(reads[int(o[0]):int(o[-1])], o - o[0]))
+                    nxt += 1
+                _, _, ms = server.wait(inflight.pop(0))
+                lat[c].append(ms)
+        except Exception as e:   # noqa: BLE001 - surfaced to the caller below
+            err.append(e)
+
+    th = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
     t0 = time.perf_counter()
-    nxt = 0
-    while nxt < nrequests or inflight:
-        while nxt < nrequests and len(inflight) < window:
-            a, b = int(starts[nxt]), int(starts[nxt] + sizes[nxt])
-            o = offs[a:b + 1]
-            inflight.append(server.submit(reads[int(o[0]):int(o[-1])], o - o[0]))
-            nxt += 1
-        _, _, ms = server.wait(inflight.pop(0))
-        lat.append(ms)
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
     wall = time.perf_counter() - t0
-    lat_a = np.array(lat, dtype=np.float64)
+    if err:
+        raise err[0]
+    lat_a = np.array([x for v in lat for x in v], dtype=np.float64)
     total = int(sizes.sum())
-    return {"window": window, "requests": nrequests, "reads": total, "wall_s": round(wall, 3),
-            "reads_per_s": round(total / wall) if wall > 0 else 0,
+    return {"window": window, "clients": clients, "requests": nrequests, "reads": total,
+            "wall_s": round(wall, 3), "reads_per_s": round(total / wall) if wall > 0 else 0,
             "p50_ms": round(float(np.percentile(lat_a, 50)), 3),
             "p99_ms": round(float(np.percentile(lat_a, 99)), 3),
             "server": server.stats()}

@@ -47,7 +47,9 @@ typedef enum {
   FMI_EDEVICE = -5,   /* HIP runtime error or no usable GPU             */
   FMI_ERANGE = -6,    /* a size is beyond what the engine supports      */
   FMI_ESTATE = -7,    /* call out of order (e.g. index not on device)   */
-  FMI_EAGAIN = -8     /* not finished yet (fmi_server_wait timed out)   */
+  FMI_EAGAIN = -8,    /* not now: fmi_server_wait timed out, or the
+                         server's queue / staging is full (submit again) */
+  FMI_ECANCELED = -9  /* the request was cancelled (fmi_server_cancel)  */
 } fmi_status;
 
 #define FMI_MAX_READ_LEN 1000
@@ -166,6 +168,10 @@ int fmi_index_get_samples(const fmi_index *ix, int forward, uint32_t *S, uint32_
 /* size_SA (search/runtime.h:86): n + 1. */
 uint64_t fmi_index_size_sa(const fmi_index *ix);
 uint32_t fmi_index_sa_ratio(const fmi_index *ix);
+/* Bytes of the device copy by part: out[0] occurrence blocks and S / R
+ * samples of both directions, out[1] the full SA / ISA and 2-bit text
+ * (fmi_index_expand), out[2] the k-mer tables (fmi_index_kmer_table). */
+int fmi_index_device_layout(const fmi_index *ix, uint64_t *out);
 /* Bytes the device copy occupies. */
 uint64_t fmi_index_device_bytes(const fmi_index *ix);
 /* Copies the index into HBM of the current device (once). */
@@ -194,6 +200,10 @@ int fmi_memcpy_d2h(void *dst, const void *d_src, size_t bytes, fmi_stream s);
 /* On a stream that is being captured this records a fill kernel, not a
  * memset node. */
 int fmi_memset_d(void *d_dst, int value, size_t bytes, fmi_stream s);
+/* Name (as HIP reports it, NUL-terminated, at most size - 1 bytes) and
+ * compute-unit count of the current device. */
+int fmi_device_name(char *name, size_t size);
+int fmi_device_cu_count(int *cus);
 /* Free and total bytes of the current device (hipMemGetInfo). */
 int fmi_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
 int fmi_stream_create(fmi_stream *s);
@@ -287,6 +297,21 @@ int fmi_search1_batch(const fmi_index *ix, int mode, int strand_mask,
 int fmi_search1_anchors_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
                               const uint64_t *d_offsets, uint32_t nreads, fmi_hit *d_anchors,
                               fmi_stream s);
+
+/* fmi_search1_batch for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, records and
+ * counts; the launch's work-queue counters live in a fixed 256-byte device
+ * array of the library (one per GPU, held until the process ends) instead
+ * of the per-stream workspace fmi_search1_batch allocates on a stream's
+ * first call, and the call records no timing events.  The call calls no
+ * hipMalloc / hipFree; every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (on any streams) race: serialise them.  Its
+ * scratch need does not depend on the batch (1 counter + 5 work counters);
+ * the batch limits are fmi_search1_batch's. */
+int fmi_search1_batch_noalloc(const fmi_index *ix, int mode, int strand_mask,
+                              const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
+                              uint32_t cap, fmi_hit *d_hits, uint32_t *d_counts, fmi_stream s);
 
 /* The same results from the plain design of the reference's GPU glue
  * (gpu.cu:118-203): one thread per (read, strand), no work queue, one launch
@@ -510,6 +535,8 @@ typedef struct fmi_server_config {
   uint32_t cap;                  /* records kept per (read, strand)           */
   int mode;                      /* FMI_MODE_EDIT or FMI_MODE_MISMATCH        */
   int strand_mask;               /* FMI_STRAND_PLUS | FMI_STRAND_MINUS        */
+  uint32_t max_pending;          /* requests queued before submit refuses (0: 4096) */
+  uint64_t staging_bytes;        /* pinned input staging (0: 2 x the budget)  */
 } fmi_server_config;
 
 typedef struct fmi_server_stats {
@@ -519,14 +546,26 @@ typedef struct fmi_server_stats {
   uint64_t max_batch_reads_seen;  /* largest launch                         */
   uint32_t max_inflight_seen;     /* most launches in flight at once         */
   uint32_t pending;               /* requests waiting for admission now      */
+  uint64_t refused;               /* submits refused with FMI_EAGAIN         */
+  uint64_t cancelled;             /* requests cancelled before they finished */
 } fmi_server_stats;
 
 int fmi_server_create(const fmi_index *ix, const fmi_server_config *cfg, fmi_server **out);
-/* Queues a request: reads/offsets as for fmi_search1_batch, copied before
- * returning.  FMI_ERANGE when the request alone exceeds the budget or
- * max_batch_reads (it is rejected, never queued). */
+/* Queues a request: reads/offsets as for fmi_search1_batch.  The reads and
+ * offsets are copied, once, into the server's pinned staging before the
+ * call returns (on the calling thread, so callers on several threads copy
+ * in parallel).  FMI_ERANGE when the request alone exceeds the budget, the
+ * staging or max_batch_reads (it is rejected, never queued); FMI_EAGAIN,
+ * nothing queued, when max_pending requests are already queued or the
+ * staging has no room now (backpressure: submit again after a wait). */
 int fmi_server_submit(fmi_server *srv, const uint8_t *reads, const uint64_t *offsets,
                       uint32_t nreads, uint64_t *request_id);
+/* Cancels a request: one still queued is removed (it is never searched and
+ * its id is released at once); one already on the GPU completes there and
+ * its results are dropped; a finished one's results are released.  A
+ * fmi_server_wait in progress on it returns FMI_ECANCELED; later calls with
+ * the id return FMI_EINVAL. */
+int fmi_server_cancel(fmi_server *srv, uint64_t request_id);
 /* Waits up to timeout_us (< 0: forever) for a request; FMI_EAGAIN on time-out.
  * latency_ms: from submit to the moment its records reached host memory;
  * nrecords: the number of stored records (sum over its 2*nreads slots of
@@ -593,6 +632,15 @@ int fmi_plain_free(fmi_plain *p);
 int fmi_gather_bench(const void *d_buf, uint64_t bytes, uint32_t line_bytes,
                      uint64_t nlines, uint64_t *d_sink, float *ms, uint64_t *lines_done,
                      fmi_stream s);
+/* The random-access roofline as a sweep point: 64-byte aligned lines at
+ * random over d_buf (bytes >= 64 MiB), lines_in_flight (1, 2, 4, 8, 16)
+ * loads outstanding per lane -- or, with cooperative, per quad of lanes that
+ * load a 16-byte quarter of the same line each -- wg_per_cu workgroups of
+ * 256 lanes per CU, at least nlines lines (*lines_done exactly).  *ms is the
+ * kernel time.  No reference counterpart (measurement only). */
+int fmi_gather_probe(const void *d_buf, uint64_t bytes, uint32_t lines_in_flight,
+                     uint32_t wg_per_cu, int cooperative, uint64_t nlines, uint64_t *d_sink,
+                     float *ms, uint64_t *lines_done, fmi_stream s);
 /* Streaming copy of bytes (a multiple of 16) from d_src to d_dst: a kernel
  * whose HBM traffic is known (bytes read, bytes written), to calibrate
  * hardware fetch/write counters and to measure the streaming bandwidth.

@@ -73,3 +73,78 @@ def test_more_reads_than_budget_wait_for_memory(served):
         assert c.shape == (1000, 2)
     assert srv.stats()["requests_done"] == 40
     srv.close()
+
+
+def test_backpressure_refuses_then_serves(served):
+    """A full queue (max_pending) or a full input staging refuses a submit
+    with FMI_EAGAIN and queues nothing; every accepted request is served with
+    the records of its reads alone, and a refused one goes through once the
+    queue drains."""
+    hp, ix, pool = served
+    reads, offs = pool
+    cap = 4
+    srv = hp.Server(ix, budget_bytes=8 << 20, max_inflight=1, max_batch_reads=2000, cap=cap,
+                    max_pending=3, staging_bytes=2 << 20)
+    accepted, refused = [], 0
+    for i in range(30):
+        a, b = i * 2000, i * 2000 + 2000
+        o = offs[a:b + 1]
+        rid = srv.submit(reads[int(o[0]):int(o[-1])], o - o[0], block=False)
+        if rid is None:
+            refused += 1
+        else:
+            accepted.append((a, rid))
+    st = srv.stats()
+    assert refused > 0 and st["refused"] == refused and st["pending"] <= 3
+    # the same request again, blocking: it waits for room and is served
+    o = offs[0:2001]
+    late = srv.submit(reads[int(o[0]):int(o[-1])], o - o[0])
+    accepted.append((0, late))
+    for a, rid in accepted:
+        c, h, ms = srv.wait(rid, timeout_us=60_000_000)
+        o = offs[a:a + 2001]
+        batch = hp.ReadBatch(reads[int(o[0]):int(o[-1])], o - o[0])
+        c2, h2 = hp.search1_batch(ix, batch, cap=cap, mode=hp.MODE_MISMATCH)
+        batch.free()
+        assert np.array_equal(c, c2)
+        alone = h2[np.arange(cap)[None, None, :] < np.minimum(c2, cap)[:, :, None]]
+        assert fo.hits_equal(h, alone)
+    assert srv.stats()["requests_done"] == len(accepted)
+    srv.close()
+
+
+def test_cancel(served):
+    """Cancelled requests that were still queued are never served (their ids
+    are released); the others are served with the records of their reads
+    alone; done + cancelled = submitted."""
+    hp, ix, pool = served
+    reads, offs = pool
+    cap = 4
+    srv = hp.Server(ix, budget_bytes=8 << 20, max_inflight=1, max_batch_reads=2000, cap=cap)
+    ids = []
+    for i in range(24):
+        a = i * 2000
+        o = offs[a:a + 2001]
+        ids.append((a, srv.submit(reads[int(o[0]):int(o[-1])], o - o[0])))
+    cancelled = set()
+    for j in range(23, 5, -3):          # from the back of the queue: still pending
+        srv.cancel(ids[j][1])
+        cancelled.add(ids[j][1])
+    for a, rid in ids:
+        if rid in cancelled:
+            with pytest.raises(hp.FmiError) as e:
+                srv.wait(rid, timeout_us=1000)
+            assert e.value.code in (-1, -9)
+            continue
+        c, h, ms = srv.wait(rid, timeout_us=60_000_000)
+        o = offs[a:a + 2001]
+        batch = hp.ReadBatch(reads[int(o[0]):int(o[-1])], o - o[0])
+        c2, h2 = hp.search1_batch(ix, batch, cap=cap, mode=hp.MODE_MISMATCH)
+        batch.free()
+        assert np.array_equal(c, c2)
+        alone = h2[np.arange(cap)[None, None, :] < np.minimum(c2, cap)[:, :, None]]
+        assert fo.hits_equal(h, alone)
+    st = srv.stats()
+    assert st["cancelled"] == len(cancelled)
+    assert st["requests_done"] + st["cancelled"] == len(ids)
+    srv.close()
