@@ -1155,6 +1155,32 @@ int fmi_map_inexact_seeds_batch(const fmi_index *ix, const uint8_t *d_reads,
   return rc ? fail(rc, err) : FMI_OK;
 }
 
+int fmi_map_reads_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64_t *d_offsets,
+                        uint32_t nreads, uint32_t filter_read_mappings, uint32_t cap,
+                        fmi_read_mapping *d_out, uint32_t *d_counts, uint8_t *d_status,
+                        fmi_stream s) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (filter_read_mappings > 4096) return fail(FMI_EINVAL, "filter_read_mappings must be 0..4096");
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_counts || !d_status) return fail(FMI_EINVAL, "NULL buffer");
+  if (cap > 0 && !d_out) return fail(FMI_EINVAL, "d_out is NULL with cap > 0");
+  if (nreads > (1u << 30)) return fail(FMI_ERANGE, "more than 2^30 reads in one batch");
+  if (ix->kstart.empty()) return fail(FMI_ESTATE, "the index has no karyotype");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  const uint32_t lanes = fmi::map_reads_lanes(nreads, filter_read_mappings);
+  void *scr = nullptr;
+  rc = scratch_for(st, fmi::map_reads_scratch_bytes(lanes, filter_read_mappings), &scr);
+  if (rc) return rc;
+  rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_map_reads(ix->dev, d_reads, d_offsets, nreads, ix->d_koffset,
+                                       ix->d_kstart, (uint32_t)ix->kstart.size(),
+                                       filter_read_mappings, cap, d_out, d_counts, d_status, scr,
+                                       lanes, st);
+  return end_timed(st, e, "map reads launch");
+}
+
 int fmi_format_results(const fmi_index *ix, uint64_t read_index, int strand, const char *mapping,
                        uint32_t nW, const fmi_hit *hits, uint32_t nhits, const int32_t *chrom,
                        const uint64_t *coord, uint32_t row_cap, char *buf, size_t buf_size,

@@ -69,6 +69,16 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
                            fmi_hitk *anchors, uint32_t *nanchors, uint8_t *full, void *scratch,
                            uint32_t lanes, uint32_t *counter, hipStream_t s);
+// The read-level mapping of __bwt_map_inexact_read (fmi_mapread.hip):
+// `lanes` (map_reads_lanes) stride over the reads, each with a scratch slice
+// of filter + 1 alignments (map_reads_scratch_bytes in all).
+uint32_t map_reads_lanes(uint32_t nreads, uint32_t filter);
+uint64_t map_reads_scratch_bytes(uint32_t lanes, uint32_t filter);
+hipError_t launch_map_reads(const DevIndex &ix, const uint8_t *reads, const uint64_t *offsets,
+                            uint32_t nreads, const uint64_t *koff, const uint64_t *kstart,
+                            uint32_t kcount, uint32_t filter, uint32_t cap, fmi_read_mapping *out,
+                            uint32_t *counts, uint8_t *status, void *scratch, uint32_t lanes,
+                            hipStream_t s);
 hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
                          unsigned long long *sink, uint32_t grid, hipStream_t s);
 uint64_t gather_loads_done(uint64_t nloads, uint32_t grid);

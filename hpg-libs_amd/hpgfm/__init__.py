@@ -54,7 +54,7 @@ EXPORTS = [
     "fmi_set_tuning", "fmi_get_tuning", "fmi_set_load_mode", "fmi_get_load_mode", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
     "fmi_gather_bench", "fmi_gather_probe",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
-    "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
+    "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_map_reads_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_kmer_table", "fmi_index_get_kmer_table",
     "fmi_index_get_full", "fmi_index_sample_count", "fmi_index_get_samples",
     "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
@@ -179,6 +179,7 @@ def _bind(lib):
                                 C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(_u64)], _i),
         "fmi_map_inexact_seeds_batch": ([_p, _p, _p, _u32, _u32, _u32, _u32, _u32, _u64, _p,
                                          _p, _p, C.POINTER(_u64), _p], _i),
+        "fmi_map_reads_batch": ([_p, _p, _p, _u32, _u32, _u32, _p, _p, _p, _p], _i),
         "fmi_seed_map_batch": ([_p, _p, _p, _u32, _u32, _u32, _u32, _p, _u64, _p, _u64,
                                 C.POINTER(_u64), C.POINTER(_u64), _p], _i),
         "fmi_server_create": ([_p, C.POINTER(ServerConfig), C.POINTER(_p)], _i),
@@ -882,6 +883,31 @@ INEXACT_REGION_DTYPE = np.dtype([
     ("pos_err", "<i4"), ("type_err", "<i4"), ("strand", "u1"), ("pad", "u1", (7,)),
 ])
 assert INEXACT_REGION_DTYPE.itemsize == 48
+
+
+READ_MAPPING_DTYPE = np.dtype([
+    ("position", "<u8"), ("chromosome", "<u4"), ("read", "<u4"), ("strand", "u1"),
+    ("secondary", "u1"), ("cigar_len", "<u2"), ("pad", "<u4"),
+])
+assert READ_MAPPING_DTYPE.itemsize == 24
+
+
+def map_reads_batch(ix, batch, filter_read_mappings, cap, stream=None):
+    """fmi_map_reads_batch (__bwt_map_inexact_read's alignments): (status[n],
+    counts[n], alignments[n, cap]) on the host."""
+    n = batch.nreads
+    d_out = DeviceBuffer(max(n * cap, 1) * 24)
+    d_cnt = DeviceBuffer(max(n, 1) * 4)
+    d_st = DeviceBuffer(max(n, 1))
+    check(lib().fmi_map_reads_batch(ix.h, batch.d_reads.ptr, batch.d_offsets.ptr, n,
+                                    filter_read_mappings, cap, d_out.ptr, d_cnt.ptr, d_st.ptr,
+                                    stream))
+    sync(stream)
+    res = (d_st.download(np.uint8, n, stream), d_cnt.download(np.uint32, n, stream),
+           d_out.download(READ_MAPPING_DTYPE, n * cap, stream).reshape(n, cap))
+    for b in (d_out, d_cnt, d_st):
+        b.free()
+    return res
 
 
 def windows_of(offsets, seed_size, min_seed_size):

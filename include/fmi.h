@@ -513,6 +513,40 @@ int fmi_map_inexact_seeds_batch(const fmi_index *ix, const uint8_t *d_reads,
                                 uint32_t *d_nregions, uint8_t *d_status, uint64_t *nwindows,
                                 fmi_stream s);
 
+/* One alignment of __bwt_map_inexact_read (bwt.c:2158-2178): 24 bytes. */
+typedef struct fmi_read_mapping {
+  uint64_t position;    /* 0-based start[idx-1] + key - offset[idx-1] (bwt.c:2166) */
+  uint32_t chromosome;  /* idx - 1 */
+  uint32_t read;        /* read index in the batch */
+  uint8_t strand;       /* !type: 0 for +, 1 for - */
+  uint8_t secondary;    /* secondary_alignment as bwt.c:2217-2233 leaves it */
+  uint16_t cigar_len;   /* the CIGAR is "<cigar_len>M", one operation */
+  uint32_t pad;
+} fmi_read_mapping;
+
+/* __bwt_map_inexact_read (bwt.c:1890-2236) for every read, up to its list
+ * of alignments: the error-free records of the read's vector-driven 1-error
+ * search of each strand (+ then -; bwt.c:2038 skips every other record),
+ * the running count of l - k over them checked against
+ * filter_read_mappings before each record's rows (bwt.c:2145-2156: over it,
+ * that strand's alignments are dropped and no later strand is searched),
+ * one alignment per SA row inside a contig (bwt.c:2158-2178), then per
+ * strand the dedup of alignments within 10 bp of an earlier one on the same
+ * chromosome (the reference's int positions, bwt.c:2190-2215) and the
+ * survivors appended from last to first, secondary unless first
+ * (bwt.c:2217-2233).  Per read i: d_out[i * cap ..] the first cap
+ * alignments, d_counts[i] their total, d_status[i] bit 0 when the filter
+ * was exceeded, bit 1 when the read has fewer than 5 bases (the reference
+ * makes one unmapped placeholder, bwt.c:1899-1915; none here), bit 2 when it
+ * is longer than FMI_MAX_READ_LEN (not searched).  The anchor lists the
+ * reference builds for reads left unmapped (bwt.c:2239-2269) are not made.
+ * filter_read_mappings is 0..4096; the call allocates its scratch on the
+ * stream (per lane, filter + 1 alignments of 16 bytes). */
+int fmi_map_reads_batch(const fmi_index *ix, const uint8_t *d_reads, const uint64_t *d_offsets,
+                        uint32_t nreads, uint32_t filter_read_mappings, uint32_t cap,
+                        fmi_read_mapping *d_out, uint32_t *d_counts, uint8_t *d_status,
+                        fmi_stream s);
+
 /* ----------------------------------------------------------- serving */
 /* A request loop over a device-resident index (no counterpart in the
  * reference, whose closest analog is the bounded work queue of

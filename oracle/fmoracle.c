@@ -803,6 +803,109 @@ uint32_t fmo_map_inexact_seeds(const fmo_index *ix, const uint8_t *W, uint32_t l
   return nwin;
 }
 
+/* __bwt_map_inexact_read (bwt.c:1890-2236) for one read, up to the list of
+ * alignments it builds: the vector-driven 1-error search of each strand (+
+ * then -, records kept in a list of filter capacity, bwt.c:1984-2003), only
+ * its error-free records (bwt.c:2026-2038), the running count of l - k over
+ * them checked against filter before each record's rows (bwt.c:2145-2156;
+ * over the filter: the strand's alignments are dropped, status bit 0, no
+ * later strand), one alignment per SA row inside a contig at 0-based
+ * start[idx-1] + key - offset[idx-1] (bwt.c:2158-2166), then per strand the
+ * within-10-bp dedup on the int positions (bwt.c:2190-2215), the survivors
+ * appended from last to first with the secondary flag num_mappings > 0 and
+ * the primary restored on the list's first when a primary was deleted
+ * (bwt.c:2217-2233).  Reads under 5 bases give no alignment and status bit 1
+ * (the reference's one unmapped placeholder, bwt.c:1899-1915).  At most cap
+ * alignments are stored; returns their total. */
+static void map_read_strand(const fmo_index *ix, const uint8_t *W, uint32_t len, int type,
+                            uint32_t filter, const uint64_t *koff, const uint64_t *kstart,
+                            uint64_t kcount, uint32_t read, fmo_hit *recs,
+                            fmo_read_mapping *tmp, uint8_t *del, uint64_t *tot, int *exceeded,
+                            fmo_read_mapping *out, uint32_t cap, uint32_t *nout) {
+  /* bwt.c:1990-2003: type 1 is the + strand (backward, forward), type 0 the
+   * - strand (forward_rev, backward_rev) */
+  const uint32_t nrec = fmo_search1_vector(ix, type ? 0 : 1, 0, W, 0, (int16_t)(len - 1), recs,
+                                           filter);
+  const uint32_t stored = nrec < filter ? nrec : filter;
+  uint32_t n = 0;
+  for (uint32_t ii = 0; ii < stored; ii++) {
+    const fmo_hit *r = &recs[ii];
+    const int error = r->num_mismatches ? r->err_kind : 0;
+    if (error != 0) continue;                          /* bwt.c:2038 */
+    const uint64_t direction = type ? r->dir : !r->dir;
+    const uint64_t len_calc = len;
+    const uint64_t k_start = (uint64_t)r->k, l_start = (uint64_t)r->l;
+    *tot += l_start - k_start;
+    if (*tot > filter) { *exceeded = 1; return; }     /* bwt.c:2147-2156 */
+    for (uint64_t j = k_start; j <= l_start; j++) {
+      const uint64_t key = direction ? fmo_size_sa(ix) - fmo_get_sa(ix, 1, j) - len_calc - 1
+                                     : fmo_get_sa(ix, 0, j);
+      const uint64_t idx = fmo_binsearch(koff, kcount, key);
+      if (key + len_calc <= koff[idx]) {
+        fmo_read_mapping *m = &tmp[n++];
+        m->position = kstart[idx - 1] + (key - koff[idx - 1]);
+        m->chromosome = (uint32_t)(idx - 1);
+        m->read = read;
+        m->strand = (uint8_t)!type;
+        /* created with (num_mappings > 0), bwt.c:2176 */
+        m->secondary = (uint8_t)(*nout > 0);
+        m->cigar_len = (uint16_t)len;
+        m->pad = 0;
+      }
+    }
+  }
+  /* bwt.c:2190-2215: every alignment carries one CIGAR operation ("<len>M"),
+   * so a pair within 10 of each other deletes the later one (a1) */
+  memset(del, 0, n);
+  for (int a1 = (int)n - 1; a1 >= 1; a1--) {
+    if (del[a1]) continue;
+    for (int a2 = a1 - 1; a2 >= 0; a2--) {
+      /* int positions and abs() of their int difference */
+      const int32_t p1 = (int32_t)(uint32_t)tmp[a1].position;
+      const int32_t p2 = (int32_t)(uint32_t)tmp[a2].position;
+      const int32_t d = (int32_t)((uint32_t)p1 - (uint32_t)p2);
+      const uint64_t dist = (uint64_t)(int64_t)(d < 0 ? -(int64_t)d : (int64_t)d);
+      if (tmp[a1].chromosome == tmp[a2].chromosome && dist < 10 && !del[a2]) {
+        /* alig_1 ops (1) < alig_2 ops (1) never holds */
+        del[a1] = 1;
+      }
+    }
+  }
+  int primary_delete = 0;
+  for (int m = (int)n - 1; m >= 0; m--) {
+    if (del[m]) {
+      if (!tmp[m].secondary) primary_delete = 1;
+    } else {
+      tmp[m].secondary = (uint8_t)(*nout > 0);
+      if (*nout < cap) out[*nout] = tmp[m];
+      (*nout)++;
+    }
+  }
+  if (primary_delete && *nout > 0 && cap > 0) out[0].secondary = 0;
+}
+
+uint32_t fmo_map_read(const fmo_index *ix, const uint8_t *W, uint32_t len, uint32_t filter,
+                      const uint64_t *koff, const uint64_t *kstart, uint64_t kcount,
+                      uint32_t read, fmo_read_mapping *out, uint32_t cap, uint8_t *status) {
+  *status = 0;
+  if (len < 5) { *status = 2; return 0; }
+  fmo_hit *recs = (fmo_hit *)malloc((size_t)(filter ? filter : 1) * sizeof(fmo_hit));
+  /* each strand's exact record has l - k <= filter rows here */
+  fmo_read_mapping *tmp = (fmo_read_mapping *)malloc((size_t)(filter + 1) * sizeof(fmo_read_mapping));
+  uint8_t *del = (uint8_t *)malloc((size_t)filter + 1);
+  uint64_t tot = 0;
+  int exceeded = 0;
+  uint32_t nout = 0;
+  for (int type = 1; type >= 0 && !exceeded; type--)
+    map_read_strand(ix, W, len, type, filter, koff, kstart, kcount, read, recs, tmp, del, &tot,
+                    &exceeded, out, cap, &nout);
+  if (exceeded) *status |= 1;
+  free(recs);
+  free(tmp);
+  free(del);
+  return nout;
+}
+
 uint64_t fmo_binsearch(const uint64_t *array, uint64_t size, uint64_t key) {
   /* binsearch, search/io.h:43-64: first index whose value is > key. */
   if (!array) return 0;

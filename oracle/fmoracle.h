@@ -224,6 +224,27 @@ double fmo_map_inexact_seeds_batch(const fmo_index *ix, const uint8_t *reads,
                                    const uint64_t *kstart, uint64_t kcount, uint32_t rec_cap,
                                    uint32_t region_cap, uint64_t *nregions_total, int threads);
 
+/* ---- read-level 1-error mapping ----------------------------------------- */
+/* One alignment of __bwt_map_inexact_read (bwt.c:2158-2178): 24 bytes,
+ * the layout of fmi_read_mapping. */
+typedef struct {
+  uint64_t position;    /* 0-based start[idx-1] + key - offset[idx-1] */
+  uint32_t chromosome;  /* idx - 1 */
+  uint32_t read;
+  uint8_t strand;       /* !type: 0 +, 1 - */
+  uint8_t secondary;    /* secondary_alignment after bwt.c:2217-2233 */
+  uint16_t cigar_len;   /* the CIGAR "<cigar_len>M" (one operation) */
+  uint32_t pad;
+} fmo_read_mapping;
+
+/* __bwt_map_inexact_read (bwt.c:1890-2236) for one read up to its list of
+ * alignments (the anchor lists of bwt.c:2239-2269 for unmapped reads are not
+ * built): see fmoracle.c.  status bit 0: filter_read_mappings exceeded,
+ * bit 1: read under 5 bases.  At most cap stored; returns the total. */
+uint32_t fmo_map_read(const fmo_index *ix, const uint8_t *W, uint32_t len, uint32_t filter,
+                      const uint64_t *koff, const uint64_t *kstart, uint64_t kcount,
+                      uint32_t read, fmo_read_mapping *out, uint32_t cap, uint8_t *status);
+
 /* ---- position resolution ---------------------------------------------- */
 /* binsearch, search/io.h:43-64: upper_bound of key in array[0..size). */
 uint64_t fmo_binsearch(const uint64_t *array, uint64_t size, uint64_t key);

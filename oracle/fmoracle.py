@@ -51,6 +51,14 @@ HITK_DTYPE = np.dtype([
 assert HITK_DTYPE.itemsize == 64
 
 
+# fmo_read_mapping (fmoracle.h) / fmi_read_mapping (include/fmi.h), 24 bytes
+READ_MAPPING_DTYPE = np.dtype([
+    ("position", "<u8"), ("chromosome", "<u4"), ("read", "<u4"), ("strand", "u1"),
+    ("secondary", "u1"), ("cigar_len", "<u2"), ("pad", "<u4"),
+])
+assert READ_MAPPING_DTYPE.itemsize == 24
+
+
 def hitsk_equal(a, b):
     """k-error records equal field by field, error entries up to num_mismatches."""
     if len(a) != len(b):
@@ -137,6 +145,9 @@ def _load_oracle():
     lib.fmo_search_k_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_uint32, C.c_uint32, _p,
                                        _p, _p, _p, _p, C.c_int]
     lib.fmo_search_k_batch.restype = C.c_double
+    lib.fmo_map_read.argtypes = [_p, _p, C.c_uint32, C.c_uint32, _p, _p, C.c_uint64, C.c_uint32,
+                                 _p, C.c_uint32, _p]
+    lib.fmo_map_read.restype = C.c_uint32
     lib.fmo_exact_batch.argtypes = [_p, _p, _p, C.c_uint64, _p, C.c_int]
     lib.fmo_exact_batch.restype = C.c_double
     lib.fmo_exact_locate_batch.argtypes = [_p, _p, _p, C.c_uint64, C.c_uint32, _p, _p, C.c_int]
@@ -346,6 +357,20 @@ class Oracle:
                                                  _ptr(koff), _ptr(kstart), len(kstart), rec_cap,
                                                  region_cap, C.byref(tot), threads)
         return float(t), int(tot.value)
+
+    def map_read(self, W, filter_read_mappings, koff, kstart, read=0, cap=None):
+        """__bwt_map_inexact_read's alignments of one read: (status, records
+        [READ_MAPPING_DTYPE], total); status bit 0 filter exceeded, bit 1 a
+        read under 5 bases."""
+        W = np.ascontiguousarray(W, dtype=np.uint8)
+        koff = np.ascontiguousarray(koff, dtype=np.uint64)
+        kstart = np.ascontiguousarray(kstart, dtype=np.uint64)
+        cap = 2 * (filter_read_mappings + 1) if cap is None else cap
+        out = np.zeros(max(cap, 1), READ_MAPPING_DTYPE)
+        st = np.zeros(1, np.uint8)
+        n = self.lib.fmo_map_read(self.ref, _ptr(W), len(W), filter_read_mappings, _ptr(koff),
+                                  _ptr(kstart), len(kstart), read, _ptr(out), cap, _ptr(st))
+        return int(st[0]), out[:min(n, cap)].copy(), int(n)
 
     def search1_batch(self, reads, offsets, cap=64, mismatch_only=False, threads=0,
                       keep_hits=True):
