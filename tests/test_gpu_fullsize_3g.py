@@ -207,7 +207,7 @@ def test_3g_planted_coordinates(huge):
     pos = huge["truth"]["pos"].astype(np.int64)
     strand = huge["truth"]["strand"].astype(np.int64)
     mat = huge["reads"].reshape(NREADS, LEN)
-    cand = np.random.default_rng(34).choice(NREADS, 60_000, replace=False)
+    cand = np.random.default_rng(34).choice(NREADS, 150_000, replace=False)
     src = text[pos[cand, None] + np.arange(LEN)[None, :]]
     rc = strand[cand] == 1
     src[rc] = (3 - src[rc])[:, ::-1]
