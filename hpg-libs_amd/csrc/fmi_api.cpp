@@ -1073,6 +1073,31 @@ int fmi_encode_bases(const char *src, uint64_t n, uint8_t *dst) {
   return FMI_OK;
 }
 
+int fmi_fastq_encode_batch(const char *d_seq, const int32_t *d_data_indices, uint32_t num_reads,
+                           uint64_t nbytes, uint8_t *d_codes, uint64_t *d_offsets, fmi_stream s) {
+  if (!d_data_indices || !d_offsets || (nbytes && (!d_seq || !d_codes)))
+    return fail(FMI_EINVAL, "NULL buffer");
+  if (nbytes < num_reads) return fail(FMI_EINVAL, "fewer bytes than reads (each read ends in a NUL)");
+  if (nbytes > (1ull << 40)) return fail(FMI_ERANGE, "nbytes too large");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  int rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_fastq_encode(d_seq, d_data_indices, num_reads, nbytes, 0, 0, d_codes,
+                                          d_offsets, st);
+  return end_timed(st, e, "fastq encode launch");
+}
+
+int fmi_pack_codes(const uint8_t *d_codes, uint64_t nbases, uint32_t *d_packed, uint16_t *d_nmask,
+                   fmi_stream s) {
+  if (nbases == 0) return FMI_OK;
+  if (!d_codes || !d_packed) return fail(FMI_EINVAL, "NULL buffer");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  int rc = begin_timed(st);
+  if (rc) return rc;
+  hipError_t e = fmi::launch_pack_codes(d_codes, nbases, d_packed, d_nmask, st);
+  return end_timed(st, e, "pack launch");
+}
+
 namespace {
 
 int resolve_common(const fmi_index *ix, const fmi_hit *d_hits, const uint32_t *d_counts,

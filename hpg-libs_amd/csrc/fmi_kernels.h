@@ -79,6 +79,15 @@ hipError_t launch_map_reads(const DevIndex &ix, const uint8_t *reads, const uint
                             uint32_t kcount, uint32_t filter, uint32_t cap, fmi_read_mapping *out,
                             uint32_t *counts, uint8_t *status, void *scratch, uint32_t lanes,
                             hipStream_t s);
+// fastq_batch_t ingestion (fmi_ingest.hip): nbytes = data_indices[nreads] -
+// data_indices[0] bytes of seq become codes at base_off.. and offsets at
+// read_off.. (values from 0); packing of nbases codes into 16-base words and
+// 16-bit N masks.
+hipError_t launch_fastq_encode(const char *seq, const int32_t *data_indices, uint32_t nreads,
+                               uint64_t nbytes, uint64_t base_off, uint64_t read_off,
+                               uint8_t *codes, uint64_t *offsets, hipStream_t s);
+hipError_t launch_pack_codes(const uint8_t *codes, uint64_t nbases, uint32_t *packed,
+                             uint16_t *nmask, hipStream_t s);
 hipError_t launch_gather(const void *buf, uint64_t bytes, uint32_t line_bytes, uint64_t nloads,
                          unsigned long long *sink, uint32_t grid, hipStream_t s);
 uint64_t gather_loads_done(uint64_t nloads, uint32_t grid);

@@ -8,8 +8,10 @@
 //   1. takes pending requests first-come first-served while their reads
 //      and results fit max_batch_reads and the device arena;
 //   2. copies each request's reads and request-relative offsets from its
-//      staging into the launch's device region (one copy each), then a
-//      rebase kernel turns the offsets into launch offsets;
+//      staging into the launch's device region (one copy each; a request
+//      submitted in the fastq_batch_t layout is copied as it is and encoded
+//      there by a kernel), then a rebase kernel turns the offsets into
+//      launch offsets;
 //   3. runs the persistent 1-error search and compacts the stored records of
 //      every (read, strand) slot into a dense array (a scan of the counts);
 //   4. copies the counts and the record total into the output mirror and
@@ -56,6 +58,8 @@ struct Request {
   uint64_t id = 0;
   uint32_t nreads = 0;
   uint64_t bases = 0;
+  bool fastq = false;                  // staged as fastq_batch_t seq + data_indices
+  uint64_t raw = 0;                    // fastq: seq bytes (bases + nreads NULs)
   uint64_t in_off = 0, in_bytes = 0;   // staging: reads, then nreads + 1 offsets
   bool staged = false;                 // holds input staging
   std::vector<uint32_t> counts;        // results (2 * nreads)
@@ -68,16 +72,19 @@ struct Request {
 
 // Device layout of one launch.
 struct Layout {
-  uint64_t o_reads, o_offs, o_table, o_hits, o_counts, o_total, o_compact, o_scratch, size;
+  uint64_t o_reads, o_offs, o_table, o_raw, o_hits, o_counts, o_total, o_compact, o_scratch, size;
 };
 
-Layout layout_for(uint64_t bases, uint64_t nreads, uint64_t nreq, uint32_t cap) {
+// raw: staging bytes of the launch's fastq requests (copied up as they are,
+// then encoded on the device)
+Layout layout_for(uint64_t bases, uint64_t nreads, uint64_t nreq, uint32_t cap, uint64_t raw) {
   const uint64_t slots = 2 * nreads;
   Layout L;
   L.o_reads = 0;
   L.o_offs = align_up(bases ? bases : 1);
   L.o_table = L.o_offs + align_up((nreads + 1) * 8);
-  L.o_hits = L.o_table + align_up(2 * nreq * 8);
+  L.o_raw = L.o_table + align_up(2 * nreq * 8);
+  L.o_hits = L.o_raw + align_up(raw);
   L.o_counts = L.o_hits + align_up(slots * (uint64_t)cap * sizeof(fmi_hit));
   L.o_total = L.o_counts + align_up(slots * 4);
   L.o_compact = L.o_total + kAlign;
@@ -88,6 +95,11 @@ Layout layout_for(uint64_t bases, uint64_t nreads, uint64_t nreq, uint32_t cap) 
 
 uint64_t input_bytes(uint64_t bases, uint64_t nreads) {
   return align_up(bases ? bases : 1) + align_up((nreads + 1) * 8);
+}
+
+// staging of a fastq request: seq bytes, then nreads + 1 int32 indices
+uint64_t fastq_bytes(uint64_t raw, uint64_t nreads) {
+  return align_up(raw ? raw : 1) + align_up((nreads + 1) * 4);
 }
 
 // First-fit allocator over [0, size) in kAlign units.
@@ -200,7 +212,7 @@ int fmi_server::start_launch(Launch &L) {
   char *d = d_arena + L.off;
   // request table of the rebase kernel, staged in the output mirror
   uint64_t *table = reinterpret_cast<uint64_t *>(h_mirror + L.off + lay.o_table);
-  uint64_t base = 0;
+  uint64_t base = 0, raw_off = 0;
   uint32_t r = 0;
   for (size_t q = 0; q < L.reqs.size(); ++q) {
     Request &rq = *L.reqs[q];
@@ -208,14 +220,28 @@ int fmi_server::start_launch(Launch &L) {
     table[2 * q] = r;
     table[2 * q + 1] = base;
     const uint8_t *src = h_in + rq.in_off;
-    if (rq.bases &&
-        hipMemcpyAsync(d + lay.o_reads + base, src, rq.bases, hipMemcpyHostToDevice, s) != hipSuccess)
-      return FMI_EDEVICE;
-    // request-relative offsets; the next request's opening entry (0)
-    // overwrites this request's closing one, the rebase kernel adds bases
-    if (hipMemcpyAsync(d + lay.o_offs + 8ull * r, src + align_up(rq.bases ? rq.bases : 1),
-                       8ull * (rq.nreads + 1), hipMemcpyHostToDevice, s) != hipSuccess)
-      return FMI_EDEVICE;
+    if (rq.fastq) {
+      // seq and indices as staged, encoded into the launch's codes and
+      // request-relative offsets on the device
+      char *dst = d + lay.o_raw + raw_off;
+      if (hipMemcpyAsync(dst, src, rq.in_bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+        return FMI_EDEVICE;
+      if (fmi::launch_fastq_encode(dst, reinterpret_cast<const int32_t *>(dst + align_up(rq.raw ? rq.raw : 1)),
+                                   rq.nreads, rq.raw, base, r,
+                                   reinterpret_cast<uint8_t *>(d + lay.o_reads),
+                                   reinterpret_cast<uint64_t *>(d + lay.o_offs), s) != hipSuccess)
+        return FMI_EDEVICE;
+      raw_off += rq.in_bytes;
+    } else {
+      if (rq.bases &&
+          hipMemcpyAsync(d + lay.o_reads + base, src, rq.bases, hipMemcpyHostToDevice, s) != hipSuccess)
+        return FMI_EDEVICE;
+      // request-relative offsets; the next request's opening entry (0)
+      // overwrites this request's closing one, the rebase kernel adds bases
+      if (hipMemcpyAsync(d + lay.o_offs + 8ull * r, src + align_up(rq.bases ? rq.bases : 1),
+                         8ull * (rq.nreads + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+        return FMI_EDEVICE;
+    }
     base += rq.bases;
     r += rq.nreads;
   }
@@ -358,18 +384,20 @@ void fmi_server::run() {
         if (!slot_busy[s]) { slot = (int)s; break; }
       if (slot < 0) break;
       Launch L;
-      uint64_t bases = 0, nr = 0;
+      uint64_t bases = 0, nr = 0, raw = 0;
       for (auto &rp : pending) {
         if (nr + rp->nreads > cfg.max_batch_reads) break;
         L.reqs.push_back(rp);
         bases += rp->bases;
         nr += rp->nreads;
+        if (rp->fastq) raw += rp->in_bytes;
       }
       while (!L.reqs.empty()) {
-        L.lay = layout_for(bases, nr, L.reqs.size(), cfg.cap);
+        L.lay = layout_for(bases, nr, L.reqs.size(), cfg.cap, raw);
         if (arena.alloc(L.lay.size, &L.off)) break;
         bases -= L.reqs.back()->bases;
         nr -= L.reqs.back()->nreads;
+        if (L.reqs.back()->fastq) raw -= L.reqs.back()->in_bytes;
         L.reqs.pop_back();
       }
       if (L.reqs.empty()) break;   // wait for memory to free up
@@ -473,7 +501,7 @@ int fmi_server_submit(fmi_server *srv, const uint8_t *reads, const uint64_t *off
   }
   const uint64_t bases = offsets[nreads] - offsets[0];
   const uint64_t in_bytes = input_bytes(bases, nreads);
-  if (layout_for(bases, nreads, 1, srv->cfg.cap).size > srv->cfg.device_budget_bytes ||
+  if (layout_for(bases, nreads, 1, srv->cfg.cap, 0).size > srv->cfg.device_budget_bytes ||
       in_bytes > srv->in_size)
     return FMI_ERANGE;
   auto r = std::make_shared<Request>();
@@ -505,6 +533,61 @@ int fmi_server_submit(fmi_server *srv, const uint8_t *reads, const uint64_t *off
     std::lock_guard<std::mutex> lk(srv->mu);
     srv->live[r->id] = r;
     if (nreads == 0) {
+      r->finished = true;
+      r->done = r->submit;
+    } else {
+      srv->pending.push_back(r);
+    }
+    *request_id = r->id;
+  }
+  srv->cv_submit.notify_one();
+  return FMI_OK;
+}
+
+int fmi_server_submit_fastq(fmi_server *srv, const char *seq, const int32_t *data_indices,
+                            uint32_t num_reads, uint64_t *request_id) {
+  if (!srv || !data_indices || !request_id || (num_reads && !seq)) return FMI_EINVAL;
+  if (num_reads > srv->cfg.max_batch_reads) return FMI_ERANGE;
+  for (uint32_t i = 0; i < num_reads; ++i) {
+    const int64_t len = (int64_t)data_indices[i + 1] - data_indices[i] - 1;
+    if (len < 0 || len > (int64_t)FMI_MAX_READ_LEN) return FMI_EINVAL;
+  }
+  const uint64_t raw = (uint64_t)((int64_t)data_indices[num_reads] - data_indices[0]);
+  const uint64_t bases = raw - num_reads;
+  const uint64_t in_bytes = fastq_bytes(raw, num_reads);
+  if (layout_for(bases, num_reads, 1, srv->cfg.cap, in_bytes).size > srv->cfg.device_budget_bytes ||
+      in_bytes > srv->in_size)
+    return FMI_ERANGE;
+  auto r = std::make_shared<Request>();
+  r->nreads = num_reads;
+  r->bases = bases;
+  r->fastq = true;
+  r->raw = raw;
+  r->in_bytes = in_bytes;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->stop) return FMI_ESTATE;
+    if (num_reads && (srv->pending.size() >= srv->cfg.max_pending ||
+                      !srv->in_arena.alloc(in_bytes, &r->in_off))) {
+      srv->stats.refused++;
+      return FMI_EAGAIN;
+    }
+    r->staged = num_reads != 0;
+    r->id = srv->next_id++;
+  }
+  if (num_reads) {
+    // the request's only host copy: seq as it is and indices from 0
+    uint8_t *dst = srv->h_in + r->in_off;
+    std::memcpy(dst, seq + data_indices[0], raw);
+    int32_t *o = reinterpret_cast<int32_t *>(dst + align_up(raw ? raw : 1));
+    const int32_t d0 = data_indices[0];
+    for (uint32_t i = 0; i <= num_reads; ++i) o[i] = data_indices[i] - d0;
+  }
+  r->submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->live[r->id] = r;
+    if (num_reads == 0) {
       r->finished = true;
       r->done = r->submit;
     } else {

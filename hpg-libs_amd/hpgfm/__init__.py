@@ -60,7 +60,7 @@ EXPORTS = [
     "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
-    "fmi_server_cancel",
+    "fmi_server_cancel", "fmi_server_submit_fastq", "fmi_fastq_encode_batch", "fmi_pack_codes",
     "fmi_server_get_stats", "fmi_server_destroy",
 ]
 
@@ -187,6 +187,9 @@ def _bind(lib):
         "fmi_server_wait": ([_p, _u64, C.c_int64, C.POINTER(C.c_double), C.POINTER(_u64)], _i),
         "fmi_server_fetch": ([_p, _u64, _p, _p], _i),
         "fmi_server_cancel": ([_p, _u64], _i),
+        "fmi_server_submit_fastq": ([_p, _p, _p, _u32, C.POINTER(_u64)], _i),
+        "fmi_fastq_encode_batch": ([_p, _p, _u32, _u64, _p, _p, _p], _i),
+        "fmi_pack_codes": ([_p, _u64, _p, _p, _p], _i),
         "fmi_server_get_stats": ([_p, C.POINTER(ServerStats)], _i),
         "fmi_server_destroy": ([_p], _i),
     }
@@ -892,6 +895,47 @@ READ_MAPPING_DTYPE = np.dtype([
 assert READ_MAPPING_DTYPE.itemsize == 24
 
 
+def fastq_batch(texts):
+    """The reference's fastq_batch_t sequence layout for a list of read
+    strings (bytes): (seq with a NUL after each read, int32 data_indices)."""
+    seq = b"".join(t + b"\0" for t in texts)
+    di = np.concatenate([[0], np.cumsum([len(t) + 1 for t in texts])]).astype(np.int32)
+    return np.frombuffer(seq, np.uint8).copy(), di
+
+
+def fastq_encode_batch(seq, data_indices, stream=None):
+    """fmi_fastq_encode_batch on the device: (codes, offsets) on the host."""
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    di = np.ascontiguousarray(data_indices, dtype=np.int32)
+    n = len(di) - 1
+    nbytes = int(di[-1] - di[0])
+    ds = DeviceBuffer.from_array(seq if len(seq) else np.zeros(1, np.uint8), stream)
+    dd = DeviceBuffer.from_array(di, stream)
+    dc = DeviceBuffer(max(nbytes - n, 1))
+    do = DeviceBuffer((n + 1) * 8)
+    check(lib().fmi_fastq_encode_batch(ds.ptr, dd.ptr, n, nbytes, dc.ptr, do.ptr, stream))
+    sync(stream)
+    res = dc.download(np.uint8, nbytes - n, stream), do.download(np.uint64, n + 1, stream)
+    for b in (ds, dd, dc, do):
+        b.free()
+    return res
+
+
+def pack_codes(codes, stream=None):
+    """fmi_pack_codes: (packed uint32 words, N masks) on the host."""
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    nw = (len(codes) + 15) // 16
+    dc = DeviceBuffer.from_array(codes if len(codes) else np.zeros(1, np.uint8), stream)
+    dp = DeviceBuffer(max(nw, 1) * 4)
+    dm = DeviceBuffer(max(nw, 1) * 2)
+    check(lib().fmi_pack_codes(dc.ptr, len(codes), dp.ptr, dm.ptr, stream))
+    sync(stream)
+    res = dp.download(np.uint32, nw, stream), dm.download(np.uint16, nw, stream)
+    for b in (dc, dp, dm):
+        b.free()
+    return res
+
+
 def map_reads_batch(ix, batch, filter_read_mappings, cap, stream=None):
     """fmi_map_reads_batch (__bwt_map_inexact_read's alignments): (status[n],
     counts[n], alignments[n, cap]) on the host."""
@@ -1018,6 +1062,31 @@ class Server:
         while True:
             rc = lib().fmi_server_submit(self.h, _ptr(reads) if len(reads) else None,
                                          _ptr(offsets), n, C.byref(rid))
+            if rc != EAGAIN:
+                break
+            if not block:
+                return None
+            t0 = t0 or _t.perf_counter()
+            if _t.perf_counter() - t0 > max_wait_s:
+                raise FmiError(rc, "server queue stayed full")
+            _t.sleep(5e-5)
+        check(rc)
+        self._n[rid.value] = n
+        return rid.value
+
+    def submit_fastq(self, seq, data_indices, block=True, max_wait_s=60.0):
+        """A request in the reference's fastq_batch_t layout: seq bytes with a
+        NUL after each read, int32 data_indices[n + 1]."""
+        import time as _t
+        seq = np.ascontiguousarray(np.frombuffer(seq, np.uint8) if isinstance(seq, bytes) else seq,
+                                   dtype=np.uint8)
+        di = np.ascontiguousarray(data_indices, dtype=np.int32)
+        n = len(di) - 1
+        rid = C.c_uint64()
+        t0 = None
+        while True:
+            rc = lib().fmi_server_submit_fastq(self.h, _ptr(seq) if len(seq) else None, _ptr(di),
+                                               n, C.byref(rid))
             if rc != EAGAIN:
                 break
             if not block:

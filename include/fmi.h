@@ -388,6 +388,27 @@ int fmi_inverse_locate_batch(const fmi_index *ix, int forward, const uint64_t *d
  * leaves those codes unset, bwt_commons.c:60-91).  Host function. */
 int fmi_encode_bases(const char *src, uint64_t n, uint8_t *dst);
 
+/* A read batch in the reference's fastq_batch_t layout
+ * (bioformats/fastq/fastq_batch.h:24-34) on the device: d_seq holds the
+ * sequences one after another, each followed by a NUL, and
+ * d_data_indices[num_reads + 1] the start of each (read r has
+ * data_indices[r+1] - data_indices[r] - 1 bases; indices increase by at
+ * least one per read; the caller checks them).  Writes the engine's batch:
+ * d_codes[offsets[r] ..] the codes of encode_bases (bwt_commons.c:99-102,
+ * alphabet "ACGT" in either case; any other byte is N, 4) and
+ * d_offsets[num_reads + 1] (offsets[r] = data_indices[r] - data_indices[0]
+ * - r).  nbytes = data_indices[num_reads] - data_indices[0] (known to the
+ * host that built the batch). */
+int fmi_fastq_encode_batch(const char *d_seq, const int32_t *d_data_indices, uint32_t num_reads,
+                           uint64_t nbytes, uint8_t *d_codes, uint64_t *d_offsets, fmi_stream s);
+/* The codes of a batch packed 2 bits per base: base i of the concatenated
+ * codes at bits 2(i % 16) of d_packed[i / 16], and (d_nmask may be NULL)
+ * whether it is an N at bit i % 16 of d_nmask[i / 16] (an N packs as 0).
+ * A quarter of the bytes for transfer or storage; no reference counterpart
+ * (the reference keeps one byte per base, bwt_commons.c:99-102). */
+int fmi_pack_codes(const uint8_t *d_codes, uint64_t nbases, uint32_t *d_packed,
+                   uint16_t *d_nmask, fmi_stream s);
+
 /* Position resolution of bwt.c:2158-2166 for 1-error records: for each hit
  * (as stored by fmi_search1_batch: hits[(2*i+s)*cap ...], counts) and each
  * SA row j of [k, l] (at most row_cap rows per hit), key =
@@ -594,6 +615,11 @@ int fmi_server_create(const fmi_index *ix, const fmi_server_config *cfg, fmi_ser
  * staging has no room now (backpressure: submit again after a wait). */
 int fmi_server_submit(fmi_server *srv, const uint8_t *reads, const uint64_t *offsets,
                       uint32_t nreads, uint64_t *request_id);
+/* The same request from a batch in the reference's fastq_batch_t layout
+ * (see fmi_fastq_encode_batch; host pointers): seq and data_indices are
+ * copied once into the staging, and the launch encodes them on the device. */
+int fmi_server_submit_fastq(fmi_server *srv, const char *seq, const int32_t *data_indices,
+                            uint32_t num_reads, uint64_t *request_id);
 /* Cancels a request: one still queued is removed (it is never searched and
  * its id is released at once); one already on the GPU completes there and
  * its results are dropped; a finished one's results are released.  A

@@ -228,7 +228,7 @@ def test_mixed_budget0_planted(mixed, base):
     strand = mixed["truth"]["strand"][idx].astype(np.int64)
     o64 = os_.astype(np.int64)
     lens = np.diff(o64)
-    sel = np.random.default_rng(32).choice(len(idx), 50_000, replace=False)
+    sel = np.random.default_rng(32).choice(len(idx), 200_000, replace=False)
     sel.sort()
     clean = np.zeros(len(sel), bool)
     for j, i in enumerate(sel):
