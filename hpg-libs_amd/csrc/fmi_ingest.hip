@@ -26,8 +26,10 @@ __device__ __forceinline__ uint8_t encode_base(uint8_t c) {
   }
 }
 
-// One thread per byte of seq (nbytes = data_indices[nreads] -
-// data_indices[0]): the read it belongs to by binary search of
+// One thread per byte seq[data_indices[0] + p], p < nbytes = data_indices
+// [nreads] - data_indices[0] (data_indices index seq, as the reference's
+// batch->seq[batch->data_indices[r]]): the read it belongs to by binary
+// search of
 // data_indices; its NUL terminator is dropped, every other byte goes to
 // codes[base_off + data_indices[r] - data_indices[0] - r + j].  Threads
 // p <= nreads also write offsets[read_off + p] = data_indices[p] -
@@ -60,7 +62,7 @@ __global__ __launch_bounds__(256) void fastq_encode_kernel(const char *seq, cons
     if (at == (int64_t)di[lo + 1] - 1) continue;          // its NUL
     const int64_t o = at - d0 - (int64_t)lo;
     if (o < 0 || (uint64_t)o >= ncodes) continue;
-    codes[base_off + (uint64_t)o] = encode_base((uint8_t)seq[p]);
+    codes[base_off + (uint64_t)o] = encode_base((uint8_t)seq[at]);
   }
 }
 
