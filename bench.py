@@ -727,7 +727,10 @@ def search1_point(args, hpgfm, synth, preset, extras=True, cpu_baseline=True, se
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                          "traffic": traffic},
             "traffic_source": traffic_src,
-            "kernel": "fmi::search1_kernel<true, true, false, %s>" % ("true" if text_mode else "false"),
+            "kernel": ("fmi_search1_batch: fmi::read_records_kernel, then "
+                       "fmi::search1_kernel<true, true, false, %s, OWN, true> (reads staged in "
+                       "LDS), the unstaged kernel over the reads longer than 256 bases, "
+                       "fmi::text_rows_kernel" % ("true" if text_mode else "false")),
             "kernel_ms_mean": round(kernel_ms, 4),
             "alg_bytes_per_launch": alg_bytes,
             "lf_queries_per_launch": lf_steps,
@@ -762,6 +765,20 @@ def search1_point(args, hpgfm, synth, preset, extras=True, cpu_baseline=True, se
             except Exception as e:   # noqa: BLE001 - reported, the point still counts
                 res["random_access_probe"] = {"error": str(e)}
         if extras and not args.no_extras:
+            # the same call with every step reading its read bases from memory
+            # (fmi_set_staging(0)), for comparison with the staged default
+            hpgfm.set_staging(False)
+            try:
+                hpgfm.search1_launch(ix, batch, out, mode=mode)
+                hpgfm.sync()
+                with hpgfm.TimingRegion() as tu:
+                    for _ in range(3):
+                        hpgfm.search1_launch(ix, batch, out, mode=mode)
+                    hpgfm.sync()
+            finally:
+                hpgfm.set_staging(True)
+            res["unstaged_kernel_ms"] = round(tu.mean_ms, 4)
+            log(f"[{preset}] unstaged (read bases from memory at every step): {tu.mean_ms:.3f} ms")
             hpgfm.search1_launch(ix, batch, out, mode=mode, plain=True)
             hpgfm.sync()
             with hpgfm.TimingRegion() as tp:

@@ -56,6 +56,7 @@ struct Tuning {
   uint32_t chunk = kDefaultChunk;            // tasks a wave takes per atomic
   uint32_t refill_min = kDefaultRefill;      // idle lanes before a wave refills
   uint32_t loads = FMI_LOADS_AUTO;           // fmi_set_load_mode
+  bool staging = true;                       // fmi_set_staging
 };
 Tuning g_tuning;
 std::mutex g_mu;
@@ -275,6 +276,11 @@ int search1_common(const fmi_index *ix, int mode, int strand_mask, const uint8_t
     grid = (uint32_t)cus * (t.wg_per_cu ? t.wg_per_cu : kDefaultWgPerCu);
     HIP_TRY(fmi::launch_zero(w.queue, 1, st), "zero queue");
   }
+  void *staged = nullptr;
+  if (persistent && t.staging && nreads <= fmi::kStagedMaxTasks) {
+    rc = scratch_for(st, fmi::search1_staged_scratch_bytes(nreads), &staged);
+    if (rc) return rc;
+  }
   rc = begin_timed(st);
   if (rc) return rc;
   const bool own = t.loads == FMI_LOADS_OWN ||
@@ -282,7 +288,7 @@ int search1_common(const fmi_index *ix, int mode, int strand_mask, const uint8_t
   hipError_t e = fmi::launch_search1(ix->dev, mode == FMI_MODE_MISMATCH, persistent, false,
                                      (uint32_t)strand_mask, d_reads, d_offsets, nreads, cap,
                                      d_hits, d_counts, w.queue, w.stats, grid, t.chunk,
-                                     t.refill_min, own, st);
+                                     t.refill_min, own, st, staged);
   return end_timed(st, e, "search1 launch");
 }
 
@@ -1003,6 +1009,20 @@ int fmi_get_load_mode(uint32_t *mode) {
   if (!mode) return fail(FMI_EINVAL, "NULL argument");
   std::lock_guard<std::mutex> lk(g_mu);
   *mode = g_tuning.loads;
+  return FMI_OK;
+}
+
+int fmi_set_staging(int enable) {
+  if (enable != 0 && enable != 1) return fail(FMI_EINVAL, "staging must be 0 or 1");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuning.staging = enable != 0;
+  return FMI_OK;
+}
+
+int fmi_get_staging(int *enable) {
+  if (!enable) return fail(FMI_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  *enable = g_tuning.staging ? 1 : 0;
   return FMI_OK;
 }
 

@@ -16,11 +16,21 @@ hipError_t launch_exact(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32
 hipError_t launch_vector(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
                          const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
                          int64_t *vk, int64_t *vl, int16_t *last_good, hipStream_t s);
+// The persistent 1-error search.  With staged (device memory of
+// search1_staged_scratch_bytes(nreads) bytes; persistent, not count) the
+// reads are first packed into per-read records and every lane holds its
+// read in LDS while it searches it; reads longer than kStagedMaxLen bases
+// are searched afterwards by the unstaged kernel.  Records are the same
+// either way.
+constexpr uint32_t kStagedMaxLen = 256;
+constexpr uint32_t kStagedMaxTasks = 1u << 27;
+uint64_t search1_staged_scratch_bytes(uint32_t nreads);
 hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, bool count,
                           uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
                           uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
-                          uint32_t chunk, uint32_t refill_min, bool own_loads, hipStream_t s);
+                          uint32_t chunk, uint32_t refill_min, bool own_loads, hipStream_t s,
+                          void *staged = nullptr);
 hipError_t launch_zero(uint32_t *p, uint32_t n, hipStream_t s);
 // Byte fill of bytes bytes with v (a kernel, so a captured graph holds no
 // memset node).

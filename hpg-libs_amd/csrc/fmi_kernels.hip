@@ -184,7 +184,22 @@ struct S1Params {
                                    // [4] k-mer table lookups
   uint32_t ntasks, strand_mask, cap, chunk;
   uint32_t refill_min;             // idle lanes before a wave takes new tasks
+  const uint4 *rec;                // staged launch: the read records (kRecWords words each)
+  const uint32_t *list;            // when set: the tasks are list[0 .. *list_n)
+  const uint32_t *list_n;
 };
+
+// The staged launch keeps every lane's read in LDS for the whole task: a
+// streaming kernel first packs each read into a 128-byte record,
+//   word 0 length, 1 / 2 the k-mer table index of pass 0 / pass 1 (as
+//   task_info computes them), 3 nonzero for a read longer than
+//   kStagedMaxLen, 4..19 the 2-bit codes (base i at bits 2 (i % 16) of word
+//   4 + i / 16), 20..27 the N flags (base i at bit i % 32 of word 20 + i / 32),
+// and a lane copies its record's codes and flags into its LDS slot when it
+// takes the task, so that no step reads the read from memory again.  The
+// longer reads are listed and searched by the unstaged kernel afterwards.
+constexpr uint32_t kRecWords = 32;
+constexpr uint32_t kSlotWords = 24;   // 16 code words + 8 flag words per lane
 
 struct S1Lane {
   uint32_t phase, pass, strand, read, nout, comp, bi;
@@ -236,10 +251,115 @@ __device__ __forceinline__ void window16(const uint4 &c0, const uint4 &c1, uint3
   }
 }
 
-template <bool MM_ONLY, bool COUNT, bool TEXT, bool OWN>
+// k-mer table index of a pass's first k bases, with this bit set when it is
+// known (a table, at least k bases, no N among them).
+constexpr uint32_t kKmerReady = 0x80000000u;
+
+// The k-mer table index of both passes' first k bases of the read at
+// reads + off (pass 1: W[0], W[1], ...; pass 0: W[len-1], W[len-2], ...),
+// from two aligned 16-byte chunks at each end of the read, each holding one
+// of those bases; 0 where unknown.
+__device__ __forceinline__ void kmer_codes(const uint8_t *reads, uint64_t off, uint32_t len,
+                                           uint32_t K, uint32_t &kc0, uint32_t &kc1) {
+  kc0 = kc1 = 0u;
+  if (!K || len < K) return;
+  const uint8_t *a = reads + off, *z = reads + off + (len - 1u);
+  const uint32_t ma = (uint32_t)(reinterpret_cast<uintptr_t>(a) & 15u);
+  const uint32_t mz = (uint32_t)(reinterpret_cast<uintptr_t>(z) & 15u);
+  const uint8_t *a0 = a - ma, *z1 = z - mz;
+  const uint8_t *a1 = ma + K - 1u >= 16u ? a0 + 16 : a0;
+  const uint8_t *z0 = K - 1u > mz ? z1 - 16 : z1;
+  const uint4 ca0 = *reinterpret_cast<const uint4 *>(a0);
+  const uint4 ca1 = *reinterpret_cast<const uint4 *>(a1);
+  const uint4 cz0 = *reinterpret_cast<const uint4 *>(z0);
+  const uint4 cz1 = *reinterpret_cast<const uint4 *>(z1);
+  const uint32_t kmask = (1u << (2u * K)) - 1u, nmask = (1u << K) - 1u;
+  uint32_t w, nb;
+  window16(ca0, ca1, ma, true, w, nb);
+  kc1 = (nb & nmask) ? 0u : ((w & kmask) | kKmerReady);
+  window16(cz0, cz1, 16u + mz, false, w, nb);
+  kc0 = (nb & nmask) ? 0u : ((w & kmask) | kKmerReady);
+}
+
+// 16 read bytes as 2-bit codes (byte i at bits 2i) and N flags (bit i: the
+// byte is > 3).
+__device__ __forceinline__ void pack16(const uint4 &c, uint32_t &code, uint32_t &flag) {
+  const uint32_t v[4] = {c.x, c.y, c.z, c.w};
+  code = 0u;
+  flag = 0u;
+#pragma unroll
+  for (uint32_t j = 0; j < 4u; ++j) {
+    uint32_t t = v[j] & 0x03030303u;
+    t = (t | (t >> 6)) & 0x000F000Fu;
+    t = (t | (t >> 12)) & 0xFFu;
+    uint32_t z = (v[j] >> 2) & 0x3F3F3F3Fu;
+    z = z | (z >> 3);
+    z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
+    z = (z | (z >> 7)) & 0x00030003u;
+    z = (z | (z >> 14)) & 0xFu;
+    code |= t << (8u * j);
+    flag |= z << (4u * j);
+  }
+}
+
+struct RecParams {
+  const uint8_t *reads;
+  const uint64_t *offsets;
+  uint32_t ntasks, kmer_k;
+  uint32_t *rec;                   // kRecWords words per task
+  uint32_t *list, *list_n;         // the reads longer than kStagedMaxLen
+};
+
+// Eight lanes per read, each packing 32 bases (from the three aligned
+// 16-byte chunks that hold them, none past the chunk of the read's last
+// byte); the first also writes the record's header.
+__global__ __launch_bounds__(256) void read_records_kernel(RecParams P) {
+  const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t t64 = gid >> 3;
+  const uint32_t j = (uint32_t)(gid & 7u);
+  if (t64 >= P.ntasks) return;
+  const uint32_t t = (uint32_t)t64;
+  const uint64_t off = P.offsets[t];
+  const uint32_t len = (uint32_t)(P.offsets[t + 1] - off);
+  uint32_t *rec = P.rec + (size_t)t * kRecWords;
+  if (len > kStagedMaxLen) {
+    if (j == 0u) {
+      *reinterpret_cast<uint4 *>(rec) = make_uint4(len, 0u, 0u, 1u);
+      P.list[atomicAdd(P.list_n, 1u)] = t;
+    }
+    return;
+  }
+  if (j == 0u) {
+    uint32_t kc0, kc1;
+    kmer_codes(P.reads, off, len, P.kmer_k, kc0, kc1);
+    *reinterpret_cast<uint4 *>(rec) = make_uint4(len, kc0, kc1, 0u);
+  }
+  if (32u * j >= len) return;
+  const uint8_t *a = P.reads + off + 32u * j;
+  const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(a) & 15u);
+  const uint8_t *c = a - mis;
+  const uint8_t *z = P.reads + off + (len - 1u);
+  const uint8_t *zc = z - (reinterpret_cast<uintptr_t>(z) & 15u);   // chunk of the last byte
+  const uint4 c0 = *reinterpret_cast<const uint4 *>(c);
+  const uint4 c1 = c + 16 <= zc ? *reinterpret_cast<const uint4 *>(c + 16) : make_uint4(0, 0, 0, 0);
+  const uint4 c2 = c + 32 <= zc ? *reinterpret_cast<const uint4 *>(c + 32) : make_uint4(0, 0, 0, 0);
+  uint32_t k0, k1, k2, f0, f1, f2;
+  pack16(c0, k0, f0);
+  pack16(c1, k1, f1);
+  pack16(c2, k2, f2);
+  const uint64_t lo = ((uint64_t)k1 << 32) | k0;
+  const uint32_t sh = 2u * mis;
+  const uint64_t codes = sh ? (lo >> sh) | ((uint64_t)k2 << (64u - sh)) : lo;
+  const uint64_t fl = (uint64_t)f0 | ((uint64_t)f1 << 16) | ((uint64_t)f2 << 32);
+  *reinterpret_cast<uint2 *>(rec + 4 + 2 * j) = make_uint2((uint32_t)codes, (uint32_t)(codes >> 32));
+  rec[20 + j] = (uint32_t)(fl >> mis);
+}
+
+template <bool MM_ONLY, bool COUNT, bool TEXT, bool OWN, bool LDS = false>
 struct S1 {
   const S1Params &P;
   S1Lane &L;
+  uint32_t *slot;                  // staged launch: this lane's LDS words (stride 256)
 
   __device__ __forceinline__ int32_t step() const { return L.pass ? 1 : -1; }
   __device__ __forceinline__ bool in_read(int32_t p) const { return p >= 0 && p <= L.len - 1; }
@@ -397,34 +517,12 @@ struct S1 {
   }
 
   // What a lane needs to start a task: the read's offset and length and the
-  // k-mer table index of both passes' first k bases (pass 1: W[0], W[1],
-  // ...; pass 0: W[len-1], W[len-2], ...; kKReady unless there is no table,
-  // the read is shorter than k or an N is among them), from two aligned
-  // 16-byte chunks at each end of the read, each holding one of those bases.
+  // k-mer table index of both passes' first k bases (kmer_codes).
   __device__ __forceinline__ void task_info(uint32_t task, uint64_t &off, uint32_t &len,
                                             uint32_t &kc0, uint32_t &kc1) const {
     off = P.offsets[task];
     len = (uint32_t)(P.offsets[task + 1] - off);
-    kc0 = kc1 = 0u;
-    const uint32_t K = P.ix.kmer_k;
-    if (K && len >= K) {
-      const uint8_t *a = P.reads + off, *z = P.reads + off + (len - 1u);
-      const uint32_t ma = (uint32_t)(reinterpret_cast<uintptr_t>(a) & 15u);
-      const uint32_t mz = (uint32_t)(reinterpret_cast<uintptr_t>(z) & 15u);
-      const uint8_t *a0 = a - ma, *z1 = z - mz;
-      const uint8_t *a1 = ma + K - 1u >= 16u ? a0 + 16 : a0;
-      const uint8_t *z0 = K - 1u > mz ? z1 - 16 : z1;
-      const uint4 ca0 = *reinterpret_cast<const uint4 *>(a0);
-      const uint4 ca1 = *reinterpret_cast<const uint4 *>(a1);
-      const uint4 cz0 = *reinterpret_cast<const uint4 *>(z0);
-      const uint4 cz1 = *reinterpret_cast<const uint4 *>(z1);
-      const uint32_t kmask = (1u << (2u * K)) - 1u, nmask = (1u << K) - 1u;
-      uint32_t w, nb;
-      window16(ca0, ca1, ma, true, w, nb);
-      kc1 = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
-      window16(cz0, cz1, 16u + mz, false, w, nb);
-      kc0 = (nb & nmask) ? 0u : ((w & kmask) | kKReady);
-    }
+    kmer_codes(P.reads, off, len, P.ix.kmer_k, kc0, kc1);
   }
 
   __device__ __forceinline__ void start_with(uint32_t task, uint64_t off, uint32_t len,
@@ -437,6 +535,59 @@ struct S1 {
     const uint32_t strand = (P.strand_mask == 2u) ? 1u : 0u;
     if (P.strand_mask != 3u) P.counts[2 * L.read + (strand ^ 1u)] = 0;
     start_strand(strand);
+  }
+
+  // Staged launch: the task's codes and N flags, copied from its record into
+  // the lane's LDS slot (only the 16-byte chunks that hold its bases; the
+  // rest of the slot keeps an earlier read's words, which no step reads),
+  // then the task starts as start_with starts it.
+  __device__ __forceinline__ void stage_with(uint32_t task, uint32_t len, uint32_t kc0,
+                                             uint32_t kc1) {
+    const uint4 *r = P.rec + (size_t)task * (kRecWords / 4u);
+    const uint32_t nc = (len + 63u) >> 6, nf = (len + 127u) >> 7;
+    uint4 c0 = opaque4(), c1 = opaque4(), c2 = opaque4(), c3 = opaque4();
+    uint4 f0 = opaque4(), f1 = opaque4();
+    if (nc > 0u) c0 = r[1];
+    if (nc > 1u) c1 = r[2];
+    if (nc > 2u) c2 = r[3];
+    if (nc > 3u) c3 = r[4];
+    if (nf > 0u) f0 = r[5];
+    if (nf > 1u) f1 = r[6];
+    if (nc > 0u) { slot[0 * 256] = c0.x; slot[1 * 256] = c0.y; slot[2 * 256] = c0.z; slot[3 * 256] = c0.w; }
+    if (nc > 1u) { slot[4 * 256] = c1.x; slot[5 * 256] = c1.y; slot[6 * 256] = c1.z; slot[7 * 256] = c1.w; }
+    if (nc > 2u) { slot[8 * 256] = c2.x; slot[9 * 256] = c2.y; slot[10 * 256] = c2.z; slot[11 * 256] = c2.w; }
+    if (nc > 3u) { slot[12 * 256] = c3.x; slot[13 * 256] = c3.y; slot[14 * 256] = c3.z; slot[15 * 256] = c3.w; }
+    if (nf > 0u) { slot[16 * 256] = f0.x; slot[17 * 256] = f0.y; slot[18 * 256] = f0.z; slot[19 * 256] = f0.w; }
+    if (nf > 1u) { slot[20 * 256] = f1.x; slot[21 * 256] = f1.y; slot[22 * 256] = f1.z; slot[23 * 256] = f1.w; }
+    start_with(task, 0u, len, kc0, kc1);
+  }
+
+  // Staged launch: the 32 read bases from W[p] in feed order (rw, 2-bit
+  // physical symbols) and their N flags (rb, bit i), from the lane's LDS
+  // slot.  The window starts at base s = p (d > 0) or p - 31 (then it is
+  // reversed); word indexes outside the slot are clamped, which changes only
+  // bases outside the read, and no step compares those.
+  __device__ __forceinline__ void slot_window(int32_t p, int32_t d, uint64_t &rw,
+                                              uint32_t &rb) const {
+    const uint32_t u = (uint32_t)((d > 0 ? p : p - 31) + 32);   // s + 32: 1 .. 287
+    const int32_t cw = (int32_t)(u >> 4) - 2, fw = (int32_t)(u >> 5) - 1;
+    const uint32_t sh = (u & 15u) << 1, fs = u & 31u;
+    const uint32_t i0 = (uint32_t)min(max(cw, 0), 15), i1 = (uint32_t)min(max(cw + 1, 0), 15);
+    const uint32_t i2 = (uint32_t)min(cw + 2, 15);
+    const uint32_t j0 = 16u + (uint32_t)max(fw, 0), j1 = 16u + (uint32_t)min(fw + 1, 7);
+    const uint32_t w0 = slot[i0 * 256u], w1 = slot[i1 * 256u], w2 = slot[i2 * 256u];
+    const uint32_t f0 = slot[j0 * 256u], f1 = slot[j1 * 256u];
+    const uint64_t lo = ((uint64_t)w1 << 32) | w0;
+    const uint64_t win = sh ? (lo >> sh) | ((uint64_t)w2 << (64u - sh)) : lo;
+    const uint32_t fl = fs ? (f0 >> fs) | (f1 << (32u - fs)) : f0;
+    if (d > 0) {
+      rw = win;
+      rb = fl;
+    } else {
+      const uint64_t v = __builtin_bitreverse64(win);
+      rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+      rb = __builtin_bitreverse32(fl);
+    }
   }
 
   __device__ __forceinline__ void start(uint32_t task) {
@@ -553,7 +704,7 @@ struct S1 {
   // of those read bases (complemented on the - strand, as the LF steps take
   // them); an N among them empties the interval as its LF step would.  The
   // prepared entry index is kept in bi (unused in EXACT) with kKReady set.
-  static constexpr uint32_t kKReady = 0x80000000u;
+  static constexpr uint32_t kKReady = kKmerReady;
 
   __device__ __forceinline__ void advance(unsigned long long &nblk_acc, unsigned long long &nq_acc) {
     const bool cont = L.phase == PH_CONT;
@@ -615,13 +766,15 @@ struct S1 {
     // direction (x1, x2); a lane reads x1 / x2 only when they hold one of
     // its bases.  On the compact index no step compares more than a k-mer
     // (at most 15 bases), so x2 is never needed there.
-    const uint8_t *pa = L.W + p;
-    const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(pa) & 15u);
-    const uint8_t *pal = pa - mis;
+    // (The staged launch reads them from the lane's LDS slot instead.)
+    const uint8_t *pa = LDS ? nullptr : L.W + p;
+    const uint32_t mis = LDS ? 0u : (uint32_t)(reinterpret_cast<uintptr_t>(pa) & 15u);
+    const uint8_t *pal = LDS ? reinterpret_cast<const uint8_t *>(P.rec + (size_t)L.read * (kRecWords / 4u))
+                             : pa - mis;
     const uint8_t *x1 = d > 0 ? pal + 16 : pal - 16;
     const uint8_t *x2 = d > 0 ? pal + 32 : pal - 32;
-    const bool n1 = d > 0 ? mis + rlim - 1u >= 16u : rlim - 1u > mis;
-    const bool n2 = TEXT && (d > 0 ? mis + rlim - 1u >= 32u : rlim - 1u > mis + 16u);
+    const bool n1 = !LDS && (d > 0 ? mis + rlim - 1u >= 16u : rlim - 1u > mis);
+    const bool n2 = !LDS && TEXT && (d > 0 ? mis + rlim - 1u >= 32u : rlim - 1u > mis + 16u);
     // First line: the occupancy block of row k (four 16-byte loads), or the
     // aligned 16 bytes that hold the word(s) the kind reads (and the next 16
     // bytes when text words continue there).  Second line (the lanes whose
@@ -650,7 +803,8 @@ struct S1 {
     // chunk of W[p] or the first 16 bytes it reads (more L1 requests, fewer
     // instructions and branches — faster while the index's lines come from
     // the caches, slower once its accesses spread over tens of GB).
-    const uint4 rp = *reinterpret_cast<const uint4 *>(pal);
+    uint4 rp = opaque4();
+    if (!LDS) rp = *reinterpret_cast<const uint4 *>(pal);
     const uint4 fx0 = *reinterpret_cast<const uint4 *>(f0);
     uint4 fx1 = opaque4(), fx2 = opaque4(), fx3 = opaque4(), rx1 = opaque4(), rx2 = opaque4();
     uint4 gx0 = opaque4(), gx1 = opaque4(), gx2 = opaque4(), gx3 = opaque4();
@@ -666,8 +820,10 @@ struct S1 {
       fx1 = *reinterpret_cast<const uint4 *>(lf || next ? f0 + 16 : f0);
       fx2 = *reinterpret_cast<const uint4 *>(lf ? f0 + 32 : f0);
       fx3 = *reinterpret_cast<const uint4 *>(lf ? f0 + 48 : f0);
-      rx1 = *reinterpret_cast<const uint4 *>(n1 ? x1 : pal);
-      if (TEXT) rx2 = *reinterpret_cast<const uint4 *>(n2 ? x2 : pal);
+      if (!LDS) {
+        rx1 = *reinterpret_cast<const uint4 *>(n1 ? x1 : pal);
+        if (TEXT) rx2 = *reinterpret_cast<const uint4 *>(n2 ? x2 : pal);
+      }
     }
     if (two) {
       gx0 = *reinterpret_cast<const uint4 *>(g0);
@@ -685,7 +841,9 @@ struct S1 {
     // groups (rw, physical symbols) and their N flags (rb, bit i)
     uint64_t rw;
     uint32_t rb;
-    if (!TEXT) {
+    if (LDS) {
+      slot_window(p, d, rw, rb);
+    } else if (!TEXT) {
       uint32_t w32;
       window16(rc0, rc1, wp, d > 0, w32, rb);
       rw = w32;
@@ -859,14 +1017,18 @@ __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
-template <bool PERSISTENT, bool MM_ONLY, bool COUNT, bool TEXT, bool OWN>
-__global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
+template <bool PERSISTENT, bool MM_ONLY, bool COUNT, bool TEXT, bool OWN, bool LDS = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void search1_kernel(S1Params P) {
+  // staged launch: word j of thread t's slot at slots[j * 256 + t], so the
+  // lanes of a wavefront reading the same word hit different banks
+  __shared__ uint32_t slots[LDS ? kSlotWords * 256u : 1u];
   S1Lane L;
   L.phase = PH_IDLE;
   L.nlook = 0;
   L.ntext = 0;
   L.nkmer = 0;
-  S1<MM_ONLY, COUNT, TEXT, OWN> m{P, L};
+  S1<MM_ONLY, COUNT, TEXT, OWN, LDS> m{P, L, slots + threadIdx.x};
   unsigned long long nblk = 0, nq = 0;
   const uint32_t lane = lane_id();
 
@@ -885,10 +1047,13 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
     // wave is), so the task set-up runs for many lanes at a time.
     //
     // The next (up to) 64 tasks of the chunk are read ahead, one per lane
-    // (their offsets and k-mer indexes, task_info): lane i holds task
-    // wbase + i, and a lane that takes a task gets it from its holder with
-    // register shuffles, so a refill waits for no memory except once per 64
-    // tasks.
+    // (their offsets and k-mer indexes, task_info, or their record's header
+    // in the staged launch): lane i holds task wbase + i, and a lane that
+    // takes a task gets it from its holder with register shuffles, so a
+    // refill waits for no memory except once per 64 tasks (and, staged, for
+    // the records of the tasks it starts).  With a task list the queue runs
+    // over list positions.
+    const uint32_t ntasks = P.list ? *P.list_n : P.ntasks;
     uint32_t wq_next = 0, wq_end = 0, wbase = 0, wend = 0;
     uint64_t h_off = 0;
     uint32_t h_len = 0, h_kc0 = 0, h_kc1 = 0;
@@ -903,26 +1068,43 @@ __global__ __launch_bounds__(256) void search1_kernel(S1Params P) {
           uint32_t base = 0;
           if (lane == 0) base = atomicAdd(P.queue, P.chunk);
           base = __shfl(base, 0);
-          if (base >= P.ntasks) { more = false; break; }
+          if (base >= ntasks) { more = false; break; }
           wq_next = base;
-          wq_end = min(base + P.chunk, P.ntasks);
+          wq_end = min(base + P.chunk, ntasks);
         }
         if (wq_next >= wend) {       // read the next 64 tasks ahead
           wbase = wq_next;
           wend = min(wbase + 64u, wq_end);
-          if (wbase + lane < wend) m.task_info(wbase + lane, h_off, h_len, h_kc0, h_kc1);
+          if (wbase + lane < wend) {
+            const uint32_t h_task = P.list ? P.list[wbase + lane] : wbase + lane;
+            if (LDS) {
+              const uint4 h = P.rec[(size_t)h_task * (kRecWords / 4u)];
+              h_len = h.w ? 0xFFFFFFFFu : h.x;   // a long read: left to the unstaged kernel
+              h_kc0 = h.y;
+              h_kc1 = h.z;
+            } else {
+              m.task_info(h_task, h_off, h_len, h_kc0, h_kc1);
+            }
+          }
         }
         const uint32_t nidle = (uint32_t)__popcll(idle);
         const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
         const uint32_t avail = wend - wq_next;
         const int src = (int)((wq_next - wbase + rank) & 63u);   // the holder of task wq_next + rank
-        const uint32_t off_lo = __shfl((uint32_t)h_off, src);
-        const uint32_t off_hi = __shfl((uint32_t)(h_off >> 32), src);
+        uint32_t task = wq_next + rank;
+        if (P.list && rank < avail) task = P.list[task];
         const uint32_t len = __shfl(h_len, src);
         const uint32_t kc0 = __shfl(h_kc0, src);
         const uint32_t kc1 = __shfl(h_kc1, src);
-        if (L.phase == PH_IDLE && rank < avail)
-          m.start_with(wq_next + rank, ((uint64_t)off_hi << 32) | off_lo, len, kc0, kc1);
+        if (LDS) {
+          if (L.phase == PH_IDLE && rank < avail && len != 0xFFFFFFFFu)
+            m.stage_with(task, len, kc0, kc1);
+        } else {
+          const uint32_t off_lo = __shfl((uint32_t)h_off, src);
+          const uint32_t off_hi = __shfl((uint32_t)(h_off >> 32), src);
+          if (L.phase == PH_IDLE && rank < avail)
+            m.start_with(task, ((uint64_t)off_hi << 32) | off_lo, len, kc0, kc1);
+        }
         wq_next += min(avail, nidle);
         idle = __ballot(L.phase == PH_IDLE);
       }
@@ -1388,20 +1570,54 @@ hipError_t launch_vector(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint3
   return hipGetLastError();
 }
 
+uint64_t search1_staged_scratch_bytes(uint32_t nreads) {
+  return ((64ull + 4ull * nreads + 127ull) & ~127ull) + (uint64_t)kRecWords * 4ull * nreads;
+}
+
 hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, bool count,
                           uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,
                           uint32_t *queue, unsigned long long *stats, uint32_t grid_persistent,
-                          uint32_t chunk, uint32_t refill_min, bool own_loads, hipStream_t s) {
+                          uint32_t chunk, uint32_t refill_min, bool own_loads, hipStream_t s,
+                          void *staged) {
   const uint32_t ntasks = nreads;   // one task per read (both strands in turn)
   S1Params P{ix, reads, offsets, hits, counts, queue, stats, ntasks, strand_mask, cap, chunk,
-             refill_min};
+             refill_min, nullptr, nullptr, nullptr};
   if (ntasks == 0) return hipSuccess;
   const bool text = ix.text != nullptr;   // expanded index: text mode
   if (persistent) {
     if (grid_persistent == 0 || chunk == 0 || refill_min == 0 || refill_min > 64)
       return hipErrorInvalidValue;
     const dim3 g(grid_persistent), b(256);
+    if (staged && !count && ntasks <= kStagedMaxTasks) {
+      // scratch: [0] the unstaged kernel's work counter, [1] the list's
+      // length; the list from byte 64; the records from the next 128 bytes
+      uint32_t *head = static_cast<uint32_t *>(staged);
+      uint32_t *list = head + 16;
+      uint32_t *rec = reinterpret_cast<uint32_t *>(static_cast<char *>(staged) +
+                                                   ((64ull + 4ull * ntasks + 127ull) & ~127ull));
+      hipError_t e = launch_zero(head, 2, s);
+      if (e != hipSuccess) return e;
+      RecParams R{reads, offsets, ntasks, ix.kmer_k, rec, list, head + 1};
+      hipLaunchKernelGGL(read_records_kernel, dim3((uint32_t)((8ull * ntasks + 255) / 256)),
+                         dim3(256), 0, s, R);
+      P.rec = reinterpret_cast<const uint4 *>(rec);
+#define FMI_S1_STAGED(MM)                                                                      \
+  do {                                                                                         \
+    if (!text) hipLaunchKernelGGL((search1_kernel<true, MM, false, false, false, true>), g, b, 0, s, P); \
+    else if (own_loads)                                                                        \
+      hipLaunchKernelGGL((search1_kernel<true, MM, false, true, true, true>), g, b, 0, s, P);  \
+    else hipLaunchKernelGGL((search1_kernel<true, MM, false, true, false, true>), g, b, 0, s, P); \
+  } while (0)
+      if (mm_only) FMI_S1_STAGED(true);
+      else FMI_S1_STAGED(false);
+#undef FMI_S1_STAGED
+      // the reads longer than a slot, unstaged, from the list
+      P.rec = nullptr;
+      P.list = list;
+      P.list_n = head + 1;
+      P.queue = head;
+    }
 #define FMI_S1_LAUNCH(MM, CNT)                                                                \
   do {                                                                                        \
     if (!text) hipLaunchKernelGGL((search1_kernel<true, MM, CNT, false, false>), g, b, 0, s, P); \

@@ -51,7 +51,7 @@ EXPORTS = [
     "fmi_graph_launch", "fmi_graph_destroy", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
     "fmi_search1_anchors_batch", "fmi_search1_batch_noalloc",
-    "fmi_set_tuning", "fmi_get_tuning", "fmi_set_load_mode", "fmi_get_load_mode", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
+    "fmi_set_tuning", "fmi_get_tuning", "fmi_set_load_mode", "fmi_get_load_mode", "fmi_set_staging", "fmi_get_staging", "fmi_locate_batch", "fmi_locate_hits", "fmi_resolve_hits", "fmi_resolve_hits_text",
     "fmi_gather_bench", "fmi_gather_probe",
     "fmi_search1_count_blocks", "fmi_timing_begin", "fmi_timing_end",
     "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_map_reads_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
@@ -145,6 +145,8 @@ def _bind(lib):
         "fmi_get_tuning": ([C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)], _i),
         "fmi_set_load_mode": ([_u32], _i),
         "fmi_get_load_mode": ([C.POINTER(_u32)], _i),
+        "fmi_set_staging": ([_i], _i),
+        "fmi_get_staging": ([C.POINTER(_i)], _i),
         "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_locate_hits": ([_p, _i, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
@@ -398,6 +400,18 @@ def get_load_mode():
     m = C.c_uint32()
     check(lib().fmi_get_load_mode(C.byref(m)))
     return int(m.value)
+
+
+def set_staging(enable):
+    """fmi_set_staging: 1 (the default) holds each lane's read in LDS during
+    the persistent 1-error search, 0 reads it from memory at every step."""
+    check(lib().fmi_set_staging(1 if enable else 0))
+
+
+def get_staging():
+    v = C.c_int()
+    check(lib().fmi_get_staging(C.byref(v)))
+    return bool(v.value)
 
 
 class Index:
