@@ -587,7 +587,7 @@ def choose_tuning(args, hpgfm, ix, g, synth, key, mode, nreads):
     if args.no_autotune:
         return None
     t0 = time.time()
-    n = min(nreads, 1_000_000)
+    n = nreads   # the timed batch's size: the best launch shape depends on it
     treads, toffs, _ = synth.reads(g, n, length=args.read_len, seed=9, sub_rate=args.sub_rate)
     tb = hpgfm.ReadBatch(treads, toffs)
     tout = hpgfm.Search1Output(n, args.cap)

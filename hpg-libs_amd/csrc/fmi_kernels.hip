@@ -477,34 +477,44 @@ struct S1 {
     L.phase = PH_SCAN;
   }
 
-  // Next branch of the current node, from L.bi on (0: deletion; 1 + 2b:
-  // insertion of b; 2 + 2b: mismatch to b).  Branches whose continuation is
-  // already past the end of the read succeed at once.  In text mode the
-  // node's children are its suffix extended by one text base.
+  // The node's next branch from L.bi on (0: deletion; 1 + 2b: insertion of
+  // b; 2 + 2b: mismatch to b), in that order: the branches that exist form
+  // a 9-bit mask and the loop visits only those.  A branch whose
+  // continuation is already past the end of the read succeeds at once (its
+  // record is stored and the next branch is taken); otherwise the lane
+  // continues it (PH_CONT).  With no branch left the scan moves on.  In text
+  // mode the node's children are its suffix extended by one text base.
   __device__ __forceinline__ void branch_next() {
-    while (L.bi < 9) {
-      const uint32_t bi = L.bi;
-      bool take = false;
-      if (bi == 0) {
-        if (!MM_ONLY) { L.qk = L.k; L.ql = L.l; L.cq = L.tq; L.cp = L.i + step(); take = true; }
-      } else {
-        const uint32_t b = (bi - 1) >> 1;
-        const uint32_t ps = L.comp ? 3u - b : b;
-        const uint32_t cb = pick4(L.ck, ps), lb = pick4(L.cl, ps);
-        const bool nonempty = cb <= lb;
-        const uint32_t cq = L.tq == kTqNone ? kTqNone : L.tq - 1u;
-        if (bi & 1u) {
-          if (!MM_ONLY && nonempty) { L.qk = cb; L.ql = lb; L.cq = cq; L.cp = L.i; take = true; }
-        } else if (nonempty && b != L.wsym) {
-          L.qk = cb; L.ql = lb; L.cq = cq; L.cp = L.i + step(); take = true;
-        }
-      }
-      if (take) {
-        if (in_read(L.cp)) { L.phase = PH_CONT; return; }
-        emit_branch();
-      }
-      L.bi++;
+    uint32_t ok = MM_ONLY ? 0u : 1u;
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; ++b) {
+      const uint32_t ps = L.comp ? 3u - b : b;
+      const bool nonempty = pick4(L.ck, ps) <= pick4(L.cl, ps);
+      if (nonempty && !MM_ONLY) ok |= 1u << (1u + 2u * b);
+      if (nonempty && b != L.wsym) ok |= 1u << (2u + 2u * b);
     }
+    ok &= ~0u << min(L.bi, 31u);
+    while (ok) {
+      const uint32_t bi = (uint32_t)__builtin_ctz(ok);
+      ok &= ok - 1u;
+      L.bi = bi;
+      if (bi == 0u) {
+        L.qk = L.k; L.ql = L.l; L.cq = L.tq; L.cp = L.i + step();
+      } else {
+        const uint32_t b = (bi - 1u) >> 1;
+        const uint32_t ps = L.comp ? 3u - b : b;
+        L.qk = pick4(L.ck, ps);
+        L.ql = pick4(L.cl, ps);
+        L.cq = L.tq == kTqNone ? kTqNone : L.tq - 1u;
+        L.cp = (bi & 1u) ? L.i : L.i + step();
+      }
+      if (in_read(L.cp)) {
+        L.phase = PH_CONT;
+        return;
+      }
+      emit_branch();
+    }
+    L.bi = 9u;
     L.i += step();
     scan_next();
   }
@@ -889,6 +899,13 @@ struct S1 {
     }
     const uint32_t r0 = (rb & 1u) ? 4u : (uint32_t)(rw & 3u);   // W[p] (4 for any N)
 
+    // Every load of the batch is waited for here, also those whose value
+    // this lane's transition does not use: a load left outstanding at the end
+    // of the iteration would make the next iteration wait, before it issues
+    // its batch, for everything issued since, the records stored included.
+    asm volatile("" ::"v"(rp.x), "v"(rx1.x), "v"(rx2.x), "v"(fx0.x), "v"(fx1.x), "v"(fx2.x),
+                 "v"(fx3.x), "v"(gx0.x), "v"(gx1.x), "v"(gx2.x), "v"(gx3.x));
+
     // ---- the transitions
     if (kind == F_TEXT) {
       const uint32_t sel = (uint32_t)(reinterpret_cast<uintptr_t>(a0) & 15u) >> 2;
@@ -1065,8 +1082,13 @@ void search1_kernel(S1Params P) {
       if (nidle0 < P.refill_min && nidle0 < 64u) idle = 0ull;
       while (idle != 0ull && more) {
         if (wq_next >= wq_end) {
+          // a drained queue is read, not incremented, so the waves that
+          // finish together do not contend for the counter
           uint32_t base = 0;
-          if (lane == 0) base = atomicAdd(P.queue, P.chunk);
+          if (lane == 0) {
+            base = __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (base < ntasks) base = atomicAdd(P.queue, P.chunk);
+          }
           base = __shfl(base, 0);
           if (base >= ntasks) { more = false; break; }
           wq_next = base;

@@ -22,7 +22,7 @@ import hpgfm
 DEFAULT_GRID = {
     "wg_per_cu": (2, 4, 8, 16),
     "chunk": (16, 64, 256),
-    "refill_min": (8, 16, 32, 48, 64),
+    "refill_min": (1, 4, 8, 16, 32, 48, 64),
     "loads": (hpgfm.LOADS_ALL, hpgfm.LOADS_OWN),
 }
 
