@@ -1071,6 +1071,7 @@ void search1_kernel(S1Params P) {
     // the records of the tasks it starts).  With a task list the queue runs
     // over list positions.
     const uint32_t ntasks = P.list ? *P.list_n : P.ntasks;
+    if (ntasks == 0u) return;   // an empty list: no wave touches the work counter
     uint32_t wq_next = 0, wq_end = 0, wbase = 0, wend = 0;
     uint64_t h_off = 0;
     uint32_t h_len = 0, h_kc0 = 0, h_kc1 = 0;
@@ -1082,13 +1083,8 @@ void search1_kernel(S1Params P) {
       if (nidle0 < P.refill_min && nidle0 < 64u) idle = 0ull;
       while (idle != 0ull && more) {
         if (wq_next >= wq_end) {
-          // a drained queue is read, not incremented, so the waves that
-          // finish together do not contend for the counter
           uint32_t base = 0;
-          if (lane == 0) {
-            base = __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (base < ntasks) base = atomicAdd(P.queue, P.chunk);
-          }
+          if (lane == 0) base = atomicAdd(P.queue, P.chunk);
           base = __shfl(base, 0);
           if (base >= ntasks) { more = false; break; }
           wq_next = base;
