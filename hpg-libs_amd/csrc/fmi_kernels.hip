@@ -190,16 +190,14 @@ struct S1Params {
 };
 
 // The staged launch keeps every lane's read in LDS for the whole task: a
-// streaming kernel first packs each read into a 128-byte record,
-//   word 0 length, 1 / 2 the k-mer table index of pass 0 / pass 1 (as
-//   task_info computes them), 3 nonzero for a read longer than
-//   kStagedMaxLen, 4..19 the 2-bit codes (base i at bits 2 (i % 16) of word
-//   4 + i / 16), 20..27 the N flags (base i at bit i % 32 of word 20 + i / 32),
-// and a lane copies its record's codes and flags into its LDS slot when it
-// takes the task, so that no step reads the read from memory again.  The
-// longer reads are listed and searched by the unstaged kernel afterwards.
-constexpr uint32_t kRecWords = 32;
+// streaming kernel first packs each read into a 96-byte record, words 0..15
+// the 2-bit codes (base i at bits 2 (i % 16) of word i / 16), 16..23 the N
+// flags (base i at bit i % 32 of word 16 + i / 32), and a lane copies its
+// read's record into its LDS slot (the same layout) when it takes the task,
+// so that no step reads the read from memory again.  The reads longer than
+// kStagedMaxLen are listed and searched by the unstaged kernel afterwards.
 constexpr uint32_t kSlotWords = 24;   // 16 code words + 8 flag words per lane
+constexpr uint32_t kRecWords = kSlotWords;
 
 struct S1Lane {
   uint32_t phase, pass, strand, read, nout, comp, bi;
@@ -305,14 +303,14 @@ __device__ __forceinline__ void pack16(const uint4 &c, uint32_t &code, uint32_t 
 struct RecParams {
   const uint8_t *reads;
   const uint64_t *offsets;
-  uint32_t ntasks, kmer_k;
+  uint32_t ntasks;
   uint32_t *rec;                   // kRecWords words per task
   uint32_t *list, *list_n;         // the reads longer than kStagedMaxLen
 };
 
 // Eight lanes per read, each packing 32 bases (from the three aligned
 // 16-byte chunks that hold them, none past the chunk of the read's last
-// byte); the first also writes the record's header.
+// byte); the first lane of a longer read lists it instead.
 __global__ __launch_bounds__(256) void read_records_kernel(RecParams P) {
   const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   const uint64_t t64 = gid >> 3;
@@ -323,16 +321,8 @@ __global__ __launch_bounds__(256) void read_records_kernel(RecParams P) {
   const uint32_t len = (uint32_t)(P.offsets[t + 1] - off);
   uint32_t *rec = P.rec + (size_t)t * kRecWords;
   if (len > kStagedMaxLen) {
-    if (j == 0u) {
-      *reinterpret_cast<uint4 *>(rec) = make_uint4(len, 0u, 0u, 1u);
-      P.list[atomicAdd(P.list_n, 1u)] = t;
-    }
+    if (j == 0u) P.list[atomicAdd(P.list_n, 1u)] = t;
     return;
-  }
-  if (j == 0u) {
-    uint32_t kc0, kc1;
-    kmer_codes(P.reads, off, len, P.kmer_k, kc0, kc1);
-    *reinterpret_cast<uint4 *>(rec) = make_uint4(len, kc0, kc1, 0u);
   }
   if (32u * j >= len) return;
   const uint8_t *a = P.reads + off + 32u * j;
@@ -351,8 +341,8 @@ __global__ __launch_bounds__(256) void read_records_kernel(RecParams P) {
   const uint32_t sh = 2u * mis;
   const uint64_t codes = sh ? (lo >> sh) | ((uint64_t)k2 << (64u - sh)) : lo;
   const uint64_t fl = (uint64_t)f0 | ((uint64_t)f1 << 16) | ((uint64_t)f2 << 32);
-  *reinterpret_cast<uint2 *>(rec + 4 + 2 * j) = make_uint2((uint32_t)codes, (uint32_t)(codes >> 32));
-  rec[20 + j] = (uint32_t)(fl >> mis);
+  *reinterpret_cast<uint2 *>(rec + 2 * j) = make_uint2((uint32_t)codes, (uint32_t)(codes >> 32));
+  rec[16 + j] = (uint32_t)(fl >> mis);
 }
 
 template <bool MM_ONLY, bool COUNT, bool TEXT, bool OWN, bool LDS = false>
@@ -547,28 +537,40 @@ struct S1 {
     start_strand(strand);
   }
 
-  // Staged launch: the task's codes and N flags, copied from its record into
+  // Staged launch: the read's codes and N flags, copied from its record into
   // the lane's LDS slot (only the 16-byte chunks that hold its bases; the
-  // rest of the slot keeps an earlier read's words, which no step reads),
-  // then the task starts as start_with starts it.
-  __device__ __forceinline__ void stage_with(uint32_t task, uint32_t len, uint32_t kc0,
-                                             uint32_t kc1) {
+  // rest of the slot keeps an earlier read's words, which no step reads);
+  // then the k-mer table index of both passes' first k bases is taken from
+  // the slot (as kmer_codes takes it from the read) and the task starts as
+  // start_with starts it.
+  __device__ __forceinline__ void stage_with(uint32_t task, uint32_t len) {
     const uint4 *r = P.rec + (size_t)task * (kRecWords / 4u);
     const uint32_t nc = (len + 63u) >> 6, nf = (len + 127u) >> 7;
     uint4 c0 = opaque4(), c1 = opaque4(), c2 = opaque4(), c3 = opaque4();
     uint4 f0 = opaque4(), f1 = opaque4();
-    if (nc > 0u) c0 = r[1];
-    if (nc > 1u) c1 = r[2];
-    if (nc > 2u) c2 = r[3];
-    if (nc > 3u) c3 = r[4];
-    if (nf > 0u) f0 = r[5];
-    if (nf > 1u) f1 = r[6];
+    if (nc > 0u) c0 = r[0];
+    if (nc > 1u) c1 = r[1];
+    if (nc > 2u) c2 = r[2];
+    if (nc > 3u) c3 = r[3];
+    if (nf > 0u) f0 = r[4];
+    if (nf > 1u) f1 = r[5];
     if (nc > 0u) { slot[0 * 256] = c0.x; slot[1 * 256] = c0.y; slot[2 * 256] = c0.z; slot[3 * 256] = c0.w; }
     if (nc > 1u) { slot[4 * 256] = c1.x; slot[5 * 256] = c1.y; slot[6 * 256] = c1.z; slot[7 * 256] = c1.w; }
     if (nc > 2u) { slot[8 * 256] = c2.x; slot[9 * 256] = c2.y; slot[10 * 256] = c2.z; slot[11 * 256] = c2.w; }
     if (nc > 3u) { slot[12 * 256] = c3.x; slot[13 * 256] = c3.y; slot[14 * 256] = c3.z; slot[15 * 256] = c3.w; }
     if (nf > 0u) { slot[16 * 256] = f0.x; slot[17 * 256] = f0.y; slot[18 * 256] = f0.z; slot[19 * 256] = f0.w; }
     if (nf > 1u) { slot[20 * 256] = f1.x; slot[21 * 256] = f1.y; slot[22 * 256] = f1.z; slot[23 * 256] = f1.w; }
+    uint32_t kc0 = 0u, kc1 = 0u;
+    const uint32_t K = P.ix.kmer_k;
+    if (K && len >= K) {
+      const uint32_t kmask = (1u << (2u * K)) - 1u, nmask = (1u << K) - 1u;
+      uint64_t w;
+      uint32_t nb;
+      slot_window(0, 1, w, nb);                     // W[0], W[1], ...
+      kc1 = (nb & nmask) ? 0u : (((uint32_t)w & kmask) | kKmerReady);
+      slot_window((int32_t)len - 1, -1, w, nb);     // W[len-1], W[len-2], ...
+      kc0 = (nb & nmask) ? 0u : (((uint32_t)w & kmask) | kKmerReady);
+    }
     start_with(task, 0u, len, kc0, kc1);
   }
 
@@ -1096,10 +1098,8 @@ void search1_kernel(S1Params P) {
           if (wbase + lane < wend) {
             const uint32_t h_task = P.list ? P.list[wbase + lane] : wbase + lane;
             if (LDS) {
-              const uint4 h = P.rec[(size_t)h_task * (kRecWords / 4u)];
-              h_len = h.w ? 0xFFFFFFFFu : h.x;   // a long read: left to the unstaged kernel
-              h_kc0 = h.y;
-              h_kc1 = h.z;
+              const uint32_t len = (uint32_t)(P.offsets[h_task + 1] - P.offsets[h_task]);
+              h_len = len > kStagedMaxLen ? 0xFFFFFFFFu : len;   // longer: the unstaged kernel's
             } else {
               m.task_info(h_task, h_off, h_len, h_kc0, h_kc1);
             }
@@ -1112,12 +1112,11 @@ void search1_kernel(S1Params P) {
         uint32_t task = wq_next + rank;
         if (P.list && rank < avail) task = P.list[task];
         const uint32_t len = __shfl(h_len, src);
-        const uint32_t kc0 = __shfl(h_kc0, src);
-        const uint32_t kc1 = __shfl(h_kc1, src);
         if (LDS) {
-          if (L.phase == PH_IDLE && rank < avail && len != 0xFFFFFFFFu)
-            m.stage_with(task, len, kc0, kc1);
+          if (L.phase == PH_IDLE && rank < avail && len != 0xFFFFFFFFu) m.stage_with(task, len);
         } else {
+          const uint32_t kc0 = __shfl(h_kc0, src);
+          const uint32_t kc1 = __shfl(h_kc1, src);
           const uint32_t off_lo = __shfl((uint32_t)h_off, src);
           const uint32_t off_hi = __shfl((uint32_t)(h_off >> 32), src);
           if (L.phase == PH_IDLE && rank < avail)
@@ -1616,7 +1615,7 @@ hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, boo
                                                    ((64ull + 4ull * ntasks + 127ull) & ~127ull));
       hipError_t e = launch_zero(head, 2, s);
       if (e != hipSuccess) return e;
-      RecParams R{reads, offsets, ntasks, ix.kmer_k, rec, list, head + 1};
+      RecParams R{reads, offsets, ntasks, rec, list, head + 1};
       hipLaunchKernelGGL(read_records_kernel, dim3((uint32_t)((8ull * ntasks + 255) / 256)),
                          dim3(256), 0, s, R);
       P.rec = reinterpret_cast<const uint4 *>(rec);

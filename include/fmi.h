@@ -359,8 +359,8 @@ int fmi_get_tuning(uint32_t *wg_per_cu, uint32_t *chunk, uint32_t *refill_min);
 int fmi_set_load_mode(uint32_t mode);
 int fmi_get_load_mode(uint32_t *mode);
 /* Whether fmi_search1_batch stages reads (the default, 1): a streaming
- * kernel packs each read into a 128-byte record (2-bit codes, N flags) in
- * the stream's scratch (132 bytes per read, for batches of at most 2^27
+ * kernel packs each read into a 96-byte record (2-bit codes, N flags) in
+ * the stream's scratch (100 bytes per read, for batches of at most 2^27
  * reads) and every lane of the persistent search holds its read in LDS, so
  * no step reads it from memory again; reads longer than 256 bases are
  * searched afterwards without staging.  0 reads the read from memory at
