@@ -513,8 +513,13 @@ def summarize_pmc(fetch_dir, write_dir, out_path, tuning, workload):
     def pick(acc, word):
         return [v for (k, _), v in sorted(acc.items(), key=lambda x: x[0][1]) if word in k]
 
-    s_f = pick(fetch, "search1_kernel<true, true, false")
-    s_w = pick(write, "search1_kernel<true, true, false")
+    # the staged search kernel of the mismatch-mode text-mode search (the
+    # last template argument, LDS staging, true), the call's dominant kernel
+    def staged(acc):
+        return [v for (k, _), v in sorted(acc.items(), key=lambda x: x[0][1])
+                if "search1_kernel<true, true, false, true," in k and k.split(">")[0].endswith("true")]
+    s_f = staged(fetch)
+    s_w = staged(write)
     c_f, c_w = pick(fetch, "copy_kernel"), pick(write, "copy_kernel")
     g_f = pick(fetch, "gather_kernel<4>")
     copy_bytes = 1 << 30
@@ -706,6 +711,17 @@ def search1_point(args, hpgfm, synth, preset, extras=True, cpu_baseline=True, se
         log(f"[{preset}] mismatch mode: {ms_per_step:.3f} ms/step wall, kernel {kernel_ms:.3f} ms "
             f"(min {tr.min_ms:.3f}, max {tr.max_ms:.3f}), {reads_per_s / 1e6:.2f} M reads/s, "
             f"{achieved:.0f} GB/s algorithmic")
+        if args.calibrate:
+            # kernels of known traffic for the FETCH_SIZE / WRITE_SIZE scales
+            # (summarize_pmc): a 1 GiB streaming copy, and 2^24 random lines
+            # of 64 and 128 bytes over a buffer of the index's random-access
+            # footprint and over 8 GiB
+            hpgfm.copy_bench(1 << 30, reps=1)
+            fb = random_footprint(ix)["total"]
+            for nb in (max(64 << 20, fb), 8 << 30):
+                for lb in (64, 128):
+                    hpgfm.gather_bench(nb, lb, 1 << 24)
+            hpgfm.sync()
         traffic, traffic_src = measured_traffic(preset, nreads, args.read_len, text_mode, kmer,
                                                 tuning)
         foot = random_footprint(ix)
@@ -903,7 +919,25 @@ def serve_point(args, hpgfm, ix, g, seconds, mode, preset):
     served = {"index": f"the {preset} index, resident", "requests": "Poisson arrivals, "
               "U[1, 50000] reads of U[50, 250] bp, 1% substitutions, <=1 mismatch, both strands",
               "runs": []}
-    srv = hpgfm.Server(ix, budget_bytes=2 << 30, max_inflight=4, max_batch_reads=1 << 20,
+    # the kernel alone on 1M reads of the same pool (lengths 50-250 bp): the
+    # rate the served figures are a fraction of
+    reads, offs = pool
+    pb = hpgfm.ReadBatch(reads, offs)
+    po = hpgfm.Search1Output(pb.nreads, args.cap)
+    try:
+        hpgfm.search1_launch(ix, pb, po, mode=mode)
+        hpgfm.sync()
+        with hpgfm.TimingRegion() as tk:
+            for _ in range(3):
+                hpgfm.search1_launch(ix, pb, po, mode=mode)
+            hpgfm.sync()
+    finally:
+        po.free()
+        pb.free()
+    served["kernel_reads_per_s_on_pool"] = round(pb.nreads / (tk.mean_ms / 1000.0))
+    log(f"[{preset}] the kernel alone on the pool's 1M reads: {tk.mean_ms:.3f} ms, "
+        f"{pb.nreads / tk.mean_ms / 1e3:.1f} M reads/s")
+    srv = hpgfm.Server(ix, budget_bytes=16 << 30, max_inflight=4, max_batch_reads=1 << 20,
                        cap=args.cap, mode=mode)
     try:
         for rate in (20, 200, 2000):
@@ -912,11 +946,15 @@ def serve_point(args, hpgfm, ix, g, seconds, mode, preset):
             served["runs"].append(r)
             log(f"[{preset}] served at {rate} req/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
                 f"{r['reads_per_s'] / 1e6:.2f} M reads/s over {r['requests']} requests")
-        sat = serve.run_closed(srv, pool, nrequests=400, window=16)
+        sat = serve.run_closed(srv, pool, nrequests=3000, window=64, clients=8)
         sat.pop("server", None)
+        sat["fraction_of_kernel_rate_on_pool"] = round(
+            sat["reads_per_s"] / served["kernel_reads_per_s_on_pool"], 3)
         served["saturated"] = sat
-        log(f"[{preset}] served closed loop (16 in flight): {sat['reads_per_s'] / 1e6:.2f} M "
-            f"reads/s, p50 {sat['p50_ms']} ms, p99 {sat['p99_ms']} ms")
+        log(f"[{preset}] served closed loop (64 in flight, 8 clients): "
+            f"{sat['reads_per_s'] / 1e6:.2f} M reads/s "
+            f"({sat['fraction_of_kernel_rate_on_pool']:.2f} of the kernel's rate on the pool), "
+            f"p50 {sat['p50_ms']} ms, p99 {sat['p99_ms']} ms")
         served["server_stats"] = srv.stats()
     finally:
         srv.close()

@@ -54,6 +54,14 @@ inline uint64_t align_up(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); 
 constexpr uint32_t kCopyWorkers = 2;
 constexpr uint32_t kDefaultMaxPending = 4096;
 
+// A launch's region of the arena (device memory and its pinned host
+// mirror), held until every request of the launch has fetched its results
+// from the mirror or been cancelled.
+struct Region {
+  uint64_t off = 0, size = 0;
+  uint32_t refs = 0;
+};
+
 struct Request {
   uint64_t id = 0;
   uint32_t nreads = 0;
@@ -62,8 +70,12 @@ struct Request {
   uint64_t raw = 0;                    // fastq: seq bytes (bases + nreads NULs)
   uint64_t in_off = 0, in_bytes = 0;   // staging: reads, then nreads + 1 offsets
   bool staged = false;                 // holds input staging
-  std::vector<uint32_t> counts;        // results (2 * nreads)
-  std::vector<fmi_hit> records;        // results (dense)
+  // results, in the launch's output mirror: 2 * nreads counts and nrec
+  // dense records, valid while the request holds its region
+  std::shared_ptr<Region> region;
+  const uint32_t *counts = nullptr;
+  const fmi_hit *records = nullptr;
+  uint64_t nrec = 0;
   Clock::time_point submit, done;
   bool finished = false;
   bool cancelled = false;
@@ -295,45 +307,68 @@ int fmi_server::fetch_records(Launch &L) {
   return FMI_OK;
 }
 
-// Copies each request's slice of the launch's output mirror into it (on a
-// copy worker), then releases the launch's region and stream slot.
+// Points each request at its slice of the launch's output mirror (on a copy
+// worker: the record counts are summed per request) and frees the stream
+// slot; the launch's region is released by the last fetch or cancel of its
+// requests (at once when none of them will fetch).
 void fmi_server::hand_out(Launch L, int status) {
+  auto region = std::make_shared<Region>();
+  region->off = L.off;
+  region->size = L.lay.size;
+  std::vector<uint64_t> nrec(L.reqs.size(), 0);
+  const uint32_t *cnt = reinterpret_cast<const uint32_t *>(h_mirror + L.off + L.lay.o_counts);
+  const fmi_hit *rec = reinterpret_cast<const fmi_hit *>(h_mirror + L.off + L.lay.o_compact);
+  std::vector<uint64_t> first_rec(L.reqs.size(), 0);
   if (status == FMI_OK) {
-    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(h_mirror + L.off + L.lay.o_counts);
-    const fmi_hit *rec = reinterpret_cast<const fmi_hit *>(h_mirror + L.off + L.lay.o_compact);
     uint64_t o = 0;
     for (size_t q = 0; q < L.reqs.size(); ++q) {
-      Request &r = *L.reqs[q];
-      const uint64_t s0 = 2ull * L.first_read[q], ns = 2ull * r.nreads;
+      const uint64_t s0 = 2ull * L.first_read[q], ns = 2ull * L.reqs[q]->nreads;
       uint64_t m = 0;
       for (uint64_t i = 0; i < ns; ++i) m += std::min<uint32_t>(cnt[s0 + i], cfg.cap);
-      if (!r.cancelled) {
-        r.counts.assign(cnt + s0, cnt + s0 + ns);
-        r.records.assign(rec + o, rec + o + m);
-      }
+      first_rec[q] = o;
+      nrec[q] = m;
       o += m;
     }
   }
   const auto now = Clock::now();
   std::lock_guard<std::mutex> lk(mu);
-  for (auto &rp : L.reqs) {
-    if (rp->staged) {
-      in_arena.release(rp->in_off, rp->in_bytes);
-      rp->staged = false;
+  for (size_t q = 0; q < L.reqs.size(); ++q) {
+    Request &r = *L.reqs[q];
+    if (r.staged) {
+      in_arena.release(r.in_off, r.in_bytes);
+      r.staged = false;
     }
-    rp->done = now;
-    rp->finished = true;
-    rp->status = rp->cancelled ? FMI_ECANCELED : status;
-    if (status == FMI_OK && !rp->cancelled) {
+    r.done = now;
+    r.finished = true;
+    r.status = r.cancelled ? FMI_ECANCELED : status;
+    if (status == FMI_OK && !r.cancelled) {
       stats.requests_done++;
-      stats.reads_done += rp->nreads;
+      stats.reads_done += r.nreads;
+      r.region = region;
+      r.counts = cnt + 2ull * L.first_read[q];
+      r.records = rec + first_rec[q];
+      r.nrec = nrec[q];
+      region->refs++;
     }
-    if (rp->cancelled) live.erase(rp->id);
+    if (r.cancelled) live.erase(r.id);
   }
-  arena.release(L.off, L.lay.size);
+  if (region->refs == 0) arena.release(region->off, region->size);
   slot_busy[L.slot] = false;
   cv_done.notify_all();
   cv_submit.notify_all();   // the scheduler may admit into the freed memory
+}
+
+// Drops request r's hold on its launch's region (mu held); the last one
+// frees the region.
+void drop_region(fmi_server *srv, Request &r) {
+  if (!r.region) return;
+  if (--r.region->refs == 0) {
+    srv->arena.release(r.region->off, r.region->size);
+    srv->cv_submit.notify_all();
+  }
+  r.region.reset();
+  r.counts = nullptr;
+  r.records = nullptr;
 }
 
 void fmi_server::worker() {
@@ -607,6 +642,7 @@ int fmi_server_cancel(fmi_server *srv, uint64_t request_id) {
   std::shared_ptr<Request> r = it->second;
   if (r->finished) {
     // its results were ready: drop them
+    drop_region(srv, *r);
     srv->live.erase(it);
     return FMI_OK;
   }
@@ -645,7 +681,7 @@ int fmi_server_wait(fmi_server *srv, uint64_t request_id, int64_t timeout_us, do
   }
   if (latency_ms)
     *latency_ms = std::chrono::duration<double, std::milli>(r->done - r->submit).count();
-  if (nrecords) *nrecords = r->records.size();
+  if (nrecords) *nrecords = r->nrec;
   return r->status;
 }
 
@@ -660,11 +696,14 @@ int fmi_server_fetch(fmi_server *srv, uint64_t request_id, uint32_t *counts, fmi
     if (!r->finished) return FMI_EAGAIN;
     srv->live.erase(it);
   }
-  if (r->status != FMI_OK) return r->status;
-  if (counts && !r->counts.empty()) std::memcpy(counts, r->counts.data(), r->counts.size() * 4);
-  if (records && !r->records.empty())
-    std::memcpy(records, r->records.data(), r->records.size() * sizeof(fmi_hit));
-  return FMI_OK;
+  // the request still holds its region: the mirror is not reused meanwhile
+  if (r->status == FMI_OK) {
+    if (counts && r->counts) std::memcpy(counts, r->counts, 2ull * r->nreads * 4);
+    if (records && r->nrec) std::memcpy(records, r->records, r->nrec * sizeof(fmi_hit));
+  }
+  std::lock_guard<std::mutex> lk(srv->mu);
+  drop_region(srv, *r);
+  return r->status;
 }
 
 int fmi_server_get_stats(fmi_server *srv, fmi_server_stats *stats) {

@@ -1125,8 +1125,8 @@ class Server:
             return None
         check(rc)
         n = self._n.pop(rid)
-        counts = np.zeros(max(2 * n, 1), np.uint32)
-        recs = np.zeros(max(int(nrec.value), 1), HIT_DTYPE)
+        counts = np.empty(max(2 * n, 1), np.uint32)
+        recs = np.empty(max(int(nrec.value), 1), HIT_DTYPE)
         check(lib().fmi_server_fetch(self.h, rid, _ptr(counts), _ptr(recs)))
         return counts[:2 * n].reshape(n, 2), recs[:int(nrec.value)], lat.value
 

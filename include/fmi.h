@@ -643,7 +643,10 @@ int fmi_server_wait(fmi_server *srv, uint64_t request_id, int64_t timeout_us,
                     double *latency_ms, uint64_t *nrecords);
 /* Copies a finished request's results and releases them: counts[2*i + s]
  * as fmi_search1_batch gives, and the stored records of every (read,
- * strand) slot, slot after slot, densely (nrecords of them). */
+ * strand) slot, slot after slot, densely (nrecords of them).  A finished
+ * request's results stay in the server's memory (its launch's region of
+ * the arena) until it is fetched or cancelled: fetch or cancel every
+ * request, or the arena fills and admission waits. */
 int fmi_server_fetch(fmi_server *srv, uint64_t request_id, uint32_t *counts, fmi_hit *records);
 int fmi_server_get_stats(fmi_server *srv, fmi_server_stats *stats);
 /* Finishes queued and in-flight work, then frees everything. */

@@ -212,8 +212,10 @@ def test_stream_destroy_releases_its_memory(gpu_engine):
     for _ in range(50):
         cycle()
     free1, _ = hp.mem_info()
-    # one cycle's scratch is hundreds of MB; 50 leaked cycles would be tens of GB
-    assert free0 - free1 < (64 << 20), (free0, free1)
+    # one cycle's scratch is hundreds of MB; 50 leaked cycles would be tens of
+    # GB.  The runtime's own pools move the free figure by a few hundred MB
+    # between calls, so the bound is 1 GiB.
+    assert free0 - free1 < (1 << 30), (free0, free1)
     for b in k_out + [fs]:
         b.free()
     out.free()
