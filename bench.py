@@ -157,6 +157,45 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     log(f"mixed budgets, one graph per step: {wall * 1000:.2f} ms per step, "
         f"{n / wall / 1e6:.2f} M reads/s")
 
+    # Algorithmic bytes of each group's launches (the line model of the
+    # 1-error search: 64 bytes per occurrence block, SA / ISA entry, text
+    # compare and table entry read at random; the k-error search's list
+    # records, 64 bytes each written or read; plus the reads, offsets and
+    # outputs), counted by the kernels' counting instantiations on the same
+    # batches, over each group's mean launch time.
+    rbytes = [int(x[2][-1]) + 8 * (len(x[0]) + 1) for x in groups]
+    alg = {}
+    wex = [0] * 5
+    for view, direction in ((hpgfm.VIEW_BACKWARD, 0), (hpgfm.VIEW_BACKWARD_REV, 1)):
+        w, _ = hpgfm.exact_count_work(ix, b0, view, direction)
+        wex = [a + b for a, b in zip(wex, w)]
+    alg["exact"] = {"blocks": wex[0], "lf_queries": wex[1], "sa_isa": wex[2], "text": wex[3],
+                    "table": wex[4],
+                    "bytes": 64 * (wex[0] + wex[2] + wex[3] + wex[4]) + 2 * rbytes[0] +
+                             2 * 32 * b0.nreads}
+    b1w = hpgfm.search1_count_work(ix, b1, cap=args.cap, mode=hpgfm.MODE_EDIT)
+    stored = int(np.minimum(s1.counts(), args.cap).sum())
+    alg["search1"] = {"blocks": b1w[0], "lf_queries": b1w[1], "sa_isa": b1w[2], "text": b1w[3],
+                      "table": b1w[4],
+                      "bytes": 64 * (b1w[0] + b1w[2] + b1w[3] + b1w[4]) + rbytes[1] +
+                               32 * stored + 8 * b1.nreads}
+    wk = np.zeros(6, np.uint64)
+    hpgfm.search_k_batch(ix, b2, fs2, list_cap, anchor_cap, work=wk)
+    wk = [int(x) for x in wk]
+    alg["search_k"] = {"blocks": wk[0], "lf_queries": wk[1], "sa_isa": wk[2], "text": wk[3],
+                       "table": wk[4], "list_records": wk[5],
+                       "bytes": 64 * (wk[0] + wk[2] + wk[3] + wk[4] + wk[5]) + rbytes[2] +
+                                2 * n2 + 4 * 4 * n2}
+    roof = {}
+    for name in ("exact", "search1", "search_k"):
+        gbps = alg[name]["bytes"] / (parts[name] / 1000.0) / 1e9
+        roof[name] = {"bound": "hbm", "alg_bytes": alg[name]["bytes"], "ms": parts[name],
+                      "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                      "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+        log(f"  {name}: {alg[name]['bytes'] / 1e9:.3f} GB algorithmic in {parts[name]} ms = "
+            f"{gbps:.0f} GB/s ({gbps / HBM_PEAK_GBPS:.3f} of 8 TB/s)")
+    step_bytes = sum(alg[x]["bytes"] for x in alg)
+
     cpu = None
     extras = {}
     if not args.no_cpu_baseline:
@@ -226,6 +265,10 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
                                f"50/35/15%", "preset": args.preset, "list_cap": list_cap,
                    "kmer_table_k": args.kmer},
         "cpu_baseline": cpu, "ms_per_group": parts,
+        "roofline": {"bound": "hbm", "achieved": round(step_bytes / wall / 1e9, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(step_bytes / wall / 1e9 / HBM_PEAK_GBPS, 4), "traffic": None},
+        "roofline_per_kernel": roof, "work_per_kernel": alg,
         "launch": "one HIP graph per step (the three groups' launches, fmi_graph_*)",
         "ms_per_step_launched_one_by_one": round(eager * 1000, 3),
         "reads_per_group": [len(x[0]) for x in groups],

@@ -844,6 +844,34 @@ int fmi_exact_batch(const fmi_index *ix, int view, int direction, const uint8_t 
   return end_timed(st, e, "exact launch");
 }
 
+int fmi_exact_count_work(const fmi_index *ix, int view, int direction, const uint8_t *d_reads,
+                         const uint64_t *d_offsets, uint32_t nreads, fmi_hit *d_out,
+                         fmi_hit *d_anchor, fmi_stream s, uint64_t *work) {
+  int rc = need_device(ix);
+  if (rc) return rc;
+  if (!work) return fail(FMI_EINVAL, "NULL argument");
+  for (int i = 0; i < 5; ++i) work[i] = 0;
+  uint32_t dsel, comp;
+  rc = view_params(view, direction, &dsel, &comp);
+  if (rc) return rc;
+  if (nreads == 0) return FMI_OK;
+  if (!d_reads || !d_offsets || !d_out) return fail(FMI_EINVAL, "NULL buffer");
+  hipStream_t st = static_cast<hipStream_t>(s);
+  if (capturing(st)) return fail(FMI_ESTATE, "a counting launch cannot be captured");
+  Workspace w;
+  rc = workspace_for(st, &w);
+  if (rc) return rc;
+  HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 2 * 5, st), "zero stats");
+  HIP_TRY(fmi::launch_exact(ix->dev, dsel, comp, (uint32_t)direction, d_reads, d_offsets, nreads,
+                            d_out, d_anchor, st, w.stats),
+          "exact count launch");
+  unsigned long long st5[5] = {0, 0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(st5, w.stats, sizeof st5, hipMemcpyDeviceToHost, st), "copy stats");
+  HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+  for (int i = 0; i < 5; ++i) work[i] = st5[i];
+  return FMI_OK;
+}
+
 int fmi_exact_vector_batch(const fmi_index *ix, int view, int direction, const uint8_t *d_reads,
                            const uint64_t *d_offsets, uint32_t nreads, int64_t *d_vec_k,
                            int64_t *d_vec_l, int16_t *d_last_good, fmi_stream s) {
@@ -937,11 +965,13 @@ int fmi_search1_vector_batch(const fmi_index *ix, int mode, int strand_mask,
   return end_timed(st, e, "search1 vector launch");
 }
 
-int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
-                       const uint64_t *d_offsets, const int16_t *d_fragsize, uint32_t nreads,
-                       uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final,
-                       uint32_t *d_nfinal, fmi_hitk *d_anchors, uint32_t *d_nanchors,
-                       uint8_t *d_full, fmi_stream s) {
+namespace {
+
+int search_k_common(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
+                    const uint64_t *d_offsets, const int16_t *d_fragsize, uint32_t nreads,
+                    uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final, uint32_t *d_nfinal,
+                    fmi_hitk *d_anchors, uint32_t *d_nanchors, uint8_t *d_full, fmi_stream s,
+                    uint64_t *work) {
   int rc = need_device(ix);
   if (rc) return rc;
   if (strand_mask < 1 || strand_mask > 3) return fail(FMI_EINVAL, "strand_mask must be 1, 2 or 3");
@@ -964,6 +994,20 @@ int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_re
   void *scr = nullptr;
   rc = scratch_for(st, fmi::search_k_scratch_bytes((uint32_t)lanes, list_cap), &scr);
   if (rc) return rc;
+  if (work) {
+    // the counting launch: synchronous, its counters in the stream's stats
+    HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 2 * 6, st), "zero stats");
+    HIP_TRY(fmi::launch_search_k(ix->dev, d_reads, d_offsets, d_fragsize, nreads,
+                                 (uint32_t)strand_mask, list_cap, anchor_cap, d_final, d_nfinal,
+                                 d_anchors, d_nanchors, d_full, scr, (uint32_t)lanes, w.queue, st,
+                                 w.stats),
+            "k-error count launch");
+    unsigned long long st6[6] = {0, 0, 0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(st6, w.stats, sizeof st6, hipMemcpyDeviceToHost, st), "copy stats");
+    HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (int i = 0; i < 6; ++i) work[i] = st6[i];
+    return FMI_OK;
+  }
   rc = begin_timed(st);
   if (rc) return rc;
   hipError_t e = fmi::launch_search_k(ix->dev, d_reads, d_offsets, d_fragsize, nreads,
@@ -972,6 +1016,30 @@ int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_re
                                       (uint32_t)lanes, w.queue, st);
   return end_timed(st, e, "k-error search launch");
 }
+
+}  // namespace
+
+int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
+                       const uint64_t *d_offsets, const int16_t *d_fragsize, uint32_t nreads,
+                       uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final,
+                       uint32_t *d_nfinal, fmi_hitk *d_anchors, uint32_t *d_nanchors,
+                       uint8_t *d_full, fmi_stream s) {
+  return search_k_common(ix, strand_mask, d_reads, d_offsets, d_fragsize, nreads, list_cap,
+                         anchor_cap, d_final, d_nfinal, d_anchors, d_nanchors, d_full, s, nullptr);
+}
+
+int fmi_search_k_count_work(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
+                            const uint64_t *d_offsets, const int16_t *d_fragsize, uint32_t nreads,
+                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final,
+                            uint32_t *d_nfinal, fmi_hitk *d_anchors, uint32_t *d_nanchors,
+                            uint8_t *d_full, fmi_stream s, uint64_t *work) {
+  if (!work) return fail(FMI_EINVAL, "NULL argument");
+  for (int i = 0; i < 6; ++i) work[i] = 0;
+  if (capturing(static_cast<hipStream_t>(s))) return fail(FMI_ESTATE, "a counting launch cannot be captured");
+  return search_k_common(ix, strand_mask, d_reads, d_offsets, d_fragsize, nreads, list_cap,
+                         anchor_cap, d_final, d_nfinal, d_anchors, d_nanchors, d_full, s, work);
+}
+
 
 int fmi_search1_batch_plain(const fmi_index *ix, int mode, int strand_mask,
                             const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,

@@ -10,9 +10,12 @@
 
 namespace fmi {
 
+// With work (5 counters), the counting instantiation: blocks, LF queries,
+// SA / ISA entries, text compares, table entries.
 hipError_t launch_exact(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
                         const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
-                        fmi_hit *out, fmi_hit *anchor, hipStream_t s);
+                        fmi_hit *out, fmi_hit *anchor, hipStream_t s,
+                        unsigned long long *work = nullptr);
 hipError_t launch_vector(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
                          const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
                          int64_t *vk, int64_t *vl, int16_t *last_good, hipStream_t s);
@@ -78,7 +81,8 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
                            const int16_t *fragsize, uint32_t nreads, uint32_t strand_mask,
                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
                            fmi_hitk *anchors, uint32_t *nanchors, uint8_t *full, void *scratch,
-                           uint32_t lanes, uint32_t *counter, hipStream_t s);
+                           uint32_t lanes, uint32_t *counter, hipStream_t s,
+                           unsigned long long *work = nullptr);
 // The read-level mapping of __bwt_map_inexact_read (fmi_mapread.hip):
 // `lanes` (map_reads_lanes) stride over the reads, each with a scratch slice
 // of filter + 1 alignments (map_reads_scratch_bytes in all).

@@ -30,6 +30,7 @@ struct ExactParams {
   const uint64_t *offsets;
   fmi_hit *out;
   fmi_hit *anchor;
+  unsigned long long *work;   // counting launch: blocks, LF queries, SA / ISA, text, table
 };
 
 // With a k-mer table the first k steps are one read of it when all k bases
@@ -39,9 +40,11 @@ struct ExactParams {
 // narrowed to one row goes on against the text at that row's suffix, 16
 // bases at a time; where it dies, one LF step from the last live row gives
 // the reference's empty interval exactly.
+template <bool COUNT>
 __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= P.nreads) return;
+  uint32_t wb = 0, wq = 0, ws = 0, wt = 0, wk = 0;   // counting launch
   const DevDir &D = dir_of(P.ix, P.dsel);
   const uint8_t *W = P.reads + P.offsets[r];
   const int32_t len = (int32_t)(P.offsets[r + 1] - P.offsets[r]);
@@ -60,6 +63,7 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
     if (P.comp) code ^= (1u << (2u * K)) - 1u;
     if (!bad) {
       const uint64_t e = P.ix.kmer[P.dsel][code];
+      if (COUNT) wk++;
       if ((uint32_t)e <= (uint32_t)(e >> 32)) {
         k = k2 = (uint32_t)e;
         l = l2 = (uint32_t)(e >> 32);
@@ -72,24 +76,34 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
       if (W[i] > 3u) { k2 = n + 1; l2 = n; break; }
       // text mode from the single row k2 on
       uint32_t q = D.sa[k2];
+      if (COUNT) ws++;
       uint32_t rem = (uint32_t)(d < 0 ? i + 1 : len - i);
       bool died = false;
       while (rem > 0u) {
         const uint32_t lim = min(min(rem, 16u), q);
         const uint32_t m = text_match(P.ix, P.dsel, P.comp, q, W, i, d, lim);
+        if (COUNT) wt++;
         q -= m;
         rem -= m;
         i += (int32_t)m * d;
         if (rem > 0u && (m < lim || q == 0u || m < 16u)) { died = true; break; }
       }
       k = l = D.isa[q];
+      if (COUNT) ws++;
       if (!died) { k2 = l2 = k; break; }
       const uint32_t s2 = W[i];
       if (s2 > 3u) { k2 = n + 1; l2 = n; }
-      else lf_one(D, k, l, P.comp ? 3u - s2 : s2, k2, l2, nb);
+      else {
+        lf_one(D, k, l, P.comp ? 3u - s2 : s2, k2, l2, nb);
+        if (COUNT) { wb += nb; wq++; }
+      }
       break;
     }
     uint32_t kn, ln;
+    if (COUNT) {
+      wb += (k2 != l2 && (l2 + 1u) / kBlockRows != k2 / kBlockRows) ? 2u : 1u;
+      wq++;
+    }
     if (lf_read(D, k2, l2, W + i, P.comp, kn, ln) > 3u) { k2 = n + 1; l2 = n; break; }
     k2 = kn; l2 = ln;
     if (k2 > l2) break;
@@ -103,6 +117,13 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
       store_hit(P.anchor + r, k, l, 0, (int16_t)i, (int16_t)(i - 1), 0, 1, 0, 0, 0, 0);
   }
   store_hit(P.out + r, k2, l2, 0, (int16_t)i, (int16_t)(len - 1), 0, (uint8_t)P.direction, 0, 0, 0, 0);
+  if (COUNT) {
+    atomicAdd(&P.work[0], (unsigned long long)wb);
+    atomicAdd(&P.work[1], (unsigned long long)wq);
+    atomicAdd(&P.work[2], (unsigned long long)ws);
+    atomicAdd(&P.work[3], (unsigned long long)wt);
+    atomicAdd(&P.work[4], (unsigned long long)wk);
+  }
 }
 
 // BWExactSearchVectorBackward / Forward (search/search.c:1407-1489).
@@ -1571,10 +1592,11 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src
 
 hipError_t launch_exact(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint32_t direction,
                         const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
-                        fmi_hit *out, fmi_hit *anchor, hipStream_t s) {
-  ExactParams P{ix, dsel, comp, direction, nreads, reads, offsets, out, anchor};
+                        fmi_hit *out, fmi_hit *anchor, hipStream_t s, unsigned long long *work) {
+  ExactParams P{ix, dsel, comp, direction, nreads, reads, offsets, out, anchor, work};
   const uint32_t grid = (nreads + 255) / 256;
-  hipLaunchKernelGGL(exact_kernel, dim3(grid), dim3(256), 0, s, P);
+  if (work) hipLaunchKernelGGL(exact_kernel<true>, dim3(grid), dim3(256), 0, s, P);
+  else hipLaunchKernelGGL(exact_kernel<false>, dim3(grid), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 

@@ -36,12 +36,22 @@ struct KView {
 
 struct KCtx {
   const DevIndex *ix;
+  // the counting launch's work counters (KWORK), else null: what the launch
+  // reads at random, in the line model of the 1-error search's count
+  uint32_t *wk;
   KView B, F;                        // the search's `backward` / `forward`
   const uint8_t *W;
   int32_t nW;
   uint8_t *D;
   bool text;                         // expanded index: single rows step on the text
 };
+
+enum : uint32_t { KW_BLOCKS = 0, KW_LF = 1, KW_SAISA = 2, KW_TEXT = 3, KW_TABLE = 4, KW_LIST = 5,
+                  KW_N = 6 };
+#define KWORK(c, i, v)                          \
+  do {                                          \
+    if ((c).wk) (c).wk[(i)] += (uint32_t)(v);   \
+  } while (0)
 
 // On an expanded index a one-row interval is stepped on the text: the LF
 // steps of W[i], W[i + step], ... (count positions) from the row whose
@@ -60,6 +70,7 @@ __device__ __forceinline__ int32_t k_run(const KCtx &c, const KView &v, uint32_t
     if (lim > q) lim = q;
     if (lim == 0u) { dies = true; break; }
     const uint32_t m = text_match(*c.ix, v.dsel, v.comp, q, c.W, i + done * step, step, lim);
+    KWORK(c, KW_TEXT, 1);
     done += (int32_t)m;
     q -= m;
     if (m < lim) { dies = true; break; }
@@ -68,8 +79,19 @@ __device__ __forceinline__ int32_t k_run(const KCtx &c, const KView &v, uint32_t
   return done;
 }
 
-__device__ __forceinline__ uint32_t k_sa(const KView &v, int64_t row) { return v.d->sa[row]; }
-__device__ __forceinline__ uint32_t k_isa(const KView &v, uint32_t q) { return v.d->isa[q]; }
+__device__ __forceinline__ uint32_t k_sa(const KCtx &c, const KView &v, int64_t row) {
+  KWORK(c, KW_SAISA, 1);
+  return v.d->sa[row];
+}
+__device__ __forceinline__ uint32_t k_isa(const KCtx &c, const KView &v, uint32_t q) {
+  KWORK(c, KW_SAISA, 1);
+  return v.d->isa[q];
+}
+// occurrence blocks an LF query on [k, l] reads (one when l + 1 shares k's
+// block or the interval is one row)
+__device__ __forceinline__ uint32_t k_blocks(int64_t k, int64_t l) {
+  return (k != l && (uint64_t)(l + 1) / kBlockRows != (uint64_t)k / kBlockRows) ? 2u : 1u;
+}
 
 // BWiteration (runtime.h:38-42) on a view; N (code > 3) gives (n + 1, n).
 // Only live intervals reach it; a dead one is passed through unchanged
@@ -81,6 +103,8 @@ __device__ __forceinline__ void klf(const KCtx &c, const KView &v, int64_t k, in
   if (k > l) { ko = k; lo = l; return; }
   uint32_t a, e, nb;
   lf_one(*v.d, (uint32_t)k, (uint32_t)l, v.comp ? 3u - b : b, a, e, nb);
+  KWORK(c, KW_BLOCKS, nb);
+  KWORK(c, KW_LF, 1);
   ko = a;
   lo = e;
 }
@@ -102,6 +126,7 @@ __device__ __forceinline__ bool k_table(const KCtx &c, const KView &v, int32_t i
   if (bad) return false;
   if (v.comp) code ^= (1u << (2u * K)) - 1u;
   const uint64_t e = c.ix->kmer[v.dsel][code];
+  KWORK(c, KW_TABLE, 1);
   if ((uint32_t)e > (uint32_t)(e >> 32)) return false;
   k = (uint32_t)e;
   l = (uint32_t)(e >> 32);
@@ -120,6 +145,8 @@ __device__ __forceinline__ void klfw(const KCtx &c, const KView &v, int64_t k, i
     return;
   }
   uint32_t a, e;
+  KWORK(c, KW_BLOCKS, k_blocks(k, l));
+  KWORK(c, KW_LF, 1);
   if (lf_read(*v.d, (uint32_t)k, (uint32_t)l, wp, v.comp, a, e) > 3u) {
     ko = (int64_t)n + 1;
     lo = n;
@@ -132,6 +159,7 @@ __device__ __forceinline__ void klfw(const KCtx &c, const KView &v, int64_t k, i
 // add_result (results.h:93-103): keep, or return true from the caller.
 #define KADD(r, L)                                                   \
   do {                                                               \
+    KWORK(c, KW_LIST, 1);                                            \
     if ((L).n < (L).cap) (L).list[(L).n++] = (r);                    \
     else return true;                                                \
   } while (0)
@@ -181,8 +209,8 @@ __device__ __forceinline__ void k_exact(const KCtx &c, const KView &v, int dir, 
       if (c.text && k2 == l2) {
         uint32_t q2;
         bool dies;
-        const int32_t done = k_run(c, v, k_sa(v, k2), i, -1, i - r.start + 1, q2, dies);
-        if (done) { k = l = k2 = l2 = k_isa(v, q2); }
+        const int32_t done = k_run(c, v, k_sa(c, v, k2), i, -1, i - r.start + 1, q2, dies);
+        if (done) { k = l = k2 = l2 = k_isa(c, v, q2); }
         i -= done;
         if (dies) { k2 = (int64_t)c.ix->n + 1; l2 = c.ix->n; }
         break;
@@ -204,8 +232,8 @@ __device__ __forceinline__ void k_exact(const KCtx &c, const KView &v, int dir, 
       if (c.text && k2 == l2) {
         uint32_t q2;
         bool dies;
-        const int32_t done = k_run(c, v, k_sa(v, k2), i, 1, r.end - i + 1, q2, dies);
-        if (done) { k = l = k2 = l2 = k_isa(v, q2); }
+        const int32_t done = k_run(c, v, k_sa(c, v, k2), i, 1, r.end - i + 1, q2, dies);
+        if (done) { k = l = k2 = l2 = k_isa(c, v, q2); }
         i += done;
         if (dies) { k2 = (int64_t)c.ix->n + 1; l2 = c.ix->n; }
         break;
@@ -232,9 +260,9 @@ __device__ __forceinline__ void k_extend(const KCtx &c, const KView &v, int64_t 
     if (c.text && k == l) {
       uint32_t q2;
       bool dies;
-      const int32_t done = k_run(c, v, k_sa(v, k), i, step, count, q2, dies);
+      const int32_t done = k_run(c, v, k_sa(c, v, k), i, step, count, q2, dies);
       if (dies) { k = (int64_t)c.ix->n + 1; l = c.ix->n; }
-      else if (done) { k = l = k_isa(v, q2); }
+      else if (done) { k = l = k_isa(c, v, q2); }
       return;
     }
     klfw(c, v, k, l, c.W + i, k, l);
@@ -276,6 +304,7 @@ __device__ __forceinline__ bool k_finals(const KCtx &c, const KView &v, int dir,
                          int32_t bs, int32_t lb) {
   for (uint32_t ii = 0; ii < prev.n; ++ii) {
     fmi_hitk &r = prev.list[ii];
+    KWORK(c, KW_LIST, 1);
     const int32_t start = r.start, pos = r.pos, end = r.end;
     int64_t k = r.k, l = r.l;
     const int32_t cb = pos / bs;
@@ -334,6 +363,8 @@ __device__ __forceinline__ bool k_branch(const KCtx &c, const KView &v, int dir,
   } else {
     uint32_t ko[4], lo[4];
     lf_four(*v.d, (uint32_t)k, (uint32_t)l, ko, lo);
+    KWORK(c, KW_BLOCKS, k_blocks(k, l));
+    KWORK(c, KW_LF, 1);
 #pragma unroll
     for (uint32_t b = 0; b < 4u; ++b) {
       const uint32_t s = v.comp ? 3u - b : b;
@@ -406,6 +437,7 @@ __device__ __forceinline__ bool k_branches(const KCtx &c, const KView &v, int di
                            int32_t bs, int32_t lb) {
   for (uint32_t ii = 0; ii < prev.n; ++ii) {
     fmi_hitk &r = prev.list[ii];
+    KWORK(c, KW_LIST, 1);
     if (r.dir != dir) continue;
     if (dir == 0 ? (r.pos < r.start) : (r.pos > r.end)) {
       if (FINAL && dir == 0) {
@@ -426,6 +458,7 @@ __device__ __forceinline__ bool k_partials(const KCtx &c, uint8_t max_errors, co
                            KList &next, KList &next_i, int32_t bs, int32_t lb) {
   for (uint32_t ii = 0; ii < prev.n; ++ii) {
     fmi_hitk &r = prev.list[ii];
+    KWORK(c, KW_LIST, 1);
     const int32_t start = r.start, pos = r.pos, end = r.end;
     int64_t kn = r.k, ln = r.l, k = 0, l = 0;
     int64_t res, res_next = ln - kn;
@@ -445,16 +478,16 @@ __device__ __forceinline__ bool k_partials(const KCtx &c, uint8_t max_errors, co
           // one row: no size change (no record) until the step that empties it
           uint32_t q2;
           bool dies;
-          const int32_t done = k_run(c, v, k_sa(v, k), i, -1, i - lbp + 1, q2, dies);
+          const int32_t done = k_run(c, v, k_sa(c, v, k), i, -1, i - lbp + 1, q2, dies);
           i -= done;
           if (!dies) {
-            if (done) kn = ln = k_isa(v, q2);
+            if (done) kn = ln = k_isa(c, v, q2);
             break;
           }
           res_next = -1;                     // the step at i empties the row
           kn = (int64_t)c.ix->n + 1; ln = c.ix->n;
           if (eb < c.D[i]) break;
-          if (done) k = l = k_isa(v, q2);
+          if (done) k = l = k_isa(c, v, q2);
           k_change(r, k, l, i);
           KADD(r, next);
           break;
@@ -477,16 +510,16 @@ __device__ __forceinline__ bool k_partials(const KCtx &c, uint8_t max_errors, co
         if (c.text && k == l) {
           uint32_t q2;
           bool dies;
-          const int32_t done = k_run(c, v, k_sa(v, k), i, 1, lbp - i + 1, q2, dies);
+          const int32_t done = k_run(c, v, k_sa(c, v, k), i, 1, lbp - i + 1, q2, dies);
           i += done;
           if (!dies) {
-            if (done) kn = ln = k_isa(v, q2);
+            if (done) kn = ln = k_isa(c, v, q2);
             break;
           }
           res_next = -1;
           kn = (int64_t)c.ix->n + 1; ln = c.ix->n;
           if (eb < c.D[i]) break;
-          if (done) k = l = k_isa(v, q2);
+          if (done) k = l = k_isa(c, v, q2);
           k_change(r, k, l, i);
           KADD(r, next);
           break;
@@ -531,6 +564,7 @@ __device__ __forceinline__ void k_turn(const KCtx &c, const KView &b, const KVie
                        int32_t bound) {
   for (uint32_t ii = L.n; ii > 0; --ii) {
     fmi_hitk &r = L.list[ii - 1];
+    KWORK(c, KW_LIST, 1);
     if (to_fwd) {
       if (r.dir == 1) break;
       if (r.pos == r.start - 1) {
@@ -571,7 +605,7 @@ __device__ __forceinline__ void k_dwalk(const KCtx &c, const KView &v, int64_t k
     if (c.text && k == l) {
       uint32_t q2;
       bool dies;
-      const int32_t done = k_run(c, v, k_sa(v, k), i, step, count, q2, dies);
+      const int32_t done = k_run(c, v, k_sa(c, v, k), i, step, count, q2, dies);
       for (int32_t t = 0; t < done; ++t) D[i + t * step] = z;
       i += done * step;
       count -= done;
@@ -714,6 +748,7 @@ __device__ __forceinline__ bool k_search(const KCtx &c, KList &prev, KList &next
 }
 
 #undef KADD
+#undef KWORK
 
 struct KParams {
   DevIndex ix;
@@ -726,12 +761,15 @@ struct KParams {
   fmi_hitk *scratch;                 // per lane: 4 lists of list_cap records
   uint8_t *dscratch;                 // per lane: FMI_MAX_READ_LEN bytes of D
   uint32_t *counter;
+  unsigned long long *work;          // counting launch: KW_N counters
   uint32_t ntasks, list_cap, anchor_cap, strand_mask;
 };
 
+template <bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void search_k_kernel(KParams P) {
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   fmi_hitk *sl = P.scratch + lane * 4ull * P.list_cap;
+  uint32_t wk[KW_N] = {0, 0, 0, 0, 0, 0};
   while (true) {
     const uint32_t t = atomicAdd(P.counter, 1u);
     if (t >= P.ntasks) break;
@@ -748,6 +786,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
     }
     KCtx c;
     c.ix = &P.ix;
+    c.wk = COUNT ? wk : nullptr;
     // + strand: (backward, forward); - strand: (forward_rev, backward_rev)
     c.B = KView{strand ? &P.ix.d[1] : &P.ix.d[0], strand, strand ? 1u : 0u};
     c.F = KView{strand ? &P.ix.d[0] : &P.ix.d[1], strand, strand ? 0u : 1u};
@@ -764,6 +803,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
     if (P.anchors) P.nanchors[t] = anc.n;
     P.full[t] = f ? 1 : 0;
   }
+  if (COUNT)
+    for (uint32_t i = 0; i < KW_N; ++i) atomicAdd(&P.work[i], (unsigned long long)wk[i]);
 }
 
 }  // namespace
@@ -776,8 +817,10 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
                            const int16_t *fragsize, uint32_t nreads, uint32_t strand_mask,
                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
                            fmi_hitk *anchors, uint32_t *nanchors, uint8_t *full, void *scratch,
-                           uint32_t lanes, uint32_t *counter, hipStream_t s) {
+                           uint32_t lanes, uint32_t *counter, hipStream_t s,
+                           unsigned long long *work) {
   KParams P;
+  P.work = work;
   P.ix = ix;
   P.reads = reads;
   P.offsets = offsets;
@@ -796,7 +839,8 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
   P.strand_mask = strand_mask;
   hipError_t e = launch_zero(counter, 1, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(search_k_kernel, dim3(lanes / 256), dim3(256), 0, s, P);
+  if (work) hipLaunchKernelGGL(search_k_kernel<true>, dim3(lanes / 256), dim3(256), 0, s, P);
+  else hipLaunchKernelGGL(search_k_kernel<false>, dim3(lanes / 256), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 

@@ -57,7 +57,7 @@ EXPORTS = [
     "fmi_seed_map_batch", "fmi_map_inexact_seeds_batch", "fmi_map_reads_batch", "fmi_format_results", "fmi_inverse_locate_batch", "fmi_encode_bases", "fmi_copy_bench",
     "fmi_search1_vector_batch", "fmi_index_expand", "fmi_index_kmer_table", "fmi_index_get_kmer_table",
     "fmi_index_get_full", "fmi_index_sample_count", "fmi_index_get_samples",
-    "fmi_search1_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
+    "fmi_search1_count_work", "fmi_search_k_count_work", "fmi_exact_count_work", "fmi_search_k_batch", "fmi_plain_create", "fmi_plain_search1", "fmi_plain_words",
     "fmi_plain_get_layout", "fmi_plain_device_bytes", "fmi_plain_free",
     "fmi_server_create", "fmi_server_submit", "fmi_server_wait", "fmi_server_fetch",
     "fmi_server_cancel", "fmi_server_submit_fastq", "fmi_fastq_encode_batch", "fmi_pack_codes",
@@ -147,6 +147,9 @@ def _bind(lib):
         "fmi_get_load_mode": ([C.POINTER(_u32)], _i),
         "fmi_set_staging": ([_i], _i),
         "fmi_get_staging": ([C.POINTER(_i)], _i),
+        "fmi_search_k_count_work": ([_p, _i, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p, _p, _p, _p,
+                                     _p], _i),
+        "fmi_exact_count_work": ([_p, _i, _i, _p, _p, _u32, _p, _p, _p, _p], _i),
         "fmi_locate_batch": ([_p, _i, _p, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_locate_hits": ([_p, _i, _p, _u32, _u32, _p, _p, _p], _i),
         "fmi_resolve_hits": ([_p, _p, _p, _p, _u32, _u32, _u32, _p, _p, _p], _i),
@@ -591,6 +594,20 @@ def exact_batch(ix, batch, view, direction, anchors=False, stream=None):
     return r, a
 
 
+def exact_count_work(ix, batch, view, direction):
+    """fmi_exact_count_work: the exact search of fmi_exact_batch with its
+    work counted; returns ((blocks, LF queries, SA / ISA reads, text
+    compares, k-mer table lookups), records)."""
+    n = batch.nreads
+    out = DeviceBuffer(max(n, 1) * 32)
+    w = np.zeros(5, np.uint64)
+    check(lib().fmi_exact_count_work(ix.h, view, direction, batch.d_reads.ptr, batch.d_offsets.ptr,
+                                     n, out.ptr, None, None, _ptr(w)))
+    r = out.download(HIT_DTYPE, n)
+    out.free()
+    return tuple(int(x) for x in w), r
+
+
 def exact_vector_batch(ix, batch, view, direction, stream=None):
     n = batch.nreads
     tot = max(batch.total, 1)
@@ -756,9 +773,13 @@ def search_k_launch(ix, batch, d_fragsize, list_cap, anchor_cap, d_fin, d_nfin, 
                                    d_nanc.ptr if d_nanc else None, d_full.ptr, stream))
 
 
-def search_k_batch(ix, batch, fragsize, list_cap=300, anchor_cap=16, strand_mask=3, stream=None):
+def search_k_batch(ix, batch, fragsize, list_cap=300, anchor_cap=16, strand_mask=3, stream=None,
+                   work=None):
     """fmi_search_k_batch (BWSearchCPU per read and strand); fragsize: int16
-    per read.  Returns a SearchKResult on the host."""
+    per read.  Returns a SearchKResult on the host.  With work (a uint64
+    array of 6) the search runs as fmi_search_k_count_work and adds its
+    (blocks, LF queries, SA / ISA reads, text compares, k-mer table lookups,
+    list records written or read) to work."""
     n = batch.nreads
     fs = DeviceBuffer.from_array(np.ascontiguousarray(fragsize, dtype=np.int16), stream)
     d_fin = DeviceBuffer(max(2 * n * list_cap, 1) * 64)
@@ -766,8 +787,16 @@ def search_k_batch(ix, batch, fragsize, list_cap=300, anchor_cap=16, strand_mask
     d_anc = DeviceBuffer(max(2 * n * anchor_cap, 1) * 64)
     d_nanc = DeviceBuffer(max(2 * n, 1) * 4)
     d_full = DeviceBuffer(max(2 * n, 1))
-    search_k_launch(ix, batch, fs, list_cap, anchor_cap, d_fin, d_nfin, d_anc, d_nanc, d_full,
-                    strand_mask, stream)
+    if work is None:
+        search_k_launch(ix, batch, fs, list_cap, anchor_cap, d_fin, d_nfin, d_anc, d_nanc, d_full,
+                        strand_mask, stream)
+    else:
+        w = np.zeros(6, np.uint64)
+        check(lib().fmi_search_k_count_work(ix.h, strand_mask, batch.d_reads.ptr,
+                                            batch.d_offsets.ptr, fs.ptr, n, list_cap, anchor_cap,
+                                            d_fin.ptr, d_nfin.ptr, d_anc.ptr, d_nanc.ptr,
+                                            d_full.ptr, stream, _ptr(w)))
+        work += w
     sync(stream)
     res = SearchKResult(d_nfin.download(np.uint32, 2 * n, stream),
                         d_fin.download(HITK_DTYPE, 2 * n * list_cap, stream),

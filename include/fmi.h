@@ -252,6 +252,14 @@ int fmi_timing_end(uint32_t *launches, double *sum_ms, float *min_ms, float *max
 int fmi_exact_batch(const fmi_index *ix, int view, int direction,
                     const uint8_t *d_reads, const uint64_t *d_offsets, uint32_t nreads,
                     fmi_hit *d_out, fmi_hit *d_anchor, fmi_stream s);
+/* fmi_exact_batch (same arguments and results, synchronous on s) with the
+ * launch's work counted, for the line model of the benchmark: work[0]
+ * occurrence blocks read, [1] LF queries, [2] SA / ISA entries read, [3]
+ * text compares (16 bases each, one text line), [4] k-mer table entries
+ * read.  No reference counterpart. */
+int fmi_exact_count_work(const fmi_index *ix, int view, int direction, const uint8_t *d_reads,
+                         const uint64_t *d_offsets, uint32_t nreads, fmi_hit *d_out,
+                         fmi_hit *d_anchor, fmi_stream s, uint64_t *work);
 
 /* BWExactSearchVectorBackward / Forward (search/search.c:1407-1489) of every
  * whole read from [0, size_SA - 1]: d_vec_k/l[offsets[i] + j] is the interval
@@ -671,6 +679,17 @@ int fmi_search_k_batch(const fmi_index *ix, int strand_mask, const uint8_t *d_re
                        uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final,
                        uint32_t *d_nfinal, fmi_hitk *d_anchors, uint32_t *d_nanchors,
                        uint8_t *d_full, fmi_stream s);
+/* fmi_search_k_batch (same arguments and results, synchronous on s) with
+ * the launch's work counted, for the line model of the benchmark: work[0]
+ * occurrence blocks read, [1] LF queries, [2] SA / ISA entries read, [3]
+ * text compares (16 bases each, one text line), [4] k-mer table entries
+ * read, [5] list records written or read (64 bytes each).  No reference
+ * counterpart. */
+int fmi_search_k_count_work(const fmi_index *ix, int strand_mask, const uint8_t *d_reads,
+                            const uint64_t *d_offsets, const int16_t *d_fragsize, uint32_t nreads,
+                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *d_final,
+                            uint32_t *d_nfinal, fmi_hitk *d_anchors, uint32_t *d_nanchors,
+                            uint8_t *d_full, fmi_stream s, uint64_t *work);
 
 /* ------------------------------------------- the plain comparison point */
 /* The reference's GPU design (gpu/gpu.cu:118-203 glue, gpu.cu:333-451
