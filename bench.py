@@ -156,6 +156,13 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     st.close()
     log(f"mixed budgets, one graph per step: {wall * 1000:.2f} ms per step, "
         f"{n / wall / 1e6:.2f} M reads/s")
+    if args.calibrate:
+        # kernels of known traffic for the FETCH_SIZE / WRITE_SIZE scales, as
+        # in the 1-error workload
+        hpgfm.copy_bench(1 << 30, reps=1)
+        for lb in (64, 128):
+            hpgfm.gather_bench(max(64 << 20, random_footprint(ix)["total"]), lb, 1 << 24)
+        hpgfm.sync()
 
     # Algorithmic bytes of each group's launches (the line model of the
     # 1-error search: 64 bytes per occurrence block, SA / ISA entry, text

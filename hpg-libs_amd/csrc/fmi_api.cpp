@@ -987,10 +987,15 @@ int search_k_common(const fmi_index *ix, int strand_mask, const uint8_t *d_reads
   Workspace w;
   rc = workspace_for(st, &w);
   if (rc) return rc;
-  // lanes: enough for the tasks, scratch within 4 GiB
-  const uint64_t per_lane = fmi::search_k_scratch_bytes(1, list_cap);
-  uint64_t lanes = std::min<uint64_t>((2ull * nreads + 255) & ~255ull, 256ull * 1024);
-  while (lanes > 256 && lanes * per_lane > (4ull << 30)) lanes -= 256;
+  // lanes: enough for the tasks, at most what the device holds resident,
+  // their lists and D within 12 GiB
+  int cus = 0;
+  rc = cu_count(&cus);
+  if (rc) return rc;
+  const uint64_t per_lane = fmi::search_k_lane_bytes(list_cap);
+  uint64_t lanes = std::min<uint64_t>((2ull * nreads + 255) & ~255ull,
+                                      (uint64_t)fmi::search_k_lanes_per_cu() * (uint64_t)cus);
+  while (lanes > 256 && lanes * per_lane > (12ull << 30)) lanes -= 256;
   void *scr = nullptr;
   rc = scratch_for(st, fmi::search_k_scratch_bytes((uint32_t)lanes, list_cap), &scr);
   if (rc) return rc;

@@ -75,8 +75,12 @@ hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_
                           uint32_t row_cap, const uint64_t *koff, const uint64_t *kstart,
                           uint32_t kcount, uint32_t record_len, int32_t *chrom, uint64_t *coord, hipStream_t s);
 // BWSearchCPU per (read, strand) task (fmi_search_k.hip); scratch holds
-// search_k_scratch_bytes(lanes, list_cap); lanes is a multiple of 256.
+// search_k_scratch_bytes(lanes, list_cap) (each lane's working lists and
+// D); lanes is a multiple of 256, best search_k_lanes_per_cu() x the CUs
+// (all resident).
+uint64_t search_k_lane_bytes(uint32_t list_cap);
 uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap);
+uint32_t search_k_lanes_per_cu();
 hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint64_t *offsets,
                            const int16_t *fragsize, uint32_t nreads, uint32_t strand_mask,
                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
