@@ -997,15 +997,19 @@ int search_k_common(const fmi_index *ix, int strand_mask, const uint8_t *d_reads
                                       (uint64_t)fmi::search_k_lanes_per_cu() * (uint64_t)cus);
   while (lanes > 256 && lanes * per_lane > (12ull << 30)) lanes -= 256;
   void *scr = nullptr;
-  rc = scratch_for(st, fmi::search_k_scratch_bytes((uint32_t)lanes, list_cap), &scr);
+  // the front kernel: all its lanes resident
+  const uint32_t front_lanes =
+      nreads <= fmi::kStagedMaxTasks ? fmi::search_k_front_lanes_per_cu() * (uint32_t)cus : 0u;
+  rc = scratch_for(st, fmi::search_k_scratch_bytes((uint32_t)lanes, list_cap) +
+                          (front_lanes ? fmi::search_k_front_bytes(nreads) : 0ull), &scr);
   if (rc) return rc;
   if (work) {
     // the counting launch: synchronous, its counters in the stream's stats
     HIP_TRY(fmi::launch_zero(reinterpret_cast<uint32_t *>(w.stats), 2 * 6, st), "zero stats");
     HIP_TRY(fmi::launch_search_k(ix->dev, d_reads, d_offsets, d_fragsize, nreads,
                                  (uint32_t)strand_mask, list_cap, anchor_cap, d_final, d_nfinal,
-                                 d_anchors, d_nanchors, d_full, scr, (uint32_t)lanes, w.queue, st,
-                                 w.stats),
+                                 d_anchors, d_nanchors, d_full, scr, (uint32_t)lanes, front_lanes,
+                                 w.queue, st, w.stats),
             "k-error count launch");
     unsigned long long st6[6] = {0, 0, 0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(st6, w.stats, sizeof st6, hipMemcpyDeviceToHost, st), "copy stats");
@@ -1018,7 +1022,7 @@ int search_k_common(const fmi_index *ix, int strand_mask, const uint8_t *d_reads
   hipError_t e = fmi::launch_search_k(ix->dev, d_reads, d_offsets, d_fragsize, nreads,
                                       (uint32_t)strand_mask, list_cap, anchor_cap, d_final,
                                       d_nfinal, d_anchors, d_nanchors, d_full, scr,
-                                      (uint32_t)lanes, w.queue, st);
+                                      (uint32_t)lanes, front_lanes, w.queue, st);
   return end_timed(st, e, "k-error search launch");
 }
 

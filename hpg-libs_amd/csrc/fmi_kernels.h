@@ -26,6 +26,14 @@ hipError_t launch_vector(const DevIndex &ix, uint32_t dsel, uint32_t comp, uint3
 // are searched afterwards by the unstaged kernel.  Records are the same
 // either way.
 constexpr uint32_t kStagedMaxLen = 256;
+// A read of at most kStagedMaxLen bases packed for staging: 16 words of
+// 2-bit codes (base j at bits 2(j % 16) of word j / 16) and 8 words of N
+// flags (bit j % 32 of word 16 + j / 32).  launch_read_records writes one
+// per read (kReadRecordWords words at rec + read * kReadRecordWords) and
+// appends the longer reads to list (list_n, zeroed by the caller).
+constexpr uint32_t kReadRecordWords = 24;
+hipError_t launch_read_records(const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
+                               uint32_t *rec, uint32_t *list, uint32_t *list_n, hipStream_t s);
 constexpr uint32_t kStagedMaxTasks = 1u << 27;
 uint64_t search1_staged_scratch_bytes(uint32_t nreads);
 hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, bool count,
@@ -76,16 +84,22 @@ hipError_t launch_resolve(const DevIndex &ix, const fmi_hit *hits, const uint32_
                           uint32_t kcount, uint32_t record_len, int32_t *chrom, uint64_t *coord, hipStream_t s);
 // BWSearchCPU per (read, strand) task (fmi_search_k.hip); scratch holds
 // search_k_scratch_bytes(lanes, list_cap) (each lane's working lists and
-// D); lanes is a multiple of 256, best search_k_lanes_per_cu() x the CUs
-// (all resident).
+// D), then, with front_lanes > 0 (at most kStagedMaxTasks reads),
+// search_k_front_bytes(nreads) for the front kernel, which computes the
+// first passes' exact searches and D of the reads of at most kStagedMaxLen
+// bases beforehand on front_lanes lanes.  lanes and front_lanes are
+// multiples of 256, best search_k_lanes_per_cu() / search_k_front_lanes_per_cu()
+// x the CUs (all resident).
 uint64_t search_k_lane_bytes(uint32_t list_cap);
 uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap);
+uint64_t search_k_front_bytes(uint32_t nreads);
 uint32_t search_k_lanes_per_cu();
+uint32_t search_k_front_lanes_per_cu();
 hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint64_t *offsets,
                            const int16_t *fragsize, uint32_t nreads, uint32_t strand_mask,
                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
                            fmi_hitk *anchors, uint32_t *nanchors, uint8_t *full, void *scratch,
-                           uint32_t lanes, uint32_t *counter, hipStream_t s,
+                           uint32_t lanes, uint32_t front_lanes, uint32_t *counter, hipStream_t s,
                            unsigned long long *work = nullptr);
 // The read-level mapping of __bwt_map_inexact_read (fmi_mapread.hip):
 // `lanes` (map_reads_lanes) stride over the reads, each with a scratch slice

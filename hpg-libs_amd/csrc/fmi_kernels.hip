@@ -1613,6 +1613,16 @@ uint64_t search1_staged_scratch_bytes(uint32_t nreads) {
   return ((64ull + 4ull * nreads + 127ull) & ~127ull) + (uint64_t)kRecWords * 4ull * nreads;
 }
 
+hipError_t launch_read_records(const uint8_t *reads, const uint64_t *offsets, uint32_t nreads,
+                               uint32_t *rec, uint32_t *list, uint32_t *list_n, hipStream_t s) {
+  static_assert(kRecWords == kReadRecordWords, "read record size");
+  if (nreads == 0) return hipSuccess;
+  RecParams R{reads, offsets, nreads, rec, list, list_n};
+  hipLaunchKernelGGL(read_records_kernel, dim3((uint32_t)((8ull * nreads + 255) / 256)), dim3(256),
+                     0, s, R);
+  return hipGetLastError();
+}
+
 hipError_t launch_search1(const DevIndex &ix, bool mm_only, bool persistent, bool count,
                           uint32_t strand_mask, const uint8_t *reads, const uint64_t *offsets,
                           uint32_t nreads, uint32_t cap, fmi_hit *hits, uint32_t *counts,

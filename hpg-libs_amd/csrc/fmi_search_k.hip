@@ -13,6 +13,7 @@
 // k = 2 reads of a mixed batch, not about occupancy.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "../../include/fmi.h"
@@ -23,6 +24,9 @@
 namespace fmi {
 
 namespace {
+
+constexpr uint32_t kFrontPasses = 3;
+constexpr uint32_t kFrontLen = 256;
 
 struct KList {
   fmi_hitk *list;
@@ -44,6 +48,11 @@ struct KCtx {
   int32_t nW;
   uint8_t *D;
   bool text;                         // expanded index: single rows step on the text
+  // the exact searches and D of the first passes, computed beforehand: pass
+  // b's D at fd + b * fstep * kFrontLen, its r and anchor at fr[2 * b * fstep]
+  const uint8_t *fd;
+  const uint4 *fr;
+  uint32_t fstep;
 };
 
 enum : uint32_t { KW_BLOCKS = 0, KW_LF = 1, KW_SAISA = 2, KW_TEXT = 3, KW_TABLE = 4, KW_LIST = 5,
@@ -682,31 +691,47 @@ __device__ __forceinline__ bool k_pass(const KCtx &c, KList &prev, KList &next, 
   r.start = (int16_t)(fs * i);
   r.end = (int16_t)(fwd || i < fragments - 1 ? fs * (i + 1) - 1 : nW - 1);
   k_change(r, 0, lsa, fwd ? r.start : r.end);
-  k_exact(c, X, dx, r, &ra);
+  KCtx cf = c;
+  const int32_t pb = fwd ? half - 1 - i : i;          // the pass's place in the order
+  const bool front = c.fd != nullptr && pb < (int32_t)kFrontPasses;
+  if (front) {
+    const uint4 *fr = c.fr + 2ull * (uint64_t)pb * c.fstep;
+    const uint4 a = fr[0];
+    const int16_t pos = (int16_t)fr[1].x;
+    k_clear(ra);
+    ra.dir = (uint8_t)dx;
+    ra.k = a.z; ra.l = a.w; ra.pos = pos;
+    if (dx == 0) { ra.start = (int16_t)(pos + 1); ra.end = r.pos; }
+    else { ra.start = r.pos; ra.end = (int16_t)(pos - 1); }
+    r.k = a.x; r.l = a.y; r.pos = pos;
+    cf.D = const_cast<uint8_t *>(c.fd) + (uint64_t)pb * c.fstep * kFrontLen;
+  } else {
+    k_exact(c, X, dx, r, &ra);
+  }
   if (anc.list) KADD(ra, anc);
   if (fwd) r.end = (int16_t)(nW - 1);
   else r.start = 0;
   if (r.k > r.l) return false;
-  k_calc_d(c, r, !fwd);
+  if (!front) k_calc_d(c, r, !fwd);
   KADD(r, prev);
   KList &out1 = two ? prev_i : fin;          // where X's complete partials go
   while (err_count > 0) {
-    if (k_partials(c, maxe, X, dx, prev, next, out1, fs, lb)) { flow = false; break; }
+    if (k_partials(cf, maxe, X, dx, prev, next, out1, fs, lb)) { flow = false; break; }
     if (two) {
-      k_turn(c, X, Y, prev_i, !fwd, bound);
-      if (k_partials(c, maxe, Y, dy, prev_i, next_i, fin, fs, lb)) { flow = false; break; }
+      k_turn(cf, X, Y, prev_i, !fwd, bound);
+      if (k_partials(cf, maxe, Y, dy, prev_i, next_i, fin, fs, lb)) { flow = false; break; }
     }
     if (err_count == 1) break;
-    if (k_branches<false>(c, X, dx, next, prev, 0, 0)) { flow = false; break; }
-    if (two && k_branches<false>(c, Y, dy, next_i, prev_i, 0, 0)) { flow = false; break; }
+    if (k_branches<false>(cf, X, dx, next, prev, 0, 0)) { flow = false; break; }
+    if (two && k_branches<false>(cf, Y, dy, next_i, prev_i, 0, 0)) { flow = false; break; }
     err_count--;
   }
   if (flow) {
-    k_branches<true>(c, X, dx, next, out1, fs, lb);
+    k_branches<true>(cf, X, dx, next, out1, fs, lb);
     if (two) {
-      k_turn(c, X, Y, prev_i, !fwd, bound);
-      k_finals(c, Y, dy, prev_i, fin, fs, lb);
-      k_branches<true>(c, Y, dy, next_i, fin, fs, lb);
+      k_turn(cf, X, Y, prev_i, !fwd, bound);
+      k_finals(cf, Y, dy, prev_i, fin, fs, lb);
+      k_branches<true>(cf, Y, dy, next_i, fin, fs, lb);
     }
   }
   return false;
@@ -762,8 +787,669 @@ struct KParams {
   uint8_t *dscratch;                 // per lane: FMI_MAX_READ_LEN bytes of D
   uint32_t *counter;
   unsigned long long *work;          // counting launch: KW_N counters
+  uint8_t *front_d;                  // per (pass < kFrontPasses, task): kFrontLen bytes of D
+  uint4 *front_r;                    // per (pass, task): the exact search's r and anchor
+  uint4 *front_m;                    // per (pass, task): the walks' restart marks
+  const uint4 *front_rec;            // the read records (kReadRecordWords words per read)
   uint32_t ntasks, list_cap, anchor_cap, strand_mask;
 };
+
+// ---- The first passes' exact searches and D, as a flat state machine.
+//
+// One unit is (pass b < kFrontPasses, task) of a read of at most kFrontLen
+// bases: k_exact on the pass's fragment (r and its anchor), then, when r is
+// live, the D walks of k_calc_d.  Each lane runs one unit at a time and takes
+// the next at once when it ends; every iteration is one memory access of
+// the lane (an LF query's block(s), an SA or ISA entry, 32 text bases, a
+// k-mer table entry), issued by all lanes together, then the lane's
+// transition.  The read is staged in the lane's LDS slot from its record.
+// D is not written position by position: a walk marks where it restarts
+// from the whole index (a bit per position in the slot), the unit stores the
+// marks, and the next launch writes D[i], the number of restarts after
+// position i in walk order (last - D[i] of calculateD*), from them, one
+// thread per 16 positions.
+
+enum : uint32_t { FK_TEXT = 0, FK_SA = 1, FK_ISA = 2, FK_TABLE = 3, FK_LF = 4, FK_UNIT = 5,
+                  FK_RECORD = 6, FK_MSA = 7, FK_MTEXT = 8 };
+enum : uint32_t { FP_EXACT = 0, FP_WALK1 = 1, FP_WALK2 = 2, FP_IDLE = 3, FP_UNIT = 4,
+                  FP_RECORD = 5 };
+constexpr uint32_t kFQNone = 0xFFFFFFFFu;
+constexpr uint32_t kFSlotWords = 32;             // 24 record words, 8 restart-mask words
+constexpr uint32_t F_TABOK = 1u;                 // exact: the first k bases may be a table read
+constexpr uint32_t F_TABFAIL = 2u;               // walk: the table entry here was empty
+constexpr uint32_t F_MOVED = 4u;                 // exact: the text run matched a base
+constexpr uint32_t F_ISA = 8u;                   // exact: the row after the run is to be read
+constexpr uint32_t F_RUNDIED = 16u;              // exact: ... and the run died
+
+struct FLane {
+  uint32_t u, rd, ph, flags, dsel, strand, fwd;
+  int32_t nW, i, step, cnt, f0, f1, p1;
+  uint32_t k, l, q, ka, la;
+  uint32_t r, q0, q1, q2, q3;      // walk: the rows of a multi-row text run (r > 1)
+};
+
+// A walk on an interval of 2..kMRows rows goes on against the text at each
+// row's suffix at once: the interval stays non-empty while one row's text
+// matches, and empties at the first base that no row's text continues with,
+// which is all a walk needs (D counts where intervals empty).
+constexpr uint32_t kMRows = 4;
+
+// The 32 read bases from W[p] in feed order (rw, 2-bit codes) and their N
+// flags (rb), from the lane's slot (words j at slot[j * 256]); the window
+// starts at p (d > 0) or p - 31 (then reversed); words outside the slot are
+// clamped, which changes only bases outside the read, never compared.
+__device__ __forceinline__ void fslot_window(const uint32_t *slot, int32_t p, int32_t d,
+                                             uint64_t &rw, uint32_t &rb) {
+  const uint32_t u = (uint32_t)((d > 0 ? p : p - 31) + 32);
+  const int32_t cw = (int32_t)(u >> 4) - 2, fw = (int32_t)(u >> 5) - 1;
+  const uint32_t sh = (u & 15u) << 1, fsh = u & 31u;
+  const uint32_t i0 = (uint32_t)min(max(cw, 0), 15), i1 = (uint32_t)min(max(cw + 1, 0), 15);
+  const uint32_t i2 = (uint32_t)min(max(cw + 2, 0), 15);
+  const uint32_t j0 = 16u + (uint32_t)min(max(fw, 0), 7), j1 = 16u + (uint32_t)min(max(fw + 1, 0), 7);
+  const uint32_t w0 = slot[i0 * 256u], w1 = slot[i1 * 256u], w2 = slot[i2 * 256u];
+  const uint32_t g0 = slot[j0 * 256u], g1 = slot[j1 * 256u];
+  const uint64_t lo = ((uint64_t)w1 << 32) | w0;
+  const uint64_t win = sh ? (lo >> sh) | ((uint64_t)w2 << (64u - sh)) : lo;
+  const uint32_t fl = fsh ? (g0 >> fsh) | (g1 << (32u - fsh)) : g0;
+  if (d > 0) {
+    rw = win;
+    rb = fl;
+  } else {
+    const uint64_t v = __builtin_bitreverse64(win);
+    rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+    rb = __builtin_bitreverse32(fl);
+  }
+}
+
+// The 32 text bases before suffix q in LF order (BWT of X: X[q-1], X[q-2],
+// ...; of reverse X: X[n-q], X[n-q+1], ...) from the words t0, t1, t2 that
+// start at word w (see front_step).
+__device__ __forceinline__ uint64_t ftext_window(uint32_t dsel, uint32_t n, uint32_t q, uint32_t t0,
+                                                 uint32_t t1, uint32_t t2) {
+  const uint64_t lo = ((uint64_t)t1 << 32) | t0;
+  if (dsel) {
+    const uint32_t sh = ((n - q) & 15u) << 1;
+    return sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
+  }
+  const int64_t b = (int64_t)q - 32;
+  uint64_t win;
+  if (b >= 0) {
+    const uint32_t sh = ((uint32_t)b & 15u) << 1;
+    win = sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
+  } else {
+    win = lo << min(2u * (uint32_t)(-b), 62u);
+  }
+  const uint64_t v = __builtin_bitreverse64(win);
+  return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+}
+
+// A new unit's read offsets and fragment size are in (the FK_UNIT access):
+// its pass, fragment and view; false when the unit has no pass to run.
+__device__ __forceinline__ bool front_unit(const KParams &P, FLane &L, uint64_t off0,
+                                           uint64_t off1, int32_t fs) {
+  const uint32_t b = L.u / P.ntasks, t = L.u - b * P.ntasks;
+  const uint32_t strand = t & 1u;
+  if (!(P.strand_mask & (1u << strand))) return false;
+  const int32_t len = (int32_t)(off1 - off0);
+  if (fs <= 0 || len > (int32_t)kFrontLen || len / fs - 1 > FMI_KMAX) return false;
+  const int32_t fragments = len / fs;
+  if (fragments <= 1 || (int32_t)b >= fragments) return false;
+  int32_t half = fragments / 2;
+  if (fragments % 2) half++;
+  const bool fwd = (int32_t)b < half;
+  const int32_t fi = (int32_t)b < half - 1 ? half - 1 - (int32_t)b
+                                           : ((int32_t)b == half - 1 ? 0 : (int32_t)b);
+  L.nW = len;
+  L.strand = strand;
+  L.fwd = fwd ? 1u : 0u;
+  L.f0 = fs * fi;
+  L.f1 = fwd || fi < fragments - 1 ? fs * (fi + 1) - 1 : len - 1;
+  return true;
+}
+
+// The read's record is in (the FK_RECORD access): its codes and N flags into
+// the slot, the restart marks cleared, and k_exact on X (forward: `forward`
+// left to right; backward: `backward` right to left) set to start from the
+// whole index.
+__device__ __forceinline__ void front_stage(const KParams &P, uint32_t *slot, FLane &L,
+                                            const uint4 &c0, const uint4 &c1, const uint4 &c2,
+                                            const uint4 &c3, const uint4 &g0, const uint4 &g1) {
+  const uint32_t nc = ((uint32_t)L.nW + 63u) >> 6, nf = ((uint32_t)L.nW + 127u) >> 7;
+  if (nc > 0u) { slot[0 * 256] = c0.x; slot[1 * 256] = c0.y; slot[2 * 256] = c0.z; slot[3 * 256] = c0.w; }
+  if (nc > 1u) { slot[4 * 256] = c1.x; slot[5 * 256] = c1.y; slot[6 * 256] = c1.z; slot[7 * 256] = c1.w; }
+  if (nc > 2u) { slot[8 * 256] = c2.x; slot[9 * 256] = c2.y; slot[10 * 256] = c2.z; slot[11 * 256] = c2.w; }
+  if (nc > 3u) { slot[12 * 256] = c3.x; slot[13 * 256] = c3.y; slot[14 * 256] = c3.z; slot[15 * 256] = c3.w; }
+  if (nf > 0u) { slot[16 * 256] = g0.x; slot[17 * 256] = g0.y; slot[18 * 256] = g0.z; slot[19 * 256] = g0.w; }
+  if (nf > 1u) { slot[20 * 256] = g1.x; slot[21 * 256] = g1.y; slot[22 * 256] = g1.z; slot[23 * 256] = g1.w; }
+#pragma unroll
+  for (uint32_t j = 24u; j < kFSlotWords; ++j) slot[j * 256u] = 0u;
+  L.dsel = L.fwd ? 1u - L.strand : L.strand;
+  L.step = L.fwd ? 1 : -1;
+  L.i = L.fwd ? L.f0 : L.f1;
+  L.cnt = L.f1 - L.f0 + 1;
+  L.k = 0u;
+  L.l = P.ix.n;
+  L.ka = 0u;
+  L.la = 0u;
+  L.q = kFQNone;
+  L.r = 0u;
+  L.flags = F_TABOK;
+  L.ph = FP_EXACT;
+}
+
+// D of a finished unit from its restart marks (m, 8 words: bit x set where
+// a walk restarted from the whole index at position x), 16 positions from
+// 16c.  The walk from p1 runs to the read's end in the pass's direction (A);
+// the other walk, if any, covers B = [0, f0) (forward pass) or (f1, nW)
+// (backward pass), and its restarts come after all of A's in walk order.
+// D[x] is the number of restarts after x in walk order: forward, x in A:
+// restarts above x plus B's, x in B: restarts below x; backward, x in A:
+// restarts below x plus B's, x in B: restarts above x.  The fragment's own
+// positions, which no walk visits, get a value no step reads.
+__device__ __forceinline__ uint4 front_d_chunk(const uint32_t (&m)[8], bool fwd, int32_t p1,
+                                               int32_t f0, int32_t f1, int32_t c) {
+  uint32_t pre[8];
+  uint32_t tot = 0u;
+#pragma unroll
+  for (uint32_t w = 0; w < 8u; ++w) {
+    pre[w] = tot;
+    tot += (uint32_t)__builtin_popcount(m[w]);
+  }
+  const int32_t xb = fwd ? f0 : f1 + 1;               // B lies below xb (forward) / from xb on
+  uint32_t bxb = xb >= 256 ? tot : 0u;
+#pragma unroll
+  for (uint32_t w = 0; w < 8u; ++w)
+    if (xb > 0 && xb < 256 && w == ((uint32_t)xb >> 5))
+      bxb = pre[w] + (uint32_t)__builtin_popcount(m[w] & ((1u << ((uint32_t)xb & 31u)) - 1u));
+  const uint32_t cntB = fwd ? bxb : tot - bxb;
+  uint32_t pw = 0u, mw = 0u;
+#pragma unroll
+  for (uint32_t w = 0; w < 8u; ++w)
+    if (w == ((uint32_t)c >> 1)) { pw = pre[w]; mw = m[w]; }
+  const uint32_t hoff = 16u * ((uint32_t)c & 1u);
+  uint32_t v[4] = {0u, 0u, 0u, 0u};
+  if (((mw >> hoff) & 0xFFFFu) == 0u) {
+    // no restart among the 16: one value below the walks' boundary p1, one above
+    const uint32_t b0 = pw + (uint32_t)__builtin_popcount(mw & ((1u << hoff) - 1u));
+    const uint32_t dlo = fwd ? b0 : b0 + cntB;
+    const uint32_t dhi = fwd ? tot - b0 + cntB : tot - b0;
+    const int32_t sp = min(max((fwd ? p1 : p1 + 1) - 16 * c, 0), 16);
+    const uint32_t rlo = (dlo & 0xFFu) * 0x01010101u, rhi = (dhi & 0xFFu) * 0x01010101u;
+#pragma unroll
+    for (int32_t qd = 0; qd < 4; ++qd) {
+      const int32_t e = min(max(sp - 4 * qd, 0), 4);
+      const uint32_t mh = e >= 4 ? 0u : (0xFFFFFFFFu << (8u * (uint32_t)e));
+      v[qd] = (rlo & ~mh) | (rhi & mh);
+    }
+    return make_uint4(v[0], v[1], v[2], v[3]);
+  }
+#pragma unroll
+  for (int32_t j = 0; j < 16; ++j) {
+    const int32_t x = 16 * c + j;
+    const uint32_t b = hoff + (uint32_t)j;
+    const uint32_t bx = pw + (uint32_t)__builtin_popcount(mw & ((1u << b) - 1u));
+    const uint32_t bx1 = pw + (uint32_t)__builtin_popcount(mw & ((2u << b) - 1u));
+    uint32_t d;
+    if (fwd) d = x >= p1 ? tot - bx1 + cntB : bx;
+    else d = x <= p1 ? bx + cntB : tot - bx1;
+    v[j >> 2] |= (d & 0xFFu) << (8u * (uint32_t)(j & 3));
+  }
+  return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+// One iteration of a lane's unit: choose the access (ALU and LDS only),
+// issue it together with the other lanes' (no value used before the last
+// load is issued), then the transition.
+template <bool COUNT>
+__device__ __forceinline__ void front_step(const KParams &P, uint32_t *slot, FLane &L,
+                                           uint32_t *wk) {
+  const uint32_t n = P.ix.n;
+  const uint32_t K = P.ix.kmer_k;
+  const DevDir &Dd = L.dsel ? P.ix.d[1] : P.ix.d[0];
+  const bool textmode = P.ix.text != nullptr && Dd.sa != nullptr;
+  const bool exact = L.ph == FP_EXACT;
+  uint64_t rw;
+  uint32_t rb;
+  fslot_window(slot, L.i, L.step, rw, rb);
+  const uint64_t rwc = L.strand ? ~rw : rw;          // complemented on the - strand
+
+  // ---- what to read
+  uint32_t kind;
+  uint32_t lim = 0u, code = 0u, j1 = 0u, j2 = 0u;
+  bool two = false, nextw = false;
+  const uint32_t *a0 = nullptr;
+  if (L.ph == FP_UNIT) {
+    kind = FK_UNIT;
+  } else if (L.ph == FP_RECORD) {
+    kind = FK_RECORD;
+  } else if (L.q != kFQNone) {
+    if (exact && (L.flags & F_ISA)) {
+      kind = FK_ISA;                                  // the row after the text run
+      a0 = Dd.isa + L.q;
+    } else {
+      kind = FK_TEXT;
+      lim = min(min((uint32_t)L.cnt, 32u), L.q);
+      const uint32_t nwords = (n + 15u) >> 4;
+      uint32_t w;
+      if (L.dsel) {
+        w = min((n - L.q) >> 4, nwords - 1u);
+      } else {
+        const int64_t b = (int64_t)L.q - 32;
+        w = b >= 0 ? (uint32_t)b >> 4 : 0u;
+      }
+      nextw = (w & 3u) >= 2u && w + 4u - (w & 3u) < nwords;
+      a0 = P.ix.text + w;
+    }
+  } else if (L.r > 1u) {
+    kind = FK_MTEXT;
+  } else if (textmode && L.k == L.l) {
+    kind = FK_SA;
+    a0 = Dd.sa + L.k;
+  } else {
+    kind = FK_LF;
+    const bool whole = K && L.k == 0u && L.l == n && (uint32_t)L.cnt >= K;
+    const bool tab = exact ? (L.flags & F_TABOK) != 0u && whole
+                           : whole && !(L.flags & F_TABFAIL);
+    if (tab) {
+      const uint32_t nmask = (1u << K) - 1u;
+      if ((rb & nmask) == 0u) {
+        kind = FK_TABLE;
+        code = (uint32_t)rwc & ((1u << (2u * K)) - 1u);
+        a0 = reinterpret_cast<const uint32_t *>((L.dsel ? P.ix.kmer[1] : P.ix.kmer[0]) + code);
+      }
+      if (exact) L.flags &= ~F_TABOK;                 // tried once, at the start
+    }
+    if (kind == FK_LF && !exact && textmode && L.l > L.k && L.l - L.k < kMRows) {
+      kind = FK_MSA;                                  // the rows' suffixes, then their text
+      a0 = Dd.sa + L.k;
+    }
+    if (kind == FK_LF) {
+      j1 = L.k / kBlockRows;
+      j2 = (L.l + 1u) / kBlockRows;
+      two = L.k != L.l && j2 != j1;
+    }
+  }
+  // multi-row text: the words of each row's 32 text bases (as FK_TEXT's)
+  const uint32_t nwords = (n + 15u) >> 4;
+  uint32_t mw0 = 0u, mw1 = 0u, mw2 = 0u, mw3 = 0u;
+  if (kind == FK_MTEXT) {
+    auto wordof = [&](uint32_t q) -> uint32_t {
+      if (L.dsel) return min((n - q) >> 4, nwords - 1u);
+      const int64_t b = (int64_t)q - 32;
+      return b >= 0 ? (uint32_t)b >> 4 : 0u;
+    };
+    mw0 = wordof(L.q0);
+    mw1 = wordof(L.q1);
+    mw2 = L.r > 2u ? wordof(L.q2) : 0u;
+    mw3 = L.r > 3u ? wordof(L.q3) : 0u;
+  }
+  const bool lf = kind == FK_LF;
+  const bool unit = kind == FK_UNIT, record = kind == FK_RECORD;
+  const uint8_t *f0;
+  if (lf) {
+    f0 = reinterpret_cast<const uint8_t *>(Dd.blk + j1);
+  } else if (unit) {
+    f0 = reinterpret_cast<const uint8_t *>(P.offsets + L.rd);
+  } else if (record) {
+    f0 = reinterpret_cast<const uint8_t *>(P.front_rec + (size_t)L.rd * (kReadRecordWords / 4u));
+  } else if (kind == FK_MTEXT) {
+    f0 = reinterpret_cast<const uint8_t *>(P.ix.text);
+  } else {
+    const uint8_t *w0 = reinterpret_cast<const uint8_t *>(a0);
+    f0 = w0 - (reinterpret_cast<uintptr_t>(w0) & 15u);
+  }
+
+  // ---- the loads
+  uint4 fx0 = opaque4();
+  uint4 fx1 = opaque4(), fx2 = opaque4(), fx3 = opaque4();
+  uint4 gx0 = opaque4(), gx1 = opaque4(), gx2 = opaque4(), gx3 = opaque4();
+  uint64_t o0 = 0ull, o1 = 0ull;
+  int32_t fsv = 0;
+  if (unit) {                                         // the unit's read offsets and fragment size
+    const uint64_t *op = reinterpret_cast<const uint64_t *>(f0);
+    o0 = op[0];
+    o1 = op[1];
+    fsv = P.fragsize[L.rd];
+  } else if (kind != FK_MTEXT) {
+    fx0 = *reinterpret_cast<const uint4 *>(f0);
+  }
+  if (lf || nextw || record) fx1 = *reinterpret_cast<const uint4 *>(f0 + 16);
+  if (lf || record) {
+    fx2 = *reinterpret_cast<const uint4 *>(f0 + 32);
+    fx3 = *reinterpret_cast<const uint4 *>(f0 + 48);
+  }
+  if (record) {
+    gx0 = *reinterpret_cast<const uint4 *>(f0 + 64);
+    gx1 = *reinterpret_cast<const uint4 *>(f0 + 80);
+  }
+  const bool mtext = kind == FK_MTEXT;
+  if (kind == FK_MSA && (L.k & 3u) + (L.l - L.k + 1u) > 4u)
+    fx1 = *reinterpret_cast<const uint4 *>(f0 + 16);
+  if (mtext) {
+    auto nxt = [&](uint32_t w) { return (w & 3u) >= 2u && w + 4u - (w & 3u) < nwords; };
+    const uint4 *tx = reinterpret_cast<const uint4 *>(P.ix.text);
+    fx0 = tx[mw0 >> 2];
+    fx1 = tx[mw1 >> 2];
+    if (L.r > 2u) fx2 = tx[mw2 >> 2];
+    if (L.r > 3u) fx3 = tx[mw3 >> 2];
+    if (nxt(mw0)) gx0 = tx[(mw0 >> 2) + 1u];
+    if (nxt(mw1)) gx1 = tx[(mw1 >> 2) + 1u];
+    if (L.r > 2u && nxt(mw2)) gx2 = tx[(mw2 >> 2) + 1u];
+    if (L.r > 3u && nxt(mw3)) gx3 = tx[(mw3 >> 2) + 1u];
+  }
+  if (two) {
+    const uint8_t *g0 = reinterpret_cast<const uint8_t *>(Dd.blk + j2);
+    gx0 = *reinterpret_cast<const uint4 *>(g0);
+    gx1 = *reinterpret_cast<const uint4 *>(g0 + 16);
+    gx2 = *reinterpret_cast<const uint4 *>(g0 + 32);
+    gx3 = *reinterpret_cast<const uint4 *>(g0 + 48);
+  }
+  asm volatile("" ::"v"(fx0.x), "v"(fx1.x), "v"(fx2.x), "v"(fx3.x), "v"(gx0.x), "v"(gx1.x),
+               "v"(gx2.x), "v"(gx3.x), "v"((uint32_t)o0), "v"((uint32_t)o1), "v"(fsv));
+  if (unit) {
+    if (front_unit(P, L, o0, o1, fsv)) L.ph = FP_RECORD;
+    else L.ph = FP_IDLE;
+    return;
+  }
+  if (record) {
+    front_stage(P, slot, L, fx0, fx1, fx2, fx3, gx0, gx1);
+    return;
+  }
+
+  // ---- the transitions
+  const uint32_t sel = (uint32_t)(reinterpret_cast<uintptr_t>(a0) & 15u) >> 2;
+  bool restart = false;      // walk: the step at i emptied the interval
+  bool ended = false;        // the exact search or the walk is over
+  if (kind == FK_MSA) {
+    if (COUNT) wk[KW_SAISA] += L.l - L.k + 1u;
+    const uint32_t c0 = L.k & 3u;
+    auto sa_at = [&](uint32_t j) -> uint32_t {   // SA[k + j], from the two chunks
+      const uint32_t x = c0 + j;
+      return x < 4u ? pick4u(fx0, x) : pick4u(fx1, x - 4u);
+    };
+    L.r = L.l - L.k + 1u;
+    L.q0 = sa_at(0u);
+    L.q1 = sa_at(1u);
+    L.q2 = L.r > 2u ? sa_at(2u) : 0u;
+    L.q3 = L.r > 3u ? sa_at(3u) : 0u;
+    return;
+  }
+  if (kind == FK_MTEXT) {
+    if (COUNT) wk[KW_TEXT] += L.r;
+    uint64_t spread = rb;                             // N flag of base j -> bit 2j
+    spread = (spread | (spread << 16)) & 0x0000FFFF0000FFFFull;
+    spread = (spread | (spread << 8)) & 0x00FF00FF00FF00FFull;
+    spread = (spread | (spread << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    spread = (spread | (spread << 2)) & 0x3333333333333333ull;
+    spread = (spread | (spread << 1)) & 0x5555555555555555ull;
+    const uint32_t lc = min((uint32_t)L.cnt, 32u);   // every row compares the same bases
+    auto run = [&](uint32_t q, uint32_t w, const uint4 &a, const uint4 &b) -> uint32_t {
+      const uint32_t sl = w & 3u;
+      const uint32_t t0 = pick4u(a, sl);
+      const uint32_t t1 = w + 1u >= nwords ? 0u : (sl == 3u ? b.x : pick4u(a, sl + 1u));
+      const uint32_t t2 = w + 2u >= nwords ? 0u : (sl >= 2u ? pick4u(b, sl - 2u) : pick4u(a, sl + 2u));
+      const uint32_t lim = min(lc, q);
+      if (lim == 0u) return 0u;
+      const uint64_t x = rwc ^ ftext_window(L.dsel, n, q, t0, t1, t2);
+      uint64_t miss = ((x | (x >> 1)) & 0x5555555555555555ull) | spread;
+      if (lim < 32u) miss |= 1ull << (2u * lim);
+      return miss ? (uint32_t)__builtin_ctzll(miss) >> 1 : 32u;
+    };
+    const uint32_t m0 = run(L.q0, mw0, fx0, gx0), m1 = run(L.q1, mw1, fx1, gx1);
+    const uint32_t m2 = L.r > 2u ? run(L.q2, mw2, fx2, gx2) : 0u;
+    const uint32_t m3 = L.r > 3u ? run(L.q3, mw3, fx3, gx3) : 0u;
+    const bool s0 = m0 == lc, s1 = m1 == lc, s2 = L.r > 2u && m2 == lc, s3 = L.r > 3u && m3 == lc;
+    const uint32_t ns = (s0 ? 1u : 0u) + (s1 ? 1u : 0u) + (s2 ? 1u : 0u) + (s3 ? 1u : 0u);
+    if (ns == 0u) {
+      // the interval empties at the first base no row continues with
+      const uint32_t mm = max(max(m0, m1), max(m2, m3));
+      L.i += (int32_t)mm * L.step;
+      L.cnt -= (int32_t)mm;
+      L.r = 0u;
+      restart = true;
+    } else {
+      // the survivors, in order, lc bases further on
+      uint32_t nq0 = 0u, nq1 = 0u, nq2 = 0u, nq3 = 0u, x = 0u;
+      auto put = [&](bool keep, uint32_t v) {
+        if (!keep) return;
+        if (x == 0u) nq0 = v;
+        else if (x == 1u) nq1 = v;
+        else if (x == 2u) nq2 = v;
+        else nq3 = v;
+        x++;
+      };
+      put(s0, L.q0 - lc);
+      put(s1, L.q1 - lc);
+      put(s2, L.q2 - lc);
+      put(s3, L.q3 - lc);
+      L.i += (int32_t)lc * L.step;
+      L.cnt -= (int32_t)lc;
+      if (ns == 1u) {
+        L.r = 0u;
+        L.q = nq0;                                    // one row: single-row text from here
+        L.k = L.l = 0u;
+      } else {
+        L.r = ns;
+        L.q0 = nq0; L.q1 = nq1; L.q2 = nq2; L.q3 = nq3;
+      }
+      if (L.cnt == 0) {
+        L.r = 0u;
+        L.q = kFQNone;
+        ended = true;
+      }
+    }
+  }
+  if (kind == FK_SA) {
+    if (COUNT) wk[KW_SAISA]++;
+    L.q = pick4u(fx0, sel);
+    L.flags &= ~(F_MOVED | F_ISA | F_RUNDIED);
+    return;
+  }
+  if (kind == FK_MTEXT) {
+    // done above
+  } else if (kind == FK_ISA) {
+    if (COUNT) wk[KW_SAISA]++;
+    const uint32_t row = pick4u(fx0, sel);
+    L.ka = L.la = row;
+    L.k = L.l = row;
+    if (L.flags & F_RUNDIED) { L.k = n + 1u; L.l = n; }
+    L.q = kFQNone;
+    ended = true;
+  } else if (kind == FK_TEXT) {
+    if (COUNT) wk[KW_TEXT]++;
+    const uint32_t nwords = (n + 15u) >> 4;
+    const uint32_t w = (uint32_t)(a0 - P.ix.text);
+    const uint32_t t0 = pick4u(fx0, sel);
+    const uint32_t t1 = w + 1u >= nwords ? 0u : (sel == 3u ? fx1.x : pick4u(fx0, sel + 1u));
+    const uint32_t t2 = w + 2u >= nwords ? 0u : (sel >= 2u ? pick4u(fx1, sel - 2u) : pick4u(fx0, sel + 2u));
+    const uint64_t tw = ftext_window(L.dsel, n, L.q, t0, t1, t2);
+    uint64_t spread = rb;                             // N flag of base j -> bit 2j
+    spread = (spread | (spread << 16)) & 0x0000FFFF0000FFFFull;
+    spread = (spread | (spread << 8)) & 0x00FF00FF00FF00FFull;
+    spread = (spread | (spread << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    spread = (spread | (spread << 2)) & 0x3333333333333333ull;
+    spread = (spread | (spread << 1)) & 0x5555555555555555ull;
+    uint32_t m = 0u;
+    if (lim > 0u) {
+      const uint64_t x = rwc ^ tw;
+      uint64_t miss = ((x | (x >> 1)) & 0x5555555555555555ull) | spread;
+      if (lim < 32u) miss |= 1ull << (2u * lim);
+      m = miss ? (uint32_t)__builtin_ctzll(miss) >> 1 : 32u;
+    }
+    L.i += (int32_t)m * L.step;
+    L.cnt -= (int32_t)m;
+    L.q -= m;
+    if (m > 0u) L.flags |= F_MOVED;
+    const bool dies = m < lim || lim == 0u;
+    if (!dies && L.cnt > 0) return;                  // the run goes on
+    if (exact) {
+      if (L.flags & F_MOVED) {                        // the row after the matched bases
+        L.flags |= F_ISA | (dies ? F_RUNDIED : 0u);
+        return;
+      }
+      L.k = n + 1u;                                   // died on its first base
+      L.l = n;
+      L.q = kFQNone;
+      ended = true;
+    } else {
+      L.q = kFQNone;
+      if (!dies) {
+        ended = true;
+      } else {
+        restart = true;
+      }
+    }
+  } else if (kind == FK_TABLE) {
+    if (COUNT) wk[KW_TABLE]++;
+    const bool hi = (reinterpret_cast<uintptr_t>(a0) & 8u) != 0u;
+    const uint32_t ek = hi ? fx0.z : fx0.x, el = hi ? fx0.w : fx0.y;
+    if (ek <= el) {
+      L.k = ek;
+      L.l = el;
+      if (exact) { L.ka = ek; L.la = el; }
+      L.i += (int32_t)K * L.step;
+      L.cnt -= (int32_t)K;
+      if (L.cnt == 0) ended = true;
+    } else if (!exact) {
+      L.flags |= F_TABFAIL;                           // step this position one base at a time
+      return;
+    } else {
+      return;                                         // exact: LF steps from the whole index
+    }
+  } else {
+    // LF of W[i] on [k, l] (an N gives the empty (n + 1, n))
+    const uint32_t sym = (rb & 1u) ? 4u : (uint32_t)(rw & 3u);
+    if (COUNT) { wk[KW_LF]++; wk[KW_BLOCKS] += two ? 2u : 1u; }
+    uint32_t kn, ln;
+    if (sym > 3u) {
+      kn = n + 1u;
+      ln = n;
+    } else {
+      const uint32_t s = L.strand ? 3u - sym : sym;
+      const uint32_t c = pick4(Dd.C, s);
+      const uint32_t t1 = L.k - j1 * kBlockRows, t2 = L.l + 1u - j2 * kBlockRows;
+      const Blk B1 = blk_of4(fx0, fx1, fx2, fx3);
+      kn = c + 1u + occ_blk(B1, Dd.dollar_blk, Dd.dollar_off, j1, s, t1);
+      if (L.k == L.l) {
+        ln = kn - 1u + (sym_at(B1, Dd.dollar_blk, Dd.dollar_off, j1, t1) == s ? 1u : 0u);
+      } else {
+        const Blk B2 = two ? blk_of4(gx0, gx1, gx2, gx3) : B1;
+        ln = c + occ_blk(B2, Dd.dollar_blk, Dd.dollar_off, j2, s, t2);
+      }
+    }
+    L.flags &= ~F_TABFAIL;
+    if (exact) {
+      L.k = kn;
+      L.l = ln;
+      if (kn > ln) {
+        ended = true;                                 // r dies here; the anchor keeps (ka, la)
+      } else {
+        L.ka = kn;
+        L.la = ln;
+        L.i += L.step;
+        L.cnt--;
+        if (L.cnt == 0) ended = true;
+      }
+    } else if (kn > ln) {
+      restart = true;
+    } else {
+      L.k = kn;
+      L.l = ln;
+      L.i += L.step;
+      L.cnt--;
+      if (L.cnt == 0) ended = true;
+    }
+  }
+  if (restart) {                                      // walk: D counts one more from i on
+    slot[(24u + ((uint32_t)L.i >> 5)) * 256u] |= 1u << ((uint32_t)L.i & 31u);
+    L.r = 0u;
+    L.q = kFQNone;
+    L.k = 0u;
+    L.l = n;
+    L.i += L.step;
+    L.cnt--;
+    L.flags = 0u;
+    if (L.cnt == 0) ended = true;
+  }
+  if (!ended) return;
+  if (exact) {
+    // r = (k, l) at position i, the anchor the last live interval
+    P.front_r[2ull * L.u] = make_uint4(L.k, L.l, L.ka, L.la);
+    P.front_r[2ull * L.u + 1ull] = make_uint4((uint32_t)L.i, 0u, 0u, 0u);   // bit 17 of .z: D follows
+    if (L.k > L.l) { L.ph = FP_IDLE; return; }
+    // walk 1 (k_calc_d): on the same view from r.pos to the read's end
+    L.p1 = L.i;
+    L.cnt = L.fwd ? L.nW - L.i : L.i + 1;
+    L.q = kFQNone;
+    L.r = 0u;
+    L.flags = 0u;
+    L.ph = FP_WALK1;
+    return;
+  }
+  if (L.ph == FP_WALK1 && (L.fwd ? L.f0 > 0 : L.f1 < L.nW - 1)) {
+    // walk 2: on the other view from the fragment's other side, whole index
+    L.dsel ^= 1u;
+    L.step = -L.step;
+    L.i = L.fwd ? L.f0 - 1 : L.f1 + 1;
+    L.cnt = L.fwd ? L.f0 : L.nW - 1 - L.f1;
+    L.k = 0u;
+    L.l = n;
+    L.q = kFQNone;
+    L.r = 0u;
+    L.flags = 0u;
+    L.ph = FP_WALK2;
+    return;
+  }
+  // the marks and what D needs besides them; D is written by the next launch
+  uint4 *fm = P.front_m + 2ull * L.u;
+  fm[0] = make_uint4(slot[24u * 256u], slot[25u * 256u], slot[26u * 256u], slot[27u * 256u]);
+  fm[1] = make_uint4(slot[28u * 256u], slot[29u * 256u], slot[30u * 256u], slot[31u * 256u]);
+  P.front_r[2ull * L.u + 1ull] = make_uint4((uint32_t)L.p1, (uint32_t)L.f0 | ((uint32_t)L.f1 << 16),
+                                            (uint32_t)L.nW | (L.fwd << 16) | (1u << 17), 0u);
+  L.ph = FP_IDLE;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void search_k_front_kernel(KParams P) {
+  __shared__ uint32_t fsl[kFSlotWords * 256u];
+  uint32_t *slot = fsl + threadIdx.x;
+  const uint32_t nunits = P.ntasks * kFrontPasses;
+  const uint32_t G = gridDim.x * 256u;
+  uint32_t unext = blockIdx.x * 256u + threadIdx.x;   // units unext, unext + G, ...
+  uint32_t wk[KW_N] = {0, 0, 0, 0, 0, 0};
+  FLane L;
+  L.ph = FP_IDLE;
+  while (true) {
+    if (L.ph == FP_IDLE) {
+      if (unext >= nunits) break;
+      L.u = unext;
+      L.rd = (unext % P.ntasks) >> 1;
+      unext += G;
+      L.ph = FP_UNIT;
+    }
+    front_step<COUNT>(P, slot, L, wk);
+  }
+  if (COUNT)
+    for (uint32_t i = 0; i < KW_N; ++i) atomicAdd(&P.work[i], (unsigned long long)wk[i]);
+}
+
+// D of every unit whose r is live: one thread per 64 positions.
+__global__ __launch_bounds__(256) void search_k_front_d_kernel(KParams P) {
+  const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t u = gid >> 2;
+  const int32_t c0 = 4 * (int32_t)(gid & 3u);
+  if (u >= (uint64_t)P.ntasks * kFrontPasses) return;
+  const uint4 e = P.front_r[2ull * u + 1ull];
+  const int32_t nW = (int32_t)(e.z & 0xFFFFu);
+  if (!(e.z & (1u << 17)) || 16 * c0 >= nW) return;
+  const uint4 a = P.front_m[2ull * u], b = P.front_m[2ull * u + 1ull];
+  const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  const bool fwd = ((e.z >> 16) & 1u) != 0u;
+  const int32_t p1 = (int32_t)e.x, f0 = (int32_t)(e.y & 0xFFFFu), f1 = (int32_t)(e.y >> 16);
+  uint8_t *D = P.front_d + u * kFrontLen;
+  for (int32_t c = c0; c < c0 + 4 && 16 * c < nW; ++c)
+    *reinterpret_cast<uint4 *>(D + 16ull * (uint32_t)c) = front_d_chunk(m, fwd, p1, f0, f1, c);
+}
 
 template <bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_k_kernel(KParams P) {
@@ -794,6 +1480,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
     c.W = P.reads + off;
     c.nW = len;
     c.D = P.dscratch + lane * (uint64_t)FMI_MAX_READ_LEN;
+    const bool fr = P.front_d != nullptr && len <= (int32_t)kFrontLen;
+    c.fd = fr ? P.front_d + (uint64_t)t * kFrontLen : nullptr;
+    c.fr = P.front_r + 2ull * t;
+    c.fstep = P.ntasks;
     KList prev{sl, 0, P.list_cap}, next{sl + P.list_cap, 0, P.list_cap};
     KList prev_i{sl + 2 * P.list_cap, 0, P.list_cap}, next_i{sl + 3 * P.list_cap, 0, P.list_cap};
     KList fin{P.fin + (uint64_t)t * P.list_cap, 0, P.list_cap};
@@ -817,6 +1507,25 @@ uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap) {
   return (uint64_t)lanes * search_k_lane_bytes(list_cap);
 }
 
+// The front kernel's scratch: D (kFrontLen bytes) and r / anchor (32 bytes)
+// per unit, then the read records, the list of longer reads and its length.
+uint64_t search_k_front_bytes(uint32_t nreads) {
+  return (uint64_t)kFrontPasses * 2ull * nreads * (kFrontLen + 64ull) +
+         (uint64_t)nreads * (4ull * kReadRecordWords + 4ull) + 64ull;
+}
+
+uint32_t search_k_front_lanes_per_cu() {
+  static const uint32_t lanes = [] {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, search_k_front_kernel<false>, 256,
+                                                     0) != hipSuccess ||
+        blocks < 1)
+      blocks = 1;
+    return 256u * (uint32_t)blocks;
+  }();
+  return lanes;
+}
+
 // Three waves per SIMD (168 registers, a few spilled): 14% faster than two
 // without spills on the 256 Mbp mixed batch, and faster than four.
 uint32_t search_k_lanes_per_cu() {
@@ -835,8 +1544,8 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
                            const int16_t *fragsize, uint32_t nreads, uint32_t strand_mask,
                            uint32_t list_cap, uint32_t anchor_cap, fmi_hitk *fin, uint32_t *nfin,
                            fmi_hitk *anchors, uint32_t *nanchors, uint8_t *full, void *scratch,
-                           uint32_t lanes, uint32_t *counter, hipStream_t s,
-                           unsigned long long *work) {
+                           uint32_t lanes, uint32_t front_lanes, uint32_t *counter,
+                           hipStream_t s, unsigned long long *work) {
   KParams P;
   P.work = work;
   P.ix = ix;
@@ -855,8 +1564,37 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
   P.list_cap = list_cap;
   P.anchor_cap = anchor_cap;
   P.strand_mask = strand_mask;
+  P.front_d = nullptr;
+  P.front_r = nullptr;
+  P.front_m = nullptr;
+  P.front_rec = nullptr;
   hipError_t e = launch_zero(counter, 1, s);
   if (e != hipSuccess) return e;
+  if (front_lanes) {
+    // the first passes' exact searches and D, from the read records
+    uint8_t *fb = static_cast<uint8_t *>(scratch) + search_k_scratch_bytes(lanes, list_cap);
+    const uint64_t units = (uint64_t)kFrontPasses * 2ull * nreads;
+    P.front_d = fb;
+    P.front_r = reinterpret_cast<uint4 *>(fb + units * kFrontLen);
+    P.front_m = reinterpret_cast<uint4 *>(fb + units * (kFrontLen + 32ull));
+    uint32_t *rec = reinterpret_cast<uint32_t *>(fb + units * (kFrontLen + 64ull));
+    uint32_t *list = rec + (uint64_t)kReadRecordWords * nreads;
+    uint32_t *list_n = list + nreads;
+    e = launch_zero(list_n, 1, s);
+    if (e != hipSuccess) return e;
+    e = launch_read_records(reads, offsets, nreads, rec, list, list_n, s);
+    if (e != hipSuccess) return e;
+    const uint32_t fl = (uint32_t)std::min<uint64_t>((units + 255ull) & ~255ull, front_lanes);
+    P.front_rec = reinterpret_cast<const uint4 *>(rec);
+    if (work) hipLaunchKernelGGL(search_k_front_kernel<true>, dim3(fl / 256u), dim3(256), 0, s, P);
+    else hipLaunchKernelGGL(search_k_front_kernel<false>, dim3(fl / 256u), dim3(256), 0, s, P);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(search_k_front_d_kernel, dim3((uint32_t)((units * 4ull + 255ull) / 256ull)),
+                       dim3(256), 0, s, P);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (work) hipLaunchKernelGGL(search_k_kernel<true>, dim3(lanes / 256), dim3(256), 0, s, P);
   else hipLaunchKernelGGL(search_k_kernel<false>, dim3(lanes / 256), dim3(256), 0, s, P);
   return hipGetLastError();
