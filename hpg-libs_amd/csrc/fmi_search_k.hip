@@ -937,35 +937,19 @@ __device__ __forceinline__ void front_stage(const KParams &P, uint32_t *slot, FL
   L.ph = FP_EXACT;
 }
 
-// D of a finished unit from its restart marks (m, 8 words: bit x set where
-// a walk restarted from the whole index at position x), 16 positions from
-// 16c.  The walk from p1 runs to the read's end in the pass's direction (A);
+// D of a finished unit from its restart marks (8 words: bit x set where a
+// walk restarted from the whole index at position x), 16 positions from
+// 16c, the marks' word holding them being mw.  The walk from p1 runs to the read's end in the pass's direction (A);
 // the other walk, if any, covers B = [0, f0) (forward pass) or (f1, nW)
 // (backward pass), and its restarts come after all of A's in walk order.
 // D[x] is the number of restarts after x in walk order: forward, x in A:
 // restarts above x plus B's, x in B: restarts below x; backward, x in A:
 // restarts below x plus B's, x in B: restarts above x.  The fragment's own
 // positions, which no walk visits, get a value no step reads.
-__device__ __forceinline__ uint4 front_d_chunk(const uint32_t (&m)[8], bool fwd, int32_t p1,
-                                               int32_t f0, int32_t f1, int32_t c) {
-  uint32_t pre[8];
-  uint32_t tot = 0u;
-#pragma unroll
-  for (uint32_t w = 0; w < 8u; ++w) {
-    pre[w] = tot;
-    tot += (uint32_t)__builtin_popcount(m[w]);
-  }
-  const int32_t xb = fwd ? f0 : f1 + 1;               // B lies below xb (forward) / from xb on
-  uint32_t bxb = xb >= 256 ? tot : 0u;
-#pragma unroll
-  for (uint32_t w = 0; w < 8u; ++w)
-    if (xb > 0 && xb < 256 && w == ((uint32_t)xb >> 5))
-      bxb = pre[w] + (uint32_t)__builtin_popcount(m[w] & ((1u << ((uint32_t)xb & 31u)) - 1u));
-  const uint32_t cntB = fwd ? bxb : tot - bxb;
-  uint32_t pw = 0u, mw = 0u;
-#pragma unroll
-  for (uint32_t w = 0; w < 8u; ++w)
-    if (w == ((uint32_t)c >> 1)) { pw = pre[w]; mw = m[w]; }
+// pw: the restarts below word mw's first position; tot: all of them; cntB:
+// those of the second walk.
+__device__ __forceinline__ uint4 front_d_chunk(uint32_t pw, uint32_t mw, uint32_t tot,
+                                               uint32_t cntB, bool fwd, int32_t p1, int32_t c) {
   const uint32_t hoff = 16u * ((uint32_t)c & 1u);
   uint32_t v[4] = {0u, 0u, 0u, 0u};
   if (((mw >> hoff) & 0xFFFFu) == 0u) {
@@ -1433,22 +1417,44 @@ __global__ __launch_bounds__(256) void search_k_front_kernel(KParams P) {
     for (uint32_t i = 0; i < KW_N; ++i) atomicAdd(&P.work[i], (unsigned long long)wk[i]);
 }
 
-// D of every unit whose r is live: one thread per 64 positions.
+// D of every unit whose r is live: one thread per 64 positions (two words
+// of marks).
 __global__ __launch_bounds__(256) void search_k_front_d_kernel(KParams P) {
   const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   const uint64_t u = gid >> 2;
-  const int32_t c0 = 4 * (int32_t)(gid & 3u);
+  const uint32_t part = (uint32_t)(gid & 3u);
   if (u >= (uint64_t)P.ntasks * kFrontPasses) return;
   const uint4 e = P.front_r[2ull * u + 1ull];
   const int32_t nW = (int32_t)(e.z & 0xFFFFu);
-  if (!(e.z & (1u << 17)) || 16 * c0 >= nW) return;
+  if (!(e.z & (1u << 17)) || 64 * (int32_t)part >= nW) return;
   const uint4 a = P.front_m[2ull * u], b = P.front_m[2ull * u + 1ull];
   const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   const bool fwd = ((e.z >> 16) & 1u) != 0u;
   const int32_t p1 = (int32_t)e.x, f0 = (int32_t)(e.y & 0xFFFFu), f1 = (int32_t)(e.y >> 16);
+  // the second walk lies below f0 (forward pass) or from f1 + 1 on
+  const int32_t xb = fwd ? f0 : f1 + 1;
+  uint32_t tot = 0u, pre = 0u, below = 0u, mw0 = 0u, mw1 = 0u;
+#pragma unroll
+  for (uint32_t j = 0; j < 8u; ++j) {
+    const uint32_t c = (uint32_t)__builtin_popcount(m[j]);
+    tot += c;
+    if (j < 2u * part) pre += c;
+    if (j == 2u * part) mw0 = m[j];
+    if (j == 2u * part + 1u) mw1 = m[j];
+    if ((int32_t)(32u * j + 32u) <= xb) below += c;
+    else if ((int32_t)(32u * j) < xb)
+      below += (uint32_t)__builtin_popcount(m[j] & ((1u << ((uint32_t)xb & 31u)) - 1u));
+  }
+  const uint32_t cntB = fwd ? below : tot - below;
+  const uint32_t pre1 = pre + (uint32_t)__builtin_popcount(mw0);
   uint8_t *D = P.front_d + u * kFrontLen;
-  for (int32_t c = c0; c < c0 + 4 && 16 * c < nW; ++c)
-    *reinterpret_cast<uint4 *>(D + 16ull * (uint32_t)c) = front_d_chunk(m, fwd, p1, f0, f1, c);
+#pragma unroll
+  for (int32_t cc = 0; cc < 4; ++cc) {
+    const int32_t c = 4 * (int32_t)part + cc;
+    if (16 * c >= nW) break;
+    *reinterpret_cast<uint4 *>(D + 16ull * (uint32_t)c) =
+        front_d_chunk(cc < 2 ? pre : pre1, cc < 2 ? mw0 : mw1, tot, cntB, fwd, p1, c);
+  }
 }
 
 template <bool COUNT>
