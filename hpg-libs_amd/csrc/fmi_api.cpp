@@ -75,7 +75,11 @@ struct StreamState {
   uint32_t graphs = 0;                    // live graphs captured on the stream
 };
 std::unordered_map<std::string, StreamState> g_streams;
-std::unordered_map<void *, std::string> g_graph_stream;   // graph -> stream key
+// graph -> the keys of the streams whose counters and scratch it holds (the
+// capturing stream and the streams that joined its capture)
+std::unordered_map<void *, std::vector<std::string>> g_graph_stream;
+// capture id -> the streams that joined it through fmi_stream_wait
+std::unordered_map<unsigned long long, std::vector<std::string>> g_capture_members;
 
 // The device memory of fmi_search1_batch_noalloc: the persistent launch's
 // work-queue counter (bytes 0..3) and its work counters (64..103), the same
@@ -754,6 +758,18 @@ int fmi_graph_end(fmi_stream s, fmi_graph *g) {
   std::string key;
   int rc = stream_key(st, &key);
   if (rc) return rc;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  unsigned long long id = 0;
+  HIP_TRY(hipStreamGetCaptureInfo(st, &cs, &id), "hipStreamGetCaptureInfo");
+  std::vector<std::string> keys{key};
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto m = g_capture_members.find(id);
+    if (m != g_capture_members.end()) {
+      keys.insert(keys.end(), m->second.begin(), m->second.end());
+      g_capture_members.erase(m);
+    }
+  }
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
   hipGraphExec_t exec = nullptr;
@@ -761,12 +777,37 @@ int fmi_graph_end(fmi_stream s, fmi_graph *g) {
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   {
-    // the stream's counters and scratch are now baked into the graph
+    // the streams' counters and scratch are now baked into the graph
     std::lock_guard<std::mutex> lk(g_mu);
-    g_streams[key].graphs++;
-    g_graph_stream[exec] = key;
+    for (const std::string &k : keys) g_streams[k].graphs++;
+    g_graph_stream[exec] = keys;
   }
   *g = exec;
+  return FMI_OK;
+}
+
+int fmi_stream_wait(fmi_stream waiter, fmi_stream signaler) {
+  if (!waiter || !signaler || waiter == signaler)
+    return fail(FMI_EINVAL, "two different streams of fmi_stream_create");
+  hipStream_t w = static_cast<hipStream_t>(waiter), sg = static_cast<hipStream_t>(signaler);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  unsigned long long id = 0;
+  HIP_TRY(hipStreamGetCaptureInfo(sg, &cs, &id), "hipStreamGetCaptureInfo");
+  const bool joins = cs == hipStreamCaptureStatusActive && !capturing(w);
+  hipEvent_t ev = nullptr;
+  HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreateWithFlags");
+  hipError_t e = hipEventRecord(ev, sg);
+  if (e == hipSuccess) e = hipStreamWaitEvent(w, ev, 0);
+  (void)hipEventDestroy(ev);
+  if (e != hipSuccess) return hip_fail(e, "fmi_stream_wait");
+  if (joins) {
+    // the waiter now records into the signaler's capture
+    std::string key;
+    int rc = stream_key(w, &key);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_capture_members[id].push_back(key);
+  }
   return FMI_OK;
 }
 
@@ -783,8 +824,10 @@ int fmi_graph_destroy(fmi_graph g) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_graph_stream.find(g);
   if (it != g_graph_stream.end()) {
-    auto st = g_streams.find(it->second);
-    if (st != g_streams.end() && st->second.graphs) st->second.graphs--;
+    for (const std::string &k : it->second) {
+      auto st = g_streams.find(k);
+      if (st != g_streams.end() && st->second.graphs) st->second.graphs--;
+    }
     g_graph_stream.erase(it);
   }
   return FMI_OK;

@@ -47,7 +47,7 @@ EXPORTS = [
     "fmi_index_set_karyotype", "fmi_device_count", "fmi_device_alloc", "fmi_device_free",
     "fmi_memcpy_h2d", "fmi_memcpy_d2h", "fmi_memset_d", "fmi_device_mem_info", "fmi_stream_create",
     "fmi_device_name", "fmi_device_cu_count",
-    "fmi_stream_destroy", "fmi_stream_sync", "fmi_graph_begin", "fmi_graph_end",
+    "fmi_stream_destroy", "fmi_stream_sync", "fmi_stream_wait", "fmi_graph_begin", "fmi_graph_end",
     "fmi_graph_launch", "fmi_graph_destroy", "fmi_last_kernel_ms", "fmi_exact_batch",
     "fmi_exact_vector_batch", "fmi_search1_batch", "fmi_search1_batch_plain",
     "fmi_search1_anchors_batch", "fmi_search1_batch_noalloc",
@@ -128,6 +128,7 @@ def _bind(lib):
         "fmi_stream_destroy": ([_p], _i),
         "fmi_stream_sync": ([_p], _i),
         "fmi_graph_begin": ([C.c_void_p], _i),
+        "fmi_stream_wait": ([C.c_void_p, C.c_void_p], _i),
         "fmi_graph_end": ([C.c_void_p, C.POINTER(C.c_void_p)], _i),
         "fmi_graph_launch": ([C.c_void_p, C.c_void_p], _i),
         "fmi_graph_destroy": ([C.c_void_p], _i),
@@ -304,6 +305,12 @@ class Stream:
 
     def sync(self):
         check(lib().fmi_stream_sync(self.handle))
+
+    def wait(self, other):
+        """fmi_stream_wait: this stream waits for the work issued so far on
+        other; during other's capture this stream joins it (a fork), and
+        other.wait(this) joins it back."""
+        check(lib().fmi_stream_wait(self.handle, other.handle))
 
     def close(self):
         """Drains the stream and frees it with the library's counters and

@@ -226,6 +226,16 @@ int fmi_stream_sync(fmi_stream s);
  * counterpart (the reference launches synchronously, gpu.cu:118-203). */
 typedef void *fmi_graph;
 int fmi_graph_begin(fmi_stream s);
+/* Orders stream waiter after the work issued so far on stream signaler
+ * (both from fmi_stream_create): an event recorded on signaler, waited for
+ * on waiter.  While signaler is being captured, a waiter that was not
+ * capturing joins the capture (a fork: the calls made on it afterwards are
+ * recorded into the same graph, and the graph holds its counters and
+ * scratch as it holds the capturing stream's); it must be joined back with
+ * fmi_stream_wait(capturing stream, waiter) before fmi_graph_end.  No
+ * reference counterpart: it lets independent batch calls (say the three
+ * error budgets of a mixed batch) run side by side in one graph. */
+int fmi_stream_wait(fmi_stream waiter, fmi_stream signaler);
 int fmi_graph_end(fmi_stream s, fmi_graph *g);
 int fmi_graph_launch(fmi_graph g, fmi_stream s);
 int fmi_graph_destroy(fmi_graph g);

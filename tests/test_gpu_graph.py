@@ -221,3 +221,96 @@ def test_stream_destroy_releases_its_memory(gpu_engine):
     out.free()
     batch.free()
     ix.close()
+
+
+def test_forked_graph_runs_branches_and_pins_their_scratch(gpu_engine):
+    """Three independent calls on three streams captured as one graph: the
+    exact search on the capturing stream, the 1-error and the k-error search
+    on two streams forked from it with fmi_stream_wait and joined back.
+    Every replay equals the eager records; while the graph lives a forked
+    stream's scratch is pinned as the capturing stream's is (a call needing
+    more is refused, the stream cannot be destroyed), and destroying the
+    graph lifts it."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(250_000, seed=121, repeat_frac=0.3, fam_len=(300, 2000))
+    ix = hp.Index.build(g, 16).to_device().expand()
+    ix.kmer_table(8)
+    lens = np.random.default_rng(122).integers(40, 220, 1500)
+    reads, offs, _ = synth.reads(g, 1500, seed=123, sub_rate=0.015, n_rate=0.002, lengths=lens)
+    batch = hp.ReadBatch(reads, offs)
+    n = batch.nreads
+    st = [hp.Stream() for _ in range(3)]
+    ex = hp.DeviceBuffer(n * 32)
+    out = hp.Search1Output(n, 8)
+    list_cap, anchor_cap = 32, 8
+    fs = hp.DeviceBuffer.from_array(np.maximum(lens // 3, 10).astype(np.int16))
+    k_out = _k_buffers(hp, n, list_cap, anchor_cap)
+    L = hp.lib()
+
+    def chain():
+        hp.check(L.fmi_exact_batch(ix.h, hp.VIEW_BACKWARD, 0, batch.d_reads.ptr,
+                                   batch.d_offsets.ptr, n, ex.ptr, None, st[0].handle))
+        hp.search1_launch(ix, batch, out, mode=hp.MODE_EDIT, stream=st[1].handle)
+        hp.search_k_launch(ix, batch, fs, list_cap, anchor_cap, *k_out, stream=st[2].handle)
+
+    def results():
+        for x in st:
+            x.sync()
+        kr = hp.SearchKResult(k_out[1].download(np.uint32, 2 * n),
+                              k_out[0].download(hp.HITK_DTYPE, 2 * n * list_cap),
+                              k_out[3].download(np.uint32, 2 * n),
+                              k_out[2].download(hp.HITK_DTYPE, 2 * n * anchor_cap),
+                              k_out[4].download(np.uint8, 2 * n).reshape(n, 2),
+                              list_cap, anchor_cap)
+        return ex.download(hp.HIT_DTYPE, n), out.counts(), out.hits(), kr
+
+    def clear():
+        for b in [ex, out.d_counts, out.d_hits] + k_out:
+            b.zero()
+        hp.sync()
+
+    clear()
+    chain()                              # eager: the reference, and sizes each stream's scratch
+    want = results()
+    graph = hp.Graph(st[0])
+    with graph:
+        st[1].wait(st[0])
+        st[2].wait(st[0])
+        chain()
+        st[0].wait(st[1])
+        st[0].wait(st[2])
+    big = _k_buffers(hp, n, 1024, anchor_cap)
+    try:
+        for rep in range(3):
+            clear()
+            hp.search_k_batch(ix, batch, np.full(n, 40, np.int16), list_cap=8)   # eager work
+            graph.launch()
+            got = results()
+            assert np.array_equal(got[0].view(np.uint8), want[0].view(np.uint8)), rep
+            assert np.array_equal(got[1], want[1]), rep
+            ns = np.minimum(want[1].reshape(n, 2), 8)
+            for i in range(n):
+                for s in (0, 1):
+                    assert fo.hits_equal(got[2][i, s, :ns[i, s]], want[2][i, s, :ns[i, s]]), (rep, i, s)
+            assert np.array_equal(got[3].full, want[3].full), rep
+            for i in range(n):
+                for s in (0, 1):
+                    assert fo.hitsk_equal(got[3].final(i, s), want[3].final(i, s)), (rep, i, s)
+                    assert fo.hitsk_equal(got[3].anchors(i, s), want[3].anchors(i, s)), (rep, i, s)
+        # the forked stream's scratch is pinned by the graph
+        with pytest.raises(hp.FmiError):
+            hp.search_k_launch(ix, batch, fs, 1024, anchor_cap, *big, stream=st[2].handle)
+        with pytest.raises(hp.FmiError):
+            st[2].close()
+    finally:
+        graph.close()
+    # destroying the graph lifts the pin
+    hp.search_k_launch(ix, batch, fs, 1024, anchor_cap, *big, stream=st[2].handle)
+    st[2].sync()
+    for b in [ex, fs] + k_out + big:
+        b.free()
+    out.free()
+    batch.free()
+    for x in st:
+        x.close()
+    ix.close()
