@@ -491,26 +491,39 @@ int fmi_index_kmer_table(fmi_index *ix, uint32_t k) {
     ix->kmer_bytes = 0;
     ix->dev.kmer[0] = ix->dev.kmer[1] = nullptr;
     ix->dev.kmer_k = 0;
+    ix->dev.kmer2[0] = ix->dev.kmer2[1] = nullptr;
+    ix->dev.kmer2_k = 0;
   }
   if (k == 0) return FMI_OK;
+  // the tables of k-strings and, from k = 8 on, of (k - 4)-strings
+  const uint32_t k2 = k >= 8 ? k - 4 : 0;
   const uint64_t per = (1ull << (2 * k)) * sizeof(uint64_t);
+  const uint64_t per2 = k2 ? (1ull << (2 * k2)) * sizeof(uint64_t) : 0;
   void *mem = nullptr;
-  hipError_t e = hipMalloc(&mem, 2 * per);
-  if (e != hipSuccess) return fail(FMI_ENOMEM, "hipMalloc(k-mer table, " + std::to_string(2 * per) + " bytes)");
-  uint64_t *t[2] = {static_cast<uint64_t *>(mem),
-                    reinterpret_cast<uint64_t *>(static_cast<char *>(mem) + per)};
-  for (int d = 0; d < 2 && e == hipSuccess; ++d)
+  hipError_t e = hipMalloc(&mem, 2 * (per + per2));
+  if (e != hipSuccess)
+    return fail(FMI_ENOMEM, "hipMalloc(k-mer table, " + std::to_string(2 * (per + per2)) + " bytes)");
+  char *base = static_cast<char *>(mem);
+  uint64_t *t[2] = {reinterpret_cast<uint64_t *>(base), reinterpret_cast<uint64_t *>(base + per)};
+  uint64_t *t2[2] = {k2 ? reinterpret_cast<uint64_t *>(base + 2 * per) : nullptr,
+                     k2 ? reinterpret_cast<uint64_t *>(base + 2 * per + per2) : nullptr};
+  for (int d = 0; d < 2 && e == hipSuccess; ++d) {
     e = fmi::launch_kmer_table(ix->dev.d[d], (uint32_t)ix->n, k, t[d], nullptr);
+    if (e == hipSuccess && k2) e = fmi::launch_kmer_table(ix->dev.d[d], (uint32_t)ix->n, k2, t2[d], nullptr);
+  }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     (void)hipFree(mem);
     return hip_fail(e, "k-mer table launch");
   }
   ix->d_kmer = mem;
-  ix->kmer_bytes = 2 * per;
+  ix->kmer_bytes = 2 * (per + per2);
   ix->dev.kmer[0] = t[0];
   ix->dev.kmer[1] = t[1];
   ix->dev.kmer_k = k;
+  ix->dev.kmer2[0] = t2[0];
+  ix->dev.kmer2[1] = t2[1];
+  ix->dev.kmer2_k = k2;
   return FMI_OK;
 }
 
@@ -519,7 +532,7 @@ int fmi_index_get_kmer_table(const fmi_index *ix, int forward, uint64_t *out) {
   if (rc) return rc;
   if (!out) return fail(FMI_EINVAL, "NULL argument");
   if (!ix->d_kmer) return fail(FMI_ESTATE, "call fmi_index_kmer_table first");
-  const uint64_t per = ix->kmer_bytes / 2;
+  const uint64_t per = (1ull << (2 * ix->dev.kmer_k)) * sizeof(uint64_t);
   HIP_TRY(hipMemcpy(out, ix->dev.kmer[forward ? 1 : 0], per, hipMemcpyDeviceToHost),
           "copy k-mer table");
   return FMI_OK;

@@ -60,6 +60,12 @@ struct DevIndex {
   // without a table.
   const uint64_t *kmer[2];
   uint32_t kmer_k;
+  // The same for strings of kmer2_k = kmer_k - 4 symbols (kmer_k >= 8), or
+  // NULL / 0: where a kmer_k-string's entry is empty or holds an N, the
+  // searches that must find where their interval empties take the first
+  // kmer2_k steps from it instead of one LF query each.
+  const uint64_t *kmer2[2];
+  uint32_t kmer2_k;
 };
 
 // Host-side view of one direction.

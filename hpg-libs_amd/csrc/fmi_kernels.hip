@@ -34,8 +34,9 @@ struct ExactParams {
 };
 
 // With a k-mer table the first k steps are one read of it when all k bases
-// are A/C/G/T and their interval is not empty (otherwise the steps are taken
-// one by one, so the position and interval where the search dies are the
+// are A/C/G/T and their interval is not empty, else the first k - 4 steps
+// from the smaller table the same way (otherwise the steps are taken one by
+// one, so the position and interval where the search dies are the
 // reference's).  With an expanded index, a search whose interval has
 // narrowed to one row goes on against the text at that row's suffix, 16
 // bases at a time; where it dies, one LF step from the last live row gives
@@ -52,8 +53,13 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   const int32_t d = P.direction ? 1 : -1;
   uint32_t k = 0, l = 0, k2 = 0, l2 = n, nb;
   int32_t i = P.direction ? 0 : len - 1;
-  const uint32_t K = P.ix.kmer_k;
-  if (K && len >= (int32_t)K) {
+  // the first K bases from the k-table, else the first K - 4 from the
+  // smaller one, else one LF step at a time
+  const uint32_t tks[2] = {P.ix.kmer_k, P.ix.kmer2_k};
+#pragma unroll
+  for (uint32_t tt = 0; tt < 2u; ++tt) {
+    const uint32_t K = tks[tt];
+    if (!K || len < (int32_t)K || k2 != 0u || l2 != n) continue;
     uint32_t code = 0u, bad = 0u;
     for (uint32_t j = 0; j < K; ++j) {
       const uint32_t s = W[i + (int32_t)j * d];
@@ -62,7 +68,7 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
     }
     if (P.comp) code ^= (1u << (2u * K)) - 1u;
     if (!bad) {
-      const uint64_t e = P.ix.kmer[P.dsel][code];
+      const uint64_t e = (tt ? P.ix.kmer2 : P.ix.kmer)[P.dsel][code];
       if (COUNT) wk++;
       if ((uint32_t)e <= (uint32_t)(e >> 32)) {
         k = k2 = (uint32_t)e;

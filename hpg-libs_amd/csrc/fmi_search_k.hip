@@ -815,8 +815,8 @@ enum : uint32_t { FP_EXACT = 0, FP_WALK1 = 1, FP_WALK2 = 2, FP_IDLE = 3, FP_UNIT
                   FP_RECORD = 5 };
 constexpr uint32_t kFQNone = 0xFFFFFFFFu;
 constexpr uint32_t kFSlotWords = 32;             // 24 record words, 8 restart-mask words
-constexpr uint32_t F_TABOK = 1u;                 // exact: the first k bases may be a table read
-constexpr uint32_t F_TABFAIL = 2u;               // walk: the table entry here was empty
+constexpr uint32_t F_TAB2FAIL = 1u;              // the (k - 4)-table cannot be used here
+constexpr uint32_t F_TABFAIL = 2u;               // the k-table cannot be used here
 constexpr uint32_t F_MOVED = 4u;                 // exact: the text run matched a base
 constexpr uint32_t F_ISA = 8u;                   // exact: the row after the run is to be read
 constexpr uint32_t F_RUNDIED = 16u;              // exact: ... and the run died
@@ -933,7 +933,7 @@ __device__ __forceinline__ void front_stage(const KParams &P, uint32_t *slot, FL
   L.la = 0u;
   L.q = kFQNone;
   L.r = 0u;
-  L.flags = F_TABOK;
+  L.flags = 0u;
   L.ph = FP_EXACT;
 }
 
@@ -999,7 +999,7 @@ __device__ __forceinline__ void front_step(const KParams &P, uint32_t *slot, FLa
 
   // ---- what to read
   uint32_t kind;
-  uint32_t lim = 0u, code = 0u, j1 = 0u, j2 = 0u;
+  uint32_t lim = 0u, code = 0u, j1 = 0u, j2 = 0u, tk = 0u;
   bool two = false, nextw = false;
   const uint32_t *a0 = nullptr;
   if (L.ph == FP_UNIT) {
@@ -1031,17 +1031,24 @@ __device__ __forceinline__ void front_step(const KParams &P, uint32_t *slot, FLa
     a0 = Dd.sa + L.k;
   } else {
     kind = FK_LF;
-    const bool whole = K && L.k == 0u && L.l == n && (uint32_t)L.cnt >= K;
-    const bool tab = exact ? (L.flags & F_TABOK) != 0u && whole
-                           : whole && !(L.flags & F_TABFAIL);
-    if (tab) {
-      const uint32_t nmask = (1u << K) - 1u;
-      if ((rb & nmask) == 0u) {
+    // From the whole index (the exact search's start, a walk's start or
+    // restart) the first k bases are one table read when they are A/C/G/T
+    // and their entry is not empty, else the first k - 4 from the smaller
+    // table, else LF queries one base at a time.
+    const uint32_t K2 = P.ix.kmer2_k;
+    if (K && L.k == 0u && L.l == n) {
+      if (!(L.flags & F_TABFAIL) && (uint32_t)L.cnt >= K && (rb & ((1u << K) - 1u)) == 0u) {
         kind = FK_TABLE;
+        tk = K;
         code = (uint32_t)rwc & ((1u << (2u * K)) - 1u);
         a0 = reinterpret_cast<const uint32_t *>((L.dsel ? P.ix.kmer[1] : P.ix.kmer[0]) + code);
+      } else if (K2 && !(L.flags & F_TAB2FAIL) && (uint32_t)L.cnt >= K2 &&
+                 (rb & ((1u << K2) - 1u)) == 0u) {
+        kind = FK_TABLE;
+        tk = K2;
+        code = (uint32_t)rwc & ((1u << (2u * K2)) - 1u);
+        a0 = reinterpret_cast<const uint32_t *>((L.dsel ? P.ix.kmer2[1] : P.ix.kmer2[0]) + code);
       }
-      if (exact) L.flags &= ~F_TABOK;                 // tried once, at the start
     }
     if (kind == FK_LF && !exact && textmode && L.l > L.k && L.l - L.k < kMRows) {
       kind = FK_MSA;                                  // the rows' suffixes, then their text
@@ -1291,14 +1298,14 @@ __device__ __forceinline__ void front_step(const KParams &P, uint32_t *slot, FLa
       L.k = ek;
       L.l = el;
       if (exact) { L.ka = ek; L.la = el; }
-      L.i += (int32_t)K * L.step;
-      L.cnt -= (int32_t)K;
+      L.i += (int32_t)tk * L.step;
+      L.cnt -= (int32_t)tk;
+      L.flags &= ~(F_TABFAIL | F_TAB2FAIL);
       if (L.cnt == 0) ended = true;
-    } else if (!exact) {
-      L.flags |= F_TABFAIL;                           // step this position one base at a time
-      return;
     } else {
-      return;                                         // exact: LF steps from the whole index
+      // the string is absent: the smaller table, or one base at a time
+      L.flags |= tk == K ? F_TABFAIL : F_TAB2FAIL;
+      return;
     }
   } else {
     // LF of W[i] on [k, l] (an N gives the empty (n + 1, n))
@@ -1321,7 +1328,7 @@ __device__ __forceinline__ void front_step(const KParams &P, uint32_t *slot, FLa
         ln = c + occ_blk(B2, Dd.dollar_blk, Dd.dollar_off, j2, s, t2);
       }
     }
-    L.flags &= ~F_TABFAIL;
+    L.flags &= ~(F_TABFAIL | F_TAB2FAIL);
     if (exact) {
       L.k = kn;
       L.l = ln;

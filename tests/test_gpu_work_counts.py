@@ -68,7 +68,8 @@ def test_exact_work_index_forms(wsetup, form):
     """On the expanded index the searches that narrow to one row go on in
     the text (one SA read, text compares, one ISA read); with a k-mer table
     every read of at least k bases free of N in its first k reads one table
-    entry."""
+    entry, and one more, of the (k - 4)-table, when that k-mer is absent
+    from the text or holds an N and its first k - 4 bases do not."""
     hp, g, reads, offs, lens, batch = wsetup
     ix = hp.Index.build(g, 16).to_device().expand()
     if form == "kmer8":
@@ -82,9 +83,25 @@ def test_exact_work_index_forms(wsetup, form):
             assert text > 0 and sa_isa > 0 and sa_isa % 2 == 0
             assert lf <= blocks <= 2 * lf
             if form == "kmer8":
-                first = [reads[offs[i]:offs[i + 1]] for i in range(batch.nreads)]
-                first = [W[-8:] if d == 0 else W[:8] for W in first if len(W) >= 8]
-                assert table == sum(1 for x in first if np.all(x <= 3))
+                ws = [reads[offs[i]:offs[i + 1]] for i in range(batch.nreads)]
+                clean8 = sum(1 for W in ws if len(W) >= 8 and np.all((W[-8:] if d == 0 else W[:8]) <= 3))
+                if d == 0:
+                    # the + strand's backward search takes W[-1], W[-2], ...: its
+                    # k-mer's interval is empty exactly when W[-8:] is absent
+                    t = g.tobytes()
+                    want = 0
+                    for W in ws:
+                        hit8 = False
+                        if len(W) >= 8:
+                            k8 = W[-8:]
+                            ok8 = bool(np.all(k8 <= 3))
+                            want += ok8
+                            hit8 = ok8 and t.find(k8.tobytes()) >= 0
+                        if not hit8 and len(W) >= 4 and np.all(W[-4:] <= 3):
+                            want += 1
+                    assert table == want
+                else:
+                    assert table >= clean8
             else:
                 assert table == 0
     finally:
