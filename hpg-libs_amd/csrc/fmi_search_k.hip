@@ -27,7 +27,10 @@ namespace {
 
 constexpr uint32_t kFrontPasses = 3;
 constexpr uint32_t kFrontLen = 256;
-
+// A lane's D: FMI_MAX_READ_LEN bytes, or the front passes' D side by side
+// (kFrontPasses x kFrontLen), 16-byte aligned.
+constexpr uint32_t kDStride = 1024;
+static_assert(kDStride >= FMI_MAX_READ_LEN && kDStride >= kFrontPasses * kFrontLen, "D stride");
 struct KList {
   fmi_hitk *list;
   uint32_t n, cap;
@@ -49,7 +52,8 @@ struct KCtx {
   uint8_t *D;
   bool text;                         // expanded index: single rows step on the text
   // the exact searches and D of the first passes, computed beforehand: pass
-  // b's D at fd + b * fstep * kFrontLen, its r and anchor at fr[2 * b * fstep]
+  // b's D at fd + b * kFrontLen (expanded into the lane's scratch when it
+  // takes the task), its r and anchor at fr[2 * b * fstep]
   const uint8_t *fd;
   const uint4 *fr;
   uint32_t fstep;
@@ -704,7 +708,7 @@ __device__ __forceinline__ bool k_pass(const KCtx &c, KList &prev, KList &next, 
     if (dx == 0) { ra.start = (int16_t)(pos + 1); ra.end = r.pos; }
     else { ra.start = r.pos; ra.end = (int16_t)(pos - 1); }
     r.k = a.x; r.l = a.y; r.pos = pos;
-    cf.D = const_cast<uint8_t *>(c.fd) + (uint64_t)pb * c.fstep * kFrontLen;
+    cf.D = const_cast<uint8_t *>(c.fd) + (uint32_t)pb * kFrontLen;
   } else {
     k_exact(c, X, dx, r, &ra);
   }
@@ -784,10 +788,9 @@ struct KParams {
   uint32_t *nfin, *nanchors;
   uint8_t *full;
   fmi_hitk *scratch;                 // per lane: 4 lists of list_cap records
-  uint8_t *dscratch;                 // per lane: FMI_MAX_READ_LEN bytes of D
+  uint8_t *dscratch;                 // per lane: kDStride bytes of D
   uint32_t *counter;
   unsigned long long *work;          // counting launch: KW_N counters
-  uint8_t *front_d;                  // per (pass < kFrontPasses, task): kFrontLen bytes of D
   uint4 *front_r;                    // per (pass, task): the exact search's r and anchor
   uint4 *front_m;                    // per (pass, task): the walks' restart marks
   const uint4 *front_rec;            // the read records (kReadRecordWords words per read)
@@ -1424,43 +1427,38 @@ __global__ __launch_bounds__(256) void search_k_front_kernel(KParams P) {
     for (uint32_t i = 0; i < KW_N; ++i) atomicAdd(&P.work[i], (unsigned long long)wk[i]);
 }
 
-// D of every unit whose r is live: one thread per 64 positions (two words
-// of marks).
-__global__ __launch_bounds__(256) void search_k_front_d_kernel(KParams P) {
-  const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-  const uint64_t u = gid >> 2;
-  const uint32_t part = (uint32_t)(gid & 3u);
-  if (u >= (uint64_t)P.ntasks * kFrontPasses) return;
+// D of unit u (if its r is live) into dst[0 .. nW), from its marks: done by
+// the main kernel's lane when it takes the task, so that the D the pass reads
+// is freshly written into the lane's own scratch.
+__device__ __forceinline__ void front_expand_d(const KParams &P, uint64_t u, uint8_t *dst) {
   const uint4 e = P.front_r[2ull * u + 1ull];
+  if (!(e.z & (1u << 17))) return;
   const int32_t nW = (int32_t)(e.z & 0xFFFFu);
-  if (!(e.z & (1u << 17)) || 64 * (int32_t)part >= nW) return;
   const uint4 a = P.front_m[2ull * u], b = P.front_m[2ull * u + 1ull];
   const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   const bool fwd = ((e.z >> 16) & 1u) != 0u;
   const int32_t p1 = (int32_t)e.x, f0 = (int32_t)(e.y & 0xFFFFu), f1 = (int32_t)(e.y >> 16);
   // the second walk lies below f0 (forward pass) or from f1 + 1 on
   const int32_t xb = fwd ? f0 : f1 + 1;
-  uint32_t tot = 0u, pre = 0u, below = 0u, mw0 = 0u, mw1 = 0u;
+  uint32_t tot = 0u, below = 0u;
 #pragma unroll
   for (uint32_t j = 0; j < 8u; ++j) {
     const uint32_t c = (uint32_t)__builtin_popcount(m[j]);
     tot += c;
-    if (j < 2u * part) pre += c;
-    if (j == 2u * part) mw0 = m[j];
-    if (j == 2u * part + 1u) mw1 = m[j];
     if ((int32_t)(32u * j + 32u) <= xb) below += c;
     else if ((int32_t)(32u * j) < xb)
       below += (uint32_t)__builtin_popcount(m[j] & ((1u << ((uint32_t)xb & 31u)) - 1u));
   }
   const uint32_t cntB = fwd ? below : tot - below;
-  const uint32_t pre1 = pre + (uint32_t)__builtin_popcount(mw0);
-  uint8_t *D = P.front_d + u * kFrontLen;
+  uint32_t pre = 0u;
 #pragma unroll
-  for (int32_t cc = 0; cc < 4; ++cc) {
-    const int32_t c = 4 * (int32_t)part + cc;
-    if (16 * c >= nW) break;
-    *reinterpret_cast<uint4 *>(D + 16ull * (uint32_t)c) =
-        front_d_chunk(cc < 2 ? pre : pre1, cc < 2 ? mw0 : mw1, tot, cntB, fwd, p1, c);
+  for (uint32_t w = 0; w < 8u; ++w) {
+    if ((int32_t)(32u * w) < nW)
+      *reinterpret_cast<uint4 *>(dst + 32u * w) = front_d_chunk(pre, m[w], tot, cntB, fwd, p1, (int32_t)(2u * w));
+    if ((int32_t)(32u * w + 16u) < nW)
+      *reinterpret_cast<uint4 *>(dst + 32u * w + 16u) =
+          front_d_chunk(pre, m[w], tot, cntB, fwd, p1, (int32_t)(2u * w + 1u));
+    pre += (uint32_t)__builtin_popcount(m[w]);
   }
 }
 
@@ -1492,11 +1490,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
     c.text = P.ix.text != nullptr && P.ix.d[0].sa != nullptr && P.ix.d[1].sa != nullptr;
     c.W = P.reads + off;
     c.nW = len;
-    c.D = P.dscratch + lane * (uint64_t)FMI_MAX_READ_LEN;
-    const bool fr = P.front_d != nullptr && len <= (int32_t)kFrontLen;
-    c.fd = fr ? P.front_d + (uint64_t)t * kFrontLen : nullptr;
+    c.D = P.dscratch + lane * (uint64_t)kDStride;
+    const bool fr = P.front_r != nullptr && len <= (int32_t)kFrontLen;
+    c.fd = fr ? c.D : nullptr;
     c.fr = P.front_r + 2ull * t;
     c.fstep = P.ntasks;
+    if (fr) {
+      // D of the front's passes, pass b at D + b * kFrontLen
+      const int32_t nb = min(len / fs, (int32_t)kFrontPasses);
+      for (int32_t b = 0; b < nb; ++b)
+        front_expand_d(P, (uint64_t)b * P.ntasks + t, c.D + (uint32_t)b * kFrontLen);
+    }
     KList prev{sl, 0, P.list_cap}, next{sl + P.list_cap, 0, P.list_cap};
     KList prev_i{sl + 2 * P.list_cap, 0, P.list_cap}, next_i{sl + 3 * P.list_cap, 0, P.list_cap};
     KList fin{P.fin + (uint64_t)t * P.list_cap, 0, P.list_cap};
@@ -1513,17 +1517,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
 }  // namespace
 
 uint64_t search_k_lane_bytes(uint32_t list_cap) {
-  return 4ull * list_cap * sizeof(fmi_hitk) + FMI_MAX_READ_LEN;
+  return 4ull * list_cap * sizeof(fmi_hitk) + kDStride;
 }
 
 uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap) {
   return (uint64_t)lanes * search_k_lane_bytes(list_cap);
 }
 
-// The front kernel's scratch: D (kFrontLen bytes) and r / anchor (32 bytes)
-// per unit, then the read records, the list of longer reads and its length.
+// The front kernel's scratch: r / anchor (32 bytes) and the restart marks
+// (32 bytes) per unit, then the read records, the list of longer reads and
+// its length.
 uint64_t search_k_front_bytes(uint32_t nreads) {
-  return (uint64_t)kFrontPasses * 2ull * nreads * (kFrontLen + 64ull) +
+  return (uint64_t)kFrontPasses * 2ull * nreads * 64ull +
          (uint64_t)nreads * (4ull * kReadRecordWords + 4ull) + 64ull;
 }
 
@@ -1577,7 +1582,6 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
   P.list_cap = list_cap;
   P.anchor_cap = anchor_cap;
   P.strand_mask = strand_mask;
-  P.front_d = nullptr;
   P.front_r = nullptr;
   P.front_m = nullptr;
   P.front_rec = nullptr;
@@ -1587,10 +1591,9 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
     // the first passes' exact searches and D, from the read records
     uint8_t *fb = static_cast<uint8_t *>(scratch) + search_k_scratch_bytes(lanes, list_cap);
     const uint64_t units = (uint64_t)kFrontPasses * 2ull * nreads;
-    P.front_d = fb;
-    P.front_r = reinterpret_cast<uint4 *>(fb + units * kFrontLen);
-    P.front_m = reinterpret_cast<uint4 *>(fb + units * (kFrontLen + 32ull));
-    uint32_t *rec = reinterpret_cast<uint32_t *>(fb + units * (kFrontLen + 64ull));
+    P.front_r = reinterpret_cast<uint4 *>(fb);
+    P.front_m = reinterpret_cast<uint4 *>(fb + units * 32ull);
+    uint32_t *rec = reinterpret_cast<uint32_t *>(fb + units * 64ull);
     uint32_t *list = rec + (uint64_t)kReadRecordWords * nreads;
     uint32_t *list_n = list + nreads;
     e = launch_zero(list_n, 1, s);
@@ -1601,10 +1604,6 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
     P.front_rec = reinterpret_cast<const uint4 *>(rec);
     if (work) hipLaunchKernelGGL(search_k_front_kernel<true>, dim3(fl / 256u), dim3(256), 0, s, P);
     else hipLaunchKernelGGL(search_k_front_kernel<false>, dim3(fl / 256u), dim3(256), 0, s, P);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(search_k_front_d_kernel, dim3((uint32_t)((units * 4ull + 255ull) / 256ull)),
-                       dim3(256), 0, s, P);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
