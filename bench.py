@@ -136,19 +136,20 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     eager = (time.perf_counter() - t0) / args.steps
     log(f"mixed budgets, one stream, launched one by one: {eager * 1000:.2f} ms per step, "
         f"{n / eager / 1e6:.2f} M reads/s")
-    # The three groups' launches are independent of each other: a step runs
-    # them side by side on three streams, forked from the first and joined
-    # back into it (fmi_stream_wait) so that a step ends when all three
-    # have; timed launched directly and captured once as one graph of three
-    # branches replayed with one call (fmi_graph_*); the faster is the step.
-    st = [hpgfm.Stream() for _ in range(3)]
+    # The groups' launches are independent of each other: a step runs the
+    # k-error search on a stream of its own beside the exact and the 1-error
+    # search, forked from the first stream and joined back into it
+    # (fmi_stream_wait) so that a step ends when both have; timed launched
+    # directly and captured once as one graph of two branches replayed with
+    # one call (fmi_graph_*); the faster is the step.  (Each group on a
+    # stream of its own measured 8.2 ms, the exact then the 1-error search
+    # on one stream beside the k-error search 8.0 ms, all on one 8.6 ms.)
+    st = [hpgfm.Stream() for _ in range(2)]
 
     def step():
         st[1].wait(st[0])
-        st[2].wait(st[0])
-        run_exact(st[0].handle); run_s1(st[1].handle); run_k(st[2].handle)
+        run_exact(st[0].handle); run_s1(st[0].handle); run_k(st[1].handle)
         st[0].wait(st[1])
-        st[0].wait(st[2])
 
     step()                               # sizes each stream's scratch
     st[0].sync()
@@ -160,7 +161,8 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     for _ in range(args.steps):
         step()
     st[0].sync()
-    timed["three streams, forked and joined per step"] = (time.perf_counter() - t0) / args.steps
+    timed["two streams (exact then 1-error | k-error), forked and joined per step"] = \
+        (time.perf_counter() - t0) / args.steps
     graph = hpgfm.Graph(st[0])
     with graph:
         step()
@@ -171,7 +173,7 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     for _ in range(args.steps):
         graph.launch()
     st[0].sync()
-    timed["one graph of three branches per step"] = (time.perf_counter() - t0) / args.steps
+    timed["one graph of two branches per step"] = (time.perf_counter() - t0) / args.steps
     graph.close()
     for x in st:
         x.close()
