@@ -794,6 +794,7 @@ struct KParams {
   uint4 *front_r;                    // per (pass, task): the exact search's r and anchor
   uint4 *front_m;                    // per (pass, task): the walks' restart marks
   const uint4 *front_rec;            // the read records (kReadRecordWords words per read)
+  uint32_t *tasks, *ntasks_dev;      // when set: the main kernel's tasks are tasks[0 .. *ntasks_dev)
   uint32_t ntasks, list_cap, anchor_cap, strand_mask;
 };
 
@@ -1462,14 +1463,76 @@ __device__ __forceinline__ void front_expand_d(const KParams &P, uint64_t u, uin
   }
 }
 
+// After the front: a task whose every pass the front ran and whose exact
+// fragment search died in each of them (most tasks of the strand a read was
+// not drawn from) has no partial to extend: its result is the passes'
+// anchors (add_result, results.h:93-103, up to anchor_cap; a full anchor
+// list ends the search as BWSearchCPU's return 1) and an empty final list,
+// written here; every other task goes to the main kernel's list.
+__global__ __launch_bounds__(256) void search_k_triage_kernel(KParams P) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= P.ntasks) return;
+  const uint32_t rd = t >> 1, strand = t & 1u;
+  const uint64_t off = P.offsets[rd];
+  const int32_t len = (int32_t)(P.offsets[rd + 1] - off);
+  const int32_t fs = P.fragsize[rd];
+  bool trivial = false;
+  int32_t fragments = 0;
+  if ((P.strand_mask & (1u << strand)) && fs > 0 && len <= (int32_t)kFrontLen &&
+      len / fs - 1 <= FMI_KMAX) {
+    fragments = len / fs;
+    if (fragments >= 2 && fragments <= (int32_t)kFrontPasses) {
+      trivial = true;
+      for (int32_t b = 0; b < fragments; ++b) {
+        const uint4 a = P.front_r[2ull * ((uint64_t)b * P.ntasks + t)];
+        if (a.x <= a.y) trivial = false;              // r live: partials to extend
+      }
+    }
+  }
+  if (!trivial) {
+    P.tasks[atomicAdd(P.ntasks_dev, 1u)] = t;
+    return;
+  }
+  int32_t half = fragments / 2;
+  if (fragments % 2) half++;
+  uint32_t na = 0u;
+  bool full = false;
+  for (int32_t b = 0; b < fragments; ++b) {
+    if (!P.anchors) break;
+    if (na >= P.anchor_cap) { full = true; break; }
+    const bool fwd = b < half;
+    const int32_t i = b < half - 1 ? half - 1 - b : (b == half - 1 ? 0 : b);
+    const int32_t start = fs * i;
+    const int32_t end = fwd || i < fragments - 1 ? fs * (i + 1) - 1 : len - 1;
+    const uint64_t u = (uint64_t)b * P.ntasks + t;
+    const uint4 a = P.front_r[2ull * u];
+    const int16_t pos = (int16_t)P.front_r[2ull * u + 1ull].x;
+    fmi_hitk ra;
+    k_clear(ra);
+    ra.dir = (uint8_t)(fwd ? 1 : 0);
+    ra.k = a.z;
+    ra.l = a.w;
+    ra.pos = pos;
+    if (!fwd) { ra.start = (int16_t)(pos + 1); ra.end = (int16_t)end; }
+    else { ra.start = (int16_t)start; ra.end = (int16_t)(pos - 1); }
+    P.anchors[(uint64_t)t * P.anchor_cap + na] = ra;
+    na++;
+  }
+  P.nfin[t] = 0;
+  if (P.anchors) P.nanchors[t] = na;
+  P.full[t] = full ? 1 : 0;
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_k_kernel(KParams P) {
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   fmi_hitk *sl = P.scratch + lane * 4ull * P.list_cap;
   uint32_t wk[KW_N] = {0, 0, 0, 0, 0, 0};
+  const uint32_t ntasks = P.tasks ? *P.ntasks_dev : P.ntasks;
   while (true) {
-    const uint32_t t = atomicAdd(P.counter, 1u);
-    if (t >= P.ntasks) break;
+    const uint32_t q = atomicAdd(P.counter, 1u);
+    if (q >= ntasks) break;
+    const uint32_t t = P.tasks ? P.tasks[q] : q;
     const uint32_t rd = t >> 1, strand = t & 1u;
     const uint64_t off = P.offsets[rd];
     const int32_t len = (int32_t)(P.offsets[rd + 1] - off);
@@ -1529,7 +1592,7 @@ uint64_t search_k_scratch_bytes(uint32_t lanes, uint32_t list_cap) {
 // its length.
 uint64_t search_k_front_bytes(uint32_t nreads) {
   return (uint64_t)kFrontPasses * 2ull * nreads * 64ull +
-         (uint64_t)nreads * (4ull * kReadRecordWords + 4ull) + 64ull;
+         (uint64_t)nreads * (4ull * kReadRecordWords + 4ull + 8ull) + 64ull;
 }
 
 uint32_t search_k_front_lanes_per_cu() {
@@ -1585,6 +1648,8 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
   P.front_r = nullptr;
   P.front_m = nullptr;
   P.front_rec = nullptr;
+  P.tasks = nullptr;
+  P.ntasks_dev = nullptr;
   hipError_t e = launch_zero(counter, 1, s);
   if (e != hipSuccess) return e;
   if (front_lanes) {
@@ -1596,7 +1661,9 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
     uint32_t *rec = reinterpret_cast<uint32_t *>(fb + units * 64ull);
     uint32_t *list = rec + (uint64_t)kReadRecordWords * nreads;
     uint32_t *list_n = list + nreads;
-    e = launch_zero(list_n, 1, s);
+    P.ntasks_dev = list_n + 1;
+    P.tasks = list_n + 4;                 // 2 x nreads entries
+    e = launch_zero(list_n, 2, s);
     if (e != hipSuccess) return e;
     e = launch_read_records(reads, offsets, nreads, rec, list, list_n, s);
     if (e != hipSuccess) return e;
@@ -1604,6 +1671,9 @@ hipError_t launch_search_k(const DevIndex &ix, const uint8_t *reads, const uint6
     P.front_rec = reinterpret_cast<const uint4 *>(rec);
     if (work) hipLaunchKernelGGL(search_k_front_kernel<true>, dim3(fl / 256u), dim3(256), 0, s, P);
     else hipLaunchKernelGGL(search_k_front_kernel<false>, dim3(fl / 256u), dim3(256), 0, s, P);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(search_k_triage_kernel, dim3((2u * nreads + 255u) / 256u), dim3(256), 0, s, P);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
