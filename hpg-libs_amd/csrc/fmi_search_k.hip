@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdint>
 
 #include "../../include/fmi.h"
@@ -1469,6 +1470,10 @@ __device__ __forceinline__ void front_expand_d(const KParams &P, uint64_t u, uin
 // anchors (add_result, results.h:93-103, up to anchor_cap; a full anchor
 // list ends the search as BWSearchCPU's return 1) and an empty final list,
 // written here; every other task goes to the main kernel's list.
+static_assert(sizeof(fmi_hitk) == 64 && offsetof(fmi_hitk, start) == 16 &&
+                  offsetof(fmi_hitk, pos) == 18 && offsetof(fmi_hitk, end) == 20 &&
+                  offsetof(fmi_hitk, dir) == 22 && offsetof(fmi_hitk, num_mismatches) == 23,
+              "fmi_hitk layout");
 __global__ __launch_bounds__(256) void search_k_triage_kernel(KParams P) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   if (t >= P.ntasks) return;
@@ -1507,15 +1512,16 @@ __global__ __launch_bounds__(256) void search_k_triage_kernel(KParams P) {
     const uint64_t u = (uint64_t)b * P.ntasks + t;
     const uint4 a = P.front_r[2ull * u];
     const int16_t pos = (int16_t)P.front_r[2ull * u + 1ull].x;
-    fmi_hitk ra;
-    k_clear(ra);
-    ra.dir = (uint8_t)(fwd ? 1 : 0);
-    ra.k = a.z;
-    ra.l = a.w;
-    ra.pos = pos;
-    if (!fwd) { ra.start = (int16_t)(pos + 1); ra.end = (int16_t)end; }
-    else { ra.start = (int16_t)start; ra.end = (int16_t)(pos - 1); }
-    P.anchors[(uint64_t)t * P.anchor_cap + na] = ra;
+    // the anchor k_exact leaves (k_clear'ed, no errors), as four 16-byte
+    // stores: k, l (k, l < 2^32), start, pos, end, dir, num_mismatches = 0
+    const int16_t as = fwd ? (int16_t)start : (int16_t)(pos + 1);
+    const int16_t ae = fwd ? (int16_t)(pos - 1) : (int16_t)end;
+    uint4 *dst = reinterpret_cast<uint4 *>(P.anchors + (uint64_t)t * P.anchor_cap + na);
+    dst[0] = make_uint4(a.z, 0u, a.w, 0u);
+    dst[1] = make_uint4((uint32_t)(uint16_t)as | ((uint32_t)(uint16_t)pos << 16),
+                        (uint32_t)(uint16_t)ae | ((fwd ? 1u : 0u) << 16), 0u, 0u);
+    dst[2] = make_uint4(0u, 0u, 0u, 0u);
+    dst[3] = make_uint4(0u, 0u, 0u, 0u);
     na++;
   }
   P.nfin[t] = 0;
