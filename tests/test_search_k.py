@@ -204,3 +204,43 @@ def test_gpu_search_k_vs_oracle(gpu_engine, expanded, kmer):
                 assert fo.hitsk_equal(res.anchors(i, s), a), (lc, i, s)
     batch.free()
     ix.close()
+
+
+@pytest.mark.gpu
+def test_gpu_search_k_front_and_triage_paths(gpu_engine):
+    """The paths around the front kernel and the triage: reads of at most
+    and of more than 256 bases in one batch (the front covers the first),
+    two and three fragments (a task whose every exact fragment search dies is
+    finished by the triage), anchor lists of one and two records (the
+    triage's add_result on a full list ends the search with full = 1) and
+    each strand alone, against the oracle task by task."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(200_000, seed=81, repeat_frac=0.3, fam_len=(300, 2000),
+                           divergence=(0.0, 0.05))
+    ix = hp.Index.build(g, 16).to_device().expand()
+    ix.kmer_table(9)
+    o = fo.Oracle(g, ratio=16)
+    rng = np.random.default_rng(82)
+    lens = np.concatenate([rng.integers(30, 256, 500), rng.integers(257, 400, 60)])
+    rng.shuffle(lens)
+    reads, offs, _ = synth.reads(g, len(lens), seed=83, sub_rate=0.02, n_rate=0.003,
+                                 lengths=lens, random_frac=0.2)
+    fs = np.where(rng.random(len(lens)) < 0.5, lens // 3, lens // 2).astype(np.int16)
+    batch = hp.ReadBatch(reads, offs)
+    try:
+        for lc, ac, mask in ((128, 1, 3), (4, 2, 3), (64, 16, 1), (64, 16, 2)):
+            res = hp.search_k_batch(ix, batch, fs, list_cap=lc, anchor_cap=ac, strand_mask=mask)
+            for i in range(batch.nreads):
+                W = reads[offs[i]:offs[i + 1]]
+                for s in (0, 1):
+                    if not mask & (1 << s):
+                        assert res.full[i, s] == 0 and len(res.final(i, s)) == 0, (lc, ac, i, s)
+                        assert len(res.anchors(i, s)) == 0, (lc, ac, i, s)
+                        continue
+                    full, f, a = o.search_k(s, W, int(fs[i]), lc, ac)
+                    assert bool(res.full[i, s]) == full, (lc, ac, mask, i, s)
+                    assert fo.hitsk_equal(res.final(i, s), f), (lc, ac, mask, i, s)
+                    assert fo.hitsk_equal(res.anchors(i, s), a), (lc, ac, mask, i, s)
+    finally:
+        batch.free()
+        ix.close()
