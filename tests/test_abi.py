@@ -104,6 +104,26 @@ def test_graph_arguments():
     assert L.fmi_graph_end(None, C.byref(g)) == -1
     assert L.fmi_graph_launch(None, None) == -1
     assert L.fmi_graph_destroy(None) == 0
+    # fork / join: two different created streams, checked before any HIP call
+    assert L.fmi_stream_wait(None, None) == -1
+    h = C.c_void_p(0x1000)
+    assert L.fmi_stream_wait(h, h) == -1
+
+
+def test_work_count_arguments():
+    """The counting calls reject a missing work array, and an index that is
+    not on the device, before any HIP call."""
+    g = np.random.default_rng(1).integers(0, 4, 1000).astype(np.uint8)
+    ix = hpgfm.Index.build(g, 8)
+    L = hpgfm.lib()
+    assert L.fmi_search_k_count_work(ix.h, 3, None, None, None, 1, 8, 4, None, None, None, None,
+                                     None, None, None) == -1
+    w = np.zeros(6, np.uint64)
+    rc = L.fmi_search_k_count_work(ix.h, 3, None, None, None, 1, 8, 4, None, None, None, None,
+                                   None, None, w.ctypes.data)
+    assert rc == -7                                        # FMI_ESTATE: not on the device
+    rc = L.fmi_exact_count_work(ix.h, 0, 0, None, None, 1, None, None, None, w.ctypes.data)
+    assert rc == -7
 
 
 def test_read_validation():
