@@ -58,6 +58,9 @@ struct KCtx {
   const uint8_t *fd;
   const uint4 *fr;
   uint32_t fstep;
+  // the read's codes and N flags in the lane's LDS slot (words j at
+  // slot[j * 256]), or null: read bytes from W
+  const uint32_t *slot;
 };
 
 enum : uint32_t { KW_BLOCKS = 0, KW_LF = 1, KW_SAISA = 2, KW_TEXT = 3, KW_TABLE = 4, KW_LIST = 5,
@@ -66,6 +69,55 @@ enum : uint32_t { KW_BLOCKS = 0, KW_LF = 1, KW_SAISA = 2, KW_TEXT = 3, KW_TABLE 
   do {                                          \
     if ((c).wk) (c).wk[(i)] += (uint32_t)(v);   \
   } while (0)
+
+// The 32 read bases from W[p] in feed order (rw, 2-bit codes) and their N
+// flags (rb), from the lane's slot (words j at slot[j * 256]); the window
+// starts at p (d > 0) or p - 31 (then reversed); words outside the slot are
+// clamped, which changes only bases outside the read, never compared.
+__device__ __forceinline__ void fslot_window(const uint32_t *slot, int32_t p, int32_t d,
+                                             uint64_t &rw, uint32_t &rb) {
+  const uint32_t u = (uint32_t)((d > 0 ? p : p - 31) + 32);
+  const int32_t cw = (int32_t)(u >> 4) - 2, fw = (int32_t)(u >> 5) - 1;
+  const uint32_t sh = (u & 15u) << 1, fsh = u & 31u;
+  const uint32_t i0 = (uint32_t)min(max(cw, 0), 15), i1 = (uint32_t)min(max(cw + 1, 0), 15);
+  const uint32_t i2 = (uint32_t)min(max(cw + 2, 0), 15);
+  const uint32_t j0 = 16u + (uint32_t)min(max(fw, 0), 7), j1 = 16u + (uint32_t)min(max(fw + 1, 0), 7);
+  const uint32_t w0 = slot[i0 * 256u], w1 = slot[i1 * 256u], w2 = slot[i2 * 256u];
+  const uint32_t g0 = slot[j0 * 256u], g1 = slot[j1 * 256u];
+  const uint64_t lo = ((uint64_t)w1 << 32) | w0;
+  const uint64_t win = sh ? (lo >> sh) | ((uint64_t)w2 << (64u - sh)) : lo;
+  const uint32_t fl = fsh ? (g0 >> fsh) | (g1 << (32u - fsh)) : g0;
+  if (d > 0) {
+    rw = win;
+    rb = fl;
+  } else {
+    const uint64_t v = __builtin_bitreverse64(win);
+    rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+    rb = __builtin_bitreverse32(fl);
+  }
+}
+
+// The 32 text bases before suffix q in LF order (BWT of X: X[q-1], X[q-2],
+// ...; of reverse X: X[n-q], X[n-q+1], ...) from the words t0, t1, t2 that
+// start at word w (see front_step).
+__device__ __forceinline__ uint64_t ftext_window(uint32_t dsel, uint32_t n, uint32_t q, uint32_t t0,
+                                                 uint32_t t1, uint32_t t2) {
+  const uint64_t lo = ((uint64_t)t1 << 32) | t0;
+  if (dsel) {
+    const uint32_t sh = ((n - q) & 15u) << 1;
+    return sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
+  }
+  const int64_t b = (int64_t)q - 32;
+  uint64_t win;
+  if (b >= 0) {
+    const uint32_t sh = ((uint32_t)b & 15u) << 1;
+    win = sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
+  } else {
+    win = lo << min(2u * (uint32_t)(-b), 62u);
+  }
+  const uint64_t v = __builtin_bitreverse64(win);
+  return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+}
 
 // On an expanded index a one-row interval is stepped on the text: the LF
 // steps of W[i], W[i + step], ... (count positions) from the row whose
@@ -79,6 +131,45 @@ __device__ __forceinline__ int32_t k_run(const KCtx &c, const KView &v, uint32_t
                                          bool &dies) {
   int32_t done = 0;
   dies = false;
+  if (c.slot) {
+    // 32 bases per compare, the read's from the LDS slot
+    const uint32_t n = c.ix->n, nwords = (n + 15u) >> 4;
+    while (done < count) {
+      uint32_t lim = (uint32_t)(count - done) < 32u ? (uint32_t)(count - done) : 32u;
+      if (lim > q) lim = q;
+      if (lim == 0u) { dies = true; break; }
+      uint64_t rw;
+      uint32_t rb;
+      fslot_window(c.slot, i + done * step, step, rw, rb);
+      uint32_t w;
+      if (v.dsel) {
+        w = min((n - q) >> 4, nwords - 1u);
+      } else {
+        const int64_t b = (int64_t)q - 32;
+        w = b >= 0 ? (uint32_t)b >> 4 : 0u;
+      }
+      const uint32_t t0 = c.ix->text[w];
+      const uint32_t t1 = w + 1u < nwords ? c.ix->text[w + 1u] : 0u;
+      const uint32_t t2 = w + 2u < nwords ? c.ix->text[w + 2u] : 0u;
+      const uint64_t tw = ftext_window(v.dsel, n, q, t0, t1, t2);
+      uint64_t spread = rb;                           // N flag of base j -> bit 2j
+      spread = (spread | (spread << 16)) & 0x0000FFFF0000FFFFull;
+      spread = (spread | (spread << 8)) & 0x00FF00FF00FF00FFull;
+      spread = (spread | (spread << 4)) & 0x0F0F0F0F0F0F0F0Full;
+      spread = (spread | (spread << 2)) & 0x3333333333333333ull;
+      spread = (spread | (spread << 1)) & 0x5555555555555555ull;
+      const uint64_t x = (v.comp ? ~rw : rw) ^ tw;
+      uint64_t miss = ((x | (x >> 1)) & 0x5555555555555555ull) | spread;
+      if (lim < 32u) miss |= 1ull << (2u * lim);
+      const uint32_t m = miss ? (uint32_t)__builtin_ctzll(miss) >> 1 : 32u;
+      KWORK(c, KW_TEXT, 1);
+      done += (int32_t)m;
+      q -= m;
+      if (m < lim) { dies = true; break; }
+    }
+    q_end = q;
+    return done;
+  }
   while (done < count) {
     uint32_t lim = (uint32_t)(count - done) < 16u ? (uint32_t)(count - done) : 16u;
     if (lim > q) lim = q;
@@ -132,10 +223,18 @@ __device__ __forceinline__ bool k_table(const KCtx &c, const KView &v, int32_t i
                                         int64_t &k, int64_t &l) {
   const uint32_t K = c.ix->kmer_k;
   uint32_t code = 0u, bad = 0u;
-  for (uint32_t j = 0; j < K; ++j) {
-    const uint32_t s = c.W[i + (int32_t)j * step];
-    bad |= s > 3u ? 1u : 0u;
-    code |= (s & 3u) << (2u * j);
+  if (c.slot) {
+    uint64_t rw;
+    uint32_t rb;
+    fslot_window(c.slot, i, step, rw, rb);
+    code = (uint32_t)rw & ((1u << (2u * K)) - 1u);
+    bad = rb & ((1u << K) - 1u);
+  } else {
+    for (uint32_t j = 0; j < K; ++j) {
+      const uint32_t s = c.W[i + (int32_t)j * step];
+      bad |= s > 3u ? 1u : 0u;
+      code |= (s & 3u) << (2u * j);
+    }
   }
   if (bad) return false;
   if (v.comp) code ^= (1u << (2u * K)) - 1u;
@@ -839,55 +938,6 @@ struct FLane {
 // which is all a walk needs (D counts where intervals empty).
 constexpr uint32_t kMRows = 4;
 
-// The 32 read bases from W[p] in feed order (rw, 2-bit codes) and their N
-// flags (rb), from the lane's slot (words j at slot[j * 256]); the window
-// starts at p (d > 0) or p - 31 (then reversed); words outside the slot are
-// clamped, which changes only bases outside the read, never compared.
-__device__ __forceinline__ void fslot_window(const uint32_t *slot, int32_t p, int32_t d,
-                                             uint64_t &rw, uint32_t &rb) {
-  const uint32_t u = (uint32_t)((d > 0 ? p : p - 31) + 32);
-  const int32_t cw = (int32_t)(u >> 4) - 2, fw = (int32_t)(u >> 5) - 1;
-  const uint32_t sh = (u & 15u) << 1, fsh = u & 31u;
-  const uint32_t i0 = (uint32_t)min(max(cw, 0), 15), i1 = (uint32_t)min(max(cw + 1, 0), 15);
-  const uint32_t i2 = (uint32_t)min(max(cw + 2, 0), 15);
-  const uint32_t j0 = 16u + (uint32_t)min(max(fw, 0), 7), j1 = 16u + (uint32_t)min(max(fw + 1, 0), 7);
-  const uint32_t w0 = slot[i0 * 256u], w1 = slot[i1 * 256u], w2 = slot[i2 * 256u];
-  const uint32_t g0 = slot[j0 * 256u], g1 = slot[j1 * 256u];
-  const uint64_t lo = ((uint64_t)w1 << 32) | w0;
-  const uint64_t win = sh ? (lo >> sh) | ((uint64_t)w2 << (64u - sh)) : lo;
-  const uint32_t fl = fsh ? (g0 >> fsh) | (g1 << (32u - fsh)) : g0;
-  if (d > 0) {
-    rw = win;
-    rb = fl;
-  } else {
-    const uint64_t v = __builtin_bitreverse64(win);
-    rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
-    rb = __builtin_bitreverse32(fl);
-  }
-}
-
-// The 32 text bases before suffix q in LF order (BWT of X: X[q-1], X[q-2],
-// ...; of reverse X: X[n-q], X[n-q+1], ...) from the words t0, t1, t2 that
-// start at word w (see front_step).
-__device__ __forceinline__ uint64_t ftext_window(uint32_t dsel, uint32_t n, uint32_t q, uint32_t t0,
-                                                 uint32_t t1, uint32_t t2) {
-  const uint64_t lo = ((uint64_t)t1 << 32) | t0;
-  if (dsel) {
-    const uint32_t sh = ((n - q) & 15u) << 1;
-    return sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
-  }
-  const int64_t b = (int64_t)q - 32;
-  uint64_t win;
-  if (b >= 0) {
-    const uint32_t sh = ((uint32_t)b & 15u) << 1;
-    win = sh ? (lo >> sh) | ((uint64_t)t2 << (64u - sh)) : lo;
-  } else {
-    win = lo << min(2u * (uint32_t)(-b), 62u);
-  }
-  const uint64_t v = __builtin_bitreverse64(win);
-  return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
-}
-
 // A new unit's read offsets and fragment size are in (the FK_UNIT access):
 // its pass, fragment and view; false when the unit has no pass to run.
 __device__ __forceinline__ bool front_unit(const KParams &P, FLane &L, uint64_t off0,
@@ -1531,6 +1581,7 @@ __global__ __launch_bounds__(256) void search_k_triage_kernel(KParams P) {
 
 template <bool COUNT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_k_kernel(KParams P) {
+  __shared__ uint32_t kslots[kReadRecordWords * 256u];   // the lanes' reads (front tasks)
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   fmi_hitk *sl = P.scratch + lane * 4ull * P.list_cap;
   uint32_t wk[KW_N] = {0, 0, 0, 0, 0, 0};
@@ -1564,11 +1615,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
     c.fd = fr ? c.D : nullptr;
     c.fr = P.front_r + 2ull * t;
     c.fstep = P.ntasks;
+    c.slot = nullptr;
     if (fr) {
       // D of the front's passes, pass b at D + b * kFrontLen
       const int32_t nb = min(len / fs, (int32_t)kFrontPasses);
       for (int32_t b = 0; b < nb; ++b)
         front_expand_d(P, (uint64_t)b * P.ntasks + t, c.D + (uint32_t)b * kFrontLen);
+      // the read's codes and N flags from its record into the lane's slot
+      uint32_t *sw = kslots + threadIdx.x;
+      const uint4 *r = P.front_rec + (size_t)rd * (kReadRecordWords / 4u);
+      const uint32_t nc = ((uint32_t)len + 63u) >> 6, nf = ((uint32_t)len + 127u) >> 7;
+      for (uint32_t j = 0; j < nc; ++j) {
+        const uint4 v4 = r[j];
+        sw[(4u * j) * 256u] = v4.x; sw[(4u * j + 1u) * 256u] = v4.y;
+        sw[(4u * j + 2u) * 256u] = v4.z; sw[(4u * j + 3u) * 256u] = v4.w;
+      }
+      for (uint32_t j = 0; j < nf; ++j) {
+        const uint4 v4 = r[4u + j];
+        sw[(16u + 4u * j) * 256u] = v4.x; sw[(17u + 4u * j) * 256u] = v4.y;
+        sw[(18u + 4u * j) * 256u] = v4.z; sw[(19u + 4u * j) * 256u] = v4.w;
+      }
+      c.slot = sw;
     }
     KList prev{sl, 0, P.list_cap}, next{sl + P.list_cap, 0, P.list_cap};
     KList prev_i{sl + 2 * P.list_cap, 0, P.list_cap}, next_i{sl + 3 * P.list_cap, 0, P.list_cap};
