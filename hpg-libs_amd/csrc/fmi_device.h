@@ -179,21 +179,19 @@ __device__ __forceinline__ uint4 opaque4() {
   return make_uint4(a, b, c, d);
 }
 
-// The LF step of the read base at wp on [k, l]: the block(s) and the base
-// are read in one batch (the block addresses do not depend on the base), so
-// a step waits for memory once.  Returns the base; for a base <= 3 ko, lo
-// are the step of the physical symbol (comp ? 3 - b : b) exactly as lf_one
+// The LF step of base sym (a read byte: 0-3, or > 3 for an N) on [k, l]:
+// the block(s) are read in one batch.  Returns sym; for sym <= 3 ko, lo are
+// the step of the physical symbol (comp ? 3 - sym : sym) exactly as lf_one
 // gives them (for any k, l); for an N the caller empties the interval.
-__device__ __forceinline__ uint32_t lf_read(const DevDir &D, uint32_t k, uint32_t l,
-                                            const uint8_t *wp, uint32_t comp, uint32_t &ko,
-                                            uint32_t &lo) {
+__device__ __forceinline__ uint32_t lf_sym(const DevDir &D, uint32_t k, uint32_t l,
+                                           uint32_t sym, uint32_t comp, uint32_t &ko,
+                                           uint32_t &lo) {
   const uint32_t m2 = l + 1;
   const uint32_t j1 = k / kBlockRows, j2 = m2 / kBlockRows;
   const uint32_t t1 = k - j1 * kBlockRows, t2 = m2 - j2 * kBlockRows;
   const bool two = k != l && j2 != j1;
   const uint4 *q1 = reinterpret_cast<const uint4 *>(D.blk + j1);
   const uint4 a0 = q1[0], a1 = q1[1], a2 = q1[2], a3 = q1[3];
-  const uint32_t sym = *wp;
   uint4 b0 = opaque4(), b1 = opaque4(), b2 = opaque4(), b3 = opaque4();
   if (two) {
     const uint4 *q2 = reinterpret_cast<const uint4 *>(D.blk + j2);
@@ -211,6 +209,14 @@ __device__ __forceinline__ uint32_t lf_read(const DevDir &D, uint32_t k, uint32_
     lo = c + occ_blk(B2, D.dollar_blk, D.dollar_off, j2, s, t2);
   }
   return sym;
+}
+
+// The LF step of the read base at wp on [k, l] (lf_sym of *wp): the base's
+// load is issued with the block loads, so a step waits for memory once.
+__device__ __forceinline__ uint32_t lf_read(const DevDir &D, uint32_t k, uint32_t l,
+                                            const uint8_t *wp, uint32_t comp, uint32_t &ko,
+                                            uint32_t &lo) {
+  return lf_sym(D, k, l, *wp, comp, ko, lo);
 }
 
 __device__ __forceinline__ const DevDir &dir_of(const DevIndex &ix, uint32_t d) {
@@ -332,6 +338,139 @@ __device__ __forceinline__ uint32_t isa_value(const DevDir &D, uint32_t ratio, u
     --i;
   }
   return j;
+}
+
+// --------------------------------------------------------------------------
+// reads staged in LDS: a lane's slot holds its read's 2-bit codes (words 0-15,
+// 16 bases each) and N flags (words 16-23, 32 bases each), word j at
+// slot[j * 256]
+// --------------------------------------------------------------------------
+
+// 16 read bytes as 2-bit codes (byte i at bits 2i) and N flags (bit i: the
+// byte is > 3).
+__device__ __forceinline__ void pack16(const uint4 &c, uint32_t &code, uint32_t &flag) {
+  const uint32_t v[4] = {c.x, c.y, c.z, c.w};
+  code = 0u;
+  flag = 0u;
+#pragma unroll
+  for (uint32_t j = 0; j < 4u; ++j) {
+    uint32_t t = v[j] & 0x03030303u;
+    t = (t | (t >> 6)) & 0x000F000Fu;
+    t = (t | (t >> 12)) & 0xFFu;
+    uint32_t z = (v[j] >> 2) & 0x3F3F3F3Fu;
+    z = z | (z >> 3);
+    z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
+    z = (z | (z >> 7)) & 0x00030003u;
+    z = (z | (z >> 14)) & 0xFu;
+    code |= t << (8u * j);
+    flag |= z << (4u * j);
+  }
+}
+
+// The 32 read bases from W[p] in feed order (rw, 2-bit codes) and their N
+// flags (rb), from the lane's slot (words j at slot[j * 256]); the window
+// starts at p (d > 0) or p - 31 (then reversed); words outside the slot are
+// clamped, which changes only bases outside the read, never compared.
+__device__ __forceinline__ void lds_window(const uint32_t *slot, int32_t p, int32_t d,
+                                           uint64_t &rw, uint32_t &rb) {
+  const uint32_t u = (uint32_t)((d > 0 ? p : p - 31) + 32);
+  const int32_t cw = (int32_t)(u >> 4) - 2, fw = (int32_t)(u >> 5) - 1;
+  const uint32_t sh = (u & 15u) << 1, fsh = u & 31u;
+  const uint32_t i0 = (uint32_t)min(max(cw, 0), 15), i1 = (uint32_t)min(max(cw + 1, 0), 15);
+  const uint32_t i2 = (uint32_t)min(max(cw + 2, 0), 15);
+  const uint32_t j0 = 16u + (uint32_t)min(max(fw, 0), 7), j1 = 16u + (uint32_t)min(max(fw + 1, 0), 7);
+  const uint32_t w0 = slot[i0 * 256u], w1 = slot[i1 * 256u], w2 = slot[i2 * 256u];
+  const uint32_t g0 = slot[j0 * 256u], g1 = slot[j1 * 256u];
+  const uint64_t lo = ((uint64_t)w1 << 32) | w0;
+  const uint64_t win = sh ? (lo >> sh) | ((uint64_t)w2 << (64u - sh)) : lo;
+  const uint32_t fl = fsh ? (g0 >> fsh) | (g1 << (32u - fsh)) : g0;
+  if (d > 0) {
+    rw = win;
+    rb = fl;
+  } else {
+    const uint64_t v = __builtin_bitreverse64(win);
+    rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+    rb = __builtin_bitreverse32(fl);
+  }
+}
+
+
+// Stage the read W of len <= 256 bases into the lane's slot from the 16-byte
+// chunks that hold its bytes (none past the chunk of its last byte).  Only
+// the words that hold its bases are written; the bases after its end in the
+// last words are those of the chunk's other bytes, and no step compares them.
+__device__ __forceinline__ void stage_read(const uint8_t *W, uint32_t len, uint32_t *slot) {
+  if (len == 0u) return;
+  const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(W) & 15u);
+  const uint4 *c = reinterpret_cast<const uint4 *>(W - mis);
+  const uint32_t last = (mis + len - 1u) >> 4;            // the last byte's chunk
+  const uint32_t nw = (len + 15u) >> 4, sh = 2u * mis;
+  uint32_t pk, pf, hf = 0u;
+  pack16(c[0], pk, pf);
+#pragma unroll
+  for (uint32_t w = 0; w < 16u; ++w) {
+    if (w < nw) {
+      uint32_t nk = 0u, nf = 0u;
+      if (w + 1u <= last) pack16(c[w + 1u], nk, nf);
+      slot[w * 256u] = sh ? (pk >> sh) | (nk << (32u - sh)) : pk;
+      const uint32_t f16 = ((pf >> mis) | (nf << (16u - mis))) & 0xFFFFu;
+      if (w & 1u) slot[(16u + (w >> 1)) * 256u] = hf | (f16 << 16);
+      else hf = f16;
+      pk = nk;
+      pf = nf;
+    }
+  }
+  if (nw & 1u) slot[(16u + (nw >> 1)) * 256u] = hf;
+}
+
+// The read base at p from the lane's slot: its code 0-3, or 4 for an N.
+__device__ __forceinline__ uint32_t lds_base(const uint32_t *slot, int32_t p) {
+  const uint32_t u = (uint32_t)p;
+  const uint32_t n = (slot[(16u + (u >> 5)) * 256u] >> (u & 31u)) & 1u;
+  return n ? 4u : (slot[(u >> 4) * 256u] >> ((u & 15u) << 1)) & 3u;
+}
+
+// The 32 text bases that precede suffix q of direction dsel in LF order (X[q-1],
+// X[q-2], ... for the BWT of X; X[n-q], X[n-q+1], ... for the BWT of reverse
+// X) as 2-bit groups in that order; groups before X[0] or past X[n-1] are 0.
+__device__ __forceinline__ uint64_t text_window32(const DevIndex &ix, uint32_t dsel, uint32_t q) {
+  const uint32_t nwords = (ix.n + 15u) >> 4;
+  const int64_t b = dsel ? (int64_t)(ix.n - q) : (int64_t)q - 32;   // first base in address order
+  uint64_t win;
+  if (b >= 0) {
+    const uint32_t w = (uint32_t)b >> 4, sh = ((uint32_t)b & 15u) << 1;
+    const uint64_t t0 = w < nwords ? ix.text[w] : 0u;
+    const uint64_t t1 = w + 1u < nwords ? ix.text[w + 1u] : 0u;
+    const uint64_t t2 = w + 2u < nwords ? ix.text[w + 2u] : 0u;
+    const uint64_t lo = (t1 << 32) | t0;
+    win = sh ? (lo >> sh) | (t2 << (64u - sh)) : lo;
+  } else {                           // dsel 0 and q < 32: X[0 .. q-1] at the top
+    const uint64_t t0 = ix.text[0];
+    const uint64_t t1 = 1u < nwords ? ix.text[1] : 0u;
+    win = ((t1 << 32) | t0) << (2u * (uint32_t)(-b));
+  }
+  if (dsel) return win;
+  const uint64_t v = __builtin_bitreverse64(win);
+  return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+}
+
+// text_match for a read staged in the lane's slot, up to 32 bases at once:
+// the number of leading read bases W[p], W[p+d], ... (at most lim <= 32,
+// lim <= q) that equal the text bases preceding suffix q in LF order, the
+// read's bases complemented when comp is set; an N never matches.
+__device__ __forceinline__ uint32_t text_match32(const DevIndex &ix, uint32_t dsel, uint32_t comp,
+                                                 uint32_t q, const uint32_t *slot, int32_t p,
+                                                 int32_t d, uint32_t lim) {
+  if (lim == 0u) return 0u;
+  uint64_t rw;
+  uint32_t rb;
+  lds_window(slot, p, d, rw, rb);
+  if (comp) rw = ~rw;
+  const uint64_t x = rw ^ text_window32(ix, dsel, q);
+  const uint64_t miss = (x | (x >> 1)) & 0x5555555555555555ull;
+  const uint32_t m = miss ? (uint32_t)__builtin_ctzll(miss) >> 1 : 32u;
+  const uint32_t nb = rb ? (uint32_t)__builtin_ctz(rb) : 32u;
+  return min(min(m, nb), lim);
 }
 
 }  // namespace fmi

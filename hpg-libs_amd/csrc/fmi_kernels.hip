@@ -39,16 +39,17 @@ struct ExactParams {
 // one, so the position and interval where the search dies are the
 // reference's).  With an expanded index, a search whose interval has
 // narrowed to one row goes on against the text at that row's suffix, 16
-// bases at a time; where it dies, one LF step from the last live row gives
-// the reference's empty interval exactly.
-template <bool COUNT>
-__global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= P.nreads) return;
+// bases at a time (32 for a staged read); where it dies, one LF step from
+// the last live row gives the reference's empty interval exactly.
+//
+// STAGED: the read (at most kStagedMaxLen bases) is in the thread's LDS slot
+// (stage_read): the k-mer codes, each LF step's base and the text runs' read
+// bases come from there instead of byte loads of W.
+template <bool COUNT, bool STAGED>
+__device__ __forceinline__ void exact_one(const ExactParams &P, uint32_t r, const uint8_t *W,
+                                          int32_t len, const uint32_t *slot) {
   uint32_t wb = 0, wq = 0, ws = 0, wt = 0, wk = 0;   // counting launch
   const DevDir &D = dir_of(P.ix, P.dsel);
-  const uint8_t *W = P.reads + P.offsets[r];
-  const int32_t len = (int32_t)(P.offsets[r + 1] - P.offsets[r]);
   const uint32_t n = P.ix.n;
   const int32_t d = P.direction ? 1 : -1;
   uint32_t k = 0, l = 0, k2 = 0, l2 = n, nb;
@@ -56,15 +57,23 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   // the first K bases from the k-table, else the first K - 4 from the
   // smaller one, else one LF step at a time
   const uint32_t tks[2] = {P.ix.kmer_k, P.ix.kmer2_k};
+  uint64_t kw = 0;                   // staged: the first 32 bases in feed order
+  uint32_t kn = 0;                   // and their N flags
+  if (STAGED && tks[0] && len > 0) lds_window(slot, i, d, kw, kn);
 #pragma unroll
   for (uint32_t tt = 0; tt < 2u; ++tt) {
     const uint32_t K = tks[tt];
     if (!K || len < (int32_t)K || k2 != 0u || l2 != n) continue;
     uint32_t code = 0u, bad = 0u;
-    for (uint32_t j = 0; j < K; ++j) {
-      const uint32_t s = W[i + (int32_t)j * d];
-      bad |= s > 3u ? 1u : 0u;
-      code |= (s & 3u) << (2u * j);
+    if (STAGED) {
+      code = (uint32_t)kw & ((1u << (2u * K)) - 1u);
+      bad = kn & ((1u << K) - 1u);
+    } else {
+      for (uint32_t j = 0; j < K; ++j) {
+        const uint32_t s = W[i + (int32_t)j * d];
+        bad |= s > 3u ? 1u : 0u;
+        code |= (s & 3u) << (2u * j);
+      }
     }
     if (P.comp) code ^= (1u << (2u * K)) - 1u;
     if (!bad) {
@@ -79,25 +88,27 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
   }
   for (; P.direction ? (i <= len - 1) : (i >= 0); i += d) {
     if (k2 == l2 && P.ix.text != nullptr) {
-      if (W[i] > 3u) { k2 = n + 1; l2 = n; break; }
+      if ((STAGED ? lds_base(slot, i) : (uint32_t)W[i]) > 3u) { k2 = n + 1; l2 = n; break; }
       // text mode from the single row k2 on
       uint32_t q = D.sa[k2];
       if (COUNT) ws++;
       uint32_t rem = (uint32_t)(d < 0 ? i + 1 : len - i);
       bool died = false;
       while (rem > 0u) {
-        const uint32_t lim = min(min(rem, 16u), q);
-        const uint32_t m = text_match(P.ix, P.dsel, P.comp, q, W, i, d, lim);
+        const uint32_t lim = min(min(rem, STAGED ? 32u : 16u), q);
+        const uint32_t m = STAGED ? text_match32(P.ix, P.dsel, P.comp, q, slot, i, d, lim)
+                                  : text_match(P.ix, P.dsel, P.comp, q, W, i, d, lim);
         if (COUNT) wt++;
         q -= m;
         rem -= m;
         i += (int32_t)m * d;
-        if (rem > 0u && (m < lim || q == 0u || m < 16u)) { died = true; break; }
+        // a mismatch or an N (m < lim), or the text's start (q == 0)
+        if (rem > 0u && (m < lim || q == 0u)) { died = true; break; }
       }
       k = l = D.isa[q];
       if (COUNT) ws++;
       if (!died) { k2 = l2 = k; break; }
-      const uint32_t s2 = W[i];
+      const uint32_t s2 = STAGED ? lds_base(slot, i) : (uint32_t)W[i];
       if (s2 > 3u) { k2 = n + 1; l2 = n; }
       else {
         lf_one(D, k, l, P.comp ? 3u - s2 : s2, k2, l2, nb);
@@ -105,13 +116,15 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
       }
       break;
     }
-    uint32_t kn, ln;
+    uint32_t kn2, ln2;
     if (COUNT) {
       wb += (k2 != l2 && (l2 + 1u) / kBlockRows != k2 / kBlockRows) ? 2u : 1u;
       wq++;
     }
-    if (lf_read(D, k2, l2, W + i, P.comp, kn, ln) > 3u) { k2 = n + 1; l2 = n; break; }
-    k2 = kn; l2 = ln;
+    const uint32_t sym = STAGED ? lf_sym(D, k2, l2, lds_base(slot, i), P.comp, kn2, ln2)
+                                : lf_read(D, k2, l2, W + i, P.comp, kn2, ln2);
+    if (sym > 3u) { k2 = n + 1; l2 = n; break; }
+    k2 = kn2; l2 = ln2;
     if (k2 > l2) break;
     k = k2; l = l2;
   }
@@ -129,6 +142,25 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
     atomicAdd(&P.work[2], (unsigned long long)ws);
     atomicAdd(&P.work[3], (unsigned long long)wt);
     atomicAdd(&P.work[4], (unsigned long long)wk);
+  }
+}
+
+// One thread per read; a read of at most kStagedMaxLen bases is staged in
+// the thread's LDS slot first (24 KB per workgroup of 256).
+template <bool COUNT>
+__global__ __launch_bounds__(256) void exact_kernel(ExactParams P) {
+  __shared__ uint32_t slots[kReadRecordWords * 256u];
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P.nreads) return;
+  const uint64_t off = P.offsets[r];
+  const uint8_t *W = P.reads + off;
+  const int32_t len = (int32_t)(P.offsets[r + 1] - off);
+  uint32_t *slot = slots + threadIdx.x;
+  if (len <= (int32_t)kStagedMaxLen) {
+    stage_read(W, (uint32_t)len, slot);
+    exact_one<COUNT, true>(P, r, W, len, slot);
+  } else {
+    exact_one<COUNT, false>(P, r, W, len, slot);
   }
 }
 
@@ -304,27 +336,6 @@ __device__ __forceinline__ void kmer_codes(const uint8_t *reads, uint64_t off, u
   kc1 = (nb & nmask) ? 0u : ((w & kmask) | kKmerReady);
   window16(cz0, cz1, 16u + mz, false, w, nb);
   kc0 = (nb & nmask) ? 0u : ((w & kmask) | kKmerReady);
-}
-
-// 16 read bytes as 2-bit codes (byte i at bits 2i) and N flags (bit i: the
-// byte is > 3).
-__device__ __forceinline__ void pack16(const uint4 &c, uint32_t &code, uint32_t &flag) {
-  const uint32_t v[4] = {c.x, c.y, c.z, c.w};
-  code = 0u;
-  flag = 0u;
-#pragma unroll
-  for (uint32_t j = 0; j < 4u; ++j) {
-    uint32_t t = v[j] & 0x03030303u;
-    t = (t | (t >> 6)) & 0x000F000Fu;
-    t = (t | (t >> 12)) & 0xFFu;
-    uint32_t z = (v[j] >> 2) & 0x3F3F3F3Fu;
-    z = z | (z >> 3);
-    z = (z | (z >> 1) | (z >> 2)) & 0x01010101u;
-    z = (z | (z >> 7)) & 0x00030003u;
-    z = (z | (z >> 14)) & 0xFu;
-    code |= t << (8u * j);
-    flag |= z << (4u * j);
-  }
 }
 
 struct RecParams {

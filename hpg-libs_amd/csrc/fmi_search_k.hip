@@ -70,33 +70,6 @@ enum : uint32_t { KW_BLOCKS = 0, KW_LF = 1, KW_SAISA = 2, KW_TEXT = 3, KW_TABLE 
     if ((c).wk) (c).wk[(i)] += (uint32_t)(v);   \
   } while (0)
 
-// The 32 read bases from W[p] in feed order (rw, 2-bit codes) and their N
-// flags (rb), from the lane's slot (words j at slot[j * 256]); the window
-// starts at p (d > 0) or p - 31 (then reversed); words outside the slot are
-// clamped, which changes only bases outside the read, never compared.
-__device__ __forceinline__ void fslot_window(const uint32_t *slot, int32_t p, int32_t d,
-                                             uint64_t &rw, uint32_t &rb) {
-  const uint32_t u = (uint32_t)((d > 0 ? p : p - 31) + 32);
-  const int32_t cw = (int32_t)(u >> 4) - 2, fw = (int32_t)(u >> 5) - 1;
-  const uint32_t sh = (u & 15u) << 1, fsh = u & 31u;
-  const uint32_t i0 = (uint32_t)min(max(cw, 0), 15), i1 = (uint32_t)min(max(cw + 1, 0), 15);
-  const uint32_t i2 = (uint32_t)min(max(cw + 2, 0), 15);
-  const uint32_t j0 = 16u + (uint32_t)min(max(fw, 0), 7), j1 = 16u + (uint32_t)min(max(fw + 1, 0), 7);
-  const uint32_t w0 = slot[i0 * 256u], w1 = slot[i1 * 256u], w2 = slot[i2 * 256u];
-  const uint32_t g0 = slot[j0 * 256u], g1 = slot[j1 * 256u];
-  const uint64_t lo = ((uint64_t)w1 << 32) | w0;
-  const uint64_t win = sh ? (lo >> sh) | ((uint64_t)w2 << (64u - sh)) : lo;
-  const uint32_t fl = fsh ? (g0 >> fsh) | (g1 << (32u - fsh)) : g0;
-  if (d > 0) {
-    rw = win;
-    rb = fl;
-  } else {
-    const uint64_t v = __builtin_bitreverse64(win);
-    rw = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
-    rb = __builtin_bitreverse32(fl);
-  }
-}
-
 // The 32 text bases before suffix q in LF order (BWT of X: X[q-1], X[q-2],
 // ...; of reverse X: X[n-q], X[n-q+1], ...) from the words t0, t1, t2 that
 // start at word w (see front_step).
@@ -140,7 +113,7 @@ __device__ __forceinline__ int32_t k_run(const KCtx &c, const KView &v, uint32_t
       if (lim == 0u) { dies = true; break; }
       uint64_t rw;
       uint32_t rb;
-      fslot_window(c.slot, i + done * step, step, rw, rb);
+      lds_window(c.slot, i + done * step, step, rw, rb);
       uint32_t w;
       if (v.dsel) {
         w = min((n - q) >> 4, nwords - 1u);
@@ -226,7 +199,7 @@ __device__ __forceinline__ bool k_table(const KCtx &c, const KView &v, int32_t i
   if (c.slot) {
     uint64_t rw;
     uint32_t rb;
-    fslot_window(c.slot, i, step, rw, rb);
+    lds_window(c.slot, i, step, rw, rb);
     code = (uint32_t)rw & ((1u << (2u * K)) - 1u);
     bad = rb & ((1u << K) - 1u);
   } else {
@@ -1049,7 +1022,7 @@ __device__ __forceinline__ void front_step(const KParams &P, uint32_t *slot, FLa
   const bool exact = L.ph == FP_EXACT;
   uint64_t rw;
   uint32_t rb;
-  fslot_window(slot, L.i, L.step, rw, rb);
+  lds_window(slot, L.i, L.step, rw, rb);
   const uint64_t rwc = L.strand ? ~rw : rw;          // complemented on the - strand
 
   // ---- what to read

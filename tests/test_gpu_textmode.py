@@ -132,6 +132,42 @@ def test_textmode_exact_with_n(gpu_engine):
     b.close()
 
 
+@pytest.mark.parametrize("form", ["compact", "expanded", "kmer8"])
+def test_exact_staged_and_long_reads_vs_oracle(gpu_engine, form):
+    """Reads of 1-400 bases in one batch, so a wavefront holds reads staged in
+    LDS (at most 256 bases) beside reads searched from their bytes, with N
+    bases: on every view and direction the record and the anchor equal the
+    oracle's (BWExactSearchBackward / Forward, search/search.h:9-65)."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(200_000, seed=141, repeat_frac=0.3, fam_len=(300, 3000))
+    lens = np.random.default_rng(142).integers(1, 401, 1200)
+    reads, offs, _ = synth.reads(g, 1200, seed=143, sub_rate=0.003, n_rate=0.002, lengths=lens)
+    assert np.any(reads > 3)
+    o = fo.Oracle(g, ratio=16)
+    lsa = o.size_sa - 1
+    ix = hp.Index.build(g, 16).to_device()
+    if form != "compact":
+        ix.expand()
+    if form == "kmer8":
+        ix.kmer_table(8)
+    batch = hp.ReadBatch(reads, offs)
+    try:
+        for view in range(4):
+            for d in (0, 1):
+                rec, anc = hp.exact_batch(ix, batch, view, d, anchors=True)
+                for i in range(batch.nreads):
+                    W = reads[offs[i]:offs[i + 1]]
+                    L = len(W)
+                    r, a = o.exact(view, d, W, 0, lsa, 0, 0 if d else L - 1, L - 1, anchor=True)
+                    for key in ("k", "l", "pos"):
+                        assert rec[i][key] == r[key], (form, view, d, i, L, key)
+                    for key in ("k", "l", "start", "pos", "end", "dir"):
+                        assert anc[i][key] == a[key], (form, view, d, i, L, "anchor", key)
+    finally:
+        batch.free()
+        ix.close()
+
+
 def test_textmode_locate_and_inverse(tgold):
     """get_SA / get_ISA read the full arrays on an expanded index: the golden
     vectors, and locate over whole intervals."""
