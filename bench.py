@@ -332,24 +332,46 @@ def bench_exact(args, pre, g, ix, hpgfm, synth, read_len=100, cap=64):
     pos = [hpgfm.DeviceBuffer(max(n * cap, 1) * 8) for _ in range(2)]
     nrows = [hpgfm.DeviceBuffer(max(n, 1) * 4) for _ in range(2)]
 
-    def step():
+    def step(st=None):
         for s, view in ((0, hpgfm.VIEW_BACKWARD), (1, hpgfm.VIEW_BACKWARD_REV)):
             hpgfm.check(L.fmi_exact_batch(ix.h, view, s, batch.d_reads.ptr, batch.d_offsets.ptr,
-                                          n, outs[s].ptr, None, None))
+                                          n, outs[s].ptr, None, st))
             hpgfm.check(L.fmi_locate_hits(ix.h, 0, outs[s].ptr, n, cap, pos[s].ptr, nrows[s].ptr,
-                                          None))
+                                          st))
 
     for _ in range(args.warmup):
         step()
     hpgfm.sync()
+    timed = {}
     with hpgfm.TimingRegion() as tr:
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step()
         hpgfm.sync()
-        wall = (time.perf_counter() - t0) / args.steps
+        timed["launched one by one"] = (time.perf_counter() - t0) / args.steps
     kern = tr.sum_ms / args.steps
-    log(f"exact + locate: {wall * 1000:.3f} ms per step (kernels {kern:.3f} ms), "
+    # The step's four launches are short (tens of microseconds each): the
+    # same step captured once as a graph on a created stream (fmi_graph_*)
+    # and replayed with one call per step; the faster is the step.
+    gst = hpgfm.Stream()
+    graph = hpgfm.Graph(gst)
+    with graph:
+        step(gst.handle)
+    for _ in range(args.warmup):
+        graph.launch()
+    gst.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    gst.sync()
+    timed["one graph per step"] = (time.perf_counter() - t0) / args.steps
+    graph.close()
+    gst.close()
+    for k, v in timed.items():
+        log(f"exact + locate, {k}: {v * 1000:.3f} ms per step, {n / v / 1e6:.2f} M reads/s")
+    launch = min(timed, key=timed.get)
+    wall = timed[launch]
+    log(f"exact + locate: {wall * 1000:.3f} ms per step ({launch}; kernels {kern:.3f} ms), "
         f"{n / wall / 1e6:.2f} M reads/s")
     g_rows = np.stack([nrows[s].download(np.uint32, n) for s in (0, 1)], axis=1)
     g_sums = np.zeros((n, 2), np.uint64)
@@ -393,7 +415,8 @@ def bench_exact(args, pre, g, ix, hpgfm, synth, read_len=100, cap=64):
                                f"exact copies of either strand, 10% random), <= {cap} rows "
                                f"per interval", "preset": args.preset,
                    "kmer_table_k": args.kmer},
-        "kernel_ms_per_step": round(kern, 4), "cpu_baseline": cpu,
+        "kernel_ms_per_step": round(kern, 4), "cpu_baseline": cpu, "launch": launch,
+        "ms_per_step_by_launch": {k: round(v * 1000, 4) for k, v in timed.items()},
     }
     res.update(extras)
     batch.free()

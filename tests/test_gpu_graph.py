@@ -314,3 +314,68 @@ def test_forked_graph_runs_branches_and_pins_their_scratch(gpu_engine):
     for x in st:
         x.close()
     ix.close()
+
+
+def test_exact_locate_chain_as_graph(gpu_engine):
+    """The exact search of both strands, each followed by locate (SA of the
+    first rows of each interval), captured on one stream and replayed three
+    times with cleared outputs and other searches run eagerly in between:
+    every replay gives the eager records, rows and positions."""
+    hp = gpu_engine
+    g, _, _ = synth.genome(200_000, seed=131)
+    reads, offs, _ = synth.reads(g, 3000, length=100, seed=132, sub_rate=0.0, random_frac=0.1)
+    ix = hp.Index.build(g, 16).to_device().expand()
+    ix.kmer_table(8)
+    batch = hp.ReadBatch(reads, offs)
+    n, cap = batch.nreads, 16
+    st = hp.Stream()
+    L = hp.lib()
+    outs = [hp.DeviceBuffer(n * 32) for _ in range(2)]
+    pos = [hp.DeviceBuffer(n * cap * 8) for _ in range(2)]
+    rows = [hp.DeviceBuffer(n * 4) for _ in range(2)]
+
+    def chain():
+        for s, view in ((0, hp.VIEW_BACKWARD), (1, hp.VIEW_BACKWARD_REV)):
+            hp.check(L.fmi_exact_batch(ix.h, view, s, batch.d_reads.ptr, batch.d_offsets.ptr, n,
+                                       outs[s].ptr, None, st.handle))
+            hp.check(L.fmi_locate_hits(ix.h, 0, outs[s].ptr, n, cap, pos[s].ptr, rows[s].ptr,
+                                       st.handle))
+
+    def results():
+        st.sync()
+        return [(outs[s].download(hp.HIT_DTYPE, n).view(np.uint8), rows[s].download(np.uint32, n),
+                 pos[s].download(np.uint64, n * cap).reshape(n, cap)) for s in (0, 1)]
+
+    def clear():
+        for b in outs + pos + rows:
+            b.zero()
+        hp.sync()
+
+    try:
+        clear()
+        chain()
+        want = results()
+        assert sum(int(w[1].sum()) for w in want) > 0
+        graph = hp.Graph(st)
+        with graph:
+            chain()
+        try:
+            for rep in range(3):
+                clear()
+                hp.exact_batch(ix, batch, hp.VIEW_FORWARD, 1)           # eager work
+                graph.launch()
+                got = results()
+                for s in (0, 1):
+                    assert np.array_equal(got[s][0], want[s][0]), (rep, s)
+                    assert np.array_equal(got[s][1], want[s][1]), (rep, s)
+                    m = np.arange(cap)[None, :] < want[s][1][:, None]
+                    assert np.array_equal(np.where(m, got[s][2], 0), np.where(m, want[s][2], 0)), \
+                        (rep, s)
+        finally:
+            graph.close()
+    finally:
+        st.close()
+        for b in outs + pos + rows:
+            b.free()
+        batch.free()
+        ix.close()
