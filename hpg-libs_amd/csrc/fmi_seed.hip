@@ -290,7 +290,11 @@ struct RegionSink {
   }
 };
 
-__global__ __launch_bounds__(256) void inexact_seed_kernel(InexactParams P) {
+// Held to 128 VGPRs for 4 waves per SIMD (157 unbounded: 3 waves): a lane's
+// windows are chains of dependent loads, so a fourth wave to hide them pays
+// for the few registers spilled (83.3 -> 76.7 ms on 8 M windows of 18 bases
+// over 64 Mbp).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void inexact_seed_kernel(InexactParams P) {
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint2 *vb = P.vscratch + lane * (uint64_t)(P.seed_size + 3);
   for (uint64_t g = lane; g < P.nwin; g += (uint64_t)gridDim.x * blockDim.x) {
