@@ -182,12 +182,7 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     launch = min(timed, key=timed.get)
     wall = timed[launch]
     if args.calibrate:
-        # kernels of known traffic for the FETCH_SIZE / WRITE_SIZE scales, as
-        # in the 1-error workload
-        hpgfm.copy_bench(1 << 30, reps=1)
-        for lb in (64, 128):
-            hpgfm.gather_bench(max(64 << 20, random_footprint(ix)["total"]), lb, 1 << 24)
-        hpgfm.sync()
+        calibration_kernels(hpgfm, ix)
 
     # Algorithmic bytes of each group's launches (the line model of the
     # 1-error search: 64 bytes per occurrence block, SA / ISA entry, text
@@ -458,6 +453,8 @@ def bench_inexact_seeds(args, pre, g, ix, hpgfm, synth, seed_size=18, min_seed_s
         step()
     hpgfm.sync()
     wall = (time.perf_counter() - t0) / args.steps
+    if args.calibrate:
+        calibration_kernels(hpgfm, ix)
     counts = nreg.download(np.uint32, 2 * nwin)
     status = st.download(np.uint8, 2 * nwin)
     kept = int(counts.sum())
@@ -507,6 +504,16 @@ def bench_inexact_seeds(args, pre, g, ix, hpgfm, synth, seed_size=18, min_seed_s
     print(json.dumps(res), flush=True)
 
 
+def calibration_kernels(hpgfm, ix):
+    """Kernels of known traffic for the FETCH_SIZE / WRITE_SIZE scales of a
+    --calibrate run: a 1 GiB copy and 2^24 random 64- and 128-byte lines over
+    a buffer of the index's random-access footprint."""
+    hpgfm.copy_bench(1 << 30, reps=1)
+    for lb in (64, 128):
+        hpgfm.gather_bench(max(64 << 20, random_footprint(ix)["total"]), lb, 1 << 24)
+    hpgfm.sync()
+
+
 def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000):
     """Exact seed mapping: every read cut into seed_len-mers at `stride`, both
     orientations, up to `filt` SA positions resolved per seed (the
@@ -533,6 +540,8 @@ def bench_seeds(args, pre, g, ix, hpgfm, synth, seed_len=18, stride=5, filt=1000
     for _ in range(args.steps):
         step()
     wall = (time.perf_counter() - t0) / args.steps
+    if args.calibrate:
+        calibration_kernels(hpgfm, ix)
     out = regs.download(hpgfm.SEED_REGION_DTYPE, nr)
     valid = int(out["valid"].sum())
     log(f"seeds: {ns} seeds, {nr} rows ({valid} regions) per batch in {wall * 1000:.2f} ms")
