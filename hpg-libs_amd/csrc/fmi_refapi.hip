@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void vhelper_given_kernel(VhGivenParams P) {
       vf[j - (half - 1)] = make_uint2((uint32_t)P.vki[v + j], (uint32_t)P.vli[v + j]);
     VhSink S{P.hits + (uint64_t)t * P.cap, P.cap, 0u, 0u, P.strand[t], (int16_t)start,
              (int16_t)end};
-    vh_replay(P.ix, P.reads + P.offsets[t], start, end, P.strand[t], 0u, vb, S);
+    vh_replay(P.ix, VhBytes{P.reads + P.offsets[t]}, start, end, P.strand[t], 0u, vb, S);
     P.counts[t] = S.cnt;
   }
 }

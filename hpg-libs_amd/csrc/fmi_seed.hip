@@ -293,7 +293,10 @@ struct RegionSink {
 // Held to 128 VGPRs for 4 waves per SIMD (157 unbounded: 3 waves): a lane's
 // windows are chains of dependent loads, so a fourth wave to hide them pays
 // for the few registers spilled (83.3 -> 76.7 ms on 8 M windows of 18 bases
-// over 64 Mbp).
+// over 64 Mbp).  REGS (seed_size <= 32): the window's bases are packed into
+// registers once (vh_regs) for both strands, so the helper's LF steps and
+// comparisons read no read bytes from memory.
+template <bool REGS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void inexact_seed_kernel(InexactParams P) {
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint2 *vb = P.vscratch + lane * (uint64_t)(P.seed_size + 3);
@@ -306,6 +309,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
     const int32_t ws = w * (int32_t)P.seed_size;
     const int32_t we = w < num_seeds ? ws + (int32_t)P.seed_size - 1 : len - 1;
     int32_t final_pos = 0;
+    VhRegs WR{0ull, 0u, ws};
+    if (REGS && we - ws >= 2) WR = vh_regs(P.reads + off, ws, (uint32_t)(we - ws + 1));
     for (uint32_t s = 0; s < 2; ++s) {
       const uint32_t type = 1u - s;          // bwt.c:1460: type 1 (+) first
       const uint64_t slot = 2 * g + s;
@@ -313,7 +318,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
                    type ? (uint32_t)ws : (uint32_t)(len - we - 1),
                    type ? (uint32_t)we : (uint32_t)(len - ws - 1), (uint32_t)len, (uint32_t)w,
                    ws, we - ws + 1, &final_pos, 0};
-      if (we - ws >= 2) vh_run(P.ix, P.reads + off, ws, we, s, 0u, vb, S);
+      if (we - ws >= 2) {
+        if (REGS) vh_run(P.ix, WR, ws, we, s, 0u, vb, S);
+        else vh_run(P.ix, VhBytes{P.reads + off}, ws, we, s, 0u, vb, S);
+      }
       fmi_inexact_region *out = S.out;
       const uint32_t m = S.m;
       // dedup within 10 bp (bwt.c:1593-1615), the mark kept in pad[0]
@@ -476,7 +484,9 @@ int map_inexact_seeds(const fmi_index *ix, const uint8_t *reads, const uint64_t 
     InexactParams P{ix->dev, reads, offsets, base, nwin, nreads, seed_size, rec_cap, region_cap,
                     ix->d_koffset, ix->d_kstart, (uint32_t)ix->kstart.size(), vscr, regions,
                     nregions, status};
-    inexact_seed_kernel<<<dim3((uint32_t)(lanes / 256)), 256, 0, s>>>(P);
+    // windows are at most seed_size bases (the remainder window is shorter)
+    if (seed_size <= 32u) inexact_seed_kernel<true><<<dim3((uint32_t)(lanes / 256)), 256, 0, s>>>(P);
+    else inexact_seed_kernel<false><<<dim3((uint32_t)(lanes / 256)), 256, 0, s>>>(P);
     SEED_TRY(hipGetLastError());
   }
 done:

@@ -46,16 +46,54 @@ struct VhView {
       lo[b] = pick4(pl, s);
     }
   }
-  // The same step with the base still in memory: the base and the block(s)
-  // are read in one batch (lf_read computes what lf_one does, for any k, l).
-  __device__ __forceinline__ void lfw(uint32_t k, uint32_t l, const uint8_t *wp, uint32_t &ko,
+  // The same step for a read base (0-3, or > 3 for an N) with the block(s)
+  // read in one batch (lf_sym computes what lf_one does, for any k, l).
+  __device__ __forceinline__ void lfs(uint32_t k, uint32_t l, uint32_t sym, uint32_t &ko,
                                       uint32_t &lo) const {
     uint32_t a, b;
-    if (lf_read(*D, k, l, wp, comp, a, b) > 3u) { ko = n + 1u; lo = n; return; }
+    if (lf_sym(*D, k, l, sym, comp, a, b) > 3u) { ko = n + 1u; lo = n; return; }
     ko = a;
     lo = b;
   }
 };
+
+// Where the helper reads the window's bases: the read's bytes in memory
+// (VhBytes), or a window of at most 32 bases held in registers as 2-bit codes
+// and N flags (VhRegs, from vh_regs).  at(i) is W[i]: 0-3, or > 3 for an N.
+struct VhBytes {
+  const uint8_t *W;
+  __device__ __forceinline__ uint32_t at(int32_t i) const { return W[i]; }
+};
+
+struct VhRegs {
+  uint64_t code;                   // base s + j at bits 2j
+  uint32_t nfl;                    // bit j: base s + j is an N
+  int32_t s;
+  __device__ __forceinline__ uint32_t at(int32_t i) const {
+    const uint32_t j = (uint32_t)(i - s);
+    return ((nfl >> j) & 1u) ? 4u : (uint32_t)(code >> (2u * j)) & 3u;
+  }
+};
+
+// W[s .. s + len - 1] (1 <= len <= 32) packed into registers, read from the
+// 16-byte chunks that hold it (none past the chunk of its last byte).
+__device__ __forceinline__ VhRegs vh_regs(const uint8_t *W, int32_t s, uint32_t len) {
+  const uint8_t *a = W + s;
+  const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(a) & 15u);
+  const uint4 *c = reinterpret_cast<const uint4 *>(a - mis);
+  const uint32_t last = (mis + len - 1u) >> 4;            // 0 .. 2
+  uint32_t k0, k1 = 0u, k2 = 0u, f0, f1 = 0u, f2 = 0u;
+  pack16(c[0], k0, f0);
+  if (last >= 1u) pack16(c[1], k1, f1);
+  if (last >= 2u) pack16(c[2], k2, f2);
+  const uint64_t lo = ((uint64_t)k1 << 32) | k0;
+  const uint32_t sh = 2u * mis;
+  VhRegs R;
+  R.code = sh ? (lo >> sh) | ((uint64_t)k2 << (64u - sh)) : lo;
+  R.nfl = (uint32_t)(((uint64_t)f0 | ((uint64_t)f1 << 16) | ((uint64_t)f2 << 32)) >> mis);
+  R.s = s;
+  return R;
+}
 
 struct VhRec {
   uint32_t k, l;
@@ -79,33 +117,35 @@ struct VhSink {
 };
 
 // BWExactSearchBackward (dir 0) / Forward (dir 1) on the record state.
-__device__ __forceinline__ void vh_exact(const VhView &V, const uint8_t *W, int32_t start,
+template <class Src>
+__device__ __forceinline__ void vh_exact(const VhView &V, const Src &W, int32_t start,
                                          int32_t end, uint32_t dir, VhRec &r) {
   uint32_t k2 = r.k, l2 = r.l;
   int32_t i = r.pos;
   if (dir == 0) {
     for (; i >= start; --i) {
-      V.lfw(k2, l2, W + i, k2, l2);
+      V.lfs(k2, l2, W.at(i), k2, l2);
       if (k2 > l2) break;
     }
   } else {
     for (; i <= end; ++i) {
-      V.lfw(k2, l2, W + i, k2, l2);
+      V.lfs(k2, l2, W.at(i), k2, l2);
       if (k2 > l2) break;
     }
   }
   r.k = k2; r.l = l2; r.pos = i;
 }
 
-template <class Sink>
-__device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t start, int32_t end,
+template <class Src, class Sink>
+__device__ inline void vh_replay(const DevIndex &ix, const Src &W, int32_t start, int32_t end,
                                  uint32_t strand, uint32_t mm_only, const uint2 *vb, Sink &S);
 
 // The helper over W[start..end] (end - start >= 2) on strand 0 (backward,
 // forward) or 1 (forward_rev, backward_rev); vb: n + 3 intervals of scratch.
-// Every record goes to S.add(r) in the reference's order.
-template <class Sink>
-__device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t start, int32_t end,
+// Every record goes to S.add(r) in the reference's order.  W: VhBytes or
+// VhRegs (the bases at start..end).
+template <class Src, class Sink>
+__device__ inline void vh_run(const DevIndex &ix, const Src &W, int32_t start, int32_t end,
                               uint32_t strand, uint32_t mm_only, uint2 *vb, Sink &S) {
   const uint32_t n_sa = ix.n;
   // + strand: backward / forward; - strand: forward_rev / backward_rev
@@ -121,7 +161,7 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     bool dead = false;
     for (int32_t j = n; j >= 0; --j) {
       if (!dead) {
-        B.lfw(k, l, W + start + j, k, l);
+        B.lfs(k, l, W.at(start + j), k, l);
         dead = k > l;
       }
       if (j <= half) vb[j] = make_uint2(k, l);
@@ -133,7 +173,7 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
     bool dead = false;
     for (int32_t j = 0; j <= n; ++j) {
       if (!dead) {
-        F.lfw(k, l, W + start + j, k, l);
+        F.lfs(k, l, W.at(start + j), k, l);
         dead = k > l;
       }
       if (j >= half - 1) vf[j - (half - 1)] = make_uint2(k, l);
@@ -144,8 +184,8 @@ __device__ inline void vh_run(const DevIndex &ix, const uint8_t *W, int32_t star
 
 // The helper's record sequence from the vectors in vb (entries 0..n/2 of the
 // backward vector, then entries n/2-1..n of the forward vector).
-template <class Sink>
-__device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t start, int32_t end,
+template <class Src, class Sink>
+__device__ inline void vh_replay(const DevIndex &ix, const Src &W, int32_t start, int32_t end,
                                  uint32_t strand, uint32_t mm_only, const uint2 *vb, Sink &S) {
   const uint32_t n_sa = ix.n;
   const VhView B{strand ? &ix.d[1] : &ix.d[0], strand, n_sa};
@@ -170,7 +210,7 @@ __device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t s
     r.k = k; r.l = l; r.pos = -1;
     r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = start;
     S.add(r);
-    const uint32_t bw = W[start];
+    const uint32_t bw = W.at(start);
     uint32_t kb4[4], lb4[4];
     B.lf4(k, l, kb4, lb4);
     for (uint32_t b = 0; b < 4u; ++b) {
@@ -213,7 +253,7 @@ __device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t s
       } else {
         r.kind = FMI_INSERTION; r.base = b;
       }
-      if (b != W[i - 1]) {
+      if (b != W.at(i - 1)) {
         r.k = ka; r.l = la; r.pos = i - 2;
         r.kind = FMI_MISMATCH;
         vh_exact(B, W, start, end, 0, r);
@@ -235,7 +275,7 @@ __device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t s
     r.k = k; r.l = l; r.pos = -1;
     r.kind = FMI_DELETION; r.base = 0xFFu; r.err_pos = end;
     S.add(r);
-    const uint32_t bw = W[end];
+    const uint32_t bw = W.at(end);
     uint32_t kb4[4], lb4[4];
     F.lf4(k, l, kb4, lb4);
     for (uint32_t b = 0; b < 4u; ++b) {
@@ -277,7 +317,7 @@ __device__ inline void vh_replay(const DevIndex &ix, const uint8_t *W, int32_t s
       } else {
         r.kind = FMI_INSERTION; r.base = b;
       }
-      if (b != W[i + 1]) {
+      if (b != W.at(i + 1)) {
         r.k = ka; r.l = la; r.pos = i + 2;
         r.kind = FMI_MISMATCH;
         vh_exact(F, W, start, end, 1, r);
@@ -292,7 +332,7 @@ __device__ inline uint32_t vh_task(const DevIndex &ix, const uint8_t *W, int32_t
                                    int32_t end, uint32_t strand, uint32_t mm_only, uint2 *vb,
                                    fmi_hit *out, uint32_t cap) {
   VhSink S{out, cap, 0u, mm_only, strand, (int16_t)start, (int16_t)end};
-  vh_run(ix, W, start, end, strand, mm_only, vb, S);
+  vh_run(ix, VhBytes{W}, start, end, strand, mm_only, vb, S);
   return S.cnt;
 }
 
