@@ -68,6 +68,9 @@ struct SeedParams {
   int64_t *kl;                 // 2 * nseeds: k of + and k of - (first row of each)
 };
 
+// REGS (seed_len <= 32): the seed's bases are packed into registers once
+// (vh_regs) and every base below comes from there, not from a byte load.
+template <bool REGS>
 __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
   const DevDir &D = P.ix.d[0];
   const uint32_t n = P.ix.n;
@@ -76,6 +79,9 @@ __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
     const uint64_t r = owner(P.seed_base, P.nreads, g);
     const uint64_t s = (g - P.seed_base[r]) * P.stride;
     const uint8_t *W = P.reads + P.offsets[r] + s;
+    VhRegs WR{0ull, 0u, 0};
+    if (REGS) WR = vh_regs(W, 0, P.seed_len);
+    auto base = [&](int32_t i) -> uint32_t { return REGS ? WR.at(i) : (uint32_t)W[i]; };
     // The first k steps of each search from the k-mer table when its k bases
     // are A/C/G/T and their interval is not empty (else step by step, so a
     // dead interval is the one the LF steps give).
@@ -90,7 +96,7 @@ __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
     if (use_k) {
       uint32_t cp = 0u, cm = 0u, bad = 0u;
       for (uint32_t j = 0; j < K; ++j) {
-        const uint32_t sp = W[ip - (int32_t)j], sm = W[j];
+        const uint32_t sp = base(ip - (int32_t)j), sm = base((int32_t)j);
         bad |= (sp > 3u || sm > 3u) ? 1u : 0u;
         cp |= (sp & 3u) << (2u * j);
         cm |= (sm & 3u) << (2u * j);
@@ -108,13 +114,13 @@ __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
     }
     for (int32_t i = ip; i >= 0; --i) {
       uint32_t kn, ln;
-      if (lf_read(D, kp, lp, W + i, 0u, kn, ln) > 3u) { kp = n + 1; lp = n; break; }
+      if (lf_sym(D, kp, lp, base(i), 0u, kn, ln) > 3u) { kp = n + 1; lp = n; break; }
       kp = kn; lp = ln;
       if (kp > lp) break;
     }
     for (uint32_t i = im; i < P.seed_len; ++i) {
       uint32_t kn, ln;
-      if (lf_read(D, km, lm, W + i, 1u, kn, ln) > 3u) { km = n + 1; lm = n; break; }
+      if (lf_sym(D, km, lm, base((int32_t)i), 1u, kn, ln) > 3u) { km = n + 1; lm = n; break; }
       km = kn; lm = ln;
       if (km > lm) break;
     }
@@ -404,7 +410,8 @@ int seed_map(const fmi_index *ix, const uint8_t *reads, const uint64_t *offsets,
   {
     SeedParams P{ix->dev, reads, offsets, base, nseeds, nreads, seed_len, stride,
                  (int32_t)(filter > (uint32_t)INT32_MAX ? INT32_MAX : filter), info, info_cap, rows, kl};
-    seed_search_kernel<<<grid_for(nseeds), 256, 0, s>>>(P);
+    if (seed_len <= 32u) seed_search_kernel<true><<<grid_for(nseeds), 256, 0, s>>>(P);
+    else seed_search_kernel<false><<<grid_for(nseeds), 256, 0, s>>>(P);
   }
   SEED_TRY(hipGetLastError());
   SEED_TRY(rocprim::exclusive_scan(nullptr, need, rows, rbase, (uint64_t)0, 2 * nseeds,
