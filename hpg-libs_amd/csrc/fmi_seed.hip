@@ -65,7 +65,9 @@ struct SeedParams {
   fmi_seed_info *info;         // may be NULL
   uint64_t info_cap;
   uint64_t *rows;              // 2 * nseeds: rows of + then - per seed
-  int64_t *kl;                 // 2 * nseeds: k of + and k of - (first row of each)
+  // 2 * nseeds: k of + and k of - (the first row of each) in the low 32 bits
+  // (rows are below 2^32 on the device), the seed's read in the high 32
+  int64_t *kl;
 };
 
 // REGS (seed_len <= 32): the seed's bases are packed into registers once
@@ -133,8 +135,8 @@ __global__ __launch_bounds__(256) void seed_search_kernel(SeedParams P) {
     const uint64_t rm = (actual > P.filter || km > lm) ? 0 : (uint64_t)cm;
     P.rows[2 * g] = rp;
     P.rows[2 * g + 1] = rm;
-    P.kl[2 * g] = kp;
-    P.kl[2 * g + 1] = km;
+    P.kl[2 * g] = (int64_t)((uint64_t)kp | (r << 32));
+    P.kl[2 * g + 1] = (int64_t)((uint64_t)km | (r << 32));
     if (P.info && g < P.info_cap) {
       fmi_seed_info inf;
       inf.k_plus = kp; inf.l_plus = lp; inf.k_minus = km; inf.l_minus = lm;
@@ -149,7 +151,7 @@ struct RowParams {
   const uint64_t *offsets;
   const uint64_t *seed_base;   // nreads + 1
   const uint64_t *row_base;    // 2 * nseeds exclusive scan
-  const int64_t *kl;
+  const int64_t *kl;           // first row | read << 32 per (seed, strand)
   const uint64_t *koff;        // kcount + 1
   const uint64_t *kstart;
   uint64_t nslots, nrows;      // nslots = 2 * nseeds
@@ -164,8 +166,9 @@ __global__ __launch_bounds__(256) void seed_rows_kernel(RowParams P) {
     const uint64_t slot = owner(P.row_base, P.nslots, q);
     const uint64_t g = slot >> 1;
     const uint32_t strand = (uint32_t)(slot & 1u);
-    const uint64_t j = (uint64_t)P.kl[slot] + (q - P.row_base[slot]);
-    const uint64_t r = owner(P.seed_base, P.nreads, g);
+    const uint64_t klr = (uint64_t)P.kl[slot];
+    const uint64_t j = (klr & 0xFFFFFFFFull) + (q - P.row_base[slot]);
+    const uint64_t r = klr >> 32;                 // the seed's read, from the search
     const uint64_t seq_start = (g - P.seed_base[r]) * P.stride;
     const uint64_t seq_end = seq_start + P.seed_len - 1;
     const uint64_t seq_len = P.offsets[r + 1] - P.offsets[r];
