@@ -140,32 +140,40 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     # k-error search on a stream of its own beside the exact and the 1-error
     # search, forked from the first stream and joined back into it
     # (fmi_stream_wait) so that a step ends when both have; timed launched
-    # directly and captured once as one graph of two branches replayed with
-    # one call (fmi_graph_*); the faster is the step.  (Each group on a
-    # stream of its own measured 8.2 ms, the exact then the 1-error search
+    # directly with either order of the first stream's two searches, and
+    # the faster order captured once as one graph of two branches replayed
+    # with one call (fmi_graph_*); the fastest is the step.  (Each group on
+    # a stream of its own measured 8.2 ms, the exact then the 1-error search
     # on one stream beside the k-error search 8.0 ms, all on one 8.6 ms.)
     st = [hpgfm.Stream() for _ in range(2)]
 
-    def step():
+    def step(s1_first=False):
         st[1].wait(st[0])
-        run_exact(st[0].handle); run_s1(st[0].handle); run_k(st[1].handle)
+        if s1_first:
+            run_s1(st[0].handle); run_exact(st[0].handle)
+        else:
+            run_exact(st[0].handle); run_s1(st[0].handle)
+        run_k(st[1].handle)
         st[0].wait(st[1])
 
     step()                               # sizes each stream's scratch
     st[0].sync()
     timed = {}
-    for _ in range(args.warmup):
-        step()
-    st[0].sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    st[0].sync()
-    timed["two streams (exact then 1-error | k-error), forked and joined per step"] = \
-        (time.perf_counter() - t0) / args.steps
+    order = {}
+    for s1_first, name in ((False, "exact then 1-error"), (True, "1-error then exact")):
+        for _ in range(args.warmup):
+            step(s1_first)
+        st[0].sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(s1_first)
+        st[0].sync()
+        order[s1_first] = (time.perf_counter() - t0) / args.steps
+        timed[f"two streams ({name} | k-error), forked and joined per step"] = order[s1_first]
+    best_s1_first = min(order, key=order.get)
     graph = hpgfm.Graph(st[0])
     with graph:
-        step()
+        step(best_s1_first)
     for _ in range(args.warmup):
         graph.launch()
     st[0].sync()
@@ -173,7 +181,9 @@ def bench_mixed(args, pre, g, ix, hpgfm, synth, list_cap=128, anchor_cap=16):
     for _ in range(args.steps):
         graph.launch()
     st[0].sync()
-    timed["one graph of two branches per step"] = (time.perf_counter() - t0) / args.steps
+    timed["one graph of two branches per step ("
+          + ("1-error then exact" if best_s1_first else "exact then 1-error") + " | k-error)"] = \
+        (time.perf_counter() - t0) / args.steps
     graph.close()
     for x in st:
         x.close()
