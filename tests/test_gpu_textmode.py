@@ -136,13 +136,28 @@ def test_textmode_exact_with_n(gpu_engine):
 def test_exact_staged_and_long_reads_vs_oracle(gpu_engine, form):
     """Reads of 1-400 bases in one batch, so a wavefront holds reads staged in
     LDS (at most 256 bases) beside reads searched from their bytes, with N
-    bases: on every view and direction the record and the anchor equal the
-    oracle's (BWExactSearchBackward / Forward, search/search.h:9-65)."""
+    bases, every length at a staging word or slot boundary, and copies of
+    the genome's ends: on every view and direction the record and the anchor
+    equal the oracle's (BWExactSearchBackward / Forward,
+    search/search.h:9-65)."""
     hp = gpu_engine
     g, _, _ = synth.genome(200_000, seed=141, repeat_frac=0.3, fam_len=(300, 3000))
-    lens = np.random.default_rng(142).integers(1, 401, 1200)
-    reads, offs, _ = synth.reads(g, 1200, seed=143, sub_rate=0.003, n_rate=0.002, lengths=lens)
+    # random lengths, then every word and slot boundary of the staging (16
+    # bases per code word, 32 per flag word, 256 per slot) four times
+    edges = np.array([1, 2, 3, 15, 16, 17, 31, 32, 33, 63, 64, 65, 127, 128, 129,
+                      255, 256, 257], np.int64)
+    lens = np.concatenate([np.random.default_rng(142).integers(1, 401, 1200), np.tile(edges, 4)])
+    reads, offs, _ = synth.reads(g, len(lens), seed=143, sub_rate=0.003, n_rate=0.002,
+                                 lengths=lens)
     assert np.any(reads > 3)
+    # copies of the genome's first and last bases, both orientations: text
+    # runs that reach the text's start (suffix 0) or its last word
+    ends = []
+    for L in (20, 33, 40, 100, 200, 256, 300):
+        for w in (g[:L], g[-L:]):
+            ends += [w, synth.revcomp(w)]
+    reads = np.concatenate([reads] + ends).astype(np.uint8)
+    offs = np.concatenate([offs, offs[-1] + np.cumsum([len(w) for w in ends])]).astype(np.uint64)
     o = fo.Oracle(g, ratio=16)
     lsa = o.size_sa - 1
     ix = hp.Index.build(g, 16).to_device()
