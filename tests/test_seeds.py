@@ -89,22 +89,31 @@ def test_seed_map_gpu_vs_oracle(gpu_engine):
     lengths = np.random.default_rng(24).integers(0, 251, 1500)
     reads, offs, _ = synth.reads(g, 1500, seed=25, sub_rate=0.01, n_rate=0.002, lengths=lengths)
     batch = hp.ReadBatch(reads, offs)
-    for sl, st, filt in ((18, 5, 1000), (12, 3, 7), (25, 25, 1)):
-        ns, nr, regs, info = hp.seed_map_batch(ix, batch, sl, st, filt)
-        want, wi = [], []
-        for i in range(batch.nreads):
-            W = reads[offs[i]:offs[i + 1]]
-            r, inf = o.seed_map(W, sl, st, filt, koff, kstart, read=i)
-            want.append(r)
-            wi.append(inf)
-        want = np.concatenate(want)
-        wi = np.concatenate([x for x in wi if len(x)]) if any(len(x) for x in wi) else np.zeros((0, 6))
-        assert ns == len(wi) and nr == len(want)
-        for key in ("k_plus", "l_plus", "k_minus", "l_minus"):
-            assert np.array_equal(info[key], wi[:, ("k_plus", "l_plus", "k_minus", "l_minus").index(key)])
-        assert np.array_equal(info["rows_plus"], wi[:, 4]) and np.array_equal(info["rows_minus"], wi[:, 5])
-        for key in want.dtype.names:
-            assert np.array_equal(regs[key], want[key]), (sl, st, filt, key)
+    # seeds of up to 32 bases are searched from registers, longer ones from
+    # the read's bytes; then the same on the expanded index with an 8-mer
+    # table (the first 8 steps of each orientation from one table read)
+    cases = ((18, 5, 1000), (12, 3, 7), (25, 25, 1), (32, 7, 1000), (33, 11, 1000), (40, 40, 50))
+    for form in ("compact", "kmer8"):
+        if form == "kmer8":
+            ix.expand()
+            ix.kmer_table(8)
+        for sl, st, filt in cases:
+            ns, nr, regs, info = hp.seed_map_batch(ix, batch, sl, st, filt)
+            want, wi = [], []
+            for i in range(batch.nreads):
+                W = reads[offs[i]:offs[i + 1]]
+                r, inf = o.seed_map(W, sl, st, filt, koff, kstart, read=i)
+                want.append(r)
+                wi.append(inf)
+            want = np.concatenate(want)
+            wi = np.concatenate([x for x in wi if len(x)]) if any(len(x) for x in wi) else np.zeros((0, 6))
+            assert ns == len(wi) and nr == len(want), (form, sl)
+            for key in ("k_plus", "l_plus", "k_minus", "l_minus"):
+                assert np.array_equal(info[key], wi[:, ("k_plus", "l_plus", "k_minus", "l_minus").index(key)]), \
+                    (form, sl, key)
+            assert np.array_equal(info["rows_plus"], wi[:, 4]) and np.array_equal(info["rows_minus"], wi[:, 5])
+            for key in want.dtype.names:
+                assert np.array_equal(regs[key], want[key]), (form, sl, st, filt, key)
     # a capacity below the row count stores the first rows and reports all
     ns, nr, regs, _ = hp.seed_map_batch(ix, batch, 18, 5, 1000, region_cap=10, with_info=False)
     assert len(regs) == 10 and nr > 10
