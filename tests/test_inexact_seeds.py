@@ -132,7 +132,11 @@ def test_gpu_inexact_seeds_vs_oracle(gpu_engine, expanded):
     rl = np.random.default_rng(42).integers(1, 220, 400)
     reads, offs, _ = synth.reads(g, 400, seed=43, sub_rate=0.015, n_rate=0.002, lengths=rl)
     batch = hp.ReadBatch(reads, offs)
-    for seed_size, min_seed, rec_cap, region_cap in ((30, 15, 256, 64), (16, 8, 6, 3)):
+    # windows of up to 32 bases are searched from registers, longer ones
+    # from the read's bytes
+    for seed_size, min_seed, rec_cap, region_cap in ((30, 15, 256, 64), (16, 8, 6, 3),
+                                                     (32, 16, 256, 64), (33, 16, 256, 64),
+                                                     (40, 20, 128, 32)):
         res = hp.map_inexact_seeds_batch(ix, batch, seed_size, min_seed, rec_cap, region_cap)
         for i in range(batch.nreads):
             W = reads[offs[i]:offs[i + 1]]
